@@ -1,0 +1,65 @@
+"""Builds the HIP library gizmo_amd/libgizmo_hip.so for gfx950 (in-tree).
+
+    python -m gizmo_amd.build
+
+Flags:
+  * --offload-arch=gfx950: MI355X only.
+  * -ffp-contract=off: the FP64 form keeps the reference's rounding order term by term, so
+    it can be compared to the CPU oracle bit for bit (the FP32-mixed kernels opt back into
+    contraction locally).
+  * -Xclang -target-feature -Xclang -packed-fp32-ops: no packed FP32 instructions.
+"""
+import os
+import shutil
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+LIB = os.path.join(HERE, "libgizmo_hip.so")
+SOURCES = ["gz_kernels.hip", "gz_api.hip"]
+HEADERS = ["gz_device.h", "gz_kernels.h", os.path.join("..", "..", "include", "gizmo_hydro.h")]
+
+
+def hipcc():
+    for c in (os.environ.get("HIPCC"), "/opt/rocm/bin/hipcc", shutil.which("hipcc")):
+        if c and os.path.exists(c):
+            return c
+    raise RuntimeError("hipcc not found")
+
+
+FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
+         "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wno-unused-result"]
+
+
+def build(verbose=False, jobs=None):
+    objs = []
+    procs = []
+    for src in SOURCES:
+        obj = os.path.join(CSRC, src.replace(".hip", ".o"))
+        cmd = [hipcc()] + FLAGS + ["-c", os.path.join(CSRC, src), "-o", obj]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        procs.append((src, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)))
+        objs.append(obj)
+    bad = []
+    for src, p in procs:
+        out, _ = p.communicate()
+        if p.returncode != 0:
+            bad.append((src, out.decode(errors="replace")))
+        elif verbose and out:
+            print(out.decode(errors="replace"))
+    if bad:
+        msg = "\n".join(f"--- {s} ---\n{o}" for s, o in bad)
+        raise RuntimeError(f"hipcc failed:\n{msg}")
+    tmp = LIB + ".tmp"
+    cmd = [hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs
+    subprocess.run(cmd, check=True)
+    os.replace(tmp, LIB)
+    for o in objs:
+        os.remove(o)
+    return LIB
+
+
+if __name__ == "__main__":
+    print(build(verbose="-v" in sys.argv))

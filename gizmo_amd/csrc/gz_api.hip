@@ -1,0 +1,866 @@
+// gz_api.hip -- host orchestration and the C ABI (include/gizmo_hydro.h).
+//
+// One context = one periodic box resident on one GPU.  The host side follows the
+// reference's stage order: density() builds a one-way list at 1.3x the kernel radius and
+// iterates the radius on the device, rebuilding only the rows whose radius outgrows the
+// buffer (src/hydro/density_loop.h:316, src/mesh/neighbor_loop_runner.cc:6060-6110);
+// hydro_gradient_calc() builds the symmetric list once and hydro_force() reuses it, as the
+// reference's hydro corridor does (src/hydro/hydro_corridor.cc:195).
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/gizmo_hydro.h"
+#include "gz_kernels.h"
+
+using namespace gz;
+
+namespace {
+
+constexpr int kBlock = 256;
+inline unsigned nblk(long n) { return (unsigned)((n + kBlock - 1) / kBlock); }
+
+}  // namespace
+
+struct gz_context {
+    gz_params par{};
+    DevParams dp{};
+    int device = 0;
+    hipStream_t stream = nullptr;
+    long cap = 0, n = 0;
+    std::string err;
+
+    // particle state, device order (SoA; vectors with column stride n)
+    double *X = nullptr, *Y = nullptr, *Z = nullptr, *VX = nullptr, *VY = nullptr, *VZ = nullptr;
+    double *M = nullptr, *H = nullptr, *U = nullptr, *NN = nullptr, *DRK = nullptr, *DIVV = nullptr;
+    double *RHO = nullptr, *PR = nullptr, *T = nullptr, *FW = nullptr, *CN = nullptr, *FCE = nullptr;
+    double *GR = nullptr, *GP = nullptr, *GV = nullptr, *ACC = nullptr, *DTU = nullptr, *MSV = nullptr, *MSV_IN = nullptr;
+    int *TB = nullptr, *WAKE = nullptr;
+    uint8_t *ACT = nullptr;   // active flag per particle (device order)
+    bool all_active = true;
+    int *perm = nullptr;      // device order -> caller index
+    uint32_t *key = nullptr;  // fine Morton key, device order
+
+    // staging and scratch
+    double *stage = nullptr;  // n*9 doubles
+    void *tmp = nullptr;
+    size_t tmp_bytes = 0;
+    uint32_t *key2 = nullptr;
+    int *ival = nullptr, *ival2 = nullptr, *iflag = nullptr, *iscan = nullptr;
+
+    // active rows
+    int *act = nullptr;       // slot -> device index
+    int nact = 0;
+
+    // grid
+    int level = -1;
+    double cell = 0;
+    int *cstart = nullptr, *cend = nullptr, *nsten = nullptr;
+    int4 *sten = nullptr;
+    double *chmax = nullptr;
+    long grid_cells_alloc = 0;
+
+    // lists
+    int *ell_d = nullptr, *cnt_d = nullptr;  // density (one-way, buffered)
+    int cap_d = 0;
+    long ld_d = 0;
+    int *ell_s = nullptr, *cnt_s = nullptr;  // symmetric
+    int cap_s = 0;
+    long ld_s = 0;
+    bool sym_valid = false;
+    bool density_done = false, gradients_done = false;
+
+    // density iteration state (per slot)
+    double *it_h = nullptr, *it_hb = nullptr, *it_l = nullptr, *it_r = nullptr, *it_cn = nullptr;
+    int *it_ci = nullptr, *it_it = nullptr, *todo = nullptr, *todo2 = nullptr;
+    int *counters = nullptr;   // device: [0] next count, [1] rebuild flag, [2] error, [3] overflow
+    int *hcounters = nullptr;  // pinned host mirror
+
+    // stats and timing
+    std::map<std::string, double> stats;
+    bool timing = false;
+    struct TEv { std::string name; hipEvent_t a, b; };
+    std::vector<TEv> pending;
+    std::map<std::string, std::pair<long, double>> timings;
+    std::vector<hipEvent_t> event_pool;
+};
+
+namespace {
+
+int fail(gz_context *c, int code, const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    if (c) c->err = buf;
+    return code;
+}
+
+#define HIPCHK(call)                                                                                      \
+    do {                                                                                                  \
+        hipError_t e_ = (call);                                                                           \
+        if (e_ != hipSuccess) return fail(ctx, GZ_E_HIP, "%s failed: %s (%s:%d)", #call,                  \
+                                          hipGetErrorString(e_), __FILE__, __LINE__);                     \
+    } while (0)
+
+#define CHK(call)                          \
+    do {                                   \
+        int rc_ = (call);                  \
+        if (rc_) return rc_;               \
+    } while (0)
+
+template <typename T>
+int dalloc(gz_context *ctx, T **p, size_t count)
+{
+    if (*p) { hipFree(*p); *p = nullptr; }
+    if (count == 0) count = 1;
+    hipError_t e = hipMalloc((void **)p, count * sizeof(T));
+    if (e != hipSuccess) { *p = nullptr; return fail(ctx, GZ_E_MEMORY, "hipMalloc of %zu bytes failed: %s", count * sizeof(T), hipGetErrorString(e)); }
+    return 0;
+}
+
+template <typename T>
+void dfree(T *&p)
+{
+    if (p) hipFree(p);
+    p = nullptr;
+}
+
+hipEvent_t get_event(gz_context *c)
+{
+    if (!c->event_pool.empty()) { hipEvent_t e = c->event_pool.back(); c->event_pool.pop_back(); return e; }
+    hipEvent_t e;
+    hipEventCreate(&e);
+    return e;
+}
+
+struct Timed {
+    gz_context *c;
+    const char *name;
+    hipEvent_t a{}, b{};
+    Timed(gz_context *c_, const char *n_) : c(c_), name(n_)
+    {
+        if (c->timing) { a = get_event(c); b = get_event(c); hipEventRecord(a, c->stream); }
+    }
+    ~Timed()
+    {
+        if (c->timing) { hipEventRecord(b, c->stream); c->pending.push_back({name, a, b}); }
+    }
+};
+
+void collect_timings(gz_context *c)
+{
+    if (c->pending.empty()) return;
+    hipStreamSynchronize(c->stream);
+    for (auto &t : c->pending) {
+        float ms = 0;
+        hipEventElapsedTime(&ms, t.a, t.b);
+        auto &e = c->timings[t.name];
+        e.first += 1;
+        e.second += ms;
+        c->event_pool.push_back(t.a);
+        c->event_pool.push_back(t.b);
+    }
+    c->pending.clear();
+}
+
+int set_params(gz_context *ctx, const gz_params *p)
+{
+    if (!p) return fail(ctx, GZ_E_ARG, "params is NULL");
+    if (!(p->box_size > 0)) return fail(ctx, GZ_E_ARG, "box_size must be > 0");
+    if (p->kernel != 3 && p->kernel != 6) return fail(ctx, GZ_E_ARG, "kernel must be 3 (cubic spline) or 6 (Wendland C2)");
+    if (!(p->gamma > 1)) return fail(ctx, GZ_E_ARG, "gamma must be > 1");
+    if (!(p->des_num_ngb > 0)) return fail(ctx, GZ_E_ARG, "des_num_ngb must be > 0");
+    if (!(p->max_kernel_radius > 0) || p->max_kernel_radius > 0.5 * p->box_size / 1.3)
+        return fail(ctx, GZ_E_ARG, "max_kernel_radius must be in (0, box/2.6]");
+    if (p->precision != GZ_FP64 && p->precision != GZ_FP32_MIXED) return fail(ctx, GZ_E_ARG, "precision must be GZ_FP64 or GZ_FP32_MIXED");
+    ctx->par = *p;
+    DevParams &d = ctx->dp;
+    d.box = p->box_size;
+    d.half = 0.5 * p->box_size;
+    d.gamma = p->gamma;
+    d.G0 = p->gamma;
+    d.G1 = ((d.G0 - 1.0) / (2.0 * d.G0));
+    d.G2 = ((d.G0 + 1.0) / (2.0 * d.G0));
+    d.G3 = ((2.0 * d.G0 / (d.G0 - 1.0)));
+    d.G4 = (2.0 / (d.G0 - 1.0));
+    d.G5 = (2.0 / (d.G0 + 1.0));
+    d.G6 = ((d.G0 - 1.0) / (d.G0 + 1.0));
+    d.G7 = (0.5 * (d.G0 - 1.0));
+    d.G8 = (1.0 / d.G0);
+    d.G9 = (d.G0 - 1.0);
+    d.des_num_ngb = p->des_num_ngb;
+    d.max_num_ngb_dev = p->max_num_ngb_dev;
+    d.min_kernel_radius = p->min_kernel_radius;
+    d.max_kernel_radius = p->max_kernel_radius;
+    d.time = p->time;
+    d.time_begin = p->time_begin;
+    return 0;
+}
+
+template <typename R>
+Gam<R> gam(const DevParams &d)
+{
+    Gam<R> g;
+    g.G0 = (R)d.G0; g.G1 = (R)d.G1; g.G2 = (R)d.G2; g.G3 = (R)d.G3; g.G4 = (R)d.G4;
+    g.G5 = (R)d.G5; g.G6 = (R)d.G6; g.G7 = (R)d.G7; g.G8 = (R)d.G8; g.G9 = (R)d.G9;
+    return g;
+}
+
+int ensure_tmp(gz_context *ctx, size_t bytes)
+{
+    if (bytes <= ctx->tmp_bytes) return 0;
+    if (ctx->tmp) hipFree(ctx->tmp);
+    ctx->tmp = nullptr;
+    size_t b = std::max(bytes, (size_t)1 << 20);
+    HIPCHK(hipMalloc(&ctx->tmp, b));
+    ctx->tmp_bytes = b;
+    return 0;
+}
+
+int alloc_state(gz_context *ctx, long cap)
+{
+    const size_t n = (size_t)cap;
+    CHK(dalloc(ctx, &ctx->X, n)); CHK(dalloc(ctx, &ctx->Y, n)); CHK(dalloc(ctx, &ctx->Z, n));
+    CHK(dalloc(ctx, &ctx->VX, n)); CHK(dalloc(ctx, &ctx->VY, n)); CHK(dalloc(ctx, &ctx->VZ, n));
+    CHK(dalloc(ctx, &ctx->M, n)); CHK(dalloc(ctx, &ctx->H, n)); CHK(dalloc(ctx, &ctx->U, n));
+    CHK(dalloc(ctx, &ctx->NN, n)); CHK(dalloc(ctx, &ctx->DRK, n)); CHK(dalloc(ctx, &ctx->DIVV, n));
+    CHK(dalloc(ctx, &ctx->RHO, n)); CHK(dalloc(ctx, &ctx->PR, n)); CHK(dalloc(ctx, &ctx->T, 6 * n));
+    CHK(dalloc(ctx, &ctx->FW, 3 * n)); CHK(dalloc(ctx, &ctx->CN, n)); CHK(dalloc(ctx, &ctx->FCE, n));
+    CHK(dalloc(ctx, &ctx->GR, 3 * n)); CHK(dalloc(ctx, &ctx->GP, 3 * n)); CHK(dalloc(ctx, &ctx->GV, 9 * n));
+    CHK(dalloc(ctx, &ctx->ACC, 3 * n)); CHK(dalloc(ctx, &ctx->DTU, n)); CHK(dalloc(ctx, &ctx->MSV, n));
+    CHK(dalloc(ctx, &ctx->MSV_IN, n));
+    CHK(dalloc(ctx, &ctx->TB, n)); CHK(dalloc(ctx, &ctx->WAKE, n)); CHK(dalloc(ctx, &ctx->ACT, n));
+    CHK(dalloc(ctx, &ctx->perm, n)); CHK(dalloc(ctx, &ctx->key, n)); CHK(dalloc(ctx, &ctx->key2, n));
+    CHK(dalloc(ctx, &ctx->ival, n)); CHK(dalloc(ctx, &ctx->ival2, n)); CHK(dalloc(ctx, &ctx->iflag, n + 1));
+    CHK(dalloc(ctx, &ctx->iscan, n + 1)); CHK(dalloc(ctx, &ctx->act, n));
+    CHK(dalloc(ctx, &ctx->stage, 9 * n));
+    CHK(dalloc(ctx, &ctx->it_h, n)); CHK(dalloc(ctx, &ctx->it_hb, n)); CHK(dalloc(ctx, &ctx->it_l, n));
+    CHK(dalloc(ctx, &ctx->it_r, n)); CHK(dalloc(ctx, &ctx->it_cn, n)); CHK(dalloc(ctx, &ctx->it_ci, n));
+    CHK(dalloc(ctx, &ctx->it_it, n)); CHK(dalloc(ctx, &ctx->todo, n)); CHK(dalloc(ctx, &ctx->todo2, n));
+    CHK(dalloc(ctx, &ctx->cnt_d, n)); CHK(dalloc(ctx, &ctx->cnt_s, n));
+    CHK(dalloc(ctx, &ctx->counters, 8));
+    if (!ctx->hcounters) {
+        hipError_t e = hipHostMalloc((void **)&ctx->hcounters, 8 * sizeof(int), hipHostMallocDefault);
+        if (e != hipSuccess) return fail(ctx, GZ_E_MEMORY, "hipHostMalloc failed");
+    }
+    ctx->cap = cap;
+    return 0;
+}
+
+void free_state(gz_context *c)
+{
+    dfree(c->X); dfree(c->Y); dfree(c->Z); dfree(c->VX); dfree(c->VY); dfree(c->VZ); dfree(c->M); dfree(c->H);
+    dfree(c->U); dfree(c->NN); dfree(c->DRK); dfree(c->DIVV); dfree(c->RHO); dfree(c->PR); dfree(c->T); dfree(c->FW);
+    dfree(c->CN); dfree(c->FCE); dfree(c->GR); dfree(c->GP); dfree(c->GV); dfree(c->ACC); dfree(c->DTU); dfree(c->MSV);
+    dfree(c->MSV_IN); dfree(c->TB); dfree(c->WAKE); dfree(c->ACT); dfree(c->perm); dfree(c->key); dfree(c->key2);
+    dfree(c->ival); dfree(c->ival2); dfree(c->iflag); dfree(c->iscan); dfree(c->act); dfree(c->stage);
+    dfree(c->it_h); dfree(c->it_hb); dfree(c->it_l); dfree(c->it_r); dfree(c->it_cn); dfree(c->it_ci); dfree(c->it_it);
+    dfree(c->todo); dfree(c->todo2); dfree(c->cnt_d); dfree(c->cnt_s); dfree(c->counters); dfree(c->ell_d); dfree(c->ell_s);
+    dfree(c->cstart); dfree(c->cend); dfree(c->nsten); dfree(c->sten); dfree(c->chmax);
+    if (c->tmp) hipFree(c->tmp);
+    c->tmp = nullptr;
+    c->tmp_bytes = 0;
+    if (c->hcounters) hipHostFree(c->hcounters);
+    c->hcounters = nullptr;
+}
+
+// upload one host field (width doubles per particle, caller order) into device order
+int upload_field(gz_context *ctx, const double *host, int width, double *dst, bool soa)
+{
+    const long n = ctx->n;
+    HIPCHK(hipMemcpyAsync(ctx->stage, host, (size_t)n * width * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    k_gather<<<nblk(n), kBlock, 0, ctx->stream>>>(n, width, ctx->perm, ctx->stage, dst, soa ? 1 : 0);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+int download_field(gz_context *ctx, double *host, int width, const double *src, bool soa)
+{
+    const long n = ctx->n;
+    k_scatter<<<nblk(n), kBlock, 0, ctx->stream>>>(n, width, ctx->perm, src, ctx->stage, soa ? 1 : 0);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(host, ctx->stage, (size_t)n * width * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    return 0;
+}
+
+// grid of level L over the current device order; the stencil of each cell
+int build_grid(gz_context *ctx, double rmax)
+{
+    const double box = ctx->par.box_size;
+    int L = 0;
+    while (L < kFineLevel && box / (double)(1 << (L + 1)) >= rmax) L++;
+    const long ncell = 1L << (3 * L);
+    if (ncell > ctx->grid_cells_alloc) {
+        CHK(dalloc(ctx, &ctx->cstart, ncell)); CHK(dalloc(ctx, &ctx->cend, ncell)); CHK(dalloc(ctx, &ctx->nsten, ncell));
+        CHK(dalloc(ctx, &ctx->sten, ncell * 27)); CHK(dalloc(ctx, &ctx->chmax, ncell));
+        ctx->grid_cells_alloc = ncell;
+    }
+    ctx->level = L;
+    ctx->cell = box / (double)(1 << L);
+    Timed t(ctx, "ngb_build");
+    k_cell_clear<<<nblk(ncell), kBlock, 0, ctx->stream>>>(ncell, ctx->cstart, ctx->cend, ctx->chmax);
+    k_cell_bounds<<<nblk(ctx->n), kBlock, 0, ctx->stream>>>(ctx->n, ctx->key, 3 * (kFineLevel - L), ctx->cstart, ctx->cend);
+    k_cell_hmax<<<nblk(ncell), kBlock, 0, ctx->stream>>>(ncell, ctx->cstart, ctx->cend, ctx->H, ctx->chmax);
+    k_cell_stencil<<<nblk(ncell), kBlock, 0, ctx->stream>>>(L, ctx->cstart, ctx->cend, ctx->sten, ctx->nsten);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+GridView grid_view(gz_context *ctx)
+{
+    GridView g;
+    g.level = ctx->level;
+    g.shift = 3 * (kFineLevel - ctx->level);
+    g.cell = ctx->cell;
+    g.box = ctx->par.box_size;
+    g.half = 0.5 * ctx->par.box_size;
+    g.sten = ctx->sten;
+    g.nsten = ctx->nsten;
+    g.chmax = ctx->chmax;
+    return g;
+}
+
+int read_counters(gz_context *ctx)
+{
+    HIPCHK(hipMemcpyAsync(ctx->hcounters, ctx->counters, 8 * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+int max_on_device(gz_context *ctx, const double *v, long n, double *out)
+{
+    size_t bytes = 0;
+    double *dout = (double *)ctx->stage;  // stage is free between uploads
+    HIPCHK(hipcub::DeviceReduce::Max(nullptr, bytes, v, dout, (int)n, ctx->stream));
+    CHK(ensure_tmp(ctx, bytes));
+    HIPCHK(hipcub::DeviceReduce::Max(ctx->tmp, bytes, v, dout, (int)n, ctx->stream));
+    HIPCHK(hipMemcpyAsync(out, dout, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+// build ELL rows (all slots when rows == nullptr) into a list that grows its width on overflow
+template <bool SYM>
+int build_list(gz_context *ctx, int nrows, const int *rows, const double *hrow, double scale, int **ell, long *ld, int *cap,
+               int *cnt, int cap_hint)
+{
+    const long need_ld = ((long)ctx->nact + 63) / 64 * 64;
+    if (*ell == nullptr || *ld != need_ld || *cap < cap_hint) {
+        *cap = std::max(*cap, cap_hint);
+        *ld = need_ld;
+        CHK(dalloc(ctx, ell, (size_t)(*cap) * (size_t)(*ld)));
+    }
+    for (int attempt = 0; attempt < 8; attempt++) {
+        HIPCHK(hipMemsetAsync(ctx->counters + 3, 0, sizeof(int), ctx->stream));
+        {
+            Timed t(ctx, "ngb_build");
+            k_build_ell<SYM><<<nblk(nrows), kBlock, 0, ctx->stream>>>(nrows, rows, ctx->act, hrow, scale, ctx->X, ctx->Y, ctx->Z, ctx->H,
+                                                                     ctx->key, grid_view(ctx), *ell, *ld, *cap, cnt, ctx->counters + 3);
+            HIPCHK(hipGetLastError());
+        }
+        CHK(read_counters(ctx));
+        const int over = ctx->hcounters[3];
+        if (over == 0) return 0;
+        // widen and rebuild every row of this list (a narrower list cannot keep its rows)
+        *cap = std::max(over + 16, *cap * 3 / 2);
+        CHK(dalloc(ctx, ell, (size_t)(*cap) * (size_t)(*ld)));
+        nrows = ctx->nact;
+        rows = nullptr;
+        ctx->stats["ell_widen"] += 1;
+    }
+    return fail(ctx, GZ_E_STATE, "neighbour list did not fit after widening");
+}
+
+int build_active(gz_context *ctx)
+{
+    const long n = ctx->n;
+    k_flag_active<<<nblk(n), kBlock, 0, ctx->stream>>>(n, ctx->all_active ? nullptr : ctx->ACT, ctx->M, ctx->iflag);
+    HIPCHK(hipGetLastError());
+    size_t bytes = 0;
+    HIPCHK(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, ctx->iflag, ctx->iscan, (int)(n + 1), ctx->stream));
+    CHK(ensure_tmp(ctx, bytes));
+    HIPCHK(hipMemsetAsync(ctx->iflag + n, 0, sizeof(int), ctx->stream));
+    HIPCHK(hipcub::DeviceScan::ExclusiveSum(ctx->tmp, bytes, ctx->iflag, ctx->iscan, (int)(n + 1), ctx->stream));
+    k_compact<<<nblk(n), kBlock, 0, ctx->stream>>>(n, ctx->iflag, ctx->iscan, ctx->act);
+    HIPCHK(hipGetLastError());
+    int na = 0;
+    HIPCHK(hipMemcpyAsync(&na, ctx->iscan + n, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    ctx->nact = na;
+    ctx->stats["n_active"] = na;
+    return 0;
+}
+
+// ---------------------------------------------------------------------------------------
+// stages
+// ---------------------------------------------------------------------------------------
+}  // namespace
+
+// iota on device
+__global__ void k_iota(int n, int *out)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = i;
+}
+
+// density()'s pre-iteration radius clamp, src/hydro/density_loop.cc:818-838
+__global__ void k_clamp_h(int na, const int *act, const double *M, double *H, double hclamp)
+{
+    int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= na) return;
+    int i = act[r];
+    double h = H[i];
+    if ((h < 0) || !isfinite(h) || (h > hclamp)) H[i] = hclamp;
+    (void)M;
+}
+
+namespace {
+
+template <int KERN, typename R>
+int density_stage(gz_context *ctx)
+{
+    const int na = ctx->nact;
+    ctx->stats["density_iterations"] = 0;
+    ctx->stats["ngb_rebuilds"] = 0;
+    if (na == 0) { ctx->density_done = true; ctx->sym_valid = false; return 0; }
+    HIPCHK(hipMemsetAsync(ctx->it_l, 0, na * sizeof(double), ctx->stream));
+    HIPCHK(hipMemsetAsync(ctx->it_r, 0, na * sizeof(double), ctx->stream));
+    HIPCHK(hipMemsetAsync(ctx->it_cn, 0, na * sizeof(double), ctx->stream));
+    HIPCHK(hipMemsetAsync(ctx->it_ci, 0, na * sizeof(int), ctx->stream));
+    HIPCHK(hipMemsetAsync(ctx->it_it, 0, na * sizeof(int), ctx->stream));
+    HIPCHK(hipMemsetAsync(ctx->counters, 0, 8 * sizeof(int), ctx->stream));
+    k_clamp_h<<<nblk(na), kBlock, 0, ctx->stream>>>(na, ctx->act, ctx->M, ctx->H, 0.99 * ctx->par.max_kernel_radius);
+    k_gather<<<nblk(na), kBlock, 0, ctx->stream>>>(na, 1, ctx->act, ctx->H, ctx->it_h, 0);
+    k_iota<<<nblk(na), kBlock, 0, ctx->stream>>>(na, ctx->todo);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(ctx->it_hb, ctx->it_h, na * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+    double hmax = 0;
+    CHK(max_on_device(ctx, ctx->it_h, na, &hmax));
+    CHK(build_grid(ctx, std::min(1.3 * hmax, 0.5 * ctx->par.box_size)));
+    const int cap_hint = std::max(64, (int)std::ceil(ctx->par.des_num_ngb * 1.3 * 1.3 * 1.3 * 1.6) + 32);
+    CHK(build_list<false>(ctx, na, nullptr, ctx->it_hb, 1.3, &ctx->ell_d, &ctx->ld_d, &ctx->cap_d, ctx->cnt_d, cap_hint));
+    DensityIter it{ctx->it_h, ctx->it_hb, ctx->it_l, ctx->it_r, ctx->it_cn, ctx->it_ci, ctx->it_it};
+    DensityOut out{ctx->n, ctx->H, ctx->NN, ctx->DRK, ctx->DIVV, ctx->RHO, ctx->PR, ctx->T, ctx->FW, ctx->CN, ctx->FCE};
+    int ntodo = na;
+    int *cur = ctx->todo, *nxt = ctx->todo2;
+    int iters = 0;
+    while (ntodo > 0) {
+        HIPCHK(hipMemsetAsync(ctx->counters, 0, 2 * sizeof(int), ctx->stream));
+        {
+            Timed t(ctx, "density");
+            k_density_iter<KERN, R><<<nblk(ntodo), kBlock, 0, ctx->stream>>>(
+                ctx->dp, ntodo, cur, ctx->act, ctx->X, ctx->Y, ctx->Z, ctx->VX, ctx->VY, ctx->VZ, ctx->M, ctx->U, ctx->ell_d, ctx->ld_d,
+                ctx->cnt_d, it, out, nxt, ctx->counters + 0, ctx->counters + 1, ctx->counters + 2);
+            HIPCHK(hipGetLastError());
+        }
+        CHK(read_counters(ctx));
+        iters++;
+        const int nn = ctx->hcounters[0], rebuild = ctx->hcounters[1], err = ctx->hcounters[2];
+        if (err) {
+            ctx->stats["density_iterations"] = iters;
+            if (err == GZ_STOP_MAXITER) return fail(ctx, GZ_STOP_MAXITER, "density(): kernel radius iteration did not converge in %d rounds", kMaxIter);
+            if (err == GZ_STOP_BRACKET) return fail(ctx, GZ_STOP_BRACKET, "density(): empty radius bracket (Right==0 && Left==0)");
+            return fail(ctx, err, "density(): device stop code %d", err);
+        }
+        std::swap(cur, nxt);
+        ntodo = nn;
+        if (ntodo > 0 && rebuild) {
+            // rows whose radius left the 1.3x buffer get a fresh list at the new radius
+            HIPCHK(hipMemcpyAsync(ctx->it_hb, ctx->it_h, na * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+            CHK(max_on_device(ctx, ctx->it_h, na, &hmax));
+            CHK(build_grid(ctx, std::min(1.3 * hmax, 0.5 * ctx->par.box_size)));
+            CHK(build_list<false>(ctx, ntodo, cur, ctx->it_hb, 1.3, &ctx->ell_d, &ctx->ld_d, &ctx->cap_d, ctx->cnt_d, cap_hint));
+            ctx->stats["ngb_rebuilds"] += 1;
+        }
+    }
+    ctx->stats["density_iterations"] = iters;
+    ctx->density_done = true;
+    ctx->gradients_done = false;
+    ctx->sym_valid = false;
+    return 0;
+}
+
+int ensure_sym(gz_context *ctx)
+{
+    if (ctx->sym_valid) return 0;
+    double hmax = 0;
+    CHK(max_on_device(ctx, ctx->H, ctx->n, &hmax));
+    CHK(build_grid(ctx, std::min(hmax, 0.5 * ctx->par.box_size)));
+    const int cap_hint = std::max(64, (int)std::ceil(ctx->par.des_num_ngb * 2.0) + 32);
+    CHK(build_list<true>(ctx, ctx->nact, nullptr, nullptr, 1.0, &ctx->ell_s, &ctx->ld_s, &ctx->cap_s, ctx->cnt_s, cap_hint));
+    ctx->sym_valid = true;
+    return 0;
+}
+
+template <int KERN, typename R>
+int gradients_stage(gz_context *ctx)
+{
+    if (!ctx->density_done) return fail(ctx, GZ_E_STATE, "hydro_gradient_calc() before density()");
+    if (ctx->nact == 0) { ctx->gradients_done = true; return 0; }
+    CHK(ensure_sym(ctx));
+    Timed t(ctx, "gradients");
+    k_gradients<KERN, R><<<nblk(ctx->nact), kBlock, 0, ctx->stream>>>(ctx->dp, ctx->nact, ctx->act, ctx->X, ctx->Y, ctx->Z, ctx->VX,
+                                                                      ctx->VY, ctx->VZ, ctx->M, ctx->H, ctx->RHO, ctx->PR, ctx->T,
+                                                                      ctx->CN, ctx->n, ctx->ell_s, ctx->ld_s, ctx->cnt_s, ctx->GR,
+                                                                      ctx->GP, ctx->GV);
+    HIPCHK(hipGetLastError());
+    ctx->gradients_done = true;
+    return 0;
+}
+
+template <int KERN, typename R>
+int force_stage(gz_context *ctx)
+{
+    if (!ctx->gradients_done) return fail(ctx, GZ_E_STATE, "hydro_force() before hydro_gradient_calc()");
+    if (ctx->nact == 0) return 0;
+    CHK(ensure_sym(ctx));
+    HIPCHK(hipMemcpyAsync(ctx->MSV_IN, ctx->MSV, ctx->n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+    HIPCHK(hipMemsetAsync(ctx->counters + 2, 0, sizeof(int), ctx->stream));
+    ForceView F{ctx->n, ctx->X, ctx->Y, ctx->Z, ctx->VX, ctx->VY, ctx->VZ, ctx->M, ctx->H, ctx->U, ctx->RHO, ctx->PR, ctx->CN,
+                ctx->FCE, ctx->DRK, ctx->GR, ctx->GP, ctx->GV, ctx->T};
+    {
+        Timed t(ctx, "force");
+        k_force<KERN, R><<<nblk(ctx->nact), kBlock, 0, ctx->stream>>>(ctx->dp, gam<R>(ctx->dp), ctx->nact, ctx->act, F, ctx->ell_s,
+                                                                      ctx->ld_s, ctx->cnt_s, ctx->all_active ? nullptr : ctx->ACT, ctx->TB,
+                                                                      ctx->MSV_IN, ctx->WAKE, ctx->ACC, ctx->DTU, ctx->MSV,
+                                                                      ctx->counters + 2);
+        HIPCHK(hipGetLastError());
+    }
+    CHK(read_counters(ctx));
+    if (ctx->hcounters[2]) return fail(ctx, GZ_STOP_RIEMANN, "hydro_force(): Riemann solver failed to find a positive pressure");
+    return 0;
+}
+
+template <template <int, typename> class F>
+int dispatch(gz_context *ctx)
+{
+    const bool k3 = ctx->par.kernel == 3;
+    if (ctx->par.precision == GZ_FP64) return k3 ? F<3, double>::run(ctx) : F<6, double>::run(ctx);
+    return k3 ? F<3, float>::run(ctx) : F<6, float>::run(ctx);
+}
+
+template <int K, typename R> struct DensityF { static int run(gz_context *c) { return density_stage<K, R>(c); } };
+template <int K, typename R> struct GradF { static int run(gz_context *c) { return gradients_stage<K, R>(c); } };
+template <int K, typename R> struct ForceF { static int run(gz_context *c) { return force_stage<K, R>(c); } };
+
+int set_device(gz_context *ctx)
+{
+    HIPCHK(hipSetDevice(ctx->device));
+    return 0;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------------------
+extern "C" {
+
+gz_context *gz_create(const gz_params *params, long capacity, int device)
+{
+    if (!params || capacity <= 0 || capacity > 0x7fffffffL) return nullptr;
+    gz_context *ctx = new gz_context();
+    ctx->device = device;
+    if (set_device(ctx) || set_params(ctx, params)) { delete ctx; return nullptr; }
+    if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) { delete ctx; return nullptr; }
+    if (alloc_state(ctx, capacity)) { free_state(ctx); hipStreamDestroy(ctx->stream); delete ctx; return nullptr; }
+    return ctx;
+}
+
+void gz_destroy(gz_context *ctx)
+{
+    if (!ctx) return;
+    hipSetDevice(ctx->device);
+    hipStreamSynchronize(ctx->stream);
+    for (auto &t : ctx->pending) { hipEventDestroy(t.a); hipEventDestroy(t.b); }
+    for (auto e : ctx->event_pool) hipEventDestroy(e);
+    free_state(ctx);
+    hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+const char *gz_last_error(const gz_context *ctx) { return ctx ? ctx->err.c_str() : "no context"; }
+
+int gz_set_params(gz_context *ctx, const gz_params *params)
+{
+    if (!ctx) return GZ_E_ARG;
+    gz_params old = ctx->par;
+    int rc = set_params(ctx, params);
+    if (rc) return rc;
+    if (old.box_size != params->box_size || old.kernel != params->kernel) {
+        ctx->density_done = ctx->gradients_done = ctx->sym_valid = false;
+    }
+    return 0;
+}
+
+int gz_load(gz_context *ctx, long n, const gz_fields *f)
+{
+    if (!ctx || !f) return GZ_E_ARG;
+    if (n <= 0 || n > ctx->cap) return fail(ctx, GZ_E_ARG, "n=%ld outside (0, capacity=%ld]", n, ctx->cap);
+    if (!f->pos || !f->vel || !f->mass || !f->u || !f->h) return fail(ctx, GZ_E_ARG, "pos, vel, mass, u and h are required");
+    CHK(set_device(ctx));
+    ctx->n = n;
+    // order along the Morton curve: keys from the caller's positions, stable radix sort
+    HIPCHK(hipMemcpyAsync(ctx->stage, f->pos, (size_t)n * 3 * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    {
+        Timed t(ctx, "sort");
+        k_morton<<<nblk(n), kBlock, 0, ctx->stream>>>(n, ctx->stage, ctx->par.box_size, ctx->key2, ctx->ival);
+        HIPCHK(hipGetLastError());
+        size_t bytes = 0;
+        HIPCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, ctx->key2, ctx->key, ctx->ival, ctx->perm, (int)n, 0, 3 * kFineLevel, ctx->stream));
+        CHK(ensure_tmp(ctx, bytes));
+        HIPCHK(hipcub::DeviceRadixSort::SortPairs(ctx->tmp, bytes, ctx->key2, ctx->key, ctx->ival, ctx->perm, (int)n, 0, 3 * kFineLevel, ctx->stream));
+    }
+    // positions and velocities split into SoA columns
+    k_gather<<<nblk(n), kBlock, 0, ctx->stream>>>(n, 3, ctx->perm, ctx->stage, ctx->stage + 3 * n, 1);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(ctx->X, ctx->stage + 3 * n, n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(ctx->Y, ctx->stage + 4 * n, n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(ctx->Z, ctx->stage + 5 * n, n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(ctx->stage, f->vel, (size_t)n * 3 * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    k_gather<<<nblk(n), kBlock, 0, ctx->stream>>>(n, 3, ctx->perm, ctx->stage, ctx->stage + 3 * n, 1);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(ctx->VX, ctx->stage + 3 * n, n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(ctx->VY, ctx->stage + 4 * n, n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(ctx->VZ, ctx->stage + 5 * n, n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+    CHK(upload_field(ctx, f->mass, 1, ctx->M, false));
+    CHK(upload_field(ctx, f->u, 1, ctx->U, false));
+    CHK(upload_field(ctx, f->h, 1, ctx->H, false));
+    struct Opt { const double *src; int w; double *dst; } opts[] = {
+        {f->numngb, 1, ctx->NN}, {f->drkern, 1, ctx->DRK}, {f->divvel, 1, ctx->DIVV}, {f->rho, 1, ctx->RHO},
+        {f->press, 1, ctx->PR}, {f->nvt, 6, ctx->T}, {f->nvt_fw, 3, ctx->FW}, {f->cn, 1, ctx->CN}, {f->fce, 1, ctx->FCE},
+        {f->grad_rho, 3, ctx->GR}, {f->grad_p, 3, ctx->GP}, {f->grad_v, 9, ctx->GV}, {f->accel, 3, ctx->ACC},
+        {f->dtu, 1, ctx->DTU}, {f->maxsig, 1, ctx->MSV}};
+    for (auto &o : opts) {
+        if (o.src) { CHK(upload_field(ctx, o.src, o.w, o.dst, o.w > 1)); }
+        else { HIPCHK(hipMemsetAsync(o.dst, 0, (size_t)n * o.w * sizeof(double), ctx->stream)); }
+    }
+    int *ti[2] = {f->timebin, f->wakeup};
+    int *td[2] = {ctx->TB, ctx->WAKE};
+    for (int q = 0; q < 2; q++) {
+        if (ti[q]) {
+            HIPCHK(hipMemcpyAsync(ctx->ival2, ti[q], n * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+            k_gather_int<<<nblk(n), kBlock, 0, ctx->stream>>>(n, ctx->perm, ctx->ival2, td[q]);
+            HIPCHK(hipGetLastError());
+        } else {
+            HIPCHK(hipMemsetAsync(td[q], 0, n * sizeof(int), ctx->stream));
+        }
+    }
+    ctx->all_active = true;
+    // a caller that hands over the state density() leaves (or also the gradients) may start at
+    // the later stage, as the reference's inactive particles keep that state between steps
+    ctx->density_done = f->rho && f->press && f->nvt && f->cn && f->fce && f->drkern;
+    ctx->gradients_done = ctx->density_done && f->grad_rho && f->grad_p && f->grad_v;
+    ctx->sym_valid = false;
+    CHK(build_active(ctx));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+int gz_store(gz_context *ctx, gz_fields *f)
+{
+    if (!ctx || !f) return GZ_E_ARG;
+    if (ctx->n <= 0) return fail(ctx, GZ_E_STATE, "nothing loaded");
+    CHK(set_device(ctx));
+    const long n = ctx->n;
+    if (f->pos) {
+        HIPCHK(hipMemcpyAsync(ctx->stage + 3 * n, ctx->X, n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+        HIPCHK(hipMemcpyAsync(ctx->stage + 4 * n, ctx->Y, n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+        HIPCHK(hipMemcpyAsync(ctx->stage + 5 * n, ctx->Z, n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+        k_scatter<<<nblk(n), kBlock, 0, ctx->stream>>>(n, 3, ctx->perm, ctx->stage + 3 * n, ctx->stage, 1);
+        HIPCHK(hipMemcpyAsync(f->pos, ctx->stage, (size_t)n * 3 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+    }
+    if (f->vel) {
+        HIPCHK(hipMemcpyAsync(ctx->stage + 3 * n, ctx->VX, n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+        HIPCHK(hipMemcpyAsync(ctx->stage + 4 * n, ctx->VY, n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+        HIPCHK(hipMemcpyAsync(ctx->stage + 5 * n, ctx->VZ, n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+        k_scatter<<<nblk(n), kBlock, 0, ctx->stream>>>(n, 3, ctx->perm, ctx->stage + 3 * n, ctx->stage, 1);
+        HIPCHK(hipMemcpyAsync(f->vel, ctx->stage, (size_t)n * 3 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+    }
+    struct Opt { double *dst; int w; const double *src; } opts[] = {
+        {f->mass, 1, ctx->M}, {f->h, 1, ctx->H}, {f->u, 1, ctx->U}, {f->numngb, 1, ctx->NN}, {f->drkern, 1, ctx->DRK},
+        {f->divvel, 1, ctx->DIVV}, {f->rho, 1, ctx->RHO}, {f->press, 1, ctx->PR}, {f->nvt, 6, ctx->T}, {f->nvt_fw, 3, ctx->FW},
+        {f->cn, 1, ctx->CN}, {f->fce, 1, ctx->FCE}, {f->grad_rho, 3, ctx->GR}, {f->grad_p, 3, ctx->GP}, {f->grad_v, 9, ctx->GV},
+        {f->accel, 3, ctx->ACC}, {f->dtu, 1, ctx->DTU}, {f->maxsig, 1, ctx->MSV}};
+    for (auto &o : opts) {
+        if (!o.dst) continue;
+        CHK(download_field(ctx, o.dst, o.w, o.src, o.w > 1));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+    }
+    int *ti[2] = {f->timebin, f->wakeup};
+    const int *td[2] = {ctx->TB, ctx->WAKE};
+    for (int q = 0; q < 2; q++) {
+        if (!ti[q]) continue;
+        k_scatter_int<<<nblk(n), kBlock, 0, ctx->stream>>>(n, ctx->perm, td[q], ctx->ival2);
+        HIPCHK(hipMemcpyAsync(ti[q], ctx->ival2, n * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+    }
+    return 0;
+}
+
+int gz_set_active(gz_context *ctx, const unsigned char *active)
+{
+    if (!ctx) return GZ_E_ARG;
+    if (ctx->n <= 0) return fail(ctx, GZ_E_STATE, "nothing loaded");
+    CHK(set_device(ctx));
+    const long n = ctx->n;
+    if (!active) {
+        ctx->all_active = true;
+    } else {
+        HIPCHK(hipMemcpyAsync(ctx->stage, active, n, hipMemcpyHostToDevice, ctx->stream));
+        k_gather_u8<<<nblk(n), kBlock, 0, ctx->stream>>>(n, ctx->perm, (const uint8_t *)ctx->stage, ctx->ACT);
+        HIPCHK(hipGetLastError());
+        ctx->all_active = false;
+    }
+    ctx->sym_valid = false;
+    CHK(build_active(ctx));
+    return 0;
+}
+
+int gz_density(gz_context *ctx)
+{
+    if (!ctx) return GZ_E_ARG;
+    if (ctx->n <= 0) return fail(ctx, GZ_E_STATE, "nothing loaded");
+    CHK(set_device(ctx));
+    int rc = dispatch<DensityF>(ctx);
+    collect_timings(ctx);
+    return rc;
+}
+
+int gz_hydro_gradient_calc(gz_context *ctx)
+{
+    if (!ctx) return GZ_E_ARG;
+    CHK(set_device(ctx));
+    int rc = dispatch<GradF>(ctx);
+    collect_timings(ctx);
+    return rc;
+}
+
+int gz_hydro_force(gz_context *ctx)
+{
+    if (!ctx) return GZ_E_ARG;
+    CHK(set_device(ctx));
+    int rc = dispatch<ForceF>(ctx);
+    collect_timings(ctx);
+    return rc;
+}
+
+int gz_synchronize(gz_context *ctx)
+{
+    if (!ctx) return GZ_E_ARG;
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+int gz_permutation(gz_context *ctx, long *perm)
+{
+    if (!ctx || !perm) return GZ_E_ARG;
+    std::vector<int> p(ctx->n);
+    HIPCHK(hipMemcpy(p.data(), ctx->perm, ctx->n * sizeof(int), hipMemcpyDeviceToHost));
+    for (long k = 0; k < ctx->n; k++) perm[k] = p[k];
+    return 0;
+}
+
+int gz_stat(gz_context *ctx, const char *name, double *value)
+{
+    if (!ctx || !name || !value) return GZ_E_ARG;
+    std::string k(name);
+    if (k == "pairs_density" || k == "pairs_symmetric") {
+        const bool sym = (k == "pairs_symmetric");
+        const int *cnt = sym ? ctx->cnt_s : ctx->cnt_d;
+        if ((sym && !ctx->sym_valid) || (!sym && !ctx->density_done) || ctx->nact == 0) { *value = 0; return 0; }
+        std::vector<int> c(ctx->nact);
+        HIPCHK(hipMemcpy(c.data(), cnt, ctx->nact * sizeof(int), hipMemcpyDeviceToHost));
+        double s = 0;
+        for (int v : c) s += v;
+        *value = s;
+        return 0;
+    }
+    auto it = ctx->stats.find(k);
+    if (it == ctx->stats.end()) return fail(ctx, GZ_E_ARG, "unknown stat %s", name);
+    *value = it->second;
+    return 0;
+}
+
+int gz_timing_enable(gz_context *ctx, int on)
+{
+    if (!ctx) return GZ_E_ARG;
+    ctx->timing = on != 0;
+    return 0;
+}
+
+int gz_timing_reset(gz_context *ctx)
+{
+    if (!ctx) return GZ_E_ARG;
+    collect_timings(ctx);
+    ctx->timings.clear();
+    return 0;
+}
+
+int gz_timing(gz_context *ctx, const char *name, long *launches, double *total_ms)
+{
+    if (!ctx || !name) return GZ_E_ARG;
+    collect_timings(ctx);
+    auto it = ctx->timings.find(name);
+    if (launches) *launches = it == ctx->timings.end() ? 0 : it->second.first;
+    if (total_ms) *total_ms = it == ctx->timings.end() ? 0 : it->second.second;
+    return 0;
+}
+
+long gz_neighbors(gz_context *ctx, int symmetric, long *offsets, int *nbr, long cap)
+{
+    if (!ctx || !offsets) return -GZ_E_ARG;
+    if (set_device(ctx)) return -GZ_E_HIP;
+    const long n = ctx->n;
+    // a fresh exact list over every particle (no buffer): one-way at h_i or symmetric
+    std::vector<int> act_h(ctx->nact);
+    if (hipMemcpy(act_h.data(), ctx->act, ctx->nact * sizeof(int), hipMemcpyDeviceToHost) != hipSuccess) return -GZ_E_HIP;
+    int *ell = nullptr, *cnt = nullptr;
+    long ld = 0;
+    int capl = 0;
+    double hmax = 0;
+    if (max_on_device(ctx, ctx->H, n, &hmax)) return -GZ_E_HIP;
+    if (build_grid(ctx, std::min(hmax, 0.5 * ctx->par.box_size))) return -GZ_E_HIP;
+    if (dalloc(ctx, &cnt, ctx->nact)) return -GZ_E_MEMORY;
+    int rc = symmetric ? build_list<true>(ctx, ctx->nact, nullptr, nullptr, 1.0, &ell, &ld, &capl, cnt, 64)
+                       : build_list<false>(ctx, ctx->nact, nullptr, nullptr, 1.0, &ell, &ld, &capl, cnt, 64);
+    if (rc) { dfree(ell); dfree(cnt); return -rc; }
+    ctx->sym_valid = false;  // the grid changed
+    std::vector<int> c(ctx->nact), e((size_t)capl * ld);
+    std::vector<int> p(n);
+    hipMemcpy(c.data(), cnt, ctx->nact * sizeof(int), hipMemcpyDeviceToHost);
+    hipMemcpy(e.data(), ell, e.size() * sizeof(int), hipMemcpyDeviceToHost);
+    hipMemcpy(p.data(), ctx->perm, n * sizeof(int), hipMemcpyDeviceToHost);
+    dfree(ell);
+    dfree(cnt);
+    // rows in caller order
+    std::vector<long> rowlen(n, 0);
+    std::vector<int> slot_of(n, -1);
+    for (int r = 0; r < ctx->nact; r++) { rowlen[p[act_h[r]]] = c[r]; slot_of[p[act_h[r]]] = r; }
+    long tot = 0;
+    for (long u = 0; u < n; u++) { offsets[u] = tot; tot += rowlen[u]; }
+    offsets[n] = tot;
+    if (nbr && tot <= cap) {
+        for (long u = 0; u < n; u++) {
+            int r = slot_of[u];
+            if (r < 0) continue;
+            for (int q = 0; q < c[r]; q++) nbr[offsets[u] + q] = p[e[(size_t)q * ld + r]];
+        }
+    }
+    return tot;
+}
+
+}  // extern "C"

@@ -1,0 +1,68 @@
+// gz_kernels.h -- kernel declarations and the small views the kernels take by value.
+#pragma once
+#include "gz_device.h"
+
+namespace gz {
+
+constexpr int kFineLevel = 10;                 // 1024^3 Morton grid
+constexpr int kFine = 1 << kFineLevel;
+
+struct GridView {
+    int level, shift;         // nc = 2^level cells per side; cell code = key >> shift
+    double cell, box, half;
+    const int4 *sten;         // 27 entries per cell: (start, end, offset code, cell code)
+    const int *nsten;
+    const double *chmax;      // largest kernel radius in each cell
+};
+
+struct DensityIter {          // per active slot
+    double *hcur, *hbuild, *left, *right, *cn_cur;
+    int *cn_init, *iter;
+};
+
+struct DensityOut {           // per particle, device order; vectors SoA with column stride n
+    long n;
+    double *h, *nn, *drk, *divv, *rho, *p, *t, *fw, *cn, *fce;
+};
+
+struct ForceView {            // per particle, device order; vectors SoA with column stride n
+    long n;
+    const double *X, *Y, *Z, *VX, *VY, *VZ, *M, *H, *U, *RHO, *PR, *CN, *FCE, *DRK, *GR, *GP, *GV, *T;
+};
+
+__global__ void k_morton(long n, const double *pos, double box, uint32_t *key, int *val);
+__global__ void k_gather(long n, int width, const int *perm, const double *in, double *out, int soa);
+__global__ void k_gather_int(long n, const int *perm, const int *in, int *out);
+__global__ void k_gather_u8(long n, const int *perm, const uint8_t *in, uint8_t *out);
+__global__ void k_scatter(long n, int width, const int *perm, const double *in, double *out, int soa);
+__global__ void k_scatter_int(long n, const int *perm, const int *in, int *out);
+__global__ void k_flag_active(long n, const uint8_t *active, const double *mass, int *flag);
+__global__ void k_compact(long n, const int *flag, const int *scan, int *act);
+__global__ void k_cell_clear(long ncell, int *cstart, int *cend, double *chmax);
+__global__ void k_cell_bounds(long n, const uint32_t *key, int shift, int *cstart, int *cend);
+__global__ void k_cell_hmax(long ncell, const int *cstart, const int *cend, const double *h, double *chmax);
+__global__ void k_cell_stencil(int level, const int *cstart, const int *cend, int4 *sten, int *nsten);
+
+template <bool SYMMETRIC>
+__global__ void k_build_ell(int nrows, const int *rows, const int *act, const double *hrow, double scale, const double *X,
+                            const double *Y, const double *Z, const double *H, const uint32_t *key, GridView g, int *ell,
+                            long ld, int cap, int *cnt, int *overflow);
+
+template <int KERN, typename R>
+__global__ void k_density_iter(DevParams P, int ntodo, const int *todo, const int *act, const double *X, const double *Y,
+                               const double *Z, const double *VX, const double *VY, const double *VZ, const double *M,
+                               const double *U, const int *ell, long ld, const int *cnt, DensityIter it, DensityOut out,
+                               int *next, int *nnext, int *rebuild, int *err);
+
+template <int KERN, typename R>
+__global__ void k_gradients(DevParams P, int nrows, const int *act, const double *X, const double *Y, const double *Z,
+                            const double *VX, const double *VY, const double *VZ, const double *M, const double *H,
+                            const double *RHO, const double *PR, const double *T, const double *CN, long n, const int *ell,
+                            long ld, const int *cnt, double *GR, double *GP, double *GV);
+
+template <int KERN, typename R>
+__global__ void k_force(DevParams P, Gam<R> G, int nrows, const int *act, ForceView F, const int *ell, long ld,
+                        const int *cnt, const uint8_t *active, const int *timebin, const double *msv_in, int *wakeup,
+                        double *ACC, double *DTU, double *MSV, int *err);
+
+}  // namespace gz

@@ -1,0 +1,203 @@
+"""Python binding of the C ABI (include/gizmo_hydro.h) over ctypes.
+
+``Hydro`` owns one device context: ``load`` the P[]/CellP[] fields, run ``density``,
+``hydro_gradient_calc`` and ``hydro_force`` (the reference's entry points,
+src/hydro/density_loop.cc:764, src/hydro/gradients_loop.cc:458,
+src/hydro/hydro_force_loop.cc:423), ``store`` the results.  The binding never falls back
+to another implementation: without the built library every call raises.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libgizmo_hip.so")
+
+GZ_FP64 = 0
+GZ_FP32_MIXED = 1
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+_lp = C.POINTER(C.c_long)
+
+
+class Params(C.Structure):
+    _fields_ = [("box_size", C.c_double), ("kernel", C.c_int), ("gamma", C.c_double),
+                ("des_num_ngb", C.c_double), ("max_num_ngb_dev", C.c_double),
+                ("min_kernel_radius", C.c_double), ("max_kernel_radius", C.c_double),
+                ("time", C.c_double), ("time_begin", C.c_double), ("precision", C.c_int)]
+
+
+FIELD_WIDTH = dict(pos=3, vel=3, mass=1, h=1, u=1, numngb=1, drkern=1, divvel=1, rho=1, press=1,
+                   nvt=6, nvt_fw=3, cn=1, fce=1, grad_rho=3, grad_p=3, grad_v=9, accel=3, dtu=1, maxsig=1)
+INT_FIELDS = ("timebin", "wakeup")
+
+
+class Fields(C.Structure):
+    _fields_ = [(k, _dp) for k in FIELD_WIDTH] + [(k, _ip) for k in INT_FIELDS]
+
+
+class GizmoHipError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def lib():
+    """The loaded libgizmo_hip.so; raises when it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise GizmoHipError(-1, f"{LIB_PATH} is not built (run python -m gizmo_amd.build)")
+        L = C.CDLL(LIB_PATH)
+        L.gz_create.argtypes = [C.POINTER(Params), C.c_long, C.c_int]
+        L.gz_create.restype = C.c_void_p
+        L.gz_destroy.argtypes = [C.c_void_p]
+        L.gz_last_error.argtypes = [C.c_void_p]
+        L.gz_last_error.restype = C.c_char_p
+        L.gz_set_params.argtypes = [C.c_void_p, C.POINTER(Params)]
+        L.gz_load.argtypes = [C.c_void_p, C.c_long, C.POINTER(Fields)]
+        L.gz_store.argtypes = [C.c_void_p, C.POINTER(Fields)]
+        L.gz_set_active.argtypes = [C.c_void_p, C.POINTER(C.c_ubyte)]
+        for f in ("gz_density", "gz_hydro_gradient_calc", "gz_hydro_force", "gz_synchronize", "gz_timing_reset"):
+            getattr(L, f).argtypes = [C.c_void_p]
+        L.gz_permutation.argtypes = [C.c_void_p, _lp]
+        L.gz_stat.argtypes = [C.c_void_p, C.c_char_p, _dp]
+        L.gz_timing_enable.argtypes = [C.c_void_p, C.c_int]
+        L.gz_timing.argtypes = [C.c_void_p, C.c_char_p, _lp, _dp]
+        L.gz_neighbors.argtypes = [C.c_void_p, C.c_int, _lp, _ip, C.c_long]
+        L.gz_neighbors.restype = C.c_long
+        _lib = L
+    return _lib
+
+
+def make_params(p, precision=GZ_FP64):
+    return Params(box_size=p["box"], kernel=p["kernel"], gamma=p["gamma"], des_num_ngb=p["des_num_ngb"],
+                  max_num_ngb_dev=p["max_num_ngb_dev"], min_kernel_radius=p["min_kernel_radius"],
+                  max_kernel_radius=p["max_kernel_radius"], time=p["time"], time_begin=p["time_begin"],
+                  precision=precision)
+
+
+class Hydro:
+    """One periodic box resident on one GPU."""
+
+    def __init__(self, params, capacity, precision=GZ_FP64, device=0):
+        self._p = make_params(params, precision)
+        self.ctx = lib().gz_create(C.byref(self._p), int(capacity), int(device))
+        if not self.ctx:
+            raise GizmoHipError(-1, "gz_create failed (no device, or parameters rejected)")
+        self.n = 0
+
+    def close(self):
+        if self.ctx:
+            lib().gz_destroy(self.ctx)
+            self.ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        if rc:
+            raise GizmoHipError(rc, lib().gz_last_error(self.ctx).decode())
+
+    def set_params(self, params, precision=None):
+        prec = self._p.precision if precision is None else precision
+        self._p = make_params(params, prec)
+        self._check(lib().gz_set_params(self.ctx, C.byref(self._p)))
+
+    def load(self, fields):
+        """fields: dict name -> numpy array (caller order). pos, vel, mass, u, h required."""
+        f = Fields()
+        keep = []
+        n = len(fields["mass"])
+        for k, w in FIELD_WIDTH.items():
+            if k in fields and fields[k] is not None:
+                a = np.ascontiguousarray(fields[k], dtype=np.float64).reshape(n * w)
+                keep.append(a)
+                setattr(f, k, a.ctypes.data_as(_dp))
+        for k in INT_FIELDS:
+            if k in fields and fields[k] is not None:
+                a = np.ascontiguousarray(fields[k], dtype=np.int32).reshape(n)
+                keep.append(a)
+                setattr(f, k, a.ctypes.data_as(_ip))
+        self._check(lib().gz_load(self.ctx, n, C.byref(f)))
+        self.n = n
+
+    def store(self, names=None):
+        names = list(FIELD_WIDTH) + list(INT_FIELDS) if names is None else list(names)
+        f = Fields()
+        out = {}
+        for k in names:
+            if k in FIELD_WIDTH:
+                w = FIELD_WIDTH[k]
+                a = np.zeros((self.n, w) if w > 1 else self.n)
+                setattr(f, k, a.ctypes.data_as(_dp))
+            else:
+                a = np.zeros(self.n, dtype=np.int32)
+                setattr(f, k, a.ctypes.data_as(_ip))
+            out[k] = a
+        self._check(lib().gz_store(self.ctx, C.byref(f)))
+        return out
+
+    def set_active(self, active=None):
+        if active is None:
+            self._check(lib().gz_set_active(self.ctx, None))
+        else:
+            a = np.ascontiguousarray(np.asarray(active, dtype=np.uint8))
+            self._check(lib().gz_set_active(self.ctx, a.ctypes.data_as(C.POINTER(C.c_ubyte))))
+
+    def density(self):
+        self._check(lib().gz_density(self.ctx))
+
+    def hydro_gradient_calc(self):
+        self._check(lib().gz_hydro_gradient_calc(self.ctx))
+
+    def hydro_force(self):
+        self._check(lib().gz_hydro_force(self.ctx))
+
+    def hydro_pass(self):
+        self.density()
+        self.hydro_gradient_calc()
+        self.hydro_force()
+
+    def synchronize(self):
+        self._check(lib().gz_synchronize(self.ctx))
+
+    def permutation(self):
+        p = np.zeros(self.n, dtype=np.int64)
+        self._check(lib().gz_permutation(self.ctx, p.ctypes.data_as(_lp)))
+        return p
+
+    def stat(self, name):
+        v = C.c_double()
+        self._check(lib().gz_stat(self.ctx, name.encode(), C.byref(v)))
+        return v.value
+
+    def timing_enable(self, on=True):
+        self._check(lib().gz_timing_enable(self.ctx, 1 if on else 0))
+
+    def timing_reset(self):
+        self._check(lib().gz_timing_reset(self.ctx))
+
+    def timing(self, name):
+        cnt, ms = C.c_long(), C.c_double()
+        self._check(lib().gz_timing(self.ctx, name.encode(), C.byref(cnt), C.byref(ms)))
+        return cnt.value, ms.value
+
+    def neighbors(self, symmetric):
+        off = np.zeros(self.n + 1, dtype=np.int64)
+        tot = lib().gz_neighbors(self.ctx, int(symmetric), off.ctypes.data_as(_lp), None, 0)
+        if tot < 0:
+            self._check(-tot)
+        nb = np.zeros(max(tot, 1), dtype=np.int32)
+        tot2 = lib().gz_neighbors(self.ctx, int(symmetric), off.ctypes.data_as(_lp), nb.ctypes.data_as(_ip), tot)
+        if tot2 != tot:
+            raise GizmoHipError(-1, "neighbour count changed between calls")
+        return off, nb[:tot]

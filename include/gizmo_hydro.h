@@ -1,0 +1,136 @@
+/*
+ * gizmo_hydro.h -- C ABI of the MI355X-native MFM hydro neighbour-loop pipeline.
+ *
+ * The library replaces the three hot-path entry points of GIZMO's timestep loop,
+ *   void density(void);              src/hydro/density_loop.cc:764
+ *   void hydro_gradient_calc(void);  src/hydro/gradients_loop.cc:458
+ *   void hydro_force(void);          src/hydro/hydro_force_loop.cc:423
+ * which the reference calls from compute_hydro_densities_and_forces()
+ * (src/core/accel.cc:80,143,169) and which read and write the P[]/CellP[] particle
+ * fields and the All parameters.  Here the particle state lives on the GPU inside a
+ * context; gz_load()/gz_store() move the P[]/CellP[] hot-path fields in and out (in the
+ * caller's particle order), and gz_density()/gz_hydro_gradient_calc()/gz_hydro_force()
+ * run the three stages on the resident state.  Plain pointers and sizes only.
+ *
+ * Error convention: every entry point returns 0 on success, or a nonzero code
+ * (GZ_E_* below, or the reference's own stop codes where a stage hits one: 1234 when no
+ * Riemann solution is found (src/hydro/hydro_core_meshless_functions.h:424), 1155 when the
+ * kernel-radius iteration does not converge in MAXITER rounds, 90001009 for an empty
+ * radius bracket (src/hydro/density_loop.h:893)).  gz_last_error() gives the message.
+ * The reference maps such codes to endrun(); a drop-in shim does the same.
+ */
+#ifndef GIZMO_HYDRO_H
+#define GIZMO_HYDRO_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GZ_OK 0
+#define GZ_E_ARG 1          /* bad argument (null pointer, size, parameter out of range) */
+#define GZ_E_HIP 2          /* a HIP runtime call failed */
+#define GZ_E_STATE 3        /* call out of order (e.g. gradients before density) */
+#define GZ_E_MEMORY 4       /* device allocation failed */
+#define GZ_STOP_RIEMANN 1234
+#define GZ_STOP_MAXITER 1155
+#define GZ_STOP_BRACKET 90001009
+
+/* arithmetic forms of the pair physics */
+#define GZ_FP64 0           /* plain FP64, the reference's own precision (MyDouble = double) */
+#define GZ_FP32_MIXED 1     /* FP32 pair math on relative coordinates; FP64 positions, 3x3 inverse, h test */
+
+/* Run parameters: the All fields the hot path reads (src/declarations/global_data_all_struct.h)
+   plus the compile-time choices of the reference build (KERNEL_FUNCTION, GAMMA_DEFAULT). */
+typedef struct {
+    double box_size;          /* All.BoxSize (periodic cube) */
+    int kernel;               /* KERNEL_FUNCTION: 3 cubic spline, 6 Wendland C2 */
+    double gamma;             /* GAMMA_DEFAULT (adiabatic index, > 1) */
+    double des_num_ngb;       /* All.DesNumNgb */
+    double max_num_ngb_dev;   /* All.MaxNumNgbDeviation */
+    double min_kernel_radius; /* All.MinKernelRadius */
+    double max_kernel_radius; /* All.MaxKernelRadius */
+    double time;              /* All.Time */
+    double time_begin;        /* All.TimeBegin */
+    int precision;            /* GZ_FP64 or GZ_FP32_MIXED */
+} gz_params;
+
+/* Host arrays of the P[]/CellP[] hot-path fields, n entries each in the caller's order;
+   vectors are interleaved (n x 3, n x 6, n x 9).  A NULL pointer means "not provided"
+   (gz_load) or "not wanted" (gz_store).  Names follow the reference fields. */
+typedef struct {
+    double *pos;       /* P.Pos[3] */
+    double *vel;       /* CellP.VelPred[3] */
+    double *mass;      /* P.Mass */
+    double *h;         /* P.KernelRadius */
+    double *u;         /* CellP.InternalEnergyPred */
+    double *numngb;    /* P.NumNgb (stored as its cube root, as the reference does) */
+    double *drkern;    /* P.DrkernNgbFactor */
+    double *divvel;    /* P.Particle_DivVel */
+    double *rho;       /* CellP.Density */
+    double *press;     /* CellP.Pressure */
+    double *nvt;       /* CellP.NV_T (inverted, [xx,yy,zz,xy,yz,xz]) */
+    double *nvt_fw;    /* CellP.NV_T_face_weights[3] */
+    double *cn;        /* CellP.ConditionNumber */
+    double *fce;       /* CellP.FaceClosureError */
+    double *grad_rho;  /* CellP.Gradients.Density[3] */
+    double *grad_p;    /* CellP.Gradients.Pressure[3] */
+    double *grad_v;    /* CellP.Gradients.Velocity[3][3] ([component][direction]) */
+    double *accel;     /* CellP.HydroAccel[3] (dv/dt after hydro_force) */
+    double *dtu;       /* CellP.DtInternalEnergy (du/dt after hydro_force) */
+    double *maxsig;    /* CellP.MaxSignalVel */
+    int *timebin;      /* P.TimeBin */
+    int *wakeup;       /* P.wakeup */
+} gz_fields;
+
+typedef struct gz_context gz_context;
+
+/* Create a context on HIP device `device` for up to `capacity` particles. NULL on failure. */
+gz_context *gz_create(const gz_params *params, long capacity, int device);
+void gz_destroy(gz_context *ctx);
+const char *gz_last_error(const gz_context *ctx);
+
+/* Replace the run parameters (All.Time changes every step; the rest normally not). */
+int gz_set_params(gz_context *ctx, const gz_params *params);
+
+/* Host -> device: n particles.  pos, vel, mass, u and h are required; the other fields are
+   optional (the state an inactive particle keeps between its active steps).  The library
+   orders the particles along a space-filling curve on the device. */
+int gz_load(gz_context *ctx, long n, const gz_fields *host);
+/* Device -> host, in the caller's order, for every non-NULL pointer of `host`. */
+int gz_store(gz_context *ctx, gz_fields *host);
+
+/* The active set (ActiveParticleList): one byte per particle in the caller's order, nonzero =
+   active; NULL = every particle active. */
+int gz_set_active(gz_context *ctx, const unsigned char *active);
+
+/* The three stages (density -> gradients -> force must run in that order after a gz_load or
+   after positions change). */
+int gz_density(gz_context *ctx);              /* replaces density(), src/hydro/density_loop.cc:764 */
+int gz_hydro_gradient_calc(gz_context *ctx);  /* replaces hydro_gradient_calc(), src/hydro/gradients_loop.cc:458 */
+int gz_hydro_force(gz_context *ctx);          /* replaces hydro_force(), src/hydro/hydro_force_loop.cc:423 */
+
+/* Wait for the context's stream. */
+int gz_synchronize(gz_context *ctx);
+
+/* Diagnostics.  gz_permutation: perm[k] = caller index of the k-th particle in device order.
+   gz_stat: named counters of the last calls ("density_iterations", "pairs_density",
+   "pairs_symmetric", "ngb_rebuilds", "n_active"). */
+int gz_permutation(gz_context *ctx, long *perm);
+int gz_stat(gz_context *ctx, const char *name, double *value);
+
+/* Kernel timing on the context's stream (HIP events around each launch).  gz_timing
+   returns the launch count and total milliseconds of the named kernel family since the last
+   gz_timing_reset ("ngb_build", "density", "gradients", "force", "sort"). */
+int gz_timing_enable(gz_context *ctx, int on);
+int gz_timing_reset(gz_context *ctx);
+int gz_timing(gz_context *ctx, const char *name, long *launches, double *total_ms);
+
+/* Neighbour lists of the resident state, in the caller's indexing, for parity tests.
+   symmetric = 0: rows r2 < h_i^2 (self included); 1: r2 < max(h_i,h_j)^2 (self excluded).
+   Row order is ascending device order.  Call with nbr = NULL to get the total count. */
+long gz_neighbors(gz_context *ctx, int symmetric, long *offsets, int *nbr, long cap);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GIZMO_HYDRO_H */

@@ -1,0 +1,71 @@
+/*
+ * gz_oracle.h -- interface of the CPU FP64 oracle (test infrastructure only; see gz_oracle.c).
+ * Arrays are per particle, in the caller's index order; vectors are interleaved (n x 3).
+ */
+#ifndef GZ_ORACLE_H
+#define GZ_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct {
+    double box;               /* periodic cubic box side */
+    int kernel;               /* 3 = cubic spline, 6 = Wendland C2 */
+    double gamma;             /* adiabatic index */
+    double des_num_ngb;       /* All.DesNumNgb */
+    double max_num_ngb_dev;   /* All.MaxNumNgbDeviation */
+    double min_kernel_radius; /* All.MinKernelRadius */
+    double max_kernel_radius; /* All.MaxKernelRadius */
+    double time, time_begin;  /* All.Time, All.TimeBegin */
+} orc_params;
+
+typedef struct {
+    long n;
+    /* P[] fields */
+    double *pos;      /* n*3 Pos */
+    double *mass;     /* Mass */
+    double *h;        /* KernelRadius */
+    double *numngb;   /* NumNgb (cube root after density) */
+    double *drkern;   /* DrkernNgbFactor */
+    double *divvel;   /* Particle_DivVel */
+    int *timebin;     /* TimeBin (may be NULL) */
+    int *wakeup;      /* wakeup (may be NULL) */
+    /* CellP[] fields */
+    double *vel;      /* n*3 VelPred */
+    double *u;        /* InternalEnergyPred */
+    double *rho;      /* Density */
+    double *press;    /* Pressure */
+    double *nvt;      /* n*6 NV_T (inverted; xx,yy,zz,xy,yz,xz) */
+    double *nvt_fw;   /* n*3 NV_T_face_weights */
+    double *cn;       /* ConditionNumber */
+    double *fce;      /* FaceClosureError */
+    double *grad_rho; /* n*3 Gradients.Density */
+    double *grad_p;   /* n*3 Gradients.Pressure */
+    double *grad_v;   /* n*9 Gradients.Velocity[component][direction] */
+    double *accel;    /* n*3 HydroAccel (dv/dt after force) */
+    double *dtu;      /* DtInternalEnergy (du/dt after force) */
+    double *maxsig;   /* MaxSignalVel */
+} orc_state;
+
+int orc_kernel(int kernel, double r, double h, double *wk, double *dwk);
+long orc_neighbors(const orc_params *par, long n, const double *pos, const double *h, int symmetric,
+                   const unsigned char *rowmask, long *offsets, int *nbr, long cap);
+double orc_matrix_invert(const double *T, double *Tinv);
+int orc_density(const orc_params *par, orc_state *s, const unsigned char *active, int *iters_out);
+int orc_density_fixed_h(const orc_params *par, orc_state *s, const unsigned char *active, double *raw);
+void orc_slopelimiter(double *g, double valmax, double valmin, double alim, double h,
+                      double shoot_tol, int pos_preserve, double d_max, double val_cen);
+int orc_gradients(const orc_params *par, orc_state *s, const unsigned char *active);
+void orc_reconstruct(double Q_i, const double *Gi, double Q_j, const double *Gj,
+                     const double *di, const double *dj, double *Q_L, double *Q_R, int mode);
+int orc_riemann(double gamma, const double *left, const double *right, const double *n,
+                double ptl, double *P_M, double *S_M, double *fv, double *fp);
+int orc_riemann_exact(double gamma, const double *left, const double *right, double *P_M, double *S_M);
+int orc_force(const orc_params *par, orc_state *s, const unsigned char *active);
+int orc_num_threads(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

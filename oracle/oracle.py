@@ -1,0 +1,247 @@
+"""ctypes binding of the CPU FP64 oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg
+use this module as the checker or the timed CPU reference.  The product package never
+imports it.
+
+``State`` holds the P[]/CellP[] hot-path fields as numpy arrays in the caller's index order;
+``density``, ``gradients`` and ``force`` restate density(), hydro_gradient_calc() and
+hydro_force() on it in place.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+_lp = C.POINTER(C.c_long)
+_up = C.POINTER(C.c_ubyte)
+
+
+class Params(C.Structure):
+    _fields_ = [("box", C.c_double), ("kernel", C.c_int), ("gamma", C.c_double),
+                ("des_num_ngb", C.c_double), ("max_num_ngb_dev", C.c_double),
+                ("min_kernel_radius", C.c_double), ("max_kernel_radius", C.c_double),
+                ("time", C.c_double), ("time_begin", C.c_double)]
+
+
+class _CState(C.Structure):
+    _fields_ = [("n", C.c_long), ("pos", _dp), ("mass", _dp), ("h", _dp), ("numngb", _dp),
+                ("drkern", _dp), ("divvel", _dp), ("timebin", _ip), ("wakeup", _ip),
+                ("vel", _dp), ("u", _dp), ("rho", _dp), ("press", _dp), ("nvt", _dp),
+                ("nvt_fw", _dp), ("cn", _dp), ("fce", _dp), ("grad_rho", _dp), ("grad_p", _dp),
+                ("grad_v", _dp), ("accel", _dp), ("dtu", _dp), ("maxsig", _dp)]
+
+
+# field name -> columns per particle
+FIELDS = dict(pos=3, mass=1, h=1, numngb=1, drkern=1, divvel=1, vel=3, u=1, rho=1, press=1,
+              nvt=6, nvt_fw=3, cn=1, fce=1, grad_rho=3, grad_p=3, grad_v=9, accel=3, dtu=1, maxsig=1)
+INT_FIELDS = ("timebin", "wakeup")
+
+
+def build(force=False):
+    if force or not os.path.exists(_LIB):
+        subprocess.run(["make", "-s", "-C", _HERE, "-B" if force else "all"], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        build()
+        L = C.CDLL(_LIB)
+        L.orc_kernel.argtypes = [C.c_int, C.c_double, C.c_double, _dp, _dp]
+        L.orc_neighbors.argtypes = [C.POINTER(Params), C.c_long, _dp, _dp, C.c_int, _up, _lp, _ip, C.c_long]
+        L.orc_neighbors.restype = C.c_long
+        L.orc_matrix_invert.argtypes = [_dp, _dp]
+        L.orc_matrix_invert.restype = C.c_double
+        L.orc_density.argtypes = [C.POINTER(Params), C.POINTER(_CState), _up, _ip]
+        L.orc_density_fixed_h.argtypes = [C.POINTER(Params), C.POINTER(_CState), _up, _dp]
+        L.orc_slopelimiter.argtypes = [_dp, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
+                                       C.c_int, C.c_double, C.c_double]
+        L.orc_gradients.argtypes = [C.POINTER(Params), C.POINTER(_CState), _up]
+        L.orc_reconstruct.argtypes = [C.c_double, _dp, C.c_double, _dp, _dp, _dp, _dp, _dp, C.c_int]
+        L.orc_riemann.argtypes = [C.c_double, _dp, _dp, _dp, C.c_double, _dp, _dp, _dp, _dp]
+        L.orc_riemann_exact.argtypes = [C.c_double, _dp, _dp, _dp, _dp]
+        L.orc_force.argtypes = [C.POINTER(Params), C.POINTER(_CState), _up]
+        L.orc_num_threads.restype = C.c_int
+        _lib = L
+    return _lib
+
+
+def _ptr(a, t=_dp):
+    return a.ctypes.data_as(t) if a is not None else None
+
+
+def params(d):
+    return Params(**{k: d[k] for k, _ in Params._fields_})
+
+
+class State:
+    """P[]/CellP[] hot-path fields of n particles (numpy, caller's order)."""
+
+    def __init__(self, n):
+        self.n = int(n)
+        for k, c in FIELDS.items():
+            setattr(self, k, np.zeros((self.n, c)) if c > 1 else np.zeros(self.n))
+        self.timebin = np.zeros(self.n, dtype=np.int32)
+        self.wakeup = np.zeros(self.n, dtype=np.int32)
+
+    @classmethod
+    def from_ic(cls, ic):
+        s = cls(len(ic["mass"]))
+        for k in ("pos", "vel", "mass", "u", "h"):
+            getattr(s, k)[...] = ic[k]
+        return s
+
+    def copy(self):
+        s = State(self.n)
+        for k in list(FIELDS) + list(INT_FIELDS):
+            getattr(s, k)[...] = getattr(self, k)
+        return s
+
+    def permuted(self, perm):
+        """A copy with particle k taken from index perm[k]."""
+        s = State(self.n)
+        for k in list(FIELDS) + list(INT_FIELDS):
+            getattr(s, k)[...] = getattr(self, k)[perm]
+        return s
+
+    def _c(self):
+        for k in list(FIELDS) + list(INT_FIELDS):
+            a = getattr(self, k)
+            if not (a.flags.c_contiguous and a.dtype in (np.float64, np.int32)):
+                setattr(self, k, np.ascontiguousarray(a))
+        cs = _CState()
+        cs.n = self.n
+        for k in FIELDS:
+            setattr(cs, k, _ptr(getattr(self, k)))
+        cs.timebin = _ptr(self.timebin, _ip)
+        cs.wakeup = _ptr(self.wakeup, _ip)
+        return cs
+
+
+def _mask(active, n):
+    if active is None:
+        return None, None
+    a = np.ascontiguousarray(np.asarray(active, dtype=np.uint8))
+    assert a.shape == (n,)
+    return a, _ptr(a, _up)
+
+
+def kernel(kid, r, h):
+    wk, dwk = C.c_double(), C.c_double()
+    rc = lib().orc_kernel(kid, r, h, C.byref(wk), C.byref(dwk))
+    if rc:
+        raise ValueError("kernel")
+    return wk.value, dwk.value
+
+
+def neighbors(par, pos, h, symmetric, rowmask=None):
+    pos = np.ascontiguousarray(pos, dtype=np.float64)
+    h = np.ascontiguousarray(h, dtype=np.float64)
+    n = len(h)
+    p = params(par)
+    off = np.zeros(n + 1, dtype=np.int64)
+    m, mp = _mask(rowmask, n)
+    tot = lib().orc_neighbors(C.byref(p), n, _ptr(pos), _ptr(h), int(symmetric), mp, _ptr(off, _lp), None, 0)
+    if tot < 0:
+        raise RuntimeError("orc_neighbors failed")
+    nb = np.zeros(max(tot, 1), dtype=np.int32)
+    tot2 = lib().orc_neighbors(C.byref(p), n, _ptr(pos), _ptr(h), int(symmetric), mp, _ptr(off, _lp), _ptr(nb, _ip), tot)
+    assert tot2 == tot
+    return off, nb[:tot]
+
+
+def matrix_invert(T):
+    T = np.ascontiguousarray(T, dtype=np.float64).reshape(9)
+    Ti = np.zeros(9)
+    cn = lib().orc_matrix_invert(_ptr(T), _ptr(Ti))
+    return cn, Ti.reshape(3, 3)
+
+
+def density(par, st, active=None, return_iters=False):
+    p = params(par)
+    cs = st._c()
+    m, mp = _mask(active, st.n)
+    it = np.zeros(st.n, dtype=np.int32)
+    rc = lib().orc_density(C.byref(p), C.byref(cs), mp, _ptr(it, _ip))
+    if rc:
+        raise RuntimeError(f"orc_density stop code {rc}")
+    return it if return_iters else None
+
+
+def density_fixed_h(par, st, active=None):
+    p = params(par)
+    cs = st._c()
+    m, mp = _mask(active, st.n)
+    raw = np.zeros((st.n, 13))
+    rc = lib().orc_density_fixed_h(C.byref(p), C.byref(cs), mp, _ptr(raw))
+    if rc:
+        raise RuntimeError(f"orc_density_fixed_h failed {rc}")
+    return raw
+
+
+def gradients(par, st, active=None):
+    p = params(par)
+    cs = st._c()
+    m, mp = _mask(active, st.n)
+    rc = lib().orc_gradients(C.byref(p), C.byref(cs), mp)
+    if rc:
+        raise RuntimeError(f"orc_gradients failed {rc}")
+
+
+def force(par, st, active=None):
+    p = params(par)
+    cs = st._c()
+    m, mp = _mask(active, st.n)
+    rc = lib().orc_force(C.byref(p), C.byref(cs), mp)
+    if rc:
+        raise RuntimeError(f"orc_force stop code {rc}")
+
+
+def slopelimiter(g, valmax, valmin, alim, h, shoot_tol, pos_preserve, d_max, val_cen):
+    g = np.array(g, dtype=np.float64)
+    lib().orc_slopelimiter(_ptr(g), valmax, valmin, alim, h, shoot_tol, pos_preserve, d_max, val_cen)
+    return g
+
+
+def reconstruct(Qi, Gi, Qj, Gj, di, dj, mode=1):
+    a = [np.ascontiguousarray(x, dtype=np.float64) for x in (Gi, Gj, di, dj)]
+    QL, QR = C.c_double(), C.c_double()
+    lib().orc_reconstruct(Qi, _ptr(a[0]), Qj, _ptr(a[1]), _ptr(a[2]), _ptr(a[3]), C.byref(QL), C.byref(QR), mode)
+    return QL.value, QR.value
+
+
+def riemann(gamma, left, right, n, ptl):
+    L = np.ascontiguousarray(left, dtype=np.float64)
+    R = np.ascontiguousarray(right, dtype=np.float64)
+    nn = np.ascontiguousarray(n, dtype=np.float64)
+    PM, SM, fp = C.c_double(), C.c_double(), C.c_double()
+    fv = np.zeros(3)
+    rc = lib().orc_riemann(gamma, _ptr(L), _ptr(R), _ptr(nn), ptl, C.byref(PM), C.byref(SM), _ptr(fv), C.byref(fp))
+    return rc, PM.value, SM.value, fv, fp.value
+
+
+def riemann_exact(gamma, left, right):
+    L = np.ascontiguousarray(left, dtype=np.float64)
+    R = np.ascontiguousarray(right, dtype=np.float64)
+    PM, SM = C.c_double(), C.c_double()
+    lib().orc_riemann_exact(gamma, _ptr(L), _ptr(R), C.byref(PM), C.byref(SM))
+    return PM.value, SM.value
+
+
+def hydro_pass(par, st, active=None):
+    """density() -> hydro_gradient_calc() -> hydro_force() on st, in place."""
+    density(par, st, active)
+    gradients(par, st, active)
+    force(par, st, active)
+
+
+def num_threads():
+    return lib().orc_num_threads()

@@ -1,0 +1,307 @@
+"""Pins the CPU oracle (oracle/gz_oracle.c) before anything is compared against it.
+
+Each check is chosen so that a plausible mistake in the restatement fails it:
+  * kernel: bit-for-bit against the reference's own kernel.h compiled in oracle/_ref/, plus
+    normalisation, W(1) = 0 and continuity of the cubic spline at u = 1/2;
+  * 3x3 inverse: bit-for-bit against the reference's Mat3::invert (oracle/_ref/), T T^-1 = I,
+    and the condition-number definition;
+  * neighbour sets: O(N^2) brute force with numpy, exact equality;
+  * density at fixed h: closed form on an unjittered lattice (sum over lattice vectors);
+  * gradients: exact for linear fields (the least-squares matrix estimator reproduces them);
+  * slope limiter and face reconstruction: closed-form cases;
+  * Riemann: Toro's exact-solver tables, symmetric states, mirror symmetry, vacuum branch;
+  * force: uniform medium at rest gives zero rates; momentum and energy sums vanish to roundoff.
+"""
+import ctypes as C
+import math
+import os
+
+import numpy as np
+import pytest
+
+from gizmo_amd import ics
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REFDIR = os.path.join(os.path.dirname(HERE), "oracle", "_ref")
+
+
+def _ref(kern):
+    p = os.path.join(REFDIR, f"libref_k{kern}.so")
+    if not os.path.exists(p):
+        pytest.skip("reference headers not compiled here (oracle/_ref is built where the reference source tree exists)")
+    L = C.CDLL(p)
+    dp = C.POINTER(C.c_double)
+    L.ref_kernel.argtypes = [C.c_double, C.c_double, dp, dp]
+    L.ref_mat3_invert.argtypes = [dp, dp]
+    L.ref_mat3_invert.restype = C.c_double
+    L.ref_mat3_frobenius_norm_sq.argtypes = [dp]
+    L.ref_mat3_frobenius_norm_sq.restype = C.c_double
+    L.ref_sym_matvec.argtypes = [dp, dp, dp]
+    return L
+
+
+# ------------------------------------------------------------------------------------------
+# kernel
+# ------------------------------------------------------------------------------------------
+@pytest.mark.parametrize("kern", [3, 6])
+def test_kernel_bitwise_vs_reference(oracle_lib, kern):
+    L = _ref(kern)
+    rng = np.random.default_rng(7)
+    for h in (0.013, 0.5, 1.0, 3.7):
+        for r in np.concatenate([rng.uniform(0, h, 200), [0.0, 0.5 * h, h * (1 - 1e-15), h]]):
+            w0, d0 = oracle_lib.kernel(kern, float(r), h)
+            w1, d1 = C.c_double(), C.c_double()
+            L.ref_kernel(float(r), h, C.byref(w1), C.byref(d1))
+            assert w0 == w1.value and d0 == d1.value, (kern, r, h)
+
+
+@pytest.mark.parametrize("kern", [3, 6])
+def test_kernel_normalisation_and_edges(oracle_lib, kern):
+    h = 0.7
+    r = np.linspace(0, h, 20001)
+    w = np.array([oracle_lib.kernel(kern, float(x), h)[0] for x in r])
+    integral = np.trapezoid(4 * np.pi * r ** 2 * w, r)
+    assert abs(integral - 1.0) < 1e-6
+    assert oracle_lib.kernel(kern, h, h) == (0.0, 0.0)
+    # dW/dr is the derivative of W (central differences)
+    x, e = 0.31 * h, 1e-6
+    wp, _ = oracle_lib.kernel(kern, x + e, h)
+    wm, _ = oracle_lib.kernel(kern, x - e, h)
+    _, dw = oracle_lib.kernel(kern, x, h)
+    assert abs((wp - wm) / (2 * e) - dw) < 1e-6 * abs(dw)
+    if kern == 3:  # value and slope continuous at u = 1/2
+        a, da = oracle_lib.kernel(3, 0.5 * h * (1 - 1e-12), h)
+        b, db = oracle_lib.kernel(3, 0.5 * h * (1 + 1e-12), h)
+        assert abs(a - b) < 1e-9 * a and abs(da - db) < 1e-9 * abs(da)
+        assert oracle_lib.kernel(3, 0.0, h)[0] == pytest.approx(8 / np.pi / h ** 3, rel=1e-15)
+    else:
+        assert oracle_lib.kernel(6, 0.0, h)[0] == pytest.approx(21 / (2 * np.pi) / h ** 3, rel=1e-15)
+
+
+# ------------------------------------------------------------------------------------------
+# 3x3 inverse
+# ------------------------------------------------------------------------------------------
+def test_matrix_invert_vs_reference_and_identity(oracle_lib):
+    L = _ref(6)
+    rng = np.random.default_rng(3)
+    dp = C.POINTER(C.c_double)
+    for _ in range(200):
+        A = rng.normal(size=(3, 3))
+        T = A @ A.T + 0.1 * np.eye(3)
+        cn, Ti = oracle_lib.matrix_invert(T)
+        Tr = np.ascontiguousarray(T.reshape(9))
+        Tir = np.zeros(9)
+        L.ref_mat3_invert(Tr.ctypes.data_as(dp), Tir.ctypes.data_as(dp))
+        assert np.array_equal(Ti.reshape(9), Tir)
+        f1 = L.ref_mat3_frobenius_norm_sq(Tr.ctypes.data_as(dp))
+        f2 = L.ref_mat3_frobenius_norm_sq(np.ascontiguousarray(Tir).ctypes.data_as(dp))
+        assert cn == max(math.sqrt(f1 * f2) / 3, 1)
+        assert np.allclose(T @ Ti, np.eye(3), atol=1e-10)
+    cn, Ti = oracle_lib.matrix_invert(np.zeros((3, 3)))
+    assert cn == 1 and not Ti.any()
+    cn, Ti = oracle_lib.matrix_invert(np.diag([1.0, 1.0, 0.0]))  # singular
+    assert cn == 1 and not Ti.any()
+
+
+# ------------------------------------------------------------------------------------------
+# neighbours
+# ------------------------------------------------------------------------------------------
+def _brute_neighbors(pos, h, box, symmetric):
+    d = pos[:, None, :] - pos[None, :, :]
+    d = np.abs(d)
+    d = np.where(d > 0.5 * box, box - d, d)
+    r2 = (d * d).sum(-1)
+    if symmetric:
+        hm = np.maximum(h[:, None], h[None, :])
+        acc = r2 < hm * hm
+        np.fill_diagonal(acc, False)
+    else:
+        acc = r2 < (h * h)[:, None]
+    return [np.nonzero(row)[0] for row in acc]
+
+
+@pytest.mark.parametrize("symmetric", [False, True])
+def test_neighbors_brute_force(oracle_lib, symmetric):
+    rng = np.random.default_rng(11)
+    n = 1500
+    pos = rng.uniform(0, 1, size=(n, 3))
+    pos[:20] = rng.uniform(0, 0.01, size=(20, 3))     # a clump at the corner: periodic images
+    pos[20:40] = 1 - rng.uniform(0, 0.01, size=(20, 3))
+    h = rng.uniform(0.05, 0.15, size=n)
+    par = dict(box=1.0, kernel=3, gamma=5 / 3, des_num_ngb=32.0, max_num_ngb_dev=0.05, min_kernel_radius=0,
+               max_kernel_radius=0.25, time=0.0, time_begin=0.0)
+    off, nb = oracle_lib.neighbors(par, pos, h, symmetric)
+    ref = _brute_neighbors(pos, h, 1.0, symmetric)
+    for i in range(n):
+        assert np.array_equal(nb[off[i]:off[i + 1]], ref[i]), i
+
+
+# ------------------------------------------------------------------------------------------
+# density
+# ------------------------------------------------------------------------------------------
+@pytest.mark.parametrize("kern,nngb", [(3, 32.0), (6, 64.0)])
+def test_density_lattice_closed_form(oracle_lib, kern, nngb):
+    O = oracle_lib
+    ns = 12
+    ic = ics.lattice_box(ns, 5, jitter=0.0, mach_sigma=0.0, kernel=kern, des_num_ngb=nngb)
+    par = dict(ic["params"])
+    st = O.State.from_ic(ic)
+    h = 2.13 / ns
+    st.h[:] = h
+    st.vel[:] = [0.3, -0.2, 0.1]   # uniform velocity: no divergence
+    O.density_fixed_h(par, st)
+    # closed form: NumNgb = (4pi/3) h^3 sum_k W(|r_k|), r_k over lattice vectors within h
+    dx = 1.0 / ns
+    g = np.arange(-3, 4) * dx
+    R = np.sqrt(g[:, None, None] ** 2 + g[None, :, None] ** 2 + g[None, None, :] ** 2).ravel()
+    R = R[R < h]
+    W = np.array([O.kernel(kern, float(x), h)[0] for x in R])
+    numngb = ics.VOLNORM * h ** 3 * W.sum()
+    rho = (1.0 / ns ** 3) * numngb / (ics.VOLNORM * h ** 3)
+    assert np.allclose(st.numngb, numngb ** (1 / 3), rtol=1e-12)
+    assert np.allclose(st.rho, rho, rtol=1e-12)
+    assert np.abs(st.divvel).max() < 1e-10
+    # cubic lattice: the inverted second-moment matrix is isotropic
+    t = st.nvt
+    assert np.abs(t[:, 3:]).max() < 1e-9 * np.abs(t[:, :3]).max()
+    assert np.allclose(t[:, 0], t[:, 1], rtol=1e-10) and np.allclose(t[:, 0], t[:, 2], rtol=1e-10)
+    assert np.allclose(st.cn, 1.0, rtol=1e-10)
+
+
+def test_density_iteration_converges_inside_band(oracle_lib):
+    O = oracle_lib
+    ic = ics.lattice_box(16, 9, kernel=3, des_num_ngb=32.0)
+    par = dict(ic["params"])
+    st = O.State.from_ic(ic)
+    st.h *= 1.37                      # start well off: forces bracketing
+    it = O.density(par, st, return_iters=True)
+    nn = st.numngb ** 3
+    assert it.max() < 150 and it.min() >= 2
+    # converged: inside the target band (dev widened by the iteration count) or a tight bracket
+    assert np.abs(nn - 32.0).max() < 0.25 * 32.0
+    assert np.median(np.abs(nn - 32.0)) < 0.2
+    # the final radius is the one the sums were evaluated at: re-evaluating there reproduces NumNgb
+    st2 = st.copy()
+    O.density_fixed_h(par, st2)
+    assert np.array_equal(st2.numngb, st.numngb) and np.array_equal(st2.rho, st.rho)
+
+
+# ------------------------------------------------------------------------------------------
+# gradients
+# ------------------------------------------------------------------------------------------
+def test_gradients_exact_for_linear_fields(oracle_lib):
+    O = oracle_lib
+    ic = ics.lattice_box(20, 4, kernel=6, des_num_ngb=64.0, jitter=0.1)
+    par = dict(ic["params"])
+    st = O.State.from_ic(ic)
+    O.density(par, st)
+    b_rho = np.array([0.3, -0.2, 0.5])
+    b_p = np.array([-0.1, 0.4, 0.2])
+    A = np.array([[0.1, 0.2, -0.3], [0.05, -0.1, 0.2], [0.3, 0.0, 0.1]])
+    st.rho[:] = 2.0 + st.pos @ b_rho
+    st.press[:] = 3.0 + st.pos @ b_p
+    st.vel[:] = st.pos @ A.T
+    O.gradients(par, st)
+    inner = np.all(np.abs(st.pos - 0.5) < 0.3, axis=1)   # kernels that do not wrap
+    assert inner.sum() > 1000
+    assert np.allclose(st.grad_rho[inner], b_rho, rtol=0, atol=1e-9)
+    assert np.allclose(st.grad_p[inner], b_p, rtol=0, atol=1e-9)
+    assert np.allclose(st.grad_v[inner].reshape(-1, 3, 3), A, rtol=0, atol=1e-9)
+
+
+def test_slopelimiter_cases(oracle_lib):
+    O = oracle_lib
+    # no limiting when the allowed jump exceeds alpha*h*|g|
+    g = O.slopelimiter([1.0, 0, 0], 1.0, -1.0, 0.25, 1.0, 0.0, 0, 0, 0)
+    assert np.array_equal(g, [1.0, 0, 0])
+    # limited: min(|min|+tol|max|, |max|)/(alpha h |g|)
+    g = O.slopelimiter([4.0, 0, 0], 0.5, -0.2, 0.25, 1.0, 0.1, 0, 0, 0)
+    assert g[0] == pytest.approx(4.0 * min(0.2 + 0.1 * 0.5, 0.5) / (0.25 * 1.0 * 4.0), rel=1e-15)
+    # positivity branch: the gradient cannot drive the value below fmin over d_max
+    g = O.slopelimiter([10.0, 0, 0], 5.0, -0.9, 0.25, 1.0, 0.0, 1, 2.0, 1.0)
+    fmin = min(1.0, max(0, max(1e-56, min(0.5 * (1.0 - 0.9), 1.0 - min(0.9, 5.0)))))
+    assert g[0] == pytest.approx(10.0 * min((1.0 - fmin) / 2.0 / 10.0, min(0.9, 5.0) / (0.25 * 10.0)), rel=1e-14)
+
+
+def test_reconstruct_cases(oracle_lib):
+    O = oracle_lib
+    z = [0.0, 0.0, 0.0]
+    assert O.reconstruct(1.0, [1, 1, 1], 1.0, [2, 2, 2], [0.1, 0, 0], [-0.1, 0, 0]) == (1.0, 1.0)
+    QL, QR = O.reconstruct(1.0, [5, 0, 0], 2.0, [5, 0, 0], [0.1, 0, 0], [-0.1, 0, 0], mode=0)
+    assert (QL, QR) == (2.0, 1.0)
+    # overshoot limited to the median bound: Q_R <= Qmed + 0.375 (Qmax - Qmin)
+    QL, QR = O.reconstruct(1.0, [50, 0, 0], 2.0, z, [0.1, 0, 0], [-0.1, 0, 0])
+    assert QR == pytest.approx(1.5 + 0.375, rel=1e-15) and QL == 2.0
+    # sign preservation: positive quantities never reconstruct below zero
+    QL, QR = O.reconstruct(0.1, [-50, 0, 0], 1.0, z, [0.1, 0, 0], [-0.1, 0, 0])
+    assert QR > 0
+
+
+# ------------------------------------------------------------------------------------------
+# Riemann problem
+# ------------------------------------------------------------------------------------------
+# Toro, Riemann Solvers and Numerical Methods for Fluid Dynamics (3rd ed.), Table 4.2
+TORO = [
+    ((1.0, 0.0, 1.0), (0.125, 0.0, 0.1), 0.30313, 0.92745),
+    ((1.0, -2.0, 0.4), (1.0, 2.0, 0.4), 0.00189, 0.0),
+    ((1.0, 0.0, 1000.0), (1.0, 0.0, 0.01), 460.894, 19.5975),
+    ((1.0, 0.0, 0.01), (1.0, 0.0, 100.0), 46.0950, -6.19633),
+    ((5.99924, 19.5975, 460.894), (5.99242, -6.19633, 46.0950), 1691.64, 8.68975),
+]
+
+
+@pytest.mark.parametrize("case", range(len(TORO)))
+def test_exact_riemann_toro(oracle_lib, case):
+    (rl, ul, pl), (rr, ur, pr), ps, us = TORO[case]
+    P, S = oracle_lib.riemann_exact(1.4, [rl, pl, ul, 0, 0], [rr, pr, ur, 0, 0])
+    assert P == pytest.approx(ps, rel=2e-4, abs=2e-5)
+    assert S == pytest.approx(us, rel=2e-4, abs=2e-5)
+
+
+def test_riemann_symmetry_and_vacuum(oracle_lib):
+    O = oracle_lib
+    n = np.array([0.6, 0.0, 0.8])
+    # identical states: no contact motion, the pressure itself
+    rc, P, S, fv, fp = O.riemann(5 / 3, [1.0, 2.0, 0.3, -0.1, 0.2], [1.0, 2.0, 0.3, -0.1, 0.2], n, 1e10)
+    vn = 0.3 * 0.6 + 0.2 * 0.8
+    assert rc == 0 and P == pytest.approx(2.0, rel=1e-14) and S == pytest.approx(vn, rel=1e-12, abs=1e-15)
+    # mirror: swap L/R and flip n -> same P_M, opposite S_M
+    L = [1.3, 0.7, 0.2, 0.1, -0.3]
+    R = [0.6, 1.9, -0.4, 0.2, 0.1]
+    rc1, P1, S1, _, _ = O.riemann(5 / 3, L, R, n, 1e10)
+    rc2, P2, S2, _, _ = O.riemann(5 / 3, R, L, -n, 1e10)
+    assert P1 == pytest.approx(P2, rel=1e-13) and S1 == pytest.approx(-S2, rel=1e-13)
+    # receding states faster than both sound speeds: the vacuum branch (P_M = MIN_REAL_NUMBER)
+    rc, P, S, fv, fp = O.riemann(5 / 3, [1.0, 1.0, -10.0, 0, 0], [1.0, 1.0, 10.0, 0, 0], np.array([1.0, 0, 0]), 1e10)
+    assert P == 1e-56 and S == 0.0
+
+
+# ------------------------------------------------------------------------------------------
+# force
+# ------------------------------------------------------------------------------------------
+def test_force_uniform_medium_at_rest(oracle_lib):
+    O = oracle_lib
+    ic = ics.lattice_box(12, 2, jitter=0.0, mach_sigma=0.0, kernel=3, des_num_ngb=32.0)
+    par = dict(ic["params"])
+    st = O.State.from_ic(ic)
+    O.hydro_pass(par, st)
+    scale = np.abs(st.press).max() * np.abs(st.nvt[:, 0]).max()
+    assert np.abs(st.accel).max() < 1e-11 * scale
+    assert np.abs(st.dtu).max() < 1e-11 * scale
+
+
+@pytest.mark.parametrize("kern,nngb", [(3, 32.0), (6, 64.0)])
+def test_force_conservation(oracle_lib, kern, nngb):
+    O = oracle_lib
+    ic = ics.lattice_box(14, 21, kernel=kern, des_num_ngb=nngb, mach_sigma=0.5)
+    par = dict(ic["params"])
+    st = O.State.from_ic(ic)
+    O.hydro_pass(par, st)
+    m = st.mass[:, None]
+    p_dot = (m * st.accel).sum(0)
+    p_scale = (m * np.abs(st.accel)).sum(0)
+    assert np.all(np.abs(p_dot) < 1e-13 * p_scale)
+    e_dot = (st.mass * (st.dtu + (st.vel * st.accel).sum(1))).sum()
+    e_scale = (st.mass * (np.abs(st.dtu) + np.abs((st.vel * st.accel).sum(1)))).sum()
+    assert abs(e_dot) < 1e-13 * e_scale
+    assert np.all(st.maxsig > 0)
