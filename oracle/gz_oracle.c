@@ -172,8 +172,9 @@ static int or_cmp_int(const void *a, const void *b)
     return (x > y) - (x < y);
 }
 
-/* candidates of position p: every particle in the (deduplicated) 27 cells around p,
-   ascending.  buf must hold n ints.  Returns the count. */
+/* candidates of position p: every particle in the (deduplicated) 27 cells around p, in
+   cell order (not sorted).  buf must hold n ints.  Returns the count.  Callers keep the
+   accepted ones and sort those (or_sort_int): every row is visited in ascending index. */
 static long or_grid_candidates(const or_grid *g, const double *p, double box, int *buf)
 {
     int nc = g->nc;
@@ -199,8 +200,19 @@ static long or_grid_candidates(const or_grid *g, const double *p, double box, in
     long m = 0;
     for (int q = 0; q < ncells; q++)
         for (long s = g->start[cells[q]]; s < g->start[cells[q] + 1]; s++) buf[m++] = g->idx[s];
-    qsort(buf, (size_t)m, sizeof(int), or_cmp_int);
     return m;
+}
+
+/* ascending order of a short accepted list (insertion sort; qsort beyond 64 entries) */
+static void or_sort_int(int *a, long m)
+{
+    if (m > 64) { qsort(a, (size_t)m, sizeof(int), or_cmp_int); return; }
+    for (long k = 1; k < m; k++) {
+        int t = a[k];
+        long q = k - 1;
+        while (q >= 0 && a[q] > t) { a[q + 1] = a[q]; q--; }
+        a[q + 1] = t;
+    }
 }
 
 /* exact pair predicate: src/mesh/sfc_tiles_functions.h:83-102 (frame.exact branch):
@@ -285,6 +297,7 @@ long orc_neighbors(const orc_params *par, long n, const double *pos, const doubl
                     if (symmetric && j == i) continue;
                     if (or_accept(&pos[3 * i], &pos[3 * j], h[i], h[j], symmetric, box, half)) nbr[o++] = j;
                 }
+                or_sort_int(&nbr[offsets[i]], o - offsets[i]);
             }
             free(buf);
         }
@@ -390,13 +403,24 @@ static double or_face_closure(const double *nvt_raw6, const double *fw, const do
 /* DivVel, NV_T[6], face_w[3].                                                    */
 /* ------------------------------------------------------------------------- */
 static void or_density_sums(const orc_params *par, const orc_state *s, long i, double h,
-                            const int *cand, long ncand, double *acc)
+                            int *cand, long ncand, double *acc)
 {
     double box = par->box, half = 0.5 * par->box;
     for (int k = 0; k < 13; k++) acc[k] = 0;
     const double *pi = &s->pos[3 * i];
     const double *vi = &s->vel[3 * i];
     const double h2 = h * h;
+    /* keep the candidates inside h (the pair test below repeats it), ascending */
+    long m = 0;
+    for (long q = 0; q < ncand; q++) {
+        int j = cand[q];
+        double e0 = or_wrap(pi[0] - s->pos[3 * j + 0], box, half);
+        double e1 = or_wrap(pi[1] - s->pos[3 * j + 1], box, half);
+        double e2 = or_wrap(pi[2] - s->pos[3 * j + 2], box, half);
+        if (e0 * e0 + e1 * e1 + e2 * e2 < h2) cand[m++] = j;
+    }
+    or_sort_int(cand, m);
+    ncand = m;
     for (long q = 0; q < ncand; q++) {
         long j = cand[q];
         if (s->mass[j] <= 0) continue;
@@ -847,11 +871,14 @@ int orc_gradients(const orc_params *par, orc_state *s, const unsigned char *acti
             double mx[5], mn[5], maxd = 0;
             for (int a = 0; a < 3; a++) for (int b = 0; b < 5; b++) G[a][b] = 0;
             for (int b = 0; b < 5; b++) { mx[b] = 0; mn[b] = 0; }
-            long m = or_grid_candidates(&g, pi, box, buf);
+            long m0 = or_grid_candidates(&g, pi, box, buf), m = 0;
+            for (long q = 0; q < m0; q++) {
+                int j = buf[q];
+                if (j != i && or_accept(pi, &s->pos[3 * j], h_i, s->h[j], 1, box, half)) buf[m++] = j;
+            }
+            or_sort_int(buf, m);
             for (long q = 0; q < m; q++) {
                 long j = buf[q];
-                if (j == i) continue;
-                if (!or_accept(pi, &s->pos[3 * j], h_i, s->h[j], 1, box, half)) continue;
                 if (!(s->mass[j] > 0) || !(s->rho[j] > 0) || !(s->h[j] > 0)) continue;
                 double dp[3];
                 dp[0] = pi[0] - s->pos[3 * j + 0];
@@ -1414,11 +1441,14 @@ int orc_force(const orc_params *par, orc_state *s, const unsigned char *active)
                 continue;
             }
             double Acc[3] = {0, 0, 0}, dE = 0, msv = 0;
-            long m = or_grid_candidates(&g, &s->pos[3 * i], box, buf);
+            long m0 = or_grid_candidates(&g, &s->pos[3 * i], box, buf), m = 0;
+            for (long q = 0; q < m0; q++) {
+                int j = buf[q];
+                if (j != i && or_accept(&s->pos[3 * i], &s->pos[3 * j], s->h[i], s->h[j], 1, box, half)) buf[m++] = j;
+            }
+            or_sort_int(buf, m);
             for (long q = 0; q < m; q++) {
                 long j = buf[q];
-                if (j == i) continue;
-                if (!or_accept(&s->pos[3 * i], &s->pos[3 * j], s->h[i], s->h[j], 1, box, half)) continue;
                 if (msv < sound_i) msv = sound_i;
                 double Fv[3], Fp, vts;
                 int rc = or_force_pair(par, &G, s, i, j, sound_i, Fv, &Fp, &vts);
