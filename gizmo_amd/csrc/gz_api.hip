@@ -78,6 +78,15 @@ struct gz_context {
     bool sym_valid = false;
     bool density_done = false, gradients_done = false;
 
+    // occupied cells of the active rows at the current grid level
+    int *ca0 = nullptr, *ca1 = nullptr, *cells = nullptr, *cflag = nullptr, *cscan = nullptr;
+    int ncells = 0, rows_level = -1;
+    uint8_t *need = nullptr;
+
+    // per-pass neighbour records
+    void *rec_grad = nullptr, *rec_force = nullptr;
+    double *dacc = nullptr;    // density pair sums of a round: 13 columns of stride nact
+
     // density iteration state (per slot)
     double *it_h = nullptr, *it_hb = nullptr, *it_l = nullptr, *it_r = nullptr, *it_cn = nullptr;
     int *it_ci = nullptr, *it_it = nullptr, *todo = nullptr, *todo2 = nullptr;
@@ -249,6 +258,11 @@ int alloc_state(gz_context *ctx, long cap)
     CHK(dalloc(ctx, &ctx->it_r, n)); CHK(dalloc(ctx, &ctx->it_cn, n)); CHK(dalloc(ctx, &ctx->it_ci, n));
     CHK(dalloc(ctx, &ctx->it_it, n)); CHK(dalloc(ctx, &ctx->todo, n)); CHK(dalloc(ctx, &ctx->todo2, n));
     CHK(dalloc(ctx, &ctx->cnt_d, n)); CHK(dalloc(ctx, &ctx->cnt_s, n));
+    CHK(dalloc(ctx, &ctx->cells, n)); CHK(dalloc(ctx, &ctx->cflag, n + 1)); CHK(dalloc(ctx, &ctx->cscan, n + 1));
+    CHK(dalloc(ctx, &ctx->need, n));
+    CHK(dalloc(ctx, &ctx->dacc, 13 * n));
+    CHK(dalloc(ctx, (char **)&ctx->rec_grad, n * sizeof(GradRec<double>)));
+    CHK(dalloc(ctx, (char **)&ctx->rec_force, n * sizeof(Cell<double>)));
     CHK(dalloc(ctx, &ctx->counters, 8));
     if (!ctx->hcounters) {
         hipError_t e = hipHostMalloc((void **)&ctx->hcounters, 8 * sizeof(int), hipHostMallocDefault);
@@ -268,6 +282,8 @@ void free_state(gz_context *c)
     dfree(c->it_h); dfree(c->it_hb); dfree(c->it_l); dfree(c->it_r); dfree(c->it_cn); dfree(c->it_ci); dfree(c->it_it);
     dfree(c->todo); dfree(c->todo2); dfree(c->cnt_d); dfree(c->cnt_s); dfree(c->counters); dfree(c->ell_d); dfree(c->ell_s);
     dfree(c->cstart); dfree(c->cend); dfree(c->nsten); dfree(c->sten); dfree(c->chmax);
+    dfree(c->ca0); dfree(c->ca1); dfree(c->cells); dfree(c->cflag); dfree(c->cscan); dfree(c->need);
+    dfree(c->rec_grad); dfree(c->rec_force); dfree(c->dacc);
     if (c->tmp) hipFree(c->tmp);
     c->tmp = nullptr;
     c->tmp_bytes = 0;
@@ -298,15 +314,19 @@ int download_field(gz_context *ctx, double *host, int width, const double *src, 
 int build_grid(gz_context *ctx, double rmax)
 {
     const double box = ctx->par.box_size;
+    // cell side >= the largest search radius, with a margin for positions that round onto a
+    // cell face (the 27-cell stencil must then still hold every partner)
     int L = 0;
-    while (L < kFineLevel && box / (double)(1 << (L + 1)) >= rmax) L++;
+    while (L < kFineLevel && box / (double)(1 << (L + 1)) >= rmax * (1.0 + 1e-6)) L++;
     const long ncell = 1L << (3 * L);
     if (ncell > ctx->grid_cells_alloc) {
         CHK(dalloc(ctx, &ctx->cstart, ncell)); CHK(dalloc(ctx, &ctx->cend, ncell)); CHK(dalloc(ctx, &ctx->nsten, ncell));
         CHK(dalloc(ctx, &ctx->sten, ncell * 27)); CHK(dalloc(ctx, &ctx->chmax, ncell));
+        CHK(dalloc(ctx, &ctx->ca0, ncell)); CHK(dalloc(ctx, &ctx->ca1, ncell));
         ctx->grid_cells_alloc = ncell;
     }
     ctx->level = L;
+    ctx->rows_level = -1;
     ctx->cell = box / (double)(1 << L);
     Timed t(ctx, "ngb_build");
     k_cell_clear<<<nblk(ncell), kBlock, 0, ctx->stream>>>(ncell, ctx->cstart, ctx->cend, ctx->chmax);
@@ -330,6 +350,30 @@ GridView grid_view(gz_context *ctx)
     g.chmax = ctx->chmax;
     return g;
 }
+
+// occupied cells of the active rows (one wavefront each in the cell-walk kernels)
+int build_cell_rows(gz_context *ctx)
+{
+    const int na = ctx->nact;
+    const int shift = 3 * (kFineLevel - ctx->level);
+    k_cell_rows_mark<<<nblk(na), kBlock, 0, ctx->stream>>>(na, ctx->act, ctx->key, shift, ctx->ca0, ctx->ca1, ctx->cflag);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemsetAsync(ctx->cflag + na, 0, sizeof(int), ctx->stream));
+    size_t bytes = 0;
+    HIPCHK(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, ctx->cflag, ctx->cscan, na + 1, ctx->stream));
+    CHK(ensure_tmp(ctx, bytes));
+    HIPCHK(hipcub::DeviceScan::ExclusiveSum(ctx->tmp, bytes, ctx->cflag, ctx->cscan, na + 1, ctx->stream));
+    k_cell_rows_list<<<nblk(na), kBlock, 0, ctx->stream>>>(na, ctx->act, ctx->key, shift, ctx->cflag, ctx->cscan, ctx->cells);
+    HIPCHK(hipGetLastError());
+    int nc = 0;
+    HIPCHK(hipMemcpyAsync(&nc, ctx->cscan + na, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    ctx->ncells = nc;
+    ctx->rows_level = ctx->level;
+    return 0;
+}
+
+CellRows cell_rows(gz_context *ctx) { return CellRows{ctx->cells, ctx->ncells, ctx->ca0, ctx->ca1, ctx->nact}; }
 
 int read_counters(gz_context *ctx)
 {
@@ -365,7 +409,7 @@ int build_list(gz_context *ctx, int nrows, const int *rows, const double *hrow, 
         HIPCHK(hipMemsetAsync(ctx->counters + 3, 0, sizeof(int), ctx->stream));
         {
             Timed t(ctx, "ngb_build");
-            k_build_ell<SYM><<<nblk(nrows), kBlock, 0, ctx->stream>>>(nrows, rows, ctx->act, hrow, scale, ctx->X, ctx->Y, ctx->Z, ctx->H,
+            k_build_ell<SYM><<<xcd_grid(nblk(nrows)), kBlock, 0, ctx->stream>>>(nrows, rows, ctx->act, hrow, scale, ctx->X, ctx->Y, ctx->Z, ctx->H,
                                                                      ctx->key, grid_view(ctx), *ell, *ld, *cap, cnt, ctx->counters + 3);
             HIPCHK(hipGetLastError());
         }
@@ -439,48 +483,45 @@ int density_stage(gz_context *ctx)
     HIPCHK(hipMemsetAsync(ctx->it_cn, 0, na * sizeof(double), ctx->stream));
     HIPCHK(hipMemsetAsync(ctx->it_ci, 0, na * sizeof(int), ctx->stream));
     HIPCHK(hipMemsetAsync(ctx->it_it, 0, na * sizeof(int), ctx->stream));
+    HIPCHK(hipMemsetAsync(ctx->need, 1, na, ctx->stream));
     HIPCHK(hipMemsetAsync(ctx->counters, 0, 8 * sizeof(int), ctx->stream));
     k_clamp_h<<<nblk(na), kBlock, 0, ctx->stream>>>(na, ctx->act, ctx->M, ctx->H, 0.99 * ctx->par.max_kernel_radius);
     k_gather<<<nblk(na), kBlock, 0, ctx->stream>>>(na, 1, ctx->act, ctx->H, ctx->it_h, 0);
-    k_iota<<<nblk(na), kBlock, 0, ctx->stream>>>(na, ctx->todo);
     HIPCHK(hipGetLastError());
-    HIPCHK(hipMemcpyAsync(ctx->it_hb, ctx->it_h, na * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
     double hmax = 0;
     CHK(max_on_device(ctx, ctx->it_h, na, &hmax));
-    CHK(build_grid(ctx, std::min(1.3 * hmax, 0.5 * ctx->par.box_size)));
-    const int cap_hint = std::max(64, (int)std::ceil(ctx->par.des_num_ngb * 1.3 * 1.3 * 1.3 * 1.6) + 32);
-    CHK(build_list<false>(ctx, na, nullptr, ctx->it_hb, 1.3, &ctx->ell_d, &ctx->ld_d, &ctx->cap_d, ctx->cnt_d, cap_hint));
+    CHK(build_grid(ctx, std::min(hmax, 0.5 * ctx->par.box_size)));
+    CHK(build_cell_rows(ctx));
     DensityIter it{ctx->it_h, ctx->it_hb, ctx->it_l, ctx->it_r, ctx->it_cn, ctx->it_ci, ctx->it_it};
     DensityOut out{ctx->n, ctx->H, ctx->NN, ctx->DRK, ctx->DIVV, ctx->RHO, ctx->PR, ctx->T, ctx->FW, ctx->CN, ctx->FCE};
-    int ntodo = na;
-    int *cur = ctx->todo, *nxt = ctx->todo2;
     int iters = 0;
-    while (ntodo > 0) {
-        HIPCHK(hipMemsetAsync(ctx->counters, 0, 2 * sizeof(int), ctx->stream));
+    while (true) {
+        HIPCHK(hipMemsetAsync(ctx->counters, 0, 3 * sizeof(int), ctx->stream));
         {
             Timed t(ctx, "density");
-            k_density_iter<KERN, R><<<nblk(ntodo), kBlock, 0, ctx->stream>>>(
-                ctx->dp, ntodo, cur, ctx->act, ctx->X, ctx->Y, ctx->Z, ctx->VX, ctx->VY, ctx->VZ, ctx->M, ctx->U, ctx->ell_d, ctx->ld_d,
-                ctx->cnt_d, it, out, nxt, ctx->counters + 0, ctx->counters + 1, ctx->counters + 2);
+            k_density_cells<KERN, R><<<xcd_grid(ctx->ncells), 64, 0, ctx->stream>>>(
+                ctx->dp, grid_view(ctx), cell_rows(ctx), ctx->act, ctx->need, ctx->X, ctx->Y, ctx->Z, ctx->VX, ctx->VY, ctx->VZ,
+                ctx->M, ctx->it_h, ctx->dacc);
+            k_density_after<<<nblk(na), kBlock, 0, ctx->stream>>>(ctx->dp, na, ctx->act, ctx->need, ctx->dacc, ctx->M, ctx->U,
+                                                                  ctx->cell, it, out, ctx->counters + 0, ctx->counters + 1,
+                                                                  ctx->counters + 2);
             HIPCHK(hipGetLastError());
         }
         CHK(read_counters(ctx));
         iters++;
-        const int nn = ctx->hcounters[0], rebuild = ctx->hcounters[1], err = ctx->hcounters[2];
+        const int remaining = ctx->hcounters[0], rebuild = ctx->hcounters[1], err = ctx->hcounters[2];
         if (err) {
             ctx->stats["density_iterations"] = iters;
             if (err == GZ_STOP_MAXITER) return fail(ctx, GZ_STOP_MAXITER, "density(): kernel radius iteration did not converge in %d rounds", kMaxIter);
             if (err == GZ_STOP_BRACKET) return fail(ctx, GZ_STOP_BRACKET, "density(): empty radius bracket (Right==0 && Left==0)");
             return fail(ctx, err, "density(): device stop code %d", err);
         }
-        std::swap(cur, nxt);
-        ntodo = nn;
-        if (ntodo > 0 && rebuild) {
-            // rows whose radius left the 1.3x buffer get a fresh list at the new radius
-            HIPCHK(hipMemcpyAsync(ctx->it_hb, ctx->it_h, na * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+        if (remaining == 0) break;
+        if (rebuild) {
+            // a radius outgrew the cell side: a coarser grid for the rows still iterating
             CHK(max_on_device(ctx, ctx->it_h, na, &hmax));
-            CHK(build_grid(ctx, std::min(1.3 * hmax, 0.5 * ctx->par.box_size)));
-            CHK(build_list<false>(ctx, ntodo, cur, ctx->it_hb, 1.3, &ctx->ell_d, &ctx->ld_d, &ctx->cap_d, ctx->cnt_d, cap_hint));
+            CHK(build_grid(ctx, std::min(hmax, 0.5 * ctx->par.box_size)));
+            CHK(build_cell_rows(ctx));
             ctx->stats["ngb_rebuilds"] += 1;
         }
     }
@@ -497,10 +538,31 @@ int ensure_sym(gz_context *ctx)
     double hmax = 0;
     CHK(max_on_device(ctx, ctx->H, ctx->n, &hmax));
     CHK(build_grid(ctx, std::min(hmax, 0.5 * ctx->par.box_size)));
+    CHK(build_cell_rows(ctx));
+    const long need_ld = ((long)ctx->nact + 63) / 64 * 64;
     const int cap_hint = std::max(64, (int)std::ceil(ctx->par.des_num_ngb * 2.0) + 32);
-    CHK(build_list<true>(ctx, ctx->nact, nullptr, nullptr, 1.0, &ctx->ell_s, &ctx->ld_s, &ctx->cap_s, ctx->cnt_s, cap_hint));
-    ctx->sym_valid = true;
-    return 0;
+    if (ctx->ell_s == nullptr || ctx->ld_s != need_ld || ctx->cap_s < cap_hint) {
+        ctx->cap_s = std::max(ctx->cap_s, cap_hint);
+        ctx->ld_s = need_ld;
+        CHK(dalloc(ctx, &ctx->ell_s, (size_t)ctx->cap_s * (size_t)ctx->ld_s));
+    }
+    for (int attempt = 0; attempt < 8; attempt++) {
+        HIPCHK(hipMemsetAsync(ctx->counters + 3, 0, sizeof(int), ctx->stream));
+        {
+            Timed t(ctx, "ngb_build");
+            k_build_sym_cells<<<xcd_grid(ctx->ncells), 64, 0, ctx->stream>>>(grid_view(ctx), cell_rows(ctx), ctx->act, ctx->X, ctx->Y,
+                                                                              ctx->Z, ctx->H, ctx->ell_s, ctx->ld_s, ctx->cap_s,
+                                                                              ctx->cnt_s, ctx->counters + 3);
+            HIPCHK(hipGetLastError());
+        }
+        CHK(read_counters(ctx));
+        const int over = ctx->hcounters[3];
+        if (over == 0) { ctx->sym_valid = true; return 0; }
+        ctx->cap_s = std::max(over + 16, ctx->cap_s * 3 / 2);
+        CHK(dalloc(ctx, &ctx->ell_s, (size_t)ctx->cap_s * (size_t)ctx->ld_s));
+        ctx->stats["ell_widen"] += 1;
+    }
+    return fail(ctx, GZ_E_STATE, "symmetric neighbour list did not fit after widening");
 }
 
 template <int KERN, typename R>
@@ -510,10 +572,11 @@ int gradients_stage(gz_context *ctx)
     if (ctx->nact == 0) { ctx->gradients_done = true; return 0; }
     CHK(ensure_sym(ctx));
     Timed t(ctx, "gradients");
-    k_gradients<KERN, R><<<nblk(ctx->nact), kBlock, 0, ctx->stream>>>(ctx->dp, ctx->nact, ctx->act, ctx->X, ctx->Y, ctx->Z, ctx->VX,
-                                                                      ctx->VY, ctx->VZ, ctx->M, ctx->H, ctx->RHO, ctx->PR, ctx->T,
-                                                                      ctx->CN, ctx->n, ctx->ell_s, ctx->ld_s, ctx->cnt_s, ctx->GR,
-                                                                      ctx->GP, ctx->GV);
+    GradRec<R> *rec = (GradRec<R> *)ctx->rec_grad;
+    k_pack_grad<R><<<nblk(ctx->n), kBlock, 0, ctx->stream>>>(ctx->n, ctx->H, ctx->M, ctx->RHO, ctx->PR, ctx->VX, ctx->VY, ctx->VZ, rec);
+    k_gradients<KERN, R><<<xcd_grid(nblk(ctx->nact)), kBlock, 0, ctx->stream>>>(ctx->dp, ctx->nact, ctx->act, ctx->X, ctx->Y, ctx->Z,
+                                                                                rec, ctx->T, ctx->CN, ctx->n, ctx->ell_s, ctx->ld_s,
+                                                                                ctx->cnt_s, ctx->GR, ctx->GP, ctx->GV);
     HIPCHK(hipGetLastError());
     ctx->gradients_done = true;
     return 0;
@@ -531,10 +594,11 @@ int force_stage(gz_context *ctx)
                 ctx->FCE, ctx->DRK, ctx->GR, ctx->GP, ctx->GV, ctx->T};
     {
         Timed t(ctx, "force");
-        k_force<KERN, R><<<nblk(ctx->nact), kBlock, 0, ctx->stream>>>(ctx->dp, gam<R>(ctx->dp), ctx->nact, ctx->act, F, ctx->ell_s,
-                                                                      ctx->ld_s, ctx->cnt_s, ctx->all_active ? nullptr : ctx->ACT, ctx->TB,
-                                                                      ctx->MSV_IN, ctx->WAKE, ctx->ACC, ctx->DTU, ctx->MSV,
-                                                                      ctx->counters + 2);
+        Cell<R> *rec = (Cell<R> *)ctx->rec_force;
+        k_pack_force<R><<<nblk(ctx->n), kBlock, 0, ctx->stream>>>(ctx->par.gamma, F, rec);
+        k_force<KERN, R><<<xcd_grid(nblk(ctx->nact)), kBlock, 0, ctx->stream>>>(
+            ctx->dp, gam<R>(ctx->dp), ctx->nact, ctx->act, F, rec, ctx->ell_s, ctx->ld_s, ctx->cnt_s,
+            ctx->all_active ? nullptr : ctx->ACT, ctx->TB, ctx->MSV_IN, ctx->WAKE, ctx->ACC, ctx->DTU, ctx->MSV, ctx->counters + 2);
         HIPCHK(hipGetLastError());
     }
     CHK(read_counters(ctx));

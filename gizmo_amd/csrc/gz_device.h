@@ -581,7 +581,9 @@ __device__ inline void riemann(const Gam<R> &G, const RState<R> &L, const RState
 // hydro_core_meshless_compute_pair (src/hydro/hydro_core_meshless_functions.h:49-616) and
 // compute_finitevol_faces (src/hydro/compute_finitevol_faces_functions.h:48-168)
 // ---------------------------------------------------------------------------------------
-template <typename R> struct Cell {
+// per-particle record of the force pass (32 values: 256 B in FP64, 128 B in FP32), packed
+// once per pass so that a neighbour costs a few wide loads instead of 32 scattered ones
+template <typename R> struct alignas(16) Cell {
     R h, m, rho, p, cs, cn, fce, drk;
     R v0, v1, v2;
     R gr0, gr1, gr2, gp0, gp1, gp2;
@@ -736,5 +738,10 @@ __device__ inline int force_pair(const Gam<R> &G, const Cell<R> &I, const Cell<R
     }
     return 1;
 }
+
+// per-particle record of the gradient pass: the neighbour fields gradient_accumulate_neighbor reads
+template <typename R> struct alignas(16) GradRec {
+    R h, m, rho, p, v0, v1, v2, pad;
+};
 
 }  // namespace gz

@@ -6,6 +6,18 @@ namespace gz {
 
 constexpr int kFineLevel = 10;                 // 1024^3 Morton grid
 constexpr int kFine = 1 << kFineLevel;
+constexpr int kXcds = 8;                       // MI355X: 8 XCDs, each with its own L2
+
+// Workgroups are handed to the XCDs round-robin (workgroup b runs on XCD b % 8).  A grid of
+// 8*q workgroups is remapped so that XCD x processes the contiguous logical range
+// [x*q, (x+1)*q): neighbouring particles (adjacent in Morton order) then share one L2.
+// Launch with xcd_grid(nblocks) workgroups; logical blocks past the real count exit.
+__device__ __forceinline__ long xcd_block()
+{
+    const long b = blockIdx.x, q = gridDim.x / kXcds;
+    return (b % kXcds) * q + b / kXcds;
+}
+inline unsigned xcd_grid(long nblocks) { return (unsigned)(((nblocks + kXcds - 1) / kXcds) * kXcds); }
 
 struct GridView {
     int level, shift;         // nc = 2^level cells per side; cell code = key >> shift
@@ -13,6 +25,13 @@ struct GridView {
     const int4 *sten;         // 27 entries per cell: (start, end, offset code, cell code)
     const int *nsten;
     const double *chmax;      // largest kernel radius in each cell
+};
+
+struct CellRows {             // occupied cells of the active rows
+    const int *cells;         // cell codes, ascending
+    int ncells;
+    const int *ca0, *ca1;     // per cell code: [first, last+1) slot of its active rows
+    int nrows;                // number of active rows (column stride of per-row sums)
 };
 
 struct DensityIter {          // per active slot
@@ -54,15 +73,33 @@ __global__ void k_density_iter(DevParams P, int ntodo, const int *todo, const in
                                const double *U, const int *ell, long ld, const int *cnt, DensityIter it, DensityOut out,
                                int *next, int *nnext, int *rebuild, int *err);
 
-template <int KERN, typename R>
-__global__ void k_gradients(DevParams P, int nrows, const int *act, const double *X, const double *Y, const double *Z,
-                            const double *VX, const double *VY, const double *VZ, const double *M, const double *H,
-                            const double *RHO, const double *PR, const double *T, const double *CN, long n, const int *ell,
-                            long ld, const int *cnt, double *GR, double *GP, double *GV);
+__global__ void k_cell_rows_mark(int nact, const int *act, const uint32_t *key, int shift, int *ca0, int *ca1, int *flag);
+__global__ void k_cell_rows_list(int nact, const int *act, const uint32_t *key, int shift, const int *flag, const int *scan,
+                                 int *cells);
+__global__ void k_build_sym_cells(GridView g, CellRows cr, const int *act, const double *X, const double *Y, const double *Z,
+                                  const double *H, int *ell, long ld, int cap, int *cnt, int *overflow);
 
 template <int KERN, typename R>
-__global__ void k_force(DevParams P, Gam<R> G, int nrows, const int *act, ForceView F, const int *ell, long ld,
-                        const int *cnt, const uint8_t *active, const int *timebin, const double *msv_in, int *wakeup,
+__global__ void k_density_cells(DevParams P, GridView g, CellRows cr, const int *act, const uint8_t *need, const double *X,
+                                const double *Y, const double *Z, const double *VX, const double *VY, const double *VZ,
+                                const double *M, const double *hcur, double *accout);
+__global__ void k_density_after(DevParams P, int nrows, const int *act, uint8_t *need, const double *accin, const double *M,
+                                const double *U, double cell, DensityIter it, DensityOut out, int *nnext, int *rebuild, int *err);
+
+template <typename R>
+__global__ void k_pack_grad(long n, const double *H, const double *M, const double *RHO, const double *PR, const double *VX,
+                            const double *VY, const double *VZ, GradRec<R> *rec);
+template <typename R>
+__global__ void k_pack_force(double gamma, ForceView F, Cell<R> *rec);
+
+template <int KERN, typename R>
+__global__ void k_gradients(DevParams P, int nrows, const int *act, const double *X, const double *Y, const double *Z,
+                            const GradRec<R> *rec, const double *T, const double *CN, long n, const int *ell, long ld,
+                            const int *cnt, double *GR, double *GP, double *GV);
+
+template <int KERN, typename R>
+__global__ void k_force(DevParams P, Gam<R> G, int nrows, const int *act, ForceView F, const Cell<R> *rec, const int *ell,
+                        long ld, const int *cnt, const uint8_t *active, const int *timebin, const double *msv_in, int *wakeup,
                         double *ACC, double *DTU, double *MSV, int *err);
 
 }  // namespace gz

@@ -196,9 +196,9 @@ __global__ void __launch_bounds__(256) k_build_ell(int nrows, const int *__restr
                                                    const uint32_t *__restrict__ key, GridView g, int *__restrict__ ell, long ld, int cap,
                                                    int *__restrict__ cnt, int *__restrict__ overflow)
 {
-    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const long k = xcd_block() * blockDim.x + threadIdx.x;
     if (k >= nrows) return;
-    const int r = rows ? rows[k] : k;
+    const int r = rows ? rows[k] : (int)k;
     const int i = act[r];
     const double xi = X[i], yi = Y[i], zi = Z[i];
     const double hi = hrow ? hrow[r] * scale : H[i];
@@ -279,7 +279,7 @@ __global__ void __launch_bounds__(256) k_density_iter(DevParams P, int ntodo, co
                                                       DensityIter it, DensityOut out, int *__restrict__ next, int *__restrict__ nnext,
                                                       int *__restrict__ rebuild, int *__restrict__ err)
 {
-    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const long k = xcd_block() * blockDim.x + threadIdx.x;
     if (k >= ntodo) return;
     const int r = todo[k];
     const int i = act[r];
@@ -352,20 +352,332 @@ __global__ void __launch_bounds__(256) k_density_iter(DevParams P, int ntodo, co
 }
 
 // ---------------------------------------------------------------------------------------
+// per-cell active rows: the active slots of one grid cell are a contiguous range of act[]
+// (act ascends in device order, cells are contiguous in it); one entry per occupied cell
+// ---------------------------------------------------------------------------------------
+__global__ void k_cell_rows_mark(int nact, const int *__restrict__ act, const uint32_t *__restrict__ key, int shift,
+                                 int *__restrict__ ca0, int *__restrict__ ca1, int *__restrict__ flag)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nact) return;
+    const uint32_t c = key[act[r]] >> shift;
+    const bool first = (r == 0) || ((key[act[r - 1]] >> shift) != c);
+    const bool last = (r == nact - 1) || ((key[act[r + 1]] >> shift) != c);
+    if (first) ca0[c] = r;
+    if (last) ca1[c] = r + 1;
+    flag[r] = first ? 1 : 0;
+}
+
+__global__ void k_cell_rows_list(int nact, const int *__restrict__ act, const uint32_t *__restrict__ key, int shift,
+                                 const int *__restrict__ flag, const int *__restrict__ scan, int *__restrict__ cells)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nact) return;
+    if (flag[r]) cells[scan[r]] = (int)(key[act[r]] >> shift);
+}
+
+__device__ __forceinline__ bool wave_any(bool v) { return __any(v ? 1 : 0) != 0; }
+
+// ---------------------------------------------------------------------------------------
+// leaf culling for the cell walks.  A wavefront's particles (one leaf) have a bounding box in
+// coordinates relative to the home cell's origin; a candidate farther from that box than the
+// reach of the pair test cannot be accepted by any lane and is dropped while its tile is
+// staged, keeping the survivors in ascending order (stable compaction over the ballot mask).
+// The test is conservative: the slack covers rounding of the relative coordinates.
+// ---------------------------------------------------------------------------------------
+struct LeafBox {
+    double ox, oy, oz;          // home cell origin
+    double lo[3], hi[3];        // leaf box, relative to the origin
+    bool cull;                  // false on grids too coarse for relative coordinates
+};
+
+__device__ __forceinline__ double wave_min(double v)
+{
+    for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
+    return v;
+}
+__device__ __forceinline__ double wave_max(double v)
+{
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+__device__ __forceinline__ LeafBox leaf_box(const GridView &g, int c, bool on, double xi, double yi, double zi)
+{
+    LeafBox b;
+    const int cx = compact10((uint32_t)c >> 2), cy = compact10((uint32_t)c >> 1), cz = compact10((uint32_t)c);
+    b.ox = cx * g.cell; b.oy = cy * g.cell; b.oz = cz * g.cell;
+    const double rx = wrap(xi - b.ox, g.box, g.half), ry = wrap(yi - b.oy, g.box, g.half), rz = wrap(zi - b.oz, g.box, g.half);
+    b.lo[0] = wave_min(on ? rx : 1e300); b.hi[0] = wave_max(on ? rx : -1e300);
+    b.lo[1] = wave_min(on ? ry : 1e300); b.hi[1] = wave_max(on ? ry : -1e300);
+    b.lo[2] = wave_min(on ? rz : 1e300); b.hi[2] = wave_max(on ? rz : -1e300);
+    b.cull = (1 << g.level) >= 4;
+    return b;
+}
+
+__device__ __forceinline__ bool leaf_reach(const GridView &g, const LeafBox &b, double xj, double yj, double zj, double reach)
+{
+    if (!b.cull) return true;
+    const double rx = wrap(xj - b.ox, g.box, g.half), ry = wrap(yj - b.oy, g.box, g.half), rz = wrap(zj - b.oz, g.box, g.half);
+    const double ex = fmax(fmax(b.lo[0] - rx, rx - b.hi[0]), 0.0);
+    const double ey = fmax(fmax(b.lo[1] - ry, ry - b.hi[1]), 0.0);
+    const double ez = fmax(fmax(b.lo[2] - rz, rz - b.hi[2]), 0.0);
+    const double rr = reach * (1.0 + 1e-9) + 1e-12 * g.box;
+    return ex * ex + ey * ey + ez * ez <= rr * rr;
+}
+
+// position of this lane among the kept lanes below it (stable compaction)
+__device__ __forceinline__ int kept_rank(bool keep, int &count)
+{
+    const unsigned long long m = __ballot(keep ? 1 : 0);
+    count = __popcll(m);
+    const unsigned lane = threadIdx.x & 63;
+    return __popcll(m & ((lane == 0) ? 0ull : (~0ull >> (64 - lane))));
+}
+
+// ---------------------------------------------------------------------------------------
+// density as a cell walk: one wavefront per occupied cell, its active particles one per lane;
+// the candidates of the cell's 27 neighbour ranges are staged through LDS 64 at a time (after
+// leaf culling) and every lane tests them in ascending order (the oracle's summation order).
+// need[r] marks the rows still iterating.  The 13 pair sums of DensitySpec::pair_kernel
+// (src/hydro/density_loop.h:1141-1385) go to acc[13*r]; k_density_after closes the round.
+// ---------------------------------------------------------------------------------------
+template <int KERN, typename R>
+__global__ void __launch_bounds__(64) k_density_cells(DevParams P, GridView g, CellRows cr, const int *__restrict__ act,
+                                                      const uint8_t *__restrict__ need, const double *__restrict__ X,
+                                                      const double *__restrict__ Y, const double *__restrict__ Z,
+                                                      const double *__restrict__ VX, const double *__restrict__ VY,
+                                                      const double *__restrict__ VZ, const double *__restrict__ M,
+                                                      const double *__restrict__ hcur, double *__restrict__ accout)
+{
+    __shared__ double sx[64], sy[64], sz[64], svx[64], svy[64], svz[64], sm[64];
+    const long cb = xcd_block();
+    if (cb >= cr.ncells) return;
+    const int c = cr.cells[cb];
+    const int a0 = cr.ca0[c], a1 = cr.ca1[c];
+    const int nst = g.nsten[c];
+    const int t = threadIdx.x;
+    for (int base = a0; base < a1; base += 64) {
+        const int r = base + t;
+        const bool on = (r < a1) && need[r];
+        if (!wave_any(on)) continue;
+        int i = 0;
+        double xi = 0, yi = 0, zi = 0, h = 0;
+        R vxi = 0, vyi = 0, vzi = 0;
+        if (on) {
+            i = act[r];
+            xi = X[i]; yi = Y[i]; zi = Z[i];
+            vxi = (R)VX[i]; vyi = (R)VY[i]; vzi = (R)VZ[i];
+            h = hcur[r];
+        }
+        const LeafBox lb = leaf_box(g, c, on, xi, yi, zi);
+        const double reach = wave_max(on ? h : 0.0);
+        R a[13];
+        for (int q = 0; q < 13; q++) a[q] = 0;
+        const R hR = (R)(on ? h : 1.0);
+        const R h2 = hR * hR;
+        R hinv, hinv3, hinv4;
+        kernel_hinv(hR, hinv, hinv3, hinv4);
+        for (int q = 0; q < nst; q++) {
+            const int4 e = g.sten[(size_t)c * 27 + q];
+            for (int ts = e.x; ts < e.y; ts += 64) {
+                const int j = ts + t;
+                bool keep = false;
+                double xj = 0, yj = 0, zj = 0;
+                if (j < e.y) {
+                    xj = X[j]; yj = Y[j]; zj = Z[j];
+                    keep = leaf_reach(g, lb, xj, yj, zj, reach);
+                }
+                int nt;
+                const int pos = kept_rank(keep, nt);
+                __syncthreads();
+                if (keep) {
+                    sx[pos] = xj; sy[pos] = yj; sz[pos] = zj;
+                    svx[pos] = VX[j]; svy[pos] = VY[j]; svz[pos] = VZ[j]; sm[pos] = M[j];
+                }
+                __syncthreads();
+                if (!on) continue;
+                for (int k = 0; k < nt; k++) {
+                    const double mj = sm[k];
+                    if (mj <= 0) continue;
+                    double d0 = xi - sx[k], d1 = yi - sy[k], d2 = zi - sz[k];
+                    d1 = wrap(d1, P.box, P.half);
+                    d0 = wrap(d0, P.box, P.half);
+                    d2 = wrap(d2, P.box, P.half);
+                    const R dp0 = (R)d0, dp1 = (R)d1, dp2 = (R)d2;
+                    const R r2 = dp0 * dp0 + dp1 * dp1 + dp2 * dp2;
+                    if (r2 >= h2) continue;
+                    const R rr = sqrt(r2);
+                    const R u = rr * hinv;
+                    R wk = 0, dwk = 0;
+                    kernel_main<KERN, 0>(u, hinv3, hinv4, wk, dwk);
+                    const R mj_wk = (R)mj * wk;
+                    a[0] += wk;
+                    a[1] += mj_wk;
+                    a[2] += -(R(3.0) * hinv * wk + u * dwk);
+                    if (rr > 0) {
+                        a[4] += wk * (dp0 * dp0);
+                        a[5] += wk * (dp1 * dp1);
+                        a[6] += wk * (dp2 * dp2);
+                        a[7] += wk * (dp0 * dp1);
+                        a[8] += wk * (dp1 * dp2);
+                        a[9] += wk * (dp0 * dp2);
+                        a[10] += wk * dp0;
+                        a[11] += wk * dp1;
+                        a[12] += wk * dp2;
+                        const R dv0 = vxi - (R)svx[k], dv1 = vyi - (R)svy[k], dv2 = vzi - (R)svz[k];
+                        a[3] -= dwk * (dp0 * dv0 + dp1 * dv1 + dp2 * dv2) / rr;
+                    }
+                }
+            }
+        }
+        if (on)
+            for (int q = 0; q < 13; q++) accout[(size_t)q * cr.nrows + r] = (double)a[q];
+    }
+}
+
+// after_iter_body (src/hydro/density_loop.h:516-961) on the sums of the round and, on
+// convergence, density_finalize_post_runner (src/hydro/density_loop.cc:303-733)
+__global__ void __launch_bounds__(256) k_density_after(DevParams P, int nrows, const int *__restrict__ act, uint8_t *__restrict__ need,
+                                                       const double *__restrict__ accin, const double *__restrict__ M,
+                                                       const double *__restrict__ U, double cell, DensityIter it, DensityOut out,
+                                                       int *__restrict__ nnext, int *__restrict__ rebuild, int *__restrict__ err)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nrows || !need[r]) return;
+    const int i = act[r];
+    double acc[13];
+    for (int q = 0; q < 13; q++) acc[q] = accin[(size_t)q * nrows + r];
+    const double h = it.hcur[r];
+    IterState sc;
+    sc.left = it.left[r]; sc.right = it.right[r]; sc.cn_cur = it.cn_cur[r]; sc.cn_init = it.cn_init[r];
+    const int iter = it.iter[r];
+    double nh = h;
+    const int rc = density_after_iter(P, out.cn[i], iter, h, acc, sc, nh);
+    it.iter[r] = iter + 1;
+    if (rc <= 0 || iter + 1 >= kMaxIter) {
+        if (rc < 0) atomicMax(err, 90001009);
+        else if (rc > 0) atomicMax(err, 1155);
+        density_finalize(P, out, i, h, acc, M[i], U[i]);
+        need[r] = 0;
+        return;
+    }
+    it.left[r] = sc.left; it.right[r] = sc.right; it.cn_cur[r] = sc.cn_cur; it.cn_init[r] = sc.cn_init;
+    it.hcur[r] = nh;
+    if (nh * (1.0 + 1e-6) > cell) atomicOr(rebuild, 1);
+    atomicAdd(nnext, 1);
+}
+
+// ---------------------------------------------------------------------------------------
+// symmetric list as a cell walk (one wavefront per occupied cell, culled LDS-staged candidates):
+// rows r2 < max(h_i,h_j)^2, self excluded, ascending (src/mesh/sfc_tiles_functions.h:83-102)
+// ---------------------------------------------------------------------------------------
+__global__ void __launch_bounds__(64) k_build_sym_cells(GridView g, CellRows cr, const int *__restrict__ act,
+                                                        const double *__restrict__ X, const double *__restrict__ Y,
+                                                        const double *__restrict__ Z, const double *__restrict__ H,
+                                                        int *__restrict__ ell, long ld, int cap, int *__restrict__ cnt,
+                                                        int *__restrict__ overflow)
+{
+    __shared__ double sx[64], sy[64], sz[64], sh[64];
+    __shared__ int sj[64];
+    const long cb = xcd_block();
+    if (cb >= cr.ncells) return;
+    const int c = cr.cells[cb];
+    const int a0 = cr.ca0[c], a1 = cr.ca1[c];
+    const int nst = g.nsten[c];
+    const int t = threadIdx.x;
+    for (int base = a0; base < a1; base += 64) {
+        const int r = base + t;
+        const bool on = r < a1;
+        int i = -1;
+        double xi = 0, yi = 0, zi = 0, hi = 0;
+        if (on) { i = act[r]; xi = X[i]; yi = Y[i]; zi = Z[i]; hi = H[i]; }
+        const LeafBox lb = leaf_box(g, c, on, xi, yi, zi);
+        const double hleaf = wave_max(on ? hi : 0.0);
+        int n = 0;
+        for (int q = 0; q < nst; q++) {
+            const int4 e = g.sten[(size_t)c * 27 + q];
+            for (int ts = e.x; ts < e.y; ts += 64) {
+                const int j = ts + t;
+                bool keep = false;
+                double xj = 0, yj = 0, zj = 0, hj = 0;
+                if (j < e.y) {
+                    xj = X[j]; yj = Y[j]; zj = Z[j]; hj = H[j];
+                    keep = leaf_reach(g, lb, xj, yj, zj, fmax(hleaf, hj));
+                }
+                int nt;
+                const int pos = kept_rank(keep, nt);
+                __syncthreads();
+                if (keep) { sx[pos] = xj; sy[pos] = yj; sz[pos] = zj; sh[pos] = hj; sj[pos] = j; }
+                __syncthreads();
+                if (!on) continue;
+                for (int k = 0; k < nt; k++) {
+                    const int jj = sj[k];
+                    if (jj == i) continue;
+                    if (accept_pair<true>(xi, yi, zi, sx[k], sy[k], sz[k], hi, sh[k], g.box, g.half)) {
+                        if (n < cap) ell[(size_t)n * ld + r] = jj;
+                        n++;
+                    }
+                }
+            }
+        }
+        if (on) {
+            cnt[r] = n;
+            if (n > cap) atomicMax(overflow, n);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// records: the neighbour-side fields of a pass, packed per particle (device order)
+// ---------------------------------------------------------------------------------------
+template <typename R>
+__global__ void k_pack_grad(long n, const double *__restrict__ H, const double *__restrict__ M, const double *__restrict__ RHO,
+                            const double *__restrict__ PR, const double *__restrict__ VX, const double *__restrict__ VY,
+                            const double *__restrict__ VZ, GradRec<R> *__restrict__ rec)
+{
+    const long j = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    GradRec<R> g;
+    g.h = (R)H[j]; g.m = (R)M[j]; g.rho = (R)RHO[j]; g.p = (R)PR[j];
+    g.v0 = (R)VX[j]; g.v1 = (R)VY[j]; g.v2 = (R)VZ[j]; g.pad = 0;
+    rec[j] = g;
+}
+
+template <typename R>
+__global__ void k_pack_force(double gamma, ForceView F, Cell<R> *__restrict__ rec)
+{
+    const long j = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    const long n = F.n;
+    if (j >= n) return;
+    Cell<R> c;
+    c.h = (R)F.H[j]; c.m = (R)F.M[j]; c.rho = (R)F.RHO[j]; c.p = (R)F.PR[j];
+    c.cn = (R)F.CN[j]; c.fce = (R)F.FCE[j]; c.drk = (R)F.DRK[j];
+    c.v0 = (R)F.VX[j]; c.v1 = (R)F.VY[j]; c.v2 = (R)F.VZ[j];
+    c.gr0 = (R)F.GR[j]; c.gr1 = (R)F.GR[n + j]; c.gr2 = (R)F.GR[2 * n + j];
+    c.gp0 = (R)F.GP[j]; c.gp1 = (R)F.GP[n + j]; c.gp2 = (R)F.GP[2 * n + j];
+    for (int k = 0; k < 9; k++) c.gv[k] = (R)F.GV[(size_t)k * n + j];
+    for (int k = 0; k < 6; k++) c.t[k] = (R)F.T[(size_t)k * n + j];
+    // effective_soundspeed, src/declarations/cell_data.h:703-712
+    const double rho = F.RHO[j];
+    c.cs = (R)((rho > 0) ? sqrt(gamma * F.PR[j] / rho) : sqrt(gamma * (gamma - 1.) * F.U[j]));
+    rec[j] = c;
+}
+
+// ---------------------------------------------------------------------------------------
 // gradients: gradient_accumulate_neighbor (src/hydro/gradient_functions.h:305-665),
 // construct_gradient (src/hydro/gradients_loop.cc:129-136) and the limiter block
 // (:1077-1130), fused per active particle
 // ---------------------------------------------------------------------------------------
 template <int KERN, typename R>
 __global__ void __launch_bounds__(256) k_gradients(DevParams P, int nrows, const int *__restrict__ act, const double *__restrict__ X,
-                                                   const double *__restrict__ Y, const double *__restrict__ Z, const double *__restrict__ VX,
-                                                   const double *__restrict__ VY, const double *__restrict__ VZ, const double *__restrict__ M,
-                                                   const double *__restrict__ H, const double *__restrict__ RHO, const double *__restrict__ PR,
-                                                   const double *__restrict__ T, const double *__restrict__ CN, long n,
-                                                   const int *__restrict__ ell, long ld, const int *__restrict__ cnt,
-                                                   double *__restrict__ GR, double *__restrict__ GP, double *__restrict__ GV)
+                                                   const double *__restrict__ Y, const double *__restrict__ Z,
+                                                   const GradRec<R> *__restrict__ rec, const double *__restrict__ T,
+                                                   const double *__restrict__ CN, long n, const int *__restrict__ ell, long ld,
+                                                   const int *__restrict__ cnt, double *__restrict__ GR, double *__restrict__ GP,
+                                                   double *__restrict__ GV)
 {
-    int r = blockIdx.x * blockDim.x + threadIdx.x;
+    const int r = (int)(xcd_block() * blockDim.x + threadIdx.x);
     if (r >= nrows) return;
     const int i = act[r];
     R G[3][5];
@@ -373,11 +685,9 @@ __global__ void __launch_bounds__(256) k_gradients(DevParams P, int nrows, const
     R mx[5], mn[5];
     for (int b = 0; b < 5; b++) { mx[b] = 0; mn[b] = 0; }
     R maxd = 0;
-    const double mi = M[i], rhoi_d = RHO[i];
-    const bool enabled = (mi > 0 && rhoi_d > 0);
-    const R hi = (R)H[i];
-    const R rhoi = (R)rhoi_d, pri = (R)PR[i];
-    const R vxi = (R)VX[i], vyi = (R)VY[i], vzi = (R)VZ[i];
+    const GradRec<R> I = rec[i];
+    const bool enabled = (I.m > 0 && I.rho > 0);
+    const R hi = I.h;
     if (enabled) {
         const double xi = X[i], yi = Y[i], zi = Z[i];
         R hinv, hinv3, hinv4;
@@ -386,17 +696,16 @@ __global__ void __launch_bounds__(256) k_gradients(DevParams P, int nrows, const
         const int c = cnt[r];
         for (int q = 0; q < c; q++) {
             const int j = ell[(size_t)q * ld + r];
-            const double hjd = H[j];
-            if (!(M[j] > 0) || !(RHO[j] > 0) || !(hjd > 0)) continue;
+            const GradRec<R> J = rec[j];
+            if (!(J.m > 0) || !(J.rho > 0) || !(J.h > 0)) continue;
             double d0 = xi - X[j], d1 = yi - Y[j], d2 = zi - Z[j];
             d1 = wrap(d1, P.box, P.half);
             d0 = wrap(d0, P.box, P.half);
             d2 = wrap(d2, P.box, P.half);
             const R dp[3] = {(R)d0, (R)d1, (R)d2};
             const R r2 = dp[0] * dp[0] + dp[1] * dp[1] + dp[2] * dp[2];
-            const R hj = (R)hjd;
             if (r2 <= 0) continue;
-            if ((r2 >= h2i) && (r2 >= hj * hj)) continue;
+            if ((r2 >= h2i) && (r2 >= J.h * J.h)) continue;
             const R rr = sqrt(r2);
             R wk = 0, dwk = 0;
             if (rr < hi) {
@@ -405,11 +714,11 @@ __global__ void __launch_bounds__(256) k_gradients(DevParams P, int nrows, const
             }
             if (rr > maxd) maxd = rr;
             R d[5];
-            d[0] = (R)RHO[j] - rhoi;
-            d[1] = (R)PR[j] - pri;
-            d[2] = (R)VX[j] - vxi;
-            d[3] = (R)VY[j] - vyi;
-            d[4] = (R)VZ[j] - vzi;
+            d[0] = J.rho - I.rho;
+            d[1] = J.p - I.p;
+            d[2] = J.v0 - I.v0;
+            d[3] = J.v1 - I.v1;
+            d[4] = J.v2 - I.v2;
             for (int b = 0; b < 5; b++) {
                 if (d[b] < mn[b]) mn[b] = d[b];
                 else if (d[b] > mx[b]) mx[b] = d[b];
@@ -436,8 +745,8 @@ __global__ void __launch_bounds__(256) k_gradients(DevParams P, int nrows, const
         const double cni = CN[i];
         R alim = R(0.25);
         if (cni > 100) alim = (R)dmin(0.5, 0.25 + 0.25 * (cni - 100) / 100);
-        slopelimiter<R>(o[0], mx[0], mn[0], alim, hi, R(0), 1, dmaxv, rhoi);
-        slopelimiter<R>(o[1], mx[1], mn[1], alim, hi, stol, 1, dmaxv, pri);
+        slopelimiter<R>(o[0], mx[0], mn[0], alim, hi, R(0), 1, dmaxv, I.rho);
+        slopelimiter<R>(o[1], mx[1], mn[1], alim, hi, stol, 1, dmaxv, I.p);
         for (int kk = 0; kk < 3; kk++) slopelimiter<R>(o[2 + kk], mx[2 + kk], mn[2 + kk], alim, hi, stol, 0, R(0), R(0));
     }
     for (int kk = 0; kk < 3; kk++) {
@@ -452,31 +761,15 @@ __global__ void __launch_bounds__(256) k_gradients(DevParams P, int nrows, const
 // (each active particle accumulates its own side of every pair), then the writeback
 // (:227-321) and hydro_final_operations_and_cleanup (src/hydro/hydro_toplevel.cc:133-250).
 // ---------------------------------------------------------------------------------------
-template <typename R>
-__device__ __forceinline__ void load_cell(const ForceView &F, int j, Cell<R> &c, double gamma)
-{
-    c.h = (R)F.H[j]; c.m = (R)F.M[j]; c.rho = (R)F.RHO[j]; c.p = (R)F.PR[j];
-    c.cn = (R)F.CN[j]; c.fce = (R)F.FCE[j]; c.drk = (R)F.DRK[j];
-    c.v0 = (R)F.VX[j]; c.v1 = (R)F.VY[j]; c.v2 = (R)F.VZ[j];
-    const long n = F.n;
-    c.gr0 = (R)F.GR[j]; c.gr1 = (R)F.GR[n + j]; c.gr2 = (R)F.GR[2 * n + j];
-    c.gp0 = (R)F.GP[j]; c.gp1 = (R)F.GP[n + j]; c.gp2 = (R)F.GP[2 * n + j];
-    for (int k = 0; k < 9; k++) c.gv[k] = (R)F.GV[(size_t)k * n + j];
-    for (int k = 0; k < 6; k++) c.t[k] = (R)F.T[(size_t)k * n + j];
-    // effective_soundspeed, src/declarations/cell_data.h:703-712
-    const double rho = F.RHO[j];
-    c.cs = (R)((rho > 0) ? sqrt(gamma * F.PR[j] / rho) : sqrt(gamma * (gamma - 1.) * F.U[j]));
-}
-
 template <int KERN, typename R>
 __global__ void __launch_bounds__(256) k_force(DevParams P, Gam<R> G, int nrows, const int *__restrict__ act, ForceView F,
-                                               const int *__restrict__ ell, long ld, const int *__restrict__ cnt,
-                                               const uint8_t *__restrict__ active, const int *__restrict__ timebin,
-                                               const double *__restrict__ msv_in, int *__restrict__ wakeup,
-                                               double *__restrict__ ACC, double *__restrict__ DTU, double *__restrict__ MSV,
-                                               int *__restrict__ err)
+                                               const Cell<R> *__restrict__ rec, const int *__restrict__ ell, long ld,
+                                               const int *__restrict__ cnt, const uint8_t *__restrict__ active,
+                                               const int *__restrict__ timebin, const double *__restrict__ msv_in,
+                                               int *__restrict__ wakeup, double *__restrict__ ACC, double *__restrict__ DTU,
+                                               double *__restrict__ MSV, int *__restrict__ err)
 {
-    int r = blockIdx.x * blockDim.x + threadIdx.x;
+    const int r = (int)(xcd_block() * blockDim.x + threadIdx.x);
     if (r >= nrows) return;
     const int i = act[r];
     const long n = F.n;
@@ -486,8 +779,7 @@ __global__ void __launch_bounds__(256) k_force(DevParams P, Gam<R> G, int nrows,
     double sound_i = (rhoi > 0) ? sqrt(P.gamma * F.PR[i] / rhoi) : sqrt(P.gamma * (P.gamma - 1.) * F.U[i]);
     const bool enabled = (mi > 0 && rhoi > 0);
     if (enabled) {
-        Cell<R> I;
-        load_cell<R>(F, i, I, P.gamma);
+        const Cell<R> I = rec[i];
         const double xi = F.X[i], yi = F.Y[i], zi = F.Z[i];
         const int tbi = timebin ? timebin[i] : 0;
         const int c = cnt[r];
@@ -495,8 +787,7 @@ __global__ void __launch_bounds__(256) k_force(DevParams P, Gam<R> G, int nrows,
         for (int q = 0; q < c; q++) {
             const int j = ell[(size_t)q * ld + r];
             if (m < I.cs) m = I.cs;
-            Cell<R> J;
-            load_cell<R>(F, j, J, P.gamma);
+            const Cell<R> J = rec[j];
             double d0 = xi - F.X[j], d1 = yi - F.Y[j], d2 = zi - F.Z[j];
             d1 = wrap(d1, P.box, P.half);
             d0 = wrap(d0, P.box, P.half);
@@ -548,18 +839,25 @@ template __global__ void k_build_ell<true>(int, const int *, const int *, const 
                                                      const double *, const double *, const double *, const double *,            \
                                                      const double *, const double *, const int *, long, const int *,             \
                                                      DensityIter, DensityOut, int *, int *, int *, int *);                       \
+    template __global__ void k_density_cells<KERN, R>(DevParams, GridView, CellRows, const int *, const uint8_t *,              \
+                                                      const double *, const double *, const double *, const double *,           \
+                                                      const double *, const double *, const double *, const double *, double *);\
     template __global__ void k_gradients<KERN, R>(DevParams, int, const int *, const double *, const double *, const double *,  \
-                                                  const double *, const double *, const double *, const double *,               \
-                                                  const double *, const double *, const double *, const double *,               \
-                                                  const double *, long, const int *, long, const int *, double *, double *,      \
-                                                  double *);                                                                    \
-    template __global__ void k_force<KERN, R>(DevParams, Gam<R>, int, const int *, ForceView, const int *, long, const int *,   \
-                                              const uint8_t *, const int *, const double *, int *, double *, double *,          \
-                                              double *, int *);
+                                                  const GradRec<R> *, const double *, const double *, long, const int *, long,  \
+                                                  const int *, double *, double *, double *);                                  \
+    template __global__ void k_force<KERN, R>(DevParams, Gam<R>, int, const int *, ForceView, const Cell<R> *, const int *,     \
+                                              long, const int *, const uint8_t *, const int *, const double *, int *, double *, \
+                                              double *, double *, int *);
 GZ_INST(3, double)
 GZ_INST(6, double)
 GZ_INST(3, float)
 GZ_INST(6, float)
+template __global__ void k_pack_grad<double>(long, const double *, const double *, const double *, const double *, const double *,
+                                             const double *, const double *, GradRec<double> *);
+template __global__ void k_pack_grad<float>(long, const double *, const double *, const double *, const double *, const double *,
+                                            const double *, const double *, GradRec<float> *);
+template __global__ void k_pack_force<double>(double, ForceView, Cell<double> *);
+template __global__ void k_pack_force<float>(double, ForceView, Cell<float> *);
 #undef GZ_INST
 
 }  // namespace gz
