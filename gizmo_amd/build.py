@@ -4,9 +4,10 @@
 
 Flags:
   * --offload-arch=gfx950: MI355X only.
-  * -ffp-contract=off: the FP64 form keeps the reference's rounding order term by term, so
-    it can be compared to the CPU oracle bit for bit (the FP32-mixed kernels opt back into
-    contraction locally).
+  * -ffp-contract=off for the FP64 kernels and the host code: the FP64 form keeps the
+    reference's rounding order term by term, so it can be compared to the CPU oracle bit for
+    bit.  The FP32-mixed kernels (gz_kernels_f32.hip) contract to FMA and use the fast FP32
+    divide and square root.
   * -Xclang -target-feature -Xclang -packed-fp32-ops: no packed FP32 instructions.
 """
 import os
@@ -17,8 +18,8 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libgizmo_hip.so")
-SOURCES = ["gz_kernels.hip", "gz_api.hip"]
-HEADERS = ["gz_device.h", "gz_kernels.h", os.path.join("..", "..", "include", "gizmo_hydro.h")]
+SOURCES = ["gz_kernels.hip", "gz_kernels_f32.hip", "gz_api.hip"]
+HEADERS = ["gz_device.h", "gz_kernels.h", "gz_kernels_impl.h", os.path.join("..", "..", "include", "gizmo_hydro.h")]
 
 
 def hipcc():
@@ -28,8 +29,15 @@ def hipcc():
     raise RuntimeError("hipcc not found")
 
 
-FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
+FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wno-unused-result"]
+# per source: the FP64 form and the host code keep strict rounding; the FP32-mixed kernels
+# contract to FMA and use the fast FP32 divide/square root
+SOURCE_FLAGS = {
+    "gz_kernels.hip": ["-ffp-contract=off"],
+    "gz_api.hip": ["-ffp-contract=off"],
+    "gz_kernels_f32.hip": ["-ffp-contract=fast", "-fno-hip-fp32-correctly-rounded-divide-sqrt"],
+}
 
 
 def build(verbose=False, jobs=None):
@@ -37,7 +45,7 @@ def build(verbose=False, jobs=None):
     procs = []
     for src in SOURCES:
         obj = os.path.join(CSRC, src.replace(".hip", ".o"))
-        cmd = [hipcc()] + FLAGS + ["-c", os.path.join(CSRC, src), "-o", obj]
+        cmd = [hipcc()] + FLAGS + SOURCE_FLAGS.get(src, []) + ["-c", os.path.join(CSRC, src), "-o", obj]
         if verbose:
             print(" ".join(cmd), flush=True)
         procs.append((src, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)))

@@ -1,0 +1,34 @@
+// gz_kernels_f32.hip -- FP32-mixed instantiations.  Built with FMA contraction and the fast FP32
+// divide/square root: this form is held to the stated tolerance against the FP64 oracle, not to
+// its rounding order.  The FP64 pieces it keeps (positions, radius test, 3x3 inverse) follow
+// the same source.
+#include "gz_kernels_impl.h"
+
+namespace gz {
+
+// ---------------------------------------------------------------------------------------
+// instantiations
+// ---------------------------------------------------------------------------------------
+
+#define GZ_INST(KERN, R)                                                                                                         \
+    template __global__ void k_density_iter<KERN, R>(DevParams, int, const int *, const int *, const double *, const double *,  \
+                                                     const double *, const double *, const double *, const double *,            \
+                                                     const double *, const double *, const int *, long, const int *,             \
+                                                     DensityIter, DensityOut, int *, int *, int *, int *);                       \
+    template __global__ void k_density_cells<KERN, R>(DevParams, GridView, CellRows, const int *, const uint8_t *,              \
+                                                      const double *, const double *, const double *, const double *,           \
+                                                      const double *, const double *, const double *, const double *, double *);\
+    template __global__ void k_gradients<KERN, R>(DevParams, int, const int *, const double *, const double *, const double *,  \
+                                                  const GradRec<R> *, const double *, const double *, long, const int *, long,  \
+                                                  const int *, double *, double *, double *);                                  \
+    template __global__ void k_force<KERN, R>(DevParams, Gam<R>, int, const int *, ForceView, const Cell<R> *, const int *,     \
+                                              long, const int *, const uint8_t *, const int *, const double *, int *, double *, \
+                                              double *, double *, int *);
+GZ_INST(3, float)
+GZ_INST(6, float)
+template __global__ void k_pack_grad<float>(long, const double *, const double *, const double *, const double *, const double *,
+                                            const double *, const double *, GradRec<float> *);
+template __global__ void k_pack_force<float>(double, ForceView, Cell<float> *);
+#undef GZ_INST
+
+}  // namespace gz

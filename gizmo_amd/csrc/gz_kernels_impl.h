@@ -1,0 +1,879 @@
+// gz_kernels_impl.h -- gfx950 kernels of the MFM hydro pipeline (templates; instantiated per
+// arithmetic form in gz_kernels.hip (FP64, strict) and gz_kernels_f32.hip (FP32-mixed)).
+//
+// Layout: particles live on the device in structure-of-arrays form, ordered by a 30-bit
+// Morton key of a fixed 1024^3 grid over the periodic box.  Any power-of-two search grid
+// (cell side >= the search radius) is then a contiguous range of that order per cell, and
+// the 27 neighbour-cell ranges of a cell, sorted by start, visit candidates in ascending
+// device index: the canonical summation order, shared with the CPU oracle.
+//
+// Neighbour lists are ELLPACK, column-major: entry q of row r sits at ell[q*ld + r], so the
+// 64 lanes of a wave (64 consecutive rows) read one coalesced line per q.  One lane per
+// active particle; wave64.
+#pragma once
+#include "gz_kernels.h"
+
+namespace gz {
+
+// ---------------------------------------------------------------------------------------
+// keys, gathers, scatters
+// ---------------------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t spread10(uint32_t v)
+{
+    v &= 0x3ffu;
+    v = (v | (v << 16)) & 0x030000ffu;
+    v = (v | (v << 8)) & 0x0300f00fu;
+    v = (v | (v << 4)) & 0x030c30c3u;
+    v = (v | (v << 2)) & 0x09249249u;
+    return v;
+}
+
+__device__ __forceinline__ uint32_t compact10(uint32_t v)
+{
+    v &= 0x09249249u;
+    v = (v | (v >> 2)) & 0x030c30c3u;
+    v = (v | (v >> 4)) & 0x0300f00fu;
+    v = (v | (v >> 8)) & 0x030000ffu;
+    v = (v | (v >> 16)) & 0x000003ffu;
+    return v;
+}
+
+__device__ __forceinline__ int fine_coord(double x, double box)
+{
+    double w = x - box * floor(x / box);
+    int c = (int)(w * (double)kFine / box);
+    return c < 0 ? 0 : (c >= kFine ? kFine - 1 : c);
+}
+
+#ifdef GZ_DEFINE_COMMON_KERNELS
+__global__ void k_morton(long n, const double *__restrict__ pos, double box, uint32_t *__restrict__ key, int *__restrict__ val)
+{
+    long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t cx = fine_coord(pos[3 * i + 0], box), cy = fine_coord(pos[3 * i + 1], box), cz = fine_coord(pos[3 * i + 2], box);
+    key[i] = (spread10(cx) << 2) | (spread10(cy) << 1) | spread10(cz);
+    val[i] = (int)i;
+}
+#endif
+
+
+// sorted field s <- user field perm[s] (width doubles per particle; interleaved in, SoA out
+// when soa != 0 with column stride n)
+#ifdef GZ_DEFINE_COMMON_KERNELS
+__global__ void k_gather(long n, int width, const int *__restrict__ perm, const double *__restrict__ in,
+                         double *__restrict__ out, int soa)
+{
+    long s = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    long u = perm[s];
+    for (int c = 0; c < width; c++) {
+        double v = in[(size_t)u * width + c];
+        if (soa) out[(size_t)c * n + s] = v; else out[(size_t)s * width + c] = v;
+    }
+}
+#endif
+
+
+#ifdef GZ_DEFINE_COMMON_KERNELS
+__global__ void k_gather_int(long n, const int *__restrict__ perm, const int *__restrict__ in, int *__restrict__ out)
+{
+    long s = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    out[s] = in[perm[s]];
+}
+#endif
+
+
+#ifdef GZ_DEFINE_COMMON_KERNELS
+__global__ void k_gather_u8(long n, const int *__restrict__ perm, const uint8_t *__restrict__ in, uint8_t *__restrict__ out)
+{
+    long s = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    out[s] = in[perm[s]];
+}
+#endif
+
+
+// user field perm[s] <- sorted field s
+#ifdef GZ_DEFINE_COMMON_KERNELS
+__global__ void k_scatter(long n, int width, const int *__restrict__ perm, const double *__restrict__ in,
+                          double *__restrict__ out, int soa)
+{
+    long s = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    long u = perm[s];
+    for (int c = 0; c < width; c++) {
+        double v = soa ? in[(size_t)c * n + s] : in[(size_t)s * width + c];
+        out[(size_t)u * width + c] = v;
+    }
+}
+#endif
+
+
+#ifdef GZ_DEFINE_COMMON_KERNELS
+__global__ void k_scatter_int(long n, const int *__restrict__ perm, const int *__restrict__ in, int *__restrict__ out)
+{
+    long s = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    out[perm[s]] = in[s];
+}
+#endif
+
+
+// active rows: act[r] = sorted index of the r-th active particle (ascending)
+#ifdef GZ_DEFINE_COMMON_KERNELS
+__global__ void k_flag_active(long n, const uint8_t *__restrict__ active, const double *__restrict__ mass, int *__restrict__ flag)
+{
+    long s = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    flag[s] = (active == nullptr || active[s]) ? 1 : 0;
+    (void)mass;
+}
+#endif
+
+
+#ifdef GZ_DEFINE_COMMON_KERNELS
+__global__ void k_compact(long n, const int *__restrict__ flag, const int *__restrict__ scan, int *__restrict__ act)
+{
+    long s = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    if (flag[s]) act[scan[s]] = (int)s;
+}
+#endif
+
+
+// ---------------------------------------------------------------------------------------
+// cell grid at level L (nc = 2^L): cells indexed by Morton code, [start, end) of the order
+// ---------------------------------------------------------------------------------------
+#ifdef GZ_DEFINE_COMMON_KERNELS
+__global__ void k_cell_clear(long ncell, int *__restrict__ cstart, int *__restrict__ cend, double *__restrict__ chmax)
+{
+    long c = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (c >= ncell) return;
+    cstart[c] = 0;
+    cend[c] = 0;
+    chmax[c] = 0;
+}
+#endif
+
+
+#ifdef GZ_DEFINE_COMMON_KERNELS
+__global__ void k_cell_bounds(long n, const uint32_t *__restrict__ key, int shift, int *__restrict__ cstart, int *__restrict__ cend)
+{
+    long s = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    uint32_t c = key[s] >> shift;
+    if (s == 0 || (key[s - 1] >> shift) != c) cstart[c] = (int)s;
+    if (s == n - 1 || (key[s + 1] >> shift) != c) cend[c] = (int)(s + 1);
+}
+#endif
+
+
+// largest kernel radius per cell (the symmetric search reaches max(h_i, h_j))
+#ifdef GZ_DEFINE_COMMON_KERNELS
+__global__ void k_cell_hmax(long ncell, const int *__restrict__ cstart, const int *__restrict__ cend,
+                            const double *__restrict__ h, double *__restrict__ chmax)
+{
+    long c = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (c >= ncell) return;
+    double m = 0;
+    for (int s = cstart[c]; s < cend[c]; s++) m = fmax(m, h[s]);
+    chmax[c] = m;
+}
+#endif
+
+
+// the (deduplicated) 27 neighbour cells of each cell, non-empty ones only, sorted by start;
+// each entry packs start, end and the offset code (dx+1)*9+(dy+1)*3+(dz+1)
+#ifdef GZ_DEFINE_COMMON_KERNELS
+__global__ void k_cell_stencil(int level, const int *__restrict__ cstart, const int *__restrict__ cend,
+                               int4 *__restrict__ sten, int *__restrict__ nsten)
+{
+    long ncell = 1L << (3 * level);
+    long c = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (c >= ncell) return;
+    int nc = 1 << level;
+    int cx = compact10((uint32_t)c >> 2), cy = compact10((uint32_t)c >> 1), cz = compact10((uint32_t)c);
+    int4 e[27];
+    int m = 0;
+    for (int dx = -1; dx <= 1; dx++)
+        for (int dy = -1; dy <= 1; dy++)
+            for (int dz = -1; dz <= 1; dz++) {
+                int a = (cx + dx + nc) % nc, b = (cy + dy + nc) % nc, d = (cz + dz + nc) % nc;
+                uint32_t code = (spread10(a) << 2) | (spread10(b) << 1) | spread10(d);
+                int st = cstart[code], en = cend[code];
+                if (en <= st) continue;
+                bool dup = false;
+                for (int q = 0; q < m; q++) if (e[q].x == st) { dup = true; break; }
+                if (dup) continue;
+                e[m++] = make_int4(st, en, (dx + 1) * 9 + (dy + 1) * 3 + (dz + 1), (int)code);
+            }
+    for (int a = 1; a < m; a++) {  // insertion sort by start
+        int4 t = e[a];
+        int b = a - 1;
+        while (b >= 0 && e[b].x > t.x) { e[b + 1] = e[b]; b--; }
+        e[b + 1] = t;
+    }
+    for (int q = 0; q < m; q++) sten[c * 27 + q] = e[q];
+    nsten[c] = m;
+}
+#endif
+
+
+// ---------------------------------------------------------------------------------------
+// neighbour lists (gpu_ngb_list_build, src/mesh/gpu_neighbor_list.cc:1950, exact FP64 accept)
+//   SYMMETRIC = false: rows r2 < (scale*h_i)^2, self included (density, scale = 1.3 buffer)
+//   SYMMETRIC = true : rows r2 < max(h_i,h_j)^2, self excluded (gradients, force)
+// rows: the slots listed in rows[0..nrows) (row r of the list is slot rows[k]); act[slot] =
+// sorted index of that slot's particle.
+// ---------------------------------------------------------------------------------------
+template <bool SYMMETRIC>
+__global__ void __launch_bounds__(256) k_build_ell(int nrows, const int *__restrict__ rows, const int *__restrict__ act,
+                                                   const double *__restrict__ hrow, double scale, const double *__restrict__ X,
+                                                   const double *__restrict__ Y, const double *__restrict__ Z, const double *__restrict__ H,
+                                                   const uint32_t *__restrict__ key, GridView g, int *__restrict__ ell, long ld, int cap,
+                                                   int *__restrict__ cnt, int *__restrict__ overflow)
+{
+    const long k = xcd_block() * blockDim.x + threadIdx.x;
+    if (k >= nrows) return;
+    const int r = rows ? rows[k] : (int)k;
+    const int i = act[r];
+    const double xi = X[i], yi = Y[i], zi = Z[i];
+    const double hi = hrow ? hrow[r] * scale : H[i];
+    const uint32_t home = key[i] >> g.shift;
+    const int nc = 1 << g.level;
+    const int cx = compact10(home >> 2), cy = compact10(home >> 1), cz = compact10(home);
+    const bool prune = nc >= 3;
+    const int m = g.nsten[home];
+    int c = 0;
+    for (int q = 0; q < m; q++) {
+        const int4 e = g.sten[(size_t)home * 27 + q];
+        if (prune) {
+            double reach = hi;
+            if (SYMMETRIC) reach = fmax(hi, g.chmax[e.w]);
+            reach = reach * (1.0 + 1e-9) + 1e-6 * g.cell;
+            const int dx = e.z / 9 - 1, dy = (e.z / 3) % 3 - 1, dz = e.z % 3 - 1;
+            const double lx = (cx + dx) * g.cell, ly = (cy + dy) * g.cell, lz = (cz + dz) * g.cell;
+            const double xw = xi - g.box * floor(xi / g.box), yw = yi - g.box * floor(yi / g.box), zw = zi - g.box * floor(zi / g.box);
+            double ex = fmax(fmax(lx - xw, xw - (lx + g.cell)), 0.0);
+            double ey = fmax(fmax(ly - yw, yw - (ly + g.cell)), 0.0);
+            double ez = fmax(fmax(lz - zw, zw - (lz + g.cell)), 0.0);
+            if (ex * ex + ey * ey + ez * ez > reach * reach) continue;
+        }
+        for (int j = e.x; j < e.y; j++) {
+            if (SYMMETRIC && j == i) continue;
+            if (accept_pair<SYMMETRIC>(xi, yi, zi, X[j], Y[j], Z[j], hi, H[j], g.box, g.half)) {
+                if (c < cap) ell[(size_t)c * ld + r] = j;
+                c++;
+            }
+        }
+    }
+    cnt[r] = c;
+    if (c > cap) atomicMax(overflow, c);
+}
+
+// ---------------------------------------------------------------------------------------
+// density: fused pair sums (DensitySpec::pair_kernel, src/hydro/density_loop.h:1141-1385),
+// after_iter_body (:516-961) and, on convergence, density_finalize_post_runner
+// (src/hydro/density_loop.cc:303-733)
+// ---------------------------------------------------------------------------------------
+__device__ inline void density_finalize(const DevParams &P, const DensityOut &o, int i, double h, const double *acc, double mass, double u)
+{
+    o.h[i] = h;
+    double nn, dr, dv;
+    density_normalise(acc, h, nn, dr, dv);
+    o.drk[i] = dr;
+    o.divv[i] = dv;
+    for (int k = 0; k < 3; k++) o.fw[(size_t)k * o.n + i] = acc[10 + k];
+    const double V_i = kVolNorm * pow(h, 3.0) / (nn > 0 ? nn : 1.0);
+    const double normalizer = pow(h, -1.0);
+    double Ti[9];
+    const double cn = nvt_inverse(&acc[4], normalizer, Ti);
+    o.t[0 * o.n + i] = Ti[0] / normalizer;
+    o.t[1 * o.n + i] = Ti[4] / normalizer;
+    o.t[2 * o.n + i] = Ti[8] / normalizer;
+    o.t[3 * o.n + i] = Ti[1] / normalizer;
+    o.t[4 * o.n + i] = Ti[5] / normalizer;
+    o.t[5 * o.n + i] = Ti[2] / normalizer;
+    o.cn[i] = cn;
+    o.fce[i] = face_closure(&acc[4], &acc[10], Ti, normalizer, V_i);
+    double rho = acc[1];
+    if (mass > 0) {
+        if ((h > 0) && (nn > 0)) rho = mass * nn / (kVolNorm * pow(h, 3.0));
+        else if (h <= 0) rho = 0;
+        else rho = mass / (kVolNorm * pow(h, 3.0));
+        o.p[i] = (P.gamma - 1) * u * rho;
+    }
+    o.rho[i] = rho;
+    o.nn[i] = (nn > 0) ? pow(nn, 1.0 / 3.0) : 0.0;
+}
+
+template <int KERN, typename R>
+__global__ void __launch_bounds__(256) k_density_iter(DevParams P, int ntodo, const int *__restrict__ todo, const int *__restrict__ act,
+                                                      const double *__restrict__ X, const double *__restrict__ Y, const double *__restrict__ Z,
+                                                      const double *__restrict__ VX, const double *__restrict__ VY, const double *__restrict__ VZ,
+                                                      const double *__restrict__ M, const double *__restrict__ U,
+                                                      const int *__restrict__ ell, long ld, const int *__restrict__ cnt,
+                                                      DensityIter it, DensityOut out, int *__restrict__ next, int *__restrict__ nnext,
+                                                      int *__restrict__ rebuild, int *__restrict__ err)
+{
+    const long k = xcd_block() * blockDim.x + threadIdx.x;
+    if (k >= ntodo) return;
+    const int r = todo[k];
+    const int i = act[r];
+    const double h = it.hcur[r];
+    const double xi = X[i], yi = Y[i], zi = Z[i];
+    const R vxi = (R)VX[i], vyi = (R)VY[i], vzi = (R)VZ[i];
+    R a[13];
+    for (int q = 0; q < 13; q++) a[q] = 0;
+    const R hR = (R)h;
+    const R h2 = hR * hR;
+    R hinv, hinv3, hinv4;
+    kernel_hinv(hR, hinv, hinv3, hinv4);
+    const int c = cnt[r];
+    for (int q = 0; q < c; q++) {
+        const int j = ell[(size_t)q * ld + r];
+        const double mj = M[j];
+        if (mj <= 0) continue;
+        double d0 = xi - X[j], d1 = yi - Y[j], d2 = zi - Z[j];
+        d1 = wrap(d1, P.box, P.half);
+        d0 = wrap(d0, P.box, P.half);
+        d2 = wrap(d2, P.box, P.half);
+        const R dp0 = (R)d0, dp1 = (R)d1, dp2 = (R)d2;
+        const R r2 = dp0 * dp0 + dp1 * dp1 + dp2 * dp2;
+        if (r2 >= h2) continue;
+        const R rr = sqrt(r2);
+        const R u = rr * hinv;
+        R wk = 0, dwk = 0;
+        kernel_main<KERN, 0>(u, hinv3, hinv4, wk, dwk);
+        const R mj_wk = (R)mj * wk;
+        a[0] += wk;
+        a[1] += mj_wk;
+        a[2] += -(R(3.0) * hinv * wk + u * dwk);
+        if (rr > 0) {
+            a[4] += wk * (dp0 * dp0);
+            a[5] += wk * (dp1 * dp1);
+            a[6] += wk * (dp2 * dp2);
+            a[7] += wk * (dp0 * dp1);
+            a[8] += wk * (dp1 * dp2);
+            a[9] += wk * (dp0 * dp2);
+            a[10] += wk * dp0;
+            a[11] += wk * dp1;
+            a[12] += wk * dp2;
+            const R dv0 = vxi - (R)VX[j], dv1 = vyi - (R)VY[j], dv2 = vzi - (R)VZ[j];
+            a[3] -= dwk * (dp0 * dv0 + dp1 * dv1 + dp2 * dv2) / rr;
+        }
+    }
+    double acc[13];
+    for (int q = 0; q < 13; q++) acc[q] = (double)a[q];
+    IterState sc;
+    sc.left = it.left[r]; sc.right = it.right[r]; sc.cn_cur = it.cn_cur[r]; sc.cn_init = it.cn_init[r];
+    const int iter = it.iter[r];
+    double nh = h;
+    int rc = density_after_iter(P, out.cn[i], iter, h, acc, sc, nh);
+    it.iter[r] = iter + 1;
+    if (rc <= 0) {
+        if (rc < 0) atomicMax(err, 90001009);
+        density_finalize(P, out, i, h, acc, M[i], U[i]);
+        return;
+    }
+    it.left[r] = sc.left; it.right[r] = sc.right; it.cn_cur[r] = sc.cn_cur; it.cn_init[r] = sc.cn_init;
+    it.hcur[r] = nh;
+    if (iter + 1 >= kMaxIter) {
+        atomicMax(err, 1155);
+        density_finalize(P, out, i, h, acc, M[i], U[i]);
+        return;
+    }
+    if (nh > 1.3 * it.hbuild[r]) atomicOr(rebuild, 1);
+    int w = atomicAdd(nnext, 1);
+    next[w] = r;
+}
+
+// ---------------------------------------------------------------------------------------
+// per-cell active rows: the active slots of one grid cell are a contiguous range of act[]
+// (act ascends in device order, cells are contiguous in it); one entry per occupied cell
+// ---------------------------------------------------------------------------------------
+#ifdef GZ_DEFINE_COMMON_KERNELS
+__global__ void k_cell_rows_mark(int nact, const int *__restrict__ act, const uint32_t *__restrict__ key, int shift,
+                                 int *__restrict__ ca0, int *__restrict__ ca1, int *__restrict__ flag)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nact) return;
+    const uint32_t c = key[act[r]] >> shift;
+    const bool first = (r == 0) || ((key[act[r - 1]] >> shift) != c);
+    const bool last = (r == nact - 1) || ((key[act[r + 1]] >> shift) != c);
+    if (first) ca0[c] = r;
+    if (last) ca1[c] = r + 1;
+    flag[r] = first ? 1 : 0;
+}
+#endif
+
+
+#ifdef GZ_DEFINE_COMMON_KERNELS
+__global__ void k_cell_rows_list(int nact, const int *__restrict__ act, const uint32_t *__restrict__ key, int shift,
+                                 const int *__restrict__ flag, const int *__restrict__ scan, int *__restrict__ cells)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nact) return;
+    if (flag[r]) cells[scan[r]] = (int)(key[act[r]] >> shift);
+}
+#endif
+
+
+__device__ __forceinline__ bool wave_any(bool v) { return __any(v ? 1 : 0) != 0; }
+
+// ---------------------------------------------------------------------------------------
+// leaf culling for the cell walks.  A wavefront's particles (one leaf) have a bounding box in
+// coordinates relative to the home cell's origin; a candidate farther from that box than the
+// reach of the pair test cannot be accepted by any lane and is dropped while its tile is
+// staged, keeping the survivors in ascending order (stable compaction over the ballot mask).
+// The test is conservative: the slack covers rounding of the relative coordinates.
+// ---------------------------------------------------------------------------------------
+struct LeafBox {
+    double ox, oy, oz;          // home cell origin
+    double lo[3], hi[3];        // leaf box, relative to the origin
+    bool cull;                  // false on grids too coarse for relative coordinates
+};
+
+__device__ __forceinline__ double wave_min(double v)
+{
+    for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
+    return v;
+}
+__device__ __forceinline__ double wave_max(double v)
+{
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+__device__ __forceinline__ LeafBox leaf_box(const GridView &g, int c, bool on, double xi, double yi, double zi)
+{
+    LeafBox b;
+    const int cx = compact10((uint32_t)c >> 2), cy = compact10((uint32_t)c >> 1), cz = compact10((uint32_t)c);
+    b.ox = cx * g.cell; b.oy = cy * g.cell; b.oz = cz * g.cell;
+    const double rx = wrap(xi - b.ox, g.box, g.half), ry = wrap(yi - b.oy, g.box, g.half), rz = wrap(zi - b.oz, g.box, g.half);
+    b.lo[0] = wave_min(on ? rx : 1e300); b.hi[0] = wave_max(on ? rx : -1e300);
+    b.lo[1] = wave_min(on ? ry : 1e300); b.hi[1] = wave_max(on ? ry : -1e300);
+    b.lo[2] = wave_min(on ? rz : 1e300); b.hi[2] = wave_max(on ? rz : -1e300);
+    b.cull = (1 << g.level) >= 4;
+    return b;
+}
+
+__device__ __forceinline__ bool leaf_reach(const GridView &g, const LeafBox &b, double xj, double yj, double zj, double reach)
+{
+    if (!b.cull) return true;
+    const double rx = wrap(xj - b.ox, g.box, g.half), ry = wrap(yj - b.oy, g.box, g.half), rz = wrap(zj - b.oz, g.box, g.half);
+    const double ex = fmax(fmax(b.lo[0] - rx, rx - b.hi[0]), 0.0);
+    const double ey = fmax(fmax(b.lo[1] - ry, ry - b.hi[1]), 0.0);
+    const double ez = fmax(fmax(b.lo[2] - rz, rz - b.hi[2]), 0.0);
+    const double rr = reach * (1.0 + 1e-9) + 1e-12 * g.box;
+    return ex * ex + ey * ey + ez * ez <= rr * rr;
+}
+
+// position of this lane among the kept lanes below it (stable compaction)
+__device__ __forceinline__ int kept_rank(bool keep, int &count)
+{
+    const unsigned long long m = __ballot(keep ? 1 : 0);
+    count = __popcll(m);
+    const unsigned lane = threadIdx.x & 63;
+    return __popcll(m & ((lane == 0) ? 0ull : (~0ull >> (64 - lane))));
+}
+
+// ---------------------------------------------------------------------------------------
+// density as a cell walk: one wavefront per occupied cell, its active particles one per lane;
+// the candidates of the cell's 27 neighbour ranges are staged through LDS 64 at a time (after
+// leaf culling) and every lane tests them in ascending order (the oracle's summation order).
+// need[r] marks the rows still iterating.  The 13 pair sums of DensitySpec::pair_kernel
+// (src/hydro/density_loop.h:1141-1385) go to acc[13*r]; k_density_after closes the round.
+// ---------------------------------------------------------------------------------------
+template <int KERN, typename R>
+__global__ void __launch_bounds__(64) k_density_cells(DevParams P, GridView g, CellRows cr, const int *__restrict__ act,
+                                                      const uint8_t *__restrict__ need, const double *__restrict__ X,
+                                                      const double *__restrict__ Y, const double *__restrict__ Z,
+                                                      const double *__restrict__ VX, const double *__restrict__ VY,
+                                                      const double *__restrict__ VZ, const double *__restrict__ M,
+                                                      const double *__restrict__ hcur, double *__restrict__ accout)
+{
+    __shared__ double sx[64], sy[64], sz[64], svx[64], svy[64], svz[64], sm[64];
+    const long cb = xcd_block();
+    if (cb >= cr.ncells) return;
+    const int c = cr.cells[cb];
+    const int a0 = cr.ca0[c], a1 = cr.ca1[c];
+    const int nst = g.nsten[c];
+    const int t = threadIdx.x;
+    for (int base = a0; base < a1; base += 64) {
+        const int r = base + t;
+        const bool on = (r < a1) && need[r];
+        if (!wave_any(on)) continue;
+        int i = 0;
+        double xi = 0, yi = 0, zi = 0, h = 0;
+        R vxi = 0, vyi = 0, vzi = 0;
+        if (on) {
+            i = act[r];
+            xi = X[i]; yi = Y[i]; zi = Z[i];
+            vxi = (R)VX[i]; vyi = (R)VY[i]; vzi = (R)VZ[i];
+            h = hcur[r];
+        }
+        const LeafBox lb = leaf_box(g, c, on, xi, yi, zi);
+        const double reach = wave_max(on ? h : 0.0);
+        R a[13];
+        for (int q = 0; q < 13; q++) a[q] = 0;
+        const R hR = (R)(on ? h : 1.0);
+        const R h2 = hR * hR;
+        R hinv, hinv3, hinv4;
+        kernel_hinv(hR, hinv, hinv3, hinv4);
+        for (int q = 0; q < nst; q++) {
+            const int4 e = g.sten[(size_t)c * 27 + q];
+            for (int ts = e.x; ts < e.y; ts += 64) {
+                const int j = ts + t;
+                bool keep = false;
+                double xj = 0, yj = 0, zj = 0;
+                if (j < e.y) {
+                    xj = X[j]; yj = Y[j]; zj = Z[j];
+                    keep = leaf_reach(g, lb, xj, yj, zj, reach);
+                }
+                int nt;
+                const int pos = kept_rank(keep, nt);
+                __syncthreads();
+                if (keep) {
+                    sx[pos] = xj; sy[pos] = yj; sz[pos] = zj;
+                    svx[pos] = VX[j]; svy[pos] = VY[j]; svz[pos] = VZ[j]; sm[pos] = M[j];
+                }
+                __syncthreads();
+                if (!on) continue;
+                for (int k = 0; k < nt; k++) {
+                    const double mj = sm[k];
+                    if (mj <= 0) continue;
+                    double d0 = xi - sx[k], d1 = yi - sy[k], d2 = zi - sz[k];
+                    d1 = wrap(d1, P.box, P.half);
+                    d0 = wrap(d0, P.box, P.half);
+                    d2 = wrap(d2, P.box, P.half);
+                    const R dp0 = (R)d0, dp1 = (R)d1, dp2 = (R)d2;
+                    const R r2 = dp0 * dp0 + dp1 * dp1 + dp2 * dp2;
+                    if (r2 >= h2) continue;
+                    const R rr = sqrt(r2);
+                    const R u = rr * hinv;
+                    R wk = 0, dwk = 0;
+                    kernel_main<KERN, 0>(u, hinv3, hinv4, wk, dwk);
+                    const R mj_wk = (R)mj * wk;
+                    a[0] += wk;
+                    a[1] += mj_wk;
+                    a[2] += -(R(3.0) * hinv * wk + u * dwk);
+                    if (rr > 0) {
+                        a[4] += wk * (dp0 * dp0);
+                        a[5] += wk * (dp1 * dp1);
+                        a[6] += wk * (dp2 * dp2);
+                        a[7] += wk * (dp0 * dp1);
+                        a[8] += wk * (dp1 * dp2);
+                        a[9] += wk * (dp0 * dp2);
+                        a[10] += wk * dp0;
+                        a[11] += wk * dp1;
+                        a[12] += wk * dp2;
+                        const R dv0 = vxi - (R)svx[k], dv1 = vyi - (R)svy[k], dv2 = vzi - (R)svz[k];
+                        a[3] -= dwk * (dp0 * dv0 + dp1 * dv1 + dp2 * dv2) / rr;
+                    }
+                }
+            }
+        }
+        if (on)
+            for (int q = 0; q < 13; q++) accout[(size_t)q * cr.nrows + r] = (double)a[q];
+    }
+}
+
+// after_iter_body (src/hydro/density_loop.h:516-961) on the sums of the round and, on
+// convergence, density_finalize_post_runner (src/hydro/density_loop.cc:303-733)
+#ifdef GZ_DEFINE_COMMON_KERNELS
+__global__ void __launch_bounds__(256) k_density_after(DevParams P, int nrows, const int *__restrict__ act, uint8_t *__restrict__ need,
+                                                       const double *__restrict__ accin, const double *__restrict__ M,
+                                                       const double *__restrict__ U, double cell, DensityIter it, DensityOut out,
+                                                       int *__restrict__ nnext, int *__restrict__ rebuild, int *__restrict__ err)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nrows || !need[r]) return;
+    const int i = act[r];
+    double acc[13];
+    for (int q = 0; q < 13; q++) acc[q] = accin[(size_t)q * nrows + r];
+    const double h = it.hcur[r];
+    IterState sc;
+    sc.left = it.left[r]; sc.right = it.right[r]; sc.cn_cur = it.cn_cur[r]; sc.cn_init = it.cn_init[r];
+    const int iter = it.iter[r];
+    double nh = h;
+    const int rc = density_after_iter(P, out.cn[i], iter, h, acc, sc, nh);
+    it.iter[r] = iter + 1;
+    if (rc <= 0 || iter + 1 >= kMaxIter) {
+        if (rc < 0) atomicMax(err, 90001009);
+        else if (rc > 0) atomicMax(err, 1155);
+        density_finalize(P, out, i, h, acc, M[i], U[i]);
+        need[r] = 0;
+        return;
+    }
+    it.left[r] = sc.left; it.right[r] = sc.right; it.cn_cur[r] = sc.cn_cur; it.cn_init[r] = sc.cn_init;
+    it.hcur[r] = nh;
+    if (nh * (1.0 + 1e-6) > cell) atomicOr(rebuild, 1);
+    atomicAdd(nnext, 1);
+}
+#endif
+
+
+// ---------------------------------------------------------------------------------------
+// symmetric list as a cell walk (one wavefront per occupied cell, culled LDS-staged candidates):
+// rows r2 < max(h_i,h_j)^2, self excluded, ascending (src/mesh/sfc_tiles_functions.h:83-102)
+// ---------------------------------------------------------------------------------------
+#ifdef GZ_DEFINE_COMMON_KERNELS
+__global__ void __launch_bounds__(64) k_build_sym_cells(GridView g, CellRows cr, const int *__restrict__ act,
+                                                        const double *__restrict__ X, const double *__restrict__ Y,
+                                                        const double *__restrict__ Z, const double *__restrict__ H,
+                                                        int *__restrict__ ell, long ld, int cap, int *__restrict__ cnt,
+                                                        int *__restrict__ overflow)
+{
+    __shared__ double sx[64], sy[64], sz[64], sh[64];
+    __shared__ int sj[64];
+    const long cb = xcd_block();
+    if (cb >= cr.ncells) return;
+    const int c = cr.cells[cb];
+    const int a0 = cr.ca0[c], a1 = cr.ca1[c];
+    const int nst = g.nsten[c];
+    const int t = threadIdx.x;
+    for (int base = a0; base < a1; base += 64) {
+        const int r = base + t;
+        const bool on = r < a1;
+        int i = -1;
+        double xi = 0, yi = 0, zi = 0, hi = 0;
+        if (on) { i = act[r]; xi = X[i]; yi = Y[i]; zi = Z[i]; hi = H[i]; }
+        const LeafBox lb = leaf_box(g, c, on, xi, yi, zi);
+        const double hleaf = wave_max(on ? hi : 0.0);
+        int n = 0;
+        for (int q = 0; q < nst; q++) {
+            const int4 e = g.sten[(size_t)c * 27 + q];
+            for (int ts = e.x; ts < e.y; ts += 64) {
+                const int j = ts + t;
+                bool keep = false;
+                double xj = 0, yj = 0, zj = 0, hj = 0;
+                if (j < e.y) {
+                    xj = X[j]; yj = Y[j]; zj = Z[j]; hj = H[j];
+                    keep = leaf_reach(g, lb, xj, yj, zj, fmax(hleaf, hj));
+                }
+                int nt;
+                const int pos = kept_rank(keep, nt);
+                __syncthreads();
+                if (keep) { sx[pos] = xj; sy[pos] = yj; sz[pos] = zj; sh[pos] = hj; sj[pos] = j; }
+                __syncthreads();
+                if (!on) continue;
+                for (int k = 0; k < nt; k++) {
+                    const int jj = sj[k];
+                    if (jj == i) continue;
+                    if (accept_pair<true>(xi, yi, zi, sx[k], sy[k], sz[k], hi, sh[k], g.box, g.half)) {
+                        if (n < cap) ell[(size_t)n * ld + r] = jj;
+                        n++;
+                    }
+                }
+            }
+        }
+        if (on) {
+            cnt[r] = n;
+            if (n > cap) atomicMax(overflow, n);
+        }
+    }
+}
+#endif
+
+
+// ---------------------------------------------------------------------------------------
+// records: the neighbour-side fields of a pass, packed per particle (device order)
+// ---------------------------------------------------------------------------------------
+template <typename R>
+__global__ void k_pack_grad(long n, const double *__restrict__ H, const double *__restrict__ M, const double *__restrict__ RHO,
+                            const double *__restrict__ PR, const double *__restrict__ VX, const double *__restrict__ VY,
+                            const double *__restrict__ VZ, GradRec<R> *__restrict__ rec)
+{
+    const long j = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    GradRec<R> g;
+    g.h = (R)H[j]; g.m = (R)M[j]; g.rho = (R)RHO[j]; g.p = (R)PR[j];
+    g.v0 = (R)VX[j]; g.v1 = (R)VY[j]; g.v2 = (R)VZ[j]; g.pad = 0;
+    rec[j] = g;
+}
+
+template <typename R>
+__global__ void k_pack_force(double gamma, ForceView F, Cell<R> *__restrict__ rec)
+{
+    const long j = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    const long n = F.n;
+    if (j >= n) return;
+    Cell<R> c;
+    c.h = (R)F.H[j]; c.m = (R)F.M[j]; c.rho = (R)F.RHO[j]; c.p = (R)F.PR[j];
+    c.cn = (R)F.CN[j]; c.fce = (R)F.FCE[j]; c.drk = (R)F.DRK[j];
+    c.v0 = (R)F.VX[j]; c.v1 = (R)F.VY[j]; c.v2 = (R)F.VZ[j];
+    c.gr0 = (R)F.GR[j]; c.gr1 = (R)F.GR[n + j]; c.gr2 = (R)F.GR[2 * n + j];
+    c.gp0 = (R)F.GP[j]; c.gp1 = (R)F.GP[n + j]; c.gp2 = (R)F.GP[2 * n + j];
+    for (int k = 0; k < 9; k++) c.gv[k] = (R)F.GV[(size_t)k * n + j];
+    for (int k = 0; k < 6; k++) c.t[k] = (R)F.T[(size_t)k * n + j];
+    // effective_soundspeed, src/declarations/cell_data.h:703-712
+    const double rho = F.RHO[j];
+    c.cs = (R)((rho > 0) ? sqrt(gamma * F.PR[j] / rho) : sqrt(gamma * (gamma - 1.) * F.U[j]));
+    rec[j] = c;
+}
+
+// ---------------------------------------------------------------------------------------
+// gradients: gradient_accumulate_neighbor (src/hydro/gradient_functions.h:305-665),
+// construct_gradient (src/hydro/gradients_loop.cc:129-136) and the limiter block
+// (:1077-1130), fused per active particle
+// ---------------------------------------------------------------------------------------
+template <int KERN, typename R>
+__global__ void __launch_bounds__(256) k_gradients(DevParams P, int nrows, const int *__restrict__ act, const double *__restrict__ X,
+                                                   const double *__restrict__ Y, const double *__restrict__ Z,
+                                                   const GradRec<R> *__restrict__ rec, const double *__restrict__ T,
+                                                   const double *__restrict__ CN, long n, const int *__restrict__ ell, long ld,
+                                                   const int *__restrict__ cnt, double *__restrict__ GR, double *__restrict__ GP,
+                                                   double *__restrict__ GV)
+{
+    const int r = (int)(xcd_block() * blockDim.x + threadIdx.x);
+    if (r >= nrows) return;
+    const int i = act[r];
+    R G[3][5];
+    for (int a = 0; a < 3; a++) for (int b = 0; b < 5; b++) G[a][b] = 0;
+    R mx[5], mn[5];
+    for (int b = 0; b < 5; b++) { mx[b] = 0; mn[b] = 0; }
+    R maxd = 0;
+    const GradRec<R> I = rec[i];
+    const bool enabled = (I.m > 0 && I.rho > 0);
+    const R hi = I.h;
+    if (enabled) {
+        const double xi = X[i], yi = Y[i], zi = Z[i];
+        R hinv, hinv3, hinv4;
+        kernel_hinv(hi, hinv, hinv3, hinv4);
+        const R h2i = hi * hi;
+        const int c = cnt[r];
+        for (int q = 0; q < c; q++) {
+            const int j = ell[(size_t)q * ld + r];
+            const GradRec<R> J = rec[j];
+            if (!(J.m > 0) || !(J.rho > 0) || !(J.h > 0)) continue;
+            double d0 = xi - X[j], d1 = yi - Y[j], d2 = zi - Z[j];
+            d1 = wrap(d1, P.box, P.half);
+            d0 = wrap(d0, P.box, P.half);
+            d2 = wrap(d2, P.box, P.half);
+            const R dp[3] = {(R)d0, (R)d1, (R)d2};
+            const R r2 = dp[0] * dp[0] + dp[1] * dp[1] + dp[2] * dp[2];
+            if (r2 <= 0) continue;
+            if ((r2 >= h2i) && (r2 >= J.h * J.h)) continue;
+            const R rr = sqrt(r2);
+            R wk = 0, dwk = 0;
+            if (rr < hi) {
+                const R u = rr * hinv;
+                kernel_main<KERN, -1>(u, hinv3, hinv4, wk, dwk);
+            }
+            if (rr > maxd) maxd = rr;
+            R d[5];
+            d[0] = J.rho - I.rho;
+            d[1] = J.p - I.p;
+            d[2] = J.v0 - I.v0;
+            d[3] = J.v1 - I.v1;
+            d[4] = J.v2 - I.v2;
+            for (int b = 0; b < 5; b++) {
+                if (d[b] < mn[b]) mn[b] = d[b];
+                else if (d[b] > mx[b]) mx[b] = d[b];
+            }
+            if (rr < hi) {
+                for (int kk = 0; kk < 3; kk++) {
+                    const R w = -wk * dp[kk];
+                    for (int b = 0; b < 5; b++) G[kk][b] += w * d[b];
+                }
+            }
+        }
+    }
+    const R t0 = (R)T[0 * n + i], t1 = (R)T[1 * n + i], t2 = (R)T[2 * n + i], t3 = (R)T[3 * n + i], t4 = (R)T[4 * n + i], t5 = (R)T[5 * n + i];
+    R o[5][3];
+    for (int b = 0; b < 5; b++) {
+        const R v0 = G[0][b], v1 = G[1][b], v2 = G[2][b];
+        o[b][0] = t0 * v0 + t3 * v1 + t5 * v2;
+        o[b][1] = t3 * v0 + t1 * v1 + t4 * v2;
+        o[b][2] = t5 * v0 + t4 * v1 + t2 * v2;
+    }
+    if (enabled) {
+        const R stol = R(0.1);
+        const R dmaxv = dmax(hi, maxd);
+        const double cni = CN[i];
+        R alim = R(0.25);
+        if (cni > 100) alim = (R)dmin(0.5, 0.25 + 0.25 * (cni - 100) / 100);
+        slopelimiter<R>(o[0], mx[0], mn[0], alim, hi, R(0), 1, dmaxv, I.rho);
+        slopelimiter<R>(o[1], mx[1], mn[1], alim, hi, stol, 1, dmaxv, I.p);
+        for (int kk = 0; kk < 3; kk++) slopelimiter<R>(o[2 + kk], mx[2 + kk], mn[2 + kk], alim, hi, stol, 0, R(0), R(0));
+    }
+    for (int kk = 0; kk < 3; kk++) {
+        GR[(size_t)kk * n + i] = (double)o[0][kk];
+        GP[(size_t)kk * n + i] = (double)o[1][kk];
+        for (int b = 0; b < 3; b++) GV[(size_t)(3 * b + kk) * n + i] = (double)o[2 + b][kk];
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// force: the pair loop of hydro_force() (src/hydro/hydro_force_loop.cc:423-480), two-sided
+// (each active particle accumulates its own side of every pair), then the writeback
+// (:227-321) and hydro_final_operations_and_cleanup (src/hydro/hydro_toplevel.cc:133-250).
+// ---------------------------------------------------------------------------------------
+template <int KERN, typename R>
+__global__ void __launch_bounds__(256) k_force(DevParams P, Gam<R> G, int nrows, const int *__restrict__ act, ForceView F,
+                                               const Cell<R> *__restrict__ rec, const int *__restrict__ ell, long ld,
+                                               const int *__restrict__ cnt, const uint8_t *__restrict__ active,
+                                               const int *__restrict__ timebin, const double *__restrict__ msv_in,
+                                               int *__restrict__ wakeup, double *__restrict__ ACC, double *__restrict__ DTU,
+                                               double *__restrict__ MSV, int *__restrict__ err)
+{
+    const int r = (int)(xcd_block() * blockDim.x + threadIdx.x);
+    if (r >= nrows) return;
+    const int i = act[r];
+    const long n = F.n;
+    const double mi = F.M[i], rhoi = F.RHO[i];
+    double A0 = 0, A1 = 0, A2 = 0, dE = 0, msv = 0;
+    // effective_soundspeed of i, src/declarations/cell_data.h:703-712 (floor of MaxSignalVel)
+    double sound_i = (rhoi > 0) ? sqrt(P.gamma * F.PR[i] / rhoi) : sqrt(P.gamma * (P.gamma - 1.) * F.U[i]);
+    const bool enabled = (mi > 0 && rhoi > 0);
+    if (enabled) {
+        const Cell<R> I = rec[i];
+        const double xi = F.X[i], yi = F.Y[i], zi = F.Z[i];
+        const int tbi = timebin ? timebin[i] : 0;
+        const int c = cnt[r];
+        R a0 = 0, a1 = 0, a2 = 0, ae = 0, m = 0;
+        for (int q = 0; q < c; q++) {
+            const int j = ell[(size_t)q * ld + r];
+            if (m < I.cs) m = I.cs;
+            const Cell<R> J = rec[j];
+            double d0 = xi - F.X[j], d1 = yi - F.Y[j], d2 = zi - F.Z[j];
+            d1 = wrap(d1, P.box, P.half);
+            d0 = wrap(d0, P.box, P.half);
+            d2 = wrap(d2, P.box, P.half);
+            const R dp0 = (R)d0, dp1 = (R)d1, dp2 = (R)d2;
+            const R r2 = dp0 * dp0 + dp1 * dp1 + dp2 * dp2;
+            R f0, f1, f2, fp, vts;
+            int rc = force_pair<KERN, R>(G, I, J, dp0, dp1, dp2, r2, f0, f1, f2, fp, vts);
+            if (rc < 0) { atomicMax(err, 1234); continue; }
+            if (rc == 0) continue;
+            a0 += f0; a1 += f1; a2 += f2; ae += fp;
+            if (vts > m) m = vts;
+            if (active && !active[j] && wakeup) {
+                if ((double)vts > kWakeup * msv_in[j]) atomicMax(&wakeup[j], tbi + 1);
+            }
+        }
+        A0 = a0; A1 = a1; A2 = a2; dE = ae; msv = m;
+    }
+    // writeback: MaxSignalVel = max(accumulated, c_s) over MIN_REAL_NUMBER (hydro_force_loop.cc:318-321)
+    double ms = kMinReal;
+    double mxs = fmax(msv, sound_i);
+    if (ms < mxs) ms = mxs;
+    // hydro_final_operations_and_cleanup (non-comoving, MFM)
+    double e = dE;
+    if (enabled) {
+        const double inv1 = 1.0 / 1.0;
+        const double vp0 = F.VX[i] * inv1, vp1 = F.VY[i] * inv1, vp2 = F.VZ[i] * inv1;
+        e -= vp0 * A0 + vp1 * A1 + vp2 * A2;
+        const double invm = 1.0 / mi;
+        A0 *= invm; A1 *= invm; A2 *= invm;
+        e /= mi;
+        if (F.H[i] >= 0.99 * P.max_kernel_radius) e = 0;
+    }
+    ACC[i] = A0; ACC[n + i] = A1; ACC[2 * n + i] = A2;
+    DTU[i] = e;
+    MSV[i] = ms;
+}
+
+}  // namespace gz
