@@ -45,6 +45,14 @@ struct gz_context {
     double *GR = nullptr, *GP = nullptr, *GV = nullptr, *ACC = nullptr, *DTU = nullptr, *MSV = nullptr, *MSV_IN = nullptr;
     int *TB = nullptr, *WAKE = nullptr;
     uint8_t *ACT = nullptr;   // active flag per particle (device order)
+    // timestep-loop fields: kicked velocity and energy, softening cache, step start and length
+    double *VKX = nullptr, *VKY = nullptr, *VKZ = nullptr, *UK = nullptr, *SOFT = nullptr;
+    long long *TIB = nullptr, *DTS = nullptr;
+    // positions when the particles were last ordered, and the largest periodic distance any
+    // particle has moved since (the search grid widens its cells by twice that)
+    double *X0 = nullptr, *Y0 = nullptr, *Z0 = nullptr;
+    double dmax = 0, last_rmax = 0;
+    double resort_fraction = 0.125;  // order again once dmax exceeds this fraction of the search radius
     bool all_active = true;
     int *perm = nullptr;      // device order -> caller index
     uint32_t *key = nullptr;  // fine Morton key, device order
@@ -90,8 +98,20 @@ struct gz_context {
     // density iteration state (per slot)
     double *it_h = nullptr, *it_hb = nullptr, *it_l = nullptr, *it_r = nullptr, *it_cn = nullptr;
     int *it_ci = nullptr, *it_it = nullptr, *todo = nullptr, *todo2 = nullptr;
-    int *counters = nullptr;   // device: [0] next count, [1] rebuild flag, [2] error, [3] overflow
+    int *counters = nullptr;   // device: [0] next count, [1] rebuild flag, [2] error, [3] overflow,
+                               // [4] wake-up flagged, [5] woken count, [6] timestep stop code
     int *hcounters = nullptr;  // pinned host mirror
+    unsigned long long *u64c = nullptr;   // device: [0..63] bin histogram, [64] displacement bits, [65] force pairs
+    unsigned long long *hu64c = nullptr;  // pinned host mirror
+
+    // the integer timeline (host) of gz_run_*
+    gz_integ_params ip{};
+    bool run_started = false;
+    long long ti_current = 0, num_steps = 0, n_woken = 0;
+    double time = 0, tbi = 0;
+    unsigned long long mask = 0;
+    long long bin_count[kTimebins] = {};
+    int need_wakeup = 0, stop_code = 0;
 
     // stats and timing
     std::map<std::string, double> stats;
@@ -263,9 +283,17 @@ int alloc_state(gz_context *ctx, long cap)
     CHK(dalloc(ctx, &ctx->dacc, 13 * n));
     CHK(dalloc(ctx, (char **)&ctx->rec_grad, n * sizeof(GradRec<double>)));
     CHK(dalloc(ctx, (char **)&ctx->rec_force, n * sizeof(Cell<double>)));
+    CHK(dalloc(ctx, &ctx->VKX, n)); CHK(dalloc(ctx, &ctx->VKY, n)); CHK(dalloc(ctx, &ctx->VKZ, n));
+    CHK(dalloc(ctx, &ctx->UK, n)); CHK(dalloc(ctx, &ctx->SOFT, n)); CHK(dalloc(ctx, &ctx->TIB, n)); CHK(dalloc(ctx, &ctx->DTS, n));
+    CHK(dalloc(ctx, &ctx->X0, n)); CHK(dalloc(ctx, &ctx->Y0, n)); CHK(dalloc(ctx, &ctx->Z0, n));
     CHK(dalloc(ctx, &ctx->counters, 8));
+    CHK(dalloc(ctx, &ctx->u64c, 80));
     if (!ctx->hcounters) {
         hipError_t e = hipHostMalloc((void **)&ctx->hcounters, 8 * sizeof(int), hipHostMallocDefault);
+        if (e != hipSuccess) return fail(ctx, GZ_E_MEMORY, "hipHostMalloc failed");
+    }
+    if (!ctx->hu64c) {
+        hipError_t e = hipHostMalloc((void **)&ctx->hu64c, 80 * sizeof(unsigned long long), hipHostMallocDefault);
         if (e != hipSuccess) return fail(ctx, GZ_E_MEMORY, "hipHostMalloc failed");
     }
     ctx->cap = cap;
@@ -284,6 +312,10 @@ void free_state(gz_context *c)
     dfree(c->cstart); dfree(c->cend); dfree(c->nsten); dfree(c->sten); dfree(c->chmax);
     dfree(c->ca0); dfree(c->ca1); dfree(c->cells); dfree(c->cflag); dfree(c->cscan); dfree(c->need);
     dfree(c->rec_grad); dfree(c->rec_force); dfree(c->dacc);
+    dfree(c->VKX); dfree(c->VKY); dfree(c->VKZ); dfree(c->UK); dfree(c->SOFT); dfree(c->TIB); dfree(c->DTS);
+    dfree(c->X0); dfree(c->Y0); dfree(c->Z0); dfree(c->u64c);
+    if (c->hu64c) hipHostFree(c->hu64c);
+    c->hu64c = nullptr;
     if (c->tmp) hipFree(c->tmp);
     c->tmp = nullptr;
     c->tmp_bytes = 0;
@@ -310,14 +342,50 @@ int download_field(gz_context *ctx, double *host, int width, const double *src, 
     return 0;
 }
 
+// a host vector field (n x 3, caller order) into three device columns, and back
+int upload_vec3(gz_context *ctx, const double *host, double *dx, double *dy, double *dz)
+{
+    const long n = ctx->n;
+    HIPCHK(hipMemcpyAsync(ctx->stage, host, (size_t)n * 3 * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    k_gather<<<nblk(n), kBlock, 0, ctx->stream>>>(n, 3, ctx->perm, ctx->stage, ctx->stage + 3 * n, 1);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(dx, ctx->stage + 3 * n, n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(dy, ctx->stage + 4 * n, n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(dz, ctx->stage + 5 * n, n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+    return 0;
+}
+
+int download_vec3(gz_context *ctx, double *host, const double *dx, const double *dy, const double *dz)
+{
+    const long n = ctx->n;
+    HIPCHK(hipMemcpyAsync(ctx->stage + 3 * n, dx, n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(ctx->stage + 4 * n, dy, n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(ctx->stage + 5 * n, dz, n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+    k_scatter<<<nblk(n), kBlock, 0, ctx->stream>>>(n, 3, ctx->perm, ctx->stage + 3 * n, ctx->stage, 1);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(host, ctx->stage, (size_t)n * 3 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+int copy_dev(gz_context *ctx, void *dst, const void *src, size_t bytes)
+{
+    HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+    return 0;
+}
+
 // grid of level L over the current device order; the stencil of each cell
 int build_grid(gz_context *ctx, double rmax)
 {
     const double box = ctx->par.box_size;
-    // cell side >= the largest search radius, with a margin for positions that round onto a
-    // cell face (the 27-cell stencil must then still hold every partner)
+    // cell side >= the largest search radius plus twice the distance any particle has moved
+    // since the particles were ordered (cells are those of the ordering), with a margin for
+    // positions that round onto a cell face (the 27-cell stencil must then still hold every
+    // partner)
+    ctx->last_rmax = rmax;
+    const double reach = rmax + 2.0 * ctx->dmax;
     int L = 0;
-    while (L < kFineLevel && box / (double)(1 << (L + 1)) >= rmax * (1.0 + 1e-6)) L++;
+    while (L < kFineLevel && box / (double)(1 << (L + 1)) >= reach * (1.0 + 1e-6)) L++;
     const long ncell = 1L << (3 * L);
     if (ncell > ctx->grid_cells_alloc) {
         CHK(dalloc(ctx, &ctx->cstart, ncell)); CHK(dalloc(ctx, &ctx->cend, ncell)); CHK(dalloc(ctx, &ctx->nsten, ncell));
@@ -503,7 +571,7 @@ int density_stage(gz_context *ctx)
                 ctx->dp, grid_view(ctx), cell_rows(ctx), ctx->act, ctx->need, ctx->X, ctx->Y, ctx->Z, ctx->VX, ctx->VY, ctx->VZ,
                 ctx->M, ctx->it_h, ctx->dacc);
             k_density_after<<<nblk(na), kBlock, 0, ctx->stream>>>(ctx->dp, na, ctx->act, ctx->need, ctx->dacc, ctx->M, ctx->U,
-                                                                  ctx->cell, it, out, ctx->counters + 0, ctx->counters + 1,
+                                                                  ctx->cell - 2.0 * ctx->dmax, it, out, ctx->counters + 0, ctx->counters + 1,
                                                                   ctx->counters + 2);
             HIPCHK(hipGetLastError());
         }
@@ -590,6 +658,7 @@ int force_stage(gz_context *ctx)
     CHK(ensure_sym(ctx));
     HIPCHK(hipMemcpyAsync(ctx->MSV_IN, ctx->MSV, ctx->n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
     HIPCHK(hipMemsetAsync(ctx->counters + 2, 0, sizeof(int), ctx->stream));
+    HIPCHK(hipMemsetAsync(ctx->counters + 4, 0, sizeof(int), ctx->stream));
     ForceView F{ctx->n, ctx->X, ctx->Y, ctx->Z, ctx->VX, ctx->VY, ctx->VZ, ctx->M, ctx->H, ctx->U, ctx->RHO, ctx->PR, ctx->CN,
                 ctx->FCE, ctx->DRK, ctx->GR, ctx->GP, ctx->GV, ctx->T};
     {
@@ -598,10 +667,12 @@ int force_stage(gz_context *ctx)
         k_pack_force<R><<<nblk(ctx->n), kBlock, 0, ctx->stream>>>(ctx->par.gamma, F, rec);
         k_force<KERN, R><<<xcd_grid(nblk(ctx->nact)), kBlock, 0, ctx->stream>>>(
             ctx->dp, gam<R>(ctx->dp), ctx->nact, ctx->act, F, rec, ctx->ell_s, ctx->ld_s, ctx->cnt_s,
-            ctx->all_active ? nullptr : ctx->ACT, ctx->TB, ctx->MSV_IN, ctx->WAKE, ctx->ACC, ctx->DTU, ctx->MSV, ctx->counters + 2);
+            ctx->all_active ? nullptr : ctx->ACT, ctx->TB, ctx->MSV_IN, ctx->WAKE, ctx->ACC, ctx->DTU, ctx->MSV, ctx->counters + 2,
+            ctx->counters + 4, ctx->u64c + 65);
         HIPCHK(hipGetLastError());
     }
     CHK(read_counters(ctx));
+    if (ctx->hcounters[4]) ctx->need_wakeup = 1;
     if (ctx->hcounters[2]) return fail(ctx, GZ_STOP_RIEMANN, "hydro_force(): Riemann solver failed to find a positive pressure");
     return 0;
 }
@@ -617,6 +688,229 @@ int dispatch(gz_context *ctx)
 template <int K, typename R> struct DensityF { static int run(gz_context *c) { return density_stage<K, R>(c); } };
 template <int K, typename R> struct GradF { static int run(gz_context *c) { return gradients_stage<K, R>(c); } };
 template <int K, typename R> struct ForceF { static int run(gz_context *c) { return force_stage<K, R>(c); } };
+
+// ---------------------------------------------------------------------------------------
+// re-ordering: particles that moved are put back in Morton order (keys of the current
+// positions), every per-particle field follows, and the displacement bound restarts at zero
+// ---------------------------------------------------------------------------------------
+int permute_doubles(gz_context *ctx, const int *order, double *field, int width)
+{
+    const long n = ctx->n;
+    k_permute_cols<<<nblk(n), kBlock, 0, ctx->stream>>>(n, width, order, (const unsigned long long *)field,
+                                                        (unsigned long long *)ctx->stage);
+    HIPCHK(hipGetLastError());
+    return copy_dev(ctx, field, ctx->stage, (size_t)n * width * sizeof(double));
+}
+
+int resort(gz_context *ctx)
+{
+    const long n = ctx->n;
+    Timed t(ctx, "sort");
+    k_morton_soa<<<nblk(n), kBlock, 0, ctx->stream>>>(n, ctx->X, ctx->Y, ctx->Z, ctx->par.box_size, ctx->key2, ctx->ival);
+    HIPCHK(hipGetLastError());
+    size_t bytes = 0;
+    HIPCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, ctx->key2, ctx->key, ctx->ival, ctx->ival2, (int)n, 0, 3 * kFineLevel, ctx->stream));
+    CHK(ensure_tmp(ctx, bytes));
+    HIPCHK(hipcub::DeviceRadixSort::SortPairs(ctx->tmp, bytes, ctx->key2, ctx->key, ctx->ival, ctx->ival2, (int)n, 0, 3 * kFineLevel, ctx->stream));
+    const int *order = ctx->ival2;
+    struct F { double *p; int w; } fields[] = {
+        {ctx->X, 1}, {ctx->Y, 1}, {ctx->Z, 1}, {ctx->VX, 1}, {ctx->VY, 1}, {ctx->VZ, 1}, {ctx->VKX, 1}, {ctx->VKY, 1},
+        {ctx->VKZ, 1}, {ctx->M, 1}, {ctx->H, 1}, {ctx->U, 1}, {ctx->UK, 1}, {ctx->NN, 1}, {ctx->DRK, 1}, {ctx->DIVV, 1},
+        {ctx->RHO, 1}, {ctx->PR, 1}, {ctx->T, 6}, {ctx->FW, 3}, {ctx->CN, 1}, {ctx->FCE, 1}, {ctx->GR, 3}, {ctx->GP, 3},
+        {ctx->GV, 9}, {ctx->ACC, 3}, {ctx->DTU, 1}, {ctx->MSV, 1}, {ctx->SOFT, 1},
+        {(double *)ctx->TIB, 1}, {(double *)ctx->DTS, 1}};
+    for (auto &f : fields) CHK(permute_doubles(ctx, order, f.p, f.w));
+    int *ints[3] = {ctx->TB, ctx->WAKE, ctx->perm};
+    for (int *a : ints) {
+        k_permute_int<<<nblk(n), kBlock, 0, ctx->stream>>>(n, order, a, ctx->ival);
+        HIPCHK(hipGetLastError());
+        CHK(copy_dev(ctx, a, ctx->ival, n * sizeof(int)));
+    }
+    k_permute_u8<<<nblk(n), kBlock, 0, ctx->stream>>>(n, order, ctx->ACT, (uint8_t *)ctx->stage);
+    HIPCHK(hipGetLastError());
+    CHK(copy_dev(ctx, ctx->ACT, ctx->stage, n));
+    CHK(copy_dev(ctx, ctx->X0, ctx->X, n * 8)); CHK(copy_dev(ctx, ctx->Y0, ctx->Y, n * 8)); CHK(copy_dev(ctx, ctx->Z0, ctx->Z, n * 8));
+    ctx->dmax = 0;
+    ctx->sym_valid = false;
+    ctx->rows_level = -1;
+    ctx->stats["resorts"] += 1;
+    return 0;
+}
+
+// ---------------------------------------------------------------------------------------
+// the timestep loop (host side of the integer timeline)
+// ---------------------------------------------------------------------------------------
+constexpr long long kTimebaseHost = 1LL << kTimebins;
+
+TimeParams time_params(gz_context *ctx)
+{
+    TimeParams T;
+    T.courant_fac = ctx->ip.courant_fac;
+    T.err_tol = ctx->ip.err_tol_int_accuracy;
+    T.max_dt = ctx->ip.max_size_timestep;
+    T.min_dt = ctx->ip.min_size_timestep;
+    T.min_egy = ctx->ip.min_egy_spec;
+    T.core_size = ctx->par.kernel == 3 ? (1.0 / 2.0) : (2.0 / 5.0);  // KERNEL_CORE_SIZE, src/mesh/kernel.h:59,113
+    T.tbi = ctx->tbi;
+    T.ti_current = ctx->ti_current;
+    T.active_mask = ctx->mask;
+    T.min_h = ctx->par.min_kernel_radius;
+    T.max_h = ctx->par.max_kernel_radius;
+    T.gamma = ctx->par.gamma;
+    T.box = ctx->par.box_size;
+    return T;
+}
+
+ParticleView particle_view(gz_context *c)
+{
+    ParticleView S;
+    S.n = c->n;
+    S.X = c->X; S.Y = c->Y; S.Z = c->Z; S.VX = c->VX; S.VY = c->VY; S.VZ = c->VZ; S.VKX = c->VKX; S.VKY = c->VKY; S.VKZ = c->VKZ;
+    S.M = c->M; S.H = c->H; S.U = c->U; S.UK = c->UK; S.NN = c->NN; S.DIVV = c->DIVV; S.RHO = c->RHO; S.PR = c->PR;
+    S.ACC = c->ACC; S.DTU = c->DTU; S.MSV = c->MSV; S.SOFT = c->SOFT;
+    S.TB = c->TB; S.WAKE = c->WAKE; S.TIB = c->TIB; S.DTS = c->DTS;
+    return S;
+}
+
+int read_bins(gz_context *ctx)
+{
+    const long n = ctx->n;
+    HIPCHK(hipMemsetAsync(ctx->u64c, 0, 64 * sizeof(unsigned long long), ctx->stream));
+    k_bin_hist<<<std::min(nblk(n), 1024u), kBlock, 0, ctx->stream>>>(n, ctx->TB, ctx->u64c);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(ctx->hu64c, ctx->u64c, 64 * sizeof(unsigned long long), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    for (int b = 0; b < kTimebins; b++) ctx->bin_count[b] = (long long)ctx->hu64c[b];
+    return 0;
+}
+
+// the earliest next kick over the occupied bins (src/core/run.cc:870-890, src/core/timestep.cc:1577-1591)
+long long next_kick(const gz_context *ctx)
+{
+    long long ti_next_kick = kTimebaseHost;
+    for (int b = 0; b < kTimebins; b++) {
+        if (!ctx->bin_count[b]) continue;
+        long long t;
+        if (b > 0) {
+            const long long dt_bin = 1LL << b;
+            t = (ctx->ti_current / dt_bin) * dt_bin + dt_bin;
+        } else {
+            t = ctx->ti_current;
+        }
+        if (t < ti_next_kick) ti_next_kick = t;
+    }
+    return ti_next_kick;
+}
+
+// the active list of the current bins (make_list_of_active_particles, src/core/run.cc:997-1009)
+int active_from_bins(gz_context *ctx)
+{
+    const long n = ctx->n;
+    k_active_from_bins<<<nblk(n), kBlock, 0, ctx->stream>>>(n, ctx->mask, ctx->TB, ctx->M, ctx->H, ctx->ACT, ctx->SOFT);
+    HIPCHK(hipGetLastError());
+    ctx->all_active = false;
+    ctx->sym_valid = false;
+    return build_active(ctx);
+}
+
+int hydro_pass(gz_context *ctx)
+{
+    ctx->dp.time = ctx->time;
+    ctx->par.time = ctx->time;
+    CHK(dispatch<DensityF>(ctx));
+    CHK(dispatch<GradF>(ctx));
+    CHK(dispatch<ForceF>(ctx));
+    ctx->stats["active_updates"] += ctx->nact;
+    return 0;
+}
+
+int run_step(gz_context *ctx)
+{
+    const long n = ctx->n;
+    TimeParams T = time_params(ctx);
+    ParticleView S = particle_view(ctx);
+    HIPCHK(hipMemsetAsync(ctx->counters, 0, 8 * sizeof(int), ctx->stream));
+    // find_timesteps: the particles of the active list of the last sync point
+    {
+        Timed t(ctx, "timeline");
+        if (ctx->nact > 0) {
+            k_find_timesteps<<<nblk(ctx->nact), kBlock, 0, ctx->stream>>>(T, S, ctx->nact, ctx->act, ctx->counters + 6);
+            HIPCHK(hipGetLastError());
+        }
+    }
+    CHK(read_bins(ctx));
+    // process_wake_ups (src/core/timestep.cc:1565-1774)
+    ctx->n_woken = 0;
+    {
+        const long long ti_next = next_kick(ctx);
+        int max_time_bin_active = 0;
+        for (int b = 0; b < kTimebins; b++)
+            if ((ti_next % (1LL << b)) == 0) max_time_bin_active = b;
+        int wakeup_bin_offset = 0;
+        while ((1LL << wakeup_bin_offset) < (long long)kWakeup) wakeup_bin_offset++;
+        if (ctx->need_wakeup) {
+            int lowest = kTimebins;
+            for (int b = 0; b < kTimebins; b++)
+                if (((ctx->mask >> b) & 1ull) && ctx->bin_count[b] > 0) { lowest = b; break; }
+            {
+                Timed t(ctx, "timeline");
+                k_wakeups<<<nblk(n), kBlock, 0, ctx->stream>>>(T, S, lowest, max_time_bin_active, wakeup_bin_offset, ctx->counters + 5);
+                HIPCHK(hipGetLastError());
+            }
+            CHK(read_counters(ctx));
+            ctx->n_woken = ctx->hcounters[5];
+            if (ctx->n_woken) CHK(read_bins(ctx));
+        }
+        ctx->need_wakeup = 0;
+    }
+    // do_first_halfstep_kick over the same active list
+    if (ctx->nact > 0) {
+        Timed t(ctx, "timeline");
+        k_halfstep_kick<<<nblk(ctx->nact), kBlock, 0, ctx->stream>>>(T, S, ctx->nact, ctx->act, 0);
+        HIPCHK(hipGetLastError());
+    }
+    // find_next_sync_point_and_drift (src/core/run.cc:857-994)
+    const long long ti_prev = ctx->ti_current;
+    const long long ti_next = next_kick(ctx);
+    ctx->num_steps++;
+    ctx->ti_current = ti_next;
+    ctx->time = ctx->ip.time_begin + ctx->ti_current * ctx->tbi;
+    ctx->mask = 1ull;
+    for (int b = 1; b < kTimebins; b++)
+        if ((ti_next % (1LL << b)) == 0) ctx->mask |= 1ull << b;
+    T = time_params(ctx);
+    HIPCHK(hipMemsetAsync(ctx->u64c + 64, 0, sizeof(unsigned long long), ctx->stream));
+    {
+        Timed t(ctx, "timeline");
+        k_drift<<<nblk(n), kBlock, 0, ctx->stream>>>(T, S, ti_prev, ti_next, ctx->X0, ctx->Y0, ctx->Z0, ctx->u64c + 64);
+        HIPCHK(hipGetLastError());
+    }
+    HIPCHK(hipMemcpyAsync(ctx->hu64c + 64, ctx->u64c + 64, sizeof(unsigned long long), hipMemcpyDeviceToHost, ctx->stream));
+    // make_list_of_active_particles and the softening cache of the active particles
+    CHK(active_from_bins(ctx));
+    {
+        unsigned long long bits = ctx->hu64c[64];
+        double d;
+        memcpy(&d, &bits, sizeof d);
+        ctx->dmax = d;
+    }
+    // particles that moved far from where they were ordered are ordered again
+    if (ctx->last_rmax > 0 && ctx->dmax > ctx->resort_fraction * ctx->last_rmax) {
+        CHK(resort(ctx));
+        CHK(build_active(ctx));
+    }
+    // compute_hydro_densities_and_forces, then do_second_halfstep_kick
+    CHK(hydro_pass(ctx));
+    if (ctx->nact > 0) {
+        Timed t(ctx, "timeline");
+        k_halfstep_kick<<<nblk(ctx->nact), kBlock, 0, ctx->stream>>>(T, S, ctx->nact, ctx->act, 1);
+        HIPCHK(hipGetLastError());
+    }
+    CHK(read_counters(ctx));
+    if (ctx->hcounters[6] && !ctx->stop_code) ctx->stop_code = ctx->hcounters[6];
+    if (ctx->stop_code) return fail(ctx, ctx->stop_code, "timestep loop: controlled stop %d (step beyond the end of the timeline)", ctx->stop_code);
+    return 0;
+}
 
 int set_device(gz_context *ctx)
 {
@@ -721,6 +1015,25 @@ int gz_load(gz_context *ctx, long n, const gz_fields *f)
             HIPCHK(hipMemsetAsync(td[q], 0, n * sizeof(int), ctx->stream));
         }
     }
+    // timestep-loop fields; absent ones start from the predicted fields (as init() sets them)
+    if (f->vel_kick) { CHK(upload_vec3(ctx, f->vel_kick, ctx->VKX, ctx->VKY, ctx->VKZ)); }
+    else { CHK(copy_dev(ctx, ctx->VKX, ctx->VX, n * 8)); CHK(copy_dev(ctx, ctx->VKY, ctx->VY, n * 8)); CHK(copy_dev(ctx, ctx->VKZ, ctx->VZ, n * 8)); }
+    if (f->u_kick) { CHK(upload_field(ctx, f->u_kick, 1, ctx->UK, false)); } else { CHK(copy_dev(ctx, ctx->UK, ctx->U, n * 8)); }
+    if (f->soft) { CHK(upload_field(ctx, f->soft, 1, ctx->SOFT, false)); } else { CHK(copy_dev(ctx, ctx->SOFT, ctx->H, n * 8)); }
+    long long *tl[2] = {f->ti_begstep, f->dt_step};
+    long long *tdst[2] = {ctx->TIB, ctx->DTS};
+    for (int q = 0; q < 2; q++) {
+        if (tl[q]) {
+            HIPCHK(hipMemcpyAsync(ctx->stage, tl[q], n * sizeof(long long), hipMemcpyHostToDevice, ctx->stream));
+            k_gather_ll<<<nblk(n), kBlock, 0, ctx->stream>>>(n, ctx->perm, (const long long *)ctx->stage, tdst[q]);
+            HIPCHK(hipGetLastError());
+        } else {
+            HIPCHK(hipMemsetAsync(tdst[q], 0, n * sizeof(long long), ctx->stream));
+        }
+    }
+    CHK(copy_dev(ctx, ctx->X0, ctx->X, n * 8)); CHK(copy_dev(ctx, ctx->Y0, ctx->Y, n * 8)); CHK(copy_dev(ctx, ctx->Z0, ctx->Z, n * 8));
+    ctx->dmax = 0;
+    ctx->run_started = false;
     ctx->all_active = true;
     // a caller that hands over the state density() leaves (or also the gradients) may start at
     // the later stage, as the reference's inactive particles keep that state between steps
@@ -770,6 +1083,22 @@ int gz_store(gz_context *ctx, gz_fields *f)
         if (!ti[q]) continue;
         k_scatter_int<<<nblk(n), kBlock, 0, ctx->stream>>>(n, ctx->perm, td[q], ctx->ival2);
         HIPCHK(hipMemcpyAsync(ti[q], ctx->ival2, n * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+    }
+    if (f->vel_kick) CHK(download_vec3(ctx, f->vel_kick, ctx->VKX, ctx->VKY, ctx->VKZ));
+    struct Opt2 { double *dst; const double *src; } o2[] = {{f->u_kick, ctx->UK}, {f->soft, ctx->SOFT}};
+    for (auto &o : o2) {
+        if (!o.dst) continue;
+        CHK(download_field(ctx, o.dst, 1, o.src, false));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+    }
+    long long *tl[2] = {f->ti_begstep, f->dt_step};
+    const long long *tsrc[2] = {ctx->TIB, ctx->DTS};
+    for (int q = 0; q < 2; q++) {
+        if (!tl[q]) continue;
+        k_scatter_ll<<<nblk(n), kBlock, 0, ctx->stream>>>(n, ctx->perm, tsrc[q], (long long *)ctx->stage);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(tl[q], ctx->stage, n * sizeof(long long), hipMemcpyDeviceToHost, ctx->stream));
         HIPCHK(hipStreamSynchronize(ctx->stream));
     }
     return 0;
@@ -853,8 +1182,19 @@ int gz_stat(gz_context *ctx, const char *name, double *value)
         *value = s;
         return 0;
     }
+    if (k == "force_pairs_total") {
+        unsigned long long v = 0;
+        HIPCHK(hipMemcpy(&v, ctx->u64c + 65, sizeof v, hipMemcpyDeviceToHost));
+        *value = (double)v;
+        return 0;
+    }
+    static const char *known[] = {"density_iterations", "ngb_rebuilds", "n_active", "ell_widen", "active_updates", "resorts"};
     auto it = ctx->stats.find(k);
-    if (it == ctx->stats.end()) return fail(ctx, GZ_E_ARG, "unknown stat %s", name);
+    if (it == ctx->stats.end()) {
+        for (const char *kn : known)
+            if (k == kn) { *value = 0; return 0; }
+        return fail(ctx, GZ_E_ARG, "unknown stat %s", name);
+    }
     *value = it->second;
     return 0;
 }
@@ -871,6 +1211,10 @@ int gz_timing_reset(gz_context *ctx)
     if (!ctx) return GZ_E_ARG;
     collect_timings(ctx);
     ctx->timings.clear();
+    ctx->stats["active_updates"] = 0;
+    ctx->stats["resorts"] = 0;
+    HIPCHK(hipMemsetAsync(ctx->u64c + 65, 0, sizeof(unsigned long long), ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
     return 0;
 }
 
@@ -881,6 +1225,85 @@ int gz_timing(gz_context *ctx, const char *name, long *launches, double *total_m
     auto it = ctx->timings.find(name);
     if (launches) *launches = it == ctx->timings.end() ? 0 : it->second.first;
     if (total_ms) *total_ms = it == ctx->timings.end() ? 0 : it->second.second;
+    return 0;
+}
+
+int gz_run_begin(gz_context *ctx, const gz_integ_params *ip)
+{
+    if (!ctx || !ip) return GZ_E_ARG;
+    if (ctx->n <= 0) return fail(ctx, GZ_E_STATE, "nothing loaded");
+    if (!(ip->time_max > ip->time_begin)) return fail(ctx, GZ_E_ARG, "time_max must exceed time_begin");
+    if (!(ip->courant_fac > 0) || !(ip->err_tol_int_accuracy > 0) || !(ip->max_size_timestep > 0) || ip->min_size_timestep < 0 ||
+        ip->min_egy_spec < 0)
+        return fail(ctx, GZ_E_ARG, "courant_fac, err_tol_int_accuracy, max_size_timestep must be > 0; min_size_timestep, min_egy_spec >= 0");
+    CHK(set_device(ctx));
+    const long n = ctx->n;
+    ctx->ip = *ip;
+    ctx->tbi = (ip->time_max - ip->time_begin) / (double)kTimebaseHost;  // src/core/init.cc:113
+    ctx->ti_current = 0;
+    ctx->time = ip->time_begin;
+    ctx->num_steps = 0;
+    ctx->n_woken = 0;
+    ctx->need_wakeup = 0;
+    ctx->stop_code = 0;
+    ctx->mask = (1ull << kTimebins) - 1;  // every bin active at the start (src/core/init.cc:583)
+    ctx->dp.time_begin = ip->time_begin;
+    ctx->par.time_begin = ip->time_begin;
+    // predicted = current, every particle in bin 0 at the start of the timeline
+    CHK(copy_dev(ctx, ctx->VX, ctx->VKX, n * 8)); CHK(copy_dev(ctx, ctx->VY, ctx->VKY, n * 8)); CHK(copy_dev(ctx, ctx->VZ, ctx->VKZ, n * 8));
+    CHK(copy_dev(ctx, ctx->U, ctx->UK, n * 8));
+    HIPCHK(hipMemsetAsync(ctx->TB, 0, n * sizeof(int), ctx->stream));
+    HIPCHK(hipMemsetAsync(ctx->WAKE, 0, n * sizeof(int), ctx->stream));
+    HIPCHK(hipMemsetAsync(ctx->TIB, 0, n * sizeof(long long), ctx->stream));
+    HIPCHK(hipMemsetAsync(ctx->DTS, 0, n * sizeof(long long), ctx->stream));
+    CHK(active_from_bins(ctx));
+    ctx->dp.time = ctx->time;
+    ctx->par.time = ctx->time;
+    int rc = dispatch<DensityF>(ctx);                    // init(): density()
+    if (!rc) rc = copy_dev(ctx, ctx->SOFT, ctx->H, n * 8);  // compute_all_force_softening(1)
+    if (!rc) rc = hydro_pass(ctx);                       // run(): compute_hydro_densities_and_forces()
+    if (!rc) rc = read_bins(ctx);
+    collect_timings(ctx);
+    if (rc) return rc;
+    ctx->run_started = true;
+    return 0;
+}
+
+int gz_run_step(gz_context *ctx)
+{
+    if (!ctx) return GZ_E_ARG;
+    if (!ctx->run_started) return fail(ctx, GZ_E_STATE, "gz_run_step before gz_run_begin");
+    CHK(set_device(ctx));
+    int rc = run_step(ctx);
+    collect_timings(ctx);
+    return rc;
+}
+
+int gz_set_option(gz_context *ctx, const char *name, double value)
+{
+    if (!ctx || !name) return GZ_E_ARG;
+    std::string k(name);
+    if (k == "resort_fraction") {
+        if (!(value > 0)) return fail(ctx, GZ_E_ARG, "resort_fraction must be > 0");
+        ctx->resort_fraction = value;
+        return 0;
+    }
+    return fail(ctx, GZ_E_ARG, "unknown option %s", name);
+}
+
+int gz_timeline_get(gz_context *ctx, gz_timeline *out)
+{
+    if (!ctx || !out) return GZ_E_ARG;
+    memset(out, 0, sizeof *out);
+    out->ti_current = ctx->ti_current;
+    out->time = ctx->time;
+    out->timebase_interval = ctx->tbi;
+    out->num_steps = ctx->num_steps;
+    out->n_active = ctx->nact;
+    out->n_woken = ctx->n_woken;
+    for (int b = 0; b < kTimebins; b++) out->bin_count[b] = ctx->bin_count[b];
+    out->bin_active = ctx->mask;
+    out->stop_code = ctx->stop_code;
     return 0;
 }
 

@@ -49,7 +49,42 @@ struct ForceView {            // per particle, device order; vectors SoA with co
     const double *X, *Y, *Z, *VX, *VY, *VZ, *M, *H, *U, *RHO, *PR, *CN, *FCE, *DRK, *GR, *GP, *GV, *T;
 };
 
+// the timestep loop: parameters of one step and the particle fields it reads and writes
+constexpr int kTimebins = 60;  // TIMEBINS, src/declarations/typedefs.h:8
+
+struct TimeParams {
+    double courant_fac, err_tol, max_dt, min_dt, min_egy;  // All.CourantFac ... All.MinEgySpec
+    double core_size;                                      // KERNEL_CORE_SIZE
+    double tbi;                                            // All.Timebase_interval
+    long long ti_current;                                  // All.Ti_Current
+    unsigned long long active_mask;                        // TimeBinActive, bit per bin
+    double min_h, max_h, gamma, box;
+};
+
+struct ParticleView {         // device order; vectors SoA with column stride n
+    long n;
+    double *X, *Y, *Z, *VX, *VY, *VZ, *VKX, *VKY, *VKZ, *M, *H, *U, *UK, *NN, *DIVV, *RHO, *PR, *ACC, *DTU, *MSV, *SOFT;
+    int *TB, *WAKE;
+    long long *TIB, *DTS;
+};
+
+__global__ void k_find_timesteps(TimeParams T, ParticleView S, int nrows, const int *act, int *err);
+__global__ void k_bin_hist(long n, const int *TB, unsigned long long *count);
+__global__ void k_wakeups(TimeParams T, ParticleView S, int lowest_occupied_active_bin, int max_time_bin_active,
+                          int wakeup_bin_offset, int *nwoken);
+__global__ void k_halfstep_kick(TimeParams T, ParticleView S, int nrows, const int *act, int second);
+__global__ void k_drift(TimeParams T, ParticleView S, long long time0, long long time1, const double *X0, const double *Y0,
+                        const double *Z0, unsigned long long *disp);
+__global__ void k_active_from_bins(long n, unsigned long long mask, const int *TB, const double *M, const double *H,
+                                   uint8_t *ACT, double *SOFT);
+__global__ void k_gather_ll(long n, const int *perm, const long long *in, long long *out);
+__global__ void k_scatter_ll(long n, const int *perm, const long long *in, long long *out);
+
 __global__ void k_morton(long n, const double *pos, double box, uint32_t *key, int *val);
+__global__ void k_morton_soa(long n, const double *X, const double *Y, const double *Z, double box, uint32_t *key, int *val);
+__global__ void k_permute_cols(long n, int width, const int *order, const unsigned long long *in, unsigned long long *out);
+__global__ void k_permute_int(long n, const int *order, const int *in, int *out);
+__global__ void k_permute_u8(long n, const int *order, const uint8_t *in, uint8_t *out);
 __global__ void k_gather(long n, int width, const int *perm, const double *in, double *out, int soa);
 __global__ void k_gather_int(long n, const int *perm, const int *in, int *out);
 __global__ void k_gather_u8(long n, const int *perm, const uint8_t *in, uint8_t *out);
@@ -100,6 +135,6 @@ __global__ void k_gradients(DevParams P, int nrows, const int *act, const double
 template <int KERN, typename R>
 __global__ void k_force(DevParams P, Gam<R> G, int nrows, const int *act, ForceView F, const Cell<R> *rec, const int *ell,
                         long ld, const int *cnt, const uint8_t *active, const int *timebin, const double *msv_in, int *wakeup,
-                        double *ACC, double *DTU, double *MSV, int *err);
+                        double *ACC, double *DTU, double *MSV, int *err, int *need_wakeup, unsigned long long *pairs);
 
 }  // namespace gz

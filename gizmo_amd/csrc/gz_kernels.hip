@@ -2,6 +2,7 @@
 // -ffp-contract=off and correctly rounded division/sqrt, the oracle's rounding order).
 #define GZ_DEFINE_COMMON_KERNELS
 #include "gz_kernels_impl.h"
+#include "gz_time_impl.h"
 
 namespace gz {
 
@@ -26,7 +27,7 @@ template __global__ void k_build_ell<true>(int, const int *, const int *, const 
                                                   const int *, double *, double *, double *);                                  \
     template __global__ void k_force<KERN, R>(DevParams, Gam<R>, int, const int *, ForceView, const Cell<R> *, const int *,     \
                                               long, const int *, const uint8_t *, const int *, const double *, int *, double *, \
-                                              double *, double *, int *);
+                                              double *, double *, int *, int *, unsigned long long *);
 GZ_INST(3, double)
 GZ_INST(6, double)
 template __global__ void k_pack_grad<double>(long, const double *, const double *, const double *, const double *, const double *,

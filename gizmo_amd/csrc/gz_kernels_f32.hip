@@ -23,7 +23,7 @@ namespace gz {
                                                   const int *, double *, double *, double *);                                  \
     template __global__ void k_force<KERN, R>(DevParams, Gam<R>, int, const int *, ForceView, const Cell<R> *, const int *,     \
                                               long, const int *, const uint8_t *, const int *, const double *, int *, double *, \
-                                              double *, double *, int *);
+                                              double *, double *, int *, int *, unsigned long long *);
 GZ_INST(3, float)
 GZ_INST(6, float)
 template __global__ void k_pack_grad<float>(long, const double *, const double *, const double *, const double *, const double *,

@@ -57,6 +57,57 @@ __global__ void k_morton(long n, const double *__restrict__ pos, double box, uin
 #endif
 
 
+#ifdef GZ_DEFINE_COMMON_KERNELS
+__global__ void k_morton_soa(long n, const double *__restrict__ X, const double *__restrict__ Y, const double *__restrict__ Z,
+                             double box, uint32_t *__restrict__ key, int *__restrict__ val)
+{
+    long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t cx = fine_coord(X[i], box), cy = fine_coord(Y[i], box), cz = fine_coord(Z[i], box);
+    key[i] = (spread10(cx) << 2) | (spread10(cy) << 1) | spread10(cz);
+    val[i] = (int)i;
+}
+
+// reorder width columns of 8-byte values (SoA, column stride n): out[c*n + s] = in[c*n + order[s]]
+__global__ void k_permute_cols(long n, int width, const int *__restrict__ order, const unsigned long long *__restrict__ in,
+                               unsigned long long *__restrict__ out)
+{
+    long s = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const long u = order[s];
+    for (int c = 0; c < width; c++) out[(size_t)c * n + s] = in[(size_t)c * n + u];
+}
+
+__global__ void k_permute_int(long n, const int *__restrict__ order, const int *__restrict__ in, int *__restrict__ out)
+{
+    long s = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    out[s] = in[order[s]];
+}
+
+__global__ void k_permute_u8(long n, const int *__restrict__ order, const uint8_t *__restrict__ in, uint8_t *__restrict__ out)
+{
+    long s = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    out[s] = in[order[s]];
+}
+
+__global__ void k_gather_ll(long n, const int *__restrict__ perm, const long long *__restrict__ in, long long *__restrict__ out)
+{
+    long s = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    out[s] = in[perm[s]];
+}
+
+__global__ void k_scatter_ll(long n, const int *__restrict__ perm, const long long *__restrict__ in, long long *__restrict__ out)
+{
+    long s = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    out[perm[s]] = in[s];
+}
+#endif
+
+
 // sorted field s <- user field perm[s] (width doubles per particle; interleaved in, SoA out
 // when soa != 0 with column stride n)
 #ifdef GZ_DEFINE_COMMON_KERNELS
@@ -817,12 +868,22 @@ __global__ void __launch_bounds__(256) k_force(DevParams P, Gam<R> G, int nrows,
                                                const int *__restrict__ cnt, const uint8_t *__restrict__ active,
                                                const int *__restrict__ timebin, const double *__restrict__ msv_in,
                                                int *__restrict__ wakeup, double *__restrict__ ACC, double *__restrict__ DTU,
-                                               double *__restrict__ MSV, int *__restrict__ err)
+                                               double *__restrict__ MSV, int *__restrict__ err, int *__restrict__ need_wakeup,
+                                               unsigned long long *__restrict__ pairs)
 {
     const int r = (int)(xcd_block() * blockDim.x + threadIdx.x);
+    // pairs of this launch (wave sum, one atomic per wave)
+    {
+        unsigned long long c = (r < nrows) ? (unsigned long long)cnt[r] : 0ull;
+        for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
+        if ((threadIdx.x & 63) == 0 && c) atomicAdd(pairs, c);
+    }
     if (r >= nrows) return;
     const int i = act[r];
     const long n = F.n;
+    // hydro_force_initial_operations_preloop (src/hydro/hydro_toplevel.cc:457-565): an active
+    // particle's own flag is cleared; only inactive neighbours are flagged below
+    if (wakeup) wakeup[i] = 0;
     const double mi = F.M[i], rhoi = F.RHO[i];
     double A0 = 0, A1 = 0, A2 = 0, dE = 0, msv = 0;
     // effective_soundspeed of i, src/declarations/cell_data.h:703-712 (floor of MaxSignalVel)
@@ -851,7 +912,7 @@ __global__ void __launch_bounds__(256) k_force(DevParams P, Gam<R> G, int nrows,
             a0 += f0; a1 += f1; a2 += f2; ae += fp;
             if (vts > m) m = vts;
             if (active && !active[j] && wakeup) {
-                if ((double)vts > kWakeup * msv_in[j]) atomicMax(&wakeup[j], tbi + 1);
+                if ((double)vts > kWakeup * msv_in[j]) { atomicMax(&wakeup[j], tbi + 1); *need_wakeup = 1; }
             }
         }
         A0 = a0; A1 = a1; A2 = a2; dE = ae; msv = m;

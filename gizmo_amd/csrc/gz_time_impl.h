@@ -1,0 +1,270 @@
+// gz_time_impl.h -- the timestep loop around the hot path on the device (FP64; compiled into
+// the strict-FP translation unit).  One lane per particle: new power-of-two steps for the
+// active particles, the wake-up pass, the two half-step kicks and the drift to the next sync
+// point, each restating the reference function named on it for a gas-only, gravity-free,
+// non-comoving MFM run.  The host keeps the integer timeline and the bin occupation (60 counts,
+// read back once per step) and passes the active bins as a bit mask.
+#pragma once
+#include "gz_kernels.h"
+
+namespace gz {
+
+constexpr long long kTimebase = 1LL << kTimebins;  // TIMEBASE, src/declarations/typedefs.h:8-9
+
+// GET_INTEGERTIME_FROM_TIMEBIN, src/declarations/macros.h:7
+__device__ __forceinline__ long long bin_len(int bin) { return bin ? (1LL << bin) : 0LL; }
+__device__ __forceinline__ bool bin_on(unsigned long long mask, int bin) { return (mask >> bin) & 1ull; }
+
+// get_timestep_bin, src/core/timestep.cc:1525-1546
+__device__ __forceinline__ int timestep_bin(long long ti_step)
+{
+    int bin = -1;
+    if (ti_step == 0) return 0;
+    if (ti_step == 1) return 0;
+    while (ti_step) {
+        bin++;
+        ti_step >>= 1;
+    }
+    return bin;
+}
+
+// internal_energy_positivity_catch without a material floor, src/core/proto.h:417-421
+__device__ __forceinline__ double positivity(double u_old, double u_new) { return (u_new < 0.5 * u_old) ? 0.5 * u_old : u_new; }
+
+// kernel_radius_drift_log_change, src/core/timestep_functions.h:312,322-328
+__device__ __forceinline__ double drift_log_change(double x)
+{
+    if (x > +0.3) return +0.3;
+    if (x < -0.3) return -0.3;
+    return x;
+}
+
+// get_timestep, src/core/timestep.cc:357-1423 for a gas particle here: acceleration criterion
+// with the kernel radius as softening (:417-440, src/gravity/forcetree.cc:85-91), Courant on
+// MaxSignalVel (:711-728), velocity divergence (:1092-1098), caps and integer step (:1360-1417)
+__device__ long long get_timestep(const TimeParams &T, const ParticleView &S, long i)
+{
+    const long n = S.n;
+    double ax = 0.0, ay = 0.0, az = 0.0;
+    ax += S.ACC[i];
+    ay += S.ACC[n + i];
+    az += S.ACC[2 * n + i];
+    double ac = sqrt(ax * ax + ay * ay + az * az);
+    if (ac == 0) ac = 1.0e-30;
+    double dt;
+    {
+        double h_for_accel_dt = T.core_size * S.SOFT[i];
+        if (h_for_accel_dt <= 0) h_for_accel_dt = 1.61199 * S.H[i] / S.NN[i] * 1.0;
+        dt = sqrt(2 * T.err_tol * 1.0 * h_for_accel_dt / ac);
+    }
+    if (S.M[i] > 0) {
+        const double csnd = 0.5 * S.MSV[i];
+        const double L_particle = 1.61199 * S.H[i] / S.NN[i];
+        const double dt_courant = T.courant_fac * (L_particle * 1.0) / csnd;
+        if (dt_courant < dt) dt = dt_courant;
+        const double divVel = S.DIVV[i];
+        if (divVel != 0) {
+            const double dt_divv = 1.5 / fabs(1.0 * divVel);
+            if (dt_divv < dt) dt = dt_divv;
+        }
+    }
+    dt *= 1.0;
+    if (dt >= T.max_dt) dt = T.max_dt;
+    if (dt >= T.max_dt) dt = T.max_dt;
+    if ((dt < T.min_dt) || !(dt / T.tbi >= 2.0)) dt = T.min_dt;
+    if (dt > 0.5 * kTimebase * T.tbi) dt = 0.5 * kTimebase * T.tbi;
+    long long ti_step = (long long)(dt / T.tbi);
+    if (ti_step <= 1) ti_step = 2;
+    if (!(ti_step > 0 && ti_step < kTimebase)) ti_step = 2;
+    return ti_step;
+}
+
+// do_the_kick, src/core/kicks.cc:284-557, MFM gas without gravity (the energy/entropy switch
+// cannot fire without gravitational work, :356-376)
+__device__ void kick(const TimeParams &T, const ParticleView &S, long i, long long tstart, long long tend, int mode)
+{
+    if (!bin_on(T.active_mask, S.TB[i])) return;
+    const long n = S.n;
+    const double mass = S.M[i];
+    const double dt_hydrokick = (tend - tstart) * T.tbi;
+    const double du_tot = S.DTU[i] * dt_hydrokick + 0.0;
+    const double dEnt = S.UK[i] + du_tot;
+    double u = positivity(S.UK[i], dEnt);
+    if (T.min_egy) {
+        if (u < T.min_egy) { u = T.min_egy; S.DTU[i] = 0; }
+    }
+    S.UK[i] = u;
+    const double fac = mass * 1.0 * dt_hydrokick;
+    const double inv = 1.0 / mass;
+    double *vk[3] = {S.VKX, S.VKY, S.VKZ};
+    for (int k = 0; k < 3; k++) {
+        double dp = 0.0;
+        dp += S.ACC[(size_t)k * n + i] * fac;
+        vk[k][i] += dp * inv;
+    }
+    if (mode == 1) {
+        S.VX[i] = S.VKX[i]; S.VY[i] = S.VKY[i]; S.VZ[i] = S.VKZ[i];
+        S.U[i] = S.UK[i];
+    }
+}
+
+// find_timesteps, src/core/timestep.cc:39-355: the new power-of-two step of each particle of
+// the active list; a larger bin only where it is synchronised now
+__global__ void __launch_bounds__(256) k_find_timesteps(TimeParams T, ParticleView S, int nrows, const int *__restrict__ act,
+                                                         int *__restrict__ err)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nrows) return;
+    const int i = act[r];
+    long long ti_step = get_timestep(T, S, i);
+    ti_step = (long long)(((double)ti_step) / 1.0);
+    long long ti_min = kTimebase;
+    while (ti_min > ti_step) ti_min >>= 1;
+    ti_step = ti_min;
+    int bin = timestep_bin(ti_step);
+    const int binold = S.TB[i];
+    if (bin > binold) {
+        while (!bin_on(T.active_mask, bin) && bin > binold) bin--;
+        ti_step = bin_len(bin);
+    }
+    if (T.ti_current >= kTimebase) { ti_step = 0; bin = 0; }
+    if ((kTimebase - T.ti_current) < ti_step) {
+        atomicMax(err, 90001005);
+        ti_step = kTimebase - T.ti_current;
+        ti_min = kTimebase;
+        while (ti_min > ti_step) ti_min >>= 1;
+        ti_step = ti_min;
+    }
+    S.TB[i] = bin;
+    S.TIB[i] += S.DTS[i];
+    S.DTS[i] = ti_step;
+}
+
+// TimeBinCount: occupation of the 60 bins (block histogram in LDS, one global add per bin)
+__global__ void __launch_bounds__(256) k_bin_hist(long n, const int *__restrict__ TB, unsigned long long *__restrict__ count)
+{
+    __shared__ unsigned int h[64];
+    if (threadIdx.x < 64) h[threadIdx.x] = 0;
+    __syncthreads();
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        atomicAdd(&h[TB[i] & 63], 1u);
+    __syncthreads();
+    if (threadIdx.x < 64 && h[threadIdx.x]) atomicAdd(&count[threadIdx.x], (unsigned long long)h[threadIdx.x]);
+}
+
+// process_wake_ups, src/core/timestep.cc:1565-1774 (every particle stands at the sync point:
+// its Ti_current is All.Ti_Current, so a wake-up never truncates a step)
+__global__ void __launch_bounds__(256) k_wakeups(TimeParams T, ParticleView S, int lowest_occupied_active_bin,
+                                                  int max_time_bin_active, int wakeup_bin_offset, int *__restrict__ nwoken)
+{
+    const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (i >= S.n) return;
+    const int w = S.WAKE[i];
+    if (!w) return;
+    if (S.M[i] <= 0) return;
+    const int binold = S.TB[i];
+    if (bin_on(T.active_mask, binold)) return;
+    int bin;
+    if (w > 0) {
+        bin = (w - 1) - wakeup_bin_offset;
+        if (bin < 0) bin = 0;
+        if (bin < lowest_occupied_active_bin) bin = lowest_occupied_active_bin;
+    } else {
+        bin = max_time_bin_active;
+    }
+    if (bin > max_time_bin_active) bin = max_time_bin_active;
+    if (bin >= binold) bin = binold;
+    if (bin == binold) return;
+    const long long old_midpoint = S.TIB[i] + S.DTS[i] / 2;
+    const bool starts_now = bin_on(T.active_mask, bin) && (S.DTS[i] == bin_len(binold));
+    S.TB[i] = bin;
+    atomicAdd(nwoken, 1);
+    if (starts_now) kick(T, S, i, old_midpoint, T.ti_current, 1);
+    S.TIB[i] = T.ti_current;
+    S.DTS[i] = bin_len(bin);
+    if (starts_now) kick(T, S, i, S.TIB[i], S.TIB[i] + S.DTS[i] / 2, 0);
+}
+
+// do_first_halfstep_kick / do_second_halfstep_kick, src/core/kicks.cc:34-97,99-185
+__global__ void __launch_bounds__(256) k_halfstep_kick(TimeParams T, ParticleView S, int nrows, const int *__restrict__ act, int second)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nrows) return;
+    const int i = act[r];
+    if (!(S.M[i] > 0)) return;
+    const long long half = S.DTS[i] / 2;
+    if (second) kick(T, S, i, S.TIB[i] + half, S.TIB[i] + S.DTS[i], 1);
+    else kick(T, S, i, S.TIB[i], S.TIB[i] + half, 0);
+}
+
+// drift_particle_impl, src/core/drift_particle_functions.h:86-250 (MFM gas; the kernel radius
+// moves with the flow, src/declarations/precompiler_logic.h:78), every particle from time0 to
+// time1, then one periodic wrap per axis.  disp: largest periodic distance of any particle
+// from where it was when the particles were last ordered (a positive double's bits order
+// like the double).
+__global__ void __launch_bounds__(256) k_drift(TimeParams T, ParticleView S, long long time0, long long time1,
+                                                const double *__restrict__ X0, const double *__restrict__ Y0,
+                                                const double *__restrict__ Z0, unsigned long long *__restrict__ disp)
+{
+    const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    double d = 0;
+    if (i < S.n && time1 != time0) {
+        const double dt_drift = (time1 - time0) * T.tbi * 1.0;
+        double x = S.X[i] + S.VKX[i] * dt_drift;
+        double y = S.Y[i] + S.VKY[i] * dt_drift;
+        double z = S.Z[i] + S.VKZ[i] * dt_drift;
+        const double divv_fac = drift_log_change(S.DIVV[i] * dt_drift);
+        const double h_drift_fac = exp(drift_log_change(S.DIVV[i] * dt_drift) / ((double)3));
+        if (S.M[i] > 0) {
+            const long n = S.n;
+            const double dt_hydrokick = (time1 - time0) * T.tbi;
+            S.VX[i] += 0.0 + S.ACC[i] * (dt_hydrokick * 1.0);
+            S.VY[i] += 0.0 + S.ACC[n + i] * (dt_hydrokick * 1.0);
+            S.VZ[i] += 0.0 + S.ACC[2 * n + i] * (dt_hydrokick * 1.0);
+            double rho = S.RHO[i] * exp(-divv_fac);
+            const double etmp = S.U[i] + S.DTU[i] * dt_hydrokick;
+            double u = positivity(S.U[i], etmp);
+            if (u < T.min_egy) u = T.min_egy;
+            double h = S.H[i] * h_drift_fac;
+            if (h < T.min_h) h = T.min_h;
+            if (h > T.max_h) h = T.max_h;
+            S.RHO[i] = rho;
+            S.U[i] = u;
+            S.H[i] = h;
+            S.PR[i] = (T.gamma - 1) * u * rho;
+        }
+        if (x >= T.box) x -= T.box; else if (x < 0) x += T.box;
+        if (y >= T.box) y -= T.box; else if (y < 0) y += T.box;
+        if (z >= T.box) z -= T.box; else if (z < 0) z += T.box;
+        S.X[i] = x; S.Y[i] = y; S.Z[i] = z;
+        const double half = 0.5 * T.box;
+        const double ex = fabs(wrap(x - X0[i], T.box, half)), ey = fabs(wrap(y - Y0[i], T.box, half)),
+                     ez = fabs(wrap(z - Z0[i], T.box, half));
+        d = fmax(ex, fmax(ey, ez));
+    }
+    // block maximum, then one atomic per block
+    __shared__ double sm[256];
+    sm[threadIdx.x] = d;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) sm[threadIdx.x] = fmax(sm[threadIdx.x], sm[threadIdx.x + o]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0 && sm[0] > 0) atomicMax(disp, (unsigned long long)__double_as_longlong(sm[0]));
+}
+
+// TimeBinActive of each particle (the byte the force pass tests a neighbour with), and the
+// softening cache of the active list (compute_all_force_softening(0),
+// src/gravity/gravtree.cc:267)
+__global__ void __launch_bounds__(256) k_active_from_bins(long n, unsigned long long mask, const int *__restrict__ TB,
+                                                           const double *__restrict__ M, const double *__restrict__ H,
+                                                           uint8_t *__restrict__ ACT, double *__restrict__ SOFT)
+{
+    const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const bool on = bin_on(mask, TB[i]);
+    ACT[i] = on ? 1 : 0;
+    if (on && M[i] > 0) SOFT[i] = H[i];
+}
+
+}  // namespace gz

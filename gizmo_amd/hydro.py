@@ -20,6 +20,7 @@ GZ_FP32_MIXED = 1
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)
 _lp = C.POINTER(C.c_long)
+_llp = C.POINTER(C.c_longlong)
 
 
 class Params(C.Structure):
@@ -32,10 +33,28 @@ class Params(C.Structure):
 FIELD_WIDTH = dict(pos=3, vel=3, mass=1, h=1, u=1, numngb=1, drkern=1, divvel=1, rho=1, press=1,
                    nvt=6, nvt_fw=3, cn=1, fce=1, grad_rho=3, grad_p=3, grad_v=9, accel=3, dtu=1, maxsig=1)
 INT_FIELDS = ("timebin", "wakeup")
+LOOP_FIELD_WIDTH = dict(vel_kick=3, u_kick=1, soft=1)
+I64_FIELDS = ("ti_begstep", "dt_step")
+TIMEBINS = 60
 
 
 class Fields(C.Structure):
-    _fields_ = [(k, _dp) for k in FIELD_WIDTH] + [(k, _ip) for k in INT_FIELDS]
+    _fields_ = ([(k, _dp) for k in FIELD_WIDTH] + [(k, _ip) for k in INT_FIELDS] +
+                [(k, _dp) for k in LOOP_FIELD_WIDTH] + [(k, _llp) for k in I64_FIELDS])
+
+
+class IntegParams(C.Structure):
+    """All.* parameters of the timestep loop (gz_integ_params)."""
+    _fields_ = [("courant_fac", C.c_double), ("err_tol_int_accuracy", C.c_double),
+                ("max_size_timestep", C.c_double), ("min_size_timestep", C.c_double),
+                ("time_begin", C.c_double), ("time_max", C.c_double), ("min_egy_spec", C.c_double)]
+
+
+class Timeline(C.Structure):
+    """The integer timeline and the time bins (gz_timeline)."""
+    _fields_ = [("ti_current", C.c_longlong), ("time", C.c_double), ("timebase_interval", C.c_double),
+                ("num_steps", C.c_longlong), ("n_active", C.c_longlong), ("n_woken", C.c_longlong),
+                ("bin_count", C.c_longlong * TIMEBINS), ("bin_active", C.c_ulonglong), ("stop_code", C.c_int)]
 
 
 class GizmoHipError(RuntimeError):
@@ -71,6 +90,10 @@ def lib():
         L.gz_timing.argtypes = [C.c_void_p, C.c_char_p, _lp, _dp]
         L.gz_neighbors.argtypes = [C.c_void_p, C.c_int, _lp, _ip, C.c_long]
         L.gz_neighbors.restype = C.c_long
+        L.gz_run_begin.argtypes = [C.c_void_p, C.POINTER(IntegParams)]
+        L.gz_run_step.argtypes = [C.c_void_p]
+        L.gz_timeline_get.argtypes = [C.c_void_p, C.POINTER(Timeline)]
+        L.gz_set_option.argtypes = [C.c_void_p, C.c_char_p, C.c_double]
         _lib = L
     return _lib
 
@@ -122,11 +145,21 @@ class Hydro:
                 a = np.ascontiguousarray(fields[k], dtype=np.float64).reshape(n * w)
                 keep.append(a)
                 setattr(f, k, a.ctypes.data_as(_dp))
+        for k, w in LOOP_FIELD_WIDTH.items():
+            if k in fields and fields[k] is not None:
+                a = np.ascontiguousarray(fields[k], dtype=np.float64).reshape(n * w)
+                keep.append(a)
+                setattr(f, k, a.ctypes.data_as(_dp))
         for k in INT_FIELDS:
             if k in fields and fields[k] is not None:
                 a = np.ascontiguousarray(fields[k], dtype=np.int32).reshape(n)
                 keep.append(a)
                 setattr(f, k, a.ctypes.data_as(_ip))
+        for k in I64_FIELDS:
+            if k in fields and fields[k] is not None:
+                a = np.ascontiguousarray(fields[k], dtype=np.int64).reshape(n)
+                keep.append(a)
+                setattr(f, k, a.ctypes.data_as(_llp))
         self._check(lib().gz_load(self.ctx, n, C.byref(f)))
         self.n = n
 
@@ -134,11 +167,15 @@ class Hydro:
         names = list(FIELD_WIDTH) + list(INT_FIELDS) if names is None else list(names)
         f = Fields()
         out = {}
+        widths = dict(FIELD_WIDTH, **LOOP_FIELD_WIDTH)
         for k in names:
-            if k in FIELD_WIDTH:
-                w = FIELD_WIDTH[k]
+            if k in widths:
+                w = widths[k]
                 a = np.zeros((self.n, w) if w > 1 else self.n)
                 setattr(f, k, a.ctypes.data_as(_dp))
+            elif k in I64_FIELDS:
+                a = np.zeros(self.n, dtype=np.int64)
+                setattr(f, k, a.ctypes.data_as(_llp))
             else:
                 a = np.zeros(self.n, dtype=np.int32)
                 setattr(f, k, a.ctypes.data_as(_ip))
@@ -190,6 +227,28 @@ class Hydro:
         cnt, ms = C.c_long(), C.c_double()
         self._check(lib().gz_timing(self.ctx, name.encode(), C.byref(cnt), C.byref(ms)))
         return cnt.value, ms.value
+
+    # ---- the timestep loop (gz_run_begin / gz_run_step) ----
+    def run_begin(self, integ):
+        """integ: dict of the IntegParams fields (All.CourantFac, ErrTolIntAccuracy, ...)."""
+        self._ip = IntegParams(**{k: float(integ[k]) for k, _ in IntegParams._fields_})
+        self._check(lib().gz_run_begin(self.ctx, C.byref(self._ip)))
+
+    def run_step(self):
+        """One sync point of the run loop; returns the number of active particles updated."""
+        self._check(lib().gz_run_step(self.ctx))
+        return self.timeline()["n_active"]
+
+    def set_option(self, name, value):
+        self._check(lib().gz_set_option(self.ctx, name.encode(), float(value)))
+
+    def timeline(self):
+        t = Timeline()
+        self._check(lib().gz_timeline_get(self.ctx, C.byref(t)))
+        return dict(ti_current=t.ti_current, time=t.time, timebase_interval=t.timebase_interval,
+                    num_steps=t.num_steps, n_active=t.n_active, n_woken=t.n_woken,
+                    bin_count=np.array(t.bin_count[:], dtype=np.int64), bin_active=t.bin_active,
+                    stop_code=t.stop_code)
 
     def neighbors(self, symmetric):
         off = np.zeros(self.n + 1, dtype=np.int64)

@@ -80,6 +80,12 @@ typedef struct {
     double *maxsig;    /* CellP.MaxSignalVel */
     int *timebin;      /* P.TimeBin */
     int *wakeup;       /* P.wakeup */
+    /* timestep-loop fields (gz_run_*): a drop-in that keeps its own loop leaves them NULL */
+    double *vel_kick;  /* P.Vel[3] (the kicked velocity; vel above is the predicted one) */
+    double *u_kick;    /* CellP.InternalEnergy (u above is InternalEnergyPred) */
+    double *soft;      /* P.ForceSoftening (kernel radius cached for the acceleration criterion) */
+    long long *ti_begstep; /* P.Ti_begstep */
+    long long *dt_step;    /* P.dt_step */
 } gz_fields;
 
 typedef struct gz_context gz_context;
@@ -114,7 +120,9 @@ int gz_synchronize(gz_context *ctx);
 
 /* Diagnostics.  gz_permutation: perm[k] = caller index of the k-th particle in device order.
    gz_stat: named counters of the last calls ("density_iterations", "pairs_density",
-   "pairs_symmetric", "ngb_rebuilds", "n_active"). */
+   "pairs_symmetric", "ngb_rebuilds", "n_active", "ell_widen"), and totals since the last
+   gz_timing_reset ("active_updates": active particles through the hot path, "resorts",
+   "force_pairs_total": neighbour pairs of every force launch). */
 int gz_permutation(gz_context *ctx, long *perm);
 int gz_stat(gz_context *ctx, const char *name, double *value);
 
@@ -124,6 +132,53 @@ int gz_stat(gz_context *ctx, const char *name, double *value);
 int gz_timing_enable(gz_context *ctx, int on);
 int gz_timing_reset(gz_context *ctx);
 int gz_timing(gz_context *ctx, const char *name, long *launches, double *total_ms);
+
+/* ---- the timestep loop around the hot path (src/core/run.cc:220-507) ----
+   A gas-only, gravity-free, non-comoving run on the resident state: power-of-two time bins on
+   the integer timeline (TIMEBINS = 60, src/declarations/typedefs.h:8-9), kick-drift-kick with
+   the reference's per-particle steps, wake-ups, and every particle drifted to each sync point.
+   The state stays on the device between steps; gz_store reads it back at any time. */
+#define GZ_TIMEBINS 60
+typedef struct {
+    double courant_fac;          /* All.CourantFac */
+    double err_tol_int_accuracy; /* All.ErrTolIntAccuracy */
+    double max_size_timestep;    /* All.MaxSizeTimestep */
+    double min_size_timestep;    /* All.MinSizeTimestep */
+    double time_begin;           /* All.TimeBegin */
+    double time_max;             /* All.TimeMax (sets All.Timebase_interval) */
+    double min_egy_spec;         /* All.MinEgySpec (0 = no floor) */
+} gz_integ_params;
+
+typedef struct {
+    long long ti_current;              /* All.Ti_Current */
+    double time;                       /* All.Time */
+    double timebase_interval;          /* All.Timebase_interval */
+    long long num_steps;               /* All.NumCurrentTiStep */
+    long long n_active;                /* active particles of the last step (hot-path updates) */
+    long long n_woken;                 /* particles the last wake-up pass moved to a shorter step */
+    long long bin_count[GZ_TIMEBINS];  /* TimeBinCount */
+    unsigned long long bin_active;     /* TimeBinActive as a bit mask */
+    int stop_code;                     /* first controlled-stop code raised, else 0 */
+} gz_timeline;
+
+/* init() (src/core/init.cc:108-113,583,1051,1061) and run()'s first
+   compute_hydro_densities_and_forces() (src/core/run.cc:128-150) on the loaded state: every
+   particle in bin 0 and active, predicted fields = current ones (vel_kick/u_kick are taken
+   from vel/u when gz_load was not given them), density, softening cache, then
+   density -> gradients -> force. */
+int gz_run_begin(gz_context *ctx, const gz_integ_params *ip);
+/* One iteration of run()'s main loop (src/core/run.cc:220-507): find_timesteps() with
+   process_wake_ups() (src/core/timestep.cc:39,1565), do_first_halfstep_kick()
+   (src/core/kicks.cc:34), find_next_sync_point_and_drift() (src/core/run.cc:857),
+   make_list_of_active_particles() (:997), compute_hydro_densities_and_forces()
+   (src/core/accel.cc:68) and do_second_halfstep_kick() (src/core/kicks.cc:99). */
+int gz_run_step(gz_context *ctx);
+int gz_timeline_get(gz_context *ctx, gz_timeline *out);
+/* Tuning knobs with no counterpart in the reference: "resort_fraction" (default 0.125) -- the
+   particles are put back in space-filling-curve order once any of them has moved farther than
+   this fraction of the search radius since they were last ordered (the search grid widens its
+   cells by twice that distance meanwhile). */
+int gz_set_option(gz_context *ctx, const char *name, double value);
 
 /* Neighbour lists of the resident state, in the caller's indexing, for parity tests.
    symmetric = 0: rows r2 < h_i^2 (self included); 1: r2 < max(h_i,h_j)^2 (self excluded).

@@ -22,7 +22,8 @@
  * (GAMMA_DEFAULT is a compile-time constant in the reference).
  *
  * Parity status of each function is stated in the header comment of the function
- * and summarised in oracle/README.md.
+ * and summarised in oracle/README.md.  The timestep loop around the hot path (kicks,
+ * drift, timebins, wake-ups) is restated in gz_oracle_time.c.
  */
 #include <math.h>
 #include <stdint.h>
@@ -1413,6 +1414,13 @@ static int or_force_pair(const orc_params *par, const or_gamma *G, const orc_sta
    WAKEUP * its MaxSignalVel.  Returns 0, or 1234 when a Riemann problem had no solution. */
 int orc_force(const orc_params *par, orc_state *s, const unsigned char *active)
 {
+    return orc_force_ex(par, s, active, NULL);
+}
+
+/* need_wakeup (optional) is set to 1 when any inactive neighbour was flagged, as the
+   reference's NeedToWakeupParticles_local (src/hydro/hydro_force_loop.cc:160-163). */
+int orc_force_ex(const orc_params *par, orc_state *s, const unsigned char *active, int *need_wakeup)
+{
     long n = s->n;
     double box = par->box, half = 0.5 * par->box;
     or_gamma G;
@@ -1431,6 +1439,8 @@ int orc_force(const orc_params *par, orc_state *s, const unsigned char *active)
 #pragma omp for schedule(dynamic, 64)
         for (long i = 0; i < n; i++) {
             if (active && !active[i]) continue;
+            /* hydro_force_initial_operations_preloop, src/hydro/hydro_toplevel.cc:457-565 */
+            if (s->wakeup) s->wakeup[i] = 0;
             s->maxsig[i] = OR_MIN_REAL;
             s->dtu[i] = 0;
             s->accel[3 * i + 0] = s->accel[3 * i + 1] = s->accel[3 * i + 2] = 0;
@@ -1465,7 +1475,7 @@ int orc_force(const orc_params *par, orc_state *s, const unsigned char *active)
                     if (vts > OR_WAKEUP * msv_in[j]) {
                         int w = (s->timebin ? s->timebin[i] : 0) + 1;
 #pragma omp critical(or_wakeup)
-                        { if (s->wakeup[j] < w) s->wakeup[j] = w; }
+                        { if (s->wakeup[j] < w) s->wakeup[j] = w; if (need_wakeup) *need_wakeup = 1; }
                     }
                 }
             }

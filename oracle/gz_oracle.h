@@ -46,7 +46,37 @@ typedef struct {
     double *accel;    /* n*3 HydroAccel (dv/dt after force) */
     double *dtu;      /* DtInternalEnergy (du/dt after force) */
     double *maxsig;   /* MaxSignalVel */
+    /* timestep-loop fields (gz_oracle_time.c; may be NULL for the hydro stages alone) */
+    double *vel_kick;         /* n*3 P[].Vel (kicked velocity; vel above is VelPred) */
+    double *u_kick;           /* CellP[].InternalEnergy (u above is InternalEnergyPred) */
+    double *soft;             /* P[].ForceSoftening: the kernel radius cached for the acceleration step */
+    long long *ti_begstep;    /* P[].Ti_begstep */
+    long long *dt_step;       /* P[].dt_step */
 } orc_state;
+
+#define ORC_TIMEBINS 60
+
+typedef struct {              /* All.* parameters of the timestep loop */
+    double courant_fac;          /* All.CourantFac */
+    double err_tol_int_accuracy; /* All.ErrTolIntAccuracy */
+    double max_size_timestep;    /* All.MaxSizeTimestep */
+    double min_size_timestep;    /* All.MinSizeTimestep */
+    double time_begin, time_max; /* All.TimeBegin, All.TimeMax */
+    double min_egy_spec;         /* All.MinEgySpec */
+} orc_integ;
+
+typedef struct {              /* the integer timeline and the time bins */
+    long long ti_current;                    /* All.Ti_Current */
+    double time;                             /* All.Time */
+    double timebase_interval;                /* All.Timebase_interval */
+    long long num_steps;                     /* All.NumCurrentTiStep */
+    long long bin_count[ORC_TIMEBINS];       /* TimeBinCount */
+    unsigned char bin_active[ORC_TIMEBINS];  /* TimeBinActive */
+    int need_wakeup;                         /* NeedToWakeupParticles_local */
+    int stop_code;                           /* first controlled-stop code raised, else 0 */
+    long long n_active;                      /* length of the active list of the last step */
+    long long n_woken;                       /* particles moved to a shorter step by the last wake-up pass */
+} orc_timeline;
 
 int orc_kernel(int kernel, double r, double h, double *wk, double *dwk);
 long orc_neighbors(const orc_params *par, long n, const double *pos, const double *h, int symmetric,
@@ -63,6 +93,14 @@ int orc_riemann(double gamma, const double *left, const double *right, const dou
                 double ptl, double *P_M, double *S_M, double *fv, double *fp);
 int orc_riemann_exact(double gamma, const double *left, const double *right, double *P_M, double *S_M);
 int orc_force(const orc_params *par, orc_state *s, const unsigned char *active);
+int orc_force_ex(const orc_params *par, orc_state *s, const unsigned char *active, int *need_wakeup);
+long long orc_get_timestep(const orc_params *par, const orc_integ *ip, const orc_timeline *tl, const orc_state *s, long i);
+int orc_run_begin(const orc_params *par, const orc_integ *ip, orc_timeline *tl, orc_state *s);
+int orc_run_step(const orc_params *par, const orc_integ *ip, orc_timeline *tl, orc_state *s);
+void orc_kick(const orc_params *par, const orc_integ *ip, const orc_timeline *tl, orc_state *s, long i,
+              long long tstart, long long tend, int mode);
+void orc_drift(const orc_params *par, const orc_integ *ip, const orc_timeline *tl, orc_state *s, long i,
+               long long time0, long long time1);
 int orc_num_threads(void);
 
 #ifdef __cplusplus

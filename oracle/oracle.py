@@ -22,6 +22,7 @@ _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)
 _lp = C.POINTER(C.c_long)
 _up = C.POINTER(C.c_ubyte)
+_llp = C.POINTER(C.c_longlong)
 
 
 class Params(C.Structure):
@@ -36,13 +37,34 @@ class _CState(C.Structure):
                 ("drkern", _dp), ("divvel", _dp), ("timebin", _ip), ("wakeup", _ip),
                 ("vel", _dp), ("u", _dp), ("rho", _dp), ("press", _dp), ("nvt", _dp),
                 ("nvt_fw", _dp), ("cn", _dp), ("fce", _dp), ("grad_rho", _dp), ("grad_p", _dp),
-                ("grad_v", _dp), ("accel", _dp), ("dtu", _dp), ("maxsig", _dp)]
+                ("grad_v", _dp), ("accel", _dp), ("dtu", _dp), ("maxsig", _dp),
+                ("vel_kick", _dp), ("u_kick", _dp), ("soft", _dp), ("ti_begstep", _llp), ("dt_step", _llp)]
+
+
+TIMEBINS = 60
+
+
+class Integ(C.Structure):
+    """All.* parameters of the timestep loop."""
+    _fields_ = [("courant_fac", C.c_double), ("err_tol_int_accuracy", C.c_double),
+                ("max_size_timestep", C.c_double), ("min_size_timestep", C.c_double),
+                ("time_begin", C.c_double), ("time_max", C.c_double), ("min_egy_spec", C.c_double)]
+
+
+class Timeline(C.Structure):
+    """The integer timeline and the time bins (All.Ti_Current, TimeBinCount, TimeBinActive ...)."""
+    _fields_ = [("ti_current", C.c_longlong), ("time", C.c_double), ("timebase_interval", C.c_double),
+                ("num_steps", C.c_longlong), ("bin_count", C.c_longlong * TIMEBINS),
+                ("bin_active", C.c_ubyte * TIMEBINS), ("need_wakeup", C.c_int), ("stop_code", C.c_int),
+                ("n_active", C.c_longlong), ("n_woken", C.c_longlong)]
 
 
 # field name -> columns per particle
 FIELDS = dict(pos=3, mass=1, h=1, numngb=1, drkern=1, divvel=1, vel=3, u=1, rho=1, press=1,
-              nvt=6, nvt_fw=3, cn=1, fce=1, grad_rho=3, grad_p=3, grad_v=9, accel=3, dtu=1, maxsig=1)
+              nvt=6, nvt_fw=3, cn=1, fce=1, grad_rho=3, grad_p=3, grad_v=9, accel=3, dtu=1, maxsig=1,
+              vel_kick=3, u_kick=1, soft=1)
 INT_FIELDS = ("timebin", "wakeup")
+I64_FIELDS = ("ti_begstep", "dt_step")
 
 
 def build(force=False):
@@ -69,6 +91,15 @@ def lib():
         L.orc_riemann.argtypes = [C.c_double, _dp, _dp, _dp, C.c_double, _dp, _dp, _dp, _dp]
         L.orc_riemann_exact.argtypes = [C.c_double, _dp, _dp, _dp, _dp]
         L.orc_force.argtypes = [C.POINTER(Params), C.POINTER(_CState), _up]
+        L.orc_force_ex.argtypes = [C.POINTER(Params), C.POINTER(_CState), _up, _ip]
+        L.orc_get_timestep.argtypes = [C.POINTER(Params), C.POINTER(Integ), C.POINTER(Timeline), C.POINTER(_CState), C.c_long]
+        L.orc_get_timestep.restype = C.c_longlong
+        for f in (L.orc_run_begin, L.orc_run_step):
+            f.argtypes = [C.POINTER(Params), C.POINTER(Integ), C.POINTER(Timeline), C.POINTER(_CState)]
+        L.orc_kick.argtypes = [C.POINTER(Params), C.POINTER(Integ), C.POINTER(Timeline), C.POINTER(_CState), C.c_long,
+                               C.c_longlong, C.c_longlong, C.c_int]
+        L.orc_drift.argtypes = [C.POINTER(Params), C.POINTER(Integ), C.POINTER(Timeline), C.POINTER(_CState), C.c_long,
+                                C.c_longlong, C.c_longlong]
         L.orc_num_threads.restype = C.c_int
         _lib = L
     return _lib
@@ -91,31 +122,37 @@ class State:
             setattr(self, k, np.zeros((self.n, c)) if c > 1 else np.zeros(self.n))
         self.timebin = np.zeros(self.n, dtype=np.int32)
         self.wakeup = np.zeros(self.n, dtype=np.int32)
+        self.ti_begstep = np.zeros(self.n, dtype=np.int64)
+        self.dt_step = np.zeros(self.n, dtype=np.int64)
+
+    ALL = list(FIELDS) + list(INT_FIELDS) + list(I64_FIELDS)
 
     @classmethod
     def from_ic(cls, ic):
         s = cls(len(ic["mass"]))
         for k in ("pos", "vel", "mass", "u", "h"):
             getattr(s, k)[...] = ic[k]
+        s.vel_kick[...] = ic["vel"]
+        s.u_kick[...] = ic["u"]
         return s
 
     def copy(self):
         s = State(self.n)
-        for k in list(FIELDS) + list(INT_FIELDS):
+        for k in self.ALL:
             getattr(s, k)[...] = getattr(self, k)
         return s
 
     def permuted(self, perm):
         """A copy with particle k taken from index perm[k]."""
         s = State(self.n)
-        for k in list(FIELDS) + list(INT_FIELDS):
+        for k in self.ALL:
             getattr(s, k)[...] = getattr(self, k)[perm]
         return s
 
     def _c(self):
-        for k in list(FIELDS) + list(INT_FIELDS):
+        for k in self.ALL:
             a = getattr(self, k)
-            if not (a.flags.c_contiguous and a.dtype in (np.float64, np.int32)):
+            if not (a.flags.c_contiguous and a.dtype in (np.float64, np.int32, np.int64)):
                 setattr(self, k, np.ascontiguousarray(a))
         cs = _CState()
         cs.n = self.n
@@ -123,6 +160,8 @@ class State:
             setattr(cs, k, _ptr(getattr(self, k)))
         cs.timebin = _ptr(self.timebin, _ip)
         cs.wakeup = _ptr(self.wakeup, _ip)
+        cs.ti_begstep = _ptr(self.ti_begstep, _llp)
+        cs.dt_step = _ptr(self.dt_step, _llp)
         return cs
 
 
@@ -245,3 +284,50 @@ def hydro_pass(par, st, active=None):
 
 def num_threads():
     return lib().orc_num_threads()
+
+
+def integ(d):
+    return Integ(**{k: float(d[k]) for k, _ in Integ._fields_})
+
+
+class Run:
+    """The timestep loop of run() on an oracle State: begin() then step() per sync point."""
+
+    def __init__(self, par, ip, st):
+        self.par = params(par)
+        self.ip = integ(ip)
+        self.tl = Timeline()
+        self.st = st
+
+    def begin(self):
+        cs = self.st._c()
+        rc = lib().orc_run_begin(C.byref(self.par), C.byref(self.ip), C.byref(self.tl), C.byref(cs))
+        if rc:
+            raise RuntimeError(f"orc_run_begin stop code {rc}")
+
+    def step(self):
+        cs = self.st._c()
+        rc = lib().orc_run_step(C.byref(self.par), C.byref(self.ip), C.byref(self.tl), C.byref(cs))
+        if rc:
+            raise RuntimeError(f"orc_run_step stop code {rc}")
+        return int(self.tl.n_active)
+
+    def get_timestep(self, i):
+        cs = self.st._c()
+        return int(lib().orc_get_timestep(C.byref(self.par), C.byref(self.ip), C.byref(self.tl), C.byref(cs), int(i)))
+
+    def kick(self, i, tstart, tend, mode):
+        cs = self.st._c()
+        lib().orc_kick(C.byref(self.par), C.byref(self.ip), C.byref(self.tl), C.byref(cs), int(i), int(tstart), int(tend), int(mode))
+
+    def drift(self, i, time0, time1):
+        cs = self.st._c()
+        lib().orc_drift(C.byref(self.par), C.byref(self.ip), C.byref(self.tl), C.byref(cs), int(i), int(time0), int(time1))
+
+    @property
+    def ti_current(self):
+        return int(self.tl.ti_current)
+
+    @property
+    def time(self):
+        return float(self.tl.time)
