@@ -1,22 +1,28 @@
 """Benchmark of the hydro hot path on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--n-side 128] [--no-cpu]
+    python bench.py [--gpus 1] [--steps 50] [--warmup 2] [--n-side 128] [--no-cpu] [--no-full-pass]
 
-Workload: the 2M-particle Sedov blast (128^3 jittered lattice in a periodic unit box, cold
+Workload (the 2M-particle Sedov job): a 128^3 jittered lattice in a periodic unit box, cold
 medium rho = 1, P = 1e-6, gamma 5/3, E = 1 deposited kernel-weighted into the 64 central
-particles; Wendland C2 kernel, 64 neighbours).  A step is one pass of the hot path over every
-particle: the neighbour builds, density() with all its kernel-radius iterations,
-hydro_gradient_calc(), hydro_force() and the final operations.  The warmup passes converge the
-radii from the initial guess; the timed steps then start each density() from the converged
-radius of the previous pass, as consecutive timesteps do.  Inputs are resident in HBM for
-the whole timed region.
+particles; Wendland C2 kernel, 64 neighbours; the timestep loop with adaptive power-of-two
+bins (CourantFac 0.05, ErrTolIntAccuracy 0.002, MaxSizeTimestep 0.01 as in the reference's
+own Sedov test, TimeMax 1).  A step is one sync point of the run loop: new steps and
+wake-ups, the first half-kick, the drift of every particle to the next sync point, the hot
+path (neighbour builds, density() with all its radius iterations, hydro_gradient_calc(),
+hydro_force(), final ops) over the particles active there, and the second half-kick.
+`value` is the active-particle updates per second of the timed steps (sum of the active
+counts over the wall time), the job starting from gz_run_begin; the state is resident in HBM
+for the whole timed region.  Warmup: W steps of an identical untimed job.
 
-Both arithmetic forms are timed (FP32-mixed is the reported `value`, plain FP64 beside it).
-The roofline entry covers the force kernel, the dominant one: algorithmic bytes per launch
-(pair-gather model: n_active (S_i + S_out) + n_pairs (S_j + 4), with S_i = S_j = 288 B and
-S_out = 40 B per value-set in FP64, 144/20 B in FP32) over its average launch time, measured
-with HIP events on the stream the kernels run on.  The CPU baseline is the FP64 oracle (C +
-OpenMP) timed on this host on a random sample of the same state, as active particles.
+Both arithmetic forms run the same job: the plain FP64 form (the reported `value`, as the
+job is the FP64-at-scale configuration) and the FP32-mixed form beside it.  The roofline
+entry covers the force kernel: algorithmic bytes of every launch in the timed region
+(pair-gather model: n_active (S_i + S_out) + n_pairs (S_j + 4), S_i = S_j = 288 B and
+S_out = 40 B per particle in FP64, 144/20 B in FP32, with n_pairs counted on the device) over
+its launch time, measured with HIP events on the stream the kernels run on.  `full_pass` adds
+one all-active pass of the hot path over the 2M particles (the kernels at full occupancy).
+The CPU baseline is the FP64 oracle (C + OpenMP) running the first steps of the same job on
+this host's cores.
 """
 import argparse
 import json
@@ -31,8 +37,11 @@ sys.path.insert(0, ROOT)
 
 METRIC = "hydro active-particle updates/s at 2M (mixed vs FP64); p50/p99 job latency"
 HBM_PEAK_GBS = 8000.0
-# pair-gather model of the force pass, bytes per value-set
+# pair-gather model of the force pass, bytes per particle value-set
 FORCE_MODEL = {"fp64": dict(S_i=288, S_out=40, S_j=288), "mixed": dict(S_i=144, S_out=20, S_j=144)}
+INTEG = dict(courant_fac=0.05, err_tol_int_accuracy=0.002, max_size_timestep=0.01, min_size_timestep=0.0,
+             time_begin=0.0, time_max=1.0, min_egy_spec=0.0)
+FAMILIES = ("timeline", "sort", "ngb_build", "density", "gradients", "force")
 
 
 def log(msg):
@@ -42,94 +51,116 @@ def log(msg):
 def physical_cores():
     try:
         cores = set()
-        phys = core = None
+        phys = None
         for line in open("/proc/cpuinfo"):
             if line.startswith("physical id"):
                 phys = line.split(":")[1].strip()
             elif line.startswith("core id"):
-                core = line.split(":")[1].strip()
-                cores.add((phys, core))
+                cores.add((phys, line.split(":")[1].strip()))
         return len(cores) or os.cpu_count()
     except OSError:
         return os.cpu_count()
 
 
-def run_gpu(ic, precision, steps, warmup):
+def new_job(ic, precision):
     from gizmo_amd.hydro import Hydro
-    par = dict(ic["params"])
-    n = len(ic["mass"])
-    g = Hydro(par, n, precision)
+    g = Hydro(dict(ic["params"]), len(ic["mass"]), precision)
     g.load(dict(pos=ic["pos"], vel=ic["vel"], mass=ic["mass"], u=ic["u"], h=ic["h"]))
-    for w in range(warmup):
-        t0 = time.perf_counter()
-        g.hydro_pass()
-        g.synchronize()
-        log(f"warmup {w + 1}/{warmup}: {1e3 * (time.perf_counter() - t0):.1f} ms, "
-            f"density iterations {g.stat('density_iterations'):.0f}")
+    g.run_begin(INTEG)
+    return g
+
+
+def run_job(ic, precision, steps, warmup):
+    if warmup > 0:
+        g = new_job(ic, precision)
+        for _ in range(warmup):
+            g.run_step()
+        g.close()
+    g = new_job(ic, precision)
+    g.synchronize()
     g.timing_enable(True)
     g.timing_reset()
-    g.synchronize()
-    iters = []
+    active, woken = [], []
     t0 = time.perf_counter()
     for _ in range(steps):
-        g.hydro_pass()
-        iters.append(g.stat("density_iterations"))
+        active.append(g.run_step())
+        woken.append(g.timeline()["n_woken"])
     g.synchronize()
     dt = time.perf_counter() - t0
-    n_act = g.stat("n_active")
-    pairs = g.stat("pairs_symmetric")
-    kern = {k: g.timing(k) for k in ("ngb_build", "density", "gradients", "force")}
+    kern = {k: g.timing(k) for k in FAMILIES}
+    res = dict(seconds=dt, active=active, woken=woken, kernels=kern, pairs=g.stat("force_pairs_total"),
+               resorts=g.stat("resorts"), updates=g.stat("active_updates"), tl=g.timeline())
     g.timing_enable(False)
-    state = g.store() if precision == 0 else None
     g.close()
-    return dict(seconds=dt, n_active=n_act, pairs=pairs, kernels=kern, density_iterations=iters, state=state)
+    return res
+
+
+def full_pass(ic, precision, reps=3):
+    """All particles active through one hot-path pass (after the radii converged)."""
+    from gizmo_amd.hydro import Hydro
+    g = Hydro(dict(ic["params"]), len(ic["mass"]), precision)
+    g.load(dict(pos=ic["pos"], vel=ic["vel"], mass=ic["mass"], u=ic["u"], h=ic["h"]))
+    for _ in range(2):
+        g.hydro_pass()
+    g.synchronize()
+    g.timing_enable(True)
+    g.timing_reset()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        g.hydro_pass()
+    g.synchronize()
+    dt = (time.perf_counter() - t0) / reps
+    n_act = g.stat("n_active")
+    kern = {k: round(g.timing(k)[1] / reps, 3) for k in FAMILIES if g.timing(k)[0]}
+    pairs = g.stat("force_pairs_total") / reps
+    g.close()
+    return dict(ms=round(1e3 * dt, 3), updates_per_s=round(n_act / dt, 1), kernel_ms=kern, n_active=n_act, pairs=pairs)
 
 
 def roofline(res, form):
     m = FORCE_MODEL[form]
     launches, total_ms = res["kernels"]["force"]
-    avg_s = total_ms / max(launches, 1) / 1e3
-    bytes_per_launch = res["n_active"] * (m["S_i"] + m["S_out"]) + res["pairs"] * (m["S_j"] + 4)
-    achieved = bytes_per_launch / avg_s / 1e9
+    n_act = sum(res["active"])
+    bytes_total = n_act * (m["S_i"] + m["S_out"]) + res["pairs"] * (m["S_j"] + 4)
+    achieved = bytes_total / (total_ms / 1e3) / 1e9
     return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
             "kernel": f"k_force<6,{'double' if form == 'fp64' else 'float'}>",
-            "bytes_per_launch": int(bytes_per_launch), "avg_launch_ms": round(avg_s * 1e3, 3)}
+            "launches": launches, "bytes_per_launch": int(bytes_total / max(launches, 1)),
+            "avg_launch_ms": round(total_ms / max(launches, 1), 4)}
 
 
-def cpu_baseline(ic, state, n_side, target_s=12.0):
+def cpu_baseline(ic, steps, target_s=20.0):
+    """The oracle (C + OpenMP, FP64) on the same job: run_begin, then sync points until the
+    step budget or the time budget is spent; the rate counts the timed steps only."""
     from oracle import oracle as O
     par = dict(ic["params"])
-    st = O.State(len(ic["mass"]))
-    for k in O.FIELDS:
-        if k in state:
-            getattr(st, k)[...] = state[k]
-    rng = np.random.default_rng(12345)
-    frac = 0.002
-    while True:
-        active = (rng.uniform(size=st.n) < frac).astype(np.uint8)
-        s = st.copy()
-        t0 = time.perf_counter()
-        O.hydro_pass(par, s, active=active)
-        dt = time.perf_counter() - t0
-        log(f"cpu oracle: {int(active.sum())} active particles in {dt:.2f} s")
-        if dt > 0.5 * target_s or frac >= 1.0:
-            break
-        frac = min(1.0, frac * max(2.0, 0.8 * target_s / max(dt, 1e-3)))
-    n_act = int(active.sum())
-    return {"value": round(n_act / dt, 1), "unit": "active-particle updates/s", "cores": O.num_threads(),
-            "kind": "port", "sample": f"{n_act} randomly chosen active particles of the converged {n_side}^3 Sedov state "
-                                      f"(one density+gradients+force pass, {dt:.1f} s, "
-                                      f"{O.num_threads()} OpenMP threads, {physical_cores()} physical cores)"}
+    st = O.State.from_ic(ic)
+    r = O.Run(par, INTEG, st)
+    t0 = time.perf_counter()
+    r.begin()
+    log(f"cpu oracle: run_begin {time.perf_counter() - t0:.1f} s")
+    n_upd, k = 0, 0
+    t0 = time.perf_counter()
+    while k < steps and (time.perf_counter() - t0) < target_s:
+        n_upd += r.step()
+        k += 1
+    dt = time.perf_counter() - t0
+    log(f"cpu oracle: {k} steps, {n_upd} updates in {dt:.1f} s")
+    thr = O.num_threads()
+    return {"value": round(n_upd / dt, 1), "unit": "active-particle updates/s", "cores": thr, "kind": "port",
+            "sample": f"steps 1..{k} of the same {len(st.mass)}-particle Sedov job ({n_upd} active-particle updates in "
+                      f"{dt:.1f} s after the untimed start; {thr} OpenMP threads on {physical_cores()} physical cores)"}
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n-side", type=int, default=128)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-full-pass", action="store_true")
     ap.add_argument("--only", choices=["both", "fp64", "mixed"], default="both")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -146,48 +177,63 @@ def main():
     n = len(ic["mass"])
     log(f"workload: {a.n_side}^3 = {n} particles, built in {time.perf_counter() - t0:.1f} s")
 
-    res = {}
-    if a.only in ("both", "fp64"):
-        log("fp64 ...")
-        res["fp64"] = run_gpu(ic, GZ_FP64, a.steps, a.warmup)
-    if a.only in ("both", "mixed"):
-        log("fp32-mixed ...")
-        res["mixed"] = run_gpu(ic, GZ_FP32_MIXED, a.steps, a.warmup)
-    head = "mixed" if "mixed" in res else "fp64"
+    forms = {"fp64": GZ_FP64, "mixed": GZ_FP32_MIXED}
+    todo = ["fp64", "mixed"] if a.only == "both" else [a.only]
+    res, fp = {}, {}
+    for f in todo:
+        log(f"{f}: {a.steps} steps ...")
+        res[f] = run_job(ic, forms[f], a.steps, a.warmup)
+        r = res[f]
+        log(f"{f}: {1e3 * r['seconds']:.1f} ms, {sum(r['active'])} updates, resorts {r['resorts']:.0f}, "
+            f"kernels {{{', '.join(f'{k}: {v[1]:.1f}' for k, v in r['kernels'].items())}}} ms")
+        if not a.no_full_pass:
+            fp[f] = full_pass(ic, forms[f])
+            log(f"{f}: full pass {fp[f]['ms']} ms, {fp[f]['updates_per_s'] / 1e6:.1f} M updates/s")
+
+    def summary(r):
+        ups = sum(r["active"]) / r["seconds"]
+        return {"value": round(ups, 1), "ms_per_step": round(1e3 * r["seconds"] / a.steps, 3),
+                "updates": int(sum(r["active"])),
+                "kernel_ms": {k: round(v[1], 3) for k, v in r["kernels"].items() if v[0]}}
+
+    head = "fp64" if "fp64" in res else "mixed"
     r = res[head]
-    ups = r["n_active"] * a.steps / r["seconds"]
     out = {
         "metric": METRIC,
-        "value": round(ups, 1),
+        "value": summary(r)["value"],
         "unit": "active-particle updates/s",
         "n_gpus": 1,
         "steps": a.steps,
         "warmup": a.warmup,
-        "ms_per_step": round(1e3 * r["seconds"] / a.steps, 3),
+        "ms_per_step": summary(r)["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32-mixed (fp64 positions, 3x3 inverse, h test)" if head == "mixed" else "f64",
+        "dtype": "f64" if head == "fp64" else "fp32-mixed (fp64 positions, 3x3 inverse, h test)",
         "data": "synthetic: seeded Sedov blast in a jittered lattice (numpy Philox)",
-        "config": {"workload": f"sedov_{a.n_side}^3_wendland_c2_64ngb_all_active_hot_path_pass", "n_particles": n,
-                   "kernel": "wendland_c2", "des_num_ngb": 64,
-                   "step": "neighbour builds + density (all radius iterations) + gradients + force + final ops"},
-        "density_iterations_per_step": r["density_iterations"],
-        "kernel_ms_per_step": {k: round(v[1] / a.steps, 3) for k, v in r["kernels"].items()},
-        "pairs_symmetric": r["pairs"],
+        "config": {"workload": f"sedov_{a.n_side}^3_wendland_c2_64ngb_adaptive_timebins_{a.steps}_steps",
+                   "n_particles": n, "kernel": "wendland_c2", "des_num_ngb": 64, "integ": INTEG,
+                   "step": "one sync point: timesteps + wake-ups, kick, drift, neighbour builds, density (all radius "
+                           "iterations), gradients, force, final ops, kick"},
+        "active_per_step": r["active"],
+        "woken_per_step": r["woken"],
+        "resorts": r["resorts"],
+        "kernel_ms": summary(r)["kernel_ms"],
         "roofline": roofline(r, head),
         "p50_ms": None, "p99_ms": None,
     }
+    for f in res:
+        if f != head:
+            out[f] = dict(summary(res[f]), roofline=roofline(res[f], f))
     if "fp64" in res and "mixed" in res:
-        f = res["fp64"]
-        ups64 = f["n_active"] * a.steps / f["seconds"]
-        out["fp64"] = {"value": round(ups64, 1), "ms_per_step": round(1e3 * f["seconds"] / a.steps, 3),
-                       "kernel_ms_per_step": {k: round(v[1] / a.steps, 3) for k, v in f["kernels"].items()},
-                       "roofline": roofline(f, "fp64")}
-        out["mixed_over_fp64"] = round(ups / ups64, 3)
-    if not a.no_cpu and "fp64" in res:
+        out["mixed_over_fp64"] = round(summary(res["mixed"])["value"] / summary(res["fp64"])["value"], 3)
+    if fp:
+        out["full_pass"] = fp
+        if "fp64" in fp and "mixed" in fp:
+            out["full_pass_mixed_over_fp64"] = round(fp["mixed"]["updates_per_s"] / fp["fp64"]["updates_per_s"], 3)
+    if not a.no_cpu:
         log("cpu baseline ...")
-        out["cpu_baseline"] = cpu_baseline(ic, res["fp64"]["state"], a.n_side)
+        out["cpu_baseline"] = cpu_baseline(ic, a.steps)
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out), flush=True)

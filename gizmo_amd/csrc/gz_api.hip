@@ -42,7 +42,7 @@ struct gz_context {
     double *X = nullptr, *Y = nullptr, *Z = nullptr, *VX = nullptr, *VY = nullptr, *VZ = nullptr;
     double *M = nullptr, *H = nullptr, *U = nullptr, *NN = nullptr, *DRK = nullptr, *DIVV = nullptr;
     double *RHO = nullptr, *PR = nullptr, *T = nullptr, *FW = nullptr, *CN = nullptr, *FCE = nullptr;
-    double *GR = nullptr, *GP = nullptr, *GV = nullptr, *ACC = nullptr, *DTU = nullptr, *MSV = nullptr, *MSV_IN = nullptr;
+    double *GR = nullptr, *GP = nullptr, *GV = nullptr, *ACC = nullptr, *DTU = nullptr, *MSV = nullptr;
     int *TB = nullptr, *WAKE = nullptr;
     uint8_t *ACT = nullptr;   // active flag per particle (device order)
     // timestep-loop fields: kicked velocity and energy, softening cache, step start and length
@@ -53,6 +53,14 @@ struct gz_context {
     double *X0 = nullptr, *Y0 = nullptr, *Z0 = nullptr;
     double dmax = 0, last_rmax = 0;
     double resort_fraction = 0.125;  // order again once dmax exceeds this fraction of the search radius
+    int few_rows = 8192;             // at most this many active rows: one wavefront per row
+    // the cell ranges and stencils depend only on the order (key_version) and the level: they
+    // are kept across calls while both stay the same
+    long key_version = 0, grid_version = -1;
+    // an upper bound on every kernel radius (< 0: unknown, reduce over all particles)
+    double hmax_bound = -1;
+    uint8_t *rec_need = nullptr;     // records a pass reads (its rows and their neighbours)
+    bool rec_need_valid = false;
     bool all_active = true;
     int *perm = nullptr;      // device order -> caller index
     uint32_t *key = nullptr;  // fine Morton key, device order
@@ -89,6 +97,8 @@ struct gz_context {
     // occupied cells of the active rows at the current grid level
     int *ca0 = nullptr, *ca1 = nullptr, *cells = nullptr, *cflag = nullptr, *cscan = nullptr;
     int ncells = 0, rows_level = -1;
+    bool ncells_exact = true;
+    long act_version = 0, rows_act_version = -1, rows_key_version = -1;
     uint8_t *need = nullptr;
 
     // per-pass neighbour records
@@ -99,7 +109,8 @@ struct gz_context {
     double *it_h = nullptr, *it_hb = nullptr, *it_l = nullptr, *it_r = nullptr, *it_cn = nullptr;
     int *it_ci = nullptr, *it_it = nullptr, *todo = nullptr, *todo2 = nullptr;
     int *counters = nullptr;   // device: [0] next count, [1] rebuild flag, [2] error, [3] overflow,
-                               // [4] wake-up flagged, [5] woken count, [6] timestep stop code
+                               // [4] wake-up flagged, [5] woken count, [6] timestep stop code,
+                               // [8..9] wake-up bin limits, [10] active count
     int *hcounters = nullptr;  // pinned host mirror
     unsigned long long *u64c = nullptr;   // device: [0..63] bin histogram, [64] displacement bits, [65] force pairs
     unsigned long long *hu64c = nullptr;  // pinned host mirror
@@ -268,7 +279,6 @@ int alloc_state(gz_context *ctx, long cap)
     CHK(dalloc(ctx, &ctx->FW, 3 * n)); CHK(dalloc(ctx, &ctx->CN, n)); CHK(dalloc(ctx, &ctx->FCE, n));
     CHK(dalloc(ctx, &ctx->GR, 3 * n)); CHK(dalloc(ctx, &ctx->GP, 3 * n)); CHK(dalloc(ctx, &ctx->GV, 9 * n));
     CHK(dalloc(ctx, &ctx->ACC, 3 * n)); CHK(dalloc(ctx, &ctx->DTU, n)); CHK(dalloc(ctx, &ctx->MSV, n));
-    CHK(dalloc(ctx, &ctx->MSV_IN, n));
     CHK(dalloc(ctx, &ctx->TB, n)); CHK(dalloc(ctx, &ctx->WAKE, n)); CHK(dalloc(ctx, &ctx->ACT, n));
     CHK(dalloc(ctx, &ctx->perm, n)); CHK(dalloc(ctx, &ctx->key, n)); CHK(dalloc(ctx, &ctx->key2, n));
     CHK(dalloc(ctx, &ctx->ival, n)); CHK(dalloc(ctx, &ctx->ival2, n)); CHK(dalloc(ctx, &ctx->iflag, n + 1));
@@ -286,10 +296,11 @@ int alloc_state(gz_context *ctx, long cap)
     CHK(dalloc(ctx, &ctx->VKX, n)); CHK(dalloc(ctx, &ctx->VKY, n)); CHK(dalloc(ctx, &ctx->VKZ, n));
     CHK(dalloc(ctx, &ctx->UK, n)); CHK(dalloc(ctx, &ctx->SOFT, n)); CHK(dalloc(ctx, &ctx->TIB, n)); CHK(dalloc(ctx, &ctx->DTS, n));
     CHK(dalloc(ctx, &ctx->X0, n)); CHK(dalloc(ctx, &ctx->Y0, n)); CHK(dalloc(ctx, &ctx->Z0, n));
-    CHK(dalloc(ctx, &ctx->counters, 8));
+    CHK(dalloc(ctx, &ctx->rec_need, n));
+    CHK(dalloc(ctx, &ctx->counters, 16));
     CHK(dalloc(ctx, &ctx->u64c, 80));
     if (!ctx->hcounters) {
-        hipError_t e = hipHostMalloc((void **)&ctx->hcounters, 8 * sizeof(int), hipHostMallocDefault);
+        hipError_t e = hipHostMalloc((void **)&ctx->hcounters, 16 * sizeof(int), hipHostMallocDefault);
         if (e != hipSuccess) return fail(ctx, GZ_E_MEMORY, "hipHostMalloc failed");
     }
     if (!ctx->hu64c) {
@@ -305,7 +316,7 @@ void free_state(gz_context *c)
     dfree(c->X); dfree(c->Y); dfree(c->Z); dfree(c->VX); dfree(c->VY); dfree(c->VZ); dfree(c->M); dfree(c->H);
     dfree(c->U); dfree(c->NN); dfree(c->DRK); dfree(c->DIVV); dfree(c->RHO); dfree(c->PR); dfree(c->T); dfree(c->FW);
     dfree(c->CN); dfree(c->FCE); dfree(c->GR); dfree(c->GP); dfree(c->GV); dfree(c->ACC); dfree(c->DTU); dfree(c->MSV);
-    dfree(c->MSV_IN); dfree(c->TB); dfree(c->WAKE); dfree(c->ACT); dfree(c->perm); dfree(c->key); dfree(c->key2);
+    dfree(c->TB); dfree(c->WAKE); dfree(c->ACT); dfree(c->perm); dfree(c->key); dfree(c->key2);
     dfree(c->ival); dfree(c->ival2); dfree(c->iflag); dfree(c->iscan); dfree(c->act); dfree(c->stage);
     dfree(c->it_h); dfree(c->it_hb); dfree(c->it_l); dfree(c->it_r); dfree(c->it_cn); dfree(c->it_ci); dfree(c->it_it);
     dfree(c->todo); dfree(c->todo2); dfree(c->cnt_d); dfree(c->cnt_s); dfree(c->counters); dfree(c->ell_d); dfree(c->ell_s);
@@ -313,7 +324,7 @@ void free_state(gz_context *c)
     dfree(c->ca0); dfree(c->ca1); dfree(c->cells); dfree(c->cflag); dfree(c->cscan); dfree(c->need);
     dfree(c->rec_grad); dfree(c->rec_force); dfree(c->dacc);
     dfree(c->VKX); dfree(c->VKY); dfree(c->VKZ); dfree(c->UK); dfree(c->SOFT); dfree(c->TIB); dfree(c->DTS);
-    dfree(c->X0); dfree(c->Y0); dfree(c->Z0); dfree(c->u64c);
+    dfree(c->X0); dfree(c->Y0); dfree(c->Z0); dfree(c->u64c); dfree(c->rec_need);
     if (c->hu64c) hipHostFree(c->hu64c);
     c->hu64c = nullptr;
     if (c->tmp) hipFree(c->tmp);
@@ -375,7 +386,7 @@ int copy_dev(gz_context *ctx, void *dst, const void *src, size_t bytes)
 }
 
 // grid of level L over the current device order; the stencil of each cell
-int build_grid(gz_context *ctx, double rmax)
+int build_grid(gz_context *ctx, double rmax, bool need_chmax = false)
 {
     const double box = ctx->par.box_size;
     // cell side >= the largest search radius plus twice the distance any particle has moved
@@ -387,21 +398,29 @@ int build_grid(gz_context *ctx, double rmax)
     int L = 0;
     while (L < kFineLevel && box / (double)(1 << (L + 1)) >= reach * (1.0 + 1e-6)) L++;
     const long ncell = 1L << (3 * L);
+    if (L == ctx->level && ctx->grid_version == ctx->key_version && !need_chmax) return 0;
     if (ncell > ctx->grid_cells_alloc) {
         CHK(dalloc(ctx, &ctx->cstart, ncell)); CHK(dalloc(ctx, &ctx->cend, ncell)); CHK(dalloc(ctx, &ctx->nsten, ncell));
         CHK(dalloc(ctx, &ctx->sten, ncell * 27)); CHK(dalloc(ctx, &ctx->chmax, ncell));
         CHK(dalloc(ctx, &ctx->ca0, ncell)); CHK(dalloc(ctx, &ctx->ca1, ncell));
         ctx->grid_cells_alloc = ncell;
+        ctx->grid_version = -1;
     }
-    ctx->level = L;
-    ctx->rows_level = -1;
-    ctx->cell = box / (double)(1 << L);
     Timed t(ctx, "ngb_build");
-    k_cell_clear<<<nblk(ncell), kBlock, 0, ctx->stream>>>(ncell, ctx->cstart, ctx->cend, ctx->chmax);
-    k_cell_bounds<<<nblk(ctx->n), kBlock, 0, ctx->stream>>>(ctx->n, ctx->key, 3 * (kFineLevel - L), ctx->cstart, ctx->cend);
-    k_cell_hmax<<<nblk(ncell), kBlock, 0, ctx->stream>>>(ncell, ctx->cstart, ctx->cend, ctx->H, ctx->chmax);
-    k_cell_stencil<<<nblk(ncell), kBlock, 0, ctx->stream>>>(L, ctx->cstart, ctx->cend, ctx->sten, ctx->nsten);
-    HIPCHK(hipGetLastError());
+    if (L != ctx->level || ctx->grid_version != ctx->key_version) {
+        ctx->level = L;
+        ctx->rows_level = -1;
+        ctx->cell = box / (double)(1 << L);
+        k_cell_clear<<<nblk(ncell), kBlock, 0, ctx->stream>>>(ncell, ctx->cstart, ctx->cend, ctx->chmax);
+        k_cell_bounds<<<nblk(ctx->n), kBlock, 0, ctx->stream>>>(ctx->n, ctx->key, 3 * (kFineLevel - L), ctx->cstart, ctx->cend);
+        k_cell_stencil<<<nblk(ncell), kBlock, 0, ctx->stream>>>(L, ctx->cstart, ctx->cend, ctx->sten, ctx->nsten);
+        HIPCHK(hipGetLastError());
+        ctx->grid_version = ctx->key_version;
+    }
+    if (need_chmax) {
+        k_cell_hmax<<<nblk(ncell), kBlock, 0, ctx->stream>>>(ncell, ctx->cstart, ctx->cend, ctx->H, ctx->chmax);
+        HIPCHK(hipGetLastError());
+    }
     return 0;
 }
 
@@ -416,6 +435,7 @@ GridView grid_view(gz_context *ctx)
     g.sten = ctx->sten;
     g.nsten = ctx->nsten;
     g.chmax = ctx->chmax;
+    g.margin = ctx->dmax;
     return g;
 }
 
@@ -423,6 +443,8 @@ GridView grid_view(gz_context *ctx)
 int build_cell_rows(gz_context *ctx)
 {
     const int na = ctx->nact;
+    if (ctx->rows_level == ctx->level && ctx->rows_act_version == ctx->act_version && ctx->rows_key_version == ctx->key_version)
+        return 0;
     const int shift = 3 * (kFineLevel - ctx->level);
     k_cell_rows_mark<<<nblk(na), kBlock, 0, ctx->stream>>>(na, ctx->act, ctx->key, shift, ctx->ca0, ctx->ca1, ctx->cflag);
     HIPCHK(hipGetLastError());
@@ -433,18 +455,30 @@ int build_cell_rows(gz_context *ctx)
     HIPCHK(hipcub::DeviceScan::ExclusiveSum(ctx->tmp, bytes, ctx->cflag, ctx->cscan, na + 1, ctx->stream));
     k_cell_rows_list<<<nblk(na), kBlock, 0, ctx->stream>>>(na, ctx->act, ctx->key, shift, ctx->cflag, ctx->cscan, ctx->cells);
     HIPCHK(hipGetLastError());
-    int nc = 0;
-    HIPCHK(hipMemcpyAsync(&nc, ctx->cscan + na, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
-    HIPCHK(hipStreamSynchronize(ctx->stream));
-    ctx->ncells = nc;
     ctx->rows_level = ctx->level;
+    ctx->rows_act_version = ctx->act_version;
+    ctx->rows_key_version = ctx->key_version;
+    if (na <= ctx->few_rows) {
+        // few rows: launch one wavefront per row at most; the walks read the cell count on the device
+        ctx->ncells = na;
+        ctx->ncells_exact = false;
+        return 0;
+    }
+    HIPCHK(hipMemcpyAsync(ctx->hcounters + 11, ctx->cscan + na, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    ctx->ncells = ctx->hcounters[11];
+    ctx->ncells_exact = true;
     return 0;
 }
 
-CellRows cell_rows(gz_context *ctx) { return CellRows{ctx->cells, ctx->ncells, ctx->ca0, ctx->ca1, ctx->nact}; }
+CellRows cell_rows(gz_context *ctx)
+{
+    return CellRows{ctx->cells, ctx->ncells, ctx->ncells_exact ? nullptr : ctx->cscan + ctx->nact, ctx->ca0, ctx->ca1, ctx->nact};
+}
 
 int read_counters(gz_context *ctx)
 {
+    HIPCHK(hipMemcpyAsync(ctx->hu64c + 64, ctx->u64c + 64, 8 * sizeof(unsigned long long), hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipMemcpyAsync(ctx->hcounters, ctx->counters, 8 * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     return 0;
@@ -506,10 +540,11 @@ int build_active(gz_context *ctx)
     HIPCHK(hipcub::DeviceScan::ExclusiveSum(ctx->tmp, bytes, ctx->iflag, ctx->iscan, (int)(n + 1), ctx->stream));
     k_compact<<<nblk(n), kBlock, 0, ctx->stream>>>(n, ctx->iflag, ctx->iscan, ctx->act);
     HIPCHK(hipGetLastError());
-    int na = 0;
-    HIPCHK(hipMemcpyAsync(&na, ctx->iscan + n, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipMemcpyAsync(ctx->hcounters + 10, ctx->iscan + n, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
+    const int na = ctx->hcounters[10];
     ctx->nact = na;
+    ctx->act_version++;
     ctx->stats["n_active"] = na;
     return 0;
 }
@@ -526,17 +561,6 @@ __global__ void k_iota(int n, int *out)
     if (i < n) out[i] = i;
 }
 
-// density()'s pre-iteration radius clamp, src/hydro/density_loop.cc:818-838
-__global__ void k_clamp_h(int na, const int *act, const double *M, double *H, double hclamp)
-{
-    int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= na) return;
-    int i = act[r];
-    double h = H[i];
-    if ((h < 0) || !isfinite(h) || (h > hclamp)) H[i] = hclamp;
-    (void)M;
-}
-
 namespace {
 
 template <int KERN, typename R>
@@ -546,20 +570,19 @@ int density_stage(gz_context *ctx)
     ctx->stats["density_iterations"] = 0;
     ctx->stats["ngb_rebuilds"] = 0;
     if (na == 0) { ctx->density_done = true; ctx->sym_valid = false; return 0; }
-    HIPCHK(hipMemsetAsync(ctx->it_l, 0, na * sizeof(double), ctx->stream));
-    HIPCHK(hipMemsetAsync(ctx->it_r, 0, na * sizeof(double), ctx->stream));
-    HIPCHK(hipMemsetAsync(ctx->it_cn, 0, na * sizeof(double), ctx->stream));
-    HIPCHK(hipMemsetAsync(ctx->it_ci, 0, na * sizeof(int), ctx->stream));
-    HIPCHK(hipMemsetAsync(ctx->it_it, 0, na * sizeof(int), ctx->stream));
-    HIPCHK(hipMemsetAsync(ctx->need, 1, na, ctx->stream));
-    HIPCHK(hipMemsetAsync(ctx->counters, 0, 8 * sizeof(int), ctx->stream));
-    k_clamp_h<<<nblk(na), kBlock, 0, ctx->stream>>>(na, ctx->act, ctx->M, ctx->H, 0.99 * ctx->par.max_kernel_radius);
-    k_gather<<<nblk(na), kBlock, 0, ctx->stream>>>(na, 1, ctx->act, ctx->H, ctx->it_h, 0);
+    // one launch: the radius clamp, the iteration state, the counters, and the largest radius
+    HIPCHK(hipMemsetAsync(ctx->u64c + 67, 0, 2 * sizeof(unsigned long long), ctx->stream));
+    k_density_init<<<nblk(na), kBlock, 0, ctx->stream>>>(na, ctx->act, ctx->H, 0.99 * ctx->par.max_kernel_radius, ctx->it_h, ctx->it_l,
+                                                         ctx->it_r, ctx->it_cn, ctx->it_ci, ctx->it_it, ctx->need, ctx->counters,
+                                                         ctx->u64c + 67);
     HIPCHK(hipGetLastError());
     double hmax = 0;
-    CHK(max_on_device(ctx, ctx->it_h, na, &hmax));
+    HIPCHK(hipMemcpyAsync(ctx->hu64c + 67, ctx->u64c + 67, sizeof(unsigned long long), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    memcpy(&hmax, &ctx->hu64c[67], sizeof hmax);
     CHK(build_grid(ctx, std::min(hmax, 0.5 * ctx->par.box_size)));
-    CHK(build_cell_rows(ctx));
+    const bool few = na <= ctx->few_rows;
+    if (!few) CHK(build_cell_rows(ctx));
     DensityIter it{ctx->it_h, ctx->it_hb, ctx->it_l, ctx->it_r, ctx->it_cn, ctx->it_ci, ctx->it_it};
     DensityOut out{ctx->n, ctx->H, ctx->NN, ctx->DRK, ctx->DIVV, ctx->RHO, ctx->PR, ctx->T, ctx->FW, ctx->CN, ctx->FCE};
     int iters = 0;
@@ -567,12 +590,17 @@ int density_stage(gz_context *ctx)
         HIPCHK(hipMemsetAsync(ctx->counters, 0, 3 * sizeof(int), ctx->stream));
         {
             Timed t(ctx, "density");
-            k_density_cells<KERN, R><<<xcd_grid(ctx->ncells), 64, 0, ctx->stream>>>(
-                ctx->dp, grid_view(ctx), cell_rows(ctx), ctx->act, ctx->need, ctx->X, ctx->Y, ctx->Z, ctx->VX, ctx->VY, ctx->VZ,
-                ctx->M, ctx->it_h, ctx->dacc);
+            if (few)
+                k_density_rows<KERN, R><<<xcd_grid(na), 64, 0, ctx->stream>>>(ctx->dp, grid_view(ctx), na, ctx->act, ctx->need, ctx->key,
+                                                                            ctx->X, ctx->Y, ctx->Z, ctx->VX, ctx->VY, ctx->VZ, ctx->M,
+                                                                            ctx->it_h, ctx->dacc);
+            else
+                k_density_cells<KERN, R><<<xcd_grid(ctx->ncells), 64, 0, ctx->stream>>>(
+                    ctx->dp, grid_view(ctx), cell_rows(ctx), ctx->act, ctx->need, ctx->X, ctx->Y, ctx->Z, ctx->VX, ctx->VY, ctx->VZ,
+                    ctx->M, ctx->it_h, ctx->dacc);
             k_density_after<<<nblk(na), kBlock, 0, ctx->stream>>>(ctx->dp, na, ctx->act, ctx->need, ctx->dacc, ctx->M, ctx->U,
                                                                   ctx->cell - 2.0 * ctx->dmax, it, out, ctx->counters + 0, ctx->counters + 1,
-                                                                  ctx->counters + 2);
+                                                                  ctx->counters + 2, ctx->u64c + 68);
             HIPCHK(hipGetLastError());
         }
         CHK(read_counters(ctx));
@@ -589,7 +617,7 @@ int density_stage(gz_context *ctx)
             // a radius outgrew the cell side: a coarser grid for the rows still iterating
             CHK(max_on_device(ctx, ctx->it_h, na, &hmax));
             CHK(build_grid(ctx, std::min(hmax, 0.5 * ctx->par.box_size)));
-            CHK(build_cell_rows(ctx));
+            if (!few) CHK(build_cell_rows(ctx));
             ctx->stats["ngb_rebuilds"] += 1;
         }
     }
@@ -597,16 +625,27 @@ int density_stage(gz_context *ctx)
     ctx->density_done = true;
     ctx->gradients_done = false;
     ctx->sym_valid = false;
+    if (ctx->hmax_bound > 0) {
+        // the largest final radius of the rows, read back with the last round's counters
+        double hf;
+        memcpy(&hf, &ctx->hu64c[68], sizeof hf);
+        ctx->hmax_bound = std::max(ctx->hmax_bound, hf);
+    }
     return 0;
 }
 
 int ensure_sym(gz_context *ctx)
 {
     if (ctx->sym_valid) return 0;
-    double hmax = 0;
-    CHK(max_on_device(ctx, ctx->H, ctx->n, &hmax));
-    CHK(build_grid(ctx, std::min(hmax, 0.5 * ctx->par.box_size)));
-    CHK(build_cell_rows(ctx));
+    double hmax = ctx->hmax_bound;
+    if (!(hmax > 0)) {
+        CHK(max_on_device(ctx, ctx->H, ctx->n, &hmax));
+        ctx->hmax_bound = hmax;
+    }
+    const bool few = ctx->nact <= ctx->few_rows;
+    // the row walks drop stencil cells beyond reach: they need each cell's largest radius
+    CHK(build_grid(ctx, std::min(hmax, 0.5 * ctx->par.box_size), few));
+    if (!few) CHK(build_cell_rows(ctx));
     const long need_ld = ((long)ctx->nact + 63) / 64 * 64;
     const int cap_hint = std::max(64, (int)std::ceil(ctx->par.des_num_ngb * 2.0) + 32);
     if (ctx->ell_s == nullptr || ctx->ld_s != need_ld || ctx->cap_s < cap_hint) {
@@ -618,14 +657,33 @@ int ensure_sym(gz_context *ctx)
         HIPCHK(hipMemsetAsync(ctx->counters + 3, 0, sizeof(int), ctx->stream));
         {
             Timed t(ctx, "ngb_build");
-            k_build_sym_cells<<<xcd_grid(ctx->ncells), 64, 0, ctx->stream>>>(grid_view(ctx), cell_rows(ctx), ctx->act, ctx->X, ctx->Y,
-                                                                              ctx->Z, ctx->H, ctx->ell_s, ctx->ld_s, ctx->cap_s,
-                                                                              ctx->cnt_s, ctx->counters + 3);
+            if (few)
+                k_build_sym_rows<<<xcd_grid(ctx->nact), 64, 0, ctx->stream>>>(grid_view(ctx), ctx->nact, ctx->act, ctx->key, ctx->X, ctx->Y,
+                                                                               ctx->Z, ctx->H, ctx->ell_s, ctx->ld_s, ctx->cap_s, ctx->cnt_s,
+                                                                               ctx->counters + 3);
+            else
+                k_build_sym_cells<<<xcd_grid(ctx->ncells), 64, 0, ctx->stream>>>(grid_view(ctx), cell_rows(ctx), ctx->act, ctx->X, ctx->Y,
+                                                                                  ctx->Z, ctx->H, ctx->ell_s, ctx->ld_s, ctx->cap_s,
+                                                                                  ctx->cnt_s, ctx->counters + 3);
             HIPCHK(hipGetLastError());
         }
         CHK(read_counters(ctx));
         const int over = ctx->hcounters[3];
-        if (over == 0) { ctx->sym_valid = true; return 0; }
+        if (over == 0) {
+            ctx->sym_valid = true;
+            // the records the gradient and force passes read: every record when many rows are
+            // active, else only those of the rows and their neighbours
+            ctx->rec_need_valid = false;
+            if ((long)ctx->nact * 4 < ctx->n) {
+                Timed t2(ctx, "ngb_build");
+                HIPCHK(hipMemsetAsync(ctx->rec_need, 0, ctx->n, ctx->stream));
+                k_mark_records<<<nblk(ctx->nact), kBlock, 0, ctx->stream>>>(ctx->nact, ctx->act, ctx->ell_s, ctx->ld_s, ctx->cnt_s,
+                                                                            ctx->rec_need);
+                HIPCHK(hipGetLastError());
+                ctx->rec_need_valid = true;
+            }
+            return 0;
+        }
         ctx->cap_s = std::max(over + 16, ctx->cap_s * 3 / 2);
         CHK(dalloc(ctx, &ctx->ell_s, (size_t)ctx->cap_s * (size_t)ctx->ld_s));
         ctx->stats["ell_widen"] += 1;
@@ -641,10 +699,16 @@ int gradients_stage(gz_context *ctx)
     CHK(ensure_sym(ctx));
     Timed t(ctx, "gradients");
     GradRec<R> *rec = (GradRec<R> *)ctx->rec_grad;
-    k_pack_grad<R><<<nblk(ctx->n), kBlock, 0, ctx->stream>>>(ctx->n, ctx->H, ctx->M, ctx->RHO, ctx->PR, ctx->VX, ctx->VY, ctx->VZ, rec);
-    k_gradients<KERN, R><<<xcd_grid(nblk(ctx->nact)), kBlock, 0, ctx->stream>>>(ctx->dp, ctx->nact, ctx->act, ctx->X, ctx->Y, ctx->Z,
-                                                                                rec, ctx->T, ctx->CN, ctx->n, ctx->ell_s, ctx->ld_s,
-                                                                                ctx->cnt_s, ctx->GR, ctx->GP, ctx->GV);
+    k_pack_grad<R><<<nblk(ctx->n), kBlock, 0, ctx->stream>>>(ctx->n, ctx->rec_need_valid ? ctx->rec_need : nullptr, ctx->H, ctx->M,
+                                                             ctx->RHO, ctx->PR, ctx->VX, ctx->VY, ctx->VZ, rec);
+    if (ctx->nact <= ctx->few_rows)
+        k_gradients_rows<KERN, R><<<xcd_grid(ctx->nact), 64, 0, ctx->stream>>>(ctx->dp, ctx->nact, ctx->act, ctx->X, ctx->Y, ctx->Z,
+                                                                               rec, ctx->T, ctx->CN, ctx->n, ctx->ell_s, ctx->ld_s,
+                                                                               ctx->cnt_s, ctx->GR, ctx->GP, ctx->GV);
+    else
+        k_gradients<KERN, R><<<xcd_grid(nblk(ctx->nact)), kBlock, 0, ctx->stream>>>(ctx->dp, ctx->nact, ctx->act, ctx->X, ctx->Y, ctx->Z,
+                                                                                    rec, ctx->T, ctx->CN, ctx->n, ctx->ell_s, ctx->ld_s,
+                                                                                    ctx->cnt_s, ctx->GR, ctx->GP, ctx->GV);
     HIPCHK(hipGetLastError());
     ctx->gradients_done = true;
     return 0;
@@ -656,7 +720,6 @@ int force_stage(gz_context *ctx)
     if (!ctx->gradients_done) return fail(ctx, GZ_E_STATE, "hydro_force() before hydro_gradient_calc()");
     if (ctx->nact == 0) return 0;
     CHK(ensure_sym(ctx));
-    HIPCHK(hipMemcpyAsync(ctx->MSV_IN, ctx->MSV, ctx->n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
     HIPCHK(hipMemsetAsync(ctx->counters + 2, 0, sizeof(int), ctx->stream));
     HIPCHK(hipMemsetAsync(ctx->counters + 4, 0, sizeof(int), ctx->stream));
     ForceView F{ctx->n, ctx->X, ctx->Y, ctx->Z, ctx->VX, ctx->VY, ctx->VZ, ctx->M, ctx->H, ctx->U, ctx->RHO, ctx->PR, ctx->CN,
@@ -664,11 +727,17 @@ int force_stage(gz_context *ctx)
     {
         Timed t(ctx, "force");
         Cell<R> *rec = (Cell<R> *)ctx->rec_force;
-        k_pack_force<R><<<nblk(ctx->n), kBlock, 0, ctx->stream>>>(ctx->par.gamma, F, rec);
-        k_force<KERN, R><<<xcd_grid(nblk(ctx->nact)), kBlock, 0, ctx->stream>>>(
-            ctx->dp, gam<R>(ctx->dp), ctx->nact, ctx->act, F, rec, ctx->ell_s, ctx->ld_s, ctx->cnt_s,
-            ctx->all_active ? nullptr : ctx->ACT, ctx->TB, ctx->MSV_IN, ctx->WAKE, ctx->ACC, ctx->DTU, ctx->MSV, ctx->counters + 2,
-            ctx->counters + 4, ctx->u64c + 65);
+        k_pack_force<R><<<nblk(ctx->n), kBlock, 0, ctx->stream>>>(ctx->par.gamma, F, ctx->rec_need_valid ? ctx->rec_need : nullptr, rec);
+        if (ctx->nact <= ctx->few_rows)
+            k_force_rows<KERN, R><<<xcd_grid(ctx->nact), 64, 0, ctx->stream>>>(
+                ctx->dp, gam<R>(ctx->dp), ctx->nact, ctx->act, F, rec, ctx->ell_s, ctx->ld_s, ctx->cnt_s,
+                ctx->all_active ? nullptr : ctx->ACT, ctx->TB, ctx->MSV, ctx->WAKE, ctx->ACC, ctx->DTU, ctx->MSV, ctx->counters + 2,
+                ctx->counters + 4, ctx->u64c + 65);
+        else
+            k_force<KERN, R><<<xcd_grid(nblk(ctx->nact)), kBlock, 0, ctx->stream>>>(
+                ctx->dp, gam<R>(ctx->dp), ctx->nact, ctx->act, F, rec, ctx->ell_s, ctx->ld_s, ctx->cnt_s,
+                ctx->all_active ? nullptr : ctx->ACT, ctx->TB, ctx->MSV, ctx->WAKE, ctx->ACC, ctx->DTU, ctx->MSV, ctx->counters + 2,
+                ctx->counters + 4, ctx->u64c + 65);
         HIPCHK(hipGetLastError());
     }
     CHK(read_counters(ctx));
@@ -731,6 +800,7 @@ int resort(gz_context *ctx)
     CHK(copy_dev(ctx, ctx->ACT, ctx->stage, n));
     CHK(copy_dev(ctx, ctx->X0, ctx->X, n * 8)); CHK(copy_dev(ctx, ctx->Y0, ctx->Y, n * 8)); CHK(copy_dev(ctx, ctx->Z0, ctx->Z, n * 8));
     ctx->dmax = 0;
+    ctx->key_version++;
     ctx->sym_valid = false;
     ctx->rows_level = -1;
     ctx->stats["resorts"] += 1;
@@ -838,31 +908,25 @@ int run_step(gz_context *ctx)
             HIPCHK(hipGetLastError());
         }
     }
-    CHK(read_bins(ctx));
-    // process_wake_ups (src/core/timestep.cc:1565-1774)
-    ctx->n_woken = 0;
-    {
-        const long long ti_next = next_kick(ctx);
-        int max_time_bin_active = 0;
-        for (int b = 0; b < kTimebins; b++)
-            if ((ti_next % (1LL << b)) == 0) max_time_bin_active = b;
+    // the bin occupation, and process_wake_ups (src/core/timestep.cc:1565-1774) on the device;
+    // one read-back of the occupation and the woken count
+    HIPCHK(hipMemsetAsync(ctx->u64c, 0, 64 * sizeof(unsigned long long), ctx->stream));
+    k_bin_hist<<<std::min(nblk(n), 1024u), kBlock, 0, ctx->stream>>>(n, ctx->TB, ctx->u64c);
+    HIPCHK(hipGetLastError());
+    if (ctx->need_wakeup) {
         int wakeup_bin_offset = 0;
         while ((1LL << wakeup_bin_offset) < (long long)kWakeup) wakeup_bin_offset++;
-        if (ctx->need_wakeup) {
-            int lowest = kTimebins;
-            for (int b = 0; b < kTimebins; b++)
-                if (((ctx->mask >> b) & 1ull) && ctx->bin_count[b] > 0) { lowest = b; break; }
-            {
-                Timed t(ctx, "timeline");
-                k_wakeups<<<nblk(n), kBlock, 0, ctx->stream>>>(T, S, lowest, max_time_bin_active, wakeup_bin_offset, ctx->counters + 5);
-                HIPCHK(hipGetLastError());
-            }
-            CHK(read_counters(ctx));
-            ctx->n_woken = ctx->hcounters[5];
-            if (ctx->n_woken) CHK(read_bins(ctx));
-        }
-        ctx->need_wakeup = 0;
+        Timed t(ctx, "timeline");
+        k_wakeup_limits<<<1, 64, 0, ctx->stream>>>(T, ctx->u64c, ctx->counters + 8);
+        k_wakeups<<<nblk(n), kBlock, 0, ctx->stream>>>(T, S, ctx->counters + 8, wakeup_bin_offset, ctx->u64c, ctx->counters + 5);
+        HIPCHK(hipGetLastError());
     }
+    ctx->need_wakeup = 0;
+    HIPCHK(hipMemcpyAsync(ctx->hu64c, ctx->u64c, 64 * sizeof(unsigned long long), hipMemcpyDeviceToHost, ctx->stream));
+    CHK(read_counters(ctx));
+    for (int b = 0; b < kTimebins; b++) ctx->bin_count[b] = (long long)ctx->hu64c[b];
+    ctx->n_woken = ctx->hcounters[5];
+    if (ctx->hcounters[6] && !ctx->stop_code) ctx->stop_code = ctx->hcounters[6];
     // do_first_halfstep_kick over the same active list
     if (ctx->nact > 0) {
         Timed t(ctx, "timeline");
@@ -880,19 +944,22 @@ int run_step(gz_context *ctx)
         if ((ti_next % (1LL << b)) == 0) ctx->mask |= 1ull << b;
     T = time_params(ctx);
     HIPCHK(hipMemsetAsync(ctx->u64c + 64, 0, sizeof(unsigned long long), ctx->stream));
+    HIPCHK(hipMemsetAsync(ctx->u64c + 66, 0, sizeof(unsigned long long), ctx->stream));
     {
         Timed t(ctx, "timeline");
-        k_drift<<<nblk(n), kBlock, 0, ctx->stream>>>(T, S, ti_prev, ti_next, ctx->X0, ctx->Y0, ctx->Z0, ctx->u64c + 64);
+        k_drift<<<std::min(nblk(n), 2048u), kBlock, 0, ctx->stream>>>(T, S, ti_prev, ti_next, ctx->X0, ctx->Y0, ctx->Z0, ctx->u64c + 64,
+                                                                     ctx->u64c + 66);
         HIPCHK(hipGetLastError());
     }
-    HIPCHK(hipMemcpyAsync(ctx->hu64c + 64, ctx->u64c + 64, sizeof(unsigned long long), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipMemcpyAsync(ctx->hu64c + 64, ctx->u64c + 64, 3 * sizeof(unsigned long long), hipMemcpyDeviceToHost, ctx->stream));
     // make_list_of_active_particles and the softening cache of the active particles
     CHK(active_from_bins(ctx));
     {
-        unsigned long long bits = ctx->hu64c[64];
-        double d;
-        memcpy(&d, &bits, sizeof d);
+        double d, hm;
+        memcpy(&d, &ctx->hu64c[64], sizeof d);
+        memcpy(&hm, &ctx->hu64c[66], sizeof hm);
         ctx->dmax = d;
+        ctx->hmax_bound = hm;  // every radius after the drift
     }
     // particles that moved far from where they were ordered are ordered again
     if (ctx->last_rmax > 0 && ctx->dmax > ctx->resort_fraction * ctx->last_rmax) {
@@ -906,8 +973,7 @@ int run_step(gz_context *ctx)
         k_halfstep_kick<<<nblk(ctx->nact), kBlock, 0, ctx->stream>>>(T, S, ctx->nact, ctx->act, 1);
         HIPCHK(hipGetLastError());
     }
-    CHK(read_counters(ctx));
-    if (ctx->hcounters[6] && !ctx->stop_code) ctx->stop_code = ctx->hcounters[6];
+
     if (ctx->stop_code) return fail(ctx, ctx->stop_code, "timestep loop: controlled stop %d (step beyond the end of the timeline)", ctx->stop_code);
     return 0;
 }
@@ -1033,6 +1099,8 @@ int gz_load(gz_context *ctx, long n, const gz_fields *f)
     }
     CHK(copy_dev(ctx, ctx->X0, ctx->X, n * 8)); CHK(copy_dev(ctx, ctx->Y0, ctx->Y, n * 8)); CHK(copy_dev(ctx, ctx->Z0, ctx->Z, n * 8));
     ctx->dmax = 0;
+    ctx->key_version++;
+    ctx->hmax_bound = -1;
     ctx->run_started = false;
     ctx->all_active = true;
     // a caller that hands over the state density() leaves (or also the gradients) may start at
@@ -1288,6 +1356,11 @@ int gz_set_option(gz_context *ctx, const char *name, double value)
         ctx->resort_fraction = value;
         return 0;
     }
+    if (k == "few_rows") {
+        if (!(value >= 0) || value > 2e9) return fail(ctx, GZ_E_ARG, "few_rows must be in [0, 2e9]");
+        ctx->few_rows = (int)value;
+        return 0;
+    }
     return fail(ctx, GZ_E_ARG, "unknown option %s", name);
 }
 
@@ -1320,7 +1393,7 @@ long gz_neighbors(gz_context *ctx, int symmetric, long *offsets, int *nbr, long 
     int capl = 0;
     double hmax = 0;
     if (max_on_device(ctx, ctx->H, n, &hmax)) return -GZ_E_HIP;
-    if (build_grid(ctx, std::min(hmax, 0.5 * ctx->par.box_size))) return -GZ_E_HIP;
+    if (build_grid(ctx, std::min(hmax, 0.5 * ctx->par.box_size), true)) return -GZ_E_HIP;
     if (dalloc(ctx, &cnt, ctx->nact)) return -GZ_E_MEMORY;
     int rc = symmetric ? build_list<true>(ctx, ctx->nact, nullptr, nullptr, 1.0, &ell, &ld, &capl, cnt, 64)
                        : build_list<false>(ctx, ctx->nact, nullptr, nullptr, 1.0, &ell, &ld, &capl, cnt, 64);

@@ -740,8 +740,12 @@ __device__ inline int force_pair(const Gam<R> &G, const Cell<R> &I, const Cell<R
 }
 
 // per-particle record of the gradient pass: the neighbour fields gradient_accumulate_neighbor reads
+// the neighbour side of the gradient pass.  The differenced fields stay FP64 in both forms:
+// a difference of two O(1) densities or pressures formed in FP32 loses the digits the gradient
+// is made of (the mixed form then rounds the FP64 difference to FP32).
 template <typename R> struct alignas(16) GradRec {
-    R h, m, rho, p, v0, v1, v2, pad;
+    R h, m;
+    double rho, p, v0, v1, v2;
 };
 
 }  // namespace gz

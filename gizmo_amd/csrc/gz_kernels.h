@@ -24,12 +24,14 @@ struct GridView {
     double cell, box, half;
     const int4 *sten;         // 27 entries per cell: (start, end, offset code, cell code)
     const int *nsten;
-    const double *chmax;      // largest kernel radius in each cell
+    const double *chmax;      // largest kernel radius in each cell (when computed)
+    double margin;            // largest distance a particle has moved from its cell since the ordering
 };
 
 struct CellRows {             // occupied cells of the active rows
     const int *cells;         // cell codes, ascending
-    int ncells;
+    int ncells;               // launch bound: the count, or an upper bound when ncells_dev is set
+    const int *ncells_dev;    // the count on the device (nullptr: ncells is exact)
     const int *ca0, *ca1;     // per cell code: [first, last+1) slot of its active rows
     int nrows;                // number of active rows (column stride of per-row sums)
 };
@@ -70,11 +72,13 @@ struct ParticleView {         // device order; vectors SoA with column stride n
 
 __global__ void k_find_timesteps(TimeParams T, ParticleView S, int nrows, const int *act, int *err);
 __global__ void k_bin_hist(long n, const int *TB, unsigned long long *count);
-__global__ void k_wakeups(TimeParams T, ParticleView S, int lowest_occupied_active_bin, int max_time_bin_active,
-                          int wakeup_bin_offset, int *nwoken);
+__global__ void k_wakeup_limits(TimeParams T, const unsigned long long *count, int *lim);
+__global__ void k_wakeups(TimeParams T, ParticleView S, const int *lim, int wakeup_bin_offset, unsigned long long *count,
+                          int *nwoken);
 __global__ void k_halfstep_kick(TimeParams T, ParticleView S, int nrows, const int *act, int second);
 __global__ void k_drift(TimeParams T, ParticleView S, long long time0, long long time1, const double *X0, const double *Y0,
-                        const double *Z0, unsigned long long *disp);
+                        const double *Z0, unsigned long long *disp, unsigned long long *hmax);
+__global__ void k_mark_records(int nrows, const int *act, const int *ell, long ld, const int *cnt, uint8_t *need);
 __global__ void k_active_from_bins(long n, unsigned long long mask, const int *TB, const double *M, const double *H,
                                    uint8_t *ACT, double *SOFT);
 __global__ void k_gather_ll(long n, const int *perm, const long long *in, long long *out);
@@ -118,19 +122,38 @@ template <int KERN, typename R>
 __global__ void k_density_cells(DevParams P, GridView g, CellRows cr, const int *act, const uint8_t *need, const double *X,
                                 const double *Y, const double *Z, const double *VX, const double *VY, const double *VZ,
                                 const double *M, const double *hcur, double *accout);
+__global__ void k_density_init(int na, const int *act, double *H, double hclamp, double *hcur, double *left, double *right,
+                               double *cn_cur, int *cn_init, int *iter, uint8_t *need, int *counters, unsigned long long *hmax);
 __global__ void k_density_after(DevParams P, int nrows, const int *act, uint8_t *need, const double *accin, const double *M,
-                                const double *U, double cell, DensityIter it, DensityOut out, int *nnext, int *rebuild, int *err);
+                                const double *U, double cell, DensityIter it, DensityOut out, int *nnext, int *rebuild, int *err,
+                                unsigned long long *hfinal);
 
 template <typename R>
-__global__ void k_pack_grad(long n, const double *H, const double *M, const double *RHO, const double *PR, const double *VX,
-                            const double *VY, const double *VZ, GradRec<R> *rec);
+__global__ void k_pack_grad(long n, const uint8_t *need, const double *H, const double *M, const double *RHO, const double *PR,
+                            const double *VX, const double *VY, const double *VZ, GradRec<R> *rec);
 template <typename R>
-__global__ void k_pack_force(double gamma, ForceView F, Cell<R> *rec);
+__global__ void k_pack_force(double gamma, ForceView F, const uint8_t *need, Cell<R> *rec);
 
 template <int KERN, typename R>
 __global__ void k_gradients(DevParams P, int nrows, const int *act, const double *X, const double *Y, const double *Z,
                             const GradRec<R> *rec, const double *T, const double *CN, long n, const int *ell, long ld,
                             const int *cnt, double *GR, double *GP, double *GV);
+
+// one wavefront per active row, for passes over few rows
+template <int KERN, typename R>
+__global__ void k_density_rows(DevParams P, GridView g, int nrows, const int *act, const uint8_t *need, const uint32_t *key,
+                               const double *X, const double *Y, const double *Z, const double *VX, const double *VY,
+                               const double *VZ, const double *M, const double *hcur, double *accout);
+__global__ void k_build_sym_rows(GridView g, int nrows, const int *act, const uint32_t *key, const double *X, const double *Y,
+                                 const double *Z, const double *H, int *ell, long ld, int cap, int *cnt, int *overflow);
+template <int KERN, typename R>
+__global__ void k_gradients_rows(DevParams P, int nrows, const int *act, const double *X, const double *Y, const double *Z,
+                                 const GradRec<R> *rec, const double *T, const double *CN, long n, const int *ell, long ld,
+                                 const int *cnt, double *GR, double *GP, double *GV);
+template <int KERN, typename R>
+__global__ void k_force_rows(DevParams P, Gam<R> G, int nrows, const int *act, ForceView F, const Cell<R> *rec, const int *ell,
+                             long ld, const int *cnt, const uint8_t *active, const int *timebin, const double *msv_in, int *wakeup,
+                             double *ACC, double *DTU, double *MSV, int *err, int *need_wakeup, unsigned long long *pairs);
 
 template <int KERN, typename R>
 __global__ void k_force(DevParams P, Gam<R> G, int nrows, const int *act, ForceView F, const Cell<R> *rec, const int *ell,

@@ -2,6 +2,7 @@
 // -ffp-contract=off and correctly rounded division/sqrt, the oracle's rounding order).
 #define GZ_DEFINE_COMMON_KERNELS
 #include "gz_kernels_impl.h"
+#include "gz_rows_impl.h"
 #include "gz_time_impl.h"
 
 namespace gz {
@@ -27,12 +28,22 @@ template __global__ void k_build_ell<true>(int, const int *, const int *, const 
                                                   const int *, double *, double *, double *);                                  \
     template __global__ void k_force<KERN, R>(DevParams, Gam<R>, int, const int *, ForceView, const Cell<R> *, const int *,     \
                                               long, const int *, const uint8_t *, const int *, const double *, int *, double *, \
-                                              double *, double *, int *, int *, unsigned long long *);
+                                              double *, double *, int *, int *, unsigned long long *);                 \
+    template __global__ void k_density_rows<KERN, R>(DevParams, GridView, int, const int *, const uint8_t *, const uint32_t *,  \
+                                                     const double *, const double *, const double *, const double *,            \
+                                                     const double *, const double *, const double *, const double *, double *); \
+    template __global__ void k_gradients_rows<KERN, R>(DevParams, int, const int *, const double *, const double *,             \
+                                                       const double *, const GradRec<R> *, const double *, const double *, long, \
+                                                       const int *, long, const int *, double *, double *, double *);           \
+    template __global__ void k_force_rows<KERN, R>(DevParams, Gam<R>, int, const int *, ForceView, const Cell<R> *,            \
+                                                   const int *, long, const int *, const uint8_t *, const int *,                \
+                                                   const double *, int *, double *, double *, double *, int *, int *,           \
+                                                   unsigned long long *);
 GZ_INST(3, double)
 GZ_INST(6, double)
-template __global__ void k_pack_grad<double>(long, const double *, const double *, const double *, const double *, const double *,
+template __global__ void k_pack_grad<double>(long, const uint8_t *, const double *, const double *, const double *, const double *, const double *,
                                              const double *, const double *, GradRec<double> *);
-template __global__ void k_pack_force<double>(double, ForceView, Cell<double> *);
+template __global__ void k_pack_force<double>(double, ForceView, const uint8_t *, Cell<double> *);
 #undef GZ_INST
 
 }  // namespace gz

@@ -3,6 +3,7 @@
 // its rounding order.  The FP64 pieces it keeps (positions, radius test, 3x3 inverse) follow
 // the same source.
 #include "gz_kernels_impl.h"
+#include "gz_rows_impl.h"
 
 namespace gz {
 
@@ -23,12 +24,22 @@ namespace gz {
                                                   const int *, double *, double *, double *);                                  \
     template __global__ void k_force<KERN, R>(DevParams, Gam<R>, int, const int *, ForceView, const Cell<R> *, const int *,     \
                                               long, const int *, const uint8_t *, const int *, const double *, int *, double *, \
-                                              double *, double *, int *, int *, unsigned long long *);
+                                              double *, double *, int *, int *, unsigned long long *);                 \
+    template __global__ void k_density_rows<KERN, R>(DevParams, GridView, int, const int *, const uint8_t *, const uint32_t *,  \
+                                                     const double *, const double *, const double *, const double *,            \
+                                                     const double *, const double *, const double *, const double *, double *); \
+    template __global__ void k_gradients_rows<KERN, R>(DevParams, int, const int *, const double *, const double *,             \
+                                                       const double *, const GradRec<R> *, const double *, const double *, long, \
+                                                       const int *, long, const int *, double *, double *, double *);           \
+    template __global__ void k_force_rows<KERN, R>(DevParams, Gam<R>, int, const int *, ForceView, const Cell<R> *,            \
+                                                   const int *, long, const int *, const uint8_t *, const int *,                \
+                                                   const double *, int *, double *, double *, double *, int *, int *,           \
+                                                   unsigned long long *);
 GZ_INST(3, float)
 GZ_INST(6, float)
-template __global__ void k_pack_grad<float>(long, const double *, const double *, const double *, const double *, const double *,
+template __global__ void k_pack_grad<float>(long, const uint8_t *, const double *, const double *, const double *, const double *, const double *,
                                             const double *, const double *, GradRec<float> *);
-template __global__ void k_pack_force<float>(double, ForceView, Cell<float> *);
+template __global__ void k_pack_force<float>(double, ForceView, const uint8_t *, Cell<float> *);
 #undef GZ_INST
 
 }  // namespace gz

@@ -547,7 +547,7 @@ __global__ void __launch_bounds__(64) k_density_cells(DevParams P, GridView g, C
 {
     __shared__ double sx[64], sy[64], sz[64], svx[64], svy[64], svz[64], sm[64];
     const long cb = xcd_block();
-    if (cb >= cr.ncells) return;
+    if (cb >= (cr.ncells_dev ? *cr.ncells_dev : cr.ncells)) return;
     const int c = cr.cells[cb];
     const int a0 = cr.ca0[c], a1 = cr.ca1[c];
     const int nst = g.nsten[c];
@@ -631,13 +631,40 @@ __global__ void __launch_bounds__(64) k_density_cells(DevParams P, GridView g, C
     }
 }
 
+// density()'s pre-iteration radius clamp (src/hydro/density_loop.cc:818-838) and the start of
+// the iteration: radius of each row, empty bracket, round counters, the largest radius (the
+// bits of a positive double order like the double)
+#ifdef GZ_DEFINE_COMMON_KERNELS
+__global__ void __launch_bounds__(256) k_density_init(int na, const int *__restrict__ act, double *__restrict__ H, double hclamp,
+                                                      double *__restrict__ hcur, double *__restrict__ left, double *__restrict__ right,
+                                                      double *__restrict__ cn_cur, int *__restrict__ cn_init, int *__restrict__ iter,
+                                                      uint8_t *__restrict__ need, int *__restrict__ counters,
+                                                      unsigned long long *__restrict__ hmax)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < 8) counters[r] = 0;
+    double h = 0;
+    if (r < na) {
+        const int i = act[r];
+        h = H[i];
+        if ((h < 0) || !isfinite(h) || (h > hclamp)) { h = hclamp; H[i] = h; }
+        hcur[r] = h;
+        left[r] = 0; right[r] = 0; cn_cur[r] = 0; cn_init[r] = 0; iter[r] = 0;
+        need[r] = 1;
+    }
+    for (int o = 32; o > 0; o >>= 1) h = fmax(h, __shfl_xor(h, o, 64));
+    if ((threadIdx.x & 63) == 0 && h > 0) atomicMax(hmax, (unsigned long long)__double_as_longlong(h));
+}
+#endif
+
 // after_iter_body (src/hydro/density_loop.h:516-961) on the sums of the round and, on
 // convergence, density_finalize_post_runner (src/hydro/density_loop.cc:303-733)
 #ifdef GZ_DEFINE_COMMON_KERNELS
 __global__ void __launch_bounds__(256) k_density_after(DevParams P, int nrows, const int *__restrict__ act, uint8_t *__restrict__ need,
                                                        const double *__restrict__ accin, const double *__restrict__ M,
                                                        const double *__restrict__ U, double cell, DensityIter it, DensityOut out,
-                                                       int *__restrict__ nnext, int *__restrict__ rebuild, int *__restrict__ err)
+                                                       int *__restrict__ nnext, int *__restrict__ rebuild, int *__restrict__ err,
+                                                       unsigned long long *__restrict__ hfinal)
 {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= nrows || !need[r]) return;
@@ -656,6 +683,7 @@ __global__ void __launch_bounds__(256) k_density_after(DevParams P, int nrows, c
         else if (rc > 0) atomicMax(err, 1155);
         density_finalize(P, out, i, h, acc, M[i], U[i]);
         need[r] = 0;
+        if (h > 0) atomicMax(hfinal, (unsigned long long)__double_as_longlong(h));
         return;
     }
     it.left[r] = sc.left; it.right[r] = sc.right; it.cn_cur[r] = sc.cn_cur; it.cn_init[r] = sc.cn_init;
@@ -680,7 +708,7 @@ __global__ void __launch_bounds__(64) k_build_sym_cells(GridView g, CellRows cr,
     __shared__ double sx[64], sy[64], sz[64], sh[64];
     __shared__ int sj[64];
     const long cb = xcd_block();
-    if (cb >= cr.ncells) return;
+    if (cb >= (cr.ncells_dev ? *cr.ncells_dev : cr.ncells)) return;
     const int c = cr.cells[cb];
     const int a0 = cr.ca0[c], a1 = cr.ca1[c];
     const int nst = g.nsten[c];
@@ -729,28 +757,43 @@ __global__ void __launch_bounds__(64) k_build_sym_cells(GridView g, CellRows cr,
 #endif
 
 
+// the records a pass over few rows reads: the rows' own and their list neighbours'
+#ifdef GZ_DEFINE_COMMON_KERNELS
+__global__ void k_mark_records(int nrows, const int *__restrict__ act, const int *__restrict__ ell, long ld,
+                               const int *__restrict__ cnt, uint8_t *__restrict__ need)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nrows) return;
+    need[act[r]] = 1;
+    const int c = cnt[r];
+    for (int q = 0; q < c; q++) need[ell[(size_t)q * ld + r]] = 1;
+}
+#endif
+
 // ---------------------------------------------------------------------------------------
 // records: the neighbour-side fields of a pass, packed per particle (device order)
 // ---------------------------------------------------------------------------------------
 template <typename R>
-__global__ void k_pack_grad(long n, const double *__restrict__ H, const double *__restrict__ M, const double *__restrict__ RHO,
-                            const double *__restrict__ PR, const double *__restrict__ VX, const double *__restrict__ VY,
-                            const double *__restrict__ VZ, GradRec<R> *__restrict__ rec)
+__global__ void k_pack_grad(long n, const uint8_t *__restrict__ need, const double *__restrict__ H, const double *__restrict__ M,
+                            const double *__restrict__ RHO, const double *__restrict__ PR, const double *__restrict__ VX,
+                            const double *__restrict__ VY, const double *__restrict__ VZ, GradRec<R> *__restrict__ rec)
 {
     const long j = blockIdx.x * (long)blockDim.x + threadIdx.x;
     if (j >= n) return;
+    if (need && !need[j]) return;
     GradRec<R> g;
-    g.h = (R)H[j]; g.m = (R)M[j]; g.rho = (R)RHO[j]; g.p = (R)PR[j];
-    g.v0 = (R)VX[j]; g.v1 = (R)VY[j]; g.v2 = (R)VZ[j]; g.pad = 0;
+    g.h = (R)H[j]; g.m = (R)M[j]; g.rho = RHO[j]; g.p = PR[j];
+    g.v0 = VX[j]; g.v1 = VY[j]; g.v2 = VZ[j];
     rec[j] = g;
 }
 
 template <typename R>
-__global__ void k_pack_force(double gamma, ForceView F, Cell<R> *__restrict__ rec)
+__global__ void k_pack_force(double gamma, ForceView F, const uint8_t *__restrict__ need, Cell<R> *__restrict__ rec)
 {
     const long j = blockIdx.x * (long)blockDim.x + threadIdx.x;
     const long n = F.n;
     if (j >= n) return;
+    if (need && !need[j]) return;
     Cell<R> c;
     c.h = (R)F.H[j]; c.m = (R)F.M[j]; c.rho = (R)F.RHO[j]; c.p = (R)F.PR[j];
     c.cn = (R)F.CN[j]; c.fce = (R)F.FCE[j]; c.drk = (R)F.DRK[j];
@@ -770,6 +813,38 @@ __global__ void k_pack_force(double gamma, ForceView F, Cell<R> *__restrict__ re
 // construct_gradient (src/hydro/gradients_loop.cc:129-136) and the limiter block
 // (:1077-1130), fused per active particle
 // ---------------------------------------------------------------------------------------
+// construct_gradient (NV_T^-1 times the sums) and the limiter block of one particle
+template <typename R>
+__device__ inline void gradients_tail(const GradRec<R> &I, bool enabled, const R (&G)[3][5], const R (&mx)[5], const R (&mn)[5],
+                                      R maxd, const double *__restrict__ T, const double *__restrict__ CN, long n, int i,
+                                      double *__restrict__ GR, double *__restrict__ GP, double *__restrict__ GV)
+{
+    const R hi = I.h;
+    const R t0 = (R)T[0 * n + i], t1 = (R)T[1 * n + i], t2 = (R)T[2 * n + i], t3 = (R)T[3 * n + i], t4 = (R)T[4 * n + i], t5 = (R)T[5 * n + i];
+    R o[5][3];
+    for (int b = 0; b < 5; b++) {
+        const R v0 = G[0][b], v1 = G[1][b], v2 = G[2][b];
+        o[b][0] = t0 * v0 + t3 * v1 + t5 * v2;
+        o[b][1] = t3 * v0 + t1 * v1 + t4 * v2;
+        o[b][2] = t5 * v0 + t4 * v1 + t2 * v2;
+    }
+    if (enabled) {
+        const R stol = R(0.1);
+        const R dmaxv = dmax(hi, maxd);
+        const double cni = CN[i];
+        R alim = R(0.25);
+        if (cni > 100) alim = (R)dmin(0.5, 0.25 + 0.25 * (cni - 100) / 100);
+        slopelimiter<R>(o[0], mx[0], mn[0], alim, hi, R(0), 1, dmaxv, (R)I.rho);
+        slopelimiter<R>(o[1], mx[1], mn[1], alim, hi, stol, 1, dmaxv, (R)I.p);
+        for (int kk = 0; kk < 3; kk++) slopelimiter<R>(o[2 + kk], mx[2 + kk], mn[2 + kk], alim, hi, stol, 0, R(0), R(0));
+    }
+    for (int kk = 0; kk < 3; kk++) {
+        GR[(size_t)kk * n + i] = (double)o[0][kk];
+        GP[(size_t)kk * n + i] = (double)o[1][kk];
+        for (int b = 0; b < 3; b++) GV[(size_t)(3 * b + kk) * n + i] = (double)o[2 + b][kk];
+    }
+}
+
 template <int KERN, typename R>
 __global__ void __launch_bounds__(256) k_gradients(DevParams P, int nrows, const int *__restrict__ act, const double *__restrict__ X,
                                                    const double *__restrict__ Y, const double *__restrict__ Z,
@@ -815,11 +890,11 @@ __global__ void __launch_bounds__(256) k_gradients(DevParams P, int nrows, const
             }
             if (rr > maxd) maxd = rr;
             R d[5];
-            d[0] = J.rho - I.rho;
-            d[1] = J.p - I.p;
-            d[2] = J.v0 - I.v0;
-            d[3] = J.v1 - I.v1;
-            d[4] = J.v2 - I.v2;
+            d[0] = (R)(J.rho - I.rho);
+            d[1] = (R)(J.p - I.p);
+            d[2] = (R)(J.v0 - I.v0);
+            d[3] = (R)(J.v1 - I.v1);
+            d[4] = (R)(J.v2 - I.v2);
             for (int b = 0; b < 5; b++) {
                 if (d[b] < mn[b]) mn[b] = d[b];
                 else if (d[b] > mx[b]) mx[b] = d[b];
@@ -832,29 +907,7 @@ __global__ void __launch_bounds__(256) k_gradients(DevParams P, int nrows, const
             }
         }
     }
-    const R t0 = (R)T[0 * n + i], t1 = (R)T[1 * n + i], t2 = (R)T[2 * n + i], t3 = (R)T[3 * n + i], t4 = (R)T[4 * n + i], t5 = (R)T[5 * n + i];
-    R o[5][3];
-    for (int b = 0; b < 5; b++) {
-        const R v0 = G[0][b], v1 = G[1][b], v2 = G[2][b];
-        o[b][0] = t0 * v0 + t3 * v1 + t5 * v2;
-        o[b][1] = t3 * v0 + t1 * v1 + t4 * v2;
-        o[b][2] = t5 * v0 + t4 * v1 + t2 * v2;
-    }
-    if (enabled) {
-        const R stol = R(0.1);
-        const R dmaxv = dmax(hi, maxd);
-        const double cni = CN[i];
-        R alim = R(0.25);
-        if (cni > 100) alim = (R)dmin(0.5, 0.25 + 0.25 * (cni - 100) / 100);
-        slopelimiter<R>(o[0], mx[0], mn[0], alim, hi, R(0), 1, dmaxv, I.rho);
-        slopelimiter<R>(o[1], mx[1], mn[1], alim, hi, stol, 1, dmaxv, I.p);
-        for (int kk = 0; kk < 3; kk++) slopelimiter<R>(o[2 + kk], mx[2 + kk], mn[2 + kk], alim, hi, stol, 0, R(0), R(0));
-    }
-    for (int kk = 0; kk < 3; kk++) {
-        GR[(size_t)kk * n + i] = (double)o[0][kk];
-        GP[(size_t)kk * n + i] = (double)o[1][kk];
-        for (int b = 0; b < 3; b++) GV[(size_t)(3 * b + kk) * n + i] = (double)o[2 + b][kk];
-    }
+    gradients_tail<R>(I, enabled, G, mx, mn, maxd, T, CN, n, i, GR, GP, GV);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -862,6 +915,33 @@ __global__ void __launch_bounds__(256) k_gradients(DevParams P, int nrows, const
 // (each active particle accumulates its own side of every pair), then the writeback
 // (:227-321) and hydro_final_operations_and_cleanup (src/hydro/hydro_toplevel.cc:133-250).
 // ---------------------------------------------------------------------------------------
+// the writeback (src/hydro/hydro_force_loop.cc:302-321) and hydro_final_operations_and_cleanup
+// (src/hydro/hydro_toplevel.cc:133-250, non-comoving MFM) of one particle
+__device__ inline void force_tail(const DevParams &P, const ForceView &F, int i, bool enabled, double sound_i, double A0, double A1,
+                                  double A2, double dE, double msv, double *__restrict__ ACC, double *__restrict__ DTU,
+                                  double *__restrict__ MSV)
+{
+    const long n = F.n;
+    const double mi = F.M[i];
+    // MaxSignalVel = max(accumulated, c_s) over MIN_REAL_NUMBER (hydro_force_loop.cc:318-321)
+    double ms = kMinReal;
+    double mxs = fmax(msv, sound_i);
+    if (ms < mxs) ms = mxs;
+    double e = dE;
+    if (enabled) {
+        const double inv1 = 1.0 / 1.0;
+        const double vp0 = F.VX[i] * inv1, vp1 = F.VY[i] * inv1, vp2 = F.VZ[i] * inv1;
+        e -= vp0 * A0 + vp1 * A1 + vp2 * A2;
+        const double invm = 1.0 / mi;
+        A0 *= invm; A1 *= invm; A2 *= invm;
+        e /= mi;
+        if (F.H[i] >= 0.99 * P.max_kernel_radius) e = 0;
+    }
+    ACC[i] = A0; ACC[n + i] = A1; ACC[2 * n + i] = A2;
+    DTU[i] = e;
+    MSV[i] = ms;
+}
+
 template <int KERN, typename R>
 __global__ void __launch_bounds__(256) k_force(DevParams P, Gam<R> G, int nrows, const int *__restrict__ act, ForceView F,
                                                const Cell<R> *__restrict__ rec, const int *__restrict__ ell, long ld,
@@ -917,24 +997,7 @@ __global__ void __launch_bounds__(256) k_force(DevParams P, Gam<R> G, int nrows,
         }
         A0 = a0; A1 = a1; A2 = a2; dE = ae; msv = m;
     }
-    // writeback: MaxSignalVel = max(accumulated, c_s) over MIN_REAL_NUMBER (hydro_force_loop.cc:318-321)
-    double ms = kMinReal;
-    double mxs = fmax(msv, sound_i);
-    if (ms < mxs) ms = mxs;
-    // hydro_final_operations_and_cleanup (non-comoving, MFM)
-    double e = dE;
-    if (enabled) {
-        const double inv1 = 1.0 / 1.0;
-        const double vp0 = F.VX[i] * inv1, vp1 = F.VY[i] * inv1, vp2 = F.VZ[i] * inv1;
-        e -= vp0 * A0 + vp1 * A1 + vp2 * A2;
-        const double invm = 1.0 / mi;
-        A0 *= invm; A1 *= invm; A2 *= invm;
-        e /= mi;
-        if (F.H[i] >= 0.99 * P.max_kernel_radius) e = 0;
-    }
-    ACC[i] = A0; ACC[n + i] = A1; ACC[2 * n + i] = A2;
-    DTU[i] = e;
-    MSV[i] = ms;
+    force_tail(P, F, i, enabled, sound_i, A0, A1, A2, dE, msv, ACC, DTU, MSV);
 }
 
 }  // namespace gz

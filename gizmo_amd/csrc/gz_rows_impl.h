@@ -1,0 +1,355 @@
+// gz_rows_impl.h -- the hot-path passes with one wavefront per active row, for steps in which
+// few particles are active (the usual case of adaptive time bins: a few hundred hot particles
+// in a resting medium).  The 64 lanes of the row's wavefront evaluate 64 candidates or list
+// entries at once; each lane's pair terms go to LDS and lane 0 adds them up in ascending
+// candidate order, so the sums are the same as those of the lane-per-row kernels (and of the
+// oracle) operation for operation.  One wavefront per workgroup: the block barrier is a wave
+// barrier.
+#pragma once
+#include "gz_kernels_impl.h"
+
+namespace gz {
+
+// The candidates of a row: the non-empty cells of its home cell's 27-stencil, sorted by start,
+// laid end to end.  Virtual index v maps to the particle of range k with pref[k] <= v < pref[k+1].
+struct RowStencil {
+    int start[27];
+    int pref[28];
+    int nst;
+};
+
+// the ranges of the stencil that can hold a partner: a cell is dropped when the row lies farther
+// from its box than the row's reach plus the distance any particle has moved from its cell
+// (reach = h_i, or max(h_i, the cell's largest radius) for the symmetric test)
+template <bool SYM>
+__device__ __forceinline__ void row_stencil(const GridView &g, uint32_t home, double xi, double yi, double zi, double hrow,
+                                            RowStencil &rs)
+{
+    const int lane = threadIdx.x;
+    const int nst = g.nsten[home];
+    if (lane == 0) {
+        const bool prune = (1 << g.level) >= 4;
+        const double ox = compact10(home >> 2) * g.cell, oy = compact10(home >> 1) * g.cell, oz = compact10(home) * g.cell;
+        const double rx = wrap(xi - ox, g.box, g.half), ry = wrap(yi - oy, g.box, g.half), rz = wrap(zi - oz, g.box, g.half);
+        int acc = 0, m = 0;
+        for (int q = 0; q < nst; q++) {
+            const int4 e = g.sten[(size_t)home * 27 + q];
+            if (prune) {
+                const int dx = e.z / 9 - 1, dy = (e.z / 3) % 3 - 1, dz = e.z % 3 - 1;
+                // gap from the row to the neighbour cell's box [d*cell, (d+1)*cell) along each axis
+                const double gx = fmax(fmax(dx * g.cell - rx, rx - (dx + 1) * g.cell), 0.0);
+                const double gy = fmax(fmax(dy * g.cell - ry, ry - (dy + 1) * g.cell), 0.0);
+                const double gz = fmax(fmax(dz * g.cell - rz, rz - (dz + 1) * g.cell), 0.0);
+                double reach = hrow;
+                if (SYM) reach = fmax(reach, g.chmax[e.w]);
+                const double rr = (reach + g.margin) * (1.0 + 1e-9) + 1e-12 * g.box;
+                if (gx * gx + gy * gy + gz * gz > rr * rr) continue;
+            }
+            rs.start[m] = e.x;
+            rs.pref[m] = acc;
+            acc += e.y - e.x;
+            m++;
+        }
+        rs.pref[m] = acc;
+        rs.nst = m;
+    }
+    __syncthreads();
+}
+
+__device__ __forceinline__ int row_candidate(const RowStencil &rs, int v)
+{
+    int k = 0;
+    while (k + 1 < rs.nst && rs.pref[k + 1] <= v) k++;
+    return rs.start[k] + (v - rs.pref[k]);
+}
+
+// density sums of the rows still iterating (DensitySpec::pair_kernel,
+// src/hydro/density_loop.h:1141-1385), the terms in k_density_cells' order
+template <int KERN, typename R>
+__global__ void __launch_bounds__(64) k_density_rows(DevParams P, GridView g, int nrows, const int *__restrict__ act,
+                                                     const uint8_t *__restrict__ need, const uint32_t *__restrict__ key,
+                                                     const double *__restrict__ X, const double *__restrict__ Y,
+                                                     const double *__restrict__ Z, const double *__restrict__ VX,
+                                                     const double *__restrict__ VY, const double *__restrict__ VZ,
+                                                     const double *__restrict__ M, const double *__restrict__ hcur,
+                                                     double *__restrict__ accout)
+{
+    __shared__ RowStencil rs;
+    __shared__ R sc[13][64];
+    __shared__ int sf[64];
+    const long r = xcd_block();
+    if (r >= nrows || !need[r]) return;
+    const int lane = threadIdx.x;
+    const int i = act[r];
+    const double xi = X[i], yi = Y[i], zi = Z[i];
+    const R vxi = (R)VX[i], vyi = (R)VY[i], vzi = (R)VZ[i];
+    const double h = hcur[r];
+    const R hR = (R)h;
+    const R h2 = hR * hR;
+    R hinv, hinv3, hinv4;
+    kernel_hinv(hR, hinv, hinv3, hinv4);
+    row_stencil<false>(g, key[i] >> g.shift, xi, yi, zi, h, rs);
+    const int total = rs.pref[rs.nst];
+    R a[13];
+    for (int q = 0; q < 13; q++) a[q] = 0;
+    for (int base = 0; base < total; base += 64) {
+        const int v = base + lane;
+        int f = 0;
+        if (v < total) {
+            const int j = row_candidate(rs, v);
+            const double mj = M[j];
+            if (mj > 0) {
+                double d0 = xi - X[j], d1 = yi - Y[j], d2 = zi - Z[j];
+                d1 = wrap(d1, P.box, P.half);
+                d0 = wrap(d0, P.box, P.half);
+                d2 = wrap(d2, P.box, P.half);
+                const R dp0 = (R)d0, dp1 = (R)d1, dp2 = (R)d2;
+                const R r2 = dp0 * dp0 + dp1 * dp1 + dp2 * dp2;
+                if (r2 < h2) {
+                    const R rr = sqrt(r2);
+                    const R u = rr * hinv;
+                    R wk = 0, dwk = 0;
+                    kernel_main<KERN, 0>(u, hinv3, hinv4, wk, dwk);
+                    sc[0][lane] = wk;
+                    sc[1][lane] = (R)mj * wk;
+                    sc[2][lane] = -(R(3.0) * hinv * wk + u * dwk);
+                    f = 1;
+                    if (rr > 0) {
+                        sc[4][lane] = wk * (dp0 * dp0);
+                        sc[5][lane] = wk * (dp1 * dp1);
+                        sc[6][lane] = wk * (dp2 * dp2);
+                        sc[7][lane] = wk * (dp0 * dp1);
+                        sc[8][lane] = wk * (dp1 * dp2);
+                        sc[9][lane] = wk * (dp0 * dp2);
+                        sc[10][lane] = wk * dp0;
+                        sc[11][lane] = wk * dp1;
+                        sc[12][lane] = wk * dp2;
+                        const R dv0 = vxi - (R)VX[j], dv1 = vyi - (R)VY[j], dv2 = vzi - (R)VZ[j];
+                        sc[3][lane] = dwk * (dp0 * dv0 + dp1 * dv1 + dp2 * dv2) / rr;
+                        f = 2;
+                    }
+                }
+            }
+        }
+        // lane 0 adds the accepted candidates in order (none: nothing to exchange)
+        unsigned long long mask = __ballot(f != 0);
+        if (!mask) continue;
+        sf[lane] = f;
+        __syncthreads();
+        if (lane == 0) {
+            while (mask) {
+                const int k = __ffsll((long long)mask) - 1;
+                mask &= mask - 1;
+                const int fk = sf[k];
+                a[0] += sc[0][k];
+                a[1] += sc[1][k];
+                a[2] += sc[2][k];
+                if (fk == 2) {
+                    for (int q = 4; q < 13; q++) a[q] += sc[q][k];
+                    a[3] -= sc[3][k];
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if (lane == 0)
+        for (int q = 0; q < 13; q++) accout[(size_t)q * nrows + r] = (double)a[q];
+}
+
+// the symmetric list of few rows (the same entries, in the same order, as k_build_sym_cells)
+#ifdef GZ_DEFINE_COMMON_KERNELS
+__global__ void __launch_bounds__(64) k_build_sym_rows(GridView g, int nrows, const int *__restrict__ act, const uint32_t *__restrict__ key,
+                                                       const double *__restrict__ X, const double *__restrict__ Y,
+                                                       const double *__restrict__ Z, const double *__restrict__ H, int *__restrict__ ell,
+                                                       long ld, int cap, int *__restrict__ cnt, int *__restrict__ overflow)
+{
+    __shared__ RowStencil rs;
+    const long r = xcd_block();
+    if (r >= nrows) return;
+    const int lane = threadIdx.x;
+    const int i = act[r];
+    const double xi = X[i], yi = Y[i], zi = Z[i], hi = H[i];
+    row_stencil<true>(g, key[i] >> g.shift, xi, yi, zi, hi, rs);
+    const int total = rs.pref[rs.nst];
+    int n = 0;
+    for (int base = 0; base < total; base += 64) {
+        const int v = base + lane;
+        bool acc = false;
+        int j = -1;
+        if (v < total) {
+            j = row_candidate(rs, v);
+            acc = (j != i) && accept_pair<true>(xi, yi, zi, X[j], Y[j], Z[j], hi, H[j], g.box, g.half);
+        }
+        int cntk;
+        const int pos = kept_rank(acc, cntk);
+        if (acc && n + pos < cap) ell[(size_t)(n + pos) * ld + r] = j;
+        n += cntk;
+    }
+    if (lane == 0) {
+        cnt[r] = n;
+        if (n > cap) atomicMax(overflow, n);
+    }
+}
+#endif
+
+// gradients of few rows: the terms of each list entry in LDS, accumulated in list order by lane 0
+template <int KERN, typename R>
+__global__ void __launch_bounds__(64) k_gradients_rows(DevParams P, int nrows, const int *__restrict__ act, const double *__restrict__ X,
+                                                       const double *__restrict__ Y, const double *__restrict__ Z,
+                                                       const GradRec<R> *__restrict__ rec, const double *__restrict__ T,
+                                                       const double *__restrict__ CN, long n, const int *__restrict__ ell, long ld,
+                                                       const int *__restrict__ cnt, double *__restrict__ GR, double *__restrict__ GP,
+                                                       double *__restrict__ GV)
+{
+    __shared__ R sd[5][64], sw[3][64], srr[64];
+    __shared__ int sf[64];
+    const long r = xcd_block();
+    if (r >= nrows) return;
+    const int lane = threadIdx.x;
+    const int i = act[r];
+    R G[3][5];
+    for (int a = 0; a < 3; a++) for (int b = 0; b < 5; b++) G[a][b] = 0;
+    R mx[5], mn[5];
+    for (int b = 0; b < 5; b++) { mx[b] = 0; mn[b] = 0; }
+    R maxd = 0;
+    const GradRec<R> I = rec[i];
+    const bool enabled = (I.m > 0 && I.rho > 0);
+    const R hi = I.h;
+    if (enabled) {
+        const double xi = X[i], yi = Y[i], zi = Z[i];
+        R hinv, hinv3, hinv4;
+        kernel_hinv(hi, hinv, hinv3, hinv4);
+        const R h2i = hi * hi;
+        const int c = cnt[r];
+        for (int base = 0; base < c; base += 64) {
+            const int q = base + lane;
+            int f = 0;
+            if (q < c) {
+                const int j = ell[(size_t)q * ld + r];
+                const GradRec<R> J = rec[j];
+                if ((J.m > 0) && (J.rho > 0) && (J.h > 0)) {
+                    double d0 = xi - X[j], d1 = yi - Y[j], d2 = zi - Z[j];
+                    d1 = wrap(d1, P.box, P.half);
+                    d0 = wrap(d0, P.box, P.half);
+                    d2 = wrap(d2, P.box, P.half);
+                    const R dp[3] = {(R)d0, (R)d1, (R)d2};
+                    const R r2 = dp[0] * dp[0] + dp[1] * dp[1] + dp[2] * dp[2];
+                    if ((r2 > 0) && !((r2 >= h2i) && (r2 >= J.h * J.h))) {
+                        const R rr = sqrt(r2);
+                        R wk = 0, dwk = 0;
+                        f = 1;
+                        if (rr < hi) {
+                            const R u = rr * hinv;
+                            kernel_main<KERN, -1>(u, hinv3, hinv4, wk, dwk);
+                            f = 2;
+                            for (int kk = 0; kk < 3; kk++) sw[kk][lane] = -wk * dp[kk];
+                        }
+                        srr[lane] = rr;
+                        sd[0][lane] = (R)(J.rho - I.rho);
+                        sd[1][lane] = (R)(J.p - I.p);
+                        sd[2][lane] = (R)(J.v0 - I.v0);
+                        sd[3][lane] = (R)(J.v1 - I.v1);
+                        sd[4][lane] = (R)(J.v2 - I.v2);
+                    }
+                }
+            }
+            sf[lane] = f;
+            __syncthreads();
+            if (lane == 0) {
+                const int m = min(64, c - base);
+                for (int k = 0; k < m; k++) {
+                    const int fk = sf[k];
+                    if (!fk) continue;
+                    const R rr = srr[k];
+                    if (rr > maxd) maxd = rr;
+                    R d[5];
+                    for (int b = 0; b < 5; b++) d[b] = sd[b][k];
+                    for (int b = 0; b < 5; b++) {
+                        if (d[b] < mn[b]) mn[b] = d[b];
+                        else if (d[b] > mx[b]) mx[b] = d[b];
+                    }
+                    if (fk == 2)
+                        for (int kk = 0; kk < 3; kk++) {
+                            const R w = sw[kk][k];
+                            for (int b = 0; b < 5; b++) G[kk][b] += w * d[b];
+                        }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    if (lane == 0) gradients_tail<R>(I, enabled, G, mx, mn, maxd, T, CN, n, i, GR, GP, GV);
+}
+
+// force of few rows: each lane evaluates one list entry (hydro_accumulate_neighbor), lane 0
+// accumulates in list order, then the writeback and final operations
+template <int KERN, typename R>
+__global__ void __launch_bounds__(64) k_force_rows(DevParams P, Gam<R> G, int nrows, const int *__restrict__ act, ForceView F,
+                                                   const Cell<R> *__restrict__ rec, const int *__restrict__ ell, long ld,
+                                                   const int *__restrict__ cnt, const uint8_t *__restrict__ active,
+                                                   const int *__restrict__ timebin, const double *__restrict__ msv_in,
+                                                   int *__restrict__ wakeup, double *__restrict__ ACC, double *__restrict__ DTU,
+                                                   double *__restrict__ MSV, int *__restrict__ err, int *__restrict__ need_wakeup,
+                                                   unsigned long long *__restrict__ pairs)
+{
+    __shared__ R sv[5][64];
+    __shared__ int src[64];
+    const long r = xcd_block();
+    if (r >= nrows) return;
+    const int lane = threadIdx.x;
+    const int i = act[r];
+    const double mi = F.M[i], rhoi = F.RHO[i];
+    double A0 = 0, A1 = 0, A2 = 0, dE = 0, msv = 0;
+    const double sound_i = (rhoi > 0) ? sqrt(P.gamma * F.PR[i] / rhoi) : sqrt(P.gamma * (P.gamma - 1.) * F.U[i]);
+    const bool enabled = (mi > 0 && rhoi > 0);
+    const int c = cnt[r];
+    if (lane == 0) {
+        if (c) atomicAdd(pairs, (unsigned long long)c);
+        if (wakeup) wakeup[i] = 0;  // hydro_force_initial_operations_preloop
+    }
+    if (enabled) {
+        const Cell<R> I = rec[i];
+        const double xi = F.X[i], yi = F.Y[i], zi = F.Z[i];
+        const int tbi = timebin ? timebin[i] : 0;
+        R a0 = 0, a1 = 0, a2 = 0, ae = 0, m = 0;
+        for (int base = 0; base < c; base += 64) {
+            const int q = base + lane;
+            int rc = 0;
+            if (q < c) {
+                const int j = ell[(size_t)q * ld + r];
+                const Cell<R> J = rec[j];
+                double d0 = xi - F.X[j], d1 = yi - F.Y[j], d2 = zi - F.Z[j];
+                d1 = wrap(d1, P.box, P.half);
+                d0 = wrap(d0, P.box, P.half);
+                d2 = wrap(d2, P.box, P.half);
+                const R dp0 = (R)d0, dp1 = (R)d1, dp2 = (R)d2;
+                const R r2 = dp0 * dp0 + dp1 * dp1 + dp2 * dp2;
+                R f0, f1, f2, fp, vts;
+                rc = force_pair<KERN, R>(G, I, J, dp0, dp1, dp2, r2, f0, f1, f2, fp, vts);
+                if (rc < 0) atomicMax(err, 1234);
+                if (rc > 0) {
+                    sv[0][lane] = f0; sv[1][lane] = f1; sv[2][lane] = f2; sv[3][lane] = fp; sv[4][lane] = vts;
+                    if (active && !active[j] && wakeup) {
+                        if ((double)vts > kWakeup * msv_in[j]) { atomicMax(&wakeup[j], tbi + 1); *need_wakeup = 1; }
+                    }
+                }
+            }
+            src[lane] = rc;
+            __syncthreads();
+            if (lane == 0) {
+                const int mm = min(64, c - base);
+                for (int k = 0; k < mm; k++) {
+                    if (m < I.cs) m = I.cs;
+                    if (src[k] <= 0) continue;
+                    a0 += sv[0][k]; a1 += sv[1][k]; a2 += sv[2][k]; ae += sv[3][k];
+                    if (sv[4][k] > m) m = sv[4][k];
+                }
+            }
+            __syncthreads();
+        }
+        A0 = a0; A1 = a1; A2 = a2; dE = ae; msv = m;
+    }
+    if (lane == 0) force_tail(P, F, i, enabled, sound_i, A0, A1, A2, dE, msv, ACC, DTU, MSV);
+}
+
+}  // namespace gz

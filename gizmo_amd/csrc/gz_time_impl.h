@@ -152,13 +152,43 @@ __global__ void __launch_bounds__(256) k_bin_hist(long n, const int *__restrict_
     if (threadIdx.x < 64 && h[threadIdx.x]) atomicAdd(&count[threadIdx.x], (unsigned long long)h[threadIdx.x]);
 }
 
+// the bin limits of process_wake_ups (src/core/timestep.cc:1577-1625) from the occupation
+// after find_timesteps: lim[0] the lowest occupied active bin, lim[1] the highest bin active at
+// the next kick (one lane)
+__global__ void k_wakeup_limits(TimeParams T, const unsigned long long *__restrict__ count, int *__restrict__ lim)
+{
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    long long ti_next_kick = kTimebase;
+    for (int b = 0; b < kTimebins; b++) {
+        if (!count[b]) continue;
+        long long t;
+        if (b > 0) {
+            const long long dt_bin = bin_len(b);
+            t = (T.ti_current / dt_bin) * dt_bin + dt_bin;
+        } else {
+            t = T.ti_current;
+        }
+        if (t < ti_next_kick) ti_next_kick = t;
+    }
+    int max_time_bin_active = 0;
+    for (int b = 0; b < kTimebins; b++)
+        if ((ti_next_kick % (1LL << b)) == 0) max_time_bin_active = b;
+    int lowest = kTimebins;
+    for (int b = 0; b < kTimebins; b++)
+        if (bin_on(T.active_mask, b) && count[b] > 0) { lowest = b; break; }
+    lim[0] = lowest;
+    lim[1] = max_time_bin_active;
+}
+
 // process_wake_ups, src/core/timestep.cc:1565-1774 (every particle stands at the sync point:
-// its Ti_current is All.Ti_Current, so a wake-up never truncates a step)
-__global__ void __launch_bounds__(256) k_wakeups(TimeParams T, ParticleView S, int lowest_occupied_active_bin,
-                                                  int max_time_bin_active, int wakeup_bin_offset, int *__restrict__ nwoken)
+// its Ti_current is All.Ti_Current, so a wake-up never truncates a step); the occupation
+// count follows each move
+__global__ void __launch_bounds__(256) k_wakeups(TimeParams T, ParticleView S, const int *__restrict__ lim, int wakeup_bin_offset,
+                                                  unsigned long long *__restrict__ count, int *__restrict__ nwoken)
 {
     const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
     if (i >= S.n) return;
+    const int lowest_occupied_active_bin = lim[0], max_time_bin_active = lim[1];
     const int w = S.WAKE[i];
     if (!w) return;
     if (S.M[i] <= 0) return;
@@ -179,6 +209,8 @@ __global__ void __launch_bounds__(256) k_wakeups(TimeParams T, ParticleView S, i
     const bool starts_now = bin_on(T.active_mask, bin) && (S.DTS[i] == bin_len(binold));
     S.TB[i] = bin;
     atomicAdd(nwoken, 1);
+    atomicAdd(&count[binold], ~0ull);  // -1
+    atomicAdd(&count[bin], 1ull);
     if (starts_now) kick(T, S, i, old_midpoint, T.ti_current, 1);
     S.TIB[i] = T.ti_current;
     S.DTS[i] = bin_len(bin);
@@ -198,59 +230,73 @@ __global__ void __launch_bounds__(256) k_halfstep_kick(TimeParams T, ParticleVie
 }
 
 // drift_particle_impl, src/core/drift_particle_functions.h:86-250 (MFM gas; the kernel radius
-// moves with the flow, src/declarations/precompiler_logic.h:78), every particle from time0 to
-// time1, then one periodic wrap per axis.  disp: largest periodic distance of any particle
-// from where it was when the particles were last ordered (a positive double's bits order
-// like the double).
+// moves with the flow, src/declarations/precompiler_logic.h:78) of particle i from time0 to
+// time1, then one periodic wrap per axis.  Returns the particle's periodic distance from where
+// it was when the particles were last ordered.
+__device__ __forceinline__ double drift_one(const TimeParams &T, const ParticleView &S, long i, long long time0, long long time1,
+                                            const double *__restrict__ X0, const double *__restrict__ Y0,
+                                            const double *__restrict__ Z0)
+{
+    const double dt_drift = (time1 - time0) * T.tbi * 1.0;
+    double x = S.X[i] + S.VKX[i] * dt_drift;
+    double y = S.Y[i] + S.VKY[i] * dt_drift;
+    double z = S.Z[i] + S.VKZ[i] * dt_drift;
+    const double divv_fac = drift_log_change(S.DIVV[i] * dt_drift);
+    const double h_drift_fac = exp(drift_log_change(S.DIVV[i] * dt_drift) / ((double)3));
+    if (S.M[i] > 0) {
+        const long n = S.n;
+        const double dt_hydrokick = (time1 - time0) * T.tbi;
+        S.VX[i] += 0.0 + S.ACC[i] * (dt_hydrokick * 1.0);
+        S.VY[i] += 0.0 + S.ACC[n + i] * (dt_hydrokick * 1.0);
+        S.VZ[i] += 0.0 + S.ACC[2 * n + i] * (dt_hydrokick * 1.0);
+        const double rho = S.RHO[i] * exp(-divv_fac);
+        const double etmp = S.U[i] + S.DTU[i] * dt_hydrokick;
+        double u = positivity(S.U[i], etmp);
+        if (u < T.min_egy) u = T.min_egy;
+        double h = S.H[i] * h_drift_fac;
+        if (h < T.min_h) h = T.min_h;
+        if (h > T.max_h) h = T.max_h;
+        S.RHO[i] = rho;
+        S.U[i] = u;
+        S.H[i] = h;
+        S.PR[i] = (T.gamma - 1) * u * rho;
+    }
+    if (x >= T.box) x -= T.box; else if (x < 0) x += T.box;
+    if (y >= T.box) y -= T.box; else if (y < 0) y += T.box;
+    if (z >= T.box) z -= T.box; else if (z < 0) z += T.box;
+    S.X[i] = x; S.Y[i] = y; S.Z[i] = z;
+    const double half = 0.5 * T.box;
+    const double ex = fabs(wrap(x - X0[i], T.box, half)), ey = fabs(wrap(y - Y0[i], T.box, half)),
+                 ez = fabs(wrap(z - Z0[i], T.box, half));
+    return fmax(ex, fmax(ey, ez));
+}
+
+// every particle drifted (grid-stride), and the largest displacement since the last ordering
+// and the largest kernel radius after the drift: wave maxima, then one atomic per workgroup
+// (a positive double's bits order like the double)
 __global__ void __launch_bounds__(256) k_drift(TimeParams T, ParticleView S, long long time0, long long time1,
                                                 const double *__restrict__ X0, const double *__restrict__ Y0,
-                                                const double *__restrict__ Z0, unsigned long long *__restrict__ disp)
+                                                const double *__restrict__ Z0, unsigned long long *__restrict__ disp,
+                                                unsigned long long *__restrict__ hmax)
 {
-    const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
-    double d = 0;
-    if (i < S.n && time1 != time0) {
-        const double dt_drift = (time1 - time0) * T.tbi * 1.0;
-        double x = S.X[i] + S.VKX[i] * dt_drift;
-        double y = S.Y[i] + S.VKY[i] * dt_drift;
-        double z = S.Z[i] + S.VKZ[i] * dt_drift;
-        const double divv_fac = drift_log_change(S.DIVV[i] * dt_drift);
-        const double h_drift_fac = exp(drift_log_change(S.DIVV[i] * dt_drift) / ((double)3));
-        if (S.M[i] > 0) {
-            const long n = S.n;
-            const double dt_hydrokick = (time1 - time0) * T.tbi;
-            S.VX[i] += 0.0 + S.ACC[i] * (dt_hydrokick * 1.0);
-            S.VY[i] += 0.0 + S.ACC[n + i] * (dt_hydrokick * 1.0);
-            S.VZ[i] += 0.0 + S.ACC[2 * n + i] * (dt_hydrokick * 1.0);
-            double rho = S.RHO[i] * exp(-divv_fac);
-            const double etmp = S.U[i] + S.DTU[i] * dt_hydrokick;
-            double u = positivity(S.U[i], etmp);
-            if (u < T.min_egy) u = T.min_egy;
-            double h = S.H[i] * h_drift_fac;
-            if (h < T.min_h) h = T.min_h;
-            if (h > T.max_h) h = T.max_h;
-            S.RHO[i] = rho;
-            S.U[i] = u;
-            S.H[i] = h;
-            S.PR[i] = (T.gamma - 1) * u * rho;
-        }
-        if (x >= T.box) x -= T.box; else if (x < 0) x += T.box;
-        if (y >= T.box) y -= T.box; else if (y < 0) y += T.box;
-        if (z >= T.box) z -= T.box; else if (z < 0) z += T.box;
-        S.X[i] = x; S.Y[i] = y; S.Z[i] = z;
-        const double half = 0.5 * T.box;
-        const double ex = fabs(wrap(x - X0[i], T.box, half)), ey = fabs(wrap(y - Y0[i], T.box, half)),
-                     ez = fabs(wrap(z - Z0[i], T.box, half));
-        d = fmax(ex, fmax(ey, ez));
+    double d = 0, hh = 0;
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < S.n; i += (long)gridDim.x * blockDim.x) {
+        if (time1 != time0) d = fmax(d, drift_one(T, S, i, time0, time1, X0, Y0, Z0));
+        hh = fmax(hh, S.H[i]);
     }
-    // block maximum, then one atomic per block
-    __shared__ double sm[256];
-    sm[threadIdx.x] = d;
+    for (int o = 32; o > 0; o >>= 1) {
+        d = fmax(d, __shfl_xor(d, o, 64));
+        hh = fmax(hh, __shfl_xor(hh, o, 64));
+    }
+    __shared__ double sm[4], sh[4];
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) { sm[w] = d; sh[w] = hh; }
     __syncthreads();
-    for (int o = 128; o > 0; o >>= 1) {
-        if (threadIdx.x < o) sm[threadIdx.x] = fmax(sm[threadIdx.x], sm[threadIdx.x + o]);
-        __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int k = 1; k < (int)(blockDim.x >> 6); k++) { d = fmax(d, sm[k]); hh = fmax(hh, sh[k]); }
+        if (d > 0) atomicMax(disp, (unsigned long long)__double_as_longlong(d));
+        if (hh > 0) atomicMax(hmax, (unsigned long long)__double_as_longlong(hh));
     }
-    if (threadIdx.x == 0 && sm[0] > 0) atomicMax(disp, (unsigned long long)__double_as_longlong(sm[0]));
 }
 
 // TimeBinActive of each particle (the byte the force pass tests a neighbour with), and the

@@ -177,7 +177,9 @@ int gz_timeline_get(gz_context *ctx, gz_timeline *out);
 /* Tuning knobs with no counterpart in the reference: "resort_fraction" (default 0.125) -- the
    particles are put back in space-filling-curve order once any of them has moved farther than
    this fraction of the search radius since they were last ordered (the search grid widens its
-   cells by twice that distance meanwhile). */
+   cells by twice that distance meanwhile); "few_rows" (default 8192) -- passes over at most this
+   many active particles run one wavefront per particle (64 lanes share its neighbours) instead
+   of one lane per particle; both give the same sums. */
 int gz_set_option(gz_context *ctx, const char *name, double value);
 
 /* Neighbour lists of the resident state, in the caller's indexing, for parity tests.

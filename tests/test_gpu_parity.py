@@ -63,8 +63,19 @@ def cases(oracle_lib):
     return out
 
 
-def gpu_from(state, par, precision=GZ_FP64, fields=None):
+# the two kernel shapes of every pass: one lane per active particle (cell walks and list loops),
+# and one wavefront per active particle (the shape for steps with few active particles)
+PATHS = {"lanes": 0, "waves": 2_000_000_000}
+
+
+@pytest.fixture(params=list(PATHS))
+def path(request):
+    return request.param
+
+
+def gpu_from(state, par, precision=GZ_FP64, fields=None, path="lanes"):
     g = Hydro(par, state.n, precision)
+    g.set_option("few_rows", PATHS[path])
     g.load(fields if fields is not None else dict(pos=state.pos, vel=state.vel, mass=state.mass, u=state.u, h=state.h))
     return g
 
@@ -86,14 +97,14 @@ def test_neighbor_sets_exact(cases, oracle_lib, name):
 
 
 @pytest.mark.parametrize("name", ["lattice32", "sedov24"])
-def test_density_fixed_h_bitwise(cases, oracle_lib, name):
+def test_density_fixed_h_bitwise(cases, oracle_lib, name, path):
     """At a radius the test fixes (a band wide enough that the first evaluation converges) the
     device sums in the oracle's order: sums and the quantities built from +,-,*,/,sqrt agree
     bit for bit when the oracle is given the particles in device order."""
     c = cases[name]
     par = dict(c["par"], time=1.0, time_begin=0.0, max_num_ngb_dev=1e30)
     s1 = c["s1"]
-    g = gpu_from(s1, par)
+    g = gpu_from(s1, par, path=path)
     perm = g.permutation()
     g.density()
     assert g.stat("density_iterations") == 1
@@ -111,9 +122,9 @@ def test_density_fixed_h_bitwise(cases, oracle_lib, name):
 
 
 @pytest.mark.parametrize("name", ["lattice32", "sedov24"])
-def test_density_iteration_fp64(cases, name):
+def test_density_iteration_fp64(cases, name, path):
     c = cases[name]
-    g = gpu_from(c["s0"], c["par"])
+    g = gpu_from(c["s0"], c["par"], path=path)
     g.density()
     out = g.store(DENSITY_FIELDS)
     assert g.stat("density_iterations") == c["iters"].max()
@@ -124,13 +135,13 @@ def test_density_iteration_fp64(cases, name):
 
 
 @pytest.mark.parametrize("name", ["lattice32", "sedov24"])
-def test_gradients_fp64(cases, name):
+def test_gradients_fp64(cases, name, path):
     c = cases[name]
     s1 = c["s1"]
     f = all_fields(s1)
     for k in GRAD_FIELDS + FORCE_FIELDS:
         f[k] = None
-    g = gpu_from(s1, c["par"], fields=f)
+    g = gpu_from(s1, c["par"], fields=f, path=path)
     g.hydro_gradient_calc()
     out = g.store(GRAD_FIELDS)
     for k in GRAD_FIELDS:
@@ -140,13 +151,13 @@ def test_gradients_fp64(cases, name):
 
 
 @pytest.mark.parametrize("name", ["lattice32", "sedov24"])
-def test_force_fp64(cases, name):
+def test_force_fp64(cases, name, path):
     c = cases[name]
     s2 = c["s2"]
     f = all_fields(s2)
     for k in FORCE_FIELDS:
         f[k] = None
-    g = gpu_from(s2, c["par"], fields=f)
+    g = gpu_from(s2, c["par"], fields=f, path=path)
     g.hydro_force()
     out = g.store(FORCE_FIELDS)
     for k in FORCE_FIELDS:
@@ -156,9 +167,9 @@ def test_force_fp64(cases, name):
 
 
 @pytest.mark.parametrize("name", ["lattice32", "sedov24"])
-def test_full_pass_fp64_and_conservation(cases, name):
+def test_full_pass_fp64_and_conservation(cases, name, path):
     c = cases[name]
-    g = gpu_from(c["s0"], c["par"])
+    g = gpu_from(c["s0"], c["par"], path=path)
     g.hydro_pass()
     out = g.store()
     for k in DENSITY_FIELDS + GRAD_FIELDS + FORCE_FIELDS:
@@ -169,7 +180,7 @@ def test_full_pass_fp64_and_conservation(cases, name):
     g.close()
 
 
-def test_active_subset_and_wakeup(cases, oracle_lib):
+def test_active_subset_and_wakeup(cases, oracle_lib, path):
     """Adaptive steps: only some particles active; inactive neighbours keep their state and are
     woken when an active neighbour's signal speed exceeds WAKEUP times theirs."""
     O = oracle_lib
@@ -183,7 +194,7 @@ def test_active_subset_and_wakeup(cases, oracle_lib):
     s.wakeup[:] = 0
     so = s.copy()
     O.hydro_pass(par, so, active=active)
-    g = gpu_from(s, par, fields=all_fields(s))
+    g = gpu_from(s, par, fields=all_fields(s), path=path)
     g.set_active(active)
     g.hydro_pass()
     out = g.store()
@@ -200,15 +211,15 @@ def test_active_subset_and_wakeup(cases, oracle_lib):
 
 
 @pytest.mark.parametrize("name", ["lattice32", "sedov24"])
-def test_fp32_mixed_error(cases, name):
+def test_fp32_mixed_error(cases, name, path):
     """FP32-mixed form against the FP64 oracle on identical inputs (errors relative to each
     field's largest magnitude; printed per quantity)."""
     c = cases[name]
-    g = gpu_from(c["s0"], c["par"], precision=GZ_FP32_MIXED)
+    g = gpu_from(c["s0"], c["par"], precision=GZ_FP32_MIXED, path=path)
     g.hydro_pass()
     out = g.store()
     errs = {k: relerr(out[k], getattr(c["s3"], k)) for k in DENSITY_FIELDS + GRAD_FIELDS + FORCE_FIELDS}
-    print(name, " ".join(f"{k}={v:.2e}" for k, v in errs.items()))
+    print(name, path, " ".join(f"{k}={v:.2e}" for k, v in errs.items()))
     assert errs["h"] <= 1e-5 and errs["rho"] <= 1e-5
     assert max(errs.values()) <= 1e-3
     g.close()
