@@ -59,6 +59,7 @@ struct gz_context {
     long key_version = 0, grid_version = -1;
     // an upper bound on every kernel radius (< 0: unknown, reduce over all particles)
     double hmax_bound = -1;
+    double hmax_rows = -1;  // the largest clamped radius of the active rows, when already known
     uint8_t *rec_need = nullptr;     // records a pass reads (its rows and their neighbours)
     bool rec_need_valid = false;
     bool all_active = true;
@@ -571,15 +572,18 @@ int density_stage(gz_context *ctx)
     ctx->stats["ngb_rebuilds"] = 0;
     if (na == 0) { ctx->density_done = true; ctx->sym_valid = false; return 0; }
     // one launch: the radius clamp, the iteration state, the counters, and the largest radius
-    HIPCHK(hipMemsetAsync(ctx->u64c + 67, 0, 2 * sizeof(unsigned long long), ctx->stream));
+    double hmax = ctx->hmax_rows;  // known when the step's timeline pass found it
+    ctx->hmax_rows = -1;
+    if (!(hmax > 0)) HIPCHK(hipMemsetAsync(ctx->u64c + 67, 0, sizeof(unsigned long long), ctx->stream));
     k_density_init<<<nblk(na), kBlock, 0, ctx->stream>>>(na, ctx->act, ctx->H, 0.99 * ctx->par.max_kernel_radius, ctx->it_h, ctx->it_l,
                                                          ctx->it_r, ctx->it_cn, ctx->it_ci, ctx->it_it, ctx->need, ctx->counters,
-                                                         ctx->u64c + 67);
+                                                         hmax > 0 ? nullptr : ctx->u64c + 67, ctx->u64c + 68);
     HIPCHK(hipGetLastError());
-    double hmax = 0;
-    HIPCHK(hipMemcpyAsync(ctx->hu64c + 67, ctx->u64c + 67, sizeof(unsigned long long), hipMemcpyDeviceToHost, ctx->stream));
-    HIPCHK(hipStreamSynchronize(ctx->stream));
-    memcpy(&hmax, &ctx->hu64c[67], sizeof hmax);
+    if (!(hmax > 0)) {
+        HIPCHK(hipMemcpyAsync(ctx->hu64c + 67, ctx->u64c + 67, sizeof(unsigned long long), hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        memcpy(&hmax, &ctx->hu64c[67], sizeof hmax);
+    }
     CHK(build_grid(ctx, std::min(hmax, 0.5 * ctx->par.box_size)));
     const bool few = na <= ctx->few_rows;
     if (!few) CHK(build_cell_rows(ctx));
@@ -677,8 +681,8 @@ int ensure_sym(gz_context *ctx)
             if ((long)ctx->nact * 4 < ctx->n) {
                 Timed t2(ctx, "ngb_build");
                 HIPCHK(hipMemsetAsync(ctx->rec_need, 0, ctx->n, ctx->stream));
-                k_mark_records<<<nblk(ctx->nact), kBlock, 0, ctx->stream>>>(ctx->nact, ctx->act, ctx->ell_s, ctx->ld_s, ctx->cnt_s,
-                                                                            ctx->rec_need);
+                k_mark_records<<<(unsigned)((ctx->nact + 3) / 4), kBlock, 0, ctx->stream>>>(ctx->nact, ctx->act, ctx->ell_s, ctx->ld_s,
+                                                                                            ctx->cnt_s, ctx->rec_need);
                 HIPCHK(hipGetLastError());
                 ctx->rec_need_valid = true;
             }
@@ -720,14 +724,13 @@ int force_stage(gz_context *ctx)
     if (!ctx->gradients_done) return fail(ctx, GZ_E_STATE, "hydro_force() before hydro_gradient_calc()");
     if (ctx->nact == 0) return 0;
     CHK(ensure_sym(ctx));
-    HIPCHK(hipMemsetAsync(ctx->counters + 2, 0, sizeof(int), ctx->stream));
-    HIPCHK(hipMemsetAsync(ctx->counters + 4, 0, sizeof(int), ctx->stream));
     ForceView F{ctx->n, ctx->X, ctx->Y, ctx->Z, ctx->VX, ctx->VY, ctx->VZ, ctx->M, ctx->H, ctx->U, ctx->RHO, ctx->PR, ctx->CN,
                 ctx->FCE, ctx->DRK, ctx->GR, ctx->GP, ctx->GV, ctx->T};
     {
         Timed t(ctx, "force");
         Cell<R> *rec = (Cell<R> *)ctx->rec_force;
-        k_pack_force<R><<<nblk(ctx->n), kBlock, 0, ctx->stream>>>(ctx->par.gamma, F, ctx->rec_need_valid ? ctx->rec_need : nullptr, rec);
+        k_pack_force<R><<<nblk(ctx->n), kBlock, 0, ctx->stream>>>(ctx->par.gamma, F, ctx->rec_need_valid ? ctx->rec_need : nullptr, rec,
+                                                                  ctx->counters + 2);
         if (ctx->nact <= ctx->few_rows)
             k_force_rows<KERN, R><<<xcd_grid(ctx->nact), 64, 0, ctx->stream>>>(
                 ctx->dp, gam<R>(ctx->dp), ctx->nact, ctx->act, F, rec, ctx->ell_s, ctx->ld_s, ctx->cnt_s,
@@ -899,67 +902,63 @@ int run_step(gz_context *ctx)
     const long n = ctx->n;
     TimeParams T = time_params(ctx);
     ParticleView S = particle_view(ctx);
-    HIPCHK(hipMemsetAsync(ctx->counters, 0, 8 * sizeof(int), ctx->stream));
-    // find_timesteps: the particles of the active list of the last sync point
+    const long long ti_prev = ctx->ti_current;
+    // The timeline part of the step runs on the device and is read back once: find_timesteps
+    // (src/core/timestep.cc:39-355) over the active list of the last sync point, the bin
+    // occupation, process_wake_ups (:1565-1774), do_first_halfstep_kick (src/core/kicks.cc:34-97),
+    // the next sync point and the drift of every particle to it (src/core/run.cc:857-994), and
+    // make_list_of_active_particles (:997-1009) with the softening cache
     {
         Timed t(ctx, "timeline");
-        if (ctx->nact > 0) {
+        k_step_prep<<<1, 64, 0, ctx->stream>>>(ctx->counters, ctx->u64c);
+        if (ctx->nact > 0)
             k_find_timesteps<<<nblk(ctx->nact), kBlock, 0, ctx->stream>>>(T, S, ctx->nact, ctx->act, ctx->counters + 6);
-            HIPCHK(hipGetLastError());
+        k_bin_hist<<<std::min(nblk(n), 1024u), kBlock, 0, ctx->stream>>>(n, ctx->TB, ctx->u64c);
+        if (ctx->need_wakeup) {
+            int wakeup_bin_offset = 0;
+            while ((1LL << wakeup_bin_offset) < (long long)kWakeup) wakeup_bin_offset++;
+            k_wakeup_limits<<<1, 64, 0, ctx->stream>>>(T, ctx->u64c, ctx->counters + 8);
+            k_wakeups<<<nblk(n), kBlock, 0, ctx->stream>>>(T, S, ctx->counters + 8, wakeup_bin_offset, ctx->u64c, ctx->counters + 5);
         }
-    }
-    // the bin occupation, and process_wake_ups (src/core/timestep.cc:1565-1774) on the device;
-    // one read-back of the occupation and the woken count
-    HIPCHK(hipMemsetAsync(ctx->u64c, 0, 64 * sizeof(unsigned long long), ctx->stream));
-    k_bin_hist<<<std::min(nblk(n), 1024u), kBlock, 0, ctx->stream>>>(n, ctx->TB, ctx->u64c);
-    HIPCHK(hipGetLastError());
-    if (ctx->need_wakeup) {
-        int wakeup_bin_offset = 0;
-        while ((1LL << wakeup_bin_offset) < (long long)kWakeup) wakeup_bin_offset++;
-        Timed t(ctx, "timeline");
-        k_wakeup_limits<<<1, 64, 0, ctx->stream>>>(T, ctx->u64c, ctx->counters + 8);
-        k_wakeups<<<nblk(n), kBlock, 0, ctx->stream>>>(T, S, ctx->counters + 8, wakeup_bin_offset, ctx->u64c, ctx->counters + 5);
+        if (ctx->nact > 0) k_halfstep_kick<<<nblk(ctx->nact), kBlock, 0, ctx->stream>>>(T, S, ctx->nact, ctx->act, 0);
+        k_next_sync<<<1, 64, 0, ctx->stream>>>(T, ctx->u64c, ctx->u64c + 70);
+        k_drift<<<std::min(nblk(n), 2048u), kBlock, 0, ctx->stream>>>(T, S, ti_prev, ctx->u64c + 70, ctx->X0, ctx->Y0, ctx->Z0,
+                                                                     ctx->u64c + 64, ctx->u64c + 66);
+        k_active_flags<<<nblk(n + 1), kBlock, 0, ctx->stream>>>(n, ctx->u64c + 70, ctx->TB, ctx->M, ctx->H,
+                                                                0.99 * ctx->par.max_kernel_radius, ctx->ACT, ctx->SOFT, ctx->iflag,
+                                                                ctx->u64c + 67);
+        HIPCHK(hipGetLastError());
+        size_t bytes = 0;
+        HIPCHK(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, ctx->iflag, ctx->iscan, (int)(n + 1), ctx->stream));
+        CHK(ensure_tmp(ctx, bytes));
+        HIPCHK(hipcub::DeviceScan::ExclusiveSum(ctx->tmp, bytes, ctx->iflag, ctx->iscan, (int)(n + 1), ctx->stream));
+        k_compact_count<<<nblk(n + 1), kBlock, 0, ctx->stream>>>(n, ctx->iflag, ctx->iscan, ctx->act, ctx->counters + 10);
         HIPCHK(hipGetLastError());
     }
     ctx->need_wakeup = 0;
-    HIPCHK(hipMemcpyAsync(ctx->hu64c, ctx->u64c, 64 * sizeof(unsigned long long), hipMemcpyDeviceToHost, ctx->stream));
-    CHK(read_counters(ctx));
+    HIPCHK(hipMemcpyAsync(ctx->hu64c, ctx->u64c, 72 * sizeof(unsigned long long), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipMemcpyAsync(ctx->hcounters, ctx->counters, 16 * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
     for (int b = 0; b < kTimebins; b++) ctx->bin_count[b] = (long long)ctx->hu64c[b];
     ctx->n_woken = ctx->hcounters[5];
     if (ctx->hcounters[6] && !ctx->stop_code) ctx->stop_code = ctx->hcounters[6];
-    // do_first_halfstep_kick over the same active list
-    if (ctx->nact > 0) {
-        Timed t(ctx, "timeline");
-        k_halfstep_kick<<<nblk(ctx->nact), kBlock, 0, ctx->stream>>>(T, S, ctx->nact, ctx->act, 0);
-        HIPCHK(hipGetLastError());
-    }
-    // find_next_sync_point_and_drift (src/core/run.cc:857-994)
-    const long long ti_prev = ctx->ti_current;
-    const long long ti_next = next_kick(ctx);
     ctx->num_steps++;
-    ctx->ti_current = ti_next;
+    ctx->ti_current = (long long)ctx->hu64c[70];
+    ctx->mask = ctx->hu64c[71];
     ctx->time = ctx->ip.time_begin + ctx->ti_current * ctx->tbi;
-    ctx->mask = 1ull;
-    for (int b = 1; b < kTimebins; b++)
-        if ((ti_next % (1LL << b)) == 0) ctx->mask |= 1ull << b;
-    T = time_params(ctx);
-    HIPCHK(hipMemsetAsync(ctx->u64c + 64, 0, sizeof(unsigned long long), ctx->stream));
-    HIPCHK(hipMemsetAsync(ctx->u64c + 66, 0, sizeof(unsigned long long), ctx->stream));
+    ctx->nact = ctx->hcounters[10];
+    ctx->act_version++;
+    ctx->all_active = false;
+    ctx->sym_valid = false;
+    ctx->stats["n_active"] = ctx->nact;
     {
-        Timed t(ctx, "timeline");
-        k_drift<<<std::min(nblk(n), 2048u), kBlock, 0, ctx->stream>>>(T, S, ti_prev, ti_next, ctx->X0, ctx->Y0, ctx->Z0, ctx->u64c + 64,
-                                                                     ctx->u64c + 66);
-        HIPCHK(hipGetLastError());
-    }
-    HIPCHK(hipMemcpyAsync(ctx->hu64c + 64, ctx->u64c + 64, 3 * sizeof(unsigned long long), hipMemcpyDeviceToHost, ctx->stream));
-    // make_list_of_active_particles and the softening cache of the active particles
-    CHK(active_from_bins(ctx));
-    {
-        double d, hm;
+        double d, hm, ha;
         memcpy(&d, &ctx->hu64c[64], sizeof d);
         memcpy(&hm, &ctx->hu64c[66], sizeof hm);
+        memcpy(&ha, &ctx->hu64c[67], sizeof ha);
         ctx->dmax = d;
-        ctx->hmax_bound = hm;  // every radius after the drift
+        ctx->hmax_bound = hm;    // every radius after the drift
+        ctx->hmax_rows = ha;     // the largest radius density() starts its rows from
     }
     // particles that moved far from where they were ordered are ordered again
     if (ctx->last_rmax > 0 && ctx->dmax > ctx->resort_fraction * ctx->last_rmax) {
@@ -967,13 +966,13 @@ int run_step(gz_context *ctx)
         CHK(build_active(ctx));
     }
     // compute_hydro_densities_and_forces, then do_second_halfstep_kick
+    T = time_params(ctx);
     CHK(hydro_pass(ctx));
     if (ctx->nact > 0) {
         Timed t(ctx, "timeline");
         k_halfstep_kick<<<nblk(ctx->nact), kBlock, 0, ctx->stream>>>(T, S, ctx->nact, ctx->act, 1);
         HIPCHK(hipGetLastError());
     }
-
     if (ctx->stop_code) return fail(ctx, ctx->stop_code, "timestep loop: controlled stop %d (step beyond the end of the timeline)", ctx->stop_code);
     return 0;
 }

@@ -76,8 +76,13 @@ __global__ void k_wakeup_limits(TimeParams T, const unsigned long long *count, i
 __global__ void k_wakeups(TimeParams T, ParticleView S, const int *lim, int wakeup_bin_offset, unsigned long long *count,
                           int *nwoken);
 __global__ void k_halfstep_kick(TimeParams T, ParticleView S, int nrows, const int *act, int second);
-__global__ void k_drift(TimeParams T, ParticleView S, long long time0, long long time1, const double *X0, const double *Y0,
-                        const double *Z0, unsigned long long *disp, unsigned long long *hmax);
+__global__ void k_drift(TimeParams T, ParticleView S, long long time0, const unsigned long long *tl, const double *X0,
+                        const double *Y0, const double *Z0, unsigned long long *disp, unsigned long long *hmax);
+__global__ void k_step_prep(int *counters, unsigned long long *u64c);
+__global__ void k_next_sync(TimeParams T, const unsigned long long *count, unsigned long long *tl);
+__global__ void k_active_flags(long n, const unsigned long long *tl, const int *TB, const double *M, const double *H, double hclamp,
+                               uint8_t *ACT, double *SOFT, int *flag, unsigned long long *hmax_act);
+__global__ void k_compact_count(long n, const int *flag, const int *scan, int *act, int *count_out);
 __global__ void k_mark_records(int nrows, const int *act, const int *ell, long ld, const int *cnt, uint8_t *need);
 __global__ void k_active_from_bins(long n, unsigned long long mask, const int *TB, const double *M, const double *H,
                                    uint8_t *ACT, double *SOFT);
@@ -123,7 +128,8 @@ __global__ void k_density_cells(DevParams P, GridView g, CellRows cr, const int 
                                 const double *Y, const double *Z, const double *VX, const double *VY, const double *VZ,
                                 const double *M, const double *hcur, double *accout);
 __global__ void k_density_init(int na, const int *act, double *H, double hclamp, double *hcur, double *left, double *right,
-                               double *cn_cur, int *cn_init, int *iter, uint8_t *need, int *counters, unsigned long long *hmax);
+                               double *cn_cur, int *cn_init, int *iter, uint8_t *need, int *counters, unsigned long long *hmax,
+                               unsigned long long *hfinal);
 __global__ void k_density_after(DevParams P, int nrows, const int *act, uint8_t *need, const double *accin, const double *M,
                                 const double *U, double cell, DensityIter it, DensityOut out, int *nnext, int *rebuild, int *err,
                                 unsigned long long *hfinal);
@@ -132,7 +138,7 @@ template <typename R>
 __global__ void k_pack_grad(long n, const uint8_t *need, const double *H, const double *M, const double *RHO, const double *PR,
                             const double *VX, const double *VY, const double *VZ, GradRec<R> *rec);
 template <typename R>
-__global__ void k_pack_force(double gamma, ForceView F, const uint8_t *need, Cell<R> *rec);
+__global__ void k_pack_force(double gamma, ForceView F, const uint8_t *need, Cell<R> *rec, int *zero2);
 
 template <int KERN, typename R>
 __global__ void k_gradients(DevParams P, int nrows, const int *act, const double *X, const double *Y, const double *Z,

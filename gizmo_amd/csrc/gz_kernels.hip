@@ -43,7 +43,7 @@ GZ_INST(3, double)
 GZ_INST(6, double)
 template __global__ void k_pack_grad<double>(long, const uint8_t *, const double *, const double *, const double *, const double *, const double *,
                                              const double *, const double *, GradRec<double> *);
-template __global__ void k_pack_force<double>(double, ForceView, const uint8_t *, Cell<double> *);
+template __global__ void k_pack_force<double>(double, ForceView, const uint8_t *, Cell<double> *, int *);
 #undef GZ_INST
 
 }  // namespace gz

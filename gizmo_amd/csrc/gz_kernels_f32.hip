@@ -39,7 +39,7 @@ GZ_INST(3, float)
 GZ_INST(6, float)
 template __global__ void k_pack_grad<float>(long, const uint8_t *, const double *, const double *, const double *, const double *, const double *,
                                             const double *, const double *, GradRec<float> *);
-template __global__ void k_pack_force<float>(double, ForceView, const uint8_t *, Cell<float> *);
+template __global__ void k_pack_force<float>(double, ForceView, const uint8_t *, Cell<float> *, int *);
 #undef GZ_INST
 
 }  // namespace gz

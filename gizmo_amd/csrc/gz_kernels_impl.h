@@ -639,10 +639,11 @@ __global__ void __launch_bounds__(256) k_density_init(int na, const int *__restr
                                                       double *__restrict__ hcur, double *__restrict__ left, double *__restrict__ right,
                                                       double *__restrict__ cn_cur, int *__restrict__ cn_init, int *__restrict__ iter,
                                                       uint8_t *__restrict__ need, int *__restrict__ counters,
-                                                      unsigned long long *__restrict__ hmax)
+                                                      unsigned long long *__restrict__ hmax, unsigned long long *__restrict__ hfinal)
 {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r < 8) counters[r] = 0;
+    if (r == 0) *hfinal = 0;
     double h = 0;
     if (r < na) {
         const int i = act[r];
@@ -652,6 +653,7 @@ __global__ void __launch_bounds__(256) k_density_init(int na, const int *__restr
         left[r] = 0; right[r] = 0; cn_cur[r] = 0; cn_init[r] = 0; iter[r] = 0;
         need[r] = 1;
     }
+    if (!hmax) return;
     for (int o = 32; o > 0; o >>= 1) h = fmax(h, __shfl_xor(h, o, 64));
     if ((threadIdx.x & 63) == 0 && h > 0) atomicMax(hmax, (unsigned long long)__double_as_longlong(h));
 }
@@ -762,11 +764,13 @@ __global__ void __launch_bounds__(64) k_build_sym_cells(GridView g, CellRows cr,
 __global__ void k_mark_records(int nrows, const int *__restrict__ act, const int *__restrict__ ell, long ld,
                                const int *__restrict__ cnt, uint8_t *__restrict__ need)
 {
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    // one wavefront per row, its lanes over the row's entries
+    const long r = blockIdx.x * (long)(blockDim.x >> 6) + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
     if (r >= nrows) return;
-    need[act[r]] = 1;
+    if (lane == 0) need[act[r]] = 1;
     const int c = cnt[r];
-    for (int q = 0; q < c; q++) need[ell[(size_t)q * ld + r]] = 1;
+    for (int q = lane; q < c; q += 64) need[ell[(size_t)q * ld + r]] = 1;
 }
 #endif
 
@@ -788,10 +792,12 @@ __global__ void k_pack_grad(long n, const uint8_t *__restrict__ need, const doub
 }
 
 template <typename R>
-__global__ void k_pack_force(double gamma, ForceView F, const uint8_t *__restrict__ need, Cell<R> *__restrict__ rec)
+__global__ void k_pack_force(double gamma, ForceView F, const uint8_t *__restrict__ need, Cell<R> *__restrict__ rec,
+                             int *__restrict__ zero2)
 {
     const long j = blockIdx.x * (long)blockDim.x + threadIdx.x;
     const long n = F.n;
+    if (j == 0 && zero2) { zero2[0] = 0; zero2[2] = 0; }  // the force pass's error and wake-up flags
     if (j >= n) return;
     if (need && !need[j]) return;
     Cell<R> c;

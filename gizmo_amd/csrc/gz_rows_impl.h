@@ -1,10 +1,10 @@
 // gz_rows_impl.h -- the hot-path passes with one wavefront per active row, for steps in which
 // few particles are active (the usual case of adaptive time bins: a few hundred hot particles
 // in a resting medium).  The 64 lanes of the row's wavefront evaluate 64 candidates or list
-// entries at once; each lane's pair terms go to LDS and lane 0 adds them up in ascending
-// candidate order, so the sums are the same as those of the lane-per-row kernels (and of the
-// oracle) operation for operation.  One wavefront per workgroup: the block barrier is a wave
-// barrier.
+// entries at once; each lane's pair terms go to LDS, and then lane k keeps running sum k (one
+// lane per accumulated quantity) over the accepted entries in ascending order, so every sum is
+// the same chain of additions as in the lane-per-row kernels (and the oracle).  One wavefront
+// per workgroup: the block barrier is a wave barrier.
 #pragma once
 #include "gz_kernels_impl.h"
 
@@ -90,8 +90,7 @@ __global__ void __launch_bounds__(64) k_density_rows(DevParams P, GridView g, in
     kernel_hinv(hR, hinv, hinv3, hinv4);
     row_stencil<false>(g, key[i] >> g.shift, xi, yi, zi, h, rs);
     const int total = rs.pref[rs.nst];
-    R a[13];
-    for (int q = 0; q < 13; q++) a[q] = 0;
+    R acc = 0;  // lane q < 13 keeps sum q
     for (int base = 0; base < total; base += 64) {
         const int v = base + lane;
         int f = 0;
@@ -136,24 +135,21 @@ __global__ void __launch_bounds__(64) k_density_rows(DevParams P, GridView g, in
         if (!mask) continue;
         sf[lane] = f;
         __syncthreads();
-        if (lane == 0) {
+        if (lane < 13) {
             while (mask) {
                 const int k = __ffsll((long long)mask) - 1;
                 mask &= mask - 1;
                 const int fk = sf[k];
-                a[0] += sc[0][k];
-                a[1] += sc[1][k];
-                a[2] += sc[2][k];
-                if (fk == 2) {
-                    for (int q = 4; q < 13; q++) a[q] += sc[q][k];
-                    a[3] -= sc[3][k];
+                if (lane < 3) acc += sc[lane][k];
+                else if (fk == 2) {
+                    if (lane == 3) acc -= sc[3][k];
+                    else acc += sc[lane][k];
                 }
             }
         }
         __syncthreads();
     }
-    if (lane == 0)
-        for (int q = 0; q < 13; q++) accout[(size_t)q * nrows + r] = (double)a[q];
+    if (lane < 13) accout[(size_t)lane * nrows + r] = (double)acc;
 }
 
 // the symmetric list of few rows (the same entries, in the same order, as k_build_sym_cells)
@@ -208,10 +204,9 @@ __global__ void __launch_bounds__(64) k_gradients_rows(DevParams P, int nrows, c
     const int lane = threadIdx.x;
     const int i = act[r];
     R G[3][5];
-    for (int a = 0; a < 3; a++) for (int b = 0; b < 5; b++) G[a][b] = 0;
     R mx[5], mn[5];
-    for (int b = 0; b < 5; b++) { mx[b] = 0; mn[b] = 0; }
     R maxd = 0;
+    R acc = 0;  // the running quantity this lane keeps
     const GradRec<R> I = rec[i];
     const bool enabled = (I.m > 0 && I.rho > 0);
     const R hi = I.h;
@@ -253,32 +248,43 @@ __global__ void __launch_bounds__(64) k_gradients_rows(DevParams P, int nrows, c
                     }
                 }
             }
+            // lanes 0-14 keep G[kk][b] (kk = lane/5, b = lane%5), 15-19 the minima, 20-24 the maxima,
+            // 25 the largest distance (with mn <= 0 <= mx the reference's else-if is two ifs)
+            unsigned long long mask = __ballot(f != 0);
+            if (!mask) continue;
             sf[lane] = f;
             __syncthreads();
-            if (lane == 0) {
-                const int m = min(64, c - base);
-                for (int k = 0; k < m; k++) {
-                    const int fk = sf[k];
-                    if (!fk) continue;
-                    const R rr = srr[k];
-                    if (rr > maxd) maxd = rr;
-                    R d[5];
-                    for (int b = 0; b < 5; b++) d[b] = sd[b][k];
-                    for (int b = 0; b < 5; b++) {
-                        if (d[b] < mn[b]) mn[b] = d[b];
-                        else if (d[b] > mx[b]) mx[b] = d[b];
+            if (lane < 26) {
+                while (mask) {
+                    const int k = __ffsll((long long)mask) - 1;
+                    mask &= mask - 1;
+                    if (lane < 15) {
+                        if (sf[k] == 2) acc += sw[lane / 5][k] * sd[lane % 5][k];
+                    } else if (lane < 20) {
+                        const R d = sd[lane - 15][k];
+                        if (d < acc) acc = d;
+                    } else if (lane < 25) {
+                        const R d = sd[lane - 20][k];
+                        if (d > acc) acc = d;
+                    } else {
+                        const R rr = srr[k];
+                        if (rr > acc) acc = rr;
                     }
-                    if (fk == 2)
-                        for (int kk = 0; kk < 3; kk++) {
-                            const R w = sw[kk][k];
-                            for (int b = 0; b < 5; b++) G[kk][b] += w * d[b];
-                        }
                 }
             }
             __syncthreads();
         }
     }
-    if (lane == 0) gradients_tail<R>(I, enabled, G, mx, mn, maxd, T, CN, n, i, GR, GP, GV);
+    __shared__ R res[26];
+    if (lane < 26) res[lane] = acc;
+    __syncthreads();
+    if (lane == 0) {
+        for (int kk = 0; kk < 3; kk++)
+            for (int b = 0; b < 5; b++) G[kk][b] = res[kk * 5 + b];
+        for (int b = 0; b < 5; b++) { mn[b] = res[15 + b]; mx[b] = res[20 + b]; }
+        maxd = res[25];
+        gradients_tail<R>(I, enabled, G, mx, mn, maxd, T, CN, n, i, GR, GP, GV);
+    }
 }
 
 // force of few rows: each lane evaluates one list entry (hydro_accumulate_neighbor), lane 0
@@ -293,7 +299,6 @@ __global__ void __launch_bounds__(64) k_force_rows(DevParams P, Gam<R> G, int nr
                                                    unsigned long long *__restrict__ pairs)
 {
     __shared__ R sv[5][64];
-    __shared__ int src[64];
     const long r = xcd_block();
     if (r >= nrows) return;
     const int lane = threadIdx.x;
@@ -311,7 +316,10 @@ __global__ void __launch_bounds__(64) k_force_rows(DevParams P, Gam<R> G, int nr
         const Cell<R> I = rec[i];
         const double xi = F.X[i], yi = F.Y[i], zi = F.Z[i];
         const int tbi = timebin ? timebin[i] : 0;
-        R a0 = 0, a1 = 0, a2 = 0, ae = 0, m = 0;
+        // lanes 0-3 keep the momentum and energy sums, lane 4 the signal speed: max(c_s, vsig)
+        // over the entries (the reference raises it to c_s before each entry, a no-op after the first)
+        R acc = 0;
+        if (lane == 4 && c > 0) acc = (acc < I.cs) ? I.cs : acc;
         for (int base = 0; base < c; base += 64) {
             const int q = base + lane;
             int rc = 0;
@@ -334,20 +342,23 @@ __global__ void __launch_bounds__(64) k_force_rows(DevParams P, Gam<R> G, int nr
                     }
                 }
             }
-            src[lane] = rc;
+            unsigned long long mask = __ballot(rc > 0);
+            if (!mask) continue;
             __syncthreads();
-            if (lane == 0) {
-                const int mm = min(64, c - base);
-                for (int k = 0; k < mm; k++) {
-                    if (m < I.cs) m = I.cs;
-                    if (src[k] <= 0) continue;
-                    a0 += sv[0][k]; a1 += sv[1][k]; a2 += sv[2][k]; ae += sv[3][k];
-                    if (sv[4][k] > m) m = sv[4][k];
+            if (lane < 5) {
+                while (mask) {
+                    const int k = __ffsll((long long)mask) - 1;
+                    mask &= mask - 1;
+                    if (lane < 4) acc += sv[lane][k];
+                    else if (sv[4][k] > acc) acc = sv[4][k];
                 }
             }
             __syncthreads();
         }
-        A0 = a0; A1 = a1; A2 = a2; dE = ae; msv = m;
+        __shared__ R res[5];
+        if (lane < 5) res[lane] = acc;
+        __syncthreads();
+        A0 = res[0]; A1 = res[1]; A2 = res[2]; dE = res[3]; msv = res[4];
     }
     if (lane == 0) force_tail(P, F, i, enabled, sound_i, A0, A1, A2, dE, msv, ACC, DTU, MSV);
 }

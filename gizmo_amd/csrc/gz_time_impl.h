@@ -146,8 +146,23 @@ __global__ void __launch_bounds__(256) k_bin_hist(long n, const int *__restrict_
     __shared__ unsigned int h[64];
     if (threadIdx.x < 64) h[threadIdx.x] = 0;
     __syncthreads();
-    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
-        atomicAdd(&h[TB[i] & 63], 1u);
+    // lanes with the same bin are counted together (most particles of a wave share one bin)
+    const int lane = threadIdx.x & 63;
+    const long stride = (long)gridDim.x * blockDim.x;
+    const long base0 = blockIdx.x * (long)blockDim.x + (threadIdx.x & ~63);
+    for (long base = base0; base < n; base += stride) {
+        const long i = base + lane;
+        int b = (i < n) ? (TB[i] & 63) : -1;
+        while (true) {
+            const unsigned long long live = __ballot(b >= 0);
+            if (!live) break;
+            const int leader = __ffsll((long long)live) - 1;
+            const int lb = __shfl(b, leader, 64);
+            const unsigned long long same = __ballot(b == lb);
+            if (lane == leader) atomicAdd(&h[lb], (unsigned)__popcll(same));
+            if (b == lb) b = -1;
+        }
+    }
     __syncthreads();
     if (threadIdx.x < 64 && h[threadIdx.x]) atomicAdd(&count[threadIdx.x], (unsigned long long)h[threadIdx.x]);
 }
@@ -274,11 +289,12 @@ __device__ __forceinline__ double drift_one(const TimeParams &T, const ParticleV
 // every particle drifted (grid-stride), and the largest displacement since the last ordering
 // and the largest kernel radius after the drift: wave maxima, then one atomic per workgroup
 // (a positive double's bits order like the double)
-__global__ void __launch_bounds__(256) k_drift(TimeParams T, ParticleView S, long long time0, long long time1,
+__global__ void __launch_bounds__(256) k_drift(TimeParams T, ParticleView S, long long time0, const unsigned long long *__restrict__ tl,
                                                 const double *__restrict__ X0, const double *__restrict__ Y0,
                                                 const double *__restrict__ Z0, unsigned long long *__restrict__ disp,
                                                 unsigned long long *__restrict__ hmax)
 {
+    const long long time1 = (long long)tl[0];  // the next sync point, found on the device
     double d = 0, hh = 0;
     for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < S.n; i += (long)gridDim.x * blockDim.x) {
         if (time1 != time0) d = fmax(d, drift_one(T, S, i, time0, time1, X0, Y0, Z0));
@@ -297,6 +313,76 @@ __global__ void __launch_bounds__(256) k_drift(TimeParams T, ParticleView S, lon
         if (d > 0) atomicMax(disp, (unsigned long long)__double_as_longlong(d));
         if (hh > 0) atomicMax(hmax, (unsigned long long)__double_as_longlong(hh));
     }
+}
+
+// the start of a step: counters and the bin occupation cleared (the running pair count kept)
+__global__ void k_step_prep(int *__restrict__ counters, unsigned long long *__restrict__ u64c)
+{
+    const int t = threadIdx.x;
+    if (t < 16) counters[t] = 0;
+    if (t < 64) u64c[t] = 0;
+    if (t == 0) { u64c[64] = 0; u64c[66] = 0; u64c[67] = 0; u64c[68] = 0; }
+}
+
+// find_next_sync_point_and_drift's timeline part (src/core/run.cc:857-930) from the occupation:
+// tl[0] the next kick (All.Ti_Current after the step), tl[1] the bins active there (one lane)
+__global__ void k_next_sync(TimeParams T, const unsigned long long *__restrict__ count, unsigned long long *__restrict__ tl)
+{
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    long long ti_next_kick = kTimebase;
+    for (int b = 0; b < kTimebins; b++) {
+        if (!count[b]) continue;
+        long long t;
+        if (b > 0) {
+            const long long dt_bin = bin_len(b);
+            t = (T.ti_current / dt_bin) * dt_bin + dt_bin;
+        } else {
+            t = T.ti_current;
+        }
+        if (t < ti_next_kick) ti_next_kick = t;
+    }
+    unsigned long long mask = 1ull;
+    for (int b = 1; b < kTimebins; b++)
+        if ((ti_next_kick % bin_len(b)) == 0) mask |= 1ull << b;
+    tl[0] = (unsigned long long)ti_next_kick;
+    tl[1] = mask;
+}
+
+// make_list_of_active_particles (src/core/run.cc:997-1009) as flags for the scan: a particle of
+// an active bin with mass; its TimeBinActive byte (the force pass tests neighbours with it); the
+// softening cache of the active particles (src/gravity/gravtree.cc:267); and the largest
+// radius of the active particles after density()'s clamp (src/hydro/density_loop.cc:818-838)
+__global__ void __launch_bounds__(256) k_active_flags(long n, const unsigned long long *__restrict__ tl, const int *__restrict__ TB,
+                                                       const double *__restrict__ M, const double *__restrict__ H, double hclamp,
+                                                       uint8_t *__restrict__ ACT, double *__restrict__ SOFT, int *__restrict__ flag,
+                                                       unsigned long long *__restrict__ hmax_act)
+{
+    const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    const unsigned long long mask = tl[1];
+    double h = 0;
+    if (i < n) {
+        const bool on = bin_on(mask, TB[i]);
+        ACT[i] = on ? 1 : 0;
+        const bool row = on && (M[i] > 0);
+        flag[i] = row ? 1 : 0;
+        if (row) {
+            h = H[i];
+            SOFT[i] = h;
+            if ((h < 0) || !isfinite(h) || (h > hclamp)) h = hclamp;
+        }
+    }
+    if (i == n) flag[n] = 0;
+    for (int o = 32; o > 0; o >>= 1) h = fmax(h, __shfl_xor(h, o, 64));
+    if ((threadIdx.x & 63) == 0 && h > 0) atomicMax(hmax_act, (unsigned long long)__double_as_longlong(h));
+}
+
+// the scanned flags to the active list; the count to count_out
+__global__ void __launch_bounds__(256) k_compact_count(long n, const int *__restrict__ flag, const int *__restrict__ scan,
+                                                        int *__restrict__ act, int *__restrict__ count_out)
+{
+    const long s = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (s < n && flag[s]) act[scan[s]] = (int)s;
+    if (s == n) *count_out = scan[n];
 }
 
 // TimeBinActive of each particle (the byte the force pass tests a neighbour with), and the
