@@ -71,6 +71,8 @@ def new_job(ic, precision):
 
 
 def run_job(ic, precision, steps, warmup):
+    """The timed job (no events on the stream), then the same job again with HIP events around
+    every launch family for the per-kernel times and the roofline."""
     if warmup > 0:
         g = new_job(ic, precision)
         for _ in range(warmup):
@@ -78,7 +80,6 @@ def run_job(ic, precision, steps, warmup):
         g.close()
     g = new_job(ic, precision)
     g.synchronize()
-    g.timing_enable(True)
     g.timing_reset()
     active, woken = [], []
     t0 = time.perf_counter()
@@ -87,9 +88,18 @@ def run_job(ic, precision, steps, warmup):
         woken.append(g.timeline()["n_woken"])
     g.synchronize()
     dt = time.perf_counter() - t0
-    kern = {k: g.timing(k) for k in FAMILIES}
-    res = dict(seconds=dt, active=active, woken=woken, kernels=kern, pairs=g.stat("force_pairs_total"),
-               resorts=g.stat("resorts"), updates=g.stat("active_updates"), tl=g.timeline())
+    res = dict(seconds=dt, active=active, woken=woken, resorts=g.stat("resorts"), updates=g.stat("active_updates"),
+               tl=g.timeline())
+    g.close()
+    g = new_job(ic, precision)
+    g.synchronize()
+    g.timing_enable(True)
+    g.timing_reset()
+    active2 = [g.run_step() for _ in range(steps)]
+    g.synchronize()
+    assert active2 == active, "the profiled job took other steps than the timed one"
+    res["kernels"] = {k: g.timing(k) for k in FAMILIES}
+    res["pairs"] = g.stat("force_pairs_total")
     g.timing_enable(False)
     g.close()
     return res
