@@ -18,8 +18,9 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libgizmo_hip.so")
-SOURCES = ["gz_kernels.hip", "gz_kernels_f32.hip", "gz_api.hip"]
-HEADERS = ["gz_device.h", "gz_kernels.h", "gz_kernels_impl.h", os.path.join("..", "..", "include", "gizmo_hydro.h")]
+SOURCES = ["gz_kernels.hip", "gz_kernels_f32.hip", "gz_api.hip", "gz_batch.hip"]
+HEADERS = ["gz_device.h", "gz_kernels.h", "gz_kernels_impl.h", "gz_rows_impl.h", "gz_time_impl.h", "gz_batch_impl.h",
+           os.path.join("..", "..", "include", "gizmo_hydro.h"), os.path.join("..", "..", "include", "gizmo_batch.h")]
 
 
 def hipcc():
@@ -36,6 +37,7 @@ FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
 SOURCE_FLAGS = {
     "gz_kernels.hip": ["-ffp-contract=off"],
     "gz_api.hip": ["-ffp-contract=off"],
+    "gz_batch.hip": ["-ffp-contract=off"],
     "gz_kernels_f32.hip": ["-ffp-contract=fast", "-fno-hip-fp32-correctly-rounded-divide-sqrt"],
 }
 

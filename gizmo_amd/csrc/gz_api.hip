@@ -437,6 +437,9 @@ GridView grid_view(gz_context *ctx)
     g.nsten = ctx->nsten;
     g.chmax = ctx->chmax;
     g.margin = ctx->dmax;
+    g.cstart = ctx->cstart;
+    g.cend = ctx->cend;
+    g.cell_off = 0;
     return g;
 }
 
@@ -577,7 +580,7 @@ int density_stage(gz_context *ctx)
     if (!(hmax > 0)) HIPCHK(hipMemsetAsync(ctx->u64c + 67, 0, sizeof(unsigned long long), ctx->stream));
     k_density_init<<<nblk(na), kBlock, 0, ctx->stream>>>(na, ctx->act, ctx->H, 0.99 * ctx->par.max_kernel_radius, ctx->it_h, ctx->it_l,
                                                          ctx->it_r, ctx->it_cn, ctx->it_ci, ctx->it_it, ctx->need, ctx->counters,
-                                                         hmax > 0 ? nullptr : ctx->u64c + 67, ctx->u64c + 68);
+                                                         hmax > 0 ? nullptr : ctx->u64c + 67, ctx->u64c + 68, BoxTable{});
     HIPCHK(hipGetLastError());
     if (!(hmax > 0)) {
         HIPCHK(hipMemcpyAsync(ctx->hu64c + 67, ctx->u64c + 67, sizeof(unsigned long long), hipMemcpyDeviceToHost, ctx->stream));
@@ -597,14 +600,14 @@ int density_stage(gz_context *ctx)
             if (few)
                 k_density_rows<KERN, R><<<xcd_grid(na), 64, 0, ctx->stream>>>(ctx->dp, grid_view(ctx), na, ctx->act, ctx->need, ctx->key,
                                                                             ctx->X, ctx->Y, ctx->Z, ctx->VX, ctx->VY, ctx->VZ, ctx->M,
-                                                                            ctx->it_h, ctx->dacc);
+                                                                            ctx->it_h, ctx->dacc, BoxTable{});
             else
                 k_density_cells<KERN, R><<<xcd_grid(ctx->ncells), 64, 0, ctx->stream>>>(
                     ctx->dp, grid_view(ctx), cell_rows(ctx), ctx->act, ctx->need, ctx->X, ctx->Y, ctx->Z, ctx->VX, ctx->VY, ctx->VZ,
                     ctx->M, ctx->it_h, ctx->dacc);
             k_density_after<<<nblk(na), kBlock, 0, ctx->stream>>>(ctx->dp, na, ctx->act, ctx->need, ctx->dacc, ctx->M, ctx->U,
                                                                   ctx->cell - 2.0 * ctx->dmax, it, out, ctx->counters + 0, ctx->counters + 1,
-                                                                  ctx->counters + 2, ctx->u64c + 68);
+                                                                  ctx->counters + 2, ctx->u64c + 68, BoxTable{});
             HIPCHK(hipGetLastError());
         }
         CHK(read_counters(ctx));
@@ -664,7 +667,7 @@ int ensure_sym(gz_context *ctx)
             if (few)
                 k_build_sym_rows<<<xcd_grid(ctx->nact), 64, 0, ctx->stream>>>(grid_view(ctx), ctx->nact, ctx->act, ctx->key, ctx->X, ctx->Y,
                                                                                ctx->Z, ctx->H, ctx->ell_s, ctx->ld_s, ctx->cap_s, ctx->cnt_s,
-                                                                               ctx->counters + 3);
+                                                                               ctx->counters + 3, BoxTable{});
             else
                 k_build_sym_cells<<<xcd_grid(ctx->ncells), 64, 0, ctx->stream>>>(grid_view(ctx), cell_rows(ctx), ctx->act, ctx->X, ctx->Y,
                                                                                   ctx->Z, ctx->H, ctx->ell_s, ctx->ld_s, ctx->cap_s,
@@ -708,7 +711,7 @@ int gradients_stage(gz_context *ctx)
     if (ctx->nact <= ctx->few_rows)
         k_gradients_rows<KERN, R><<<xcd_grid(ctx->nact), 64, 0, ctx->stream>>>(ctx->dp, ctx->nact, ctx->act, ctx->X, ctx->Y, ctx->Z,
                                                                                rec, ctx->T, ctx->CN, ctx->n, ctx->ell_s, ctx->ld_s,
-                                                                               ctx->cnt_s, ctx->GR, ctx->GP, ctx->GV);
+                                                                               ctx->cnt_s, ctx->GR, ctx->GP, ctx->GV, BoxTable{});
     else
         k_gradients<KERN, R><<<xcd_grid(nblk(ctx->nact)), kBlock, 0, ctx->stream>>>(ctx->dp, ctx->nact, ctx->act, ctx->X, ctx->Y, ctx->Z,
                                                                                     rec, ctx->T, ctx->CN, ctx->n, ctx->ell_s, ctx->ld_s,
@@ -735,7 +738,7 @@ int force_stage(gz_context *ctx)
             k_force_rows<KERN, R><<<xcd_grid(ctx->nact), 64, 0, ctx->stream>>>(
                 ctx->dp, gam<R>(ctx->dp), ctx->nact, ctx->act, F, rec, ctx->ell_s, ctx->ld_s, ctx->cnt_s,
                 ctx->all_active ? nullptr : ctx->ACT, ctx->TB, ctx->MSV, ctx->WAKE, ctx->ACC, ctx->DTU, ctx->MSV, ctx->counters + 2,
-                ctx->counters + 4, ctx->u64c + 65);
+                ctx->counters + 4, ctx->u64c + 65, BoxTable{});
         else
             k_force<KERN, R><<<xcd_grid(nblk(ctx->nact)), kBlock, 0, ctx->stream>>>(
                 ctx->dp, gam<R>(ctx->dp), ctx->nact, ctx->act, F, rec, ctx->ell_s, ctx->ld_s, ctx->cnt_s,
@@ -849,7 +852,7 @@ int read_bins(gz_context *ctx)
 {
     const long n = ctx->n;
     HIPCHK(hipMemsetAsync(ctx->u64c, 0, 64 * sizeof(unsigned long long), ctx->stream));
-    k_bin_hist<<<std::min(nblk(n), 1024u), kBlock, 0, ctx->stream>>>(n, ctx->TB, ctx->u64c);
+    k_bin_hist<<<std::min(nblk(n), 1024u), kBlock, 0, ctx->stream>>>(n, ctx->TB, ctx->u64c, BoxTable{});
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(ctx->hu64c, ctx->u64c, 64 * sizeof(unsigned long long), hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
@@ -912,21 +915,21 @@ int run_step(gz_context *ctx)
         Timed t(ctx, "timeline");
         k_step_prep<<<1, 64, 0, ctx->stream>>>(ctx->counters, ctx->u64c);
         if (ctx->nact > 0)
-            k_find_timesteps<<<nblk(ctx->nact), kBlock, 0, ctx->stream>>>(T, S, ctx->nact, ctx->act, ctx->counters + 6);
-        k_bin_hist<<<std::min(nblk(n), 1024u), kBlock, 0, ctx->stream>>>(n, ctx->TB, ctx->u64c);
+            k_find_timesteps<<<nblk(ctx->nact), kBlock, 0, ctx->stream>>>(T, S, ctx->nact, ctx->act, ctx->counters + 6, BoxTable{});
+        k_bin_hist<<<std::min(nblk(n), 1024u), kBlock, 0, ctx->stream>>>(n, ctx->TB, ctx->u64c, BoxTable{});
         if (ctx->need_wakeup) {
             int wakeup_bin_offset = 0;
             while ((1LL << wakeup_bin_offset) < (long long)kWakeup) wakeup_bin_offset++;
-            k_wakeup_limits<<<1, 64, 0, ctx->stream>>>(T, ctx->u64c, ctx->counters + 8);
-            k_wakeups<<<nblk(n), kBlock, 0, ctx->stream>>>(T, S, ctx->counters + 8, wakeup_bin_offset, ctx->u64c, ctx->counters + 5);
+            k_wakeup_limits<<<1, 64, 0, ctx->stream>>>(T, ctx->u64c, ctx->counters + 8, BoxTable{});
+            k_wakeups<<<nblk(n), kBlock, 0, ctx->stream>>>(T, S, ctx->counters + 8, wakeup_bin_offset, ctx->u64c, ctx->counters + 5, BoxTable{});
         }
-        if (ctx->nact > 0) k_halfstep_kick<<<nblk(ctx->nact), kBlock, 0, ctx->stream>>>(T, S, ctx->nact, ctx->act, 0);
-        k_next_sync<<<1, 64, 0, ctx->stream>>>(T, ctx->u64c, ctx->u64c + 70);
+        if (ctx->nact > 0) k_halfstep_kick<<<nblk(ctx->nact), kBlock, 0, ctx->stream>>>(T, S, ctx->nact, ctx->act, 0, BoxTable{});
+        k_next_sync<<<1, 64, 0, ctx->stream>>>(T, ctx->u64c, ctx->u64c + 70, BoxTable{});
         k_drift<<<std::min(nblk(n), 2048u), kBlock, 0, ctx->stream>>>(T, S, ti_prev, ctx->u64c + 70, ctx->X0, ctx->Y0, ctx->Z0,
-                                                                     ctx->u64c + 64, ctx->u64c + 66);
+                                                                     ctx->u64c + 64, ctx->u64c + 66, BoxTable{});
         k_active_flags<<<nblk(n + 1), kBlock, 0, ctx->stream>>>(n, ctx->u64c + 70, ctx->TB, ctx->M, ctx->H,
                                                                 0.99 * ctx->par.max_kernel_radius, ctx->ACT, ctx->SOFT, ctx->iflag,
-                                                                ctx->u64c + 67);
+                                                                ctx->u64c + 67, BoxTable{});
         HIPCHK(hipGetLastError());
         size_t bytes = 0;
         HIPCHK(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, ctx->iflag, ctx->iscan, (int)(n + 1), ctx->stream));
@@ -970,7 +973,7 @@ int run_step(gz_context *ctx)
     CHK(hydro_pass(ctx));
     if (ctx->nact > 0) {
         Timed t(ctx, "timeline");
-        k_halfstep_kick<<<nblk(ctx->nact), kBlock, 0, ctx->stream>>>(T, S, ctx->nact, ctx->act, 1);
+        k_halfstep_kick<<<nblk(ctx->nact), kBlock, 0, ctx->stream>>>(T, S, ctx->nact, ctx->act, 1, BoxTable{});
         HIPCHK(hipGetLastError());
     }
     if (ctx->stop_code) return fail(ctx, ctx->stop_code, "timestep loop: controlled stop %d (step beyond the end of the timeline)", ctx->stop_code);

@@ -26,7 +26,47 @@ struct GridView {
     const int *nsten;
     const double *chmax;      // largest kernel radius in each cell (when computed)
     double margin;            // largest distance a particle has moved from its cell since the ordering
+    const int *cstart, *cend; // [start, end) of each cell
+    long long cell_off;       // first cell of the box in the cell arrays (batches of boxes)
 };
+
+// One periodic box of a batch: geometry, grid and timeline.  A batch holds many independent
+// boxes in one particle array (box b in its own index range, each in its own Morton order);
+// the row and timeline kernels look the box of each particle up here.
+struct BoxDev {
+    double box, half, min_h, max_h;      // side, All.MinKernelRadius, All.MaxKernelRadius
+    double cell, margin, rowlim;         // grid cell side, displacement bound, radius limit of density rows
+    int level, shift;
+    long long cell_off;                  // first cell of the box in the cell arrays
+    double time, time_begin, tbi;        // All.Time, All.TimeBegin, All.Timebase_interval
+    long long ti_current, ti_next;       // All.Ti_Current and the next sync point
+    unsigned long long mask, mask_next;  // TimeBinActive now and at the next sync point
+    int stepping;                        // 1: the box takes part in the current batch step
+    int need_wakeup;                     // NeedToWakeupParticles of the box (set by its force pass)
+    double courant_fac, err_tol, max_dt, min_dt, min_egy;  // the box's All.CourantFac ... All.MinEgySpec
+};
+
+struct BoxTable {
+    BoxDev *b;                // nullptr: one box, held by the kernel arguments
+    const int *BOX;           // box of each particle
+    unsigned long long *slot; // per box, kBoxSlots running maxima (bits of positive doubles)
+    int nbox;
+};
+// the per-box maxima: displacement since the ordering, radius after the drift, radius of the
+// active rows, radius of density rows that outgrew their cell, final radius of density rows
+// (and the particles woken, and the controlled-stop code of the box's timeline)
+enum { kSlotDisp = 0, kSlotHmax = 1, kSlotHrows = 2, kSlotHgrow = 3, kSlotHfinal = 4, kSlotWoken = 5, kSlotStop = 6, kBoxSlots = 8 };
+
+// the parameters of particle i's box over the single-box ones
+__device__ __forceinline__ void resolve_box(const BoxTable &bt, int i, DevParams &P, GridView &g)
+{
+    if (!bt.b) return;
+    const BoxDev &B = bt.b[bt.BOX[i]];
+    P.box = B.box; P.half = B.half; P.min_kernel_radius = B.min_h; P.max_kernel_radius = B.max_h;
+    P.time = B.time; P.time_begin = B.time_begin;
+    g.level = B.level; g.shift = B.shift; g.cell = B.cell; g.box = B.box; g.half = B.half;
+    g.margin = B.margin; g.cell_off = B.cell_off;
+}
 
 struct CellRows {             // occupied cells of the active rows
     const int *cells;         // cell codes, ascending
@@ -63,6 +103,16 @@ struct TimeParams {
     double min_h, max_h, gamma, box;
 };
 
+// the timeline of particle i's box over the single-box one
+__device__ __forceinline__ void resolve_time(const BoxTable &bt, int i, TimeParams &T)
+{
+    if (!bt.b) return;
+    const BoxDev &B = bt.b[bt.BOX[i]];
+    T.tbi = B.tbi; T.ti_current = B.ti_current; T.active_mask = B.mask;
+    T.min_h = B.min_h; T.max_h = B.max_h; T.box = B.box;
+    T.courant_fac = B.courant_fac; T.err_tol = B.err_tol; T.max_dt = B.max_dt; T.min_dt = B.min_dt; T.min_egy = B.min_egy;
+}
+
 struct ParticleView {         // device order; vectors SoA with column stride n
     long n;
     double *X, *Y, *Z, *VX, *VY, *VZ, *VKX, *VKY, *VKZ, *M, *H, *U, *UK, *NN, *DIVV, *RHO, *PR, *ACC, *DTU, *MSV, *SOFT;
@@ -70,22 +120,35 @@ struct ParticleView {         // device order; vectors SoA with column stride n
     long long *TIB, *DTS;
 };
 
-__global__ void k_find_timesteps(TimeParams T, ParticleView S, int nrows, const int *act, int *err);
-__global__ void k_bin_hist(long n, const int *TB, unsigned long long *count);
-__global__ void k_wakeup_limits(TimeParams T, const unsigned long long *count, int *lim);
+__global__ void k_find_timesteps(TimeParams T, ParticleView S, int nrows, const int *act, int *err, BoxTable bt);
+__global__ void k_bin_hist(long n, const int *TB, unsigned long long *count, BoxTable bt);
+__global__ void k_wakeup_limits(TimeParams T, const unsigned long long *count, int *lim, BoxTable bt);
 __global__ void k_wakeups(TimeParams T, ParticleView S, const int *lim, int wakeup_bin_offset, unsigned long long *count,
-                          int *nwoken);
-__global__ void k_halfstep_kick(TimeParams T, ParticleView S, int nrows, const int *act, int second);
+                          int *nwoken, BoxTable bt);
+__global__ void k_halfstep_kick(TimeParams T, ParticleView S, int nrows, const int *act, int second, BoxTable bt);
 __global__ void k_drift(TimeParams T, ParticleView S, long long time0, const unsigned long long *tl, const double *X0,
-                        const double *Y0, const double *Z0, unsigned long long *disp, unsigned long long *hmax);
+                        const double *Y0, const double *Z0, unsigned long long *disp, unsigned long long *hmax, BoxTable bt);
 __global__ void k_step_prep(int *counters, unsigned long long *u64c);
-__global__ void k_next_sync(TimeParams T, const unsigned long long *count, unsigned long long *tl);
+__global__ void k_next_sync(TimeParams T, const unsigned long long *count, unsigned long long *tl, BoxTable bt);
 __global__ void k_active_flags(long n, const unsigned long long *tl, const int *TB, const double *M, const double *H, double hclamp,
-                               uint8_t *ACT, double *SOFT, int *flag, unsigned long long *hmax_act);
+                               uint8_t *ACT, double *SOFT, int *flag, unsigned long long *hmax_act, BoxTable bt);
 __global__ void k_compact_count(long n, const int *flag, const int *scan, int *act, int *count_out);
-__global__ void k_mark_records(int nrows, const int *act, const int *ell, long ld, const int *cnt, uint8_t *need);
+__global__ void k_advance_boxes(BoxTable bt);
 __global__ void k_active_from_bins(long n, unsigned long long mask, const int *TB, const double *M, const double *H,
                                    uint8_t *ACT, double *SOFT);
+__global__ void k_mark_records(int nrows, const int *act, const int *ell, long ld, const int *cnt, uint8_t *need);
+
+// batches of independent boxes
+__global__ void k_batch_keys(long n, const double *X, const double *Y, const double *Z, const int *BOX, const BoxDev *tab,
+                             const int *rank, const uint8_t *due, uint32_t *key, unsigned long long *key64, int *val);
+__global__ void k_batch_cell_bounds(long n, const uint32_t *key, const int *BOX, const BoxDev *tab, int *cstart, int *cend);
+__global__ void k_batch_flag_boxes(long n, const int *BOX, const uint8_t *sel, const double *M, int *flag, uint8_t *ACT);
+__global__ void k_batch_flag_active(long n, const int *BOX, const BoxDev *tab, const uint8_t *ACT, const double *M, int *flag);
+__global__ void k_batch_box_hmax(long n, const double *H, const uint8_t *sel, BoxTable bt);
+__global__ void k_batch_start(long n, const int *BOX, const uint8_t *sel, ParticleView S);
+__global__ void k_batch_soft(long n, const int *BOX, const uint8_t *sel, const double *H, double *SOFT);
+__global__ void k_batch_retire(long n, int *BOX, const uint8_t *sel, int retired, double *M, uint8_t *ACT);
+__global__ void k_fill_int(long n, int *a, int v);
 __global__ void k_gather_ll(long n, const int *perm, const long long *in, long long *out);
 __global__ void k_scatter_ll(long n, const int *perm, const long long *in, long long *out);
 
@@ -129,10 +192,10 @@ __global__ void k_density_cells(DevParams P, GridView g, CellRows cr, const int 
                                 const double *M, const double *hcur, double *accout);
 __global__ void k_density_init(int na, const int *act, double *H, double hclamp, double *hcur, double *left, double *right,
                                double *cn_cur, int *cn_init, int *iter, uint8_t *need, int *counters, unsigned long long *hmax,
-                               unsigned long long *hfinal);
+                               unsigned long long *hfinal, BoxTable bt);
 __global__ void k_density_after(DevParams P, int nrows, const int *act, uint8_t *need, const double *accin, const double *M,
                                 const double *U, double cell, DensityIter it, DensityOut out, int *nnext, int *rebuild, int *err,
-                                unsigned long long *hfinal);
+                                unsigned long long *hfinal, BoxTable bt);
 
 template <typename R>
 __global__ void k_pack_grad(long n, const uint8_t *need, const double *H, const double *M, const double *RHO, const double *PR,
@@ -149,17 +212,18 @@ __global__ void k_gradients(DevParams P, int nrows, const int *act, const double
 template <int KERN, typename R>
 __global__ void k_density_rows(DevParams P, GridView g, int nrows, const int *act, const uint8_t *need, const uint32_t *key,
                                const double *X, const double *Y, const double *Z, const double *VX, const double *VY,
-                               const double *VZ, const double *M, const double *hcur, double *accout);
+                               const double *VZ, const double *M, const double *hcur, double *accout, BoxTable bt);
 __global__ void k_build_sym_rows(GridView g, int nrows, const int *act, const uint32_t *key, const double *X, const double *Y,
-                                 const double *Z, const double *H, int *ell, long ld, int cap, int *cnt, int *overflow);
+                                 const double *Z, const double *H, int *ell, long ld, int cap, int *cnt, int *overflow, BoxTable bt);
 template <int KERN, typename R>
 __global__ void k_gradients_rows(DevParams P, int nrows, const int *act, const double *X, const double *Y, const double *Z,
                                  const GradRec<R> *rec, const double *T, const double *CN, long n, const int *ell, long ld,
-                                 const int *cnt, double *GR, double *GP, double *GV);
+                                 const int *cnt, double *GR, double *GP, double *GV, BoxTable bt);
 template <int KERN, typename R>
 __global__ void k_force_rows(DevParams P, Gam<R> G, int nrows, const int *act, ForceView F, const Cell<R> *rec, const int *ell,
                              long ld, const int *cnt, const uint8_t *active, const int *timebin, const double *msv_in, int *wakeup,
-                             double *ACC, double *DTU, double *MSV, int *err, int *need_wakeup, unsigned long long *pairs);
+                             double *ACC, double *DTU, double *MSV, int *err, int *need_wakeup, unsigned long long *pairs,
+                             BoxTable bt);
 
 template <int KERN, typename R>
 __global__ void k_force(DevParams P, Gam<R> G, int nrows, const int *act, ForceView F, const Cell<R> *rec, const int *ell,

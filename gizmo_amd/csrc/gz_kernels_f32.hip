@@ -27,14 +27,15 @@ namespace gz {
                                               double *, double *, int *, int *, unsigned long long *);                 \
     template __global__ void k_density_rows<KERN, R>(DevParams, GridView, int, const int *, const uint8_t *, const uint32_t *,  \
                                                      const double *, const double *, const double *, const double *,            \
-                                                     const double *, const double *, const double *, const double *, double *); \
+                                                     const double *, const double *, const double *, const double *, double *,  \
+                                                     BoxTable);                                                                  \
     template __global__ void k_gradients_rows<KERN, R>(DevParams, int, const int *, const double *, const double *,             \
                                                        const double *, const GradRec<R> *, const double *, const double *, long, \
-                                                       const int *, long, const int *, double *, double *, double *);           \
+                                                       const int *, long, const int *, double *, double *, double *, BoxTable); \
     template __global__ void k_force_rows<KERN, R>(DevParams, Gam<R>, int, const int *, ForceView, const Cell<R> *,            \
                                                    const int *, long, const int *, const uint8_t *, const int *,                \
                                                    const double *, int *, double *, double *, double *, int *, int *,           \
-                                                   unsigned long long *);
+                                                   unsigned long long *, BoxTable);
 GZ_INST(3, float)
 GZ_INST(6, float)
 template __global__ void k_pack_grad<float>(long, const uint8_t *, const double *, const double *, const double *, const double *, const double *,

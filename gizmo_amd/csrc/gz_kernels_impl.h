@@ -639,7 +639,8 @@ __global__ void __launch_bounds__(256) k_density_init(int na, const int *__restr
                                                       double *__restrict__ hcur, double *__restrict__ left, double *__restrict__ right,
                                                       double *__restrict__ cn_cur, int *__restrict__ cn_init, int *__restrict__ iter,
                                                       uint8_t *__restrict__ need, int *__restrict__ counters,
-                                                      unsigned long long *__restrict__ hmax, unsigned long long *__restrict__ hfinal)
+                                                      unsigned long long *__restrict__ hmax, unsigned long long *__restrict__ hfinal,
+                                                      BoxTable bt)
 {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r < 8) counters[r] = 0;
@@ -647,6 +648,7 @@ __global__ void __launch_bounds__(256) k_density_init(int na, const int *__restr
     double h = 0;
     if (r < na) {
         const int i = act[r];
+        if (bt.b) hclamp = 0.99 * bt.b[bt.BOX[i]].max_h;
         h = H[i];
         if ((h < 0) || !isfinite(h) || (h > hclamp)) { h = hclamp; H[i] = h; }
         hcur[r] = h;
@@ -666,11 +668,16 @@ __global__ void __launch_bounds__(256) k_density_after(DevParams P, int nrows, c
                                                        const double *__restrict__ accin, const double *__restrict__ M,
                                                        const double *__restrict__ U, double cell, DensityIter it, DensityOut out,
                                                        int *__restrict__ nnext, int *__restrict__ rebuild, int *__restrict__ err,
-                                                       unsigned long long *__restrict__ hfinal)
+                                                       unsigned long long *__restrict__ hfinal, BoxTable bt)
 {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= nrows || !need[r]) return;
     const int i = act[r];
+    if (bt.b) {
+        GridView g{};
+        resolve_box(bt, i, P, g);
+        cell = bt.b[bt.BOX[i]].rowlim;
+    }
     double acc[13];
     for (int q = 0; q < 13; q++) acc[q] = accin[(size_t)q * nrows + r];
     const double h = it.hcur[r];
@@ -685,12 +692,18 @@ __global__ void __launch_bounds__(256) k_density_after(DevParams P, int nrows, c
         else if (rc > 0) atomicMax(err, 1155);
         density_finalize(P, out, i, h, acc, M[i], U[i]);
         need[r] = 0;
-        if (h > 0) atomicMax(hfinal, (unsigned long long)__double_as_longlong(h));
+        if (h > 0) {
+            if (bt.b) atomicMax(&bt.slot[(size_t)bt.BOX[i] * kBoxSlots + kSlotHfinal], (unsigned long long)__double_as_longlong(h));
+            else atomicMax(hfinal, (unsigned long long)__double_as_longlong(h));
+        }
         return;
     }
     it.left[r] = sc.left; it.right[r] = sc.right; it.cn_cur[r] = sc.cn_cur; it.cn_init[r] = sc.cn_init;
     it.hcur[r] = nh;
-    if (nh * (1.0 + 1e-6) > cell) atomicOr(rebuild, 1);
+    if (nh * (1.0 + 1e-6) > cell) {
+        atomicOr(rebuild, 1);
+        if (bt.b) atomicMax(&bt.slot[(size_t)bt.BOX[i] * kBoxSlots + kSlotHgrow], (unsigned long long)__double_as_longlong(nh));
+    }
     atomicAdd(nnext, 1);
 }
 #endif

@@ -10,46 +10,74 @@
 
 namespace gz {
 
-// The candidates of a row: the non-empty cells of its home cell's 27-stencil, sorted by start,
-// laid end to end.  Virtual index v maps to the particle of range k with pref[k] <= v < pref[k+1].
+// The candidates of a row: the non-empty cells of its home cell's 27-stencil in ascending cell
+// code (cells are consecutive ranges of the Morton order, so this is ascending start: the
+// candidates come in ascending particle index whatever the grid level), laid end to end.
+// Virtual index v maps to the particle of range k with pref[k] <= v < pref[k+1].
 struct RowStencil {
     int start[27];
     int pref[28];
+    int code[27];
+    int len[27];
     int nst;
 };
 
-// the ranges of the stencil that can hold a partner: a cell is dropped when the row lies farther
-// from its box than the row's reach plus the distance any particle has moved from its cell
-// (reach = h_i, or max(h_i, the cell's largest radius) for the symmetric test)
+// The stencil of the row's home cell, built by 27 lanes (one per neighbour cell): periodic
+// neighbours, duplicates of small grids dropped, cells beyond reach dropped (the row lies
+// farther from the cell's box than its reach plus the distance any particle has moved from its
+// cell; reach = h_i, or max(h_i, the cell's largest radius) for the symmetric test), ranks by
+// code, then the prefix of the lengths.
 template <bool SYM>
 __device__ __forceinline__ void row_stencil(const GridView &g, uint32_t home, double xi, double yi, double zi, double hrow,
                                             RowStencil &rs)
 {
     const int lane = threadIdx.x;
-    const int nst = g.nsten[home];
-    if (lane == 0) {
-        const bool prune = (1 << g.level) >= 4;
-        const double ox = compact10(home >> 2) * g.cell, oy = compact10(home >> 1) * g.cell, oz = compact10(home) * g.cell;
-        const double rx = wrap(xi - ox, g.box, g.half), ry = wrap(yi - oy, g.box, g.half), rz = wrap(zi - oz, g.box, g.half);
-        int acc = 0, m = 0;
-        for (int q = 0; q < nst; q++) {
-            const int4 e = g.sten[(size_t)home * 27 + q];
-            if (prune) {
-                const int dx = e.z / 9 - 1, dy = (e.z / 3) % 3 - 1, dz = e.z % 3 - 1;
-                // gap from the row to the neighbour cell's box [d*cell, (d+1)*cell) along each axis
-                const double gx = fmax(fmax(dx * g.cell - rx, rx - (dx + 1) * g.cell), 0.0);
-                const double gy = fmax(fmax(dy * g.cell - ry, ry - (dy + 1) * g.cell), 0.0);
-                const double gz = fmax(fmax(dz * g.cell - rz, rz - (dz + 1) * g.cell), 0.0);
-                double reach = hrow;
-                if (SYM) reach = fmax(reach, g.chmax[e.w]);
-                const double rr = (reach + g.margin) * (1.0 + 1e-9) + 1e-12 * g.box;
-                if (gx * gx + gy * gy + gz * gz > rr * rr) continue;
-            }
-            rs.start[m] = e.x;
-            rs.pref[m] = acc;
-            acc += e.y - e.x;
-            m++;
+    const int nc = 1 << g.level;
+    const int cx = compact10(home >> 2), cy = compact10(home >> 1), cz = compact10(home);
+    int code = 0x7fffffff, st = 0, en = 0;
+    if (lane < 27) {
+        const int dx = lane / 9 - 1, dy = (lane / 3) % 3 - 1, dz = lane % 3 - 1;
+        const int a = (cx + dx + nc) % nc, b = (cy + dy + nc) % nc, d = (cz + dz + nc) % nc;
+        const int c = (int)((spread10(a) << 2) | (spread10(b) << 1) | spread10(d));
+        st = g.cstart[g.cell_off + c];
+        en = g.cend[g.cell_off + c];
+        bool keep = en > st;
+        if (keep && nc >= 4) {
+            const double ox = cx * g.cell, oy = cy * g.cell, oz = cz * g.cell;
+            const double rx = wrap(xi - ox, g.box, g.half), ry = wrap(yi - oy, g.box, g.half), rz = wrap(zi - oz, g.box, g.half);
+            // gap from the row to the neighbour cell's box [d*cell, (d+1)*cell) along each axis
+            const double gx = fmax(fmax(dx * g.cell - rx, rx - (dx + 1) * g.cell), 0.0);
+            const double gy = fmax(fmax(dy * g.cell - ry, ry - (dy + 1) * g.cell), 0.0);
+            const double gz = fmax(fmax(dz * g.cell - rz, rz - (dz + 1) * g.cell), 0.0);
+            double reach = hrow;
+            if (SYM) reach = fmax(reach, g.chmax[g.cell_off + c]);
+            const double rr = (reach + g.margin) * (1.0 + 1e-9) + 1e-12 * g.box;
+            if (gx * gx + gy * gy + gz * gz > rr * rr) keep = false;
         }
+        if (keep) code = c;
+        rs.code[lane] = code;
+    }
+    __syncthreads();
+    // of equal codes (grids under 3 cells a side) the first lane stays
+    bool first = false;
+    if (lane < 27 && code != 0x7fffffff) {
+        first = true;
+        for (int q = 0; q < lane; q++)
+            if (rs.code[q] == code) { first = false; break; }
+    }
+    const unsigned long long firsts = __ballot(first);
+    if (first) {
+        int rank = 0;  // distinct kept codes below this one
+        for (int q = 0; q < 27; q++)
+            if (((firsts >> q) & 1ull) && rs.code[q] < code) rank++;
+        rs.start[rank] = st;
+        rs.len[rank] = en - st;
+    }
+    __syncthreads();
+    if (lane == 0) {
+        const int m = __popcll(firsts);
+        int acc = 0;
+        for (int k = 0; k < m; k++) { rs.pref[k] = acc; acc += rs.len[k]; }
         rs.pref[m] = acc;
         rs.nst = m;
     }
@@ -72,7 +100,7 @@ __global__ void __launch_bounds__(64) k_density_rows(DevParams P, GridView g, in
                                                      const double *__restrict__ Z, const double *__restrict__ VX,
                                                      const double *__restrict__ VY, const double *__restrict__ VZ,
                                                      const double *__restrict__ M, const double *__restrict__ hcur,
-                                                     double *__restrict__ accout)
+                                                     double *__restrict__ accout, BoxTable bt)
 {
     __shared__ RowStencil rs;
     __shared__ R sc[13][64];
@@ -81,6 +109,7 @@ __global__ void __launch_bounds__(64) k_density_rows(DevParams P, GridView g, in
     if (r >= nrows || !need[r]) return;
     const int lane = threadIdx.x;
     const int i = act[r];
+    resolve_box(bt, i, P, g);
     const double xi = X[i], yi = Y[i], zi = Z[i];
     const R vxi = (R)VX[i], vyi = (R)VY[i], vzi = (R)VZ[i];
     const double h = hcur[r];
@@ -157,13 +186,18 @@ __global__ void __launch_bounds__(64) k_density_rows(DevParams P, GridView g, in
 __global__ void __launch_bounds__(64) k_build_sym_rows(GridView g, int nrows, const int *__restrict__ act, const uint32_t *__restrict__ key,
                                                        const double *__restrict__ X, const double *__restrict__ Y,
                                                        const double *__restrict__ Z, const double *__restrict__ H, int *__restrict__ ell,
-                                                       long ld, int cap, int *__restrict__ cnt, int *__restrict__ overflow)
+                                                       long ld, int cap, int *__restrict__ cnt, int *__restrict__ overflow,
+                                                       BoxTable bt)
 {
     __shared__ RowStencil rs;
     const long r = xcd_block();
     if (r >= nrows) return;
     const int lane = threadIdx.x;
     const int i = act[r];
+    {
+        DevParams P{};
+        resolve_box(bt, i, P, g);
+    }
     const double xi = X[i], yi = Y[i], zi = Z[i], hi = H[i];
     row_stencil<true>(g, key[i] >> g.shift, xi, yi, zi, hi, rs);
     const int total = rs.pref[rs.nst];
@@ -195,7 +229,7 @@ __global__ void __launch_bounds__(64) k_gradients_rows(DevParams P, int nrows, c
                                                        const GradRec<R> *__restrict__ rec, const double *__restrict__ T,
                                                        const double *__restrict__ CN, long n, const int *__restrict__ ell, long ld,
                                                        const int *__restrict__ cnt, double *__restrict__ GR, double *__restrict__ GP,
-                                                       double *__restrict__ GV)
+                                                       double *__restrict__ GV, BoxTable bt)
 {
     __shared__ R sd[5][64], sw[3][64], srr[64];
     __shared__ int sf[64];
@@ -203,6 +237,10 @@ __global__ void __launch_bounds__(64) k_gradients_rows(DevParams P, int nrows, c
     if (r >= nrows) return;
     const int lane = threadIdx.x;
     const int i = act[r];
+    {
+        GridView g{};
+        resolve_box(bt, i, P, g);
+    }
     R G[3][5];
     R mx[5], mn[5];
     R maxd = 0;
@@ -296,13 +334,17 @@ __global__ void __launch_bounds__(64) k_force_rows(DevParams P, Gam<R> G, int nr
                                                    const int *__restrict__ timebin, const double *__restrict__ msv_in,
                                                    int *__restrict__ wakeup, double *__restrict__ ACC, double *__restrict__ DTU,
                                                    double *__restrict__ MSV, int *__restrict__ err, int *__restrict__ need_wakeup,
-                                                   unsigned long long *__restrict__ pairs)
+                                                   unsigned long long *__restrict__ pairs, BoxTable bt)
 {
     __shared__ R sv[5][64];
     const long r = xcd_block();
     if (r >= nrows) return;
     const int lane = threadIdx.x;
     const int i = act[r];
+    {
+        GridView g{};
+        resolve_box(bt, i, P, g);
+    }
     const double mi = F.M[i], rhoi = F.RHO[i];
     double A0 = 0, A1 = 0, A2 = 0, dE = 0, msv = 0;
     const double sound_i = (rhoi > 0) ? sqrt(P.gamma * F.PR[i] / rhoi) : sqrt(P.gamma * (P.gamma - 1.) * F.U[i]);
@@ -338,7 +380,11 @@ __global__ void __launch_bounds__(64) k_force_rows(DevParams P, Gam<R> G, int nr
                 if (rc > 0) {
                     sv[0][lane] = f0; sv[1][lane] = f1; sv[2][lane] = f2; sv[3][lane] = fp; sv[4][lane] = vts;
                     if (active && !active[j] && wakeup) {
-                        if ((double)vts > kWakeup * msv_in[j]) { atomicMax(&wakeup[j], tbi + 1); *need_wakeup = 1; }
+                        if ((double)vts > kWakeup * msv_in[j]) {
+                            atomicMax(&wakeup[j], tbi + 1);
+                            if (bt.b) bt.b[bt.BOX[i]].need_wakeup = 1;
+                            else *need_wakeup = 1;
+                        }
                     }
                 }
             }

@@ -108,14 +108,37 @@ __device__ void kick(const TimeParams &T, const ParticleView &S, long i, long lo
     }
 }
 
+// the maximum of v over the wave into slot (one atomic when the wave holds one box; the bits of
+// a positive double order like the double)
+__device__ __forceinline__ void wave_max_to(double v, bool valid, int box, unsigned long long *single, const BoxTable &bt, int slot)
+{
+    if (!bt.b) {
+        for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+        if ((threadIdx.x & 63) == 0 && v > 0) atomicMax(single, (unsigned long long)__double_as_longlong(v));
+        return;
+    }
+    const unsigned long long vm = __ballot(valid);
+    if (!vm) return;
+    const int b0 = __shfl(box, __ffsll((long long)vm) - 1, 64);
+    if (__all(!valid || box == b0)) {
+        double w = valid ? v : 0.0;
+        for (int o = 32; o > 0; o >>= 1) w = fmax(w, __shfl_xor(w, o, 64));
+        if ((threadIdx.x & 63) == 0 && w > 0)
+            atomicMax(&bt.slot[(size_t)b0 * kBoxSlots + slot], (unsigned long long)__double_as_longlong(w));
+    } else if (valid && v > 0) {
+        atomicMax(&bt.slot[(size_t)box * kBoxSlots + slot], (unsigned long long)__double_as_longlong(v));
+    }
+}
+
 // find_timesteps, src/core/timestep.cc:39-355: the new power-of-two step of each particle of
 // the active list; a larger bin only where it is synchronised now
 __global__ void __launch_bounds__(256) k_find_timesteps(TimeParams T, ParticleView S, int nrows, const int *__restrict__ act,
-                                                         int *__restrict__ err)
+                                                         int *__restrict__ err, BoxTable bt)
 {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= nrows) return;
     const int i = act[r];
+    resolve_time(bt, i, T);
     long long ti_step = get_timestep(T, S, i);
     ti_step = (long long)(((double)ti_step) / 1.0);
     long long ti_min = kTimebase;
@@ -129,7 +152,8 @@ __global__ void __launch_bounds__(256) k_find_timesteps(TimeParams T, ParticleVi
     }
     if (T.ti_current >= kTimebase) { ti_step = 0; bin = 0; }
     if ((kTimebase - T.ti_current) < ti_step) {
-        atomicMax(err, 90001005);
+        if (bt.b) atomicMax(&bt.slot[(size_t)bt.BOX[i] * kBoxSlots + kSlotStop], 90001005ull);
+        else atomicMax(err, 90001005);
         ti_step = kTimebase - T.ti_current;
         ti_min = kTimebase;
         while (ti_min > ti_step) ti_min >>= 1;
@@ -140,73 +164,76 @@ __global__ void __launch_bounds__(256) k_find_timesteps(TimeParams T, ParticleVi
     S.DTS[i] = ti_step;
 }
 
-// TimeBinCount: occupation of the 60 bins (block histogram in LDS, one global add per bin)
-__global__ void __launch_bounds__(256) k_bin_hist(long n, const int *__restrict__ TB, unsigned long long *__restrict__ count)
+// TimeBinCount of every box (count[64*box + bin]): lanes of the same box and bin are counted
+// together (most particles of a wave share both), one atomic per group
+__global__ void __launch_bounds__(256) k_bin_hist(long n, const int *__restrict__ TB, unsigned long long *__restrict__ count, BoxTable bt)
 {
-    __shared__ unsigned int h[64];
-    if (threadIdx.x < 64) h[threadIdx.x] = 0;
-    __syncthreads();
-    // lanes with the same bin are counted together (most particles of a wave share one bin)
     const int lane = threadIdx.x & 63;
     const long stride = (long)gridDim.x * blockDim.x;
     const long base0 = blockIdx.x * (long)blockDim.x + (threadIdx.x & ~63);
     for (long base = base0; base < n; base += stride) {
         const long i = base + lane;
-        int b = (i < n) ? (TB[i] & 63) : -1;
+        int b = (i < n) ? ((bt.b ? bt.BOX[i] * 64 : 0) + (TB[i] & 63)) : -1;
         while (true) {
             const unsigned long long live = __ballot(b >= 0);
             if (!live) break;
             const int leader = __ffsll((long long)live) - 1;
             const int lb = __shfl(b, leader, 64);
             const unsigned long long same = __ballot(b == lb);
-            if (lane == leader) atomicAdd(&h[lb], (unsigned)__popcll(same));
+            if (lane == leader) atomicAdd(&count[lb], (unsigned long long)__popcll(same));
             if (b == lb) b = -1;
         }
     }
-    __syncthreads();
-    if (threadIdx.x < 64 && h[threadIdx.x]) atomicAdd(&count[threadIdx.x], (unsigned long long)h[threadIdx.x]);
 }
 
 // the bin limits of process_wake_ups (src/core/timestep.cc:1577-1625) from the occupation
-// after find_timesteps: lim[0] the lowest occupied active bin, lim[1] the highest bin active at
-// the next kick (one lane)
-__global__ void k_wakeup_limits(TimeParams T, const unsigned long long *__restrict__ count, int *__restrict__ lim)
+// after find_timesteps, one lane per box: lim[2b] the lowest occupied active bin, lim[2b+1] the
+// highest bin active at the next kick
+__global__ void k_wakeup_limits(TimeParams T, const unsigned long long *__restrict__ count, int *__restrict__ lim, BoxTable bt)
 {
-    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= (bt.b ? bt.nbox : 1)) return;
+    long long tic = T.ti_current;
+    unsigned long long mask = T.active_mask;
+    if (bt.b) { tic = bt.b[b].ti_current; mask = bt.b[b].mask; }
+    const unsigned long long *c = count + (size_t)64 * b;
     long long ti_next_kick = kTimebase;
-    for (int b = 0; b < kTimebins; b++) {
-        if (!count[b]) continue;
+    for (int k = 0; k < kTimebins; k++) {
+        if (!c[k]) continue;
         long long t;
-        if (b > 0) {
-            const long long dt_bin = bin_len(b);
-            t = (T.ti_current / dt_bin) * dt_bin + dt_bin;
+        if (k > 0) {
+            const long long dt_bin = bin_len(k);
+            t = (tic / dt_bin) * dt_bin + dt_bin;
         } else {
-            t = T.ti_current;
+            t = tic;
         }
         if (t < ti_next_kick) ti_next_kick = t;
     }
     int max_time_bin_active = 0;
-    for (int b = 0; b < kTimebins; b++)
-        if ((ti_next_kick % (1LL << b)) == 0) max_time_bin_active = b;
+    for (int k = 0; k < kTimebins; k++)
+        if ((ti_next_kick % (1LL << k)) == 0) max_time_bin_active = k;
     int lowest = kTimebins;
-    for (int b = 0; b < kTimebins; b++)
-        if (bin_on(T.active_mask, b) && count[b] > 0) { lowest = b; break; }
-    lim[0] = lowest;
-    lim[1] = max_time_bin_active;
+    for (int k = 0; k < kTimebins; k++)
+        if (bin_on(mask, k) && c[k] > 0) { lowest = k; break; }
+    lim[2 * b] = lowest;
+    lim[2 * b + 1] = max_time_bin_active;
 }
 
 // process_wake_ups, src/core/timestep.cc:1565-1774 (every particle stands at the sync point:
 // its Ti_current is All.Ti_Current, so a wake-up never truncates a step); the occupation
 // count follows each move
 __global__ void __launch_bounds__(256) k_wakeups(TimeParams T, ParticleView S, const int *__restrict__ lim, int wakeup_bin_offset,
-                                                  unsigned long long *__restrict__ count, int *__restrict__ nwoken)
+                                                  unsigned long long *__restrict__ count, int *__restrict__ nwoken, BoxTable bt)
 {
     const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
     if (i >= S.n) return;
-    const int lowest_occupied_active_bin = lim[0], max_time_bin_active = lim[1];
     const int w = S.WAKE[i];
     if (!w) return;
     if (S.M[i] <= 0) return;
+    const int box = bt.b ? bt.BOX[i] : 0;
+    if (bt.b && !(bt.b[box].stepping && bt.b[box].need_wakeup)) return;
+    resolve_time(bt, (int)i, T);
+    const int lowest_occupied_active_bin = lim[2 * box], max_time_bin_active = lim[2 * box + 1];
     const int binold = S.TB[i];
     if (bin_on(T.active_mask, binold)) return;
     int bin;
@@ -224,8 +251,9 @@ __global__ void __launch_bounds__(256) k_wakeups(TimeParams T, ParticleView S, c
     const bool starts_now = bin_on(T.active_mask, bin) && (S.DTS[i] == bin_len(binold));
     S.TB[i] = bin;
     atomicAdd(nwoken, 1);
-    atomicAdd(&count[binold], ~0ull);  // -1
-    atomicAdd(&count[bin], 1ull);
+    if (bt.b) atomicAdd(&bt.slot[(size_t)box * kBoxSlots + kSlotWoken], 1ull);
+    atomicAdd(&count[(size_t)64 * box + binold], ~0ull);  // -1
+    atomicAdd(&count[(size_t)64 * box + bin], 1ull);
     if (starts_now) kick(T, S, i, old_midpoint, T.ti_current, 1);
     S.TIB[i] = T.ti_current;
     S.DTS[i] = bin_len(bin);
@@ -233,12 +261,14 @@ __global__ void __launch_bounds__(256) k_wakeups(TimeParams T, ParticleView S, c
 }
 
 // do_first_halfstep_kick / do_second_halfstep_kick, src/core/kicks.cc:34-97,99-185
-__global__ void __launch_bounds__(256) k_halfstep_kick(TimeParams T, ParticleView S, int nrows, const int *__restrict__ act, int second)
+__global__ void __launch_bounds__(256) k_halfstep_kick(TimeParams T, ParticleView S, int nrows, const int *__restrict__ act, int second,
+                                                        BoxTable bt)
 {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= nrows) return;
     const int i = act[r];
     if (!(S.M[i] > 0)) return;
+    resolve_time(bt, i, T);
     const long long half = S.DTS[i] / 2;
     if (second) kick(T, S, i, S.TIB[i] + half, S.TIB[i] + S.DTS[i], 1);
     else kick(T, S, i, S.TIB[i], S.TIB[i] + half, 0);
@@ -286,32 +316,58 @@ __device__ __forceinline__ double drift_one(const TimeParams &T, const ParticleV
     return fmax(ex, fmax(ey, ez));
 }
 
-// every particle drifted (grid-stride), and the largest displacement since the last ordering
-// and the largest kernel radius after the drift: wave maxima, then one atomic per workgroup
-// (a positive double's bits order like the double)
+// every particle drifted to its box's next sync point (grid-stride), and the per-box largest
+// displacement since the last ordering and largest kernel radius after the drift
 __global__ void __launch_bounds__(256) k_drift(TimeParams T, ParticleView S, long long time0, const unsigned long long *__restrict__ tl,
                                                 const double *__restrict__ X0, const double *__restrict__ Y0,
                                                 const double *__restrict__ Z0, unsigned long long *__restrict__ disp,
-                                                unsigned long long *__restrict__ hmax)
+                                                unsigned long long *__restrict__ hmax, BoxTable bt)
 {
-    const long long time1 = (long long)tl[0];  // the next sync point, found on the device
-    double d = 0, hh = 0;
-    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < S.n; i += (long)gridDim.x * blockDim.x) {
-        if (time1 != time0) d = fmax(d, drift_one(T, S, i, time0, time1, X0, Y0, Z0));
-        hh = fmax(hh, S.H[i]);
+    const long long time1_single = bt.b ? 0 : (long long)tl[0];  // the next sync point, found on the device
+    const long stride = (long)gridDim.x * blockDim.x;
+    const long base0 = blockIdx.x * (long)blockDim.x + (threadIdx.x & ~63);
+    const long nround = (S.n + stride - 1) / stride * stride;
+    double d1 = 0, h1 = 0;
+    for (long base = base0; base < nround; base += stride) {
+        const long i = base + (threadIdx.x & 63);
+        const bool valid = i < S.n;
+        double d = 0, hh = 0;
+        int box = 0;
+        if (valid) {
+            TimeParams Ti = T;
+            long long t0 = time0, t1 = time1_single;
+            if (bt.b) {
+                box = bt.BOX[i];
+                const BoxDev &B = bt.b[box];
+                resolve_time(bt, (int)i, Ti);
+                t0 = B.ti_current;
+                t1 = B.ti_next;
+            }
+            if (t1 != t0) d = drift_one(Ti, S, i, t0, t1, X0, Y0, Z0);
+            hh = S.H[i];
+        }
+        if (bt.b) {
+            wave_max_to(d, valid, box, disp, bt, kSlotDisp);
+            wave_max_to(hh, valid, box, hmax, bt, kSlotHmax);
+        } else {
+            d1 = fmax(d1, d);
+            h1 = fmax(h1, hh);
+        }
     }
-    for (int o = 32; o > 0; o >>= 1) {
-        d = fmax(d, __shfl_xor(d, o, 64));
-        hh = fmax(hh, __shfl_xor(hh, o, 64));
-    }
-    __shared__ double sm[4], sh[4];
-    const int w = threadIdx.x >> 6;
-    if ((threadIdx.x & 63) == 0) { sm[w] = d; sh[w] = hh; }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        for (int k = 1; k < (int)(blockDim.x >> 6); k++) { d = fmax(d, sm[k]); hh = fmax(hh, sh[k]); }
-        if (d > 0) atomicMax(disp, (unsigned long long)__double_as_longlong(d));
-        if (hh > 0) atomicMax(hmax, (unsigned long long)__double_as_longlong(hh));
+    if (!bt.b) {
+        for (int o = 32; o > 0; o >>= 1) {
+            d1 = fmax(d1, __shfl_xor(d1, o, 64));
+            h1 = fmax(h1, __shfl_xor(h1, o, 64));
+        }
+        __shared__ double sm[4], sh[4];
+        const int w = threadIdx.x >> 6;
+        if ((threadIdx.x & 63) == 0) { sm[w] = d1; sh[w] = h1; }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            for (int k = 1; k < (int)(blockDim.x >> 6); k++) { d1 = fmax(d1, sm[k]); h1 = fmax(h1, sh[k]); }
+            if (d1 > 0) atomicMax(disp, (unsigned long long)__double_as_longlong(d1));
+            if (h1 > 0) atomicMax(hmax, (unsigned long long)__double_as_longlong(h1));
+        }
     }
 }
 
@@ -324,28 +380,42 @@ __global__ void k_step_prep(int *__restrict__ counters, unsigned long long *__re
     if (t == 0) { u64c[64] = 0; u64c[66] = 0; u64c[67] = 0; u64c[68] = 0; }
 }
 
-// find_next_sync_point_and_drift's timeline part (src/core/run.cc:857-930) from the occupation:
-// tl[0] the next kick (All.Ti_Current after the step), tl[1] the bins active there (one lane)
-__global__ void k_next_sync(TimeParams T, const unsigned long long *__restrict__ count, unsigned long long *__restrict__ tl)
+// find_next_sync_point_and_drift's timeline part (src/core/run.cc:857-930) from the occupation,
+// one lane per box: the next kick (All.Ti_Current after the step) and the bins active there
+// (single box: tl[0], tl[1]; a box not stepping keeps its time and has no active bin)
+__global__ void k_next_sync(TimeParams T, const unsigned long long *__restrict__ count, unsigned long long *__restrict__ tl, BoxTable bt)
 {
-    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= (bt.b ? bt.nbox : 1)) return;
+    long long tic = T.ti_current;
+    if (bt.b) {
+        tic = bt.b[b].ti_current;
+        bt.b[b].need_wakeup = 0;  // consumed by the wake-up pass of this step
+        if (!bt.b[b].stepping) { bt.b[b].ti_next = tic; bt.b[b].mask_next = 0; return; }
+    }
+    const unsigned long long *c = count + (size_t)64 * b;
     long long ti_next_kick = kTimebase;
-    for (int b = 0; b < kTimebins; b++) {
-        if (!count[b]) continue;
+    for (int k = 0; k < kTimebins; k++) {
+        if (!c[k]) continue;
         long long t;
-        if (b > 0) {
-            const long long dt_bin = bin_len(b);
-            t = (T.ti_current / dt_bin) * dt_bin + dt_bin;
+        if (k > 0) {
+            const long long dt_bin = bin_len(k);
+            t = (tic / dt_bin) * dt_bin + dt_bin;
         } else {
-            t = T.ti_current;
+            t = tic;
         }
         if (t < ti_next_kick) ti_next_kick = t;
     }
     unsigned long long mask = 1ull;
-    for (int b = 1; b < kTimebins; b++)
-        if ((ti_next_kick % bin_len(b)) == 0) mask |= 1ull << b;
-    tl[0] = (unsigned long long)ti_next_kick;
-    tl[1] = mask;
+    for (int k = 1; k < kTimebins; k++)
+        if ((ti_next_kick % bin_len(k)) == 0) mask |= 1ull << k;
+    if (bt.b) {
+        bt.b[b].ti_next = ti_next_kick;
+        bt.b[b].mask_next = mask;
+    } else {
+        tl[0] = (unsigned long long)ti_next_kick;
+        tl[1] = mask;
+    }
 }
 
 // make_list_of_active_particles (src/core/run.cc:997-1009) as flags for the scan: a particle of
@@ -355,12 +425,21 @@ __global__ void k_next_sync(TimeParams T, const unsigned long long *__restrict__
 __global__ void __launch_bounds__(256) k_active_flags(long n, const unsigned long long *__restrict__ tl, const int *__restrict__ TB,
                                                        const double *__restrict__ M, const double *__restrict__ H, double hclamp,
                                                        uint8_t *__restrict__ ACT, double *__restrict__ SOFT, int *__restrict__ flag,
-                                                       unsigned long long *__restrict__ hmax_act)
+                                                       unsigned long long *__restrict__ hmax_act, BoxTable bt)
 {
     const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
-    const unsigned long long mask = tl[1];
+    const bool valid = i < n;
     double h = 0;
-    if (i < n) {
+    int box = 0;
+    if (valid) {
+        unsigned long long mask;
+        if (bt.b) {
+            box = bt.BOX[i];
+            mask = bt.b[box].mask_next;
+            hclamp = 0.99 * bt.b[box].max_h;
+        } else {
+            mask = tl[1];
+        }
         const bool on = bin_on(mask, TB[i]);
         ACT[i] = on ? 1 : 0;
         const bool row = on && (M[i] > 0);
@@ -372,8 +451,7 @@ __global__ void __launch_bounds__(256) k_active_flags(long n, const unsigned lon
         }
     }
     if (i == n) flag[n] = 0;
-    for (int o = 32; o > 0; o >>= 1) h = fmax(h, __shfl_xor(h, o, 64));
-    if ((threadIdx.x & 63) == 0 && h > 0) atomicMax(hmax_act, (unsigned long long)__double_as_longlong(h));
+    wave_max_to(h, valid, box, hmax_act, bt, kSlotHrows);
 }
 
 // the scanned flags to the active list; the count to count_out
@@ -383,6 +461,18 @@ __global__ void __launch_bounds__(256) k_compact_count(long n, const int *__rest
     const long s = blockIdx.x * (long)blockDim.x + threadIdx.x;
     if (s < n && flag[s]) act[scan[s]] = (int)s;
     if (s == n) *count_out = scan[n];
+}
+
+// the boxes that stepped move to their next sync point (All.Ti_Current, All.Time, TimeBinActive)
+__global__ void k_advance_boxes(BoxTable bt)
+{
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (!bt.b || b >= bt.nbox) return;
+    BoxDev &B = bt.b[b];
+    if (!B.stepping) return;
+    B.ti_current = B.ti_next;
+    B.mask = B.mask_next;
+    B.time = B.time_begin + B.ti_current * B.tbi;
 }
 
 // TimeBinActive of each particle (the byte the force pass tests a neighbour with), and the

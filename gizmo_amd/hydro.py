@@ -98,6 +98,60 @@ def lib():
     return _lib
 
 
+def fields_struct(fields, n):
+    """A gz_fields of the given host arrays (dict name -> array, n particles in caller order);
+    returns it with the contiguous copies it points into (keep them alive for the call)."""
+    f = Fields()
+    keep = []
+    for k, w in list(FIELD_WIDTH.items()) + list(LOOP_FIELD_WIDTH.items()):
+        if k in fields and fields[k] is not None:
+            a = np.ascontiguousarray(fields[k], dtype=np.float64).reshape(n * w)
+            keep.append(a)
+            setattr(f, k, a.ctypes.data_as(_dp))
+    for k in INT_FIELDS:
+        if k in fields and fields[k] is not None:
+            a = np.ascontiguousarray(fields[k], dtype=np.int32).reshape(n)
+            keep.append(a)
+            setattr(f, k, a.ctypes.data_as(_ip))
+    for k in I64_FIELDS:
+        if k in fields and fields[k] is not None:
+            a = np.ascontiguousarray(fields[k], dtype=np.int64).reshape(n)
+            keep.append(a)
+            setattr(f, k, a.ctypes.data_as(_llp))
+    return f, keep
+
+
+def output_struct(names, n):
+    """A gz_fields pointing at fresh host arrays for the named fields (all P[]/CellP[] fields
+    and the two int fields when names is None); returns it and the dict of arrays."""
+    names = list(FIELD_WIDTH) + list(INT_FIELDS) if names is None else list(names)
+    f = Fields()
+    out = {}
+    widths = dict(FIELD_WIDTH, **LOOP_FIELD_WIDTH)
+    for k in names:
+        if k in widths:
+            w = widths[k]
+            a = np.zeros((n, w) if w > 1 else n)
+            setattr(f, k, a.ctypes.data_as(_dp))
+        elif k in I64_FIELDS:
+            a = np.zeros(n, dtype=np.int64)
+            setattr(f, k, a.ctypes.data_as(_llp))
+        elif k in INT_FIELDS:
+            a = np.zeros(n, dtype=np.int32)
+            setattr(f, k, a.ctypes.data_as(_ip))
+        else:
+            raise KeyError(k)
+        out[k] = a
+    return f, out
+
+
+def timeline_dict(t):
+    return dict(ti_current=t.ti_current, time=t.time, timebase_interval=t.timebase_interval,
+                num_steps=t.num_steps, n_active=t.n_active, n_woken=t.n_woken,
+                bin_count=np.array(t.bin_count[:], dtype=np.int64), bin_active=t.bin_active,
+                stop_code=t.stop_code)
+
+
 def make_params(p, precision=GZ_FP64):
     return Params(box_size=p["box"], kernel=p["kernel"], gamma=p["gamma"], des_num_ngb=p["des_num_ngb"],
                   max_num_ngb_dev=p["max_num_ngb_dev"], min_kernel_radius=p["min_kernel_radius"],
@@ -137,49 +191,13 @@ class Hydro:
 
     def load(self, fields):
         """fields: dict name -> numpy array (caller order). pos, vel, mass, u, h required."""
-        f = Fields()
-        keep = []
         n = len(fields["mass"])
-        for k, w in FIELD_WIDTH.items():
-            if k in fields and fields[k] is not None:
-                a = np.ascontiguousarray(fields[k], dtype=np.float64).reshape(n * w)
-                keep.append(a)
-                setattr(f, k, a.ctypes.data_as(_dp))
-        for k, w in LOOP_FIELD_WIDTH.items():
-            if k in fields and fields[k] is not None:
-                a = np.ascontiguousarray(fields[k], dtype=np.float64).reshape(n * w)
-                keep.append(a)
-                setattr(f, k, a.ctypes.data_as(_dp))
-        for k in INT_FIELDS:
-            if k in fields and fields[k] is not None:
-                a = np.ascontiguousarray(fields[k], dtype=np.int32).reshape(n)
-                keep.append(a)
-                setattr(f, k, a.ctypes.data_as(_ip))
-        for k in I64_FIELDS:
-            if k in fields and fields[k] is not None:
-                a = np.ascontiguousarray(fields[k], dtype=np.int64).reshape(n)
-                keep.append(a)
-                setattr(f, k, a.ctypes.data_as(_llp))
+        f, keep = fields_struct(fields, n)
         self._check(lib().gz_load(self.ctx, n, C.byref(f)))
         self.n = n
 
     def store(self, names=None):
-        names = list(FIELD_WIDTH) + list(INT_FIELDS) if names is None else list(names)
-        f = Fields()
-        out = {}
-        widths = dict(FIELD_WIDTH, **LOOP_FIELD_WIDTH)
-        for k in names:
-            if k in widths:
-                w = widths[k]
-                a = np.zeros((self.n, w) if w > 1 else self.n)
-                setattr(f, k, a.ctypes.data_as(_dp))
-            elif k in I64_FIELDS:
-                a = np.zeros(self.n, dtype=np.int64)
-                setattr(f, k, a.ctypes.data_as(_llp))
-            else:
-                a = np.zeros(self.n, dtype=np.int32)
-                setattr(f, k, a.ctypes.data_as(_ip))
-            out[k] = a
+        f, out = output_struct(names, self.n)
         self._check(lib().gz_store(self.ctx, C.byref(f)))
         return out
 
@@ -245,10 +263,7 @@ class Hydro:
     def timeline(self):
         t = Timeline()
         self._check(lib().gz_timeline_get(self.ctx, C.byref(t)))
-        return dict(ti_current=t.ti_current, time=t.time, timebase_interval=t.timebase_interval,
-                    num_steps=t.num_steps, n_active=t.n_active, n_woken=t.n_woken,
-                    bin_count=np.array(t.bin_count[:], dtype=np.int64), bin_active=t.bin_active,
-                    stop_code=t.stop_code)
+        return timeline_dict(t)
 
     def neighbors(self, symmetric):
         off = np.zeros(self.n + 1, dtype=np.int64)
