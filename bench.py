@@ -140,6 +140,100 @@ def roofline(res, form):
             "avg_launch_ms": round(total_ms / max(launches, 1), 4)}
 
 
+def serve_bench(precision, n_jobs=48, seed=1005, capacity=4 * 1024 * 1024, max_boxes=64, load=0.8, calib_jobs=8):
+    """The serving loop: a Poisson stream of n_jobs jobs (N log-uniform in [8192, 1048576], rms
+    Mach uniform in [0.1, 5], steps uniform in [10, 200]) into a batch with a budget of
+    `capacity` resident particles.  The arrival rate is set to `load` times the device's
+    capacity for this job mix, measured first on calib_jobs other jobs of the same mix run
+    together (particle-steps per second).  Every request's particles are made before the clock
+    starts (the request carries them); admission, upload, start, steps, read-back of the
+    result and removal are inside the latencies."""
+    from gizmo_amd.serve import Server, draw_jobs, latency_summary, poisson_arrivals
+    jobs = draw_jobs(n_jobs, seed)
+    cal = draw_jobs(calib_jobs, seed + 1)
+    for j in jobs + cal:
+        j.fields = j.ic()
+    for j in cal:
+        j.arrival = 0.0
+    s = Server(capacity, max_boxes, precision, keep=["u"])
+    wall_cal = s.run(cal)
+    ps_rate = sum(j.n * j.steps for j in cal) / wall_cal
+    mean_work = float(np.mean([j.n * j.steps for j in jobs]))
+    rate = load * ps_rate / mean_work
+    poisson_arrivals(jobs, rate, seed)
+    log(f"serve: calibration {wall_cal:.2f} s ({ps_rate / 1e6:.1f} M particle-steps/s); arrival rate {rate:.2f} jobs/s")
+    wall = s.run(jobs)
+    dev_bytes = s.batch.stat("device_bytes")
+    s.close()
+    out = latency_summary(jobs, wall)
+    out.update(arrival_rate_per_s=round(rate, 4), target_load=load, capacity_particles=capacity,
+               device_bytes=int(dev_bytes), bytes_per_particle=round(dev_bytes / capacity, 1),
+               calibration_particle_steps_per_s=round(ps_rate, 1),
+               max_queue_wait_s=round(max(j.admitted - j.arrival for j in jobs), 4),
+               jobs_n=[j.n for j in jobs], jobs_steps=[j.steps for j in jobs])
+    for k in ("p50_s", "p99_s", "mean_s", "max_s", "wall_s"):
+        out[k] = round(out[k], 4)
+    out["updates_per_s"] = round(out["updates_per_s"], 1)
+    return out
+
+
+def ensemble_boxes(n_boxes=256, seed=1004):
+    """256 independent boxes: N log-uniform in [4096, 262144] (the nearest lattice cube), rms
+    Mach uniform in [0.1, 5], gamma 5/3, Wendland C2 with 64 neighbours."""
+    from gizmo_amd.serve import draw_jobs
+    return draw_jobs(n_boxes, seed, n_min=4096, n_max=262144, steps=(100, 100))
+
+
+def ensemble_bench(steps=100, n_boxes=256, seed=1004):
+    """The ensemble: every box started together in one batch (FP32-mixed) and advanced `steps`
+    sync points, against looping the plain FP64 form box by box (one context, begin and
+    `steps` steps per box).  Both time the steps only; the start of each box is untimed."""
+    from gizmo_amd.batch import Batch
+    from gizmo_amd.hydro import GZ_FP32_MIXED, GZ_FP64, Hydro
+    from gizmo_amd.serve import JOB_INTEG, job_params
+    boxes = ensemble_boxes(n_boxes, seed)
+    fields = [j.ic() for j in boxes]
+    total = sum(j.n for j in boxes)
+    log(f"ensemble: {n_boxes} boxes, {total} particles")
+    b = Batch(job_params(), total, n_boxes, GZ_FP32_MIXED)
+    for f in fields:
+        b.add(f, job_params(), JOB_INTEG)
+    t0 = time.perf_counter()
+    b.begin()
+    b.synchronize()
+    t_begin = time.perf_counter() - t0
+    u0 = b.stat("active_updates")
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        b.step()
+    b.synchronize()
+    t_b = time.perf_counter() - t0
+    upd_b = b.stat("active_updates") - u0
+    dev_bytes = b.stat("device_bytes")
+    b.close()
+    log(f"ensemble batched mixed: {t_b:.2f} s, {upd_b / t_b / 1e6:.1f} M updates/s (start {t_begin:.2f} s)")
+    t_l, upd_l = 0.0, 0.0
+    for k, f in enumerate(fields):
+        g = Hydro(job_params(), len(f["mass"]), GZ_FP64)
+        g.load(f)
+        g.run_begin(JOB_INTEG)
+        g.synchronize()
+        u0 = g.stat("active_updates")
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            g.run_step()
+        g.synchronize()
+        t_l += time.perf_counter() - t0
+        upd_l += g.stat("active_updates") - u0
+        g.close()
+    log(f"ensemble looped fp64: {t_l:.2f} s, {upd_l / t_l / 1e6:.1f} M updates/s")
+    return {"boxes": n_boxes, "particles": total, "steps": steps,
+            "batched_mixed": {"seconds": round(t_b, 3), "updates": int(upd_b), "updates_per_s": round(upd_b / t_b, 1),
+                              "start_seconds": round(t_begin, 3), "device_bytes": int(dev_bytes)},
+            "looped_fp64": {"seconds": round(t_l, 3), "updates": int(upd_l), "updates_per_s": round(upd_l / t_l, 1)},
+            "batched_over_looped": round((upd_b / t_b) / (upd_l / t_l), 3)}
+
+
 def cpu_baseline(ic, steps, target_s=20.0):
     """The oracle (C + OpenMP, FP64) on the same job: run_begin, then sync points until the
     step budget or the time budget is spent; the rate counts the timed steps only."""
@@ -172,6 +266,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-full-pass", action="store_true")
     ap.add_argument("--only", choices=["both", "fp64", "mixed"], default="both")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving-loop latency runs")
+    ap.add_argument("--serve-jobs", type=int, default=48)
+    ap.add_argument("--workload", choices=["sedov", "ensemble"], default="sedov",
+                    help="sedov: the 2M Sedov job (+ serving); ensemble: 256 boxes batched vs looped")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print(f"bench.py runs on one GPU: this build has no multi-GPU path (asked for {a.gpus}); "
@@ -181,6 +279,18 @@ def main():
 
     from gizmo_amd import ics
     from gizmo_amd.hydro import GZ_FP32_MIXED, GZ_FP64
+
+    if a.workload == "ensemble":
+        e = ensemble_bench(steps=a.steps if a.steps != 50 else 100)
+        print(json.dumps({
+            "metric": "hydro active-particle updates/s, 256-box ensemble (batched FP32-mixed vs looped FP64)",
+            "value": e["batched_mixed"]["updates_per_s"], "unit": "active-particle updates/s", "n_gpus": 1,
+            "steps": e["steps"], "warmup": 0, "ms_per_step": round(1e3 * e["batched_mixed"]["seconds"] / e["steps"], 3),
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32-mixed",
+            "data": "synthetic: seeded jittered lattices with Gaussian velocities (numpy Philox)",
+            "config": {"workload": "ensemble_256_boxes_N_loguniform_4096_262144_mach_0.1_5_100_steps"},
+            "ensemble": e}), flush=True)
+        return
 
     t0 = time.perf_counter()
     ic = ics.sedov_box(a.n_side, 1002, kernel=6, des_num_ngb=64.0)
@@ -241,6 +351,18 @@ def main():
         out["full_pass"] = fp
         if "fp64" in fp and "mixed" in fp:
             out["full_pass_mixed_over_fp64"] = round(fp["mixed"]["updates_per_s"] / fp["fp64"]["updates_per_s"], 3)
+    if not a.no_serve:
+        serving = {}
+        for f in todo:
+            log(f"serve {f}: {a.serve_jobs} jobs ...")
+            serving[f] = serve_bench(forms[f], n_jobs=a.serve_jobs)
+            sv = serving[f]
+            log(f"serve {f}: p50 {sv['p50_s']:.3f} s, p99 {sv['p99_s']:.3f} s, {sv['updates_per_s'] / 1e6:.1f} M updates/s")
+        out["serving"] = serving
+        sv = serving["mixed" if "mixed" in serving else head]
+        out["p50_ms"] = round(1e3 * sv["p50_s"], 1)
+        out["p99_ms"] = round(1e3 * sv["p99_s"], 1)
+        out["latency_form"] = "mixed" if "mixed" in serving else head
     if not a.no_cpu:
         log("cpu baseline ...")
         out["cpu_baseline"] = cpu_baseline(ic, a.steps)

@@ -23,6 +23,8 @@
 
 using namespace gz;
 
+__global__ void k_iota(int n, int *out);
+
 namespace {
 
 constexpr int kBlock = 256;
@@ -65,6 +67,16 @@ struct gz_context {
     bool all_active = true;
     int *perm = nullptr;      // device order -> caller index
     uint32_t *key = nullptr;  // fine Morton key, device order
+
+    // search over many grid levels (kernel radii spanning orders of magnitude): per-level cell
+    // tables of the order and of each level's own particles
+    int wide = 0;
+    WideGrid wide_h{};
+    WideGrid *wide_dev = nullptr;
+    int *wstart = nullptr, *wend = nullptr, *wlstart = nullptr, *wlend = nullptr, *lidx = nullptr, *plev = nullptr, *lcount = nullptr;
+    long long wcells_alloc = 0;
+    long wide_key_version = -1;
+    int wide_lmax = -1;
 
     // staging and scratch
     double *stage = nullptr;  // n*9 doubles
@@ -318,6 +330,10 @@ void free_state(gz_context *c)
     dfree(c->U); dfree(c->NN); dfree(c->DRK); dfree(c->DIVV); dfree(c->RHO); dfree(c->PR); dfree(c->T); dfree(c->FW);
     dfree(c->CN); dfree(c->FCE); dfree(c->GR); dfree(c->GP); dfree(c->GV); dfree(c->ACC); dfree(c->DTU); dfree(c->MSV);
     dfree(c->TB); dfree(c->WAKE); dfree(c->ACT); dfree(c->perm); dfree(c->key); dfree(c->key2);
+    dfree(c->wstart); dfree(c->wend); dfree(c->wlstart); dfree(c->wlend); dfree(c->lidx); dfree(c->plev); dfree(c->lcount);
+    dfree(c->wide_dev);
+    c->wcells_alloc = 0;
+    c->wide_key_version = -1;
     dfree(c->ival); dfree(c->ival2); dfree(c->iflag); dfree(c->iscan); dfree(c->act); dfree(c->stage);
     dfree(c->it_h); dfree(c->it_hb); dfree(c->it_l); dfree(c->it_r); dfree(c->it_cn); dfree(c->it_ci); dfree(c->it_it);
     dfree(c->todo); dfree(c->todo2); dfree(c->cnt_d); dfree(c->cnt_s); dfree(c->counters); dfree(c->ell_d); dfree(c->ell_s);
@@ -427,7 +443,8 @@ int build_grid(gz_context *ctx, double rmax, bool need_chmax = false)
 
 GridView grid_view(gz_context *ctx)
 {
-    GridView g;
+    GridView g{};
+    g.wide = ctx->wide ? ctx->wide_dev : nullptr;
     g.level = ctx->level;
     g.shift = 3 * (kFineLevel - ctx->level);
     g.cell = ctx->cell;
@@ -497,6 +514,92 @@ int max_on_device(gz_context *ctx, const double *v, long n, double *out)
     HIPCHK(hipcub::DeviceReduce::Max(ctx->tmp, bytes, v, dout, (int)n, ctx->stream));
     HIPCHK(hipMemcpyAsync(out, dout, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+int min_on_device(gz_context *ctx, const double *v, long n, double *out)
+{
+    size_t bytes = 0;
+    double *dout = (double *)ctx->stage;
+    HIPCHK(hipcub::DeviceReduce::Min(nullptr, bytes, v, dout, (int)n, ctx->stream));
+    CHK(ensure_tmp(ctx, bytes));
+    HIPCHK(hipcub::DeviceReduce::Min(ctx->tmp, bytes, v, dout, (int)n, ctx->stream));
+    HIPCHK(hipMemcpyAsync(out, dout, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+// the finest level (at most lmax) whose cells are no smaller than reach (host twin of the
+// device wide_level)
+int host_wide_level(double box, int lmax, double reach)
+{
+    int L = 0;
+    while (L < lmax && box / (double)(1 << (L + 1)) >= reach * (1.0 + 1e-6)) L++;
+    return L;
+}
+
+// Tables of the search over many levels: cell ranges of the order at every level down to the
+// one of the smallest radius (kept while the order and that level stay), and with lists, every
+// particle's own level and the per-level tables of each level's particles (from the radii now).
+int build_wide(gz_context *ctx, bool lists)
+{
+    const long n = ctx->n;
+    const double box = ctx->par.box_size;
+    double hmin = 0;
+    CHK(min_on_device(ctx, ctx->H, n, &hmin));
+    if (!(hmin > 0)) hmin = 1e-300;
+    // at most level 9 (512^3 cells, 1.2 G table entries over all levels; finer radii walk a
+    // few more candidates there)
+    const int lmax = host_wide_level(box, kFineLevel - 1, hmin + 2.0 * ctx->dmax);
+    long long cells = 0;
+    WideGrid &W = ctx->wide_h;
+    for (int L = 0; L <= lmax; L++) { W.loff[L] = cells; cells += 1LL << (3 * L); }
+    if (cells > ctx->wcells_alloc) {
+        // entries are validated on read: fresh tables need no clearing
+        CHK(dalloc(ctx, &ctx->wstart, cells)); CHK(dalloc(ctx, &ctx->wend, cells));
+        CHK(dalloc(ctx, &ctx->wlstart, cells)); CHK(dalloc(ctx, &ctx->wlend, cells));
+        ctx->wcells_alloc = cells;
+        ctx->wide_key_version = -1;
+    }
+    if (!ctx->lidx) {
+        CHK(dalloc(ctx, &ctx->lidx, ctx->cap)); CHK(dalloc(ctx, &ctx->plev, ctx->cap)); CHK(dalloc(ctx, &ctx->lcount, kFineLevel + 2));
+        CHK(dalloc(ctx, &ctx->wide_dev, 1));
+    }
+    W.key = ctx->key;
+    W.n = (int)n;
+    W.lmax = lmax;
+    W.start = ctx->wstart; W.end = ctx->wend; W.lidx = ctx->lidx; W.lstart = ctx->wlstart; W.lend = ctx->wlend;
+    Timed t(ctx, "ngb_build");
+    if (ctx->wide_key_version != ctx->key_version || lmax > ctx->wide_lmax) {
+        k_wide_bounds<<<nblk(n), kBlock, 0, ctx->stream>>>(W);
+        HIPCHK(hipGetLastError());
+        ctx->wide_key_version = ctx->key_version;
+        ctx->wide_lmax = lmax;
+    }
+    if (lists) {
+        HIPCHK(hipMemsetAsync(ctx->lcount, 0, (kFineLevel + 2) * sizeof(int), ctx->stream));
+        k_particle_level<<<nblk(n), kBlock, 0, ctx->stream>>>((int)n, ctx->H, box, ctx->dmax, lmax, ctx->plev, ctx->lcount);
+        k_iota<<<nblk(n), kBlock, 0, ctx->stream>>>((int)n, ctx->ival);
+        HIPCHK(hipGetLastError());
+        size_t bytes = 0;
+        HIPCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, ctx->plev, (int *)ctx->key2, ctx->ival, ctx->lidx, (int)n, 0, 4, ctx->stream));
+        CHK(ensure_tmp(ctx, bytes));
+        HIPCHK(hipcub::DeviceRadixSort::SortPairs(ctx->tmp, bytes, ctx->plev, (int *)ctx->key2, ctx->ival, ctx->lidx, (int)n, 0, 4, ctx->stream));
+        int counts[kFineLevel + 2];
+        HIPCHK(hipMemcpyAsync(counts, ctx->lcount, sizeof counts, hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        int acc = 0;
+        for (int L = 0; L <= kFineLevel + 1; L++) {
+            W.lofs[L] = acc;
+            if (L <= kFineLevel) acc += counts[L];
+        }
+        k_wide_list_bounds<<<nblk(n), kBlock, 0, ctx->stream>>>(W, ctx->plev);
+        HIPCHK(hipGetLastError());
+    }
+    HIPCHK(hipMemcpyAsync(ctx->wide_dev, &W, sizeof W, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    // the displacement that calls for a new order is measured against the smallest radius
+    ctx->last_rmax = hmin;
     return 0;
 }
 
@@ -587,8 +690,10 @@ int density_stage(gz_context *ctx)
         HIPCHK(hipStreamSynchronize(ctx->stream));
         memcpy(&hmax, &ctx->hu64c[67], sizeof hmax);
     }
-    CHK(build_grid(ctx, std::min(hmax, 0.5 * ctx->par.box_size)));
-    const bool few = na <= ctx->few_rows;
+    const bool wide = ctx->wide != 0;
+    if (wide) CHK(build_wide(ctx, false));
+    else CHK(build_grid(ctx, std::min(hmax, 0.5 * ctx->par.box_size)));
+    const bool few = wide || na <= ctx->few_rows;
     if (!few) CHK(build_cell_rows(ctx));
     DensityIter it{ctx->it_h, ctx->it_hb, ctx->it_l, ctx->it_r, ctx->it_cn, ctx->it_ci, ctx->it_it};
     DensityOut out{ctx->n, ctx->H, ctx->NN, ctx->DRK, ctx->DIVV, ctx->RHO, ctx->PR, ctx->T, ctx->FW, ctx->CN, ctx->FCE};
@@ -606,7 +711,7 @@ int density_stage(gz_context *ctx)
                     ctx->dp, grid_view(ctx), cell_rows(ctx), ctx->act, ctx->need, ctx->X, ctx->Y, ctx->Z, ctx->VX, ctx->VY, ctx->VZ,
                     ctx->M, ctx->it_h, ctx->dacc);
             k_density_after<<<nblk(na), kBlock, 0, ctx->stream>>>(ctx->dp, na, ctx->act, ctx->need, ctx->dacc, ctx->M, ctx->U,
-                                                                  ctx->cell - 2.0 * ctx->dmax, it, out, ctx->counters + 0, ctx->counters + 1,
+                                                                  wide ? 1e300 : ctx->cell - 2.0 * ctx->dmax, it, out, ctx->counters + 0, ctx->counters + 1,
                                                                   ctx->counters + 2, ctx->u64c + 68, BoxTable{});
             HIPCHK(hipGetLastError());
         }
@@ -620,7 +725,7 @@ int density_stage(gz_context *ctx)
             return fail(ctx, err, "density(): device stop code %d", err);
         }
         if (remaining == 0) break;
-        if (rebuild) {
+        if (rebuild && !wide) {
             // a radius outgrew the cell side: a coarser grid for the rows still iterating
             CHK(max_on_device(ctx, ctx->it_h, na, &hmax));
             CHK(build_grid(ctx, std::min(hmax, 0.5 * ctx->par.box_size)));
@@ -649,9 +754,11 @@ int ensure_sym(gz_context *ctx)
         CHK(max_on_device(ctx, ctx->H, ctx->n, &hmax));
         ctx->hmax_bound = hmax;
     }
-    const bool few = ctx->nact <= ctx->few_rows;
+    const bool wide = ctx->wide != 0;
+    const bool few = wide || ctx->nact <= ctx->few_rows;
     // the row walks drop stencil cells beyond reach: they need each cell's largest radius
-    CHK(build_grid(ctx, std::min(hmax, 0.5 * ctx->par.box_size), few));
+    if (wide) CHK(build_wide(ctx, true));
+    else CHK(build_grid(ctx, std::min(hmax, 0.5 * ctx->par.box_size), few));
     if (!few) CHK(build_cell_rows(ctx));
     const long need_ld = ((long)ctx->nact + 63) / 64 * 64;
     const int cap_hint = std::max(64, (int)std::ceil(ctx->par.des_num_ngb * 2.0) + 32);
@@ -664,7 +771,11 @@ int ensure_sym(gz_context *ctx)
         HIPCHK(hipMemsetAsync(ctx->counters + 3, 0, sizeof(int), ctx->stream));
         {
             Timed t(ctx, "ngb_build");
-            if (few)
+            if (wide)
+                k_build_sym_wide<<<xcd_grid(ctx->nact), 64, 0, ctx->stream>>>(grid_view(ctx), ctx->nact, ctx->act, ctx->X, ctx->Y, ctx->Z,
+                                                                               ctx->H, ctx->ell_s, ctx->ld_s, ctx->cap_s, ctx->cnt_s,
+                                                                               ctx->counters + 3);
+            else if (few)
                 k_build_sym_rows<<<xcd_grid(ctx->nact), 64, 0, ctx->stream>>>(grid_view(ctx), ctx->nact, ctx->act, ctx->key, ctx->X, ctx->Y,
                                                                                ctx->Z, ctx->H, ctx->ell_s, ctx->ld_s, ctx->cap_s, ctx->cnt_s,
                                                                                ctx->counters + 3, BoxTable{});
@@ -1363,6 +1474,12 @@ int gz_set_option(gz_context *ctx, const char *name, double value)
         ctx->few_rows = (int)value;
         return 0;
     }
+    if (k == "wide_search") {
+        if (value != 0 && value != 1) return fail(ctx, GZ_E_ARG, "wide_search must be 0 or 1");
+        ctx->wide = (int)value;
+        ctx->sym_valid = false;
+        return 0;
+    }
     return fail(ctx, GZ_E_ARG, "unknown option %s", name);
 }
 
@@ -1397,8 +1514,28 @@ long gz_neighbors(gz_context *ctx, int symmetric, long *offsets, int *nbr, long 
     if (max_on_device(ctx, ctx->H, n, &hmax)) return -GZ_E_HIP;
     if (build_grid(ctx, std::min(hmax, 0.5 * ctx->par.box_size), true)) return -GZ_E_HIP;
     if (dalloc(ctx, &cnt, ctx->nact)) return -GZ_E_MEMORY;
-    int rc = symmetric ? build_list<true>(ctx, ctx->nact, nullptr, nullptr, 1.0, &ell, &ld, &capl, cnt, 64)
+    int rc = 0;
+    if (symmetric && ctx->wide) {
+        // the search over many levels (the list the gradient and force passes use)
+        rc = build_wide(ctx, true);
+        ld = ((long)ctx->nact + 63) / 64 * 64;
+        capl = 64;
+        for (int attempt = 0; !rc && attempt < 12; attempt++) {
+            rc = dalloc(ctx, &ell, (size_t)capl * ld);
+            if (rc) break;
+            if (hipMemsetAsync(ctx->counters + 3, 0, sizeof(int), ctx->stream) != hipSuccess) { rc = GZ_E_HIP; break; }
+            k_build_sym_wide<<<xcd_grid(ctx->nact), 64, 0, ctx->stream>>>(grid_view(ctx), ctx->nact, ctx->act, ctx->X, ctx->Y, ctx->Z,
+                                                                           ctx->H, ell, ld, capl, cnt, ctx->counters + 3);
+            if (hipGetLastError() != hipSuccess) { rc = GZ_E_HIP; break; }
+            rc = read_counters(ctx);
+            if (rc || ctx->hcounters[3] == 0) break;
+            capl = std::max(ctx->hcounters[3] + 16, capl * 3 / 2);
+        }
+        if (!rc && ctx->hcounters[3]) rc = GZ_E_STATE;
+    } else {
+        rc = symmetric ? build_list<true>(ctx, ctx->nact, nullptr, nullptr, 1.0, &ell, &ld, &capl, cnt, 64)
                        : build_list<false>(ctx, ctx->nact, nullptr, nullptr, 1.0, &ell, &ld, &capl, cnt, 64);
+    }
     if (rc) { dfree(ell); dfree(cnt); return -rc; }
     ctx->sym_valid = false;  // the grid changed
     std::vector<int> c(ctx->nact), e((size_t)capl * ld);

@@ -95,6 +95,9 @@ struct gz_batch {
     int *counters = nullptr, *hcounters = nullptr;
     int *lim = nullptr;
 
+    std::map<void *, size_t> alloc_bytes;  // live device allocations
+    size_t dev_bytes = 0;
+
     std::vector<BoxHost> boxes;
     std::vector<int> order;  // live boxes in storage (rank) order
     int nact = 0;
@@ -127,16 +130,23 @@ int fail(gz_batch *c, int code, const char *fmt, ...)
         if (rc_) return rc_; \
     } while (0)
 
+// device allocation of count elements (any previous one freed), counted in the batch's footprint
 template <typename T>
 int balloc(gz_batch *b, T **p, size_t count)
 {
-    if (*p) (void)hipFree(*p);
+    if (*p) {
+        auto it = b->alloc_bytes.find((void *)*p);
+        if (it != b->alloc_bytes.end()) { b->dev_bytes -= it->second; b->alloc_bytes.erase(it); }
+        (void)hipFree(*p);
+    }
     *p = nullptr;
     if (count == 0) count = 1;
     if (hipMalloc((void **)p, count * sizeof(T)) != hipSuccess) {
         *p = nullptr;
         return fail(b, GZ_E_MEMORY, "hipMalloc of %zu bytes failed", count * sizeof(T));
     }
+    b->alloc_bytes[(void *)*p] = count * sizeof(T);
+    b->dev_bytes += count * sizeof(T);
     return 0;
 }
 
@@ -150,11 +160,10 @@ void bfree(T *&p)
 int ensure_tmp(gz_batch *b, size_t bytes)
 {
     if (bytes <= b->tmp_bytes) return 0;
-    if (b->tmp) (void)hipFree(b->tmp);
-    b->tmp = nullptr;
-    b->tmp_bytes = 0;
     const size_t nbytes = std::max(bytes, (size_t)1 << 20);
-    BHIP(hipMalloc(&b->tmp, nbytes));
+    char *t = (char *)b->tmp;
+    BCHK(balloc(b, &t, nbytes));
+    b->tmp = t;
     b->tmp_bytes = nbytes;
     return 0;
 }
@@ -256,10 +265,15 @@ int sync(gz_batch *b)
     return 0;
 }
 
+// Host-to-device uploads of the small host tables wait for their copy: the host may change
+// the table again right after, and the source of an asynchronous copy must stay unchanged
+// until the copy has run.  They are issued when no other work is queued, so the wait is short.
+
 // one whole table entry (a box that is not stepping: no kernel writes it meanwhile)
 int upload_entry(gz_batch *b, int id)
 {
     BHIP(hipMemcpyAsync(b->tab + id, &b->htab[id], sizeof(BoxDev), hipMemcpyHostToDevice, b->stream));
+    BHIP(hipStreamSynchronize(b->stream));
     return 0;
 }
 
@@ -269,12 +283,14 @@ int upload_grid(gz_batch *b)
     const size_t lo = offsetof(BoxDev, cell), hi = offsetof(BoxDev, time);
     BHIP(hipMemcpy2DAsync((char *)b->tab + lo, sizeof(BoxDev), (const char *)b->htab.data() + lo, sizeof(BoxDev), hi - lo,
                           nbox_all(b), hipMemcpyHostToDevice, b->stream));
+    BHIP(hipStreamSynchronize(b->stream));
     return 0;
 }
 
 int upload_bytes(gz_batch *b, const std::vector<uint8_t> &s, uint8_t *dst)
 {
     BHIP(hipMemcpyAsync(dst, s.data(), s.size(), hipMemcpyHostToDevice, b->stream));
+    BHIP(hipStreamSynchronize(b->stream));
     return 0;
 }
 
@@ -358,7 +374,7 @@ int reorder(gz_batch *b, const std::vector<uint8_t> &due_h)
     std::vector<int> rk(nbox_all(b), b->max_boxes + 1);
     for (size_t r = 0; r < b->order.size(); r++) rk[b->order[r]] = (int)r;
     BHIP(hipMemcpyAsync(b->rank, rk.data(), rk.size() * sizeof(int), hipMemcpyHostToDevice, b->stream));
-    BCHK(upload_bytes(b, due_h, b->due));
+    BCHK(upload_bytes(b, due_h, b->due));  // (waits for both copies)
     k_batch_keys<<<nb(n), kB, 0, b->stream>>>(n, b->X, b->Y, b->Z, b->BOX, b->tab, b->rank, b->due, b->key, b->key64, b->ival);
     BHIP(hipGetLastError());
     int bits = 30;
@@ -443,6 +459,8 @@ int density(gz_batch *b)
     const int na = b->nact;
     if (na == 0) return 0;
     const BoxTable bt = table(b);
+    // cells of this step's order, radius bound and displacement bound
+    BCHK(build_grid(b));
     BHIP(hipMemsetAsync(b->slot, 0, kBoxSlots * nbox_all(b) * 8, b->stream));
     k_density_init<<<nb(na), kB, 0, b->stream>>>(na, b->act, b->H, 0, b->it_h, b->it_l, b->it_r, b->it_cn, b->it_ci, b->it_it, b->need,
                                                  b->counters, nullptr, b->u64 + 1, bt);
@@ -1029,6 +1047,10 @@ int gz_batch_stat(gz_batch *b, const char *name, double *value)
         BHIP(hipMemcpyAsync(b->hu64, b->u64, 8, hipMemcpyDeviceToHost, b->stream));
         BCHK(sync(b));
         *value = (double)b->hu64[0];
+        return 0;
+    }
+    if (!strcmp(name, "device_bytes")) {
+        *value = (double)b->dev_bytes;
         return 0;
     }
     auto it = b->stats.find(name);

@@ -19,6 +19,24 @@ __device__ __forceinline__ long xcd_block()
 }
 inline unsigned xcd_grid(long nblocks) { return (unsigned)(((nblocks + kXcds - 1) / kXcds) * kXcds); }
 
+// Search over kernel radii that span many grid levels.  Every cell of every level 0..lmax is a
+// contiguous range of the Morton order; `start`/`end` hold each cell's range (level L from
+// entry loff[L]).  An entry is used only if it is exactly a maximal run of the cell's code, so
+// entries of cells that are empty now need no clearing.  Each particle also has its own level
+// (the finest whose cells hold its kernel radius plus twice the displacement bound); the
+// particles of one level, in ascending index (list `lidx`, slice [lofs[L], lofs[L+1])), have
+// tables of their own (`lstart`/`lend`, positions in lidx), so a row can reach the few
+// particles of coarser levels whose radius covers it without walking the fine ones.
+struct WideGrid {
+    const uint32_t *key;       // Morton keys of the order
+    int n, lmax;
+    int *start, *end;          // (written by the table builds only)
+    const int *lidx;
+    int *lstart, *lend;
+    long long loff[kFineLevel + 1];
+    int lofs[kFineLevel + 2];
+};
+
 struct GridView {
     int level, shift;         // nc = 2^level cells per side; cell code = key >> shift
     double cell, box, half;
@@ -28,6 +46,7 @@ struct GridView {
     double margin;            // largest distance a particle has moved from its cell since the ordering
     const int *cstart, *cend; // [start, end) of each cell
     long long cell_off;       // first cell of the box in the cell arrays (batches of boxes)
+    const WideGrid *wide;     // device copy; non-null: per-row levels over the tables above
 };
 
 // One periodic box of a batch: geometry, grid and timeline.  A batch holds many independent
@@ -137,6 +156,13 @@ __global__ void k_advance_boxes(BoxTable bt);
 __global__ void k_active_from_bins(long n, unsigned long long mask, const int *TB, const double *M, const double *H,
                                    uint8_t *ACT, double *SOFT);
 __global__ void k_mark_records(int nrows, const int *act, const int *ell, long ld, const int *cnt, uint8_t *need);
+
+// search over many grid levels
+__global__ void k_wide_bounds(WideGrid W);
+__global__ void k_particle_level(int n, const double *H, double box, double margin, int lmax, int *plev, int *level_count);
+__global__ void k_wide_list_bounds(WideGrid W, const int *plev);
+__global__ void k_build_sym_wide(GridView g, int nrows, const int *act, const double *X, const double *Y, const double *Z,
+                                 const double *H, int *ell, long ld, int cap, int *cnt, int *overflow);
 
 // batches of independent boxes
 __global__ void k_batch_keys(long n, const double *X, const double *Y, const double *Z, const int *BOX, const BoxDev *tab,

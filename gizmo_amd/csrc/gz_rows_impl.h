@@ -22,6 +22,38 @@ struct RowStencil {
     int nst;
 };
 
+// of equal codes (grids under 3 cells a side) the first lane stays; ranks by code; prefix of
+// the lengths.  Lanes below 27 bring their cell's code (0x7fffffff: dropped) and range, and
+// have stored the code in rs.code.
+__device__ __forceinline__ void stencil_finish(RowStencil &rs, int code, int st, int en)
+{
+    const int lane = threadIdx.x;
+    __syncthreads();
+    bool first = false;
+    if (lane < 27 && code != 0x7fffffff) {
+        first = true;
+        for (int q = 0; q < lane; q++)
+            if (rs.code[q] == code) { first = false; break; }
+    }
+    const unsigned long long firsts = __ballot(first);
+    if (first) {
+        int rank = 0;  // distinct kept codes below this one
+        for (int q = 0; q < 27; q++)
+            if (((firsts >> q) & 1ull) && rs.code[q] < code) rank++;
+        rs.start[rank] = st;
+        rs.len[rank] = en - st;
+    }
+    __syncthreads();
+    if (lane == 0) {
+        const int m = __popcll(firsts);
+        int acc = 0;
+        for (int k = 0; k < m; k++) { rs.pref[k] = acc; acc += rs.len[k]; }
+        rs.pref[m] = acc;
+        rs.nst = m;
+    }
+    __syncthreads();
+}
+
 // The stencil of the row's home cell, built by 27 lanes (one per neighbour cell): periodic
 // neighbours, duplicates of small grids dropped, cells beyond reach dropped (the row lies
 // farther from the cell's box than its reach plus the distance any particle has moved from its
@@ -57,31 +89,7 @@ __device__ __forceinline__ void row_stencil(const GridView &g, uint32_t home, do
         if (keep) code = c;
         rs.code[lane] = code;
     }
-    __syncthreads();
-    // of equal codes (grids under 3 cells a side) the first lane stays
-    bool first = false;
-    if (lane < 27 && code != 0x7fffffff) {
-        first = true;
-        for (int q = 0; q < lane; q++)
-            if (rs.code[q] == code) { first = false; break; }
-    }
-    const unsigned long long firsts = __ballot(first);
-    if (first) {
-        int rank = 0;  // distinct kept codes below this one
-        for (int q = 0; q < 27; q++)
-            if (((firsts >> q) & 1ull) && rs.code[q] < code) rank++;
-        rs.start[rank] = st;
-        rs.len[rank] = en - st;
-    }
-    __syncthreads();
-    if (lane == 0) {
-        const int m = __popcll(firsts);
-        int acc = 0;
-        for (int k = 0; k < m; k++) { rs.pref[k] = acc; acc += rs.len[k]; }
-        rs.pref[m] = acc;
-        rs.nst = m;
-    }
-    __syncthreads();
+    stencil_finish(rs, code, st, en);
 }
 
 __device__ __forceinline__ int row_candidate(const RowStencil &rs, int v)
@@ -89,6 +97,71 @@ __device__ __forceinline__ int row_candidate(const RowStencil &rs, int v)
     int k = 0;
     while (k + 1 < rs.nst && rs.pref[k + 1] <= v) k++;
     return rs.start[k] + (v - rs.pref[k]);
+}
+
+// ---------------------------------------------------------------------------------------
+// search over many grid levels (WideGrid)
+// ---------------------------------------------------------------------------------------
+
+// the finest level (at most lmax) whose cells are no smaller than reach
+__device__ __forceinline__ int wide_level(double box, int lmax, double reach)
+{
+    int L = 0;
+    while (L < lmax && box / (double)(1 << (L + 1)) >= reach * (1.0 + 1e-6)) L++;
+    return L;
+}
+
+// the range [a, b) of the cell with code c (key >> s) from table entry e, if the entry is
+// exactly a maximal run of that code among positions [lo, hi) (map: position -> particle,
+// nullptr for the order itself); otherwise the cell is empty
+__device__ __forceinline__ bool wide_range(const int *__restrict__ start, const int *__restrict__ end, long long e,
+                                           const uint32_t *__restrict__ key, const int *__restrict__ map, int lo, int hi, int s,
+                                           uint32_t c, int &a, int &b)
+{
+    a = start[e];
+    b = end[e];
+    if (!(a >= lo && a < b && b <= hi)) return false;
+    auto code = [&](int p) { return key[map ? map[p] : p] >> s; };
+    if (code(a) != c || code(b - 1) != c) return false;
+    if (a > lo && code(a - 1) == c) return false;
+    if (b < hi && code(b) == c) return false;
+    return true;
+}
+
+// The 27-cell stencil at level L around the row (home = the row's key >> s), from the tables
+// of all particles (LIST false: ranges of the order) or of the particles of level L (LIST
+// true: ranges of positions in lidx).  Cells beyond reach + margin are dropped when reach > 0.
+template <bool LIST>
+__device__ __forceinline__ void row_stencil_wide(const GridView &g, const WideGrid &W, int L, uint32_t home, double xi, double yi,
+                                                 double zi, double reach, RowStencil &rs)
+{
+    const int lane = threadIdx.x;
+    const int nc = 1 << L;
+    const int s = 3 * (kFineLevel - L);
+    const double cell = g.box / (double)nc;
+    const int cx = compact10(home >> 2), cy = compact10(home >> 1), cz = compact10(home);
+    int code = 0x7fffffff, st = 0, en = 0;
+    if (lane < 27) {
+        const int dx = lane / 9 - 1, dy = (lane / 3) % 3 - 1, dz = lane % 3 - 1;
+        const int a = (cx + dx + nc) % nc, b = (cy + dy + nc) % nc, d = (cz + dz + nc) % nc;
+        const uint32_t c = (spread10(a) << 2) | (spread10(b) << 1) | spread10(d);
+        const long long e = W.loff[L] + c;
+        bool keep = LIST ? wide_range(W.lstart, W.lend, e, W.key, W.lidx, W.lofs[L], W.lofs[L + 1], s, c, st, en)
+                         : wide_range(W.start, W.end, e, W.key, nullptr, 0, W.n, s, c, st, en);
+        if (keep && nc >= 4 && reach > 0) {
+            const double ox = cx * cell, oy = cy * cell, oz = cz * cell;
+            const double rx = wrap(xi - ox, g.box, g.half), ry = wrap(yi - oy, g.box, g.half), rz = wrap(zi - oz, g.box, g.half);
+            const double gx = fmax(fmax(dx * cell - rx, rx - (dx + 1) * cell), 0.0);
+            const double gy = fmax(fmax(dy * cell - ry, ry - (dy + 1) * cell), 0.0);
+            const double gz = fmax(fmax(dz * cell - rz, rz - (dz + 1) * cell), 0.0);
+            const double rr = (reach + g.margin) * (1.0 + 1e-9) + 1e-12 * g.box;
+            if (gx * gx + gy * gy + gz * gz > rr * rr) keep = false;
+        }
+        if (!keep) { st = 0; en = 0; }
+        if (keep) code = (int)c;
+        rs.code[lane] = code;
+    }
+    stencil_finish(rs, code, st, en);
 }
 
 // density sums of the rows still iterating (DensitySpec::pair_kernel,
@@ -117,7 +190,14 @@ __global__ void __launch_bounds__(64) k_density_rows(DevParams P, GridView g, in
     const R h2 = hR * hR;
     R hinv, hinv3, hinv4;
     kernel_hinv(hR, hinv, hinv3, hinv4);
-    row_stencil<false>(g, key[i] >> g.shift, xi, yi, zi, h, rs);
+    if (g.wide) {
+        // the row's own level: cells no smaller than its radius plus twice the displacement bound
+        const WideGrid &W = *g.wide;
+        const int L = wide_level(g.box, W.lmax, h + 2.0 * g.margin);
+        row_stencil_wide<false>(g, W, L, key[i] >> (3 * (kFineLevel - L)), xi, yi, zi, h, rs);
+    } else {
+        row_stencil<false>(g, key[i] >> g.shift, xi, yi, zi, h, rs);
+    }
     const int total = rs.pref[rs.nst];
     R acc = 0;  // lane q < 13 keeps sum q
     for (int base = 0; base < total; base += 64) {
@@ -214,6 +294,147 @@ __global__ void __launch_bounds__(64) k_build_sym_rows(GridView g, int nrows, co
         const int pos = kept_rank(acc, cntk);
         if (acc && n + pos < cap) ell[(size_t)(n + pos) * ld + r] = j;
         n += cntk;
+    }
+    if (lane == 0) {
+        cnt[r] = n;
+        if (n > cap) atomicMax(overflow, n);
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// search over many grid levels: tables, particle levels, symmetric rows
+// ---------------------------------------------------------------------------------------
+
+// the range of every cell of every level 0..lmax that holds particles (one lane per position)
+__global__ void __launch_bounds__(256) k_wide_bounds(WideGrid W)
+{
+    const long p = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (p >= W.n) return;
+    const uint32_t k = W.key[p];
+    const uint32_t kp = p > 0 ? W.key[p - 1] : 0, kn = p + 1 < W.n ? W.key[p + 1] : 0;
+    for (int L = 0; L <= W.lmax; L++) {
+        const int s = 3 * (kFineLevel - L);
+        const uint32_t c = k >> s;
+        if (p == 0 || (kp >> s) != c) W.start[W.loff[L] + c] = (int)p;
+        if (p + 1 == W.n || (kn >> s) != c) W.end[W.loff[L] + c] = (int)(p + 1);
+    }
+}
+
+// each particle's own level (the finest, at most lmax, whose cells hold its radius plus twice
+// the displacement bound), and the number of particles per level
+__global__ void __launch_bounds__(256) k_particle_level(int n, const double *__restrict__ H, double box, double margin, int lmax,
+                                                         int *__restrict__ plev, int *__restrict__ level_count)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const int L = wide_level(box, lmax, H[p] + 2.0 * margin);
+    plev[p] = L;
+    atomicAdd(&level_count[L], 1);
+}
+
+// the range of every cell of level L among the particles of level L (positions in lidx)
+__global__ void __launch_bounds__(256) k_wide_list_bounds(WideGrid W, const int *__restrict__ plev)
+{
+    const long p = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (p >= W.n) return;
+    const int j = W.lidx[p];
+    const int L = plev[j];
+    const int s = 3 * (kFineLevel - L);
+    const uint32_t c = W.key[j] >> s;
+    const int lo = W.lofs[L], hi = W.lofs[L + 1];
+    if (p == lo || (W.key[W.lidx[p - 1]] >> s) != c) W.lstart[W.loff[L] + c] = (int)p;
+    if (p + 1 == hi || (W.key[W.lidx[p + 1]] >> s) != c) W.lend[W.loff[L] + c] = (int)(p + 1);
+}
+
+__device__ __forceinline__ bool periodic_adjacent(int a, int b, int nc)
+{
+    const int d = ((a - b) % nc + nc) % nc;
+    return d <= 1 || d >= nc - 1;
+}
+
+// Symmetric rows r2 < max(h_i,h_j)^2 (src/mesh/sfc_tiles_functions.h:83-102) over many levels.
+// Part A: every particle in the 27 cells of the row's own level L_i (it holds every partner of
+// level L_i or finer, and every partner within h_i).  Part B: for each coarser level L, the
+// particles of level L in the 27 cells of level L around the row (a partner j of level L with
+// r < h_j lies there) that Part A did not cover.  The row is then put in ascending order.
+__global__ void __launch_bounds__(64) k_build_sym_wide(GridView g, int nrows, const int *__restrict__ act, const double *__restrict__ X,
+                                                       const double *__restrict__ Y, const double *__restrict__ Z,
+                                                       const double *__restrict__ H, int *__restrict__ ell, long ld, int cap,
+                                                       int *__restrict__ cnt, int *__restrict__ overflow)
+{
+    __shared__ RowStencil rs;
+    __shared__ int srt[1024];
+    const long r = xcd_block();
+    if (r >= nrows) return;
+    const int lane = threadIdx.x;
+    const int i = act[r];
+    const WideGrid &W = *g.wide;
+    const double xi = X[i], yi = Y[i], zi = Z[i], hi = H[i];
+    const int Li = wide_level(g.box, W.lmax, hi + 2.0 * g.margin);
+    const int si = 3 * (kFineLevel - Li);
+    const uint32_t homei = W.key[i] >> si;
+    row_stencil_wide<false>(g, W, Li, homei, xi, yi, zi, -1.0, rs);
+    int total = rs.pref[rs.nst];
+    int n = 0;
+    for (int base = 0; base < total; base += 64) {
+        const int v = base + lane;
+        bool acc = false;
+        int j = -1;
+        if (v < total) {
+            j = row_candidate(rs, v);
+            acc = (j != i) && accept_pair<true>(xi, yi, zi, X[j], Y[j], Z[j], hi, H[j], g.box, g.half);
+        }
+        int cntk;
+        const int pos = kept_rank(acc, cntk);
+        if (acc && n + pos < cap) ell[(size_t)(n + pos) * ld + r] = j;
+        n += cntk;
+    }
+    const int nA = n;
+    const int nci = 1 << Li;
+    const int hx = compact10(homei >> 2), hy = compact10(homei >> 1), hz = compact10(homei);
+    for (int L = 0; L < Li; L++) {
+        if (W.lofs[L + 1] == W.lofs[L]) continue;  // no particle of this level
+        __syncthreads();  // every lane is done with rs before it is built again
+        const int s = 3 * (kFineLevel - L);
+        row_stencil_wide<true>(g, W, L, W.key[i] >> s, xi, yi, zi, -1.0, rs);
+        total = rs.pref[rs.nst];
+        for (int base = 0; base < total; base += 64) {
+            const int v = base + lane;
+            bool acc = false;
+            int j = -1;
+            if (v < total) {
+                j = W.lidx[row_candidate(rs, v)];
+                const uint32_t cj = W.key[j] >> si;
+                const bool in_a = periodic_adjacent((int)compact10(cj >> 2), hx, nci) && periodic_adjacent((int)compact10(cj >> 1), hy, nci) &&
+                                  periodic_adjacent((int)compact10(cj), hz, nci);
+                acc = !in_a && (j != i) && accept_pair<true>(xi, yi, zi, X[j], Y[j], Z[j], hi, H[j], g.box, g.half);
+            }
+            int cntk;
+            const int pos = kept_rank(acc, cntk);
+            if (acc && n + pos < cap) ell[(size_t)(n + pos) * ld + r] = j;
+            n += cntk;
+        }
+    }
+    __syncthreads();  // the row's entries written by every lane are visible to all
+    if (n > nA && n <= cap) {
+        // ascending order: each entry's rank among the row's distinct entries
+        if (n <= 1024) {
+            for (int q = lane; q < n; q += 64) srt[q] = ell[(size_t)q * ld + r];
+            __syncthreads();
+            for (int q = lane; q < n; q += 64) {
+                const int e = srt[q];
+                int rank = 0;
+                for (int t = 0; t < n; t++) rank += srt[t] < e;
+                ell[(size_t)rank * ld + r] = e;
+            }
+        } else if (lane == 0) {
+            for (int q = 1; q < n; q++) {
+                const int e = ell[(size_t)q * ld + r];
+                int t = q - 1;
+                while (t >= 0 && ell[(size_t)t * ld + r] > e) { ell[(size_t)(t + 1) * ld + r] = ell[(size_t)t * ld + r]; t--; }
+                ell[(size_t)(t + 1) * ld + r] = e;
+            }
+        }
     }
     if (lane == 0) {
         cnt[r] = n;

@@ -71,6 +71,75 @@ def sedov_box(n_side, seed, *, gamma=5.0 / 3.0, pressure=1e-6, energy=1.0, n_dep
     return ic
 
 
+def _gaussian_field(ng, slope, rng, box=1.0):
+    """A real Gaussian random field on an ng^3 periodic grid with power spectrum P(k) ~ k^slope
+    (the k = 0 mode removed), unit variance; returned with the wave vectors for projections."""
+    k1 = np.fft.fftfreq(ng, d=box / ng) * 2.0 * np.pi
+    kz1 = np.fft.rfftfreq(ng, d=box / ng) * 2.0 * np.pi
+    kx, ky, kz = np.meshgrid(k1, k1, kz1, indexing="ij", sparse=True)
+    k2 = kx * kx + ky * ky + kz * kz
+    amp = np.zeros_like(k2)
+    nz = k2 > 0
+    amp[nz] = k2[nz] ** (0.25 * slope)  # sqrt(P(k)) = k^(slope/2)
+    shape = (ng, ng, ng // 2 + 1)
+    white = rng.normal(size=shape) + 1j * rng.normal(size=shape)
+    return white * amp, (kx, ky, kz, k2)
+
+
+def lognormal_box(n_particles, seed, *, ng=256, sigma_ln=2.0, slope=-3.0, mach=2.0, gamma=1.001, kernel=6,
+                  des_num_ngb=64.0, box=1.0):
+    """Particles sampled from a lognormal density field: delta is a Gaussian random field with
+    P(k) ~ k^slope on an ng^3 grid, normalised to std(delta) = sigma_ln, rho ~ exp(delta); the
+    n_particles (equal mass, total mass 1) are spread over the cells by a multinomial draw with
+    probabilities rho_c / sum(rho), uniformly inside each cell.  Isothermal: P = rho with
+    gamma = 1.001 (u = 1/(gamma-1)).  Velocities: a solenoidal Gaussian field (three fields of
+    the same spectrum, longitudinal part projected out) at the particle's cell, scaled to rms
+    Mach `mach` (3D, c_s = sqrt(gamma P/rho)).  The first radius guess encloses des_num_ngb
+    neighbours at the cell's density."""
+    rng = _rng(seed)
+    fk, (kx, ky, kz, k2) = _gaussian_field(ng, slope, rng, box)
+    delta = np.fft.irfftn(fk, s=(ng, ng, ng))
+    delta -= delta.mean()
+    delta *= sigma_ln / delta.std()
+    rho = np.exp(delta).ravel()
+    del delta, fk
+    prob = rho / rho.sum()
+    counts = rng.multinomial(n_particles, prob)
+    cells = np.repeat(np.arange(ng ** 3, dtype=np.int64), counts)
+    del prob
+    dx = box / ng
+    ci = np.stack(np.unravel_index(cells, (ng, ng, ng)), axis=1).astype(np.float64)
+    pos = (ci + rng.uniform(0.0, 1.0, size=(n_particles, 3))) * dx
+    pos %= box
+    del ci
+    # solenoidal velocity field
+    vk = [rng.normal(size=(ng, ng, ng // 2 + 1)) + 1j * rng.normal(size=(ng, ng, ng // 2 + 1)) for _ in range(3)]
+    amp = np.zeros_like(k2)
+    nz = k2 > 0
+    amp[nz] = k2[nz] ** (0.25 * slope)
+    vk = [v * amp for v in vk]
+    kd = np.where(nz, (kx * vk[0] + ky * vk[1] + kz * vk[2]) / np.where(nz, k2, 1.0), 0.0)
+    vk = [vk[0] - kx * kd, vk[1] - ky * kd, vk[2] - kz * kd]
+    del kd, amp
+    vel = np.empty((n_particles, 3))
+    for c in range(3):
+        vg = np.fft.irfftn(vk[c], s=(ng, ng, ng)).ravel()
+        vel[:, c] = vg[cells]
+        del vg
+    del vk
+    cs = np.sqrt(gamma)  # P = rho: c_s^2 = gamma P / rho
+    vel *= mach * cs / np.sqrt((vel * vel).sum(axis=1).mean())
+    mass = np.full(n_particles, 1.0 / n_particles)
+    u = np.full(n_particles, 1.0 / (gamma - 1.0))
+    nloc = rho[cells] / rho.mean() * n_particles / box ** 3
+    h = (des_num_ngb / (VOLNORM * nloc)) ** (1.0 / 3.0)
+    h = np.minimum(h, 0.2 * box)
+    params = dict(box=box, kernel=kernel, gamma=gamma, des_num_ngb=float(des_num_ngb),
+                  max_num_ngb_dev=max(des_num_ngb / 640.0, 0.05),
+                  min_kernel_radius=0.0, max_kernel_radius=0.25 * box, time=0.0, time_begin=0.0)
+    return dict(pos=np.ascontiguousarray(pos), vel=np.ascontiguousarray(vel), mass=mass, u=u, h=h, params=params)
+
+
 def workload(name, seed=None):
     """The named single-box workloads: 'c1' (32^3 cubic-spline lattice) and 'c2' (128^3 Sedov)."""
     if name == "c1":

@@ -49,7 +49,9 @@ int gz_batch_add(gz_batch *b, long n, const gz_fields *host, const gz_box_params
 int gz_batch_begin(gz_batch *b);
 /* One sync point of every started box (each at its own time): gz_run_step for all of them. */
 int gz_batch_step(gz_batch *b);
-/* Leave the batch; the box's particles are freed at the next step. */
+/* Leave the batch; the box's particles are retired and the other boxes packed together (the box
+   id may then be given to a new box).  A box whose timeline reached its end (gz_timeline's
+   stop_code set) no longer steps and waits for this call. */
 int gz_batch_remove(gz_batch *b, int box_id);
 /* Device -> host for one box, in the order its particles were given to gz_batch_add. */
 int gz_batch_store(gz_batch *b, int box_id, gz_fields *host);
@@ -59,7 +61,8 @@ long gz_batch_particles(const gz_batch *b);
 int gz_batch_boxes(const gz_batch *b);
 int gz_batch_synchronize(gz_batch *b);
 /* Totals since the last gz_batch_timing_reset: "active_updates" (active particles through the
-   hot path), "steps", "begins", "resorts", "force_pairs_total". */
+   hot path), "steps", "begins", "resorts", "force_pairs_total"; and "device_bytes", the device
+   memory the batch holds now. */
 int gz_batch_stat(gz_batch *b, const char *name, double *value);
 int gz_batch_timing_reset(gz_batch *b);
 

@@ -179,7 +179,10 @@ int gz_timeline_get(gz_context *ctx, gz_timeline *out);
    this fraction of the search radius since they were last ordered (the search grid widens its
    cells by twice that distance meanwhile); "few_rows" (default 8192) -- passes over at most this
    many active particles run one wavefront per particle (64 lanes share its neighbours) instead
-   of one lane per particle; both give the same sums. */
+   of one lane per particle; both give the same sums; "wide_search" (default 0) -- 1: every
+   particle searches at the grid level of its own kernel radius (cell tables of every level),
+   for radii that span orders of magnitude, where one grid sized by the largest radius would
+   walk far too many candidates; same neighbour sets, same sums. */
 int gz_set_option(gz_context *ctx, const char *name, double value);
 
 /* Neighbour lists of the resident state, in the caller's indexing, for parity tests.
