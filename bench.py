@@ -234,6 +234,74 @@ def ensemble_bench(steps=100, n_boxes=256, seed=1004):
             "batched_over_looped": round((upd_b / t_b) / (upd_l / t_l), 3)}
 
 
+LOGNORMAL_QUANTITIES = {"h": "h", "rho": "rho", "grad_rho": "grad_rho", "grad_p": "grad_p", "grad_v": "grad_v",
+                        "dv_dt": "accel", "du_dt": "dtu"}
+
+
+def lognormal_bench(n_particles=4 * 1024 * 1024, ng=256, reps=2):
+    """The FP32-mixed form against the plain FP64 form on identical inputs: particles sampled
+    from a lognormal density field (sigma(ln rho) = 2, P(k) ~ k^-3 on an ng^3 grid), isothermal
+    (gamma 1.001, P = rho), solenoidal velocities at rms Mach 2; one hot-path pass per form
+    after the radii converged, with the search over many grid levels.  Reports per-quantity
+    errors of the mixed form against FP64 (max and median over particles, each relative to the
+    quantity's largest magnitude, and the median per-particle relative error) and both rates."""
+    from gizmo_amd import ics
+    from gizmo_amd.hydro import GZ_FP32_MIXED, GZ_FP64, Hydro
+    t0 = time.perf_counter()
+    ic = ics.lognormal_box(n_particles, 1003, ng=ng)
+    log(f"lognormal: {n_particles} particles built in {time.perf_counter() - t0:.1f} s")
+    par = dict(ic["params"])
+    # the radii converged once (FP64), then both forms start from them
+    g = Hydro(par, n_particles, GZ_FP64)
+    g.set_option("wide_search", 1)
+    g.load(dict(pos=ic["pos"], vel=ic["vel"], mass=ic["mass"], u=ic["u"], h=ic["h"]))
+    t0 = time.perf_counter()
+    g.density()
+    g.synchronize()
+    log(f"lognormal: first density {time.perf_counter() - t0:.2f} s, {g.stat('density_iterations'):.0f} rounds")
+    h0 = g.store(["h"])["h"]
+    g.close()
+    fields = dict(pos=ic["pos"], vel=ic["vel"], mass=ic["mass"], u=ic["u"], h=h0)
+    out, rate = {}, {}
+    names = sorted(set(LOGNORMAL_QUANTITIES.values()))
+    for form, prec in (("fp64", GZ_FP64), ("mixed", GZ_FP32_MIXED)):
+        g = Hydro(par, n_particles, prec)
+        g.set_option("wide_search", 1)
+        g.load(fields)
+        g.hydro_pass()  # untimed: allocations and tables
+        g.synchronize()
+        # timed passes from the same inputs (their upload stays outside)
+        dts = []
+        for _ in range(reps):
+            g.load(fields)
+            g.synchronize()
+            t1 = time.perf_counter()
+            g.hydro_pass()
+            g.synchronize()
+            dts.append(time.perf_counter() - t1)
+        dt = min(dts)
+        rate[form] = dict(ms=round(1e3 * dt, 2), updates_per_s=round(n_particles / dt, 1),
+                          density_iterations=g.stat("density_iterations"))
+        out[form] = g.store(names)
+        g.close()
+        log(f"lognormal {form}: {1e3 * dt:.1f} ms per pass, {n_particles / dt / 1e6:.1f} M updates/s")
+    errs = {}
+    for q, k in LOGNORMAL_QUANTITIES.items():
+        a = np.asarray(out["mixed"][k], dtype=np.float64).reshape(n_particles, -1)
+        b = np.asarray(out["fp64"][k], dtype=np.float64).reshape(n_particles, -1)
+        scale = np.maximum(np.abs(b).max(axis=0), 1e-300)
+        e_scale = (np.abs(a - b) / scale).max(axis=1)
+        mag = np.maximum(np.sqrt((b * b).sum(axis=1)), 1e-300)
+        e_part = np.sqrt(((a - b) ** 2).sum(axis=1)) / mag
+        errs[q] = {"max_rel_to_scale": float(e_scale.max()), "median_rel_to_scale": float(np.median(e_scale)),
+                   "median_rel_per_particle": float(np.median(e_part)),
+                   "p99_rel_per_particle": float(np.percentile(e_part, 99))}
+    hq = out["fp64"]["h"]
+    return {"particles": n_particles, "grid": ng, "h_min": float(hq.min()), "h_max": float(hq.max()),
+            "h_span": float(hq.max() / hq.min()), "fp64": rate["fp64"], "mixed": rate["mixed"],
+            "mixed_over_fp64": round(rate["fp64"]["ms"] / rate["mixed"]["ms"], 3), "errors_mixed_vs_fp64": errs}
+
+
 def cpu_baseline(ic, steps, target_s=20.0):
     """The oracle (C + OpenMP, FP64) on the same job: run_begin, then sync points until the
     step budget or the time budget is spent; the rate counts the timed steps only."""
@@ -268,8 +336,9 @@ def main():
     ap.add_argument("--only", choices=["both", "fp64", "mixed"], default="both")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving-loop latency runs")
     ap.add_argument("--serve-jobs", type=int, default=48)
-    ap.add_argument("--workload", choices=["sedov", "ensemble"], default="sedov",
-                    help="sedov: the 2M Sedov job (+ serving); ensemble: 256 boxes batched vs looped")
+    ap.add_argument("--workload", choices=["sedov", "ensemble", "lognormal"], default="sedov",
+                    help="sedov: the 2M Sedov job (+ serving); ensemble: 256 boxes batched vs looped; "
+                         "lognormal: 4M lognormal field, FP32-mixed against FP64")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print(f"bench.py runs on one GPU: this build has no multi-GPU path (asked for {a.gpus}); "
@@ -290,6 +359,18 @@ def main():
             "data": "synthetic: seeded jittered lattices with Gaussian velocities (numpy Philox)",
             "config": {"workload": "ensemble_256_boxes_N_loguniform_4096_262144_mach_0.1_5_100_steps"},
             "ensemble": e}), flush=True)
+        return
+
+    if a.workload == "lognormal":
+        e = lognormal_bench()
+        print(json.dumps({
+            "metric": "hydro active-particle updates/s, 4M lognormal field (FP32-mixed vs FP64, per-quantity error)",
+            "value": e["mixed"]["updates_per_s"], "unit": "active-particle updates/s", "n_gpus": 1, "steps": 1,
+            "warmup": 1, "ms_per_step": e["mixed"]["ms"], "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "fp32-mixed",
+            "data": "synthetic: seeded lognormal density field, solenoidal Mach-2 velocities (numpy Philox)",
+            "config": {"workload": "lognormal_4M_sigma2_k-3_256grid_isothermal_mach2_one_pass"},
+            "lognormal": e}), flush=True)
         return
 
     t0 = time.perf_counter()

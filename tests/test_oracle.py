@@ -55,6 +55,23 @@ def test_kernel_bitwise_vs_reference(oracle_lib, kern):
             assert w0 == w1.value and d0 == d1.value, (kern, r, h)
 
 
+GOLDEN = os.path.join(HERE, "golden", "ref_kernel_mat3.npz")
+
+
+def test_oracle_against_committed_reference_vectors(oracle_lib):
+    """The vectors tests/golden/make_ref_vectors.py wrote from the reference's compiled kernel.h
+    and mat3.h: the oracle reproduces every one bit for bit (runs without the reference tree)."""
+    g = np.load(GOLDEN)
+    for kern in (3, 6):
+        for h, r, w, dw in zip(g[f"k{kern}_h"], g[f"k{kern}_r"], g[f"k{kern}_w"], g[f"k{kern}_dw"]):
+            assert oracle_lib.kernel(kern, float(r), float(h)) == (w, dw), (kern, r, h)
+    for T, Ti, (f1, f2) in zip(g["mat_T"], g["mat_Tinv"], g["mat_fro"]):
+        cn, inv = oracle_lib.matrix_invert(T.reshape(3, 3))
+        assert np.array_equal(inv.reshape(9), Ti)
+        if Ti.any():
+            assert cn == max(math.sqrt(f1 * f2) / 3, 1)
+
+
 @pytest.mark.parametrize("kern", [3, 6])
 def test_kernel_normalisation_and_edges(oracle_lib, kern):
     h = 0.7

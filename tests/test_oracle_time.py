@@ -1,0 +1,161 @@
+"""Pins of the oracle's timestep loop (oracle/gz_oracle_time.c) by what the mathematics fixes:
+
+  * kick-drift-kick is exact for a constant acceleration: x(T) = x0 + v0 T + a T^2 / 2 and
+    v(T) = v0 + a T after any number of steps of any length;
+  * the drift's predicted density, radius and energy follow their closed forms
+    (rho0 exp(-divv dt), h0 exp(divv dt / 3), u0 + du/dt dt), compose over split intervals,
+    cap the log change at 0.3 and keep P = (gamma - 1) u rho;
+  * the kick's positivity catch halves the energy instead of letting it fall below half;
+  * get_timestep: the maximum step binds exactly; the acceleration criterion scales as
+    |a|^(-1/2) and the Courant criterion as the kernel radius (bins shift by one);
+  * the run loop with one forced bin advances every particle every step by the same power of
+    two, and conserves total momentum to roundoff (antisymmetric pair forces, equal kicks).
+"""
+import math
+
+import numpy as np
+import pytest
+
+from gizmo_amd import ics
+
+TBI = 1.0 / (1 << 40)
+
+
+def one_particle(O, pos=(10.0, 20.0, 30.0), vel=(0.3, -0.2, 0.1), acc=(0.5, 0.25, -1.0), mass=2.0):
+    s = O.State(1)
+    s.pos[0] = pos
+    s.vel[0] = vel
+    s.vel_kick[0] = vel
+    s.accel[0] = acc
+    s.mass[0] = mass
+    s.u[0] = s.u_kick[0] = 1.0
+    s.rho[0] = 1.0
+    s.h[0] = 0.5
+    s.numngb[0] = 4.0
+    s.maxsig[0] = 1.0
+    s.soft[0] = 0.5
+    return s
+
+
+def runner(O, s, box=1e6, **integ):
+    par = dict(box=box, kernel=6, gamma=5.0 / 3.0, des_num_ngb=64.0, max_num_ngb_dev=0.1, min_kernel_radius=0.0,
+               max_kernel_radius=0.25 * box, time=0.0, time_begin=0.0)
+    ip = dict(courant_fac=0.2, err_tol_int_accuracy=0.02, max_size_timestep=0.01, min_size_timestep=0.0,
+              time_begin=0.0, time_max=1.0, min_egy_spec=0.0)
+    ip.update(integ)
+    r = O.Run(par, ip, s)
+    r.tl.timebase_interval = TBI
+    for k in range(60):
+        r.tl.bin_active[k] = 1
+    return r
+
+
+def test_kdk_exact_for_constant_acceleration(oracle_lib):
+    O = oracle_lib
+    s = one_particle(O)
+    r = runner(O, s)
+    x0, v0, a = s.pos[0].copy(), s.vel_kick[0].copy(), s.accel[0].copy()
+    t = 0
+    for dt in (1 << 20, 1 << 22, 1 << 18, 1 << 22, 1 << 21):
+        half = dt // 2
+        r.kick(0, t, t + half, 0)
+        r.drift(0, t, t + dt)
+        r.kick(0, t + half, t + dt, 1)
+        t += dt
+    T = t * TBI
+    assert np.allclose(s.pos[0], x0 + v0 * T + 0.5 * a * T * T, rtol=0, atol=1e-12 * np.abs(x0).max())
+    assert np.allclose(s.vel_kick[0], v0 + a * T, rtol=1e-14, atol=1e-16)
+    assert np.array_equal(s.vel[0], s.vel_kick[0])  # mode 1: predicted = kicked
+
+
+@pytest.mark.parametrize("divv", [0.7, -1.3])
+def test_drift_closed_forms_and_composition(oracle_lib, divv):
+    O = oracle_lib
+    out = []
+    for split in (False, True):
+        s = one_particle(O)
+        s.divvel[0] = divv
+        s.dtu[0] = -0.25
+        r = runner(O, s)
+        T = 1 << 36  # dt = 1/16: |divv dt| < 0.3
+        if split:
+            r.drift(0, 0, T // 2)
+            r.drift(0, T // 2, T)
+        else:
+            r.drift(0, 0, T)
+        out.append(s)
+    a, b = out
+    dt = (1 << 36) * TBI
+    assert a.rho[0] == pytest.approx(math.exp(-divv * dt), rel=1e-14)
+    assert a.h[0] == pytest.approx(0.5 * math.exp(divv * dt / 3), rel=1e-14)
+    assert a.u[0] == pytest.approx(1.0 - 0.25 * dt, rel=1e-14)
+    assert a.press[0] == pytest.approx((5.0 / 3.0 - 1) * a.u[0] * a.rho[0], rel=1e-15)
+    for k in ("rho", "h", "u", "press"):
+        assert getattr(a, k)[0] == pytest.approx(getattr(b, k)[0], rel=1e-14), k
+    assert np.allclose(a.pos, b.pos, rtol=1e-15)
+
+
+def test_drift_log_change_capped(oracle_lib):
+    O = oracle_lib
+    s = one_particle(O)
+    s.divvel[0] = 100.0
+    r = runner(O, s)
+    r.drift(0, 0, 1 << 36)
+    assert s.rho[0] == math.exp(-0.3)
+    assert s.h[0] == 0.5 * math.exp(0.3 / 3)
+
+
+def test_kick_energy_positivity(oracle_lib):
+    O = oracle_lib
+    s = one_particle(O)
+    s.dtu[0] = -1e6
+    r = runner(O, s)
+    r.kick(0, 0, 1 << 30, 1)
+    assert s.u_kick[0] == 0.5 and s.u[0] == 0.5
+
+
+def test_get_timestep_limits_and_scalings(oracle_lib):
+    O = oracle_lib
+    # the maximum step binds: a tiny acceleration, a slow signal, no divergence
+    s = one_particle(O, acc=(1e-12, 0, 0))
+    s.maxsig[0] = 1e-9
+    r = runner(O, s, max_size_timestep=0.01)
+    assert r.get_timestep(0) == int(0.01 / TBI)
+    # acceleration criterion alone: sqrt(2 eta core h / |a|) scales as |a|^(-1/2)
+    s = one_particle(O, acc=(1.0, 0, 0))
+    s.maxsig[0] = 1e-9
+    r = runner(O, s, max_size_timestep=1e3, err_tol_int_accuracy=0.02)
+    t1 = r.get_timestep(0)
+    s.accel[0] = (4.0, 0, 0)
+    t4 = r.get_timestep(0)
+    assert abs(t1 - 2 * t4) <= 2
+    assert t1 * TBI == pytest.approx(math.sqrt(2 * 0.02 * 0.4 * 0.5 / 1.0), rel=1e-9)
+    # Courant criterion alone scales as the particle size (kernel radius)
+    s = one_particle(O, acc=(1e-12, 0, 0))
+    r = runner(O, s, max_size_timestep=1e3, err_tol_int_accuracy=1e30)
+    c1 = r.get_timestep(0)
+    s.h[0] *= 2
+    c2 = r.get_timestep(0)
+    assert abs(c2 - 2 * c1) <= 2
+
+
+def test_run_loop_single_bin_conserves_momentum(oracle_lib):
+    O = oracle_lib
+    ic = ics.lattice_box(8, 4001, kernel=3, des_num_ngb=32.0, mach_sigma=0.3)
+    par = dict(ic["params"])
+    st = O.State.from_ic(ic)
+    # the forced single bin: MaxSizeTimestep = MinSizeTimestep below every Courant step
+    integ = dict(courant_fac=0.2, err_tol_int_accuracy=0.02, max_size_timestep=1e-4, min_size_timestep=1e-4,
+                 time_begin=0.0, time_max=1.0, min_egy_spec=0.0)
+    r = O.Run(par, integ, st)
+    r.begin()
+    p0 = (st.mass[:, None] * st.vel_kick).sum(axis=0)
+    scale = (st.mass[:, None] * np.abs(st.vel_kick)).sum()
+    tis = []
+    for _ in range(6):
+        assert r.step() == st.n
+        tis.append(r.ti_current)
+    steps = np.diff([0] + tis)
+    assert len(set(steps)) == 1 and steps[0] & (steps[0] - 1) == 0
+    p1 = (st.mass[:, None] * st.vel_kick).sum(axis=0)
+    assert np.abs(p1 - p0).max() <= 1e-13 * scale
