@@ -849,12 +849,14 @@ int force_stage(gz_context *ctx)
             k_force_rows<KERN, R><<<xcd_grid(ctx->nact), 64, 0, ctx->stream>>>(
                 ctx->dp, gam<R>(ctx->dp), ctx->nact, ctx->act, F, rec, ctx->ell_s, ctx->ld_s, ctx->cnt_s,
                 ctx->all_active ? nullptr : ctx->ACT, ctx->TB, ctx->MSV, ctx->WAKE, ctx->ACC, ctx->DTU, ctx->MSV, ctx->counters + 2,
-                ctx->counters + 4, ctx->u64c + 65, BoxTable{});
+                ctx->counters + 4, nullptr, BoxTable{});
         else
             k_force<KERN, R><<<xcd_grid(nblk(ctx->nact)), kBlock, 0, ctx->stream>>>(
                 ctx->dp, gam<R>(ctx->dp), ctx->nact, ctx->act, F, rec, ctx->ell_s, ctx->ld_s, ctx->cnt_s,
                 ctx->all_active ? nullptr : ctx->ACT, ctx->TB, ctx->MSV, ctx->WAKE, ctx->ACC, ctx->DTU, ctx->MSV, ctx->counters + 2,
                 ctx->counters + 4, ctx->u64c + 65);
+        if (ctx->nact <= ctx->few_rows)
+            k_sum_counts<<<std::min(nblk(ctx->nact), 1024u), kBlock, 0, ctx->stream>>>(ctx->nact, ctx->cnt_s, ctx->u64c + 65);
         HIPCHK(hipGetLastError());
     }
     CHK(read_counters(ctx));

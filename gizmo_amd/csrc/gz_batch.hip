@@ -442,7 +442,7 @@ int build_grid(gz_batch *b)
     }
     b->cells_total = off;
     k_cell_clear<<<nb(off), kB, 0, b->stream>>>(off, b->cstart, b->cend, b->chmax);
-    k_batch_cell_bounds<<<nb(b->cap), kB, 0, b->stream>>>(b->cap, b->key, b->BOX, b->tab, b->cstart, b->cend);
+    k_batch_cell_bounds<<<nb(b->cap), kB, 0, b->stream>>>(b->cap, b->key, b->BOX, b->tab, b->retired, b->cstart, b->cend);
     BHIP(hipGetLastError());
     b->grid_valid = true;
     return 0;
@@ -550,8 +550,9 @@ int gradients_force(gz_batch *b)
     G.G5 = (R)b->dp.G5; G.G6 = (R)b->dp.G6; G.G7 = (R)b->dp.G7; G.G8 = (R)b->dp.G8; G.G9 = (R)b->dp.G9;
     k_pack_force<R><<<nb(b->cap), kB, 0, b->stream>>>(b->par.gamma, F, b->rec_need, rf, b->counters + 2);
     k_force_rows<KERN, R><<<xcd_grid(na), 64, 0, b->stream>>>(b->dp, G, na, b->act, F, rf, b->ell, ld, b->cnt, b->ACT, b->TB, b->MSV,
-                                                               b->WAKE, b->ACC, b->DTU, b->MSV, b->counters + 2, b->counters + 4, b->u64,
+                                                               b->WAKE, b->ACC, b->DTU, b->MSV, b->counters + 2, b->counters + 4, nullptr,
                                                                bt);
+    k_sum_counts<<<std::min(nb(na), 1024u), kB, 0, b->stream>>>(na, b->cnt, b->u64);
     BHIP(hipGetLastError());
     BCHK(read_back(b, false));
     if (b->hcounters[2]) return fail(b, GZ_STOP_RIEMANN, "hydro_force(): Riemann solver failed to find a positive pressure");

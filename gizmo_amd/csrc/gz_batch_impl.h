@@ -29,13 +29,16 @@ __global__ void __launch_bounds__(256) k_batch_keys(long n, const double *__rest
     val[i] = (int)i;
 }
 
-// [start, end) of every cell of every box (cell index = the box's first cell + code)
+// [start, end) of every cell of every live box (cell index = the box's first cell + code); the
+// retired particles stay out of the cells (no pass visits them)
 __global__ void __launch_bounds__(256) k_batch_cell_bounds(long n, const uint32_t *__restrict__ key, const int *__restrict__ BOX,
-                                                            const BoxDev *__restrict__ tab, int *__restrict__ cstart, int *__restrict__ cend)
+                                                            const BoxDev *__restrict__ tab, int retired, int *__restrict__ cstart,
+                                                            int *__restrict__ cend)
 {
     const long s = blockIdx.x * (long)blockDim.x + threadIdx.x;
     if (s >= n) return;
     const int b = BOX[s];
+    if (b == retired) return;
     const int shift = tab[b].shift;
     const uint32_t c = key[s] >> shift;
     const long long cell = tab[b].cell_off + c;

@@ -156,6 +156,7 @@ __global__ void k_advance_boxes(BoxTable bt);
 __global__ void k_active_from_bins(long n, unsigned long long mask, const int *TB, const double *M, const double *H,
                                    uint8_t *ACT, double *SOFT);
 __global__ void k_mark_records(int nrows, const int *act, const int *ell, long ld, const int *cnt, uint8_t *need);
+__global__ void k_sum_counts(int nrows, const int *cnt, unsigned long long *out);
 
 // search over many grid levels
 __global__ void k_wide_bounds(WideGrid W);
@@ -167,7 +168,7 @@ __global__ void k_build_sym_wide(GridView g, int nrows, const int *act, const do
 // batches of independent boxes
 __global__ void k_batch_keys(long n, const double *X, const double *Y, const double *Z, const int *BOX, const BoxDev *tab,
                              const int *rank, const uint8_t *due, uint32_t *key, unsigned long long *key64, int *val);
-__global__ void k_batch_cell_bounds(long n, const uint32_t *key, const int *BOX, const BoxDev *tab, int *cstart, int *cend);
+__global__ void k_batch_cell_bounds(long n, const uint32_t *key, const int *BOX, const BoxDev *tab, int retired, int *cstart, int *cend);
 __global__ void k_batch_flag_boxes(long n, const int *BOX, const uint8_t *sel, const double *M, int *flag, uint8_t *ACT);
 __global__ void k_batch_flag_active(long n, const int *BOX, const BoxDev *tab, const uint8_t *ACT, const double *M, int *flag);
 __global__ void k_batch_box_hmax(long n, const double *H, const uint8_t *sel, BoxTable bt);

@@ -661,6 +661,45 @@ __global__ void __launch_bounds__(256) k_density_init(int na, const int *__restr
 }
 #endif
 
+// the maximum of v over the wave into slot (one atomic when the wave holds one box; the bits of
+// a positive double order like the double)
+__device__ __forceinline__ void wave_max_to(double v, bool valid, int box, unsigned long long *single, const BoxTable &bt, int slot)
+{
+    if (!bt.b) {
+        for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+        if ((threadIdx.x & 63) == 0 && v > 0) atomicMax(single, (unsigned long long)__double_as_longlong(v));
+        return;
+    }
+    const unsigned long long vm = __ballot(valid);
+    if (!vm) return;
+    const int b0 = __shfl(box, __ffsll((long long)vm) - 1, 64);
+    if (__all(!valid || box == b0)) {
+        double w = valid ? v : 0.0;
+        for (int o = 32; o > 0; o >>= 1) w = fmax(w, __shfl_xor(w, o, 64));
+        if ((threadIdx.x & 63) == 0 && w > 0)
+            atomicMax(&bt.slot[(size_t)b0 * kBoxSlots + slot], (unsigned long long)__double_as_longlong(w));
+    } else if (valid && v > 0) {
+        atomicMax(&bt.slot[(size_t)box * kBoxSlots + slot], (unsigned long long)__double_as_longlong(v));
+    }
+}
+
+// the sum of the list lengths (the pairs a force pass evaluates): one atomic per block
+#ifdef GZ_DEFINE_COMMON_KERNELS
+__global__ void __launch_bounds__(256) k_sum_counts(int nrows, const int *__restrict__ cnt, unsigned long long *__restrict__ out)
+{
+    __shared__ unsigned long long part[4];
+    unsigned long long s = 0;
+    for (long r = blockIdx.x * (long)blockDim.x + threadIdx.x; r < nrows; r += (long)gridDim.x * blockDim.x) s += (unsigned long long)cnt[r];
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        s = part[0] + part[1] + part[2] + part[3];
+        if (s) atomicAdd(out, s);
+    }
+}
+#endif
+
 // after_iter_body (src/hydro/density_loop.h:516-961) on the sums of the round and, on
 // convergence, density_finalize_post_runner (src/hydro/density_loop.cc:303-733)
 #ifdef GZ_DEFINE_COMMON_KERNELS
@@ -670,41 +709,49 @@ __global__ void __launch_bounds__(256) k_density_after(DevParams P, int nrows, c
                                                        int *__restrict__ nnext, int *__restrict__ rebuild, int *__restrict__ err,
                                                        unsigned long long *__restrict__ hfinal, BoxTable bt)
 {
+    // every lane reaches the wave-wide counts and maxima at the end
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= nrows || !need[r]) return;
-    const int i = act[r];
-    if (bt.b) {
-        GridView g{};
-        resolve_box(bt, i, P, g);
-        cell = bt.b[bt.BOX[i]].rowlim;
-    }
-    double acc[13];
-    for (int q = 0; q < 13; q++) acc[q] = accin[(size_t)q * nrows + r];
-    const double h = it.hcur[r];
-    IterState sc;
-    sc.left = it.left[r]; sc.right = it.right[r]; sc.cn_cur = it.cn_cur[r]; sc.cn_init = it.cn_init[r];
-    const int iter = it.iter[r];
-    double nh = h;
-    const int rc = density_after_iter(P, out.cn[i], iter, h, acc, sc, nh);
-    it.iter[r] = iter + 1;
-    if (rc <= 0 || iter + 1 >= kMaxIter) {
-        if (rc < 0) atomicMax(err, 90001009);
-        else if (rc > 0) atomicMax(err, 1155);
-        density_finalize(P, out, i, h, acc, M[i], U[i]);
-        need[r] = 0;
-        if (h > 0) {
-            if (bt.b) atomicMax(&bt.slot[(size_t)bt.BOX[i] * kBoxSlots + kSlotHfinal], (unsigned long long)__double_as_longlong(h));
-            else atomicMax(hfinal, (unsigned long long)__double_as_longlong(h));
+    const bool row = r < nrows && need[r];
+    double h_final = 0, h_grow = 0;
+    bool cont = false;
+    int box = 0;
+    if (row) {
+        const int i = act[r];
+        if (bt.b) {
+            GridView g{};
+            resolve_box(bt, i, P, g);
+            box = bt.BOX[i];
+            cell = bt.b[box].rowlim;
         }
-        return;
+        double acc[13];
+        for (int q = 0; q < 13; q++) acc[q] = accin[(size_t)q * nrows + r];
+        const double h = it.hcur[r];
+        IterState sc;
+        sc.left = it.left[r]; sc.right = it.right[r]; sc.cn_cur = it.cn_cur[r]; sc.cn_init = it.cn_init[r];
+        const int iter = it.iter[r];
+        double nh = h;
+        const int rc = density_after_iter(P, out.cn[i], iter, h, acc, sc, nh);
+        it.iter[r] = iter + 1;
+        if (rc <= 0 || iter + 1 >= kMaxIter) {
+            if (rc < 0) atomicMax(err, 90001009);
+            else if (rc > 0) atomicMax(err, 1155);
+            density_finalize(P, out, i, h, acc, M[i], U[i]);
+            need[r] = 0;
+            if (h > 0) h_final = h;
+        } else {
+            it.left[r] = sc.left; it.right[r] = sc.right; it.cn_cur[r] = sc.cn_cur; it.cn_init[r] = sc.cn_init;
+            it.hcur[r] = nh;
+            if (nh * (1.0 + 1e-6) > cell) h_grow = nh;
+            cont = true;
+        }
     }
-    it.left[r] = sc.left; it.right[r] = sc.right; it.cn_cur[r] = sc.cn_cur; it.cn_init[r] = sc.cn_init;
-    it.hcur[r] = nh;
-    if (nh * (1.0 + 1e-6) > cell) {
-        atomicOr(rebuild, 1);
-        if (bt.b) atomicMax(&bt.slot[(size_t)bt.BOX[i] * kBoxSlots + kSlotHgrow], (unsigned long long)__double_as_longlong(nh));
+    const unsigned long long cm = __ballot(cont), gm = __ballot(h_grow > 0);
+    if ((threadIdx.x & 63) == 0) {
+        if (cm) atomicAdd(nnext, (int)__popcll(cm));
+        if (gm) atomicOr(rebuild, 1);
     }
-    atomicAdd(nnext, 1);
+    wave_max_to(h_final, row, box, hfinal, bt, kSlotHfinal);
+    if (bt.b && gm) wave_max_to(h_grow, row, box, nullptr, bt, kSlotHgrow);
 }
 #endif
 

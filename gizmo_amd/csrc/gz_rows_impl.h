@@ -325,11 +325,17 @@ __global__ void __launch_bounds__(256) k_wide_bounds(WideGrid W)
 __global__ void __launch_bounds__(256) k_particle_level(int n, const double *__restrict__ H, double box, double margin, int lmax,
                                                          int *__restrict__ plev, int *__restrict__ level_count)
 {
+    __shared__ int hist[kFineLevel + 1];
+    if (threadIdx.x <= kFineLevel) hist[threadIdx.x] = 0;
+    __syncthreads();
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n) return;
-    const int L = wide_level(box, lmax, H[p] + 2.0 * margin);
-    plev[p] = L;
-    atomicAdd(&level_count[L], 1);
+    if (p < n) {
+        const int L = wide_level(box, lmax, H[p] + 2.0 * margin);
+        plev[p] = L;
+        atomicAdd(&hist[L], 1);
+    }
+    __syncthreads();
+    if (threadIdx.x <= kFineLevel && hist[threadIdx.x]) atomicAdd(&level_count[threadIdx.x], hist[threadIdx.x]);
 }
 
 // the range of every cell of level L among the particles of level L (positions in lidx)
@@ -572,7 +578,7 @@ __global__ void __launch_bounds__(64) k_force_rows(DevParams P, Gam<R> G, int nr
     const bool enabled = (mi > 0 && rhoi > 0);
     const int c = cnt[r];
     if (lane == 0) {
-        if (c) atomicAdd(pairs, (unsigned long long)c);
+        if (c && pairs) atomicAdd(pairs, (unsigned long long)c);
         if (wakeup) wakeup[i] = 0;  // hydro_force_initial_operations_preloop
     }
     if (enabled) {

@@ -108,28 +108,6 @@ __device__ void kick(const TimeParams &T, const ParticleView &S, long i, long lo
     }
 }
 
-// the maximum of v over the wave into slot (one atomic when the wave holds one box; the bits of
-// a positive double order like the double)
-__device__ __forceinline__ void wave_max_to(double v, bool valid, int box, unsigned long long *single, const BoxTable &bt, int slot)
-{
-    if (!bt.b) {
-        for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
-        if ((threadIdx.x & 63) == 0 && v > 0) atomicMax(single, (unsigned long long)__double_as_longlong(v));
-        return;
-    }
-    const unsigned long long vm = __ballot(valid);
-    if (!vm) return;
-    const int b0 = __shfl(box, __ffsll((long long)vm) - 1, 64);
-    if (__all(!valid || box == b0)) {
-        double w = valid ? v : 0.0;
-        for (int o = 32; o > 0; o >>= 1) w = fmax(w, __shfl_xor(w, o, 64));
-        if ((threadIdx.x & 63) == 0 && w > 0)
-            atomicMax(&bt.slot[(size_t)b0 * kBoxSlots + slot], (unsigned long long)__double_as_longlong(w));
-    } else if (valid && v > 0) {
-        atomicMax(&bt.slot[(size_t)box * kBoxSlots + slot], (unsigned long long)__double_as_longlong(v));
-    }
-}
-
 // find_timesteps, src/core/timestep.cc:39-355: the new power-of-two step of each particle of
 // the active list; a larger bin only where it is synchronised now
 __global__ void __launch_bounds__(256) k_find_timesteps(TimeParams T, ParticleView S, int nrows, const int *__restrict__ act,
@@ -164,26 +142,57 @@ __global__ void __launch_bounds__(256) k_find_timesteps(TimeParams T, ParticleVi
     S.DTS[i] = ti_step;
 }
 
-// TimeBinCount of every box (count[64*box + bin]): lanes of the same box and bin are counted
-// together (most particles of a wave share both), one atomic per group
+// TimeBinCount of every box (count[64*box + bin]).  Each block counts a contiguous range of
+// particles into an LDS histogram of the box it is in (particles of one box are contiguous, so
+// the box changes rarely inside a range; the histogram is added to the global counts whenever
+// it does, and at the end); lanes of the same bin are added together (most particles of a wave
+// share one bin).  Particles of another box than the block's current one (a range that starts
+// inside one box and reaches into the next) are added to the global counts directly.
 __global__ void __launch_bounds__(256) k_bin_hist(long n, const int *__restrict__ TB, unsigned long long *__restrict__ count, BoxTable bt)
 {
+    __shared__ unsigned int h[64];
+    __shared__ int cur;
     const int lane = threadIdx.x & 63;
-    const long stride = (long)gridDim.x * blockDim.x;
-    const long base0 = blockIdx.x * (long)blockDim.x + (threadIdx.x & ~63);
-    for (long base = base0; base < n; base += stride) {
-        const long i = base + lane;
-        int b = (i < n) ? ((bt.b ? bt.BOX[i] * 64 : 0) + (TB[i] & 63)) : -1;
+    const long span = (n + gridDim.x - 1) / gridDim.x;
+    const long lo = blockIdx.x * span;
+    const long hi = lo + span < n ? lo + span : n;
+    if (threadIdx.x < 64) h[threadIdx.x] = 0;
+    if (threadIdx.x == 0) cur = (bt.b && lo < hi) ? bt.BOX[lo] : 0;
+    __syncthreads();
+    for (long base = lo; base < hi; base += blockDim.x) {
+        const int cb = bt.b ? bt.BOX[base] : 0;  // the same for the whole block
+        if (cb != cur) {
+            __syncthreads();
+            if (threadIdx.x < 64) {
+                if (h[threadIdx.x]) atomicAdd(&count[(size_t)64 * cur + threadIdx.x], (unsigned long long)h[threadIdx.x]);
+                h[threadIdx.x] = 0;
+            }
+            __syncthreads();
+            if (threadIdx.x == 0) cur = cb;
+            __syncthreads();
+        }
+        const long i = base + threadIdx.x;
+        int b = -1;
+        if (i < hi) {
+            const int box = bt.b ? bt.BOX[i] : 0;
+            b = TB[i] & 63;
+            if (box != cb) {
+                atomicAdd(&count[(size_t)64 * box + b], 1ull);
+                b = -1;
+            }
+        }
         while (true) {
             const unsigned long long live = __ballot(b >= 0);
             if (!live) break;
             const int leader = __ffsll((long long)live) - 1;
             const int lb = __shfl(b, leader, 64);
             const unsigned long long same = __ballot(b == lb);
-            if (lane == leader) atomicAdd(&count[lb], (unsigned long long)__popcll(same));
+            if (lane == leader) atomicAdd(&h[lb], (unsigned int)__popcll(same));
             if (b == lb) b = -1;
         }
     }
+    __syncthreads();
+    if (threadIdx.x < 64 && h[threadIdx.x]) atomicAdd(&count[(size_t)64 * cur + threadIdx.x], (unsigned long long)h[threadIdx.x]);
 }
 
 // the bin limits of process_wake_ups (src/core/timestep.cc:1577-1625) from the occupation
