@@ -861,7 +861,16 @@ int force_stage(gz_context *ctx)
     }
     CHK(read_counters(ctx));
     if (ctx->hcounters[4]) ctx->need_wakeup = 1;
-    if (ctx->hcounters[2]) return fail(ctx, GZ_STOP_RIEMANN, "hydro_force(): Riemann solver failed to find a positive pressure");
+    if (ctx->hcounters[2]) {
+        // name the first pair without a solution (caller's indices)
+        int pair[2] = {-1, -1}, pi = -1, pj = -1;
+        if (hipMemcpy(pair, ctx->counters + 12, sizeof pair, hipMemcpyDeviceToHost) == hipSuccess && pair[0] >= 0 && pair[1] >= 0) {
+            (void)hipMemcpy(&pi, ctx->perm + pair[0], sizeof(int), hipMemcpyDeviceToHost);
+            (void)hipMemcpy(&pj, ctx->perm + pair[1], sizeof(int), hipMemcpyDeviceToHost);
+        }
+        return fail(ctx, GZ_STOP_RIEMANN, "hydro_force(): Riemann solver failed to find a positive pressure (first pair: particles %d and %d)",
+                    pi, pj);
+    }
     return 0;
 }
 

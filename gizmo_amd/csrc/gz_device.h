@@ -592,9 +592,36 @@ template <typename R> struct alignas(16) Cell {
 };
 
 // dp = x_i - x_j (wrapped), r2 = |dp|^2 already known to satisfy the pair predicate.
-// returns 0 skipped, 1 fluxes produced (maybe zero), -1 no Riemann solution (stop 1234)
+// The Riemann solve of a face and its two retries (hydro_core_meshless_functions.h:351-427):
+// the reconstructed states, then the first-order states with the face velocity at 1.4 times
+// the pressure limit, then zero velocities at twice it.  false: no positive pressure.
+template <typename T>
+__device__ inline bool riemann_ladder(const Gam<T> &G, RState<T> L, RState<T> Rs, const T *n, T ptl, T pI, T pJ, T rhoI, T rhoJ,
+                                      T vi0, T vi1, T vi2, T vj0, T vj1, T vj2, ROut<T> &o)
+{
+    o.P_M = o.S_M = o.f0 = o.f1 = o.f2 = o.fp = 0;
+    riemann(G, L, Rs, n, ptl, o);
+    if ((o.P_M < 0) || (isnan(o.P_M)) || (o.P_M > T(1.4) * ptl)) {
+        Rs.p = pI; L.p = pJ; Rs.rho = rhoI; L.rho = rhoJ;
+        Rs.v0 = vi0; Rs.v1 = vi1; Rs.v2 = vi2;
+        L.v0 = vj0; L.v1 = vj1; L.v2 = vj2;
+        riemann(G, L, Rs, n, T(1.4) * ptl, o);
+        if ((o.P_M < 0) || (isnan(o.P_M))) {
+            Rs.p = pI; L.p = pJ; Rs.rho = rhoI; L.rho = rhoJ;
+            Rs.v0 = Rs.v1 = Rs.v2 = 0; L.v0 = L.v1 = L.v2 = 0;
+            riemann(G, L, Rs, n, T(2.0) * ptl, o);
+            if ((o.P_M < 0) || (isnan(o.P_M))) return false;
+        }
+    }
+    return true;
+}
+
+// returns 0 skipped, 1 fluxes produced (maybe zero), -1 no Riemann solution (stop 1234).  In the
+// FP32-mixed form a face whose FP32 ladder finds no positive pressure (gamma near 1 puts the
+// solver's exponents (gamma-1)/(2 gamma) at the edge of FP32) is solved again in FP64 from the
+// same states before it counts as a failure.
 template <int KERN, typename R>
-__device__ inline int force_pair(const Gam<R> &G, const Cell<R> &I, const Cell<R> &J, R dp0, R dp1, R dp2, R r2,
+__device__ inline int force_pair(const DevParams &P, const Gam<R> &G, const Cell<R> &I, const Cell<R> &J, R dp0, R dp1, R dp2, R r2,
                                  R &F0, R &F1, R &F2, R &Fp, R &vsig_ts)
 {
     F0 = F1 = F2 = Fp = 0;
@@ -647,6 +674,12 @@ __device__ inline int force_pair(const Gam<R> &G, const Cell<R> &I, const Cell<R
         An = sqrt(An);
     }
     if (An == 0) return 1;
+    if constexpr (sizeof(R) != sizeof(double)) {
+        // A face whose area vector is below 1e-18 (a partner at the very edge of its kernel,
+        // W ~ 0) carries no flux at FP32 precision: its squares underflow FP32 and its normal
+        // would not be a unit vector.  In FP64 its flux is of the order of the area itself.
+        if (!(dmax(fabs(A0), dmax(fabs(A1), fabs(A2))) > R(1e-18))) return 1;
+    }
     R n[3];
     {
         R inv = R(1.0) / An;
@@ -684,18 +717,22 @@ __device__ inline int force_pair(const Gam<R> &G, const Cell<R> &I, const Cell<R
     R ptl = R(1.1 * 1.0) * dmax(pit, pjt);
     if (recon == 0) ptl = dmax(ptl, dmax(dmax(I.p, J.p), R(2.) * dmax(I.rho, J.rho) * v2a));
     ROut<R> o;
-    o.P_M = o.S_M = o.f0 = o.f1 = o.f2 = o.fp = 0;
-    riemann(G, L, Rs, n, ptl, o);
-    if ((o.P_M < 0) || (isnan(o.P_M)) || (o.P_M > R(1.4) * ptl)) {
-        Rs.p = I.p; L.p = J.p; Rs.rho = I.rho; L.rho = J.rho;
-        Rs.v0 = I.v0 - vf0; Rs.v1 = I.v1 - vf1; Rs.v2 = I.v2 - vf2;
-        L.v0 = J.v0 - vf0; L.v1 = J.v1 - vf1; L.v2 = J.v2 - vf2;
-        riemann(G, L, Rs, n, R(1.4) * ptl, o);
-        if ((o.P_M < 0) || (isnan(o.P_M))) {
-            Rs.p = I.p; L.p = J.p; Rs.rho = I.rho; L.rho = J.rho;
-            Rs.v0 = Rs.v1 = Rs.v2 = 0; L.v0 = L.v1 = L.v2 = 0;
-            riemann(G, L, Rs, n, R(2.0) * ptl, o);
-            if ((o.P_M < 0) || (isnan(o.P_M))) return -1;
+    if (!riemann_ladder<R>(G, L, Rs, n, ptl, I.p, J.p, I.rho, J.rho, I.v0 - vf0, I.v1 - vf1, I.v2 - vf2, J.v0 - vf0, J.v1 - vf1,
+                           J.v2 - vf2, o)) {
+        if constexpr (sizeof(R) == sizeof(double)) {
+            return -1;
+        } else {
+            Gam<double> Gd;
+            Gd.G0 = P.G0; Gd.G1 = P.G1; Gd.G2 = P.G2; Gd.G3 = P.G3; Gd.G4 = P.G4;
+            Gd.G5 = P.G5; Gd.G6 = P.G6; Gd.G7 = P.G7; Gd.G8 = P.G8; Gd.G9 = P.G9;
+            const RState<double> Ld{L.rho, L.p, L.v0, L.v1, L.v2}, Rd{Rs.rho, Rs.p, Rs.v0, Rs.v1, Rs.v2};
+            const double nd[3] = {n[0], n[1], n[2]};
+            ROut<double> od;
+            if (!riemann_ladder<double>(Gd, Ld, Rd, nd, (double)ptl, (double)I.p, (double)J.p, (double)I.rho, (double)J.rho,
+                                        (double)(I.v0 - vf0), (double)(I.v1 - vf1), (double)(I.v2 - vf2), (double)(J.v0 - vf0),
+                                        (double)(J.v1 - vf1), (double)(J.v2 - vf2), od))
+                return -1;
+            o.P_M = (R)od.P_M; o.S_M = (R)od.S_M; o.f0 = (R)od.f0; o.f1 = (R)od.f1; o.f2 = (R)od.f2; o.fp = (R)od.fp;
         }
     }
     if (!((o.P_M > 0) && (!isnan(o.P_M)))) return 1;

@@ -857,7 +857,8 @@ __global__ void k_pack_force(double gamma, ForceView F, const uint8_t *__restric
 {
     const long j = blockIdx.x * (long)blockDim.x + threadIdx.x;
     const long n = F.n;
-    if (j == 0 && zero2) { zero2[0] = 0; zero2[2] = 0; }  // the force pass's error and wake-up flags
+    // the force pass's error and wake-up flags, and the first pair without a Riemann solution
+    if (j == 0 && zero2) { zero2[0] = 0; zero2[2] = 0; zero2[10] = -1; zero2[11] = -1; }
     if (j >= n) return;
     if (need && !need[j]) return;
     Cell<R> c;
@@ -1052,8 +1053,12 @@ __global__ void __launch_bounds__(256) k_force(DevParams P, Gam<R> G, int nrows,
             const R dp0 = (R)d0, dp1 = (R)d1, dp2 = (R)d2;
             const R r2 = dp0 * dp0 + dp1 * dp1 + dp2 * dp2;
             R f0, f1, f2, fp, vts;
-            int rc = force_pair<KERN, R>(G, I, J, dp0, dp1, dp2, r2, f0, f1, f2, fp, vts);
-            if (rc < 0) { atomicMax(err, 1234); continue; }
+            int rc = force_pair<KERN, R>(P, G, I, J, dp0, dp1, dp2, r2, f0, f1, f2, fp, vts);
+            if (rc < 0) {
+                atomicMax(err, 1234);
+                if (atomicCAS(err + 10, -1, i) == -1) err[11] = j;
+                continue;
+            }
             if (rc == 0) continue;
             a0 += f0; a1 += f1; a2 += f2; ae += fp;
             if (vts > m) m = vts;

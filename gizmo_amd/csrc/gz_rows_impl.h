@@ -602,8 +602,11 @@ __global__ void __launch_bounds__(64) k_force_rows(DevParams P, Gam<R> G, int nr
                 const R dp0 = (R)d0, dp1 = (R)d1, dp2 = (R)d2;
                 const R r2 = dp0 * dp0 + dp1 * dp1 + dp2 * dp2;
                 R f0, f1, f2, fp, vts;
-                rc = force_pair<KERN, R>(G, I, J, dp0, dp1, dp2, r2, f0, f1, f2, fp, vts);
-                if (rc < 0) atomicMax(err, 1234);
+                rc = force_pair<KERN, R>(P, G, I, J, dp0, dp1, dp2, r2, f0, f1, f2, fp, vts);
+                if (rc < 0) {
+                    atomicMax(err, 1234);
+                    if (atomicCAS(err + 10, -1, i) == -1) err[11] = j;
+                }
                 if (rc > 0) {
                     sv[0][lane] = f0; sv[1][lane] = f1; sv[2][lane] = f2; sv[3][lane] = fp; sv[4][lane] = vts;
                     if (active && !active[j] && wakeup) {

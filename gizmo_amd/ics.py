@@ -98,7 +98,7 @@ def lognormal_box(n_particles, seed, *, ng=256, sigma_ln=2.0, slope=-3.0, mach=2
     neighbours at the cell's density."""
     rng = _rng(seed)
     fk, (kx, ky, kz, k2) = _gaussian_field(ng, slope, rng, box)
-    delta = np.fft.irfftn(fk, s=(ng, ng, ng))
+    delta = np.fft.irfftn(fk, s=(ng, ng, ng), axes=(0, 1, 2))
     delta -= delta.mean()
     delta *= sigma_ln / delta.std()
     rho = np.exp(delta).ravel()
@@ -123,7 +123,7 @@ def lognormal_box(n_particles, seed, *, ng=256, sigma_ln=2.0, slope=-3.0, mach=2
     del kd, amp
     vel = np.empty((n_particles, 3))
     for c in range(3):
-        vg = np.fft.irfftn(vk[c], s=(ng, ng, ng)).ravel()
+        vg = np.fft.irfftn(vk[c], s=(ng, ng, ng), axes=(0, 1, 2)).ravel()
         vel[:, c] = vg[cells]
         del vg
     del vk
