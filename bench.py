@@ -15,12 +15,16 @@ counts over the wall time), the job starting from gz_run_begin; the state is res
 for the whole timed region.  Warmup: W steps of an identical untimed job.
 
 Both arithmetic forms run the same job: the plain FP64 form (the reported `value`, as the
-job is the FP64-at-scale configuration) and the FP32-mixed form beside it.  The roofline
-entry covers the force kernel: algorithmic bytes of every launch in the timed region
-(pair-gather model: n_active (S_i + S_out) + n_pairs (S_j + 4), S_i = S_j = 288 B and
-S_out = 40 B per particle in FP64, 144/20 B in FP32, with n_pairs counted on the device) over
-its launch time, measured with HIP events on the stream the kernels run on.  `full_pass` adds
-one all-active pass of the hot path over the 2M particles (the kernels at full occupancy).
+job is the FP64-at-scale configuration) and the FP32-mixed form beside it.  `roofline` covers
+the dominant kernel of the timed job, the drift of all 2M particles to each sync point
+(k_drift: 248 algorithmic bytes per particle, 21 doubles read and 10 written), its average
+launch time measured with HIP events on the stream the kernels run on.  `roofline_force` is
+the force kernel over the same job (pair-gather model: n_active (S_i + S_out) + n_pairs
+(S_j + 4), S_i = S_j = 288 B and S_out = 40 B per particle in FP64, 144/20 B in FP32, n_pairs
+counted on the device); with a few hundred active rows per step its launches are short and
+latency-bound.  `full_pass` adds one all-active pass of the hot path over the 2M particles
+(the kernels at full occupancy) and `roofline_force_full_pass` the force kernel there.
+The serving loop (`serving`, p50/p99 job latency) and the CPU baseline follow.
 The CPU baseline is the FP64 oracle (C + OpenMP) running the first steps of the same job on
 this host's cores.
 """
@@ -41,7 +45,10 @@ HBM_PEAK_GBS = 8000.0
 FORCE_MODEL = {"fp64": dict(S_i=288, S_out=40, S_j=288), "mixed": dict(S_i=144, S_out=20, S_j=144)}
 INTEG = dict(courant_fac=0.05, err_tol_int_accuracy=0.002, max_size_timestep=0.01, min_size_timestep=0.0,
              time_begin=0.0, time_max=1.0, min_egy_spec=0.0)
-FAMILIES = ("timeline", "sort", "ngb_build", "density", "gradients", "force")
+FAMILIES = ("timeline", "drift", "sort", "ngb_build", "density", "gradients", "force")
+# the drift of one particle (k_drift): reads X,Y,Z, VelKick3, DivVel, m, VelPred3, HydroAccel3, rho, u,
+# du/dt, h, the three ordering positions (21 doubles); writes X,Y,Z, VelPred3, rho, u, h, P (10)
+DRIFT_BYTES = {"fp64": 248, "mixed": 248}
 
 
 def log(msg):
@@ -128,6 +135,7 @@ def full_pass(ic, precision, reps=3):
 
 
 def roofline(res, form):
+    """The force kernel over the timed job: pair-gather bytes of all its launches over their time."""
     m = FORCE_MODEL[form]
     launches, total_ms = res["kernels"]["force"]
     n_act = sum(res["active"])
@@ -135,9 +143,32 @@ def roofline(res, form):
     achieved = bytes_total / (total_ms / 1e3) / 1e9
     return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-            "kernel": f"k_force<6,{'double' if form == 'fp64' else 'float'}>",
+            "kernel": f"k_force_rows<6,{'double' if form == 'fp64' else 'float'}>",
             "launches": launches, "bytes_per_launch": int(bytes_total / max(launches, 1)),
             "avg_launch_ms": round(total_ms / max(launches, 1), 4)}
+
+
+def roofline_drift(res, n, form):
+    """The dominant kernel of the timed job: the drift of every particle to each sync point."""
+    launches, total_ms = res["kernels"]["drift"]
+    per_launch = n * DRIFT_BYTES[form]
+    avg_ms = total_ms / max(launches, 1)
+    achieved = per_launch / (avg_ms / 1e3) / 1e9
+    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": "k_drift",
+            "launches": launches, "bytes_per_launch": per_launch, "avg_launch_ms": round(avg_ms, 4),
+            "bytes_model": f"{DRIFT_BYTES[form]} B per particle x {n} particles"}
+
+
+def roofline_full_pass(fp, form):
+    """The force kernel at full occupancy (every particle active, lane-per-row form)."""
+    m = FORCE_MODEL[form]
+    ms = fp["kernel_ms"]["force"]
+    bytes_total = fp["n_active"] * (m["S_i"] + m["S_out"]) + fp["pairs"] * (m["S_j"] + 4)
+    achieved = bytes_total / (ms / 1e3) / 1e9
+    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "kernel": f"k_force<6,{'double' if form == 'fp64' else 'float'}>",
+            "launches": 1, "bytes_per_launch": int(bytes_total), "avg_launch_ms": ms}
 
 
 def serve_bench(precision, n_jobs=48, seed=1005, capacity=4 * 1024 * 1024, max_boxes=64, load=0.8, calib_jobs=8):
@@ -420,16 +451,18 @@ def main():
         "woken_per_step": r["woken"],
         "resorts": r["resorts"],
         "kernel_ms": summary(r)["kernel_ms"],
-        "roofline": roofline(r, head),
+        "roofline": roofline_drift(r, n, head),
+        "roofline_force": roofline(r, head),
         "p50_ms": None, "p99_ms": None,
     }
     for f in res:
         if f != head:
-            out[f] = dict(summary(res[f]), roofline=roofline(res[f], f))
+            out[f] = dict(summary(res[f]), roofline=roofline_drift(res[f], n, f), roofline_force=roofline(res[f], f))
     if "fp64" in res and "mixed" in res:
         out["mixed_over_fp64"] = round(summary(res["mixed"])["value"] / summary(res["fp64"])["value"], 3)
     if fp:
         out["full_pass"] = fp
+        out["roofline_force_full_pass"] = {f: roofline_full_pass(fp[f], f) for f in fp}
         if "fp64" in fp and "mixed" in fp:
             out["full_pass_mixed_over_fp64"] = round(fp["mixed"]["updates_per_s"] / fp["fp64"]["updates_per_s"], 3)
     if not a.no_serve:

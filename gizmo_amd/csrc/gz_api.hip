@@ -1047,8 +1047,14 @@ int run_step(gz_context *ctx)
         }
         if (ctx->nact > 0) k_halfstep_kick<<<nblk(ctx->nact), kBlock, 0, ctx->stream>>>(T, S, ctx->nact, ctx->act, 0, BoxTable{});
         k_next_sync<<<1, 64, 0, ctx->stream>>>(T, ctx->u64c, ctx->u64c + 70, BoxTable{});
+    }
+    {
+        Timed t(ctx, "drift");
         k_drift<<<std::min(nblk(n), 2048u), kBlock, 0, ctx->stream>>>(T, S, ti_prev, ctx->u64c + 70, ctx->X0, ctx->Y0, ctx->Z0,
                                                                      ctx->u64c + 64, ctx->u64c + 66, BoxTable{});
+    }
+    {
+        Timed t(ctx, "timeline");
         k_active_flags<<<nblk(n + 1), kBlock, 0, ctx->stream>>>(n, ctx->u64c + 70, ctx->TB, ctx->M, ctx->H,
                                                                 0.99 * ctx->par.max_kernel_radius, ctx->ACT, ctx->SOFT, ctx->iflag,
                                                                 ctx->u64c + 67, BoxTable{});

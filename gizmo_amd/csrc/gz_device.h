@@ -616,10 +616,7 @@ __device__ inline bool riemann_ladder(const Gam<T> &G, RState<T> L, RState<T> Rs
     return true;
 }
 
-// returns 0 skipped, 1 fluxes produced (maybe zero), -1 no Riemann solution (stop 1234).  In the
-// FP32-mixed form a face whose FP32 ladder finds no positive pressure (gamma near 1 puts the
-// solver's exponents (gamma-1)/(2 gamma) at the edge of FP32) is solved again in FP64 from the
-// same states before it counts as a failure.
+// returns 0 skipped, 1 fluxes produced (maybe zero), -1 no Riemann solution (stop 1234)
 template <int KERN, typename R>
 __device__ inline int force_pair(const DevParams &P, const Gam<R> &G, const Cell<R> &I, const Cell<R> &J, R dp0, R dp1, R dp2, R r2,
                                  R &F0, R &F1, R &F2, R &Fp, R &vsig_ts)
@@ -718,23 +715,8 @@ __device__ inline int force_pair(const DevParams &P, const Gam<R> &G, const Cell
     if (recon == 0) ptl = dmax(ptl, dmax(dmax(I.p, J.p), R(2.) * dmax(I.rho, J.rho) * v2a));
     ROut<R> o;
     if (!riemann_ladder<R>(G, L, Rs, n, ptl, I.p, J.p, I.rho, J.rho, I.v0 - vf0, I.v1 - vf1, I.v2 - vf2, J.v0 - vf0, J.v1 - vf1,
-                           J.v2 - vf2, o)) {
-        if constexpr (sizeof(R) == sizeof(double)) {
-            return -1;
-        } else {
-            Gam<double> Gd;
-            Gd.G0 = P.G0; Gd.G1 = P.G1; Gd.G2 = P.G2; Gd.G3 = P.G3; Gd.G4 = P.G4;
-            Gd.G5 = P.G5; Gd.G6 = P.G6; Gd.G7 = P.G7; Gd.G8 = P.G8; Gd.G9 = P.G9;
-            const RState<double> Ld{L.rho, L.p, L.v0, L.v1, L.v2}, Rd{Rs.rho, Rs.p, Rs.v0, Rs.v1, Rs.v2};
-            const double nd[3] = {n[0], n[1], n[2]};
-            ROut<double> od;
-            if (!riemann_ladder<double>(Gd, Ld, Rd, nd, (double)ptl, (double)I.p, (double)J.p, (double)I.rho, (double)J.rho,
-                                        (double)(I.v0 - vf0), (double)(I.v1 - vf1), (double)(I.v2 - vf2), (double)(J.v0 - vf0),
-                                        (double)(J.v1 - vf1), (double)(J.v2 - vf2), od))
-                return -1;
-            o.P_M = (R)od.P_M; o.S_M = (R)od.S_M; o.f0 = (R)od.f0; o.f1 = (R)od.f1; o.f2 = (R)od.f2; o.fp = (R)od.fp;
-        }
-    }
+                           J.v2 - vf2, o))
+        return -1;
     if (!((o.P_M > 0) && (!isnan(o.P_M)))) return 1;
     o.fp += vf0 * o.f0 + vf1 * o.f1 + vf2 * o.f2;
     Fp = An * o.fp;

@@ -128,7 +128,9 @@ int gz_stat(gz_context *ctx, const char *name, double *value);
 
 /* Kernel timing on the context's stream (HIP events around each launch).  gz_timing
    returns the launch count and total milliseconds of the named kernel family since the last
-   gz_timing_reset ("ngb_build", "density", "gradients", "force", "sort"). */
+   gz_timing_reset ("ngb_build", "density", "gradients", "force", "sort"; in the timestep loop
+   also "drift" -- the drift of every particle -- and "timeline" -- timesteps, bins, wake-ups,
+   kicks and the active list). */
 int gz_timing_enable(gz_context *ctx, int on);
 int gz_timing_reset(gz_context *ctx);
 int gz_timing(gz_context *ctx, const char *name, long *launches, double *total_ms);
