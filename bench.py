@@ -49,6 +49,9 @@ FAMILIES = ("timeline", "drift", "sort", "ngb_build", "density", "gradients", "f
 # the drift of one particle (k_drift): reads X,Y,Z, VelKick3, DivVel, m, VelPred3, HydroAccel3, rho, u,
 # du/dt, h, the three ordering positions (21 doubles); writes X,Y,Z, VelPred3, rho, u, h, P (10)
 DRIFT_BYTES = {"fp64": 248, "mixed": 248}
+# HBM bytes per particle that k_drift moves, measured: FETCH_SIZE + WRITE_SIZE per dispatch at 2,097,152
+# particles (FETCH_SIZE scaled x2 by a 16 MiB device copy), profiles/sedov128_timestep_job_traffic.md
+DRIFT_TRAFFIC_PER_PARTICLE = 520.7e6 / 2097152
 
 
 def log(msg):
@@ -155,7 +158,7 @@ def roofline_drift(res, n, form):
     avg_ms = total_ms / max(launches, 1)
     achieved = per_launch / (avg_ms / 1e3) / 1e9
     return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": "k_drift",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": int(DRIFT_TRAFFIC_PER_PARTICLE * n), "kernel": "k_drift",
             "launches": launches, "bytes_per_launch": per_launch, "avg_launch_ms": round(avg_ms, 4),
             "bytes_model": f"{DRIFT_BYTES[form]} B per particle x {n} particles"}
 
