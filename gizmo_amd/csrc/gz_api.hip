@@ -105,6 +105,9 @@ struct gz_context {
     int cap_s = 0;
     long ld_s = 0;
     bool sym_valid = false;
+    // inside run_step's hydro pass the symmetric list's overflow is read back with the force
+    // pass's counters (one wait fewer); on overflow both passes run again on a wider list
+    bool defer_sym_check = false, sym_check_pending = false;
     bool density_done = false, gradients_done = false;
 
     // occupied cells of the active rows at the current grid level
@@ -785,10 +788,14 @@ int ensure_sym(gz_context *ctx)
                                                                                   ctx->cnt_s, ctx->counters + 3);
             HIPCHK(hipGetLastError());
         }
-        CHK(read_counters(ctx));
-        const int over = ctx->hcounters[3];
+        int over = 0;
+        if (!ctx->defer_sym_check) {
+            CHK(read_counters(ctx));
+            over = ctx->hcounters[3];
+        }
         if (over == 0) {
             ctx->sym_valid = true;
+            ctx->sym_check_pending = ctx->defer_sym_check;
             // the records the gradient and force passes read: every record when many rows are
             // active, else only those of the rows and their neighbours
             ctx->rec_need_valid = false;
@@ -860,6 +867,23 @@ int force_stage(gz_context *ctx)
         HIPCHK(hipGetLastError());
     }
     CHK(read_counters(ctx));
+    if (ctx->sym_check_pending) {
+        ctx->sym_check_pending = false;
+        const int over = ctx->hcounters[3];
+        if (over) {
+            // the list the two passes used was cut short: a wider one, and both passes again
+            ctx->cap_s = std::max(over + 16, ctx->cap_s * 3 / 2);
+            CHK(dalloc(ctx, &ctx->ell_s, (size_t)ctx->cap_s * (size_t)ctx->ld_s));
+            ctx->stats["ell_widen"] += 1;
+            ctx->sym_valid = false;
+            const bool d = ctx->defer_sym_check;
+            ctx->defer_sym_check = false;
+            int rc = gradients_stage<KERN, R>(ctx);
+            if (!rc) rc = force_stage<KERN, R>(ctx);
+            ctx->defer_sym_check = d;
+            return rc;
+        }
+    }
     if (ctx->hcounters[4]) ctx->need_wakeup = 1;
     if (ctx->hcounters[2]) {
         // name the first pair without a solution (caller's indices)
@@ -1016,8 +1040,12 @@ int hydro_pass(gz_context *ctx)
     ctx->dp.time = ctx->time;
     ctx->par.time = ctx->time;
     CHK(dispatch<DensityF>(ctx));
-    CHK(dispatch<GradF>(ctx));
-    CHK(dispatch<ForceF>(ctx));
+    ctx->defer_sym_check = true;
+    int rc = dispatch<GradF>(ctx);
+    if (!rc) rc = dispatch<ForceF>(ctx);
+    ctx->defer_sym_check = false;
+    ctx->sym_check_pending = false;
+    if (rc) return rc;
     ctx->stats["active_updates"] += ctx->nact;
     return 0;
 }
@@ -1037,8 +1065,8 @@ int run_step(gz_context *ctx)
         Timed t(ctx, "timeline");
         k_step_prep<<<1, 64, 0, ctx->stream>>>(ctx->counters, ctx->u64c);
         if (ctx->nact > 0)
-            k_find_timesteps<<<nblk(ctx->nact), kBlock, 0, ctx->stream>>>(T, S, ctx->nact, ctx->act, ctx->counters + 6, BoxTable{});
-        k_bin_hist<<<std::min(nblk(n), 1024u), kBlock, 0, ctx->stream>>>(n, ctx->TB, ctx->u64c, BoxTable{});
+            k_find_timesteps<<<nblk(ctx->nact), kBlock, 0, ctx->stream>>>(T, S, ctx->nact, ctx->act, ctx->counters + 6, ctx->u64c,
+                                                                          BoxTable{});
         if (ctx->need_wakeup) {
             int wakeup_bin_offset = 0;
             while ((1LL << wakeup_bin_offset) < (long long)kWakeup) wakeup_bin_offset++;

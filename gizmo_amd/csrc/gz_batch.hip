@@ -871,15 +871,15 @@ int gz_batch_step(gz_batch *b)
     BHIP(hipMemsetAsync(b->hist, 0, 64 * nbx * 8, b->stream));
     BHIP(hipMemsetAsync(b->slot, 0, kBoxSlots * nbx * 8, b->stream));
     BHIP(hipMemsetAsync(b->counters, 0, 16 * sizeof(int), b->stream));
-    if (b->nact > 0) k_find_timesteps<<<nb(b->nact), kB, 0, b->stream>>>(T, S, b->nact, b->act, b->counters + 6, bt);
+    if (b->nact > 0) k_find_timesteps<<<nb(b->nact), kB, 0, b->stream>>>(T, S, b->nact, b->act, b->counters + 6, nullptr, bt);
     k_bin_hist<<<std::min(nb(n), 1024u), kB, 0, b->stream>>>(n, b->TB, b->hist, bt);
     int wakeup_bin_offset = 0;
     while ((1LL << wakeup_bin_offset) < (long long)kWakeup) wakeup_bin_offset++;
     const unsigned nbb = (unsigned)((nbx + 63) / 64);
-    k_wakeup_limits<<<nbb, 64, 0, b->stream>>>(T, b->hist, b->lim, bt);
+    k_wakeup_limits<<<(unsigned)nbx, 64, 0, b->stream>>>(T, b->hist, b->lim, bt);  // one wavefront per box
     k_wakeups<<<nb(n), kB, 0, b->stream>>>(T, S, b->lim, wakeup_bin_offset, b->hist, b->counters + 5, bt);
     if (b->nact > 0) k_halfstep_kick<<<nb(b->nact), kB, 0, b->stream>>>(T, S, b->nact, b->act, 0, bt);
-    k_next_sync<<<nbb, 64, 0, b->stream>>>(T, b->hist, nullptr, bt);
+    k_next_sync<<<(unsigned)nbx, 64, 0, b->stream>>>(T, b->hist, nullptr, bt);
     k_drift<<<std::min(nb(n), 2048u), kB, 0, b->stream>>>(T, S, 0, nullptr, b->X0, b->Y0, b->Z0, nullptr, nullptr, bt);
     k_active_flags<<<nb(n + 1), kB, 0, b->stream>>>(n, nullptr, b->TB, b->M, b->H, 0, b->ACT, b->SOFT, b->iflag, nullptr, bt);
     BHIP(hipGetLastError());

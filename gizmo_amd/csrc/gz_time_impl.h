@@ -111,35 +111,48 @@ __device__ void kick(const TimeParams &T, const ParticleView &S, long i, long lo
 // find_timesteps, src/core/timestep.cc:39-355: the new power-of-two step of each particle of
 // the active list; a larger bin only where it is synchronised now
 __global__ void __launch_bounds__(256) k_find_timesteps(TimeParams T, ParticleView S, int nrows, const int *__restrict__ act,
-                                                         int *__restrict__ err, BoxTable bt)
+                                                         int *__restrict__ err, unsigned long long *__restrict__ count, BoxTable bt)
 {
+    // count (one box, may be null): TimeBinCount kept up to date with the moves of the rows,
+    // added up per block in LDS
+    __shared__ int moved[64];
+    if (count && threadIdx.x < 64) moved[threadIdx.x] = 0;
+    if (count) __syncthreads();
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= nrows) return;
-    const int i = act[r];
-    resolve_time(bt, i, T);
-    long long ti_step = get_timestep(T, S, i);
-    ti_step = (long long)(((double)ti_step) / 1.0);
-    long long ti_min = kTimebase;
-    while (ti_min > ti_step) ti_min >>= 1;
-    ti_step = ti_min;
-    int bin = timestep_bin(ti_step);
-    const int binold = S.TB[i];
-    if (bin > binold) {
-        while (!bin_on(T.active_mask, bin) && bin > binold) bin--;
-        ti_step = bin_len(bin);
-    }
-    if (T.ti_current >= kTimebase) { ti_step = 0; bin = 0; }
-    if ((kTimebase - T.ti_current) < ti_step) {
-        if (bt.b) atomicMax(&bt.slot[(size_t)bt.BOX[i] * kBoxSlots + kSlotStop], 90001005ull);
-        else atomicMax(err, 90001005);
-        ti_step = kTimebase - T.ti_current;
-        ti_min = kTimebase;
+    if (r < nrows) {
+        const int i = act[r];
+        resolve_time(bt, i, T);
+        long long ti_step = get_timestep(T, S, i);
+        ti_step = (long long)(((double)ti_step) / 1.0);
+        long long ti_min = kTimebase;
         while (ti_min > ti_step) ti_min >>= 1;
         ti_step = ti_min;
+        int bin = timestep_bin(ti_step);
+        const int binold = S.TB[i];
+        if (bin > binold) {
+            while (!bin_on(T.active_mask, bin) && bin > binold) bin--;
+            ti_step = bin_len(bin);
+        }
+        if (T.ti_current >= kTimebase) { ti_step = 0; bin = 0; }
+        if ((kTimebase - T.ti_current) < ti_step) {
+            if (bt.b) atomicMax(&bt.slot[(size_t)bt.BOX[i] * kBoxSlots + kSlotStop], 90001005ull);
+            else atomicMax(err, 90001005);
+            ti_step = kTimebase - T.ti_current;
+            ti_min = kTimebase;
+            while (ti_min > ti_step) ti_min >>= 1;
+            ti_step = ti_min;
+        }
+        S.TB[i] = bin;
+        S.TIB[i] += S.DTS[i];
+        S.DTS[i] = ti_step;
+        if (count && bin != binold) {
+            atomicAdd(&moved[binold & 63], -1);
+            atomicAdd(&moved[bin & 63], 1);
+        }
     }
-    S.TB[i] = bin;
-    S.TIB[i] += S.DTS[i];
-    S.DTS[i] = ti_step;
+    if (!count) return;
+    __syncthreads();
+    if (threadIdx.x < 64 && moved[threadIdx.x]) atomicAdd(&count[threadIdx.x], (unsigned long long)(long long)moved[threadIdx.x]);
 }
 
 // TimeBinCount of every box (count[64*box + bin]).  Each block counts a contiguous range of
@@ -195,37 +208,47 @@ __global__ void __launch_bounds__(256) k_bin_hist(long n, const int *__restrict_
     if (threadIdx.x < 64 && h[threadIdx.x]) atomicAdd(&count[(size_t)64 * cur + threadIdx.x], (unsigned long long)h[threadIdx.x]);
 }
 
-// the bin limits of process_wake_ups (src/core/timestep.cc:1577-1625) from the occupation
-// after find_timesteps, one lane per box: lim[2b] the lowest occupied active bin, lim[2b+1] the
-// highest bin active at the next kick
-__global__ void k_wakeup_limits(TimeParams T, const unsigned long long *__restrict__ count, int *__restrict__ lim, BoxTable bt)
+// The earliest next kick over the occupied bins of one box (src/core/run.cc:870-890,
+// src/core/timestep.cc:1577-1591), one wavefront per box with lane k taking bin k; every lane
+// returns the minimum.
+__device__ __forceinline__ long long wave_next_kick(const unsigned long long *__restrict__ c, long long tic)
 {
-    const int b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= (bt.b ? bt.nbox : 1)) return;
-    long long tic = T.ti_current;
-    unsigned long long mask = T.active_mask;
-    if (bt.b) { tic = bt.b[b].ti_current; mask = bt.b[b].mask; }
-    const unsigned long long *c = count + (size_t)64 * b;
-    long long ti_next_kick = kTimebase;
-    for (int k = 0; k < kTimebins; k++) {
-        if (!c[k]) continue;
-        long long t;
+    const int k = threadIdx.x & 63;
+    long long t = kTimebase;
+    if (k < kTimebins && c[k]) {
         if (k > 0) {
             const long long dt_bin = bin_len(k);
             t = (tic / dt_bin) * dt_bin + dt_bin;
         } else {
             t = tic;
         }
-        if (t < ti_next_kick) ti_next_kick = t;
     }
-    int max_time_bin_active = 0;
-    for (int k = 0; k < kTimebins; k++)
-        if ((ti_next_kick % (1LL << k)) == 0) max_time_bin_active = k;
-    int lowest = kTimebins;
-    for (int k = 0; k < kTimebins; k++)
-        if (bin_on(mask, k) && c[k] > 0) { lowest = k; break; }
-    lim[2 * b] = lowest;
-    lim[2 * b + 1] = max_time_bin_active;
+    for (int o = 32; o > 0; o >>= 1) {
+        const long long u = __shfl_xor(t, o, 64);
+        t = u < t ? u : t;
+    }
+    return t;
+}
+
+// the bin limits of process_wake_ups (src/core/timestep.cc:1577-1625) from the occupation
+// after find_timesteps, one wavefront per box (block b = box b, lane k = bin k): lim[2b] the
+// lowest occupied active bin, lim[2b+1] the highest bin active at the next kick
+__global__ void __launch_bounds__(64) k_wakeup_limits(TimeParams T, const unsigned long long *__restrict__ count, int *__restrict__ lim,
+                                                      BoxTable bt)
+{
+    const int b = blockIdx.x;
+    const int k = threadIdx.x;
+    long long tic = T.ti_current;
+    unsigned long long mask = T.active_mask;
+    if (bt.b) { tic = bt.b[b].ti_current; mask = bt.b[b].mask; }
+    const unsigned long long *c = count + (size_t)64 * b;
+    const long long ti_next_kick = wave_next_kick(c, tic);
+    const unsigned long long divides = __ballot(k < kTimebins && (ti_next_kick % (1LL << k)) == 0);
+    const unsigned long long occupied_active = __ballot(k < kTimebins && bin_on(mask, k) && c[k] > 0);
+    if (k == 0) {
+        lim[2 * b] = occupied_active ? __ffsll((long long)occupied_active) - 1 : kTimebins;
+        lim[2 * b + 1] = 63 - __clzll((long long)divides);  // bit 0 is always set
+    }
 }
 
 // process_wake_ups, src/core/timestep.cc:1565-1774 (every particle stands at the sync point:
@@ -380,45 +403,36 @@ __global__ void __launch_bounds__(256) k_drift(TimeParams T, ParticleView S, lon
     }
 }
 
-// the start of a step: counters and the bin occupation cleared (the running pair count kept)
+// the start of a step: counters and maxima cleared (the bin occupation and the running pair count kept)
 __global__ void k_step_prep(int *__restrict__ counters, unsigned long long *__restrict__ u64c)
 {
     const int t = threadIdx.x;
     if (t < 16) counters[t] = 0;
-    if (t < 64) u64c[t] = 0;
     if (t == 0) { u64c[64] = 0; u64c[66] = 0; u64c[67] = 0; u64c[68] = 0; }
 }
 
 // find_next_sync_point_and_drift's timeline part (src/core/run.cc:857-930) from the occupation,
-// one lane per box: the next kick (All.Ti_Current after the step) and the bins active there
-// (single box: tl[0], tl[1]; a box not stepping keeps its time and has no active bin)
-__global__ void k_next_sync(TimeParams T, const unsigned long long *__restrict__ count, unsigned long long *__restrict__ tl, BoxTable bt)
+// one wavefront per box (block b = box b, lane k = bin k): the next kick (All.Ti_Current after
+// the step) and the bins active there (single box: tl[0], tl[1]; a box not stepping keeps its
+// time and has no active bin)
+__global__ void __launch_bounds__(64) k_next_sync(TimeParams T, const unsigned long long *__restrict__ count,
+                                                  unsigned long long *__restrict__ tl, BoxTable bt)
 {
-    const int b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= (bt.b ? bt.nbox : 1)) return;
+    const int b = blockIdx.x;
+    const int k = threadIdx.x;
     long long tic = T.ti_current;
     if (bt.b) {
         tic = bt.b[b].ti_current;
-        bt.b[b].need_wakeup = 0;  // consumed by the wake-up pass of this step
-        if (!bt.b[b].stepping) { bt.b[b].ti_next = tic; bt.b[b].mask_next = 0; return; }
-    }
-    const unsigned long long *c = count + (size_t)64 * b;
-    long long ti_next_kick = kTimebase;
-    for (int k = 0; k < kTimebins; k++) {
-        if (!c[k]) continue;
-        long long t;
-        if (k > 0) {
-            const long long dt_bin = bin_len(k);
-            t = (tic / dt_bin) * dt_bin + dt_bin;
-        } else {
-            t = tic;
+        if (!bt.b[b].stepping) {
+            if (k == 0) { bt.b[b].need_wakeup = 0; bt.b[b].ti_next = tic; bt.b[b].mask_next = 0; }
+            return;
         }
-        if (t < ti_next_kick) ti_next_kick = t;
     }
-    unsigned long long mask = 1ull;
-    for (int k = 1; k < kTimebins; k++)
-        if ((ti_next_kick % bin_len(k)) == 0) mask |= 1ull << k;
+    const long long ti_next_kick = wave_next_kick(count + (size_t)64 * b, tic);
+    const unsigned long long mask = __ballot(k == 0 || (k < kTimebins && (ti_next_kick % bin_len(k)) == 0));
+    if (k != 0) return;
     if (bt.b) {
+        bt.b[b].need_wakeup = 0;  // consumed by the wake-up pass of this step
         bt.b[b].ti_next = ti_next_kick;
         bt.b[b].mask_next = mask;
     } else {
