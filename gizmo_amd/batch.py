@@ -41,6 +41,7 @@ def blib():
         L.gz_batch_particles.restype = C.c_long
         L.gz_batch_boxes.argtypes = [C.c_void_p]
         L.gz_batch_stat.argtypes = [C.c_void_p, C.c_char_p, C.POINTER(C.c_double)]
+        L.gz_batch_set_option.argtypes = [C.c_void_p, C.c_char_p, C.c_double]
         _bound = True
     return L
 
@@ -120,6 +121,10 @@ class Batch:
         v = C.c_double()
         self._check(blib().gz_batch_stat(self.ctx, name.encode(), C.byref(v)))
         return v.value
+
+    def set_option(self, name, value):
+        """Tuning options ("few_rows"): the same lists and sums whatever their value."""
+        self._check(blib().gz_batch_set_option(self.ctx, name.encode(), float(value)))
 
     def timing_reset(self):
         self._check(blib().gz_batch_timing_reset(self.ctx))

@@ -19,7 +19,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libgizmo_hip.so")
 SOURCES = ["gz_kernels.hip", "gz_kernels_f32.hip", "gz_api.hip", "gz_batch.hip"]
-HEADERS = ["gz_device.h", "gz_kernels.h", "gz_kernels_impl.h", "gz_rows_impl.h", "gz_time_impl.h", "gz_batch_impl.h",
+HEADERS = ["gz_device.h", "gz_kernels.h", "gz_groups.h", "gz_kernels_impl.h", "gz_rows_impl.h", "gz_time_impl.h", "gz_batch_impl.h",
            os.path.join("..", "..", "include", "gizmo_hydro.h"), os.path.join("..", "..", "include", "gizmo_batch.h")]
 
 

@@ -20,6 +20,7 @@
 
 #include "../../include/gizmo_hydro.h"
 #include "gz_kernels.h"
+#include "gz_groups.h"
 
 using namespace gz;
 
@@ -77,6 +78,8 @@ struct gz_context {
     long long wcells_alloc = 0;
     long wide_key_version = -1;
     int wide_lmax = -1;
+    // many rows over many levels: rows grouped by the cell of their own level
+    RowGroups grp;
 
     // staging and scratch
     double *stage = nullptr;  // n*9 doubles
@@ -335,6 +338,12 @@ void free_state(gz_context *c)
     dfree(c->TB); dfree(c->WAKE); dfree(c->ACT); dfree(c->perm); dfree(c->key); dfree(c->key2);
     dfree(c->wstart); dfree(c->wend); dfree(c->wlstart); dfree(c->wlend); dfree(c->lidx); dfree(c->plev); dfree(c->lcount);
     dfree(c->wide_dev);
+    {
+        RowGroups &G = c->grp;
+        dfree(G.key); dfree(G.skey); dfree(G.gkey); dfree(G.val); dfree(G.row); dfree(G.flag); dfree(G.scan); dfree(G.start);
+        dfree(G.count); dfree(G.lev); dfree(G.code); dfree(G.nst); dfree(G.sten);
+        G = RowGroups{};
+    }
     c->wcells_alloc = 0;
     c->wide_key_version = -1;
     dfree(c->ival); dfree(c->ival2); dfree(c->iflag); dfree(c->iscan); dfree(c->act); dfree(c->stage);
@@ -606,6 +615,37 @@ int build_wide(gz_context *ctx, bool lists)
     return 0;
 }
 
+// The active rows with need set (all when need is null) grouped by the cell of their own level
+// (radius from hslot per slot, else from H) for the cell walks over many levels
+int build_groups(gz_context *ctx, const uint8_t *need, const double *hslot)
+{
+    RowGroups &G = ctx->grp;
+    const int na = ctx->nact;
+    if (G.rows_alloc < (long)na + 1) {
+        const size_t m = (size_t)ctx->cap + 1;
+        CHK(dalloc(ctx, &G.key, m)); CHK(dalloc(ctx, &G.skey, m)); CHK(dalloc(ctx, &G.gkey, m)); CHK(dalloc(ctx, &G.val, m));
+        CHK(dalloc(ctx, &G.row, m)); CHK(dalloc(ctx, &G.flag, m)); CHK(dalloc(ctx, &G.scan, m)); CHK(dalloc(ctx, &G.start, m));
+        CHK(dalloc(ctx, &G.count, 1));
+        G.rows_alloc = (long)m;
+    }
+    size_t bytes = 0;
+    HIPCHK(group_tmp_bytes(na, bytes));
+    CHK(ensure_tmp(ctx, bytes));
+    Timed t(ctx, "ngb_build");
+    HIPCHK(group_rows(G, ctx->stream, ctx->tmp, ctx->tmp_bytes, na, ctx->act, need, ctx->key, hslot, ctx->H, ctx->par.box_size, ctx->dmax,
+                      ctx->wide_dev, BoxTable{}, true));
+    HIPCHK(hipMemcpyAsync(ctx->hcounters + 12, G.count, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    G.ngroups = ctx->hcounters[12];
+    if (G.groups_alloc < G.ngroups) {
+        const long m = std::min((long)na, (long)G.ngroups + G.ngroups / 4 + 64);
+        CHK(dalloc(ctx, &G.lev, m)); CHK(dalloc(ctx, &G.code, m)); CHK(dalloc(ctx, &G.nst, m)); CHK(dalloc(ctx, &G.sten, (size_t)27 * m));
+        G.groups_alloc = m;
+    }
+    HIPCHK(group_stencils(G, ctx->stream, grid_view(ctx), ctx->act, BoxTable{}));
+    return 0;
+}
+
 // build ELL rows (all slots when rows == nullptr) into a list that grows its width on overflow
 template <bool SYM>
 int build_list(gz_context *ctx, int nrows, const int *rows, const double *hrow, double scale, int **ell, long *ld, int *cap,
@@ -698,14 +738,22 @@ int density_stage(gz_context *ctx)
     else CHK(build_grid(ctx, std::min(hmax, 0.5 * ctx->par.box_size)));
     const bool few = wide || na <= ctx->few_rows;
     if (!few) CHK(build_cell_rows(ctx));
+    int remaining = na;  // rows of the coming round
     DensityIter it{ctx->it_h, ctx->it_hb, ctx->it_l, ctx->it_r, ctx->it_cn, ctx->it_ci, ctx->it_it};
     DensityOut out{ctx->n, ctx->H, ctx->NN, ctx->DRK, ctx->DIVV, ctx->RHO, ctx->PR, ctx->T, ctx->FW, ctx->CN, ctx->FCE};
     int iters = 0;
     while (true) {
         HIPCHK(hipMemsetAsync(ctx->counters, 0, 3 * sizeof(int), ctx->stream));
         {
+            // over many levels, a round of many rows walks the groups of their current radii
+            const bool groups = wide && remaining > ctx->few_rows;
+            if (groups) CHK(build_groups(ctx, ctx->need, ctx->it_h));
             Timed t(ctx, "density");
-            if (few)
+            if (groups)
+                k_density_cells<KERN, R><<<xcd_grid(ctx->grp.ngroups), 64, 0, ctx->stream>>>(
+                    ctx->dp, grid_view(ctx), ctx->grp.view(BoxTable{}), ctx->act, ctx->need, ctx->X, ctx->Y, ctx->Z, ctx->VX, ctx->VY,
+                    ctx->VZ, ctx->M, ctx->it_h, ctx->dacc);
+            else if (few)
                 k_density_rows<KERN, R><<<xcd_grid(na), 64, 0, ctx->stream>>>(ctx->dp, grid_view(ctx), na, ctx->act, ctx->need, ctx->key,
                                                                             ctx->X, ctx->Y, ctx->Z, ctx->VX, ctx->VY, ctx->VZ, ctx->M,
                                                                             ctx->it_h, ctx->dacc, BoxTable{});
@@ -720,7 +768,8 @@ int density_stage(gz_context *ctx)
         }
         CHK(read_counters(ctx));
         iters++;
-        const int remaining = ctx->hcounters[0], rebuild = ctx->hcounters[1], err = ctx->hcounters[2];
+        remaining = ctx->hcounters[0];
+        const int rebuild = ctx->hcounters[1], err = ctx->hcounters[2];
         if (err) {
             ctx->stats["density_iterations"] = iters;
             if (err == GZ_STOP_MAXITER) return fail(ctx, GZ_STOP_MAXITER, "density(): kernel radius iteration did not converge in %d rounds", kMaxIter);
@@ -758,11 +807,12 @@ int ensure_sym(gz_context *ctx)
         ctx->hmax_bound = hmax;
     }
     const bool wide = ctx->wide != 0;
-    const bool few = wide || ctx->nact <= ctx->few_rows;
+    const bool few = ctx->nact <= ctx->few_rows;
     // the row walks drop stencil cells beyond reach: they need each cell's largest radius
     if (wide) CHK(build_wide(ctx, true));
     else CHK(build_grid(ctx, std::min(hmax, 0.5 * ctx->par.box_size), few));
-    if (!few) CHK(build_cell_rows(ctx));
+    if (wide && !few) CHK(build_groups(ctx, nullptr, nullptr));
+    if (!wide && !few) CHK(build_cell_rows(ctx));
     const long need_ld = ((long)ctx->nact + 63) / 64 * 64;
     const int cap_hint = std::max(64, (int)std::ceil(ctx->par.des_num_ngb * 2.0) + 32);
     if (ctx->ell_s == nullptr || ctx->ld_s != need_ld || ctx->cap_s < cap_hint) {
@@ -774,7 +824,11 @@ int ensure_sym(gz_context *ctx)
         HIPCHK(hipMemsetAsync(ctx->counters + 3, 0, sizeof(int), ctx->stream));
         {
             Timed t(ctx, "ngb_build");
-            if (wide)
+            if (wide && !few)
+                k_build_sym_cells<<<xcd_grid(ctx->grp.ngroups), 64, 0, ctx->stream>>>(grid_view(ctx), ctx->grp.view(BoxTable{}), ctx->act,
+                                                                                       ctx->X, ctx->Y, ctx->Z, ctx->H, ctx->ell_s, ctx->ld_s,
+                                                                                       ctx->cap_s, ctx->cnt_s, ctx->counters + 3);
+            else if (wide)
                 k_build_sym_wide<<<xcd_grid(ctx->nact), 64, 0, ctx->stream>>>(grid_view(ctx), ctx->nact, ctx->act, ctx->X, ctx->Y, ctx->Z,
                                                                                ctx->H, ctx->ell_s, ctx->ld_s, ctx->cap_s, ctx->cnt_s,
                                                                                ctx->counters + 3);
@@ -1563,14 +1617,20 @@ long gz_neighbors(gz_context *ctx, int symmetric, long *offsets, int *nbr, long 
     if (symmetric && ctx->wide) {
         // the search over many levels (the list the gradient and force passes use)
         rc = build_wide(ctx, true);
+        if (!rc && ctx->nact > ctx->few_rows) rc = build_groups(ctx, nullptr, nullptr);
         ld = ((long)ctx->nact + 63) / 64 * 64;
         capl = 64;
         for (int attempt = 0; !rc && attempt < 12; attempt++) {
             rc = dalloc(ctx, &ell, (size_t)capl * ld);
             if (rc) break;
             if (hipMemsetAsync(ctx->counters + 3, 0, sizeof(int), ctx->stream) != hipSuccess) { rc = GZ_E_HIP; break; }
-            k_build_sym_wide<<<xcd_grid(ctx->nact), 64, 0, ctx->stream>>>(grid_view(ctx), ctx->nact, ctx->act, ctx->X, ctx->Y, ctx->Z,
-                                                                           ctx->H, ell, ld, capl, cnt, ctx->counters + 3);
+            if (ctx->nact > ctx->few_rows)
+                k_build_sym_cells<<<xcd_grid(ctx->grp.ngroups), 64, 0, ctx->stream>>>(grid_view(ctx), ctx->grp.view(BoxTable{}), ctx->act,
+                                                                                       ctx->X, ctx->Y, ctx->Z, ctx->H, ell, ld, capl, cnt,
+                                                                                       ctx->counters + 3);
+            else
+                k_build_sym_wide<<<xcd_grid(ctx->nact), 64, 0, ctx->stream>>>(grid_view(ctx), ctx->nact, ctx->act, ctx->X, ctx->Y, ctx->Z,
+                                                                               ctx->H, ell, ld, capl, cnt, ctx->counters + 3);
             if (hipGetLastError() != hipSuccess) { rc = GZ_E_HIP; break; }
             rc = read_counters(ctx);
             if (rc || ctx->hcounters[3] == 0) break;

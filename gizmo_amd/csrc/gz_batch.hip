@@ -33,6 +33,7 @@
 
 #include "../../include/gizmo_batch.h"
 #include "gz_kernels.h"
+#include "gz_groups.h"
 
 using namespace gz;
 
@@ -101,6 +102,11 @@ struct gz_batch {
     std::vector<BoxHost> boxes;
     std::vector<int> order;  // live boxes in storage (rank) order
     int nact = 0;
+    // passes over more active rows than this walk the rows grouped by box cell (one wavefront
+    // per group), fewer take one wavefront per row
+    int few_rows = 8192;
+    RowGroups grp;
+    bool grp_valid = false;
     bool act_rows_valid = false;
     std::map<std::string, double> stats;
 };
@@ -250,6 +256,12 @@ void free_all(gz_batch *c)
     bfree(c->key); bfree(c->key64); bfree(c->key64b); bfree(c->rec_grad); bfree(c->rec_force); bfree(c->tab);
     bfree(c->hist); bfree(c->slot); bfree(c->u64);
     bfree(c->tmp);
+    {
+        RowGroups &G = c->grp;
+        bfree(G.key); bfree(G.skey); bfree(G.gkey); bfree(G.val); bfree(G.row); bfree(G.flag); bfree(G.scan); bfree(G.start);
+        bfree(G.count); bfree(G.lev); bfree(G.code); bfree(G.nst); bfree(G.sten);
+        G = RowGroups{};
+    }
     unsigned long long **hp[] = {&c->hhist, &c->hslot, &c->hu64};
     for (auto p : hp) {
         if (*p) (void)hipHostFree(*p);
@@ -435,6 +447,7 @@ int build_grid(gz_batch *b)
     off += 1;
     if (off != b->cells_total) changed = true;
     BCHK(upload_grid(b));
+    if (changed) b->grp_valid = false;
     if (!changed) return 0;
     if (off > b->cells_alloc) {
         BCHK(balloc(b, &b->cstart, off)); BCHK(balloc(b, &b->cend, off)); BCHK(balloc(b, &b->chmax, off));
@@ -445,6 +458,37 @@ int build_grid(gz_batch *b)
     k_batch_cell_bounds<<<nb(b->cap), kB, 0, b->stream>>>(b->cap, b->key, b->BOX, b->tab, b->retired, b->cstart, b->cend);
     BHIP(hipGetLastError());
     b->grid_valid = true;
+    return 0;
+}
+
+// The active rows grouped by the cell of their box's grid (rows come in storage order, so the
+// groups need no sort); kept until the rows, the order or a grid change
+int build_groups(gz_batch *b)
+{
+    if (b->grp_valid) return 0;
+    RowGroups &G = b->grp;
+    const int na = b->nact;
+    if (G.rows_alloc < (long)na + 1) {
+        const size_t m = (size_t)b->cap + 1;
+        BCHK(balloc(b, &G.key, m)); BCHK(balloc(b, &G.gkey, m)); BCHK(balloc(b, &G.flag, m)); BCHK(balloc(b, &G.scan, m));
+        BCHK(balloc(b, &G.start, m)); BCHK(balloc(b, &G.count, 1));
+        G.rows_alloc = (long)m;
+    }
+    size_t bytes = 0;
+    BHIP(group_tmp_bytes(na, bytes));
+    BCHK(ensure_tmp(b, bytes));
+    const BoxTable bt = table(b);
+    BHIP(group_rows(G, b->stream, b->tmp, b->tmp_bytes, na, b->act, nullptr, b->key, nullptr, b->H, 0.0, 0.0, nullptr, bt, false));
+    BHIP(hipMemcpyAsync(b->hcounters + 12, G.count, sizeof(int), hipMemcpyDeviceToHost, b->stream));
+    BCHK(sync(b));
+    G.ngroups = b->hcounters[12];
+    if (G.groups_alloc < G.ngroups) {
+        const long m = std::min((long)na, (long)G.ngroups + G.ngroups / 4 + 64);
+        BCHK(balloc(b, &G.lev, m)); BCHK(balloc(b, &G.code, m)); BCHK(balloc(b, &G.nst, m)); BCHK(balloc(b, &G.sten, (size_t)27 * m));
+        G.groups_alloc = m;
+    }
+    BHIP(group_stencils(G, b->stream, gview(b), b->act, bt));
+    b->grp_valid = true;
     return 0;
 }
 
@@ -459,6 +503,7 @@ int density(gz_batch *b)
     const int na = b->nact;
     if (na == 0) return 0;
     const BoxTable bt = table(b);
+    b->grp_valid = false;  // this pass's rows
     // cells of this step's order, radius bound and displacement bound
     BCHK(build_grid(b));
     BHIP(hipMemsetAsync(b->slot, 0, kBoxSlots * nbox_all(b) * 8, b->stream));
@@ -470,8 +515,15 @@ int density(gz_batch *b)
     int iters = 0;
     while (true) {
         BHIP(hipMemsetAsync(b->counters, 0, 3 * sizeof(int), b->stream));
-        k_density_rows<KERN, R><<<xcd_grid(na), 64, 0, b->stream>>>(b->dp, gview(b), na, b->act, b->need, b->key, b->X, b->Y, b->Z, b->VX,
-                                                                     b->VY, b->VZ, b->M, b->it_h, b->dacc, bt);
+        if (na > b->few_rows) {
+            BCHK(build_groups(b));
+            k_density_cells<KERN, R><<<xcd_grid(b->grp.ngroups), 64, 0, b->stream>>>(b->dp, gview(b), b->grp.view(bt), b->act, b->need,
+                                                                                      b->X, b->Y, b->Z, b->VX, b->VY, b->VZ, b->M,
+                                                                                      b->it_h, b->dacc);
+        } else {
+            k_density_rows<KERN, R><<<xcd_grid(na), 64, 0, b->stream>>>(b->dp, gview(b), na, b->act, b->need, b->key, b->X, b->Y, b->Z,
+                                                                         b->VX, b->VY, b->VZ, b->M, b->it_h, b->dacc, bt);
+        }
         k_density_after<<<nb(na), kB, 0, b->stream>>>(b->dp, na, b->act, b->need, b->dacc, b->M, b->U, 0, it, out, b->counters + 0,
                                                       b->counters + 1, b->counters + 2, b->u64 + 1, bt);
         BHIP(hipGetLastError());
@@ -513,8 +565,14 @@ int sym_list(gz_batch *b, const std::vector<uint8_t> &in_pass)
     }
     for (int attempt = 0;; attempt++) {
         BHIP(hipMemsetAsync(b->counters + 3, 0, sizeof(int), b->stream));
-        k_build_sym_rows<<<xcd_grid(na), 64, 0, b->stream>>>(gview(b), na, b->act, b->key, b->X, b->Y, b->Z, b->H, b->ell, ld, b->ell_cap,
-                                                              b->cnt, b->counters + 3, bt);
+        if (na > b->few_rows) {
+            BCHK(build_groups(b));
+            k_build_sym_cells<<<xcd_grid(b->grp.ngroups), 64, 0, b->stream>>>(gview(b), b->grp.view(bt), b->act, b->X, b->Y, b->Z, b->H,
+                                                                               b->ell, ld, b->ell_cap, b->cnt, b->counters + 3);
+        } else {
+            k_build_sym_rows<<<xcd_grid(na), 64, 0, b->stream>>>(gview(b), na, b->act, b->key, b->X, b->Y, b->Z, b->H, b->ell, ld,
+                                                                  b->ell_cap, b->cnt, b->counters + 3, bt);
+        }
         BHIP(hipGetLastError());
         BCHK(read_back(b, false));
         const int over = b->hcounters[3];
@@ -1057,6 +1115,17 @@ int gz_batch_stat(gz_batch *b, const char *name, double *value)
     auto it = b->stats.find(name);
     *value = it == b->stats.end() ? 0 : it->second;
     return 0;
+}
+
+int gz_batch_set_option(gz_batch *b, const char *name, double value)
+{
+    if (!b || !name) return GZ_E_ARG;
+    if (!strcmp(name, "few_rows")) {
+        if (!(value >= 0) || value > 2e9) return fail(b, GZ_E_ARG, "few_rows must be in [0, 2e9]");
+        b->few_rows = (int)value;
+        return 0;
+    }
+    return fail(b, GZ_E_ARG, "unknown option %s", name);
 }
 
 int gz_batch_timing_reset(gz_batch *b)

@@ -93,6 +93,12 @@ struct CellRows {             // occupied cells of the active rows
     const int *ncells_dev;    // the count on the device (nullptr: ncells is exact)
     const int *ca0, *ca1;     // per cell code: [first, last+1) slot of its active rows
     int nrows;                // number of active rows (column stride of per-row sums)
+    // Groups (gsten non-null; search over many levels, batches of boxes): wavefront cb walks
+    // group cb, the rows grow[gstart[cb] .. gstart[cb+1]) (slots) sharing one cell of one grid:
+    // level glev[cb], code gcode[cb], stencil gsten[27*cb ..) of gnst[cb] ranges ascending
+    const int *grow, *gstart, *glev, *gcode, *gnst;
+    const int4 *gsten;
+    BoxTable bt;              // batches: the group's box geometry
 };
 
 struct DensityIter {          // per active slot
@@ -158,6 +164,13 @@ __global__ void k_active_from_bins(long n, unsigned long long mask, const int *T
                                    uint8_t *ACT, double *SOFT);
 __global__ void k_mark_records(int nrows, const int *act, const int *ell, long ld, const int *cnt, uint8_t *need);
 __global__ void k_sum_counts(int nrows, const int *cnt, unsigned long long *out);
+// groups of rows sharing one cell of one grid (the search over many levels, batches of boxes)
+__global__ void k_row_group_key(int nrows, const int *act, const uint8_t *need, const uint32_t *key, const double *hslot,
+                                const double *H, double box, double margin, const WideGrid *W, BoxTable bt, uint32_t *gkey, int *val);
+__global__ void k_group_heads(int nrows, const uint32_t *skey, int *flag);
+__global__ void k_group_list(int nrows, const uint32_t *skey, const int *flag, const int *scan, int *gstart, uint32_t *gkey, int *ngroups);
+__global__ void k_group_stencil(GridView g, int ngroups, const uint32_t *gkey, const int *gstart, const int *grow, const int *act,
+                                BoxTable bt, int *glev, int *gcode, int *gnst, int4 *gsten);
 
 // search over many grid levels
 __global__ void k_wide_bounds(WideGrid W);

@@ -548,13 +548,24 @@ __global__ void __launch_bounds__(64) k_density_cells(DevParams P, GridView g, C
     __shared__ double sx[64], sy[64], sz[64], svx[64], svy[64], svz[64], sm[64];
     const long cb = xcd_block();
     if (cb >= (cr.ncells_dev ? *cr.ncells_dev : cr.ncells)) return;
-    const int c = cr.cells[cb];
-    const int a0 = cr.ca0[c], a1 = cr.ca1[c];
-    const int nst = g.nsten[c];
+    int c, a0, a1, nst;
+    const int4 *sten;
+    if (cr.gsten) {
+        c = cr.gcode[cb]; a0 = cr.gstart[cb]; a1 = cr.gstart[cb + 1]; nst = cr.gnst[cb]; sten = cr.gsten + (size_t)27 * cb;
+        if (cr.bt.b) {
+            resolve_box(cr.bt, act[cr.grow ? cr.grow[a0] : a0], P, g);
+        } else {
+            g.level = cr.glev[cb];
+            g.cell = g.box / (double)(1 << g.level);
+        }
+    } else {
+        c = cr.cells[cb]; a0 = cr.ca0[c]; a1 = cr.ca1[c]; nst = g.nsten[c]; sten = g.sten + (size_t)c * 27;
+    }
     const int t = threadIdx.x;
     for (int base = a0; base < a1; base += 64) {
-        const int r = base + t;
-        const bool on = (r < a1) && need[r];
+        const bool in = base + t < a1;
+        const int r = in ? (cr.grow ? cr.grow[base + t] : base + t) : 0;
+        const bool on = in && need[r];
         if (!wave_any(on)) continue;
         int i = 0;
         double xi = 0, yi = 0, zi = 0, h = 0;
@@ -574,7 +585,7 @@ __global__ void __launch_bounds__(64) k_density_cells(DevParams P, GridView g, C
         R hinv, hinv3, hinv4;
         kernel_hinv(hR, hinv, hinv3, hinv4);
         for (int q = 0; q < nst; q++) {
-            const int4 e = g.sten[(size_t)c * 27 + q];
+            const int4 e = sten[q];
             for (int ts = e.x; ts < e.y; ts += 64) {
                 const int j = ts + t;
                 bool keep = false;
@@ -752,69 +763,6 @@ __global__ void __launch_bounds__(256) k_density_after(DevParams P, int nrows, c
     }
     wave_max_to(h_final, row, box, hfinal, bt, kSlotHfinal);
     if (bt.b && gm) wave_max_to(h_grow, row, box, nullptr, bt, kSlotHgrow);
-}
-#endif
-
-
-// ---------------------------------------------------------------------------------------
-// symmetric list as a cell walk (one wavefront per occupied cell, culled LDS-staged candidates):
-// rows r2 < max(h_i,h_j)^2, self excluded, ascending (src/mesh/sfc_tiles_functions.h:83-102)
-// ---------------------------------------------------------------------------------------
-#ifdef GZ_DEFINE_COMMON_KERNELS
-__global__ void __launch_bounds__(64) k_build_sym_cells(GridView g, CellRows cr, const int *__restrict__ act,
-                                                        const double *__restrict__ X, const double *__restrict__ Y,
-                                                        const double *__restrict__ Z, const double *__restrict__ H,
-                                                        int *__restrict__ ell, long ld, int cap, int *__restrict__ cnt,
-                                                        int *__restrict__ overflow)
-{
-    __shared__ double sx[64], sy[64], sz[64], sh[64];
-    __shared__ int sj[64];
-    const long cb = xcd_block();
-    if (cb >= (cr.ncells_dev ? *cr.ncells_dev : cr.ncells)) return;
-    const int c = cr.cells[cb];
-    const int a0 = cr.ca0[c], a1 = cr.ca1[c];
-    const int nst = g.nsten[c];
-    const int t = threadIdx.x;
-    for (int base = a0; base < a1; base += 64) {
-        const int r = base + t;
-        const bool on = r < a1;
-        int i = -1;
-        double xi = 0, yi = 0, zi = 0, hi = 0;
-        if (on) { i = act[r]; xi = X[i]; yi = Y[i]; zi = Z[i]; hi = H[i]; }
-        const LeafBox lb = leaf_box(g, c, on, xi, yi, zi);
-        const double hleaf = wave_max(on ? hi : 0.0);
-        int n = 0;
-        for (int q = 0; q < nst; q++) {
-            const int4 e = g.sten[(size_t)c * 27 + q];
-            for (int ts = e.x; ts < e.y; ts += 64) {
-                const int j = ts + t;
-                bool keep = false;
-                double xj = 0, yj = 0, zj = 0, hj = 0;
-                if (j < e.y) {
-                    xj = X[j]; yj = Y[j]; zj = Z[j]; hj = H[j];
-                    keep = leaf_reach(g, lb, xj, yj, zj, fmax(hleaf, hj));
-                }
-                int nt;
-                const int pos = kept_rank(keep, nt);
-                __syncthreads();
-                if (keep) { sx[pos] = xj; sy[pos] = yj; sz[pos] = zj; sh[pos] = hj; sj[pos] = j; }
-                __syncthreads();
-                if (!on) continue;
-                for (int k = 0; k < nt; k++) {
-                    const int jj = sj[k];
-                    if (jj == i) continue;
-                    if (accept_pair<true>(xi, yi, zi, sx[k], sy[k], sz[k], hi, sh[k], g.box, g.half)) {
-                        if (n < cap) ell[(size_t)n * ld + r] = jj;
-                        n++;
-                    }
-                }
-            }
-        }
-        if (on) {
-            cnt[r] = n;
-            if (n > cap) atomicMax(overflow, n);
-        }
-    }
 }
 #endif
 

@@ -449,6 +449,211 @@ __global__ void __launch_bounds__(64) k_build_sym_wide(GridView g, int nrows, co
 }
 #endif
 
+// ---------------------------------------------------------------------------------------
+// symmetric list as a cell walk (one wavefront per occupied cell, culled LDS-staged candidates):
+// rows r2 < max(h_i,h_j)^2, self excluded, ascending (src/mesh/sfc_tiles_functions.h:83-102);
+// with row groups (CellRows.gsten) a group is one cell of one grid level or of one box
+// ---------------------------------------------------------------------------------------
+#ifdef GZ_DEFINE_COMMON_KERNELS
+__global__ void __launch_bounds__(64) k_build_sym_cells(GridView g, CellRows cr, const int *__restrict__ act,
+                                                        const double *__restrict__ X, const double *__restrict__ Y,
+                                                        const double *__restrict__ Z, const double *__restrict__ H,
+                                                        int *__restrict__ ell, long ld, int cap, int *__restrict__ cnt,
+                                                        int *__restrict__ overflow)
+{
+    __shared__ double sx[64], sy[64], sz[64], sh[64];
+    __shared__ int sj[64];
+    __shared__ RowStencil rs;
+    const long cb = xcd_block();
+    if (cb >= (cr.ncells_dev ? *cr.ncells_dev : cr.ncells)) return;
+    int c, a0, a1, nst;
+    const int4 *sten;
+    if (cr.gsten) {
+        c = cr.gcode[cb]; a0 = cr.gstart[cb]; a1 = cr.gstart[cb + 1]; nst = cr.gnst[cb]; sten = cr.gsten + (size_t)27 * cb;
+        if (cr.bt.b) {
+            DevParams P{};
+            resolve_box(cr.bt, act[cr.grow ? cr.grow[a0] : a0], P, g);
+        } else {
+            g.level = cr.glev[cb];
+            g.cell = g.box / (double)(1 << g.level);
+        }
+    } else {
+        c = cr.cells[cb]; a0 = cr.ca0[c]; a1 = cr.ca1[c]; nst = g.nsten[c]; sten = g.sten + (size_t)c * 27;
+    }
+    // the search over many levels adds part B (its groups are cells of the rows' own levels)
+    const bool wide = cr.gsten && !cr.bt.b && g.wide;
+    const int t = threadIdx.x;
+    for (int base = a0; base < a1; base += 64) {
+        const bool on = base + t < a1;
+        const int r = on ? (cr.grow ? cr.grow[base + t] : base + t) : 0;
+        int i = -1;
+        double xi = 0, yi = 0, zi = 0, hi = 0;
+        if (on) { i = act[r]; xi = X[i]; yi = Y[i]; zi = Z[i]; hi = H[i]; }
+        const LeafBox lb = leaf_box(g, c, on, xi, yi, zi);
+        const double hleaf = wave_max(on ? hi : 0.0);
+        int n = 0;
+        for (int q = 0; q < nst; q++) {
+            const int4 e = sten[q];
+            for (int ts = e.x; ts < e.y; ts += 64) {
+                const int j = ts + t;
+                bool keep = false;
+                double xj = 0, yj = 0, zj = 0, hj = 0;
+                if (j < e.y) {
+                    xj = X[j]; yj = Y[j]; zj = Z[j]; hj = H[j];
+                    keep = leaf_reach(g, lb, xj, yj, zj, fmax(hleaf, hj));
+                }
+                int nt;
+                const int pos = kept_rank(keep, nt);
+                __syncthreads();
+                if (keep) { sx[pos] = xj; sy[pos] = yj; sz[pos] = zj; sh[pos] = hj; sj[pos] = j; }
+                __syncthreads();
+                if (!on) continue;
+                for (int k = 0; k < nt; k++) {
+                    const int jj = sj[k];
+                    if (jj == i) continue;
+                    if (accept_pair<true>(xi, yi, zi, sx[k], sy[k], sz[k], hi, sh[k], g.box, g.half)) {
+                        if (n < cap) ell[(size_t)n * ld + r] = jj;
+                        n++;
+                    }
+                }
+            }
+        }
+        if (wide) {
+            // Part B of the search over many levels (as k_build_sym_wide): the particles of each
+            // coarser level L in the 27 cells of level L around the group's cell that lie outside
+            // the 27 cells of the group's level, each accepted one inserted in ascending place
+            const int nci = 1 << g.level;
+            const int hx = compact10((uint32_t)c >> 2), hy = compact10((uint32_t)c >> 1), hz = compact10((uint32_t)c);
+            const int si = 3 * (kFineLevel - g.level);
+            const WideGrid &W = *g.wide;
+            for (int L = 0; L < g.level; L++) {
+                if (W.lofs[L + 1] == W.lofs[L]) continue;  // no particle of this level
+                __syncthreads();  // every lane is done with rs before it is built again
+                row_stencil_wide<true>(g, W, L, (uint32_t)c >> (3 * (g.level - L)), 0.0, 0.0, 0.0, -1.0, rs);
+                const int total = rs.pref[rs.nst];
+                for (int base2 = 0; base2 < total; base2 += 64) {
+                    const int v = base2 + t;
+                    bool keep = false;
+                    int j = -1;
+                    double xj = 0, yj = 0, zj = 0, hj = 0;
+                    if (v < total) {
+                        j = W.lidx[row_candidate(rs, v)];
+                        const uint32_t cj = W.key[j] >> si;
+                        const bool in_a = periodic_adjacent((int)compact10(cj >> 2), hx, nci) &&
+                                          periodic_adjacent((int)compact10(cj >> 1), hy, nci) && periodic_adjacent((int)compact10(cj), hz, nci);
+                        if (!in_a) {
+                            xj = X[j]; yj = Y[j]; zj = Z[j]; hj = H[j];
+                            keep = leaf_reach(g, lb, xj, yj, zj, fmax(hleaf, hj));
+                        }
+                    }
+                    int nt;
+                    const int pos = kept_rank(keep, nt);
+                    __syncthreads();
+                    if (keep) { sx[pos] = xj; sy[pos] = yj; sz[pos] = zj; sh[pos] = hj; sj[pos] = j; }
+                    __syncthreads();
+                    if (!on) continue;
+                    for (int k = 0; k < nt; k++) {
+                        const int jj = sj[k];
+                        if (jj == i || !accept_pair<true>(xi, yi, zi, sx[k], sy[k], sz[k], hi, sh[k], g.box, g.half)) continue;
+                        if (n < cap) {
+                            int q = n;
+                            while (q > 0 && ell[(size_t)(q - 1) * ld + r] > jj) { ell[(size_t)q * ld + r] = ell[(size_t)(q - 1) * ld + r]; q--; }
+                            ell[(size_t)q * ld + r] = jj;
+                        }
+                        n++;
+                    }
+                }
+            }
+        }
+        if (on) {
+            cnt[r] = n;
+            if (n > cap) atomicMax(overflow, n);
+        }
+    }
+}
+#endif
+
+
+// ---------------------------------------------------------------------------------------
+// groups of rows sharing one cell of one grid, for the cell walks of the search over many
+// levels (the cell of the level of each row's radius) and of batches (the cell of the row's
+// box): key = the level's or the box's first table entry + the cell code
+// ---------------------------------------------------------------------------------------
+#ifdef GZ_DEFINE_COMMON_KERNELS
+__global__ void __launch_bounds__(256) k_row_group_key(int nrows, const int *__restrict__ act, const uint8_t *__restrict__ need,
+                                                        const uint32_t *__restrict__ key, const double *__restrict__ hslot,
+                                                        const double *__restrict__ H, double box, double margin,
+                                                        const WideGrid *__restrict__ W, BoxTable bt, uint32_t *__restrict__ gkey,
+                                                        int *__restrict__ val)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nrows) return;
+    if (val) val[r] = r;
+    if (need && !need[r]) { gkey[r] = 0xffffffffu; return; }  // not in this pass: after every group
+    const int i = act[r];
+    if (bt.b) {
+        const BoxDev &B = bt.b[bt.BOX[i]];
+        gkey[r] = (uint32_t)(B.cell_off + (key[i] >> B.shift));
+    } else {
+        const double h = hslot ? hslot[r] : H[i];
+        const int L = wide_level(box, W->lmax, h + 2.0 * margin);
+        gkey[r] = (uint32_t)(W->loff[L] + (key[i] >> (3 * (kFineLevel - L))));
+    }
+}
+
+// group heads of the sorted keys (flag[nrows] = 0 for the scan)
+__global__ void __launch_bounds__(256) k_group_heads(int nrows, const uint32_t *__restrict__ skey, int *__restrict__ flag)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p > nrows) return;
+    flag[p] = (p < nrows && skey[p] != 0xffffffffu && (p == 0 || skey[p - 1] != skey[p])) ? 1 : 0;
+}
+
+// group starts (positions in the sorted rows), keys and count; gstart[count] = the number of
+// rows in groups
+__global__ void __launch_bounds__(256) k_group_list(int nrows, const uint32_t *__restrict__ skey, const int *__restrict__ flag,
+                                                     const int *__restrict__ scan, int *__restrict__ gstart, uint32_t *__restrict__ gkey,
+                                                     int *__restrict__ ngroups)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= nrows) return;
+    if (flag[p]) { gstart[scan[p]] = p; gkey[scan[p]] = skey[p]; }
+    const bool last = skey[p] != 0xffffffffu && (p + 1 == nrows || skey[p + 1] == 0xffffffffu);
+    if (last) { gstart[scan[p] + flag[p]] = p + 1; *ngroups = scan[p] + flag[p]; }
+    if (p == 0 && skey[0] == 0xffffffffu) { gstart[0] = 0; *ngroups = 0; }
+}
+
+// the stencil of every group (one wavefront each): the 27 cells around its cell at its level,
+// duplicates dropped, ascending code (ascending start)
+__global__ void __launch_bounds__(64) k_group_stencil(GridView g, int ngroups, const uint32_t *__restrict__ gkey,
+                                                       const int *__restrict__ gstart, const int *__restrict__ grow,
+                                                       const int *__restrict__ act, BoxTable bt, int *__restrict__ glev,
+                                                       int *__restrict__ gcode, int *__restrict__ gnst, int4 *__restrict__ gsten)
+{
+    __shared__ RowStencil rs;
+    const long gb = blockIdx.x;
+    if (gb >= ngroups) return;
+    const uint32_t k = gkey[gb];
+    int L, c;
+    if (bt.b) {
+        DevParams P{};
+        resolve_box(bt, act[grow ? grow[gstart[gb]] : gstart[gb]], P, g);
+        L = g.level;
+        c = (int)(k - g.cell_off);
+        row_stencil<false>(g, (uint32_t)c, 0.0, 0.0, 0.0, 1e300, rs);
+    } else {
+        const WideGrid &W = *g.wide;
+        L = 0;
+        while (L < W.lmax && (long long)k >= W.loff[L + 1]) L++;
+        c = (int)(k - W.loff[L]);
+        row_stencil_wide<false>(g, W, L, (uint32_t)c, 0.0, 0.0, 0.0, -1.0, rs);
+    }
+    const int lane = threadIdx.x;
+    if (lane < rs.nst) gsten[(size_t)27 * gb + lane] = make_int4(rs.start[lane], rs.start[lane] + rs.len[lane], 0, 0);
+    if (lane == 0) { glev[gb] = L; gcode[gb] = c; gnst[gb] = rs.nst; }
+}
+#endif
+
 // gradients of few rows: the terms of each list entry in LDS, accumulated in list order by lane 0
 template <int KERN, typename R>
 __global__ void __launch_bounds__(64) k_gradients_rows(DevParams P, int nrows, const int *__restrict__ act, const double *__restrict__ X,

@@ -65,6 +65,11 @@ int gz_batch_synchronize(gz_batch *b);
    memory the batch holds now. */
 int gz_batch_stat(gz_batch *b, const char *name, double *value);
 int gz_batch_timing_reset(gz_batch *b);
+/* Options: "few_rows" (default 8192) -- passes over at most this many active rows (all boxes
+   together) take one wavefront per row; more rows are walked in groups that share a cell of
+   their box's grid (the same lists and sums either way).  No counterpart in the reference (a
+   tuning knob of this library, as gz_set_option's "few_rows"). */
+int gz_batch_set_option(gz_batch *b, const char *name, double value);
 
 #ifdef __cplusplus
 }

@@ -92,7 +92,11 @@ def run_alone(kind, steps):
     return tls, out
 
 
-def test_mix_equals_alone_with_joins_and_leaves():
+# few_rows None: the default split (passes over more than 8192 rows walk row groups, one
+# wavefront per cell of each box's grid); 0: every pass in row groups.  The boxes alone take one
+# wavefront per row (each holds at most 8000 particles).
+@pytest.mark.parametrize("few_rows", [None, 0])
+def test_mix_equals_alone_with_joins_and_leaves(few_rows):
     # sedov20 and lattice14 from the start; sedov12_b2 joins after 5 steps; lattice14 leaves
     # after 12; sedov16 joins after 15 into the room it left
     steps = {"sedov20": 30, "lattice14": 12, "sedov12_b2": 25, "sedov16": 15}
@@ -101,6 +105,8 @@ def test_mix_equals_alone_with_joins_and_leaves():
     size = {k: len(box_ic(k)[0]["mass"]) for k in steps}
     cap = max(size["sedov20"] + size["lattice14"] + size["sedov12_b2"], size["sedov20"] + size["sedov12_b2"] + size["sedov16"])
     b = Batch(par, cap, 3, GZ_FP64)
+    if few_rows is not None:
+        b.set_option("few_rows", few_rows)
     ids, done, tls = {}, {}, {k: [] for k in steps}
 
     def join(kind):
@@ -117,7 +123,7 @@ def test_mix_equals_alone_with_joins_and_leaves():
         if t == 15:
             join("sedov16")
             b.begin()
-        print(f"mix step {t}: boxes={b.boxes()} particles={b.particles()}", flush=True)
+        print(f"mix step {t}: few_rows={few_rows} boxes={b.boxes()} particles={b.particles()}", flush=True)
         b.step()
         for kind, bid in list(ids.items()):
             tls[kind].append(b.timeline(bid))

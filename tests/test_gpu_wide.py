@@ -33,9 +33,11 @@ def wide_case(oracle_lib):
     return dict(ic=ic, par=par, s1=s1, s3=s3)
 
 
-def device(ic, par, wide, precision=GZ_FP64):
+def device(ic, par, wide, precision=GZ_FP64, few_rows=None):
     g = Hydro(par, len(ic["mass"]), precision)
     g.set_option("wide_search", wide)
+    if few_rows is not None:
+        g.set_option("few_rows", few_rows)
     g.load(dict(pos=ic["pos"], vel=ic["vel"], mass=ic["mass"], u=ic["u"], h=ic["h"]))
     return g
 
@@ -45,31 +47,40 @@ def test_radii_span_levels(wide_case):
     assert h.max() / h.min() > 10.0
 
 
-def test_symmetric_sets_exact(wide_case, oracle_lib):
+# few_rows 0: every row in the cell walk of row groups (one wavefront per cell of each row's own
+# level); 10**9: one wavefront per row
+@pytest.mark.parametrize("few_rows", [0, 10**9])
+def test_symmetric_sets_exact(wide_case, oracle_lib, few_rows):
     s1, par = wide_case["s1"], wide_case["par"]
     g = Hydro(par, s1.n, GZ_FP64)
     g.set_option("wide_search", 1)
+    g.set_option("few_rows", few_rows)
     g.load(dict(pos=s1.pos, vel=s1.vel, mass=s1.mass, u=s1.u, h=s1.h))
-    print(f"wide sets: n={s1.n} h in [{s1.h.min():.2e}, {s1.h.max():.2e}]", flush=True)
+    print(f"wide sets: n={s1.n} few_rows={few_rows} h in [{s1.h.min():.2e}, {s1.h.max():.2e}]", flush=True)
     off_g, nb_g = g.neighbors(1)
     g.close()
     off_o, nb_o = oracle_lib.neighbors(par, s1.pos, s1.h, 1)
     assert np.array_equal(off_g, off_o)
     for i in range(s1.n):
-        assert np.array_equal(np.sort(nb_g[off_g[i]:off_g[i + 1]]), nb_o[off_o[i]:off_o[i + 1]]), i
+        row = nb_g[off_g[i]:off_g[i + 1]]
+        assert np.array_equal(np.sort(row), nb_o[off_o[i]:off_o[i + 1]]), i
 
 
 def test_full_pass_same_bits_as_one_grid(wide_case):
     ic, par = wide_case["ic"], wide_case["par"]
     outs = {}
-    for wide in (0, 1):
-        print(f"full pass: wide_search={wide}", flush=True)
-        g = device(ic, par, wide)
+    # one grid; many levels with the default split; every round and list in row groups; every
+    # round and list one wavefront per row
+    cases = {"one grid": (0, None), "wide": (1, None), "wide groups": (1, 0), "wide rows": (1, 10**9)}
+    for name, (wide, few) in cases.items():
+        print(f"full pass: {name}", flush=True)
+        g = device(ic, par, wide, few_rows=few)
         g.hydro_pass()
-        outs[wide] = g.store(FIELDS)
+        outs[name] = g.store(FIELDS)
         g.close()
-    for k in FIELDS:
-        assert np.array_equal(outs[0][k], outs[1][k]), k
+    for name in cases:
+        for k in FIELDS:
+            assert np.array_equal(outs["one grid"][k], outs[name][k]), (name, k)
 
 
 def test_full_pass_against_oracle(wide_case):
