@@ -78,8 +78,12 @@ struct gz_context {
     long long wcells_alloc = 0;
     long wide_key_version = -1;
     int wide_lmax = -1;
-    // many rows over many levels: rows grouped by the cell of their own level
+    // the symmetric list of many rows over many levels: rows grouped by the cell of their own level
     RowGroups grp;
+    // the force pass's row order over the lanes (longest lists first within blocks of rows)
+    int *sched = nullptr, *sched_val = nullptr;
+    uint32_t *sched_key = nullptr, *sched_key2 = nullptr;
+    bool sched_on = true, sched_valid = false;
 
     // staging and scratch
     double *stage = nullptr;  // n*9 doubles
@@ -339,6 +343,8 @@ void free_state(gz_context *c)
     dfree(c->wstart); dfree(c->wend); dfree(c->wlstart); dfree(c->wlend); dfree(c->lidx); dfree(c->plev); dfree(c->lcount);
     dfree(c->wide_dev);
     {
+        dfree(c->sched); dfree(c->sched_val); dfree(c->sched_key); dfree(c->sched_key2);
+        c->sched_valid = false;
         RowGroups &G = c->grp;
         dfree(G.key); dfree(G.skey); dfree(G.gkey); dfree(G.val); dfree(G.row); dfree(G.flag); dfree(G.scan); dfree(G.start);
         dfree(G.count); dfree(G.lev); dfree(G.code); dfree(G.nst); dfree(G.sten);
@@ -615,9 +621,9 @@ int build_wide(gz_context *ctx, bool lists)
     return 0;
 }
 
-// The active rows with need set (all when need is null) grouped by the cell of their own level
-// (radius from hslot per slot, else from H) for the cell walks over many levels
-int build_groups(gz_context *ctx, const uint8_t *need, const double *hslot)
+// The active rows grouped by the cell of their own level (from their radii) for the cell walk of
+// the symmetric list over many levels
+int build_groups(gz_context *ctx)
 {
     RowGroups &G = ctx->grp;
     const int na = ctx->nact;
@@ -632,8 +638,8 @@ int build_groups(gz_context *ctx, const uint8_t *need, const double *hslot)
     HIPCHK(group_tmp_bytes(na, bytes));
     CHK(ensure_tmp(ctx, bytes));
     Timed t(ctx, "ngb_build");
-    HIPCHK(group_rows(G, ctx->stream, ctx->tmp, ctx->tmp_bytes, na, ctx->act, need, ctx->key, hslot, ctx->H, ctx->par.box_size, ctx->dmax,
-                      ctx->wide_dev, BoxTable{}, true));
+    HIPCHK(group_rows(G, ctx->stream, ctx->tmp, ctx->tmp_bytes, na, ctx->act, nullptr, ctx->key, nullptr, ctx->H, ctx->par.box_size,
+                      ctx->dmax, ctx->wide_dev, BoxTable{}, true));
     HIPCHK(hipMemcpyAsync(ctx->hcounters + 12, G.count, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     G.ngroups = ctx->hcounters[12];
@@ -738,22 +744,16 @@ int density_stage(gz_context *ctx)
     else CHK(build_grid(ctx, std::min(hmax, 0.5 * ctx->par.box_size)));
     const bool few = wide || na <= ctx->few_rows;
     if (!few) CHK(build_cell_rows(ctx));
-    int remaining = na;  // rows of the coming round
     DensityIter it{ctx->it_h, ctx->it_hb, ctx->it_l, ctx->it_r, ctx->it_cn, ctx->it_ci, ctx->it_it};
     DensityOut out{ctx->n, ctx->H, ctx->NN, ctx->DRK, ctx->DIVV, ctx->RHO, ctx->PR, ctx->T, ctx->FW, ctx->CN, ctx->FCE};
     int iters = 0;
     while (true) {
         HIPCHK(hipMemsetAsync(ctx->counters, 0, 3 * sizeof(int), ctx->stream));
         {
-            // over many levels, a round of many rows walks the groups of their current radii
-            const bool groups = wide && remaining > ctx->few_rows;
-            if (groups) CHK(build_groups(ctx, ctx->need, ctx->it_h));
+            // over many levels the rounds take one wavefront per row whatever their number: the
+            // cells of the rows' own levels hold too few rows to fill a wavefront
             Timed t(ctx, "density");
-            if (groups)
-                k_density_cells<KERN, R><<<xcd_grid(ctx->grp.ngroups), 64, 0, ctx->stream>>>(
-                    ctx->dp, grid_view(ctx), ctx->grp.view(BoxTable{}), ctx->act, ctx->need, ctx->X, ctx->Y, ctx->Z, ctx->VX, ctx->VY,
-                    ctx->VZ, ctx->M, ctx->it_h, ctx->dacc);
-            else if (few)
+            if (few)
                 k_density_rows<KERN, R><<<xcd_grid(na), 64, 0, ctx->stream>>>(ctx->dp, grid_view(ctx), na, ctx->act, ctx->need, ctx->key,
                                                                             ctx->X, ctx->Y, ctx->Z, ctx->VX, ctx->VY, ctx->VZ, ctx->M,
                                                                             ctx->it_h, ctx->dacc, BoxTable{});
@@ -768,8 +768,7 @@ int density_stage(gz_context *ctx)
         }
         CHK(read_counters(ctx));
         iters++;
-        remaining = ctx->hcounters[0];
-        const int rebuild = ctx->hcounters[1], err = ctx->hcounters[2];
+        const int remaining = ctx->hcounters[0], rebuild = ctx->hcounters[1], err = ctx->hcounters[2];
         if (err) {
             ctx->stats["density_iterations"] = iters;
             if (err == GZ_STOP_MAXITER) return fail(ctx, GZ_STOP_MAXITER, "density(): kernel radius iteration did not converge in %d rounds", kMaxIter);
@@ -798,6 +797,31 @@ int density_stage(gz_context *ctx)
     return 0;
 }
 
+// the order in which the lanes of the force pass take the rows: longest symmetric list first
+// within each block of kSchedSpan rows (the gradient pass, cheaper per pair, keeps the plain
+// order: its scattered writes would cost more than the even waves save)
+int build_sched(gz_context *ctx)
+{
+    const int na = ctx->nact;
+    if (!ctx->sched) {
+        CHK(dalloc(ctx, &ctx->sched, ctx->cap)); CHK(dalloc(ctx, &ctx->sched_val, ctx->cap));
+        CHK(dalloc(ctx, &ctx->sched_key, ctx->cap)); CHK(dalloc(ctx, &ctx->sched_key2, ctx->cap));
+    }
+    int end_bit = 11;
+    while ((((long)na - 1) / kSchedSpan) >> (end_bit - 11)) end_bit++;
+    Timed t(ctx, "ngb_build");
+    k_sched_key<<<nblk(na), kBlock, 0, ctx->stream>>>(na, ctx->cnt_s, ctx->sched_key, ctx->sched_val);
+    HIPCHK(hipGetLastError());
+    size_t bytes = 0;
+    HIPCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, ctx->sched_key, ctx->sched_key2, ctx->sched_val, ctx->sched, na, 0, end_bit,
+                                              ctx->stream));
+    CHK(ensure_tmp(ctx, bytes));
+    HIPCHK(hipcub::DeviceRadixSort::SortPairs(ctx->tmp, bytes, ctx->sched_key, ctx->sched_key2, ctx->sched_val, ctx->sched, na, 0, end_bit,
+                                              ctx->stream));
+    ctx->sched_valid = true;
+    return 0;
+}
+
 int ensure_sym(gz_context *ctx)
 {
     if (ctx->sym_valid) return 0;
@@ -811,7 +835,7 @@ int ensure_sym(gz_context *ctx)
     // the row walks drop stencil cells beyond reach: they need each cell's largest radius
     if (wide) CHK(build_wide(ctx, true));
     else CHK(build_grid(ctx, std::min(hmax, 0.5 * ctx->par.box_size), few));
-    if (wide && !few) CHK(build_groups(ctx, nullptr, nullptr));
+    if (wide && !few) CHK(build_groups(ctx));
     if (!wide && !few) CHK(build_cell_rows(ctx));
     const long need_ld = ((long)ctx->nact + 63) / 64 * 64;
     const int cap_hint = std::max(64, (int)std::ceil(ctx->par.des_num_ngb * 2.0) + 32);
@@ -850,6 +874,8 @@ int ensure_sym(gz_context *ctx)
         if (over == 0) {
             ctx->sym_valid = true;
             ctx->sym_check_pending = ctx->defer_sym_check;
+            ctx->sched_valid = false;
+            if (!few && ctx->sched_on) CHK(build_sched(ctx));
             // the records the gradient and force passes read: every record when many rows are
             // active, else only those of the rows and their neighbours
             ctx->rec_need_valid = false;
@@ -887,7 +913,7 @@ int gradients_stage(gz_context *ctx)
     else
         k_gradients<KERN, R><<<xcd_grid(nblk(ctx->nact)), kBlock, 0, ctx->stream>>>(ctx->dp, ctx->nact, ctx->act, ctx->X, ctx->Y, ctx->Z,
                                                                                     rec, ctx->T, ctx->CN, ctx->n, ctx->ell_s, ctx->ld_s,
-                                                                                    ctx->cnt_s, ctx->GR, ctx->GP, ctx->GV);
+                                                                                    ctx->cnt_s, ctx->GR, ctx->GP, ctx->GV, BoxTable{});
     HIPCHK(hipGetLastError());
     ctx->gradients_done = true;
     return 0;
@@ -915,7 +941,7 @@ int force_stage(gz_context *ctx)
             k_force<KERN, R><<<xcd_grid(nblk(ctx->nact)), kBlock, 0, ctx->stream>>>(
                 ctx->dp, gam<R>(ctx->dp), ctx->nact, ctx->act, F, rec, ctx->ell_s, ctx->ld_s, ctx->cnt_s,
                 ctx->all_active ? nullptr : ctx->ACT, ctx->TB, ctx->MSV, ctx->WAKE, ctx->ACC, ctx->DTU, ctx->MSV, ctx->counters + 2,
-                ctx->counters + 4, ctx->u64c + 65);
+                ctx->counters + 4, ctx->u64c + 65, ctx->sched_valid ? ctx->sched : nullptr, BoxTable{});
         if (ctx->nact <= ctx->few_rows)
             k_sum_counts<<<std::min(nblk(ctx->nact), 1024u), kBlock, 0, ctx->stream>>>(ctx->nact, ctx->cnt_s, ctx->u64c + 65);
         HIPCHK(hipGetLastError());
@@ -1573,6 +1599,12 @@ int gz_set_option(gz_context *ctx, const char *name, double value)
         ctx->few_rows = (int)value;
         return 0;
     }
+    if (k == "row_schedule") {
+        if (value != 0 && value != 1) return fail(ctx, GZ_E_ARG, "row_schedule must be 0 or 1");
+        ctx->sched_on = value != 0;
+        ctx->sym_valid = false;
+        return 0;
+    }
     if (k == "wide_search") {
         if (value != 0 && value != 1) return fail(ctx, GZ_E_ARG, "wide_search must be 0 or 1");
         ctx->wide = (int)value;
@@ -1617,7 +1649,7 @@ long gz_neighbors(gz_context *ctx, int symmetric, long *offsets, int *nbr, long 
     if (symmetric && ctx->wide) {
         // the search over many levels (the list the gradient and force passes use)
         rc = build_wide(ctx, true);
-        if (!rc && ctx->nact > ctx->few_rows) rc = build_groups(ctx, nullptr, nullptr);
+        if (!rc && ctx->nact > ctx->few_rows) rc = build_groups(ctx);
         ld = ((long)ctx->nact + 63) / 64 * 64;
         capl = 64;
         for (int attempt = 0; !rc && attempt < 12; attempt++) {

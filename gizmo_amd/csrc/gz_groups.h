@@ -53,9 +53,9 @@ inline hipError_t group_tmp_bytes(int nrows, size_t &bytes)
     return e;
 }
 
-// keys (level cells from hslot[r] when given, else from H; box cells when bt.b), the sort when
-// sort is set (else the keys must not decrease along the rows), group heads, their scan and the
-// group list; the count is left in G.count on the device
+// keys (cells of the rows' own levels, radii from hslot[r] when given, else from H; box cells
+// when bt.b), the sort when sort is set (else the keys must not decrease along the rows), group
+// heads, their scan and the group list; the count is left in G.count on the device
 inline hipError_t group_rows(RowGroups &G, hipStream_t s, void *tmp, size_t tmp_bytes, int nrows, const int *act, const uint8_t *need,
                              const uint32_t *key, const double *hslot, const double *H, double box, double margin,
                              const WideGrid *W, BoxTable bt, bool sort)

@@ -247,7 +247,7 @@ __global__ void k_pack_force(double gamma, ForceView F, const uint8_t *need, Cel
 template <int KERN, typename R>
 __global__ void k_gradients(DevParams P, int nrows, const int *act, const double *X, const double *Y, const double *Z,
                             const GradRec<R> *rec, const double *T, const double *CN, long n, const int *ell, long ld,
-                            const int *cnt, double *GR, double *GP, double *GV);
+                            const int *cnt, double *GR, double *GP, double *GV, BoxTable bt);
 
 // one wavefront per active row, for passes over few rows
 template <int KERN, typename R>
@@ -269,6 +269,12 @@ __global__ void k_force_rows(DevParams P, Gam<R> G, int nrows, const int *act, F
 template <int KERN, typename R>
 __global__ void k_force(DevParams P, Gam<R> G, int nrows, const int *act, ForceView F, const Cell<R> *rec, const int *ell,
                         long ld, const int *cnt, const uint8_t *active, const int *timebin, const double *msv_in, int *wakeup,
-                        double *ACC, double *DTU, double *MSV, int *err, int *need_wakeup, unsigned long long *pairs);
+                        double *ACC, double *DTU, double *MSV, int *err, int *need_wakeup, unsigned long long *pairs,
+                        const int *sched, BoxTable bt);
+
+// the order in which the force pass's lanes take the rows: within each block of kSchedSpan
+// rows, longest list first (a wavefront runs as long as its longest row)
+constexpr int kSchedSpan = 2048;
+__global__ void k_sched_key(int nrows, const int *cnt, uint32_t *key, int *val);
 
 }  // namespace gz

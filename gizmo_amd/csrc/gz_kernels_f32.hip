@@ -21,10 +21,11 @@ namespace gz {
                                                       const double *, const double *, const double *, const double *, double *);\
     template __global__ void k_gradients<KERN, R>(DevParams, int, const int *, const double *, const double *, const double *,  \
                                                   const GradRec<R> *, const double *, const double *, long, const int *, long,  \
-                                                  const int *, double *, double *, double *);                                  \
+                                                  const int *, double *, double *, double *, BoxTable);                        \
     template __global__ void k_force<KERN, R>(DevParams, Gam<R>, int, const int *, ForceView, const Cell<R> *, const int *,     \
                                               long, const int *, const uint8_t *, const int *, const double *, int *, double *, \
-                                              double *, double *, int *, int *, unsigned long long *);                 \
+                                              double *, double *, int *, int *, unsigned long long *, const int *,     \
+                                              BoxTable);                                                               \
     template __global__ void k_density_rows<KERN, R>(DevParams, GridView, int, const int *, const uint8_t *, const uint32_t *,  \
                                                      const double *, const double *, const double *, const double *,            \
                                                      const double *, const double *, const double *, const double *, double *,  \

@@ -866,11 +866,15 @@ __global__ void __launch_bounds__(256) k_gradients(DevParams P, int nrows, const
                                                    const GradRec<R> *__restrict__ rec, const double *__restrict__ T,
                                                    const double *__restrict__ CN, long n, const int *__restrict__ ell, long ld,
                                                    const int *__restrict__ cnt, double *__restrict__ GR, double *__restrict__ GP,
-                                                   double *__restrict__ GV)
+                                                   double *__restrict__ GV, BoxTable bt)
 {
     const int r = (int)(xcd_block() * blockDim.x + threadIdx.x);
     if (r >= nrows) return;
     const int i = act[r];
+    {
+        GridView g{};
+        resolve_box(bt, i, P, g);
+    }
     R G[3][5];
     for (int a = 0; a < 3; a++) for (int b = 0; b < 5; b++) G[a][b] = 0;
     R mx[5], mn[5];
@@ -964,9 +968,10 @@ __global__ void __launch_bounds__(256) k_force(DevParams P, Gam<R> G, int nrows,
                                                const int *__restrict__ timebin, const double *__restrict__ msv_in,
                                                int *__restrict__ wakeup, double *__restrict__ ACC, double *__restrict__ DTU,
                                                double *__restrict__ MSV, int *__restrict__ err, int *__restrict__ need_wakeup,
-                                               unsigned long long *__restrict__ pairs)
+                                               unsigned long long *__restrict__ pairs, const int *__restrict__ sched, BoxTable bt)
 {
-    const int r = (int)(xcd_block() * blockDim.x + threadIdx.x);
+    const int k = (int)(xcd_block() * blockDim.x + threadIdx.x);
+    const int r = k < nrows ? (sched ? sched[k] : k) : nrows;
     // pairs of this launch (wave sum, one atomic per wave)
     {
         unsigned long long c = (r < nrows) ? (unsigned long long)cnt[r] : 0ull;
@@ -975,6 +980,10 @@ __global__ void __launch_bounds__(256) k_force(DevParams P, Gam<R> G, int nrows,
     }
     if (r >= nrows) return;
     const int i = act[r];
+    {
+        GridView g{};
+        resolve_box(bt, i, P, g);
+    }
     const long n = F.n;
     // hydro_force_initial_operations_preloop (src/hydro/hydro_toplevel.cc:457-565): an active
     // particle's own flag is cleared; only inactive neighbours are flagged below
@@ -1011,12 +1020,30 @@ __global__ void __launch_bounds__(256) k_force(DevParams P, Gam<R> G, int nrows,
             a0 += f0; a1 += f1; a2 += f2; ae += fp;
             if (vts > m) m = vts;
             if (active && !active[j] && wakeup) {
-                if ((double)vts > kWakeup * msv_in[j]) { atomicMax(&wakeup[j], tbi + 1); *need_wakeup = 1; }
+                if ((double)vts > kWakeup * msv_in[j]) {
+                    atomicMax(&wakeup[j], tbi + 1);
+                    if (bt.b) bt.b[bt.BOX[i]].need_wakeup = 1;
+                    else *need_wakeup = 1;
+                }
             }
         }
         A0 = a0; A1 = a1; A2 = a2; dE = ae; msv = m;
     }
     force_tail(P, F, i, enabled, sound_i, A0, A1, A2, dE, msv, ACC, DTU, MSV);
 }
+
+// sort key of each row for the lane kernels' order: its block of kSchedSpan rows, then its
+// list length descending (the rows of a block stay close in space; their waves even out)
+#ifdef GZ_DEFINE_COMMON_KERNELS
+__global__ void __launch_bounds__(256) k_sched_key(int nrows, const int *__restrict__ cnt, uint32_t *__restrict__ key,
+                                                    int *__restrict__ val)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nrows) return;
+    const int c = min(cnt[r], kSchedSpan - 1);
+    key[r] = ((uint32_t)(r / kSchedSpan) << 11) | (uint32_t)(kSchedSpan - 1 - c);
+    val[r] = r;
+}
+#endif
 
 }  // namespace gz

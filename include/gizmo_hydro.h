@@ -184,7 +184,12 @@ int gz_timeline_get(gz_context *ctx, gz_timeline *out);
    of one lane per particle; both give the same sums; "wide_search" (default 0) -- 1: every
    particle searches at the grid level of its own kernel radius (cell tables of every level),
    for radii that span orders of magnitude, where one grid sized by the largest radius would
-   walk far too many candidates; same neighbour sets, same sums. */
+   walk far too many candidates; same neighbour sets, same sums (the symmetric list of more than
+   few_rows rows is then built one wavefront per cell of the rows' own levels, the density
+   rounds one wavefront per row); "row_schedule" (default 1) -- the one-lane-per-particle force pass takes the rows of each
+   block of 2048 longest list first (a wavefront lasts as long as its longest list).  Neither
+   changes a sum.  None of these options has a counterpart in the reference (tuning knobs of this
+   library). */
 int gz_set_option(gz_context *ctx, const char *name, double value);
 
 /* Neighbour lists of the resident state, in the caller's indexing, for parity tests.

@@ -33,11 +33,11 @@ def wide_case(oracle_lib):
     return dict(ic=ic, par=par, s1=s1, s3=s3)
 
 
-def device(ic, par, wide, precision=GZ_FP64, few_rows=None):
+def device(ic, par, wide, precision=GZ_FP64, **options):
     g = Hydro(par, len(ic["mass"]), precision)
     g.set_option("wide_search", wide)
-    if few_rows is not None:
-        g.set_option("few_rows", few_rows)
+    for k, v in options.items():
+        g.set_option(k, v)
     g.load(dict(pos=ic["pos"], vel=ic["vel"], mass=ic["mass"], u=ic["u"], h=ic["h"]))
     return g
 
@@ -69,12 +69,13 @@ def test_symmetric_sets_exact(wide_case, oracle_lib, few_rows):
 def test_full_pass_same_bits_as_one_grid(wide_case):
     ic, par = wide_case["ic"], wide_case["par"]
     outs = {}
-    # one grid; many levels with the default split; every round and list in row groups; every
-    # round and list one wavefront per row
-    cases = {"one grid": (0, None), "wide": (1, None), "wide groups": (1, 0), "wide rows": (1, 10**9)}
-    for name, (wide, few) in cases.items():
+    # one grid; many levels with the default split; every list in row groups; every list one
+    # wavefront per row; the force pass's lanes in plain row order
+    cases = {"one grid": (0, {}), "wide": (1, {}), "wide groups": (1, dict(few_rows=0)), "wide rows": (1, dict(few_rows=10**9)),
+             "wide rows in order": (1, dict(row_schedule=0)), "one grid rows in order": (0, dict(row_schedule=0))}
+    for name, (wide, opts) in cases.items():
         print(f"full pass: {name}", flush=True)
-        g = device(ic, par, wide, few_rows=few)
+        g = device(ic, par, wide, **opts)
         g.hydro_pass()
         outs[name] = g.store(FIELDS)
         g.close()
