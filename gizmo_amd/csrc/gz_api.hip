@@ -83,7 +83,9 @@ struct gz_context {
     // the force pass's row order over the lanes (longest lists first within blocks of rows)
     int *sched = nullptr, *sched_val = nullptr;
     uint32_t *sched_key = nullptr, *sched_key2 = nullptr;
-    bool sched_on = true, sched_valid = false;
+    unsigned long long *sched_part = nullptr;
+    int sched_span = 2048;  // rows per sorted block (0: plain order)
+    bool sched_valid = false;
 
     // staging and scratch
     double *stage = nullptr;  // n*9 doubles
@@ -343,7 +345,7 @@ void free_state(gz_context *c)
     dfree(c->wstart); dfree(c->wend); dfree(c->wlstart); dfree(c->wlend); dfree(c->lidx); dfree(c->plev); dfree(c->lcount);
     dfree(c->wide_dev);
     {
-        dfree(c->sched); dfree(c->sched_val); dfree(c->sched_key); dfree(c->sched_key2);
+        dfree(c->sched); dfree(c->sched_val); dfree(c->sched_key); dfree(c->sched_key2); dfree(c->sched_part);
         c->sched_valid = false;
         RowGroups &G = c->grp;
         dfree(G.key); dfree(G.skey); dfree(G.gkey); dfree(G.val); dfree(G.row); dfree(G.flag); dfree(G.scan); dfree(G.start);
@@ -798,19 +800,26 @@ int density_stage(gz_context *ctx)
 }
 
 // the order in which the lanes of the force pass take the rows: longest symmetric list first
-// within each block of kSchedSpan rows (the gradient pass, cheaper per pair, keeps the plain
+// within each block of sched_span rows (the gradient pass, cheaper per pair, keeps the plain
 // order: its scattered writes would cost more than the even waves save)
 int build_sched(gz_context *ctx)
 {
     const int na = ctx->nact;
+    const int nparts = (int)((na + 1023) / 1024);
     if (!ctx->sched) {
         CHK(dalloc(ctx, &ctx->sched, ctx->cap)); CHK(dalloc(ctx, &ctx->sched_val, ctx->cap));
         CHK(dalloc(ctx, &ctx->sched_key, ctx->cap)); CHK(dalloc(ctx, &ctx->sched_key2, ctx->cap));
+        CHK(dalloc(ctx, &ctx->sched_part, 2 * ((ctx->cap + 1023) / 1024) + 2));
     }
     int end_bit = 11;
-    while ((((long)na - 1) / kSchedSpan) >> (end_bit - 11)) end_bit++;
+    while ((((long)na - 1) / ctx->sched_span) >> (end_bit - 11)) end_bit++;
+    // FP64 pairs are compute-bound: always sorted; FP32 pairs only when the waves are uneven
+    const double min_even = ctx->par.precision == GZ_FP64 ? 2.0 : 0.75;
+    unsigned long long *total = ctx->sched_part + 2 * nparts;
     Timed t(ctx, "ngb_build");
-    k_sched_key<<<nblk(na), kBlock, 0, ctx->stream>>>(na, ctx->cnt_s, ctx->sched_key, ctx->sched_val);
+    k_sched_stats<<<nparts, 1024, 0, ctx->stream>>>(na, ctx->cnt_s, ctx->sched_part);
+    k_sched_total<<<1, 1024, 0, ctx->stream>>>(nparts, ctx->sched_part, total);
+    k_sched_key<<<nblk(na), kBlock, 0, ctx->stream>>>(na, ctx->sched_span, ctx->cnt_s, total, min_even, ctx->sched_key, ctx->sched_val);
     HIPCHK(hipGetLastError());
     size_t bytes = 0;
     HIPCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, ctx->sched_key, ctx->sched_key2, ctx->sched_val, ctx->sched, na, 0, end_bit,
@@ -875,7 +884,7 @@ int ensure_sym(gz_context *ctx)
             ctx->sym_valid = true;
             ctx->sym_check_pending = ctx->defer_sym_check;
             ctx->sched_valid = false;
-            if (!few && ctx->sched_on) CHK(build_sched(ctx));
+            if (!few && ctx->sched_span > 0) CHK(build_sched(ctx));
             // the records the gradient and force passes read: every record when many rows are
             // active, else only those of the rows and their neighbours
             ctx->rec_need_valid = false;
@@ -911,7 +920,7 @@ int gradients_stage(gz_context *ctx)
                                                                                rec, ctx->T, ctx->CN, ctx->n, ctx->ell_s, ctx->ld_s,
                                                                                ctx->cnt_s, ctx->GR, ctx->GP, ctx->GV, BoxTable{});
     else
-        k_gradients<KERN, R><<<xcd_grid(nblk(ctx->nact)), kBlock, 0, ctx->stream>>>(ctx->dp, ctx->nact, ctx->act, ctx->X, ctx->Y, ctx->Z,
+        k_gradients<KERN, R, false><<<xcd_grid(nblk(ctx->nact)), kBlock, 0, ctx->stream>>>(ctx->dp, ctx->nact, ctx->act, ctx->X, ctx->Y, ctx->Z,
                                                                                     rec, ctx->T, ctx->CN, ctx->n, ctx->ell_s, ctx->ld_s,
                                                                                     ctx->cnt_s, ctx->GR, ctx->GP, ctx->GV, BoxTable{});
     HIPCHK(hipGetLastError());
@@ -938,7 +947,7 @@ int force_stage(gz_context *ctx)
                 ctx->all_active ? nullptr : ctx->ACT, ctx->TB, ctx->MSV, ctx->WAKE, ctx->ACC, ctx->DTU, ctx->MSV, ctx->counters + 2,
                 ctx->counters + 4, nullptr, BoxTable{});
         else
-            k_force<KERN, R><<<xcd_grid(nblk(ctx->nact)), kBlock, 0, ctx->stream>>>(
+            k_force<KERN, R, false><<<xcd_grid(nblk(ctx->nact)), kBlock, 0, ctx->stream>>>(
                 ctx->dp, gam<R>(ctx->dp), ctx->nact, ctx->act, F, rec, ctx->ell_s, ctx->ld_s, ctx->cnt_s,
                 ctx->all_active ? nullptr : ctx->ACT, ctx->TB, ctx->MSV, ctx->WAKE, ctx->ACC, ctx->DTU, ctx->MSV, ctx->counters + 2,
                 ctx->counters + 4, ctx->u64c + 65, ctx->sched_valid ? ctx->sched : nullptr, BoxTable{});
@@ -1600,8 +1609,9 @@ int gz_set_option(gz_context *ctx, const char *name, double value)
         return 0;
     }
     if (k == "row_schedule") {
-        if (value != 0 && value != 1) return fail(ctx, GZ_E_ARG, "row_schedule must be 0 or 1");
-        ctx->sched_on = value != 0;
+        const long v = (long)value;
+        if ((double)v != value || v < 0 || v > (1L << 20) || v % 64) return fail(ctx, GZ_E_ARG, "row_schedule must be 0 or a multiple of 64 up to 2^20");
+        ctx->sched_span = (int)v;
         ctx->sym_valid = false;
         return 0;
     }

@@ -599,7 +599,7 @@ int gradients_force(gz_batch *b)
     k_pack_grad<R><<<nb(b->cap), kB, 0, b->stream>>>(b->cap, b->rec_need, b->H, b->M, b->RHO, b->PR, b->VX, b->VY, b->VZ, rg);
     const bool lanes = na > b->few_rows;  // many rows: one lane per row
     if (lanes)
-        k_gradients<KERN, R><<<xcd_grid(nb(na)), kB, 0, b->stream>>>(b->dp, na, b->act, b->X, b->Y, b->Z, rg, b->T, b->CN, b->cap, b->ell, ld,
+        k_gradients<KERN, R, true><<<xcd_grid(nb(na)), kB, 0, b->stream>>>(b->dp, na, b->act, b->X, b->Y, b->Z, rg, b->T, b->CN, b->cap, b->ell, ld,
                                                                       b->cnt, b->GR, b->GP, b->GV, bt);
     else
         k_gradients_rows<KERN, R><<<xcd_grid(na), 64, 0, b->stream>>>(b->dp, na, b->act, b->X, b->Y, b->Z, rg, b->T, b->CN, b->cap,
@@ -613,7 +613,7 @@ int gradients_force(gz_batch *b)
     G.G5 = (R)b->dp.G5; G.G6 = (R)b->dp.G6; G.G7 = (R)b->dp.G7; G.G8 = (R)b->dp.G8; G.G9 = (R)b->dp.G9;
     k_pack_force<R><<<nb(b->cap), kB, 0, b->stream>>>(b->par.gamma, F, b->rec_need, rf, b->counters + 2);
     if (lanes) {
-        k_force<KERN, R><<<xcd_grid(nb(na)), kB, 0, b->stream>>>(b->dp, G, na, b->act, F, rf, b->ell, ld, b->cnt, b->ACT, b->TB, b->MSV,
+        k_force<KERN, R, true><<<xcd_grid(nb(na)), kB, 0, b->stream>>>(b->dp, G, na, b->act, F, rf, b->ell, ld, b->cnt, b->ACT, b->TB, b->MSV,
                                                                   b->WAKE, b->ACC, b->DTU, b->MSV, b->counters + 2, b->counters + 4, b->u64,
                                                                   nullptr, bt);
     } else {

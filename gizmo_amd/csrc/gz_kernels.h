@@ -244,7 +244,8 @@ __global__ void k_pack_grad(long n, const uint8_t *need, const double *H, const 
 template <typename R>
 __global__ void k_pack_force(double gamma, ForceView F, const uint8_t *need, Cell<R> *rec, int *zero2);
 
-template <int KERN, typename R>
+// BOXES: each row's box geometry from bt (batches); else the single box of the arguments
+template <int KERN, typename R, bool BOXES>
 __global__ void k_gradients(DevParams P, int nrows, const int *act, const double *X, const double *Y, const double *Z,
                             const GradRec<R> *rec, const double *T, const double *CN, long n, const int *ell, long ld,
                             const int *cnt, double *GR, double *GP, double *GV, BoxTable bt);
@@ -266,15 +267,19 @@ __global__ void k_force_rows(DevParams P, Gam<R> G, int nrows, const int *act, F
                              double *ACC, double *DTU, double *MSV, int *err, int *need_wakeup, unsigned long long *pairs,
                              BoxTable bt);
 
-template <int KERN, typename R>
+template <int KERN, typename R, bool BOXES>
 __global__ void k_force(DevParams P, Gam<R> G, int nrows, const int *act, ForceView F, const Cell<R> *rec, const int *ell,
                         long ld, const int *cnt, const uint8_t *active, const int *timebin, const double *msv_in, int *wakeup,
                         double *ACC, double *DTU, double *MSV, int *err, int *need_wakeup, unsigned long long *pairs,
                         const int *sched, BoxTable bt);
 
-// the order in which the force pass's lanes take the rows: within each block of kSchedSpan
-// rows, longest list first (a wavefront runs as long as its longest row)
-constexpr int kSchedSpan = 2048;
-__global__ void k_sched_key(int nrows, const int *cnt, uint32_t *key, int *val);
+// the order in which the force pass's lanes take the rows: within each block of span rows,
+// longest list first (a wavefront runs as long as its longest row), when the waves of the plain
+// order are uneven enough (list sum over 64 x the sum of the waves' longest lists below
+// min_even; the scattered reads of the sorted order cost more than even waves save otherwise)
+__global__ void k_sched_stats(int nrows, const int *cnt, unsigned long long *partial);
+__global__ void k_sched_total(int nparts, const unsigned long long *partial, unsigned long long *total);
+__global__ void k_sched_key(int nrows, int span, const int *cnt, const unsigned long long *total, double min_even, uint32_t *key,
+                            int *val);
 
 }  // namespace gz

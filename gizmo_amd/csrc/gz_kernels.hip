@@ -16,6 +16,16 @@ template __global__ void k_build_ell<false>(int, const int *, const int *, const
 template __global__ void k_build_ell<true>(int, const int *, const int *, const double *, double, const double *, const double *,
                                            const double *, const double *, const uint32_t *, GridView, int *, long, int, int *, int *);
 
+#define GZ_INST_LANES(KERN, R, BOXES)                                                                                           \
+    template __global__ void k_gradients<KERN, R, BOXES>(DevParams, int, const int *, const double *, const double *,           \
+                                                         const double *, const GradRec<R> *, const double *, const double *,    \
+                                                         long, const int *, long, const int *, double *, double *, double *,    \
+                                                         BoxTable);                                                              \
+    template __global__ void k_force<KERN, R, BOXES>(DevParams, Gam<R>, int, const int *, ForceView, const Cell<R> *,           \
+                                                     const int *, long, const int *, const uint8_t *, const int *,              \
+                                                     const double *, int *, double *, double *, double *, int *, int *,         \
+                                                     unsigned long long *, const int *, BoxTable);
+
 #define GZ_INST(KERN, R)                                                                                                         \
     template __global__ void k_density_iter<KERN, R>(DevParams, int, const int *, const int *, const double *, const double *,  \
                                                      const double *, const double *, const double *, const double *,            \
@@ -24,13 +34,8 @@ template __global__ void k_build_ell<true>(int, const int *, const int *, const 
     template __global__ void k_density_cells<KERN, R>(DevParams, GridView, CellRows, const int *, const uint8_t *,              \
                                                       const double *, const double *, const double *, const double *,           \
                                                       const double *, const double *, const double *, const double *, double *);\
-    template __global__ void k_gradients<KERN, R>(DevParams, int, const int *, const double *, const double *, const double *,  \
-                                                  const GradRec<R> *, const double *, const double *, long, const int *, long,  \
-                                                  const int *, double *, double *, double *, BoxTable);                        \
-    template __global__ void k_force<KERN, R>(DevParams, Gam<R>, int, const int *, ForceView, const Cell<R> *, const int *,     \
-                                              long, const int *, const uint8_t *, const int *, const double *, int *, double *, \
-                                              double *, double *, int *, int *, unsigned long long *, const int *,     \
-                                              BoxTable);                                                               \
+    GZ_INST_LANES(KERN, R, false)                                                                                                \
+    GZ_INST_LANES(KERN, R, true)                                                                                                 \
     template __global__ void k_density_rows<KERN, R>(DevParams, GridView, int, const int *, const uint8_t *, const uint32_t *,  \
                                                      const double *, const double *, const double *, const double *,            \
                                                      const double *, const double *, const double *, const double *, double *,  \

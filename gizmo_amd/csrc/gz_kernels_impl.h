@@ -860,7 +860,7 @@ __device__ inline void gradients_tail(const GradRec<R> &I, bool enabled, const R
     }
 }
 
-template <int KERN, typename R>
+template <int KERN, typename R, bool BOXES>
 __global__ void __launch_bounds__(256) k_gradients(DevParams P, int nrows, const int *__restrict__ act, const double *__restrict__ X,
                                                    const double *__restrict__ Y, const double *__restrict__ Z,
                                                    const GradRec<R> *__restrict__ rec, const double *__restrict__ T,
@@ -871,7 +871,7 @@ __global__ void __launch_bounds__(256) k_gradients(DevParams P, int nrows, const
     const int r = (int)(xcd_block() * blockDim.x + threadIdx.x);
     if (r >= nrows) return;
     const int i = act[r];
-    {
+    if (BOXES) {
         GridView g{};
         resolve_box(bt, i, P, g);
     }
@@ -961,7 +961,7 @@ __device__ inline void force_tail(const DevParams &P, const ForceView &F, int i,
     MSV[i] = ms;
 }
 
-template <int KERN, typename R>
+template <int KERN, typename R, bool BOXES>
 __global__ void __launch_bounds__(256) k_force(DevParams P, Gam<R> G, int nrows, const int *__restrict__ act, ForceView F,
                                                const Cell<R> *__restrict__ rec, const int *__restrict__ ell, long ld,
                                                const int *__restrict__ cnt, const uint8_t *__restrict__ active,
@@ -980,7 +980,7 @@ __global__ void __launch_bounds__(256) k_force(DevParams P, Gam<R> G, int nrows,
     }
     if (r >= nrows) return;
     const int i = act[r];
-    {
+    if (BOXES) {
         GridView g{};
         resolve_box(bt, i, P, g);
     }
@@ -1022,7 +1022,7 @@ __global__ void __launch_bounds__(256) k_force(DevParams P, Gam<R> G, int nrows,
             if (active && !active[j] && wakeup) {
                 if ((double)vts > kWakeup * msv_in[j]) {
                     atomicMax(&wakeup[j], tbi + 1);
-                    if (bt.b) bt.b[bt.BOX[i]].need_wakeup = 1;
+                    if (BOXES) bt.b[bt.BOX[i]].need_wakeup = 1;
                     else *need_wakeup = 1;
                 }
             }
@@ -1032,16 +1032,64 @@ __global__ void __launch_bounds__(256) k_force(DevParams P, Gam<R> G, int nrows,
     force_tail(P, F, i, enabled, sound_i, A0, A1, A2, dE, msv, ACC, DTU, MSV);
 }
 
-// sort key of each row for the lane kernels' order: its block of kSchedSpan rows, then its
-// list length descending (the rows of a block stay close in space; their waves even out)
 #ifdef GZ_DEFINE_COMMON_KERNELS
-__global__ void __launch_bounds__(256) k_sched_key(int nrows, const int *__restrict__ cnt, uint32_t *__restrict__ key,
-                                                    int *__restrict__ val)
+// per block of 1024 rows: the sum of the list lengths and the sum of each wave's longest
+__global__ void __launch_bounds__(1024) k_sched_stats(int nrows, const int *__restrict__ cnt, unsigned long long *__restrict__ partial)
+{
+    __shared__ unsigned long long ss[16], sm[16];
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    int c = r < nrows ? cnt[r] : 0;
+    int mx = c;
+    unsigned long long sum = (unsigned long long)c;
+    for (int o = 32; o > 0; o >>= 1) {
+        mx = max(mx, __shfl_xor(mx, o, 64));
+        sum += __shfl_xor(sum, o, 64);
+    }
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) { ss[w] = sum; sm[w] = (unsigned long long)mx; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long a = 0, b = 0;
+        for (int k = 0; k < (int)(blockDim.x >> 6); k++) { a += ss[k]; b += sm[k]; }
+        partial[2 * blockIdx.x] = a;
+        partial[2 * blockIdx.x + 1] = b;
+    }
+}
+
+// the totals of the partial sums (one block)
+__global__ void __launch_bounds__(1024) k_sched_total(int nparts, const unsigned long long *__restrict__ partial,
+                                                      unsigned long long *__restrict__ total)
+{
+    __shared__ unsigned long long ss[16], sm[16];
+    unsigned long long a = 0, b = 0;
+    for (int k = threadIdx.x; k < nparts; k += blockDim.x) { a += partial[2 * k]; b += partial[2 * k + 1]; }
+    for (int o = 32; o > 0; o >>= 1) {
+        a += __shfl_xor(a, o, 64);
+        b += __shfl_xor(b, o, 64);
+    }
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) { ss[w] = a; sm[w] = b; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long x = 0, y = 0;
+        for (int k = 0; k < (int)(blockDim.x >> 6); k++) { x += ss[k]; y += sm[k]; }
+        total[0] = x;
+        total[1] = y;
+    }
+}
+
+// sort key of each row for the force pass's order: its block of span rows, then its list
+// length descending (lengths above 2047 tie); the rows of a block stay close in space, their
+// waves even out.  Even waves: the key keeps the plain order.
+__global__ void __launch_bounds__(256) k_sched_key(int nrows, int span, const int *__restrict__ cnt,
+                                                    const unsigned long long *__restrict__ total, double min_even,
+                                                    uint32_t *__restrict__ key, int *__restrict__ val)
 {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= nrows) return;
-    const int c = min(cnt[r], kSchedSpan - 1);
-    key[r] = ((uint32_t)(r / kSchedSpan) << 11) | (uint32_t)(kSchedSpan - 1 - c);
+    const double even = (double)total[0] / (64.0 * (double)(total[1] > 0 ? total[1] : 1));
+    const int c = even < min_even ? min(cnt[r], 2047) : 2047;
+    key[r] = ((uint32_t)(r / span) << 11) | (uint32_t)(2047 - c);
     val[r] = r;
 }
 #endif
