@@ -929,6 +929,27 @@ __global__ void __launch_bounds__(256) k_gradients(DevParams P, int nrows, const
     gradients_tail<R>(I, enabled, G, mx, mn, maxd, T, CN, n, i, GR, GP, GV);
 }
 
+// Running sum of a conserved flux total.  FP32: Neumaier-compensated (the lost low bits of each
+// addition are kept in c, so sums that cancel to a small net keep their accuracy); FP64: the
+// plain sum in list order (the oracle's additions).
+template <typename R>
+struct CSum {
+    R s = 0, c = 0;
+    __device__ __forceinline__ void add(R x)
+    {
+        const R t = s + x;
+        c += (fabs(s) >= fabs(x)) ? ((s - t) + x) : ((x - t) + s);
+        s = t;
+    }
+    __device__ __forceinline__ double value() const { return (double)s + (double)c; }
+};
+template <>
+struct CSum<double> {
+    double s = 0;
+    __device__ __forceinline__ void add(double x) { s += x; }
+    __device__ __forceinline__ double value() const { return s; }
+};
+
 // ---------------------------------------------------------------------------------------
 // force: the pair loop of hydro_force() (src/hydro/hydro_force_loop.cc:423-480), two-sided
 // (each active particle accumulates its own side of every pair), then the writeback
@@ -998,7 +1019,8 @@ __global__ void __launch_bounds__(256) k_force(DevParams P, Gam<R> G, int nrows,
         const double xi = F.X[i], yi = F.Y[i], zi = F.Z[i];
         const int tbi = timebin ? timebin[i] : 0;
         const int c = cnt[r];
-        R a0 = 0, a1 = 0, a2 = 0, ae = 0, m = 0;
+        CSum<R> a0, a1, a2, ae;
+        R m = 0;
         for (int q = 0; q < c; q++) {
             const int j = ell[(size_t)q * ld + r];
             if (m < I.cs) m = I.cs;
@@ -1017,7 +1039,7 @@ __global__ void __launch_bounds__(256) k_force(DevParams P, Gam<R> G, int nrows,
                 continue;
             }
             if (rc == 0) continue;
-            a0 += f0; a1 += f1; a2 += f2; ae += fp;
+            a0.add(f0); a1.add(f1); a2.add(f2); ae.add(fp);
             if (vts > m) m = vts;
             if (active && !active[j] && wakeup) {
                 if ((double)vts > kWakeup * msv_in[j]) {
@@ -1027,7 +1049,7 @@ __global__ void __launch_bounds__(256) k_force(DevParams P, Gam<R> G, int nrows,
                 }
             }
         }
-        A0 = a0; A1 = a1; A2 = a2; dE = ae; msv = m;
+        A0 = a0.value(); A1 = a1.value(); A2 = a2.value(); dE = ae.value(); msv = m;
     }
     force_tail(P, F, i, enabled, sound_i, A0, A1, A2, dE, msv, ACC, DTU, MSV);
 }

@@ -792,7 +792,8 @@ __global__ void __launch_bounds__(64) k_force_rows(DevParams P, Gam<R> G, int nr
         const int tbi = timebin ? timebin[i] : 0;
         // lanes 0-3 keep the momentum and energy sums, lane 4 the signal speed: max(c_s, vsig)
         // over the entries (the reference raises it to c_s before each entry, a no-op after the first)
-        R acc = 0;
+        R acc = 0;       // lane 4
+        CSum<R> sum;     // lanes 0-3
         if (lane == 4 && c > 0) acc = (acc < I.cs) ? I.cs : acc;
         for (int base = 0; base < c; base += 64) {
             const int q = base + lane;
@@ -830,14 +831,15 @@ __global__ void __launch_bounds__(64) k_force_rows(DevParams P, Gam<R> G, int nr
                 while (mask) {
                     const int k = __ffsll((long long)mask) - 1;
                     mask &= mask - 1;
-                    if (lane < 4) acc += sv[lane][k];
+                    if (lane < 4) sum.add(sv[lane][k]);
                     else if (sv[4][k] > acc) acc = sv[4][k];
                 }
             }
             __syncthreads();
         }
-        __shared__ R res[5];
-        if (lane < 5) res[lane] = acc;
+        __shared__ double res[5];
+        if (lane < 4) res[lane] = sum.value();
+        if (lane == 4) res[lane] = (double)acc;
         __syncthreads();
         A0 = res[0]; A1 = res[1]; A2 = res[2]; dE = res[3]; msv = res[4];
     }
