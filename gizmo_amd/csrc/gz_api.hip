@@ -756,7 +756,7 @@ int density_stage(gz_context *ctx)
             // cells of the rows' own levels hold too few rows to fill a wavefront
             Timed t(ctx, "density");
             if (few)
-                k_density_rows<KERN, R><<<xcd_grid(na), 64, 0, ctx->stream>>>(ctx->dp, grid_view(ctx), na, ctx->act, ctx->need, ctx->key,
+                k_density_rows<KERN, R><<<xcd_grid(na), kRowThreads, 0, ctx->stream>>>(ctx->dp, grid_view(ctx), na, ctx->act, ctx->need, ctx->key,
                                                                             ctx->X, ctx->Y, ctx->Z, ctx->VX, ctx->VY, ctx->VZ, ctx->M,
                                                                             ctx->it_h, ctx->dacc, BoxTable{});
             else
@@ -841,9 +841,12 @@ int ensure_sym(gz_context *ctx)
     }
     const bool wide = ctx->wide != 0;
     const bool few = ctx->nact <= ctx->few_rows;
-    // the row walks drop stencil cells beyond reach: they need each cell's largest radius
+    // the row walks drop stencil cells beyond reach: every radius is at most hmax
     if (wide) CHK(build_wide(ctx, true));
-    else CHK(build_grid(ctx, std::min(hmax, 0.5 * ctx->par.box_size), few));
+    else CHK(build_grid(ctx, std::min(hmax, 0.5 * ctx->par.box_size)));
+    GridView gs = grid_view(ctx);
+    gs.chmax = nullptr;
+    gs.hall = hmax;
     if (wide && !few) CHK(build_groups(ctx));
     if (!wide && !few) CHK(build_cell_rows(ctx));
     const long need_ld = ((long)ctx->nact + 63) / 64 * 64;
@@ -866,7 +869,7 @@ int ensure_sym(gz_context *ctx)
                                                                                ctx->H, ctx->ell_s, ctx->ld_s, ctx->cap_s, ctx->cnt_s,
                                                                                ctx->counters + 3);
             else if (few)
-                k_build_sym_rows<<<xcd_grid(ctx->nact), 64, 0, ctx->stream>>>(grid_view(ctx), ctx->nact, ctx->act, ctx->key, ctx->X, ctx->Y,
+                k_build_sym_rows<<<xcd_grid(ctx->nact), kRowThreads, 0, ctx->stream>>>(gs, ctx->nact, ctx->act, ctx->key, ctx->X, ctx->Y,
                                                                                ctx->Z, ctx->H, ctx->ell_s, ctx->ld_s, ctx->cap_s, ctx->cnt_s,
                                                                                ctx->counters + 3, BoxTable{});
             else
@@ -916,7 +919,7 @@ int gradients_stage(gz_context *ctx)
     k_pack_grad<R><<<nblk(ctx->n), kBlock, 0, ctx->stream>>>(ctx->n, ctx->rec_need_valid ? ctx->rec_need : nullptr, ctx->H, ctx->M,
                                                              ctx->RHO, ctx->PR, ctx->VX, ctx->VY, ctx->VZ, rec);
     if (ctx->nact <= ctx->few_rows)
-        k_gradients_rows<KERN, R><<<xcd_grid(ctx->nact), 64, 0, ctx->stream>>>(ctx->dp, ctx->nact, ctx->act, ctx->X, ctx->Y, ctx->Z,
+        k_gradients_rows<KERN, R><<<xcd_grid(ctx->nact), kRowThreads, 0, ctx->stream>>>(ctx->dp, ctx->nact, ctx->act, ctx->X, ctx->Y, ctx->Z,
                                                                                rec, ctx->T, ctx->CN, ctx->n, ctx->ell_s, ctx->ld_s,
                                                                                ctx->cnt_s, ctx->GR, ctx->GP, ctx->GV, BoxTable{});
     else
@@ -942,7 +945,7 @@ int force_stage(gz_context *ctx)
         k_pack_force<R><<<nblk(ctx->n), kBlock, 0, ctx->stream>>>(ctx->par.gamma, F, ctx->rec_need_valid ? ctx->rec_need : nullptr, rec,
                                                                   ctx->counters + 2);
         if (ctx->nact <= ctx->few_rows)
-            k_force_rows<KERN, R><<<xcd_grid(ctx->nact), 64, 0, ctx->stream>>>(
+            k_force_rows<KERN, R><<<xcd_grid(ctx->nact), kRowThreads, 0, ctx->stream>>>(
                 ctx->dp, gam<R>(ctx->dp), ctx->nact, ctx->act, F, rec, ctx->ell_s, ctx->ld_s, ctx->cnt_s,
                 ctx->all_active ? nullptr : ctx->ACT, ctx->TB, ctx->MSV, ctx->WAKE, ctx->ACC, ctx->DTU, ctx->MSV, ctx->counters + 2,
                 ctx->counters + 4, nullptr, BoxTable{});

@@ -521,7 +521,7 @@ int density(gz_batch *b)
                                                                                       b->X, b->Y, b->Z, b->VX, b->VY, b->VZ, b->M,
                                                                                       b->it_h, b->dacc);
         } else {
-            k_density_rows<KERN, R><<<xcd_grid(na), 64, 0, b->stream>>>(b->dp, gview(b), na, b->act, b->need, b->key, b->X, b->Y, b->Z,
+            k_density_rows<KERN, R><<<xcd_grid(na), kRowThreads, 0, b->stream>>>(b->dp, gview(b), na, b->act, b->need, b->key, b->X, b->Y, b->Z,
                                                                          b->VX, b->VY, b->VZ, b->M, b->it_h, b->dacc, bt);
         }
         k_density_after<<<nb(na), kB, 0, b->stream>>>(b->dp, na, b->act, b->need, b->dacc, b->M, b->U, 0, it, out, b->counters + 0,
@@ -570,7 +570,7 @@ int sym_list(gz_batch *b, const std::vector<uint8_t> &in_pass)
             k_build_sym_cells<<<xcd_grid(b->grp.ngroups), 64, 0, b->stream>>>(gview(b), b->grp.view(bt), b->act, b->X, b->Y, b->Z, b->H,
                                                                                b->ell, ld, b->ell_cap, b->cnt, b->counters + 3);
         } else {
-            k_build_sym_rows<<<xcd_grid(na), 64, 0, b->stream>>>(gview(b), na, b->act, b->key, b->X, b->Y, b->Z, b->H, b->ell, ld,
+            k_build_sym_rows<<<xcd_grid(na), kRowThreads, 0, b->stream>>>(gview(b), na, b->act, b->key, b->X, b->Y, b->Z, b->H, b->ell, ld,
                                                                   b->ell_cap, b->cnt, b->counters + 3, bt);
         }
         BHIP(hipGetLastError());
@@ -602,7 +602,7 @@ int gradients_force(gz_batch *b)
         k_gradients<KERN, R, true><<<xcd_grid(nb(na)), kB, 0, b->stream>>>(b->dp, na, b->act, b->X, b->Y, b->Z, rg, b->T, b->CN, b->cap, b->ell, ld,
                                                                       b->cnt, b->GR, b->GP, b->GV, bt);
     else
-        k_gradients_rows<KERN, R><<<xcd_grid(na), 64, 0, b->stream>>>(b->dp, na, b->act, b->X, b->Y, b->Z, rg, b->T, b->CN, b->cap,
+        k_gradients_rows<KERN, R><<<xcd_grid(na), kRowThreads, 0, b->stream>>>(b->dp, na, b->act, b->X, b->Y, b->Z, rg, b->T, b->CN, b->cap,
                                                                        b->ell, ld, b->cnt, b->GR, b->GP, b->GV, bt);
     BHIP(hipGetLastError());
     ForceView F{b->cap, b->X, b->Y, b->Z, b->VX, b->VY, b->VZ, b->M, b->H, b->U, b->RHO, b->PR, b->CN, b->FCE, b->DRK, b->GR, b->GP,
@@ -617,7 +617,7 @@ int gradients_force(gz_batch *b)
                                                                   b->WAKE, b->ACC, b->DTU, b->MSV, b->counters + 2, b->counters + 4, b->u64,
                                                                   nullptr, bt);
     } else {
-        k_force_rows<KERN, R><<<xcd_grid(na), 64, 0, b->stream>>>(b->dp, G, na, b->act, F, rf, b->ell, ld, b->cnt, b->ACT, b->TB, b->MSV,
+        k_force_rows<KERN, R><<<xcd_grid(na), kRowThreads, 0, b->stream>>>(b->dp, G, na, b->act, F, rf, b->ell, ld, b->cnt, b->ACT, b->TB, b->MSV,
                                                                    b->WAKE, b->ACC, b->DTU, b->MSV, b->counters + 2, b->counters + 4,
                                                                    nullptr, bt);
         k_sum_counts<<<std::min(nb(na), 1024u), kB, 0, b->stream>>>(na, b->cnt, b->u64);

@@ -42,7 +42,8 @@ struct GridView {
     double cell, box, half;
     const int4 *sten;         // 27 entries per cell: (start, end, offset code, cell code)
     const int *nsten;
-    const double *chmax;      // largest kernel radius in each cell (when computed)
+    const double *chmax;      // largest kernel radius in each cell (when computed; else hall bounds every radius)
+    double hall;
     double margin;            // largest distance a particle has moved from its cell since the ordering
     const int *cstart, *cend; // [start, end) of each cell
     long long cell_off;       // first cell of the box in the cell arrays (batches of boxes)
@@ -250,7 +251,10 @@ __global__ void k_gradients(DevParams P, int nrows, const int *act, const double
                             const GradRec<R> *rec, const double *T, const double *CN, long n, const int *ell, long ld,
                             const int *cnt, double *GR, double *GP, double *GV, BoxTable bt);
 
-// one wavefront per active row, for passes over few rows
+// one workgroup per active row, for passes over few rows: kRowThreads lanes (4 wavefronts)
+// evaluate that many candidates or list entries at once, then lanes of the first wavefront add
+// the accepted terms in order
+constexpr int kRowThreads = 256;
 template <int KERN, typename R>
 __global__ void k_density_rows(DevParams P, GridView g, int nrows, const int *act, const uint8_t *need, const uint32_t *key,
                                const double *X, const double *Y, const double *Z, const double *VX, const double *VY,

@@ -82,7 +82,7 @@ __device__ __forceinline__ void row_stencil(const GridView &g, uint32_t home, do
             const double gy = fmax(fmax(dy * g.cell - ry, ry - (dy + 1) * g.cell), 0.0);
             const double gz = fmax(fmax(dz * g.cell - rz, rz - (dz + 1) * g.cell), 0.0);
             double reach = hrow;
-            if (SYM) reach = fmax(reach, g.chmax[g.cell_off + c]);
+            if (SYM) reach = fmax(reach, g.chmax ? g.chmax[g.cell_off + c] : g.hall);
             const double rr = (reach + g.margin) * (1.0 + 1e-9) + 1e-12 * g.box;
             if (gx * gx + gy * gy + gz * gz > rr * rr) keep = false;
         }
@@ -167,7 +167,7 @@ __device__ __forceinline__ void row_stencil_wide(const GridView &g, const WideGr
 // density sums of the rows still iterating (DensitySpec::pair_kernel,
 // src/hydro/density_loop.h:1141-1385), the terms in k_density_cells' order
 template <int KERN, typename R>
-__global__ void __launch_bounds__(64) k_density_rows(DevParams P, GridView g, int nrows, const int *__restrict__ act,
+__global__ void __launch_bounds__(kRowThreads) k_density_rows(DevParams P, GridView g, int nrows, const int *__restrict__ act,
                                                      const uint8_t *__restrict__ need, const uint32_t *__restrict__ key,
                                                      const double *__restrict__ X, const double *__restrict__ Y,
                                                      const double *__restrict__ Z, const double *__restrict__ VX,
@@ -176,11 +176,12 @@ __global__ void __launch_bounds__(64) k_density_rows(DevParams P, GridView g, in
                                                      double *__restrict__ accout, BoxTable bt)
 {
     __shared__ RowStencil rs;
-    __shared__ R sc[13][64];
-    __shared__ int sf[64];
+    __shared__ R sc[13][kRowThreads];
+    __shared__ int sf[kRowThreads];
+    __shared__ unsigned long long smask[kRowThreads / 64];
     const long r = xcd_block();
     if (r >= nrows || !need[r]) return;
-    const int lane = threadIdx.x;
+    const int tid = threadIdx.x;
     const int i = act[r];
     resolve_box(bt, i, P, g);
     const double xi = X[i], yi = Y[i], zi = Z[i];
@@ -199,9 +200,9 @@ __global__ void __launch_bounds__(64) k_density_rows(DevParams P, GridView g, in
         row_stencil<false>(g, key[i] >> g.shift, xi, yi, zi, h, rs);
     }
     const int total = rs.pref[rs.nst];
-    R acc = 0;  // lane q < 13 keeps sum q
-    for (int base = 0; base < total; base += 64) {
-        const int v = base + lane;
+    R acc = 0;  // thread q < 13 keeps sum q
+    for (int base = 0; base < total; base += kRowThreads) {
+        const int v = base + tid;
         int f = 0;
         if (v < total) {
             const int j = row_candidate(rs, v);
@@ -218,61 +219,65 @@ __global__ void __launch_bounds__(64) k_density_rows(DevParams P, GridView g, in
                     const R u = rr * hinv;
                     R wk = 0, dwk = 0;
                     kernel_main<KERN, 0>(u, hinv3, hinv4, wk, dwk);
-                    sc[0][lane] = wk;
-                    sc[1][lane] = (R)mj * wk;
-                    sc[2][lane] = -(R(3.0) * hinv * wk + u * dwk);
+                    sc[0][tid] = wk;
+                    sc[1][tid] = (R)mj * wk;
+                    sc[2][tid] = -(R(3.0) * hinv * wk + u * dwk);
                     f = 1;
                     if (rr > 0) {
-                        sc[4][lane] = wk * (dp0 * dp0);
-                        sc[5][lane] = wk * (dp1 * dp1);
-                        sc[6][lane] = wk * (dp2 * dp2);
-                        sc[7][lane] = wk * (dp0 * dp1);
-                        sc[8][lane] = wk * (dp1 * dp2);
-                        sc[9][lane] = wk * (dp0 * dp2);
-                        sc[10][lane] = wk * dp0;
-                        sc[11][lane] = wk * dp1;
-                        sc[12][lane] = wk * dp2;
+                        sc[4][tid] = wk * (dp0 * dp0);
+                        sc[5][tid] = wk * (dp1 * dp1);
+                        sc[6][tid] = wk * (dp2 * dp2);
+                        sc[7][tid] = wk * (dp0 * dp1);
+                        sc[8][tid] = wk * (dp1 * dp2);
+                        sc[9][tid] = wk * (dp0 * dp2);
+                        sc[10][tid] = wk * dp0;
+                        sc[11][tid] = wk * dp1;
+                        sc[12][tid] = wk * dp2;
                         const R dv0 = vxi - (R)VX[j], dv1 = vyi - (R)VY[j], dv2 = vzi - (R)VZ[j];
-                        sc[3][lane] = dwk * (dp0 * dv0 + dp1 * dv1 + dp2 * dv2) / rr;
+                        sc[3][tid] = dwk * (dp0 * dv0 + dp1 * dv1 + dp2 * dv2) / rr;
                         f = 2;
                     }
                 }
             }
         }
-        // lane 0 adds the accepted candidates in order (none: nothing to exchange)
-        unsigned long long mask = __ballot(f != 0);
-        if (!mask) continue;
-        sf[lane] = f;
+        // threads 0-12 add the accepted candidates in order, wavefront by wavefront
+        const unsigned long long mask = __ballot(f != 0);
+        sf[tid] = f;
+        if ((tid & 63) == 0) smask[tid >> 6] = mask;
         __syncthreads();
-        if (lane < 13) {
-            while (mask) {
-                const int k = __ffsll((long long)mask) - 1;
-                mask &= mask - 1;
-                const int fk = sf[k];
-                if (lane < 3) acc += sc[lane][k];
-                else if (fk == 2) {
-                    if (lane == 3) acc -= sc[3][k];
-                    else acc += sc[lane][k];
+        if (tid < 13) {
+            for (int w = 0; w < kRowThreads / 64; w++) {
+                unsigned long long m = smask[w];
+                while (m) {
+                    const int k = w * 64 + __ffsll((long long)m) - 1;
+                    m &= m - 1;
+                    const int fk = sf[k];
+                    if (tid < 3) acc += sc[tid][k];
+                    else if (fk == 2) {
+                        if (tid == 3) acc -= sc[3][k];
+                        else acc += sc[tid][k];
+                    }
                 }
             }
         }
         __syncthreads();
     }
-    if (lane < 13) accout[(size_t)lane * nrows + r] = (double)acc;
+    if (tid < 13) accout[(size_t)tid * nrows + r] = (double)acc;
 }
 
 // the symmetric list of few rows (the same entries, in the same order, as k_build_sym_cells)
 #ifdef GZ_DEFINE_COMMON_KERNELS
-__global__ void __launch_bounds__(64) k_build_sym_rows(GridView g, int nrows, const int *__restrict__ act, const uint32_t *__restrict__ key,
+__global__ void __launch_bounds__(kRowThreads) k_build_sym_rows(GridView g, int nrows, const int *__restrict__ act, const uint32_t *__restrict__ key,
                                                        const double *__restrict__ X, const double *__restrict__ Y,
                                                        const double *__restrict__ Z, const double *__restrict__ H, int *__restrict__ ell,
                                                        long ld, int cap, int *__restrict__ cnt, int *__restrict__ overflow,
                                                        BoxTable bt)
 {
     __shared__ RowStencil rs;
+    __shared__ int scount[2][kRowThreads / 64];
     const long r = xcd_block();
     if (r >= nrows) return;
-    const int lane = threadIdx.x;
+    const int tid = threadIdx.x, w = tid >> 6;
     const int i = act[r];
     {
         DevParams P{};
@@ -282,20 +287,29 @@ __global__ void __launch_bounds__(64) k_build_sym_rows(GridView g, int nrows, co
     row_stencil<true>(g, key[i] >> g.shift, xi, yi, zi, hi, rs);
     const int total = rs.pref[rs.nst];
     int n = 0;
-    for (int base = 0; base < total; base += 64) {
-        const int v = base + lane;
+    for (int base = 0, it = 0; base < total; base += kRowThreads, it ^= 1) {
+        const int v = base + tid;
         bool acc = false;
         int j = -1;
         if (v < total) {
             j = row_candidate(rs, v);
             acc = (j != i) && accept_pair<true>(xi, yi, zi, X[j], Y[j], Z[j], hi, H[j], g.box, g.half);
         }
+        // place among the accepted candidates: the wavefronts before this one, then the lanes
         int cntk;
         const int pos = kept_rank(acc, cntk);
-        if (acc && n + pos < cap) ell[(size_t)(n + pos) * ld + r] = j;
-        n += cntk;
+        if ((tid & 63) == 0) scount[it][w] = cntk;
+        __syncthreads();  // (the two count buffers alternate: one barrier per round suffices)
+        int off = 0, all = 0;
+        for (int q = 0; q < kRowThreads / 64; q++) {
+            const int cq = scount[it][q];
+            if (q < w) off += cq;
+            all += cq;
+        }
+        if (acc && n + off + pos < cap) ell[(size_t)(n + off + pos) * ld + r] = j;
+        n += all;
     }
-    if (lane == 0) {
+    if (tid == 0) {
         cnt[r] = n;
         if (n > cap) atomicMax(overflow, n);
     }
@@ -656,18 +670,19 @@ __global__ void __launch_bounds__(64) k_group_stencil(GridView g, int ngroups, c
 
 // gradients of few rows: the terms of each list entry in LDS, accumulated in list order by lane 0
 template <int KERN, typename R>
-__global__ void __launch_bounds__(64) k_gradients_rows(DevParams P, int nrows, const int *__restrict__ act, const double *__restrict__ X,
+__global__ void __launch_bounds__(kRowThreads) k_gradients_rows(DevParams P, int nrows, const int *__restrict__ act, const double *__restrict__ X,
                                                        const double *__restrict__ Y, const double *__restrict__ Z,
                                                        const GradRec<R> *__restrict__ rec, const double *__restrict__ T,
                                                        const double *__restrict__ CN, long n, const int *__restrict__ ell, long ld,
                                                        const int *__restrict__ cnt, double *__restrict__ GR, double *__restrict__ GP,
                                                        double *__restrict__ GV, BoxTable bt)
 {
-    __shared__ R sd[5][64], sw[3][64], srr[64];
-    __shared__ int sf[64];
+    __shared__ R sd[5][kRowThreads], sw[3][kRowThreads], srr[kRowThreads];
+    __shared__ int sf[kRowThreads];
+    __shared__ unsigned long long smask[kRowThreads / 64];
     const long r = xcd_block();
     if (r >= nrows) return;
-    const int lane = threadIdx.x;
+    const int lane = threadIdx.x;  // thread of the workgroup
     const int i = act[r];
     {
         GridView g{};
@@ -686,7 +701,7 @@ __global__ void __launch_bounds__(64) k_gradients_rows(DevParams P, int nrows, c
         kernel_hinv(hi, hinv, hinv3, hinv4);
         const R h2i = hi * hi;
         const int c = cnt[r];
-        for (int base = 0; base < c; base += 64) {
+        for (int base = 0; base < c; base += kRowThreads) {
             const int q = base + lane;
             int f = 0;
             if (q < c) {
@@ -718,16 +733,19 @@ __global__ void __launch_bounds__(64) k_gradients_rows(DevParams P, int nrows, c
                     }
                 }
             }
-            // lanes 0-14 keep G[kk][b] (kk = lane/5, b = lane%5), 15-19 the minima, 20-24 the maxima,
-            // 25 the largest distance (with mn <= 0 <= mx the reference's else-if is two ifs)
-            unsigned long long mask = __ballot(f != 0);
-            if (!mask) continue;
+            // threads 0-14 keep G[kk][b] (kk = lane/5, b = lane%5), 15-19 the minima, 20-24 the
+            // maxima, 25 the largest distance (with mn <= 0 <= mx the reference's else-if is two
+            // ifs); entries in order, wavefront by wavefront
+            const unsigned long long mask = __ballot(f != 0);
             sf[lane] = f;
+            if ((lane & 63) == 0) smask[lane >> 6] = mask;
             __syncthreads();
             if (lane < 26) {
-                while (mask) {
-                    const int k = __ffsll((long long)mask) - 1;
-                    mask &= mask - 1;
+                for (int w = 0; w < kRowThreads / 64; w++) {
+                unsigned long long m = smask[w];
+                while (m) {
+                    const int k = w * 64 + __ffsll((long long)m) - 1;
+                    m &= m - 1;
                     if (lane < 15) {
                         if (sf[k] == 2) acc += sw[lane / 5][k] * sd[lane % 5][k];
                     } else if (lane < 20) {
@@ -740,6 +758,7 @@ __global__ void __launch_bounds__(64) k_gradients_rows(DevParams P, int nrows, c
                         const R rr = srr[k];
                         if (rr > acc) acc = rr;
                     }
+                }
                 }
             }
             __syncthreads();
@@ -760,7 +779,7 @@ __global__ void __launch_bounds__(64) k_gradients_rows(DevParams P, int nrows, c
 // force of few rows: each lane evaluates one list entry (hydro_accumulate_neighbor), lane 0
 // accumulates in list order, then the writeback and final operations
 template <int KERN, typename R>
-__global__ void __launch_bounds__(64) k_force_rows(DevParams P, Gam<R> G, int nrows, const int *__restrict__ act, ForceView F,
+__global__ void __launch_bounds__(kRowThreads) k_force_rows(DevParams P, Gam<R> G, int nrows, const int *__restrict__ act, ForceView F,
                                                    const Cell<R> *__restrict__ rec, const int *__restrict__ ell, long ld,
                                                    const int *__restrict__ cnt, const uint8_t *__restrict__ active,
                                                    const int *__restrict__ timebin, const double *__restrict__ msv_in,
@@ -768,10 +787,11 @@ __global__ void __launch_bounds__(64) k_force_rows(DevParams P, Gam<R> G, int nr
                                                    double *__restrict__ MSV, int *__restrict__ err, int *__restrict__ need_wakeup,
                                                    unsigned long long *__restrict__ pairs, BoxTable bt)
 {
-    __shared__ R sv[5][64];
+    __shared__ R sv[5][kRowThreads];
+    __shared__ unsigned long long smask[kRowThreads / 64];
     const long r = xcd_block();
     if (r >= nrows) return;
-    const int lane = threadIdx.x;
+    const int lane = threadIdx.x;  // thread of the workgroup
     const int i = act[r];
     {
         GridView g{};
@@ -795,7 +815,7 @@ __global__ void __launch_bounds__(64) k_force_rows(DevParams P, Gam<R> G, int nr
         R acc = 0;       // lane 4
         CSum<R> sum;     // lanes 0-3
         if (lane == 4 && c > 0) acc = (acc < I.cs) ? I.cs : acc;
-        for (int base = 0; base < c; base += 64) {
+        for (int base = 0; base < c; base += kRowThreads) {
             const int q = base + lane;
             int rc = 0;
             if (q < c) {
@@ -824,15 +844,18 @@ __global__ void __launch_bounds__(64) k_force_rows(DevParams P, Gam<R> G, int nr
                     }
                 }
             }
-            unsigned long long mask = __ballot(rc > 0);
-            if (!mask) continue;
+            const unsigned long long mask = __ballot(rc > 0);
+            if ((lane & 63) == 0) smask[lane >> 6] = mask;
             __syncthreads();
             if (lane < 5) {
-                while (mask) {
-                    const int k = __ffsll((long long)mask) - 1;
-                    mask &= mask - 1;
-                    if (lane < 4) sum.add(sv[lane][k]);
-                    else if (sv[4][k] > acc) acc = sv[4][k];
+                for (int w = 0; w < kRowThreads / 64; w++) {
+                    unsigned long long m = smask[w];
+                    while (m) {
+                        const int k = w * 64 + __ffsll((long long)m) - 1;
+                        m &= m - 1;
+                        if (lane < 4) sum.add(sv[lane][k]);
+                        else if (sv[4][k] > acc) acc = sv[4][k];
+                    }
                 }
             }
             __syncthreads();
