@@ -97,6 +97,8 @@ struct gz_context {
     // active rows
     int *act = nullptr;       // slot -> device index
     int nact = 0;
+    unsigned long long *act_bits = nullptr;  // the timestep loop's active list: row flags per block
+    int *act_bcount = nullptr;               // and rows per block
 
     // grid
     int level = -1;
@@ -345,6 +347,7 @@ void free_state(gz_context *c)
     dfree(c->wstart); dfree(c->wend); dfree(c->wlstart); dfree(c->wlend); dfree(c->lidx); dfree(c->plev); dfree(c->lcount);
     dfree(c->wide_dev);
     {
+        dfree(c->act_bits); dfree(c->act_bcount);
         dfree(c->sched); dfree(c->sched_val); dfree(c->sched_key); dfree(c->sched_key2); dfree(c->sched_part);
         c->sched_valid = false;
         RowGroups &G = c->grp;
@@ -1174,16 +1177,17 @@ int run_step(gz_context *ctx)
                                                                      ctx->u64c + 64, ctx->u64c + 66, BoxTable{});
     }
     {
+        // make_list_of_active_particles: row flags and counts per block of particles, then the
+        // list in ascending order (no scan over every particle)
         Timed t(ctx, "timeline");
-        k_active_flags<<<nblk(n + 1), kBlock, 0, ctx->stream>>>(n, ctx->u64c + 70, ctx->TB, ctx->M, ctx->H,
-                                                                0.99 * ctx->par.max_kernel_radius, ctx->ACT, ctx->SOFT, ctx->iflag,
-                                                                ctx->u64c + 67, BoxTable{});
-        HIPCHK(hipGetLastError());
-        size_t bytes = 0;
-        HIPCHK(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, ctx->iflag, ctx->iscan, (int)(n + 1), ctx->stream));
-        CHK(ensure_tmp(ctx, bytes));
-        HIPCHK(hipcub::DeviceScan::ExclusiveSum(ctx->tmp, bytes, ctx->iflag, ctx->iscan, (int)(n + 1), ctx->stream));
-        k_compact_count<<<nblk(n + 1), kBlock, 0, ctx->stream>>>(n, ctx->iflag, ctx->iscan, ctx->act, ctx->counters + 10);
+        const int nb_act = (int)((n + kActChunk - 1) / kActChunk);
+        if (!ctx->act_bits) {
+            const long m = (ctx->cap + kActChunk - 1) / kActChunk;
+            CHK(dalloc(ctx, &ctx->act_bits, (size_t)m * (kActChunk / 64))); CHK(dalloc(ctx, &ctx->act_bcount, m));
+        }
+        k_active_count<<<nb_act, 256, 0, ctx->stream>>>(n, ctx->u64c + 70, ctx->TB, ctx->M, ctx->H, 0.99 * ctx->par.max_kernel_radius,
+                                                        ctx->ACT, ctx->SOFT, ctx->act_bits, ctx->act_bcount, ctx->u64c + 67);
+        k_active_compact<<<nb_act, 256, 0, ctx->stream>>>(n, nb_act, ctx->act_bits, ctx->act_bcount, ctx->act, ctx->counters + 10);
         HIPCHK(hipGetLastError());
     }
     ctx->need_wakeup = 0;

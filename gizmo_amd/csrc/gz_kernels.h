@@ -160,6 +160,10 @@ __global__ void k_next_sync(TimeParams T, const unsigned long long *count, unsig
 __global__ void k_active_flags(long n, const unsigned long long *tl, const int *TB, const double *M, const double *H, double hclamp,
                                uint8_t *ACT, double *SOFT, int *flag, unsigned long long *hmax_act, BoxTable bt);
 __global__ void k_compact_count(long n, const int *flag, const int *scan, int *act, int *count_out);
+constexpr int kActChunk = 2048;  // particles per block of the active-list kernels
+__global__ void k_active_count(long n, const unsigned long long *tl, const int *TB, const double *M, const double *H, double hclamp,
+                               uint8_t *ACT, double *SOFT, unsigned long long *bits, int *bcount, unsigned long long *hmax_act);
+__global__ void k_active_compact(long n, int nblocks, const unsigned long long *bits, const int *bcount, int *act, int *count_out);
 __global__ void k_advance_boxes(BoxTable bt);
 __global__ void k_active_from_bins(long n, unsigned long long mask, const int *TB, const double *M, const double *H,
                                    uint8_t *ACT, double *SOFT);

@@ -477,6 +477,83 @@ __global__ void __launch_bounds__(256) k_active_flags(long n, const unsigned lon
     wave_max_to(h, valid, box, hmax_act, bt, kSlotHrows);
 }
 
+// The active list of one box without a scan over every particle: blocks of kActChunk
+// particles (thread t takes particles t, t+256, ...), each wavefront's 64 row flags kept as a
+// ballot.  k_active_count: the flags of k_active_flags and each block's row count; then
+// k_active_compact: each block's offset (the counts of the blocks before it) and the rows in
+// ascending order.
+__global__ void __launch_bounds__(256) k_active_count(long n, const unsigned long long *__restrict__ tl, const int *__restrict__ TB,
+                                                      const double *__restrict__ M, const double *__restrict__ H, double hclamp,
+                                                      uint8_t *__restrict__ ACT, double *__restrict__ SOFT,
+                                                      unsigned long long *__restrict__ bits, int *__restrict__ bcount,
+                                                      unsigned long long *__restrict__ hmax_act)
+{
+    __shared__ int wcount[4];
+    const long base = (long)blockIdx.x * kActChunk;
+    const int tid = threadIdx.x, w = tid >> 6;
+    const unsigned long long mask = tl[1];
+    double hm = 0;
+    int count = 0;
+    for (int k = 0; k < kActChunk / 256; k++) {
+        const long i = base + k * 256 + tid;
+        bool row = false;
+        if (i < n) {
+            const bool on = bin_on(mask, TB[i]);
+            ACT[i] = on ? 1 : 0;
+            row = on && (M[i] > 0);
+            if (row) {
+                double h = H[i];
+                SOFT[i] = h;
+                if ((h < 0) || !isfinite(h) || (h > hclamp)) h = hclamp;
+                hm = fmax(hm, h);
+            }
+        }
+        const unsigned long long b = __ballot(row);
+        if ((tid & 63) == 0) bits[(long)blockIdx.x * (kActChunk / 64) + k * 4 + w] = b;
+        count += __popcll(b);
+    }
+    __shared__ double whm[4];
+    for (int o = 32; o > 0; o >>= 1) hm = fmax(hm, __shfl_xor(hm, o, 64));
+    if ((tid & 63) == 0) { wcount[w] = count; whm[w] = hm; }
+    __syncthreads();
+    if (tid == 0) {
+        bcount[blockIdx.x] = wcount[0] + wcount[1] + wcount[2] + wcount[3];
+        hm = fmax(fmax(whm[0], whm[1]), fmax(whm[2], whm[3]));
+        if (hm > 0) atomicMax(hmax_act, (unsigned long long)__double_as_longlong(hm));
+    }
+}
+
+__global__ void __launch_bounds__(256) k_active_compact(long n, int nblocks, const unsigned long long *__restrict__ bits,
+                                                        const int *__restrict__ bcount, int *__restrict__ act, int *__restrict__ count_out)
+{
+    __shared__ int spart[4];
+    __shared__ int sprefix[kActChunk / 64 + 1];
+    const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+    // the rows of the blocks before this one
+    int before = 0;
+    for (int b = tid; b < (int)blockIdx.x; b += 256) before += bcount[b];
+    for (int o = 32; o > 0; o >>= 1) before += __shfl_xor(before, o, 64);
+    if (lane == 0) spart[w] = before;
+    const unsigned long long *mb = bits + (long)blockIdx.x * (kActChunk / 64);
+    if (tid == 0) {
+        int acc = 0;
+        for (int q = 0; q < kActChunk / 64; q++) { sprefix[q] = acc; acc += __popcll(mb[q]); }
+        sprefix[kActChunk / 64] = acc;
+    }
+    __syncthreads();
+    const int off = spart[0] + spart[1] + spart[2] + spart[3];
+    const long base = (long)blockIdx.x * kActChunk;
+    for (int k = 0; k < kActChunk / 256; k++) {
+        const int q = k * 4 + w;
+        const unsigned long long m = mb[q];
+        if ((m >> lane) & 1ull) {
+            const int below = __popcll(m & ((lane == 0) ? 0ull : (~0ull >> (64 - lane))));
+            act[off + sprefix[q] + below] = (int)(base + k * 256 + tid);
+        }
+    }
+    if (blockIdx.x == nblocks - 1 && tid == 0) *count_out = off + sprefix[kActChunk / 64];
+}
+
 // the scanned flags to the active list; the count to count_out
 __global__ void __launch_bounds__(256) k_compact_count(long n, const int *__restrict__ flag, const int *__restrict__ scan,
                                                         int *__restrict__ act, int *__restrict__ count_out)
