@@ -752,16 +752,22 @@ int density_stage(gz_context *ctx)
     DensityIter it{ctx->it_h, ctx->it_hb, ctx->it_l, ctx->it_r, ctx->it_cn, ctx->it_ci, ctx->it_it};
     DensityOut out{ctx->n, ctx->H, ctx->NN, ctx->DRK, ctx->DIVV, ctx->RHO, ctx->PR, ctx->T, ctx->FW, ctx->CN, ctx->FCE};
     int iters = 0;
+    int remaining = na;  // rows of the coming round
     while (true) {
         HIPCHK(hipMemsetAsync(ctx->counters, 0, 3 * sizeof(int), ctx->stream));
         {
-            // over many levels the rounds take one wavefront per row whatever their number: the
-            // cells of the rows' own levels hold too few rows to fill a wavefront
+            // over many levels the rounds take one workgroup per row whatever their number (the
+            // cells of the rows' own levels hold too few rows to fill a wavefront): four
+            // wavefronts per row while few rows iterate, one while many do
             Timed t(ctx, "density");
-            if (few)
-                k_density_rows<KERN, R><<<xcd_grid(na), kRowThreads, 0, ctx->stream>>>(ctx->dp, grid_view(ctx), na, ctx->act, ctx->need, ctx->key,
-                                                                            ctx->X, ctx->Y, ctx->Z, ctx->VX, ctx->VY, ctx->VZ, ctx->M,
-                                                                            ctx->it_h, ctx->dacc, BoxTable{});
+            if (few && remaining <= ctx->few_rows)
+                k_density_rows<KERN, R, kRowThreads><<<xcd_grid(na), kRowThreads, 0, ctx->stream>>>(
+                    ctx->dp, grid_view(ctx), na, ctx->act, ctx->need, ctx->key, ctx->X, ctx->Y, ctx->Z, ctx->VX, ctx->VY, ctx->VZ, ctx->M,
+                    ctx->it_h, ctx->dacc, BoxTable{});
+            else if (few)
+                k_density_rows<KERN, R, 64><<<xcd_grid(na), 64, 0, ctx->stream>>>(
+                    ctx->dp, grid_view(ctx), na, ctx->act, ctx->need, ctx->key, ctx->X, ctx->Y, ctx->Z, ctx->VX, ctx->VY, ctx->VZ, ctx->M,
+                    ctx->it_h, ctx->dacc, BoxTable{});
             else
                 k_density_cells<KERN, R><<<xcd_grid(ctx->ncells), 64, 0, ctx->stream>>>(
                     ctx->dp, grid_view(ctx), cell_rows(ctx), ctx->act, ctx->need, ctx->X, ctx->Y, ctx->Z, ctx->VX, ctx->VY, ctx->VZ,
@@ -773,7 +779,8 @@ int density_stage(gz_context *ctx)
         }
         CHK(read_counters(ctx));
         iters++;
-        const int remaining = ctx->hcounters[0], rebuild = ctx->hcounters[1], err = ctx->hcounters[2];
+        remaining = ctx->hcounters[0];
+        const int rebuild = ctx->hcounters[1], err = ctx->hcounters[2];
         if (err) {
             ctx->stats["density_iterations"] = iters;
             if (err == GZ_STOP_MAXITER) return fail(ctx, GZ_STOP_MAXITER, "density(): kernel radius iteration did not converge in %d rounds", kMaxIter);

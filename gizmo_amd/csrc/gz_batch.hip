@@ -521,7 +521,7 @@ int density(gz_batch *b)
                                                                                       b->X, b->Y, b->Z, b->VX, b->VY, b->VZ, b->M,
                                                                                       b->it_h, b->dacc);
         } else {
-            k_density_rows<KERN, R><<<xcd_grid(na), kRowThreads, 0, b->stream>>>(b->dp, gview(b), na, b->act, b->need, b->key, b->X, b->Y, b->Z,
+            k_density_rows<KERN, R, kRowThreads><<<xcd_grid(na), kRowThreads, 0, b->stream>>>(b->dp, gview(b), na, b->act, b->need, b->key, b->X, b->Y, b->Z,
                                                                          b->VX, b->VY, b->VZ, b->M, b->it_h, b->dacc, bt);
         }
         k_density_after<<<nb(na), kB, 0, b->stream>>>(b->dp, na, b->act, b->need, b->dacc, b->M, b->U, 0, it, out, b->counters + 0,

@@ -259,7 +259,9 @@ __global__ void k_gradients(DevParams P, int nrows, const int *act, const double
 // evaluate that many candidates or list entries at once, then lanes of the first wavefront add
 // the accepted terms in order
 constexpr int kRowThreads = 256;
-template <int KERN, typename R>
+// TPB threads per row: kRowThreads when few rows run (each row's rounds are latency-bound),
+// 64 when many do (lanes past a row's last candidate would idle)
+template <int KERN, typename R, int TPB>
 __global__ void k_density_rows(DevParams P, GridView g, int nrows, const int *act, const uint8_t *need, const uint32_t *key,
                                const double *X, const double *Y, const double *Z, const double *VX, const double *VY,
                                const double *VZ, const double *M, const double *hcur, double *accout, BoxTable bt);

@@ -26,6 +26,12 @@ template __global__ void k_build_ell<true>(int, const int *, const int *, const 
                                                      const double *, int *, double *, double *, double *, int *, int *,         \
                                                      unsigned long long *, const int *, BoxTable);
 
+#define GZ_INST_DROWS(KERN, R, TPB)                                                                                             \
+    template __global__ void k_density_rows<KERN, R, TPB>(DevParams, GridView, int, const int *, const uint8_t *,               \
+                                                          const uint32_t *, const double *, const double *, const double *,     \
+                                                          const double *, const double *, const double *, const double *,       \
+                                                          const double *, double *, BoxTable);
+
 #define GZ_INST(KERN, R)                                                                                                         \
     template __global__ void k_density_iter<KERN, R>(DevParams, int, const int *, const int *, const double *, const double *,  \
                                                      const double *, const double *, const double *, const double *,            \
@@ -36,10 +42,8 @@ template __global__ void k_build_ell<true>(int, const int *, const int *, const 
                                                       const double *, const double *, const double *, const double *, double *);\
     GZ_INST_LANES(KERN, R, false)                                                                                                \
     GZ_INST_LANES(KERN, R, true)                                                                                                 \
-    template __global__ void k_density_rows<KERN, R>(DevParams, GridView, int, const int *, const uint8_t *, const uint32_t *,  \
-                                                     const double *, const double *, const double *, const double *,            \
-                                                     const double *, const double *, const double *, const double *, double *,  \
-                                                     BoxTable);                                                                  \
+    GZ_INST_DROWS(KERN, R, 64)                                                                                                   \
+    GZ_INST_DROWS(KERN, R, kRowThreads)                                                                                          \
     template __global__ void k_gradients_rows<KERN, R>(DevParams, int, const int *, const double *, const double *,             \
                                                        const double *, const GradRec<R> *, const double *, const double *, long, \
                                                        const int *, long, const int *, double *, double *, double *, BoxTable); \

@@ -166,8 +166,8 @@ __device__ __forceinline__ void row_stencil_wide(const GridView &g, const WideGr
 
 // density sums of the rows still iterating (DensitySpec::pair_kernel,
 // src/hydro/density_loop.h:1141-1385), the terms in k_density_cells' order
-template <int KERN, typename R>
-__global__ void __launch_bounds__(kRowThreads) k_density_rows(DevParams P, GridView g, int nrows, const int *__restrict__ act,
+template <int KERN, typename R, int TPB>
+__global__ void __launch_bounds__(TPB) k_density_rows(DevParams P, GridView g, int nrows, const int *__restrict__ act,
                                                      const uint8_t *__restrict__ need, const uint32_t *__restrict__ key,
                                                      const double *__restrict__ X, const double *__restrict__ Y,
                                                      const double *__restrict__ Z, const double *__restrict__ VX,
@@ -176,9 +176,9 @@ __global__ void __launch_bounds__(kRowThreads) k_density_rows(DevParams P, GridV
                                                      double *__restrict__ accout, BoxTable bt)
 {
     __shared__ RowStencil rs;
-    __shared__ R sc[13][kRowThreads];
-    __shared__ int sf[kRowThreads];
-    __shared__ unsigned long long smask[kRowThreads / 64];
+    __shared__ R sc[13][TPB];
+    __shared__ int sf[TPB];
+    __shared__ unsigned long long smask[TPB / 64];
     const long r = xcd_block();
     if (r >= nrows || !need[r]) return;
     const int tid = threadIdx.x;
@@ -201,7 +201,7 @@ __global__ void __launch_bounds__(kRowThreads) k_density_rows(DevParams P, GridV
     }
     const int total = rs.pref[rs.nst];
     R acc = 0;  // thread q < 13 keeps sum q
-    for (int base = 0; base < total; base += kRowThreads) {
+    for (int base = 0; base < total; base += TPB) {
         const int v = base + tid;
         int f = 0;
         if (v < total) {
@@ -246,7 +246,7 @@ __global__ void __launch_bounds__(kRowThreads) k_density_rows(DevParams P, GridV
         if ((tid & 63) == 0) smask[tid >> 6] = mask;
         __syncthreads();
         if (tid < 13) {
-            for (int w = 0; w < kRowThreads / 64; w++) {
+            for (int w = 0; w < TPB / 64; w++) {
                 unsigned long long m = smask[w];
                 while (m) {
                     const int k = w * 64 + __ffsll((long long)m) - 1;
