@@ -81,8 +81,7 @@ struct gz_context {
     // the symmetric list of many rows over many levels: rows grouped by the cell of their own level
     RowGroups grp;
     // the force pass's row order over the lanes (longest lists first within blocks of rows)
-    int *sched = nullptr, *sched_val = nullptr;
-    uint32_t *sched_key = nullptr, *sched_key2 = nullptr;
+    int *sched = nullptr;
     unsigned long long *sched_part = nullptr;
     int sched_span = 2048;  // rows per sorted block (0: plain order)
     bool sched_valid = false;
@@ -348,7 +347,7 @@ void free_state(gz_context *c)
     dfree(c->wide_dev);
     {
         dfree(c->act_bits); dfree(c->act_bcount);
-        dfree(c->sched); dfree(c->sched_val); dfree(c->sched_key); dfree(c->sched_key2); dfree(c->sched_part);
+        dfree(c->sched); dfree(c->sched_part);
         c->sched_valid = false;
         RowGroups &G = c->grp;
         dfree(G.key); dfree(G.skey); dfree(G.gkey); dfree(G.val); dfree(G.row); dfree(G.flag); dfree(G.scan); dfree(G.start);
@@ -817,26 +816,18 @@ int build_sched(gz_context *ctx)
     const int na = ctx->nact;
     const int nparts = (int)((na + 1023) / 1024);
     if (!ctx->sched) {
-        CHK(dalloc(ctx, &ctx->sched, ctx->cap)); CHK(dalloc(ctx, &ctx->sched_val, ctx->cap));
-        CHK(dalloc(ctx, &ctx->sched_key, ctx->cap)); CHK(dalloc(ctx, &ctx->sched_key2, ctx->cap));
+        CHK(dalloc(ctx, &ctx->sched, ctx->cap));
         CHK(dalloc(ctx, &ctx->sched_part, 2 * ((ctx->cap + 1023) / 1024) + 2));
     }
-    int end_bit = 11;
-    while ((((long)na - 1) / ctx->sched_span) >> (end_bit - 11)) end_bit++;
     // FP64 pairs are compute-bound: always sorted; FP32 pairs only when the waves are uneven
     const double min_even = ctx->par.precision == GZ_FP64 ? 2.0 : 0.75;
     unsigned long long *total = ctx->sched_part + 2 * nparts;
+    const int span = ctx->sched_span;
     Timed t(ctx, "ngb_build");
     k_sched_stats<<<nparts, 1024, 0, ctx->stream>>>(na, ctx->cnt_s, ctx->sched_part);
     k_sched_total<<<1, 1024, 0, ctx->stream>>>(nparts, ctx->sched_part, total);
-    k_sched_key<<<nblk(na), kBlock, 0, ctx->stream>>>(na, ctx->sched_span, ctx->cnt_s, total, min_even, ctx->sched_key, ctx->sched_val);
+    k_sched_sort<<<(unsigned)((na + span - 1) / span), 1024, 0, ctx->stream>>>(na, span, ctx->cnt_s, total, min_even, ctx->sched);
     HIPCHK(hipGetLastError());
-    size_t bytes = 0;
-    HIPCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, ctx->sched_key, ctx->sched_key2, ctx->sched_val, ctx->sched, na, 0, end_bit,
-                                              ctx->stream));
-    CHK(ensure_tmp(ctx, bytes));
-    HIPCHK(hipcub::DeviceRadixSort::SortPairs(ctx->tmp, bytes, ctx->sched_key, ctx->sched_key2, ctx->sched_val, ctx->sched, na, 0, end_bit,
-                                              ctx->stream));
     ctx->sched_valid = true;
     return 0;
 }
@@ -1624,7 +1615,8 @@ int gz_set_option(gz_context *ctx, const char *name, double value)
     }
     if (k == "row_schedule") {
         const long v = (long)value;
-        if ((double)v != value || v < 0 || v > (1L << 20) || v % 64) return fail(ctx, GZ_E_ARG, "row_schedule must be 0 or a multiple of 64 up to 2^20");
+        if ((double)v != value || v < 0 || v > kSchedMaxSpan || (v & (v - 1)) || (v > 0 && v < 64))
+            return fail(ctx, GZ_E_ARG, "row_schedule must be 0 or a power of two in [64, %d]", kSchedMaxSpan);
         ctx->sched_span = (int)v;
         ctx->sym_valid = false;
         return 0;

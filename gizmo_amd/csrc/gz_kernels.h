@@ -289,7 +289,7 @@ __global__ void k_force(DevParams P, Gam<R> G, int nrows, const int *act, ForceV
 // min_even; the scattered reads of the sorted order cost more than even waves save otherwise)
 __global__ void k_sched_stats(int nrows, const int *cnt, unsigned long long *partial);
 __global__ void k_sched_total(int nparts, const unsigned long long *partial, unsigned long long *total);
-__global__ void k_sched_key(int nrows, int span, const int *cnt, const unsigned long long *total, double min_even, uint32_t *key,
-                            int *val);
+constexpr int kSchedMaxSpan = 2048;
+__global__ void k_sched_sort(int nrows, int span, const int *cnt, const unsigned long long *total, double min_even, int *sched);
 
 }  // namespace gz

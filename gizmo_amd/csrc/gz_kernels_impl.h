@@ -11,6 +11,7 @@
 // 64 lanes of a wave (64 consecutive rows) read one coalesced line per q.  One lane per
 // active particle; wave64.
 #pragma once
+#include <type_traits>
 #include "gz_kernels.h"
 
 namespace gz {
@@ -578,6 +579,13 @@ __global__ void __launch_bounds__(64) k_density_cells(DevParams P, GridView g, C
         }
         const LeafBox lb = leaf_box(g, c, on, xi, yi, zi);
         const double reach = wave_max(on ? h : 0.0);
+        // FP32 form on grids of at least 8 cells a side: positions relative to the home cell's
+        // origin (one FP64 difference and wrap per row and per staged candidate, rounded to
+        // FP32; every partner then lies within half a box of the origin), so the pair
+        // differences are FP32 subtractions
+        const bool rel = std::is_same<R, float>::value && (1 << g.level) >= 8;
+        const R rxi = (R)wrap(xi - lb.ox, g.box, g.half), ryi = (R)wrap(yi - lb.oy, g.box, g.half),
+                rzi = (R)wrap(zi - lb.oz, g.box, g.half);
         R a[13];
         for (int q = 0; q < 13; q++) a[q] = 0;
         const R hR = (R)(on ? h : 1.0);
@@ -598,7 +606,13 @@ __global__ void __launch_bounds__(64) k_density_cells(DevParams P, GridView g, C
                 const int pos = kept_rank(keep, nt);
                 __syncthreads();
                 if (keep) {
-                    sx[pos] = xj; sy[pos] = yj; sz[pos] = zj;
+                    if (rel) {
+                        sx[pos] = (R)wrap(xj - lb.ox, g.box, g.half);
+                        sy[pos] = (R)wrap(yj - lb.oy, g.box, g.half);
+                        sz[pos] = (R)wrap(zj - lb.oz, g.box, g.half);
+                    } else {
+                        sx[pos] = xj; sy[pos] = yj; sz[pos] = zj;
+                    }
                     svx[pos] = VX[j]; svy[pos] = VY[j]; svz[pos] = VZ[j]; sm[pos] = M[j];
                 }
                 __syncthreads();
@@ -606,11 +620,16 @@ __global__ void __launch_bounds__(64) k_density_cells(DevParams P, GridView g, C
                 for (int k = 0; k < nt; k++) {
                     const double mj = sm[k];
                     if (mj <= 0) continue;
-                    double d0 = xi - sx[k], d1 = yi - sy[k], d2 = zi - sz[k];
-                    d1 = wrap(d1, P.box, P.half);
-                    d0 = wrap(d0, P.box, P.half);
-                    d2 = wrap(d2, P.box, P.half);
-                    const R dp0 = (R)d0, dp1 = (R)d1, dp2 = (R)d2;
+                    R dp0, dp1, dp2;
+                    if (rel) {
+                        dp0 = rxi - (R)sx[k]; dp1 = ryi - (R)sy[k]; dp2 = rzi - (R)sz[k];
+                    } else {
+                        double d0 = xi - sx[k], d1 = yi - sy[k], d2 = zi - sz[k];
+                        d1 = wrap(d1, P.box, P.half);
+                        d0 = wrap(d0, P.box, P.half);
+                        d2 = wrap(d2, P.box, P.half);
+                        dp0 = (R)d0; dp1 = (R)d1; dp2 = (R)d2;
+                    }
                     const R r2 = dp0 * dp0 + dp1 * dp1 + dp2 * dp2;
                     if (r2 >= h2) continue;
                     const R rr = sqrt(r2);
@@ -1100,19 +1119,42 @@ __global__ void __launch_bounds__(1024) k_sched_total(int nparts, const unsigned
     }
 }
 
-// sort key of each row for the force pass's order: its block of span rows, then its list
-// length descending (lengths above 2047 tie); the rows of a block stay close in space, their
-// waves even out.  Even waves: the key keeps the plain order.
-__global__ void __launch_bounds__(256) k_sched_key(int nrows, int span, const int *__restrict__ cnt,
-                                                    const unsigned long long *__restrict__ total, double min_even,
-                                                    uint32_t *__restrict__ key, int *__restrict__ val)
+// The force pass's order of each block of span rows (span a power of two, at most
+// kSchedMaxSpan): longest list first, ties in row order (lengths above 2047 tie), by a bitonic
+// sort of (2047 - length, row) keys in LDS; the plain order when the waves are even.  The rows
+// of a block stay close in space; their waves even out.
+__global__ void __launch_bounds__(1024) k_sched_sort(int nrows, int span, const int *__restrict__ cnt,
+                                                     const unsigned long long *__restrict__ total, double min_even,
+                                                     int *__restrict__ sched)
 {
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= nrows) return;
+    __shared__ unsigned int key[kSchedMaxSpan];
+    const int base = blockIdx.x * span;
+    const int m = min(span, nrows - base);  // rows of this block
     const double even = (double)total[0] / (64.0 * (double)(total[1] > 0 ? total[1] : 1));
-    const int c = even < min_even ? min(cnt[r], 2047) : 2047;
-    key[r] = ((uint32_t)(r / span) << 11) | (uint32_t)(2047 - c);
-    val[r] = r;
+    if (!(even < min_even)) {
+        for (int k = threadIdx.x; k < m; k += blockDim.x) sched[base + k] = base + k;
+        return;
+    }
+    for (int k = threadIdx.x; k < span; k += blockDim.x) {
+        // rows past the end sort last
+        const unsigned c = k < m ? (unsigned)min(cnt[base + k], 2047) : 0u;
+        key[k] = k < m ? (((2047u - c) << 11) | (unsigned)k) : 0xffffffffu;
+    }
+    __syncthreads();
+    for (int size = 2; size <= span; size <<= 1) {
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+            for (int k = threadIdx.x; k < span / 2; k += blockDim.x) {
+                // the pair (lo, hi) of this compare-exchange, ascending within blocks of size
+                const int lo = 2 * k - (k & (stride - 1));
+                const int hi = lo + stride;
+                const bool up = (lo & size) == 0;
+                const unsigned a = key[lo], b2 = key[hi];
+                if ((a > b2) == up) { key[lo] = b2; key[hi] = a; }
+            }
+            __syncthreads();
+        }
+    }
+    for (int k = threadIdx.x; k < m; k += blockDim.x) sched[base + k] = base + (int)(key[k] & 2047u);
 }
 #endif
 

@@ -186,7 +186,7 @@ int gz_timeline_get(gz_context *ctx, gz_timeline *out);
    for radii that span orders of magnitude, where one grid sized by the largest radius would
    walk far too many candidates; same neighbour sets, same sums (the symmetric list of more than
    few_rows rows is then built one wavefront per cell of the rows' own levels, the density
-   rounds one wavefront per row); "row_schedule" (default 2048; 0 or a multiple of 64) -- the one-lane-per-particle force pass
+   rounds one wavefront per row); "row_schedule" (default 2048; 0 or a power of two from 64 to 2048) -- the one-lane-per-particle force pass
    takes the rows of each block of this many longest list first (a wavefront lasts as long as
    its longest list; 0: plain order).  Neither changes a sum.  None of these options has a counterpart in the reference (tuning knobs of this
    library). */
