@@ -6,8 +6,9 @@ Flags:
   * --offload-arch=gfx950: MI355X only.
   * -ffp-contract=off for the FP64 kernels and the host code: the FP64 form keeps the
     reference's rounding order term by term, so it can be compared to the CPU oracle bit for
-    bit.  The FP32-mixed kernels (gz_kernels_f32.hip) contract to FMA and use the fast FP32
-    divide and square root.
+    bit.  The FP32-mixed kernels (gz_kernels_f32.hip) contract to FMA within each expression
+    (so the kernel shapes round alike and agree bit for bit) and use the fast FP32 divide and
+    square root.
   * -Xclang -target-feature -Xclang -packed-fp32-ops: no packed FP32 instructions.
 """
 import os
@@ -33,12 +34,14 @@ def hipcc():
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wno-unused-result"]
 # per source: the FP64 form and the host code keep strict rounding; the FP32-mixed kernels
-# contract to FMA and use the fast FP32 divide/square root
+# contract to FMA within each expression (not across statements: the kernel shapes, which
+# spread one pair's terms over different statements, then round them alike and give the same
+# bits) and use the fast FP32 divide/square root
 SOURCE_FLAGS = {
     "gz_kernels.hip": ["-ffp-contract=off"],
     "gz_api.hip": ["-ffp-contract=off"],
     "gz_batch.hip": ["-ffp-contract=off"],
-    "gz_kernels_f32.hip": ["-ffp-contract=fast", "-fno-hip-fp32-correctly-rounded-divide-sqrt"],
+    "gz_kernels_f32.hip": ["-ffp-contract=on", "-fno-hip-fp32-correctly-rounded-divide-sqrt"],
 }
 
 

@@ -317,6 +317,8 @@ int alloc_state(gz_context *ctx, long cap)
     CHK(dalloc(ctx, &ctx->cells, n)); CHK(dalloc(ctx, &ctx->cflag, n + 1)); CHK(dalloc(ctx, &ctx->cscan, n + 1));
     CHK(dalloc(ctx, &ctx->need, n));
     CHK(dalloc(ctx, &ctx->dacc, 13 * n));
+    static_assert(sizeof(GradRec<float>) <= sizeof(GradRec<double>) && sizeof(Cell<float>) <= sizeof(Cell<double>),
+                  "the record buffers hold either form");
     CHK(dalloc(ctx, (char **)&ctx->rec_grad, n * sizeof(GradRec<double>)));
     CHK(dalloc(ctx, (char **)&ctx->rec_force, n * sizeof(Cell<double>)));
     CHK(dalloc(ctx, &ctx->VKX, n)); CHK(dalloc(ctx, &ctx->VKY, n)); CHK(dalloc(ctx, &ctx->VKZ, n));
@@ -918,7 +920,8 @@ int gradients_stage(gz_context *ctx)
     Timed t(ctx, "gradients");
     GradRec<R> *rec = (GradRec<R> *)ctx->rec_grad;
     k_pack_grad<R><<<nblk(ctx->n), kBlock, 0, ctx->stream>>>(ctx->n, ctx->rec_need_valid ? ctx->rec_need : nullptr, ctx->H, ctx->M,
-                                                             ctx->RHO, ctx->PR, ctx->VX, ctx->VY, ctx->VZ, rec);
+                                                             ctx->RHO, ctx->PR, ctx->VX, ctx->VY, ctx->VZ, rec, ctx->X, ctx->Y, ctx->Z,
+                                                             ctx->par.box_size, BoxTable{});
     if (ctx->nact <= ctx->few_rows)
         k_gradients_rows<KERN, R><<<xcd_grid(ctx->nact), kRowThreads, 0, ctx->stream>>>(ctx->dp, ctx->nact, ctx->act, ctx->X, ctx->Y, ctx->Z,
                                                                                rec, ctx->T, ctx->CN, ctx->n, ctx->ell_s, ctx->ld_s,
@@ -944,7 +947,7 @@ int force_stage(gz_context *ctx)
         Timed t(ctx, "force");
         Cell<R> *rec = (Cell<R> *)ctx->rec_force;
         k_pack_force<R><<<nblk(ctx->n), kBlock, 0, ctx->stream>>>(ctx->par.gamma, F, ctx->rec_need_valid ? ctx->rec_need : nullptr, rec,
-                                                                  ctx->counters + 2);
+                                                                  ctx->counters + 2, ctx->par.box_size, BoxTable{});
         if (ctx->nact <= ctx->few_rows)
             k_force_rows<KERN, R><<<xcd_grid(ctx->nact), kRowThreads, 0, ctx->stream>>>(
                 ctx->dp, gam<R>(ctx->dp), ctx->nact, ctx->act, F, rec, ctx->ell_s, ctx->ld_s, ctx->cnt_s,

@@ -596,7 +596,8 @@ int gradients_force(gz_batch *b)
     const long ld = ld_rows(na);
     const BoxTable bt = table(b);
     GradRec<R> *rg = (GradRec<R> *)b->rec_grad;
-    k_pack_grad<R><<<nb(b->cap), kB, 0, b->stream>>>(b->cap, b->rec_need, b->H, b->M, b->RHO, b->PR, b->VX, b->VY, b->VZ, rg);
+    k_pack_grad<R><<<nb(b->cap), kB, 0, b->stream>>>(b->cap, b->rec_need, b->H, b->M, b->RHO, b->PR, b->VX, b->VY, b->VZ, rg, b->X, b->Y,
+                                                      b->Z, 1.0, bt);
     const bool lanes = na > b->few_rows;  // many rows: one lane per row
     if (lanes)
         k_gradients<KERN, R, true><<<xcd_grid(nb(na)), kB, 0, b->stream>>>(b->dp, na, b->act, b->X, b->Y, b->Z, rg, b->T, b->CN, b->cap, b->ell, ld,
@@ -611,7 +612,7 @@ int gradients_force(gz_batch *b)
     Gam<R> G;
     G.G0 = (R)b->dp.G0; G.G1 = (R)b->dp.G1; G.G2 = (R)b->dp.G2; G.G3 = (R)b->dp.G3; G.G4 = (R)b->dp.G4;
     G.G5 = (R)b->dp.G5; G.G6 = (R)b->dp.G6; G.G7 = (R)b->dp.G7; G.G8 = (R)b->dp.G8; G.G9 = (R)b->dp.G9;
-    k_pack_force<R><<<nb(b->cap), kB, 0, b->stream>>>(b->par.gamma, F, b->rec_need, rf, b->counters + 2);
+    k_pack_force<R><<<nb(b->cap), kB, 0, b->stream>>>(b->par.gamma, F, b->rec_need, rf, b->counters + 2, 1.0, bt);
     if (lanes) {
         k_force<KERN, R, true><<<xcd_grid(nb(na)), kB, 0, b->stream>>>(b->dp, G, na, b->act, F, rf, b->ell, ld, b->cnt, b->ACT, b->TB, b->MSV,
                                                                   b->WAKE, b->ACC, b->DTU, b->MSV, b->counters + 2, b->counters + 4, b->u64,

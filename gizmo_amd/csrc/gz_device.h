@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <stdint.h>
+#include <type_traits>
 
 namespace gz {
 
@@ -583,7 +584,63 @@ __device__ inline void riemann(const Gam<R> &G, const RState<R> &L, const RState
 // ---------------------------------------------------------------------------------------
 // per-particle record of the force pass (32 values: 256 B in FP64, 128 B in FP32), packed
 // once per pass so that a neighbour costs a few wide loads instead of 32 scattered ones
-template <typename R> struct alignas(16) Cell {
+// FP32 form: the value as rounded to FP32, so that no later addition fuses with the
+// multiplication that made it (the FP32 kernels are built with FMA contraction; the kernel
+// shapes then add the same rounded terms and give the same bits).  No effect on FP64.
+template <typename R>
+__device__ __forceinline__ R rounded(R v)
+{
+    if constexpr (std::is_same<R, float>::value) __asm__ volatile("" : "+v"(v));
+    return v;
+}
+
+// FP32 form: a particle's position as its cell of a 1024^3 grid over its box (10 bits per axis,
+// x in bits 20-29) and its FP32 offset from that cell's corner.  The FP64 form reads positions
+// from the FP64 arrays (PosRec<double> is empty).
+struct FinePos {
+    uint32_t c;
+    float o0, o1, o2;
+};
+template <typename R> struct PosRec {};
+template <> struct PosRec<float> { FinePos fp; };
+
+__device__ __forceinline__ FinePos fine_pos(double x, double y, double z, double box)
+{
+    const double cs = box * (1.0 / 1024.0);
+    int c[3];
+    float o[3];
+    const double v[3] = {x, y, z};
+    for (int a = 0; a < 3; a++) {
+        int k = (int)floor(v[a] / cs);
+        k = k < 0 ? 0 : (k > 1023 ? 1023 : k);
+        c[a] = k;
+        o[a] = (float)(v[a] - k * cs);
+    }
+    FinePos p;
+    p.c = ((uint32_t)c[0] << 20) | ((uint32_t)c[1] << 10) | (uint32_t)c[2];
+    p.o0 = o[0]; p.o1 = o[1]; p.o2 = o[2];
+    return p;
+}
+
+// x_i - x_j, periodic nearest image, in FP32: the cell difference is exact (integers, wrapped
+// over the 1024 cells of the box), scaled by the cell side, plus the offset difference.  Holds
+// for any pair closer than half a box.
+__device__ __forceinline__ void fine_dp(const FinePos &a, const FinePos &b, float cs, float &d0, float &d1, float &d2)
+{
+    int dc[3];
+    for (int k = 0; k < 3; k++) {
+        const int s = 20 - 10 * k;
+        int d = (int)((a.c >> s) & 1023u) - (int)((b.c >> s) & 1023u);
+        if (d > 512) d -= 1024;
+        else if (d < -512) d += 1024;
+        dc[k] = d;
+    }
+    d0 = (float)dc[0] * cs + (a.o0 - b.o0);
+    d1 = (float)dc[1] * cs + (a.o1 - b.o1);
+    d2 = (float)dc[2] * cs + (a.o2 - b.o2);
+}
+
+template <typename R> struct alignas(16) Cell : PosRec<R> {
     R h, m, rho, p, cs, cn, fce, drk;
     R v0, v1, v2;
     R gr0, gr1, gr2, gp0, gp1, gp2;
@@ -762,7 +819,7 @@ __device__ inline int force_pair(const DevParams &P, const Gam<R> &G, const Cell
 // the neighbour side of the gradient pass.  The differenced fields stay FP64 in both forms:
 // a difference of two O(1) densities or pressures formed in FP32 loses the digits the gradient
 // is made of (the mixed form then rounds the FP64 difference to FP32).
-template <typename R> struct alignas(16) GradRec {
+template <typename R> struct alignas(16) GradRec : PosRec<R> {
     R h, m;
     double rho, p, v0, v1, v2;
 };

@@ -245,9 +245,10 @@ __global__ void k_density_after(DevParams P, int nrows, const int *act, uint8_t 
 
 template <typename R>
 __global__ void k_pack_grad(long n, const uint8_t *need, const double *H, const double *M, const double *RHO, const double *PR,
-                            const double *VX, const double *VY, const double *VZ, GradRec<R> *rec);
+                            const double *VX, const double *VY, const double *VZ, GradRec<R> *rec, const double *X, const double *Y,
+                            const double *Z, double box, BoxTable bt);
 template <typename R>
-__global__ void k_pack_force(double gamma, ForceView F, const uint8_t *need, Cell<R> *rec, int *zero2);
+__global__ void k_pack_force(double gamma, ForceView F, const uint8_t *need, Cell<R> *rec, int *zero2, double box, BoxTable bt);
 
 // BOXES: each row's box geometry from bt (batches); else the single box of the arguments
 template <int KERN, typename R, bool BOXES>

@@ -49,8 +49,9 @@ namespace gz {
 GZ_INST(3, float)
 GZ_INST(6, float)
 template __global__ void k_pack_grad<float>(long, const uint8_t *, const double *, const double *, const double *, const double *, const double *,
-                                            const double *, const double *, GradRec<float> *);
-template __global__ void k_pack_force<float>(double, ForceView, const uint8_t *, Cell<float> *, int *);
+                                            const double *, const double *, GradRec<float> *, const double *,
+                                             const double *, const double *, double, BoxTable);
+template __global__ void k_pack_force<float>(double, ForceView, const uint8_t *, Cell<float> *, int *, double, BoxTable);
 #undef GZ_INST
 
 }  // namespace gz

@@ -637,21 +637,21 @@ __global__ void __launch_bounds__(64) k_density_cells(DevParams P, GridView g, C
                     R wk = 0, dwk = 0;
                     kernel_main<KERN, 0>(u, hinv3, hinv4, wk, dwk);
                     const R mj_wk = (R)mj * wk;
-                    a[0] += wk;
-                    a[1] += mj_wk;
-                    a[2] += -(R(3.0) * hinv * wk + u * dwk);
+                    a[0] += rounded(wk);
+                    a[1] += rounded(mj_wk);
+                    a[2] += rounded(-(R(3.0) * hinv * wk + u * dwk));
                     if (rr > 0) {
-                        a[4] += wk * (dp0 * dp0);
-                        a[5] += wk * (dp1 * dp1);
-                        a[6] += wk * (dp2 * dp2);
-                        a[7] += wk * (dp0 * dp1);
-                        a[8] += wk * (dp1 * dp2);
-                        a[9] += wk * (dp0 * dp2);
-                        a[10] += wk * dp0;
-                        a[11] += wk * dp1;
-                        a[12] += wk * dp2;
+                        a[4] += rounded(wk * (dp0 * dp0));
+                        a[5] += rounded(wk * (dp1 * dp1));
+                        a[6] += rounded(wk * (dp2 * dp2));
+                        a[7] += rounded(wk * (dp0 * dp1));
+                        a[8] += rounded(wk * (dp1 * dp2));
+                        a[9] += rounded(wk * (dp0 * dp2));
+                        a[10] += rounded(wk * dp0);
+                        a[11] += rounded(wk * dp1);
+                        a[12] += rounded(wk * dp2);
                         const R dv0 = vxi - (R)svx[k], dv1 = vyi - (R)svy[k], dv2 = vzi - (R)svz[k];
-                        a[3] -= dwk * (dp0 * dv0 + dp1 * dv1 + dp2 * dv2) / rr;
+                        a[3] -= rounded(dwk * (dp0 * dv0 + dp1 * dv1 + dp2 * dv2) / rr);
                     }
                 }
             }
@@ -807,7 +807,9 @@ __global__ void k_mark_records(int nrows, const int *__restrict__ act, const int
 template <typename R>
 __global__ void k_pack_grad(long n, const uint8_t *__restrict__ need, const double *__restrict__ H, const double *__restrict__ M,
                             const double *__restrict__ RHO, const double *__restrict__ PR, const double *__restrict__ VX,
-                            const double *__restrict__ VY, const double *__restrict__ VZ, GradRec<R> *__restrict__ rec)
+                            const double *__restrict__ VY, const double *__restrict__ VZ, GradRec<R> *__restrict__ rec,
+                            const double *__restrict__ X, const double *__restrict__ Y, const double *__restrict__ Z, double box,
+                            BoxTable bt)
 {
     const long j = blockIdx.x * (long)blockDim.x + threadIdx.x;
     if (j >= n) return;
@@ -815,12 +817,13 @@ __global__ void k_pack_grad(long n, const uint8_t *__restrict__ need, const doub
     GradRec<R> g;
     g.h = (R)H[j]; g.m = (R)M[j]; g.rho = RHO[j]; g.p = PR[j];
     g.v0 = VX[j]; g.v1 = VY[j]; g.v2 = VZ[j];
+    if constexpr (std::is_same<R, float>::value) g.fp = fine_pos(X[j], Y[j], Z[j], bt.b ? bt.b[bt.BOX[j]].box : box);
     rec[j] = g;
 }
 
 template <typename R>
 __global__ void k_pack_force(double gamma, ForceView F, const uint8_t *__restrict__ need, Cell<R> *__restrict__ rec,
-                             int *__restrict__ zero2)
+                             int *__restrict__ zero2, double box, BoxTable bt)
 {
     const long j = blockIdx.x * (long)blockDim.x + threadIdx.x;
     const long n = F.n;
@@ -839,6 +842,7 @@ __global__ void k_pack_force(double gamma, ForceView F, const uint8_t *__restric
     // effective_soundspeed, src/declarations/cell_data.h:703-712
     const double rho = F.RHO[j];
     c.cs = (R)((rho > 0) ? sqrt(gamma * F.PR[j] / rho) : sqrt(gamma * (gamma - 1.) * F.U[j]));
+    if constexpr (std::is_same<R, float>::value) c.fp = fine_pos(F.X[j], F.Y[j], F.Z[j], bt.b ? bt.b[bt.BOX[j]].box : box);
     rec[j] = c;
 }
 
@@ -912,11 +916,8 @@ __global__ void __launch_bounds__(256) k_gradients(DevParams P, int nrows, const
             const int j = ell[(size_t)q * ld + r];
             const GradRec<R> J = rec[j];
             if (!(J.m > 0) || !(J.rho > 0) || !(J.h > 0)) continue;
-            double d0 = xi - X[j], d1 = yi - Y[j], d2 = zi - Z[j];
-            d1 = wrap(d1, P.box, P.half);
-            d0 = wrap(d0, P.box, P.half);
-            d2 = wrap(d2, P.box, P.half);
-            const R dp[3] = {(R)d0, (R)d1, (R)d2};
+            R dp[3];
+            record_dp<R>(P, I, J, xi, yi, zi, X, Y, Z, j, dp[0], dp[1], dp[2]);
             const R r2 = dp[0] * dp[0] + dp[1] * dp[1] + dp[2] * dp[2];
             if (r2 <= 0) continue;
             if ((r2 >= h2i) && (r2 >= J.h * J.h)) continue;
@@ -940,7 +941,7 @@ __global__ void __launch_bounds__(256) k_gradients(DevParams P, int nrows, const
             if (rr < hi) {
                 for (int kk = 0; kk < 3; kk++) {
                     const R w = -wk * dp[kk];
-                    for (int b = 0; b < 5; b++) G[kk][b] += w * d[b];
+                    for (int b = 0; b < 5; b++) G[kk][b] += rounded(w * d[b]);
                 }
             }
         }
@@ -968,6 +969,24 @@ struct CSum<double> {
     __device__ __forceinline__ void add(double x) { s += x; }
     __device__ __forceinline__ double value() const { return s; }
 };
+
+// x_i - x_j of a pair of records: FP32 form from the records' fine cells and offsets (no FP64
+// position reads); FP64 form the wrapped FP64 difference of the positions
+template <typename R, typename Rec>
+__device__ __forceinline__ void record_dp(const DevParams &P, const Rec &I, const Rec &J, double xi, double yi, double zi,
+                                          const double *__restrict__ X, const double *__restrict__ Y,
+                                          const double *__restrict__ Z, int j, R &dp0, R &dp1, R &dp2)
+{
+    if constexpr (std::is_same<R, float>::value) {
+        fine_dp(I.fp, J.fp, (float)(P.box * (1.0 / 1024.0)), dp0, dp1, dp2);
+    } else {
+        double d0 = xi - X[j], d1 = yi - Y[j], d2 = zi - Z[j];
+        d1 = wrap(d1, P.box, P.half);
+        d0 = wrap(d0, P.box, P.half);
+        d2 = wrap(d2, P.box, P.half);
+        dp0 = (R)d0; dp1 = (R)d1; dp2 = (R)d2;
+    }
+}
 
 // ---------------------------------------------------------------------------------------
 // force: the pair loop of hydro_force() (src/hydro/hydro_force_loop.cc:423-480), two-sided
@@ -1044,11 +1063,8 @@ __global__ void __launch_bounds__(256) k_force(DevParams P, Gam<R> G, int nrows,
             const int j = ell[(size_t)q * ld + r];
             if (m < I.cs) m = I.cs;
             const Cell<R> J = rec[j];
-            double d0 = xi - F.X[j], d1 = yi - F.Y[j], d2 = zi - F.Z[j];
-            d1 = wrap(d1, P.box, P.half);
-            d0 = wrap(d0, P.box, P.half);
-            d2 = wrap(d2, P.box, P.half);
-            const R dp0 = (R)d0, dp1 = (R)d1, dp2 = (R)d2;
+            R dp0, dp1, dp2;
+            record_dp<R>(P, I, J, xi, yi, zi, F.X, F.Y, F.Z, j, dp0, dp1, dp2);
             const R r2 = dp0 * dp0 + dp1 * dp1 + dp2 * dp2;
             R f0, f1, f2, fp, vts;
             int rc = force_pair<KERN, R>(P, G, I, J, dp0, dp1, dp2, r2, f0, f1, f2, fp, vts);
@@ -1058,7 +1074,7 @@ __global__ void __launch_bounds__(256) k_force(DevParams P, Gam<R> G, int nrows,
                 continue;
             }
             if (rc == 0) continue;
-            a0.add(f0); a1.add(f1); a2.add(f2); ae.add(fp);
+            a0.add(rounded(f0)); a1.add(rounded(f1)); a2.add(rounded(f2)); ae.add(rounded(fp));
             if (vts > m) m = vts;
             if (active && !active[j] && wakeup) {
                 if ((double)vts > kWakeup * msv_in[j]) {

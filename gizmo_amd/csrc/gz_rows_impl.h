@@ -199,6 +199,15 @@ __global__ void __launch_bounds__(TPB) k_density_rows(DevParams P, GridView g, i
     } else {
         row_stencil<false>(g, key[i] >> g.shift, xi, yi, zi, h, rs);
     }
+    // FP32 form on one grid of at least 8 cells a side: differences of positions relative to the
+    // home cell's origin, the cell walk's arithmetic (the two walks give the same bits)
+    const bool rel = std::is_same<R, float>::value && !g.wide && (1 << g.level) >= 8;
+    double ox = 0, oy = 0, oz = 0;
+    if (rel) {
+        const uint32_t home = key[i] >> g.shift;
+        ox = compact10(home >> 2) * g.cell; oy = compact10(home >> 1) * g.cell; oz = compact10(home) * g.cell;
+    }
+    const R rxi = (R)wrap(xi - ox, g.box, g.half), ryi = (R)wrap(yi - oy, g.box, g.half), rzi = (R)wrap(zi - oz, g.box, g.half);
     const int total = rs.pref[rs.nst];
     R acc = 0;  // thread q < 13 keeps sum q
     for (int base = 0; base < total; base += TPB) {
@@ -208,11 +217,18 @@ __global__ void __launch_bounds__(TPB) k_density_rows(DevParams P, GridView g, i
             const int j = row_candidate(rs, v);
             const double mj = M[j];
             if (mj > 0) {
-                double d0 = xi - X[j], d1 = yi - Y[j], d2 = zi - Z[j];
-                d1 = wrap(d1, P.box, P.half);
-                d0 = wrap(d0, P.box, P.half);
-                d2 = wrap(d2, P.box, P.half);
-                const R dp0 = (R)d0, dp1 = (R)d1, dp2 = (R)d2;
+                R dp0, dp1, dp2;
+                if (rel) {
+                    dp0 = rxi - (R)wrap(X[j] - ox, g.box, g.half);
+                    dp1 = ryi - (R)wrap(Y[j] - oy, g.box, g.half);
+                    dp2 = rzi - (R)wrap(Z[j] - oz, g.box, g.half);
+                } else {
+                    double d0 = xi - X[j], d1 = yi - Y[j], d2 = zi - Z[j];
+                    d1 = wrap(d1, P.box, P.half);
+                    d0 = wrap(d0, P.box, P.half);
+                    d2 = wrap(d2, P.box, P.half);
+                    dp0 = (R)d0; dp1 = (R)d1; dp2 = (R)d2;
+                }
                 const R r2 = dp0 * dp0 + dp1 * dp1 + dp2 * dp2;
                 if (r2 < h2) {
                     const R rr = sqrt(r2);
@@ -708,11 +724,8 @@ __global__ void __launch_bounds__(kRowThreads) k_gradients_rows(DevParams P, int
                 const int j = ell[(size_t)q * ld + r];
                 const GradRec<R> J = rec[j];
                 if ((J.m > 0) && (J.rho > 0) && (J.h > 0)) {
-                    double d0 = xi - X[j], d1 = yi - Y[j], d2 = zi - Z[j];
-                    d1 = wrap(d1, P.box, P.half);
-                    d0 = wrap(d0, P.box, P.half);
-                    d2 = wrap(d2, P.box, P.half);
-                    const R dp[3] = {(R)d0, (R)d1, (R)d2};
+                    R dp[3];
+                    record_dp<R>(P, I, J, xi, yi, zi, X, Y, Z, j, dp[0], dp[1], dp[2]);
                     const R r2 = dp[0] * dp[0] + dp[1] * dp[1] + dp[2] * dp[2];
                     if ((r2 > 0) && !((r2 >= h2i) && (r2 >= J.h * J.h))) {
                         const R rr = sqrt(r2);
@@ -747,7 +760,7 @@ __global__ void __launch_bounds__(kRowThreads) k_gradients_rows(DevParams P, int
                     const int k = w * 64 + __ffsll((long long)m) - 1;
                     m &= m - 1;
                     if (lane < 15) {
-                        if (sf[k] == 2) acc += sw[lane / 5][k] * sd[lane % 5][k];
+                        if (sf[k] == 2) acc += rounded(sw[lane / 5][k] * sd[lane % 5][k]);
                     } else if (lane < 20) {
                         const R d = sd[lane - 15][k];
                         if (d < acc) acc = d;
@@ -821,11 +834,8 @@ __global__ void __launch_bounds__(kRowThreads) k_force_rows(DevParams P, Gam<R> 
             if (q < c) {
                 const int j = ell[(size_t)q * ld + r];
                 const Cell<R> J = rec[j];
-                double d0 = xi - F.X[j], d1 = yi - F.Y[j], d2 = zi - F.Z[j];
-                d1 = wrap(d1, P.box, P.half);
-                d0 = wrap(d0, P.box, P.half);
-                d2 = wrap(d2, P.box, P.half);
-                const R dp0 = (R)d0, dp1 = (R)d1, dp2 = (R)d2;
+                R dp0, dp1, dp2;
+                record_dp<R>(P, I, J, xi, yi, zi, F.X, F.Y, F.Z, j, dp0, dp1, dp2);
                 const R r2 = dp0 * dp0 + dp1 * dp1 + dp2 * dp2;
                 R f0, f1, f2, fp, vts;
                 rc = force_pair<KERN, R>(P, G, I, J, dp0, dp1, dp2, r2, f0, f1, f2, fp, vts);

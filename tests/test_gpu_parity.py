@@ -223,3 +223,18 @@ def test_fp32_mixed_error(cases, name, path):
     assert errs["h"] <= 1e-5 and errs["rho"] <= 1e-5
     assert max(errs.values()) <= 1e-3
     g.close()
+
+
+@pytest.mark.parametrize("name", ["lattice32", "sedov24"])
+def test_fp32_mixed_same_bits_both_shapes(cases, name):
+    """The FP32-mixed form gives the same bits through both kernel shapes (lane per particle and
+    workgroup per particle): a batch's result may not depend on how many rows share its pass."""
+    c = cases[name]
+    outs = {}
+    for p in PATHS:
+        g = gpu_from(c["s0"], c["par"], precision=GZ_FP32_MIXED, path=p)
+        g.hydro_pass()
+        outs[p] = g.store()
+        g.close()
+    for k in DENSITY_FIELDS + GRAD_FIELDS + FORCE_FIELDS:
+        assert np.array_equal(outs["lanes"][k], outs["waves"][k]), k
