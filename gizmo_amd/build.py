@@ -71,7 +71,31 @@ def build(verbose=False, jobs=None):
     os.replace(tmp, LIB)
     for o in objs:
         os.remove(o)
+    build_io(verbose)
     return LIB
+
+
+# HDF5 ICs and snapshots (include/gizmo_io.h): host code against the HDF5 C library of the
+# image (/opt/conda), its own library so the hot path does not depend on HDF5
+IO_LIB = os.path.join(HERE, "libgizmo_io.so")
+HDF5_ROOT = os.environ.get("GZ_HDF5_ROOT", "/opt/conda")
+
+
+def build_io(verbose=False):
+    inc = os.path.join(HDF5_ROOT, "include")
+    lib = os.path.join(HDF5_ROOT, "lib")
+    if not os.path.exists(os.path.join(inc, "hdf5.h")):
+        if verbose:
+            print(f"no HDF5 under {HDF5_ROOT}: libgizmo_io.so not built", flush=True)
+        return None
+    tmp = IO_LIB + ".tmp"
+    cmd = ["g++", "-O2", "-std=c++17", "-shared", "-fPIC", "-Wall", os.path.join(CSRC, "gz_io.cpp"), "-I" + inc, "-L" + lib,
+           "-Wl,-rpath," + lib, "-lhdf5", "-o", tmp]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(tmp, IO_LIB)
+    return IO_LIB
 
 
 if __name__ == "__main__":

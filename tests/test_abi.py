@@ -9,9 +9,9 @@ from gizmo_amd import hydro
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
-def declared_symbols():
+def declared_symbols(headers=None):
     names = set()
-    for h in glob.glob(os.path.join(ROOT, "include", "*.h")):
+    for h in headers or glob.glob(os.path.join(ROOT, "include", "*.h")):
         src = open(h).read()
         src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
         for m in re.finditer(r"\b(gz_[a-z_0-9]+)\s*\(", src):
@@ -19,10 +19,13 @@ def declared_symbols():
     return names
 
 
+IO_HEADER = os.path.join(ROOT, "include", "gizmo_io.h")
+
+
 def test_library_exports_declared_symbols():
     assert os.path.exists(hydro.LIB_PATH), "build with python -m gizmo_amd.build"
     L = ctypes.CDLL(hydro.LIB_PATH)
-    names = declared_symbols()
+    names = declared_symbols([h for h in glob.glob(os.path.join(ROOT, "include", "*.h")) if h != IO_HEADER])
     assert {"gz_density", "gz_hydro_gradient_calc", "gz_hydro_force", "gz_load", "gz_store"} <= names
     missing = [n for n in sorted(names) if not hasattr(L, n)]
     assert not missing, missing
@@ -39,3 +42,21 @@ def test_binding_matches_header_structs():
     body = body[:body.index("} gz_params;")]
     params = re.findall(r"\b(?:double|int)\s+(\w+);", body)
     assert params == [f for f, _ in hydro.Params._fields_]
+
+
+def test_io_library_exports_declared_symbols():
+    from gizmo_amd import io
+    L = io.iolib()
+    names = declared_symbols([IO_HEADER])
+    assert {"gz_ic_header", "gz_ic_read", "gz_snapshot_write", "gz_io_last_error"} <= names
+    missing = [n for n in sorted(names) if not hasattr(L, n)]
+    assert not missing, missing
+
+
+def test_io_binding_matches_header_struct():
+    from gizmo_amd import io
+    src = open(IO_HEADER).read()
+    body = src[src.index("typedef struct {"):]
+    body = body[:body.index("} gz_snap_header;")]
+    fields = re.findall(r"\b(?:long long|double|int)\s+(\w+);", body)
+    assert fields == [f for f, _ in io.SnapHeader._fields_]
