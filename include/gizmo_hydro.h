@@ -180,16 +180,17 @@ int gz_timeline_get(gz_context *ctx, gz_timeline *out);
    particles are put back in space-filling-curve order once any of them has moved farther than
    this fraction of the search radius since they were last ordered (the search grid widens its
    cells by twice that distance meanwhile); "few_rows" (default 8192) -- passes over at most this
-   many active particles run one wavefront per particle (64 lanes share its neighbours) instead
-   of one lane per particle; both give the same sums; "wide_search" (default 0) -- 1: every
-   particle searches at the grid level of its own kernel radius (cell tables of every level),
-   for radii that span orders of magnitude, where one grid sized by the largest radius would
-   walk far too many candidates; same neighbour sets, same sums (the symmetric list of more than
-   few_rows rows is then built one wavefront per cell of the rows' own levels, the density
-   rounds one wavefront per row); "row_schedule" (default 2048; 0 or a power of two from 64 to 2048) -- the one-lane-per-particle force pass
-   takes the rows of each block of this many longest list first (a wavefront lasts as long as
-   its longest list; 0: plain order).  Neither changes a sum.  None of these options has a counterpart in the reference (tuning knobs of this
-   library). */
+   many active particles run one workgroup per particle (its lanes share the particle's
+   candidates or list entries) instead of one lane per particle; "wide_search" (default 0) --
+   1: every particle searches at the grid level of its own kernel radius (cell tables of every
+   level), for radii that span orders of magnitude, where one grid sized by the largest radius
+   would walk far too many candidates (the symmetric list of more than few_rows rows is then
+   built one wavefront per cell of the rows' own levels, the density rounds one workgroup per
+   row); "row_schedule" (default 2048; 0 or a power of two from 64 to 2048) -- the
+   one-lane-per-particle force pass takes the rows of each block of this many longest list
+   first (a wavefront lasts as long as its longest list; 0: plain order).  None of these
+   changes a neighbour set or a sum, and none has a counterpart in the reference (tuning knobs
+   of this library). */
 int gz_set_option(gz_context *ctx, const char *name, double value);
 
 /* Neighbour lists of the resident state, in the caller's indexing, for parity tests.
