@@ -63,8 +63,8 @@ struct gz_context {
     // an upper bound on every kernel radius (< 0: unknown, reduce over all particles)
     double hmax_bound = -1;
     double hmax_rows = -1;  // the largest clamped radius of the active rows, when already known
-    uint8_t *rec_need = nullptr;     // records a pass reads (its rows and their neighbours)
-    bool rec_need_valid = false;
+    // passes over few rows pack only the records of the rows and their list entries
+    bool pack_list = false;
     bool all_active = true;
     int *perm = nullptr;      // device order -> caller index
     uint32_t *key = nullptr;  // fine Morton key, device order
@@ -324,17 +324,15 @@ int alloc_state(gz_context *ctx, long cap)
     CHK(dalloc(ctx, &ctx->VKX, n)); CHK(dalloc(ctx, &ctx->VKY, n)); CHK(dalloc(ctx, &ctx->VKZ, n));
     CHK(dalloc(ctx, &ctx->UK, n)); CHK(dalloc(ctx, &ctx->SOFT, n)); CHK(dalloc(ctx, &ctx->TIB, n)); CHK(dalloc(ctx, &ctx->DTS, n));
     CHK(dalloc(ctx, &ctx->X0, n)); CHK(dalloc(ctx, &ctx->Y0, n)); CHK(dalloc(ctx, &ctx->Z0, n));
-    CHK(dalloc(ctx, &ctx->rec_need, n));
-    CHK(dalloc(ctx, &ctx->counters, 16));
-    CHK(dalloc(ctx, &ctx->u64c, 80));
-    if (!ctx->hcounters) {
-        hipError_t e = hipHostMalloc((void **)&ctx->hcounters, 16 * sizeof(int), hipHostMallocDefault);
-        if (e != hipSuccess) return fail(ctx, GZ_E_MEMORY, "hipHostMalloc failed");
-    }
+    // the 64-bit counters and maxima, then the 16 int counters, in one allocation: one copy
+    // reads a stage's results back
+    CHK(dalloc(ctx, &ctx->u64c, 80 + 8));
+    ctx->counters = (int *)(ctx->u64c + 80);
     if (!ctx->hu64c) {
-        hipError_t e = hipHostMalloc((void **)&ctx->hu64c, 80 * sizeof(unsigned long long), hipHostMallocDefault);
+        hipError_t e = hipHostMalloc((void **)&ctx->hu64c, (80 + 8) * sizeof(unsigned long long), hipHostMallocDefault);
         if (e != hipSuccess) return fail(ctx, GZ_E_MEMORY, "hipHostMalloc failed");
     }
+    ctx->hcounters = (int *)(ctx->hu64c + 80);
     ctx->cap = cap;
     return 0;
 }
@@ -360,19 +358,19 @@ void free_state(gz_context *c)
     c->wide_key_version = -1;
     dfree(c->ival); dfree(c->ival2); dfree(c->iflag); dfree(c->iscan); dfree(c->act); dfree(c->stage);
     dfree(c->it_h); dfree(c->it_hb); dfree(c->it_l); dfree(c->it_r); dfree(c->it_cn); dfree(c->it_ci); dfree(c->it_it);
-    dfree(c->todo); dfree(c->todo2); dfree(c->cnt_d); dfree(c->cnt_s); dfree(c->counters); dfree(c->ell_d); dfree(c->ell_s);
+    dfree(c->todo); dfree(c->todo2); dfree(c->cnt_d); dfree(c->cnt_s); dfree(c->ell_d); dfree(c->ell_s);
+    c->counters = nullptr;  // inside u64c
     dfree(c->cstart); dfree(c->cend); dfree(c->nsten); dfree(c->sten); dfree(c->chmax);
     dfree(c->ca0); dfree(c->ca1); dfree(c->cells); dfree(c->cflag); dfree(c->cscan); dfree(c->need);
     dfree(c->rec_grad); dfree(c->rec_force); dfree(c->dacc);
     dfree(c->VKX); dfree(c->VKY); dfree(c->VKZ); dfree(c->UK); dfree(c->SOFT); dfree(c->TIB); dfree(c->DTS);
-    dfree(c->X0); dfree(c->Y0); dfree(c->Z0); dfree(c->u64c); dfree(c->rec_need);
+    dfree(c->X0); dfree(c->Y0); dfree(c->Z0); dfree(c->u64c);
     if (c->hu64c) hipHostFree(c->hu64c);
     c->hu64c = nullptr;
     if (c->tmp) hipFree(c->tmp);
     c->tmp = nullptr;
     c->tmp_bytes = 0;
-    if (c->hcounters) hipHostFree(c->hcounters);
-    c->hcounters = nullptr;
+    c->hcounters = nullptr;  // inside hu64c
 }
 
 // upload one host field (width doubles per particle, caller order) into device order
@@ -523,8 +521,8 @@ CellRows cell_rows(gz_context *ctx)
 
 int read_counters(gz_context *ctx)
 {
-    HIPCHK(hipMemcpyAsync(ctx->hu64c + 64, ctx->u64c + 64, 8 * sizeof(unsigned long long), hipMemcpyDeviceToHost, ctx->stream));
-    HIPCHK(hipMemcpyAsync(ctx->hcounters, ctx->counters, 8 * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    // u64c[64..71] and the int counters behind them (u64c[80..87])
+    HIPCHK(hipMemcpyAsync(ctx->hu64c + 64, ctx->u64c + 64, 24 * sizeof(unsigned long long), hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     return 0;
 }
@@ -892,16 +890,8 @@ int ensure_sym(gz_context *ctx)
             ctx->sched_valid = false;
             if (!few && ctx->sched_span > 0) CHK(build_sched(ctx));
             // the records the gradient and force passes read: every record when many rows are
-            // active, else only those of the rows and their neighbours
-            ctx->rec_need_valid = false;
-            if ((long)ctx->nact * 4 < ctx->n) {
-                Timed t2(ctx, "ngb_build");
-                HIPCHK(hipMemsetAsync(ctx->rec_need, 0, ctx->n, ctx->stream));
-                k_mark_records<<<(unsigned)((ctx->nact + 3) / 4), kBlock, 0, ctx->stream>>>(ctx->nact, ctx->act, ctx->ell_s, ctx->ld_s,
-                                                                                            ctx->cnt_s, ctx->rec_need);
-                HIPCHK(hipGetLastError());
-                ctx->rec_need_valid = true;
-            }
+            // active, else only those of the rows and their neighbours (packed from the list)
+            ctx->pack_list = (long)ctx->nact * 4 < ctx->n;
             return 0;
         }
         ctx->cap_s = std::max(over + 16, ctx->cap_s * 3 / 2);
@@ -919,9 +909,13 @@ int gradients_stage(gz_context *ctx)
     CHK(ensure_sym(ctx));
     Timed t(ctx, "gradients");
     GradRec<R> *rec = (GradRec<R> *)ctx->rec_grad;
-    k_pack_grad<R><<<nblk(ctx->n), kBlock, 0, ctx->stream>>>(ctx->n, ctx->rec_need_valid ? ctx->rec_need : nullptr, ctx->H, ctx->M,
-                                                             ctx->RHO, ctx->PR, ctx->VX, ctx->VY, ctx->VZ, rec, ctx->X, ctx->Y, ctx->Z,
-                                                             ctx->par.box_size, BoxTable{});
+    if (ctx->pack_list)
+        k_pack_grad_list<R><<<xcd_grid(ctx->nact), 64, 0, ctx->stream>>>(ctx->nact, ctx->act, ctx->ell_s, ctx->ld_s, ctx->cnt_s, ctx->H,
+                                                                         ctx->M, ctx->RHO, ctx->PR, ctx->VX, ctx->VY, ctx->VZ, rec, ctx->X,
+                                                                         ctx->Y, ctx->Z, ctx->par.box_size, BoxTable{});
+    else
+        k_pack_grad<R><<<nblk(ctx->n), kBlock, 0, ctx->stream>>>(ctx->n, ctx->H, ctx->M, ctx->RHO, ctx->PR, ctx->VX, ctx->VY, ctx->VZ, rec,
+                                                                 ctx->X, ctx->Y, ctx->Z, ctx->par.box_size, BoxTable{});
     if (ctx->nact <= ctx->few_rows)
         k_gradients_rows<KERN, R><<<xcd_grid(ctx->nact), kRowThreads, 0, ctx->stream>>>(ctx->dp, ctx->nact, ctx->act, ctx->X, ctx->Y, ctx->Z,
                                                                                rec, ctx->T, ctx->CN, ctx->n, ctx->ell_s, ctx->ld_s,
@@ -946,8 +940,13 @@ int force_stage(gz_context *ctx)
     {
         Timed t(ctx, "force");
         Cell<R> *rec = (Cell<R> *)ctx->rec_force;
-        k_pack_force<R><<<nblk(ctx->n), kBlock, 0, ctx->stream>>>(ctx->par.gamma, F, ctx->rec_need_valid ? ctx->rec_need : nullptr, rec,
-                                                                  ctx->counters + 2, ctx->par.box_size, BoxTable{});
+        if (ctx->pack_list)
+            k_pack_force_list<R><<<xcd_grid(ctx->nact), 64, 0, ctx->stream>>>(ctx->nact, ctx->act, ctx->ell_s, ctx->ld_s, ctx->cnt_s,
+                                                                              ctx->par.gamma, F, rec, ctx->counters + 2, ctx->par.box_size,
+                                                                              BoxTable{});
+        else
+            k_pack_force<R><<<nblk(ctx->n), kBlock, 0, ctx->stream>>>(ctx->par.gamma, F, rec, ctx->counters + 2, ctx->par.box_size,
+                                                                      BoxTable{});
         if (ctx->nact <= ctx->few_rows)
             k_force_rows<KERN, R><<<xcd_grid(ctx->nact), kRowThreads, 0, ctx->stream>>>(
                 ctx->dp, gam<R>(ctx->dp), ctx->nact, ctx->act, F, rec, ctx->ell_s, ctx->ld_s, ctx->cnt_s,
@@ -1192,8 +1191,7 @@ int run_step(gz_context *ctx)
         HIPCHK(hipGetLastError());
     }
     ctx->need_wakeup = 0;
-    HIPCHK(hipMemcpyAsync(ctx->hu64c, ctx->u64c, 72 * sizeof(unsigned long long), hipMemcpyDeviceToHost, ctx->stream));
-    HIPCHK(hipMemcpyAsync(ctx->hcounters, ctx->counters, 16 * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipMemcpyAsync(ctx->hu64c, ctx->u64c, (80 + 8) * sizeof(unsigned long long), hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     for (int b = 0; b < kTimebins; b++) ctx->bin_count[b] = (long long)ctx->hu64c[b];
     ctx->n_woken = ctx->hcounters[5];

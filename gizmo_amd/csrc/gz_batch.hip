@@ -71,7 +71,7 @@ struct gz_batch {
     double *MSV = nullptr, *SOFT = nullptr, *X0 = nullptr, *Y0 = nullptr, *Z0 = nullptr;
     long long *TIB = nullptr, *DTS = nullptr;
     int *TB = nullptr, *WAKE = nullptr, *BOX = nullptr, *perm = nullptr;
-    uint8_t *ACT = nullptr, *need = nullptr, *rec_need = nullptr, *sel = nullptr, *due = nullptr;
+    uint8_t *ACT = nullptr, *need = nullptr, *sel = nullptr, *due = nullptr;
     uint32_t *key = nullptr;
     unsigned long long *key64 = nullptr, *key64b = nullptr;
     int *ival = nullptr, *ival2 = nullptr, *iflag = nullptr, *iscan = nullptr, *act = nullptr, *rank = nullptr;
@@ -227,7 +227,7 @@ int alloc_all(gz_batch *b)
     int **i1[] = {&b->TB, &b->WAKE, &b->BOX, &b->perm, &b->ival, &b->ival2, &b->act, &b->it_ci, &b->it_it, &b->cnt};
     for (auto p : i1) BCHK(balloc(b, p, n));
     BCHK(balloc(b, &b->iflag, n + 1)); BCHK(balloc(b, &b->iscan, n + 1));
-    BCHK(balloc(b, &b->ACT, n)); BCHK(balloc(b, &b->need, n)); BCHK(balloc(b, &b->rec_need, n));
+    BCHK(balloc(b, &b->ACT, n)); BCHK(balloc(b, &b->need, n));
     BCHK(balloc(b, &b->key, n)); BCHK(balloc(b, &b->key64, n)); BCHK(balloc(b, &b->key64b, n));
     BCHK(balloc(b, (char **)&b->rec_grad, n * sizeof(GradRec<double>)));
     BCHK(balloc(b, (char **)&b->rec_force, n * sizeof(Cell<double>)));
@@ -252,7 +252,7 @@ void free_all(gz_batch *c)
     int **i1[] = {&c->TB, &c->WAKE, &c->BOX, &c->perm, &c->ival, &c->ival2, &c->act, &c->it_ci, &c->it_it, &c->cnt, &c->iflag,
                   &c->iscan, &c->rank, &c->lim, &c->counters, &c->ell, &c->cstart, &c->cend};
     for (auto p : i1) bfree(*p);
-    bfree(c->TIB); bfree(c->DTS); bfree(c->ACT); bfree(c->need); bfree(c->rec_need); bfree(c->sel); bfree(c->due);
+    bfree(c->TIB); bfree(c->DTS); bfree(c->ACT); bfree(c->need); bfree(c->sel); bfree(c->due);
     bfree(c->key); bfree(c->key64); bfree(c->key64b); bfree(c->rec_grad); bfree(c->rec_force); bfree(c->tab);
     bfree(c->hist); bfree(c->slot); bfree(c->u64);
     bfree(c->tmp);
@@ -582,9 +582,6 @@ int sym_list(gz_batch *b, const std::vector<uint8_t> &in_pass)
         BCHK(balloc(b, &b->ell, (size_t)b->ell_cap * ld_max));
         b->stats["ell_widen"] += 1;
     }
-    BHIP(hipMemsetAsync(b->rec_need, 0, b->cap, b->stream));
-    k_mark_records<<<(unsigned)((na + 3) / 4), kB, 0, b->stream>>>(na, b->act, b->ell, ld, b->cnt, b->rec_need);
-    BHIP(hipGetLastError());
     return 0;
 }
 
@@ -596,8 +593,14 @@ int gradients_force(gz_batch *b)
     const long ld = ld_rows(na);
     const BoxTable bt = table(b);
     GradRec<R> *rg = (GradRec<R> *)b->rec_grad;
-    k_pack_grad<R><<<nb(b->cap), kB, 0, b->stream>>>(b->cap, b->rec_need, b->H, b->M, b->RHO, b->PR, b->VX, b->VY, b->VZ, rg, b->X, b->Y,
-                                                      b->Z, 1.0, bt);
+    // few rows among many particles: only the records of the rows and their list entries
+    const bool list = (long)na * 4 < b->cap;
+    if (list)
+        k_pack_grad_list<R><<<xcd_grid(na), 64, 0, b->stream>>>(na, b->act, b->ell, ld, b->cnt, b->H, b->M, b->RHO, b->PR, b->VX, b->VY,
+                                                                b->VZ, rg, b->X, b->Y, b->Z, 1.0, bt);
+    else
+        k_pack_grad<R><<<nb(b->cap), kB, 0, b->stream>>>(b->cap, b->H, b->M, b->RHO, b->PR, b->VX, b->VY, b->VZ, rg, b->X, b->Y, b->Z, 1.0,
+                                                          bt);
     const bool lanes = na > b->few_rows;  // many rows: one lane per row
     if (lanes)
         k_gradients<KERN, R, true><<<xcd_grid(nb(na)), kB, 0, b->stream>>>(b->dp, na, b->act, b->X, b->Y, b->Z, rg, b->T, b->CN, b->cap, b->ell, ld,
@@ -612,7 +615,11 @@ int gradients_force(gz_batch *b)
     Gam<R> G;
     G.G0 = (R)b->dp.G0; G.G1 = (R)b->dp.G1; G.G2 = (R)b->dp.G2; G.G3 = (R)b->dp.G3; G.G4 = (R)b->dp.G4;
     G.G5 = (R)b->dp.G5; G.G6 = (R)b->dp.G6; G.G7 = (R)b->dp.G7; G.G8 = (R)b->dp.G8; G.G9 = (R)b->dp.G9;
-    k_pack_force<R><<<nb(b->cap), kB, 0, b->stream>>>(b->par.gamma, F, b->rec_need, rf, b->counters + 2, 1.0, bt);
+    if (list)
+        k_pack_force_list<R><<<xcd_grid(na), 64, 0, b->stream>>>(na, b->act, b->ell, ld, b->cnt, b->par.gamma, F, rf, b->counters + 2, 1.0,
+                                                                 bt);
+    else
+        k_pack_force<R><<<nb(b->cap), kB, 0, b->stream>>>(b->par.gamma, F, rf, b->counters + 2, 1.0, bt);
     if (lanes) {
         k_force<KERN, R, true><<<xcd_grid(nb(na)), kB, 0, b->stream>>>(b->dp, G, na, b->act, F, rf, b->ell, ld, b->cnt, b->ACT, b->TB, b->MSV,
                                                                   b->WAKE, b->ACC, b->DTU, b->MSV, b->counters + 2, b->counters + 4, b->u64,

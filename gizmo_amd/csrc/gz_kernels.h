@@ -167,7 +167,6 @@ __global__ void k_active_compact(long n, int nblocks, const unsigned long long *
 __global__ void k_advance_boxes(BoxTable bt);
 __global__ void k_active_from_bins(long n, unsigned long long mask, const int *TB, const double *M, const double *H,
                                    uint8_t *ACT, double *SOFT);
-__global__ void k_mark_records(int nrows, const int *act, const int *ell, long ld, const int *cnt, uint8_t *need);
 __global__ void k_sum_counts(int nrows, const int *cnt, unsigned long long *out);
 // groups of rows sharing one cell of one grid (the search over many levels, batches of boxes)
 __global__ void k_row_group_key(int nrows, const int *act, const uint8_t *need, const uint32_t *key, const double *hslot,
@@ -244,11 +243,18 @@ __global__ void k_density_after(DevParams P, int nrows, const int *act, uint8_t 
                                 unsigned long long *hfinal, BoxTable bt);
 
 template <typename R>
-__global__ void k_pack_grad(long n, const uint8_t *need, const double *H, const double *M, const double *RHO, const double *PR,
-                            const double *VX, const double *VY, const double *VZ, GradRec<R> *rec, const double *X, const double *Y,
-                            const double *Z, double box, BoxTable bt);
+__global__ void k_pack_grad(long n, const double *H, const double *M, const double *RHO, const double *PR, const double *VX,
+                            const double *VY, const double *VZ, GradRec<R> *rec, const double *X, const double *Y, const double *Z,
+                            double box, BoxTable bt);
 template <typename R>
-__global__ void k_pack_force(double gamma, ForceView F, const uint8_t *need, Cell<R> *rec, int *zero2, double box, BoxTable bt);
+__global__ void k_pack_force(double gamma, ForceView F, Cell<R> *rec, int *zero2, double box, BoxTable bt);
+template <typename R>
+__global__ void k_pack_grad_list(int nrows, const int *act, const int *ell, long ld, const int *cnt, const double *H, const double *M,
+                                 const double *RHO, const double *PR, const double *VX, const double *VY, const double *VZ,
+                                 GradRec<R> *rec, const double *X, const double *Y, const double *Z, double box, BoxTable bt);
+template <typename R>
+__global__ void k_pack_force_list(int nrows, const int *act, const int *ell, long ld, const int *cnt, double gamma, ForceView F,
+                                  Cell<R> *rec, int *zero2, double box, BoxTable bt);
 
 // BOXES: each row's box geometry from bt (batches); else the single box of the arguments
 template <int KERN, typename R, bool BOXES>

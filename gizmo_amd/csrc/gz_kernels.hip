@@ -53,10 +53,15 @@ template __global__ void k_build_ell<true>(int, const int *, const int *, const 
                                                    unsigned long long *, BoxTable);
 GZ_INST(3, double)
 GZ_INST(6, double)
-template __global__ void k_pack_grad<double>(long, const uint8_t *, const double *, const double *, const double *, const double *, const double *,
-                                             const double *, const double *, GradRec<double> *, const double *,
-                                             const double *, const double *, double, BoxTable);
-template __global__ void k_pack_force<double>(double, ForceView, const uint8_t *, Cell<double> *, int *, double, BoxTable);
+template __global__ void k_pack_grad<double>(long, const double *, const double *, const double *, const double *, const double *,
+                                             const double *, const double *, GradRec<double> *, const double *, const double *,
+                                             const double *, double, BoxTable);
+template __global__ void k_pack_force<double>(double, ForceView, Cell<double> *, int *, double, BoxTable);
+template __global__ void k_pack_grad_list<double>(int, const int *, const int *, long, const int *, const double *, const double *,
+                                               const double *, const double *, const double *, const double *, const double *,
+                                               GradRec<double> *, const double *, const double *, const double *, double, BoxTable);
+template __global__ void k_pack_force_list<double>(int, const int *, const int *, long, const int *, double, ForceView, Cell<double> *,
+                                                int *, double, BoxTable);
 #undef GZ_INST
 
 }  // namespace gz

@@ -48,10 +48,15 @@ namespace gz {
                                                    unsigned long long *, BoxTable);
 GZ_INST(3, float)
 GZ_INST(6, float)
-template __global__ void k_pack_grad<float>(long, const uint8_t *, const double *, const double *, const double *, const double *, const double *,
-                                            const double *, const double *, GradRec<float> *, const double *,
-                                             const double *, const double *, double, BoxTable);
-template __global__ void k_pack_force<float>(double, ForceView, const uint8_t *, Cell<float> *, int *, double, BoxTable);
+template __global__ void k_pack_grad<float>(long, const double *, const double *, const double *, const double *, const double *,
+                                             const double *, const double *, GradRec<float> *, const double *, const double *,
+                                             const double *, double, BoxTable);
+template __global__ void k_pack_force<float>(double, ForceView, Cell<float> *, int *, double, BoxTable);
+template __global__ void k_pack_grad_list<float>(int, const int *, const int *, long, const int *, const double *, const double *,
+                                               const double *, const double *, const double *, const double *, const double *,
+                                               GradRec<float> *, const double *, const double *, const double *, double, BoxTable);
+template __global__ void k_pack_force_list<float>(int, const int *, const int *, long, const int *, double, ForceView, Cell<float> *,
+                                                int *, double, BoxTable);
 #undef GZ_INST
 
 }  // namespace gz

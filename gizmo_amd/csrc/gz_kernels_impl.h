@@ -320,7 +320,7 @@ __global__ void __launch_bounds__(256) k_build_ell(int nrows, const int *__restr
             }
         }
     }
-    cnt[r] = c;
+    cnt[r] = min(c, cap);  // readers stay inside the list; the overflow carries the full length
     if (c > cap) atomicMax(overflow, c);
 }
 
@@ -786,34 +786,19 @@ __global__ void __launch_bounds__(256) k_density_after(DevParams P, int nrows, c
 #endif
 
 
-// the records a pass over few rows reads: the rows' own and their list neighbours'
-#ifdef GZ_DEFINE_COMMON_KERNELS
-__global__ void k_mark_records(int nrows, const int *__restrict__ act, const int *__restrict__ ell, long ld,
-                               const int *__restrict__ cnt, uint8_t *__restrict__ need)
-{
-    // one wavefront per row, its lanes over the row's entries
-    const long r = blockIdx.x * (long)(blockDim.x >> 6) + (threadIdx.x >> 6);
-    const int lane = threadIdx.x & 63;
-    if (r >= nrows) return;
-    if (lane == 0) need[act[r]] = 1;
-    const int c = cnt[r];
-    for (int q = lane; q < c; q += 64) need[ell[(size_t)q * ld + r]] = 1;
-}
-#endif
 
 // ---------------------------------------------------------------------------------------
 // records: the neighbour-side fields of a pass, packed per particle (device order)
 // ---------------------------------------------------------------------------------------
+// the gradient record of particle j
 template <typename R>
-__global__ void k_pack_grad(long n, const uint8_t *__restrict__ need, const double *__restrict__ H, const double *__restrict__ M,
-                            const double *__restrict__ RHO, const double *__restrict__ PR, const double *__restrict__ VX,
-                            const double *__restrict__ VY, const double *__restrict__ VZ, GradRec<R> *__restrict__ rec,
-                            const double *__restrict__ X, const double *__restrict__ Y, const double *__restrict__ Z, double box,
-                            BoxTable bt)
+__device__ __forceinline__ void pack_grad_one(long j, const double *__restrict__ H, const double *__restrict__ M,
+                                              const double *__restrict__ RHO, const double *__restrict__ PR,
+                                              const double *__restrict__ VX, const double *__restrict__ VY,
+                                              const double *__restrict__ VZ, GradRec<R> *__restrict__ rec,
+                                              const double *__restrict__ X, const double *__restrict__ Y,
+                                              const double *__restrict__ Z, double box, const BoxTable &bt)
 {
-    const long j = blockIdx.x * (long)blockDim.x + threadIdx.x;
-    if (j >= n) return;
-    if (need && !need[j]) return;
     GradRec<R> g;
     g.h = (R)H[j]; g.m = (R)M[j]; g.rho = RHO[j]; g.p = PR[j];
     g.v0 = VX[j]; g.v1 = VY[j]; g.v2 = VZ[j];
@@ -821,16 +806,25 @@ __global__ void k_pack_grad(long n, const uint8_t *__restrict__ need, const doub
     rec[j] = g;
 }
 
+// every record (passes over many rows)
 template <typename R>
-__global__ void k_pack_force(double gamma, ForceView F, const uint8_t *__restrict__ need, Cell<R> *__restrict__ rec,
-                             int *__restrict__ zero2, double box, BoxTable bt)
+__global__ void k_pack_grad(long n, const double *__restrict__ H, const double *__restrict__ M,
+                            const double *__restrict__ RHO, const double *__restrict__ PR, const double *__restrict__ VX,
+                            const double *__restrict__ VY, const double *__restrict__ VZ, GradRec<R> *__restrict__ rec,
+                            const double *__restrict__ X, const double *__restrict__ Y, const double *__restrict__ Z, double box,
+                            BoxTable bt)
 {
     const long j = blockIdx.x * (long)blockDim.x + threadIdx.x;
-    const long n = F.n;
-    // the force pass's error and wake-up flags, and the first pair without a Riemann solution
-    if (j == 0 && zero2) { zero2[0] = 0; zero2[2] = 0; zero2[10] = -1; zero2[11] = -1; }
     if (j >= n) return;
-    if (need && !need[j]) return;
+    pack_grad_one<R>(j, H, M, RHO, PR, VX, VY, VZ, rec, X, Y, Z, box, bt);
+}
+
+// the force record of particle j (effective_soundspeed, src/declarations/cell_data.h:703-712)
+template <typename R>
+__device__ __forceinline__ void pack_force_one(long j, double gamma, const ForceView &F, Cell<R> *__restrict__ rec, double box,
+                                               const BoxTable &bt)
+{
+    const long n = F.n;
     Cell<R> c;
     c.h = (R)F.H[j]; c.m = (R)F.M[j]; c.rho = (R)F.RHO[j]; c.p = (R)F.PR[j];
     c.cn = (R)F.CN[j]; c.fce = (R)F.FCE[j]; c.drk = (R)F.DRK[j];
@@ -839,11 +833,61 @@ __global__ void k_pack_force(double gamma, ForceView F, const uint8_t *__restric
     c.gp0 = (R)F.GP[j]; c.gp1 = (R)F.GP[n + j]; c.gp2 = (R)F.GP[2 * n + j];
     for (int k = 0; k < 9; k++) c.gv[k] = (R)F.GV[(size_t)k * n + j];
     for (int k = 0; k < 6; k++) c.t[k] = (R)F.T[(size_t)k * n + j];
-    // effective_soundspeed, src/declarations/cell_data.h:703-712
     const double rho = F.RHO[j];
     c.cs = (R)((rho > 0) ? sqrt(gamma * F.PR[j] / rho) : sqrt(gamma * (gamma - 1.) * F.U[j]));
     if constexpr (std::is_same<R, float>::value) c.fp = fine_pos(F.X[j], F.Y[j], F.Z[j], bt.b ? bt.b[bt.BOX[j]].box : box);
     rec[j] = c;
+}
+
+// the force pass's error and wake-up flags, and the first pair without a Riemann solution
+__device__ __forceinline__ void force_flags_clear(int *zero2)
+{
+    zero2[0] = 0; zero2[2] = 0; zero2[10] = -1; zero2[11] = -1;
+}
+
+template <typename R>
+__global__ void k_pack_force(double gamma, ForceView F, Cell<R> *__restrict__ rec, int *__restrict__ zero2, double box, BoxTable bt)
+{
+    const long j = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (j == 0 && zero2) force_flags_clear(zero2);
+    if (j >= F.n) return;
+    pack_force_one<R>(j, gamma, F, rec, box, bt);
+}
+
+// Few rows: the records of the rows and of their list entries only, one wavefront per row
+// (a record shared by several rows is written several times with the same bytes)
+template <typename R>
+__global__ void __launch_bounds__(64) k_pack_grad_list(int nrows, const int *__restrict__ act, const int *__restrict__ ell, long ld,
+                                                       const int *__restrict__ cnt, const double *__restrict__ H,
+                                                       const double *__restrict__ M, const double *__restrict__ RHO,
+                                                       const double *__restrict__ PR, const double *__restrict__ VX,
+                                                       const double *__restrict__ VY, const double *__restrict__ VZ,
+                                                       GradRec<R> *__restrict__ rec, const double *__restrict__ X,
+                                                       const double *__restrict__ Y, const double *__restrict__ Z, double box,
+                                                       BoxTable bt)
+{
+    const long r = xcd_block();
+    if (r >= nrows) return;
+    const int c = cnt[r];
+    for (int q = threadIdx.x; q <= c; q += 64) {
+        const int j = q == 0 ? act[r] : ell[(size_t)(q - 1) * ld + r];
+        pack_grad_one<R>(j, H, M, RHO, PR, VX, VY, VZ, rec, X, Y, Z, box, bt);
+    }
+}
+
+template <typename R>
+__global__ void __launch_bounds__(64) k_pack_force_list(int nrows, const int *__restrict__ act, const int *__restrict__ ell, long ld,
+                                                        const int *__restrict__ cnt, double gamma, ForceView F,
+                                                        Cell<R> *__restrict__ rec, int *__restrict__ zero2, double box, BoxTable bt)
+{
+    const long r = xcd_block();
+    if (r == 0 && threadIdx.x == 0 && zero2) force_flags_clear(zero2);
+    if (r >= nrows) return;
+    const int c = cnt[r];
+    for (int q = threadIdx.x; q <= c; q += 64) {
+        const int j = q == 0 ? act[r] : ell[(size_t)(q - 1) * ld + r];
+        pack_force_one<R>(j, gamma, F, rec, box, bt);
+    }
 }
 
 // ---------------------------------------------------------------------------------------

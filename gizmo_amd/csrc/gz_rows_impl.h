@@ -326,7 +326,7 @@ __global__ void __launch_bounds__(kRowThreads) k_build_sym_rows(GridView g, int 
         n += all;
     }
     if (tid == 0) {
-        cnt[r] = n;
+        cnt[r] = min(n, cap);  // readers stay inside the list; the overflow carries the full length
         if (n > cap) atomicMax(overflow, n);
     }
 }
@@ -473,7 +473,7 @@ __global__ void __launch_bounds__(64) k_build_sym_wide(GridView g, int nrows, co
         }
     }
     if (lane == 0) {
-        cnt[r] = n;
+        cnt[r] = min(n, cap);  // readers stay inside the list; the overflow carries the full length
         if (n > cap) atomicMax(overflow, n);
     }
 }
@@ -596,7 +596,7 @@ __global__ void __launch_bounds__(64) k_build_sym_cells(GridView g, CellRows cr,
             }
         }
         if (on) {
-            cnt[r] = n;
+            cnt[r] = min(n, cap);  // readers stay inside the list; the overflow carries the full length
             if (n > cap) atomicMax(overflow, n);
         }
     }
