@@ -92,11 +92,17 @@ __device__ __forceinline__ void row_stencil(const GridView &g, uint32_t home, do
     stencil_finish(rs, code, st, en);
 }
 
+// (v < the total; the range k with pref[k] <= v < pref[k+1], the last such k when empty ranges
+// share a prefix value: binary search over the at most 27 prefixes)
 __device__ __forceinline__ int row_candidate(const RowStencil &rs, int v)
 {
-    int k = 0;
-    while (k + 1 < rs.nst && rs.pref[k + 1] <= v) k++;
-    return rs.start[k] + (v - rs.pref[k]);
+    int lo = 0, hi = rs.nst - 1;  // invariant: pref[lo] <= v, and the answer is at most hi
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (rs.pref[mid] <= v) lo = mid;
+        else hi = mid - 1;
+    }
+    return rs.start[lo] + (v - rs.pref[lo]);
 }
 
 // ---------------------------------------------------------------------------------------

@@ -85,11 +85,17 @@ def test_full_pass_same_bits_as_one_grid(wide_case):
 
 
 def test_full_pass_against_oracle(wide_case):
+    """Also the path of a symmetric list that overflows its first width: the lognormal field's
+    longest lists exceed the initial width, the gradient and force passes run on the cut lists
+    (their lengths stored clamped to the width), the overflow read back with the force pass
+    calls for a wider list and both passes run again."""
     ic, par, s3 = wide_case["ic"], wide_case["par"], wide_case["s3"]
     g = device(ic, par, 1)
     g.hydro_pass()
+    widened = g.stat("ell_widen")
     out = g.store(FIELDS)
     g.close()
+    assert widened >= 1
     for k in FIELDS:
         a = np.asarray(out[k], dtype=np.float64).reshape(s3.n, -1)
         b = np.asarray(getattr(s3, k), dtype=np.float64).reshape(s3.n, -1)
