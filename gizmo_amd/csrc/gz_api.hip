@@ -890,8 +890,10 @@ int ensure_sym(gz_context *ctx)
             ctx->sched_valid = false;
             if (!few && ctx->sched_span > 0) CHK(build_sched(ctx));
             // the records the gradient and force passes read: every record when many rows are
-            // active, else only those of the rows and their neighbours (packed from the list)
-            ctx->pack_list = (long)ctx->nact * 4 < ctx->n;
+            // active, else only those of the rows and their neighbours, packed from the list (a
+            // record once per list entry that names it: cheaper while the lists hold fewer
+            // entries than there are particles)
+            ctx->pack_list = (long)ctx->nact * 64 < ctx->n;
             return 0;
         }
         ctx->cap_s = std::max(over + 16, ctx->cap_s * 3 / 2);

@@ -593,8 +593,10 @@ int gradients_force(gz_batch *b)
     const long ld = ld_rows(na);
     const BoxTable bt = table(b);
     GradRec<R> *rg = (GradRec<R> *)b->rec_grad;
-    // few rows among many particles: only the records of the rows and their list entries
-    const bool list = (long)na * 4 < b->cap;
+    // few rows among many particles: only the records of the rows and their list entries (a
+    // record once per list entry that names it: cheaper while the lists hold fewer entries than
+    // there are particles)
+    const bool list = (long)na * 64 < b->cap;
     if (list)
         k_pack_grad_list<R><<<xcd_grid(na), 64, 0, b->stream>>>(na, b->act, b->ell, ld, b->cnt, b->H, b->M, b->RHO, b->PR, b->VX, b->VY,
                                                                 b->VZ, rg, b->X, b->Y, b->Z, 1.0, bt);
