@@ -958,9 +958,8 @@ int force_stage(gz_context *ctx)
             k_force<KERN, R, false><<<xcd_grid(nblk(ctx->nact)), kBlock, 0, ctx->stream>>>(
                 ctx->dp, gam<R>(ctx->dp), ctx->nact, ctx->act, F, rec, ctx->ell_s, ctx->ld_s, ctx->cnt_s,
                 ctx->all_active ? nullptr : ctx->ACT, ctx->TB, ctx->MSV, ctx->WAKE, ctx->ACC, ctx->DTU, ctx->MSV, ctx->counters + 2,
-                ctx->counters + 4, ctx->u64c + 65, ctx->sched_valid ? ctx->sched : nullptr, BoxTable{});
-        if (ctx->nact <= ctx->few_rows)
-            k_sum_counts<<<std::min(nblk(ctx->nact), 1024u), kBlock, 0, ctx->stream>>>(ctx->nact, ctx->cnt_s, ctx->u64c + 65);
+                ctx->counters + 4, nullptr, ctx->sched_valid ? ctx->sched : nullptr, BoxTable{});
+        k_sum_counts<<<std::min(nblk(ctx->nact), 256u), kBlock, 0, ctx->stream>>>(ctx->nact, ctx->cnt_s, ctx->u64c + 65);
         HIPCHK(hipGetLastError());
     }
     CHK(read_counters(ctx));

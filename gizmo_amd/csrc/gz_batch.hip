@@ -624,14 +624,14 @@ int gradients_force(gz_batch *b)
         k_pack_force<R><<<nb(b->cap), kB, 0, b->stream>>>(b->par.gamma, F, rf, b->counters + 2, 1.0, bt);
     if (lanes) {
         k_force<KERN, R, true><<<xcd_grid(nb(na)), kB, 0, b->stream>>>(b->dp, G, na, b->act, F, rf, b->ell, ld, b->cnt, b->ACT, b->TB, b->MSV,
-                                                                  b->WAKE, b->ACC, b->DTU, b->MSV, b->counters + 2, b->counters + 4, b->u64,
+                                                                  b->WAKE, b->ACC, b->DTU, b->MSV, b->counters + 2, b->counters + 4, nullptr,
                                                                   nullptr, bt);
     } else {
         k_force_rows<KERN, R><<<xcd_grid(na), kRowThreads, 0, b->stream>>>(b->dp, G, na, b->act, F, rf, b->ell, ld, b->cnt, b->ACT, b->TB, b->MSV,
                                                                    b->WAKE, b->ACC, b->DTU, b->MSV, b->counters + 2, b->counters + 4,
                                                                    nullptr, bt);
-        k_sum_counts<<<std::min(nb(na), 1024u), kB, 0, b->stream>>>(na, b->cnt, b->u64);
     }
+    k_sum_counts<<<std::min(nb(na), 256u), kB, 0, b->stream>>>(na, b->cnt, b->u64);
     BHIP(hipGetLastError());
     BCHK(read_back(b, false));
     if (b->hcounters[2]) return fail(b, GZ_STOP_RIEMANN, "hydro_force(): Riemann solver failed to find a positive pressure");

@@ -83,10 +83,9 @@ __global__ void __launch_bounds__(256) k_batch_box_hmax(long n, const double *__
     const int b0 = __shfl(box, __ffsll((long long)vm) - 1, 64);
     if (__all(!valid || box == b0)) {
         for (int o = 32; o > 0; o >>= 1) h = fmax(h, __shfl_xor(h, o, 64));
-        if ((threadIdx.x & 63) == 0 && h > 0)
-            atomicMax(&bt.slot[(size_t)b0 * kBoxSlots + kSlotHmax], (unsigned long long)__double_as_longlong(h));
+        if ((threadIdx.x & 63) == 0 && h > 0) atomic_max_bits(&bt.slot[(size_t)b0 * kBoxSlots + kSlotHmax], h);
     } else if (valid && h > 0) {
-        atomicMax(&bt.slot[(size_t)box * kBoxSlots + kSlotHmax], (unsigned long long)__double_as_longlong(h));
+        atomic_max_bits(&bt.slot[(size_t)box * kBoxSlots + kSlotHmax], h);
     }
 }
 

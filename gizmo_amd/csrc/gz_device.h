@@ -110,6 +110,16 @@ __device__ __forceinline__ bool accept_pair(double xi, double yi, double zi, dou
 template <typename T> __device__ __forceinline__ T dmax(T a, T b) { return (a > b) ? a : b; }
 template <typename T> __device__ __forceinline__ T dmin(T a, T b) { return (a < b) ? a : b; }
 
+// atomicMax of the bits of a positive double (they order like the double) that skips the
+// read-modify-write when the value held is not smaller.  Atomics on one address are executed
+// one after another at the memory side (~10 ns each), so a maximum over many waves costs only
+// the loads; the maxima only grow while a kernel runs, so a stale read costs only an atomic.
+__device__ __forceinline__ void atomic_max_bits(unsigned long long *p, double v)
+{
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    if (__hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < b) atomicMax(p, b);
+}
+
 // ---------------------------------------------------------------------------------------
 // 3x3 inverse with condition number: matrix_invert_ndims, src/core/proto.h:75-95 with
 // Mat3::invert, src/math_types/mat3.h:50-64.  Row-major.
@@ -203,8 +213,24 @@ struct IterState {
     int cn_init;
 };
 
+// a row's sums normalised at radius h (src/hydro/density_loop.h:516-620): what the radius
+// iteration and the finalisation both read, computed once per round
+struct DensityNorm {
+    double nn, dr, dv, vol, normalizer, cn, fce;
+    double Ti[9];
+};
+
+__device__ inline void density_norm(const double *acc, double h, DensityNorm &d)
+{
+    density_normalise(acc, h, d.nn, d.dr, d.dv);
+    d.vol = kVolNorm * pow(h, 3.0) / (d.nn > 0 ? d.nn : 1.0);
+    d.normalizer = pow(h, -1.0);
+    d.cn = nvt_inverse(&acc[4], d.normalizer, d.Ti);
+    d.fce = face_closure(&acc[4], &acc[10], d.Ti, d.normalizer, d.vol);
+}
+
 // returns 0 converged, 1 new radius in new_h, -1 empty bracket (stop code 90001009)
-__device__ inline int density_after_iter(const DevParams &P, double cn_stored, int iter, double h, const double *acc,
+__device__ inline int density_after_iter(const DevParams &P, double cn_stored, int iter, double h, const DensityNorm &dn,
                                          IterState &sc, double &new_h_out)
 {
     double desnumngb = P.des_num_ngb;
@@ -213,13 +239,7 @@ __device__ inline int density_after_iter(const DevParams &P, double cn_stored, i
         if (P.max_num_ngb_dev > 0.05) desnumngbdev = 0.05;
     }
     const double dev0 = desnumngbdev;
-    double NumNgb, Dr, Dv;
-    density_normalise(acc, h, NumNgb, Dr, Dv);
-    const double V_i = kVolNorm * pow(h, 3.0) / (NumNgb > 0 ? NumNgb : 1.0);
-    const double normalizer = pow(h, -1.0);
-    double Ti[9];
-    const double CN = nvt_inverse(&acc[4], normalizer, Ti);
-    const double fce = face_closure(&acc[4], &acc[10], Ti, normalizer, V_i);
+    const double NumNgb = dn.nn, Dr = dn.dr, CN = dn.cn, fce = dn.fce;
     {
         const double c0 = 0.1 * kCnDanger;
         if ((iter == 0) && (CN > cn_stored) && (cn_stored > 0)) {

@@ -219,12 +219,6 @@ __global__ void k_build_ell(int nrows, const int *rows, const int *act, const do
                             const double *Y, const double *Z, const double *H, const uint32_t *key, GridView g, int *ell,
                             long ld, int cap, int *cnt, int *overflow);
 
-template <int KERN, typename R>
-__global__ void k_density_iter(DevParams P, int ntodo, const int *todo, const int *act, const double *X, const double *Y,
-                               const double *Z, const double *VX, const double *VY, const double *VZ, const double *M,
-                               const double *U, const int *ell, long ld, const int *cnt, DensityIter it, DensityOut out,
-                               int *next, int *nnext, int *rebuild, int *err);
-
 __global__ void k_cell_rows_mark(int nact, const int *act, const uint32_t *key, int shift, int *ca0, int *ca1, int *flag);
 __global__ void k_cell_rows_list(int nact, const int *act, const uint32_t *key, int shift, const int *flag, const int *scan,
                                  int *cells);

@@ -28,10 +28,6 @@ namespace gz {
                                                           const double *, double *, BoxTable);
 
 #define GZ_INST(KERN, R)                                                                                                         \
-    template __global__ void k_density_iter<KERN, R>(DevParams, int, const int *, const int *, const double *, const double *,  \
-                                                     const double *, const double *, const double *, const double *,            \
-                                                     const double *, const double *, const int *, long, const int *,             \
-                                                     DensityIter, DensityOut, int *, int *, int *, int *);                       \
     template __global__ void k_density_cells<KERN, R>(DevParams, GridView, CellRows, const int *, const uint8_t *,              \
                                                       const double *, const double *, const double *, const double *,           \
                                                       const double *, const double *, const double *, const double *, double *);\

@@ -329,26 +329,22 @@ __global__ void __launch_bounds__(256) k_build_ell(int nrows, const int *__restr
 // after_iter_body (:516-961) and, on convergence, density_finalize_post_runner
 // (src/hydro/density_loop.cc:303-733)
 // ---------------------------------------------------------------------------------------
-__device__ inline void density_finalize(const DevParams &P, const DensityOut &o, int i, double h, const double *acc, double mass, double u)
+__device__ inline void density_finalize(const DevParams &P, const DensityOut &o, int i, double h, const double *acc, const DensityNorm &dn,
+                                        double mass, double u)
 {
     o.h[i] = h;
-    double nn, dr, dv;
-    density_normalise(acc, h, nn, dr, dv);
-    o.drk[i] = dr;
-    o.divv[i] = dv;
+    const double nn = dn.nn, normalizer = dn.normalizer;
+    o.drk[i] = dn.dr;
+    o.divv[i] = dn.dv;
     for (int k = 0; k < 3; k++) o.fw[(size_t)k * o.n + i] = acc[10 + k];
-    const double V_i = kVolNorm * pow(h, 3.0) / (nn > 0 ? nn : 1.0);
-    const double normalizer = pow(h, -1.0);
-    double Ti[9];
-    const double cn = nvt_inverse(&acc[4], normalizer, Ti);
-    o.t[0 * o.n + i] = Ti[0] / normalizer;
-    o.t[1 * o.n + i] = Ti[4] / normalizer;
-    o.t[2 * o.n + i] = Ti[8] / normalizer;
-    o.t[3 * o.n + i] = Ti[1] / normalizer;
-    o.t[4 * o.n + i] = Ti[5] / normalizer;
-    o.t[5 * o.n + i] = Ti[2] / normalizer;
-    o.cn[i] = cn;
-    o.fce[i] = face_closure(&acc[4], &acc[10], Ti, normalizer, V_i);
+    o.t[0 * o.n + i] = dn.Ti[0] / normalizer;
+    o.t[1 * o.n + i] = dn.Ti[4] / normalizer;
+    o.t[2 * o.n + i] = dn.Ti[8] / normalizer;
+    o.t[3 * o.n + i] = dn.Ti[1] / normalizer;
+    o.t[4 * o.n + i] = dn.Ti[5] / normalizer;
+    o.t[5 * o.n + i] = dn.Ti[2] / normalizer;
+    o.cn[i] = dn.cn;
+    o.fce[i] = dn.fce;
     double rho = acc[1];
     if (mass > 0) {
         if ((h > 0) && (nn > 0)) rho = mass * nn / (kVolNorm * pow(h, 3.0));
@@ -358,87 +354,6 @@ __device__ inline void density_finalize(const DevParams &P, const DensityOut &o,
     }
     o.rho[i] = rho;
     o.nn[i] = (nn > 0) ? pow(nn, 1.0 / 3.0) : 0.0;
-}
-
-template <int KERN, typename R>
-__global__ void __launch_bounds__(256) k_density_iter(DevParams P, int ntodo, const int *__restrict__ todo, const int *__restrict__ act,
-                                                      const double *__restrict__ X, const double *__restrict__ Y, const double *__restrict__ Z,
-                                                      const double *__restrict__ VX, const double *__restrict__ VY, const double *__restrict__ VZ,
-                                                      const double *__restrict__ M, const double *__restrict__ U,
-                                                      const int *__restrict__ ell, long ld, const int *__restrict__ cnt,
-                                                      DensityIter it, DensityOut out, int *__restrict__ next, int *__restrict__ nnext,
-                                                      int *__restrict__ rebuild, int *__restrict__ err)
-{
-    const long k = xcd_block() * blockDim.x + threadIdx.x;
-    if (k >= ntodo) return;
-    const int r = todo[k];
-    const int i = act[r];
-    const double h = it.hcur[r];
-    const double xi = X[i], yi = Y[i], zi = Z[i];
-    const R vxi = (R)VX[i], vyi = (R)VY[i], vzi = (R)VZ[i];
-    R a[13];
-    for (int q = 0; q < 13; q++) a[q] = 0;
-    const R hR = (R)h;
-    const R h2 = hR * hR;
-    R hinv, hinv3, hinv4;
-    kernel_hinv(hR, hinv, hinv3, hinv4);
-    const int c = cnt[r];
-    for (int q = 0; q < c; q++) {
-        const int j = ell[(size_t)q * ld + r];
-        const double mj = M[j];
-        if (mj <= 0) continue;
-        double d0 = xi - X[j], d1 = yi - Y[j], d2 = zi - Z[j];
-        d1 = wrap(d1, P.box, P.half);
-        d0 = wrap(d0, P.box, P.half);
-        d2 = wrap(d2, P.box, P.half);
-        const R dp0 = (R)d0, dp1 = (R)d1, dp2 = (R)d2;
-        const R r2 = dp0 * dp0 + dp1 * dp1 + dp2 * dp2;
-        if (r2 >= h2) continue;
-        const R rr = sqrt(r2);
-        const R u = rr * hinv;
-        R wk = 0, dwk = 0;
-        kernel_main<KERN, 0>(u, hinv3, hinv4, wk, dwk);
-        const R mj_wk = (R)mj * wk;
-        a[0] += wk;
-        a[1] += mj_wk;
-        a[2] += -(R(3.0) * hinv * wk + u * dwk);
-        if (rr > 0) {
-            a[4] += wk * (dp0 * dp0);
-            a[5] += wk * (dp1 * dp1);
-            a[6] += wk * (dp2 * dp2);
-            a[7] += wk * (dp0 * dp1);
-            a[8] += wk * (dp1 * dp2);
-            a[9] += wk * (dp0 * dp2);
-            a[10] += wk * dp0;
-            a[11] += wk * dp1;
-            a[12] += wk * dp2;
-            const R dv0 = vxi - (R)VX[j], dv1 = vyi - (R)VY[j], dv2 = vzi - (R)VZ[j];
-            a[3] -= dwk * (dp0 * dv0 + dp1 * dv1 + dp2 * dv2) / rr;
-        }
-    }
-    double acc[13];
-    for (int q = 0; q < 13; q++) acc[q] = (double)a[q];
-    IterState sc;
-    sc.left = it.left[r]; sc.right = it.right[r]; sc.cn_cur = it.cn_cur[r]; sc.cn_init = it.cn_init[r];
-    const int iter = it.iter[r];
-    double nh = h;
-    int rc = density_after_iter(P, out.cn[i], iter, h, acc, sc, nh);
-    it.iter[r] = iter + 1;
-    if (rc <= 0) {
-        if (rc < 0) atomicMax(err, 90001009);
-        density_finalize(P, out, i, h, acc, M[i], U[i]);
-        return;
-    }
-    it.left[r] = sc.left; it.right[r] = sc.right; it.cn_cur[r] = sc.cn_cur; it.cn_init[r] = sc.cn_init;
-    it.hcur[r] = nh;
-    if (iter + 1 >= kMaxIter) {
-        atomicMax(err, 1155);
-        density_finalize(P, out, i, h, acc, M[i], U[i]);
-        return;
-    }
-    if (nh > 1.3 * it.hbuild[r]) atomicOr(rebuild, 1);
-    int w = atomicAdd(nnext, 1);
-    next[w] = r;
 }
 
 // ---------------------------------------------------------------------------------------
@@ -661,6 +576,19 @@ __global__ void __launch_bounds__(64) k_density_cells(DevParams P, GridView g, C
     }
 }
 
+// the maximum of v over the block into *dst (one conditional atomic per block)
+__device__ __forceinline__ void block_max_to(double v, unsigned long long *dst)
+{
+    __shared__ double wv[16];
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    if ((threadIdx.x & 63) == 0) wv[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int k = 1; k < (int)((blockDim.x + 63) >> 6); k++) v = fmax(v, wv[k]);
+        if (v > 0) atomic_max_bits(dst, v);
+    }
+}
+
 // density()'s pre-iteration radius clamp (src/hydro/density_loop.cc:818-838) and the start of
 // the iteration: radius of each row, empty bracket, round counters, the largest radius (the
 // bits of a positive double order like the double)
@@ -686,8 +614,7 @@ __global__ void __launch_bounds__(256) k_density_init(int na, const int *__restr
         need[r] = 1;
     }
     if (!hmax) return;
-    for (int o = 32; o > 0; o >>= 1) h = fmax(h, __shfl_xor(h, o, 64));
-    if ((threadIdx.x & 63) == 0 && h > 0) atomicMax(hmax, (unsigned long long)__double_as_longlong(h));
+    block_max_to(h, hmax);
 }
 #endif
 
@@ -697,7 +624,7 @@ __device__ __forceinline__ void wave_max_to(double v, bool valid, int box, unsig
 {
     if (!bt.b) {
         for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
-        if ((threadIdx.x & 63) == 0 && v > 0) atomicMax(single, (unsigned long long)__double_as_longlong(v));
+        if ((threadIdx.x & 63) == 0 && v > 0) atomic_max_bits(single, v);
         return;
     }
     const unsigned long long vm = __ballot(valid);
@@ -706,10 +633,9 @@ __device__ __forceinline__ void wave_max_to(double v, bool valid, int box, unsig
     if (__all(!valid || box == b0)) {
         double w = valid ? v : 0.0;
         for (int o = 32; o > 0; o >>= 1) w = fmax(w, __shfl_xor(w, o, 64));
-        if ((threadIdx.x & 63) == 0 && w > 0)
-            atomicMax(&bt.slot[(size_t)b0 * kBoxSlots + slot], (unsigned long long)__double_as_longlong(w));
+        if ((threadIdx.x & 63) == 0 && w > 0) atomic_max_bits(&bt.slot[(size_t)b0 * kBoxSlots + slot], w);
     } else if (valid && v > 0) {
-        atomicMax(&bt.slot[(size_t)box * kBoxSlots + slot], (unsigned long long)__double_as_longlong(v));
+        atomic_max_bits(&bt.slot[(size_t)box * kBoxSlots + slot], v);
     }
 }
 
@@ -760,12 +686,14 @@ __global__ void __launch_bounds__(256) k_density_after(DevParams P, int nrows, c
         sc.left = it.left[r]; sc.right = it.right[r]; sc.cn_cur = it.cn_cur[r]; sc.cn_init = it.cn_init[r];
         const int iter = it.iter[r];
         double nh = h;
-        const int rc = density_after_iter(P, out.cn[i], iter, h, acc, sc, nh);
+        DensityNorm dn;
+        density_norm(acc, h, dn);
+        const int rc = density_after_iter(P, out.cn[i], iter, h, dn, sc, nh);
         it.iter[r] = iter + 1;
         if (rc <= 0 || iter + 1 >= kMaxIter) {
             if (rc < 0) atomicMax(err, 90001009);
             else if (rc > 0) atomicMax(err, 1155);
-            density_finalize(P, out, i, h, acc, M[i], U[i]);
+            density_finalize(P, out, i, h, acc, dn, M[i], U[i]);
             need[r] = 0;
             if (h > 0) h_final = h;
         } else {
@@ -775,13 +703,23 @@ __global__ void __launch_bounds__(256) k_density_after(DevParams P, int nrows, c
             cont = true;
         }
     }
+    // rows still iterating and the grid flag: one atomic per block
     const unsigned long long cm = __ballot(cont), gm = __ballot(h_grow > 0);
-    if ((threadIdx.x & 63) == 0) {
-        if (cm) atomicAdd(nnext, (int)__popcll(cm));
-        if (gm) atomicOr(rebuild, 1);
+    __shared__ int wcont[16], wgrow[16];
+    if ((threadIdx.x & 63) == 0) { wcont[threadIdx.x >> 6] = (int)__popcll(cm); wgrow[threadIdx.x >> 6] = gm != 0; }
+    if (bt.b) {
+        wave_max_to(h_final, row, box, hfinal, bt, kSlotHfinal);
+        if (gm) wave_max_to(h_grow, row, box, nullptr, bt, kSlotHgrow);
+        __syncthreads();
+    } else {
+        block_max_to(h_final, hfinal);  // synchronises the block
     }
-    wave_max_to(h_final, row, box, hfinal, bt, kSlotHfinal);
-    if (bt.b && gm) wave_max_to(h_grow, row, box, nullptr, bt, kSlotHgrow);
+    if (threadIdx.x == 0) {
+        int c = 0, g = 0;
+        for (int k = 0; k < (int)((blockDim.x + 63) >> 6); k++) { c += wcont[k]; g |= wgrow[k]; }
+        if (c) atomicAdd(nnext, c);
+        if (g) atomicOr(rebuild, 1);
+    }
 }
 #endif
 
@@ -1075,8 +1013,9 @@ __global__ void __launch_bounds__(256) k_force(DevParams P, Gam<R> G, int nrows,
 {
     const int k = (int)(xcd_block() * blockDim.x + threadIdx.x);
     const int r = k < nrows ? (sched ? sched[k] : k) : nrows;
-    // pairs of this launch (wave sum, one atomic per wave)
-    {
+    // pairs of this launch when asked (wave sum, one atomic per wave; a launch of k_sum_counts
+    // after the pass costs fewer atomics on the one address)
+    if (pairs) {
         unsigned long long c = (r < nrows) ? (unsigned long long)cnt[r] : 0ull;
         for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
         if ((threadIdx.x & 63) == 0 && c) atomicAdd(pairs, c);

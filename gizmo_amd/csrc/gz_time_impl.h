@@ -397,8 +397,8 @@ __global__ void __launch_bounds__(256) k_drift(TimeParams T, ParticleView S, lon
         __syncthreads();
         if (threadIdx.x == 0) {
             for (int k = 1; k < (int)(blockDim.x >> 6); k++) { d1 = fmax(d1, sm[k]); h1 = fmax(h1, sh[k]); }
-            if (d1 > 0) atomicMax(disp, (unsigned long long)__double_as_longlong(d1));
-            if (h1 > 0) atomicMax(hmax, (unsigned long long)__double_as_longlong(h1));
+            if (d1 > 0) atomic_max_bits(disp, d1);
+            if (h1 > 0) atomic_max_bits(hmax, h1);
         }
     }
 }
@@ -519,7 +519,7 @@ __global__ void __launch_bounds__(256) k_active_count(long n, const unsigned lon
     if (tid == 0) {
         bcount[blockIdx.x] = wcount[0] + wcount[1] + wcount[2] + wcount[3];
         hm = fmax(fmax(whm[0], whm[1]), fmax(whm[2], whm[3]));
-        if (hm > 0) atomicMax(hmax_act, (unsigned long long)__double_as_longlong(hm));
+        if (hm > 0) atomic_max_bits(hmax_act, hm);
     }
 }
 
