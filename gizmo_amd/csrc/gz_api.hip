@@ -88,6 +88,7 @@ struct gz_context {
 
     // staging and scratch
     double *stage = nullptr;  // n*9 doubles
+    std::vector<double *> spare[10];  // by width: capacity x width buffers a re-ordering gathers into
     void *tmp = nullptr;
     size_t tmp_bytes = 0;
     uint32_t *key2 = nullptr;
@@ -357,6 +358,10 @@ void free_state(gz_context *c)
     c->wcells_alloc = 0;
     c->wide_key_version = -1;
     dfree(c->ival); dfree(c->ival2); dfree(c->iflag); dfree(c->iscan); dfree(c->act); dfree(c->stage);
+    for (auto &v : c->spare) {
+        for (double *p : v) hipFree(p);
+        v.clear();
+    }
     dfree(c->it_h); dfree(c->it_hb); dfree(c->it_l); dfree(c->it_r); dfree(c->it_cn); dfree(c->it_ci); dfree(c->it_it);
     dfree(c->todo); dfree(c->todo2); dfree(c->cnt_d); dfree(c->cnt_s); dfree(c->ell_d); dfree(c->ell_s);
     c->counters = nullptr;  // inside u64c
@@ -1010,15 +1015,6 @@ template <int K, typename R> struct ForceF { static int run(gz_context *c) { ret
 // re-ordering: particles that moved are put back in Morton order (keys of the current
 // positions), every per-particle field follows, and the displacement bound restarts at zero
 // ---------------------------------------------------------------------------------------
-int permute_doubles(gz_context *ctx, const int *order, double *field, int width)
-{
-    const long n = ctx->n;
-    k_permute_cols<<<nblk(n), kBlock, 0, ctx->stream>>>(n, width, order, (const unsigned long long *)field,
-                                                        (unsigned long long *)ctx->stage);
-    HIPCHK(hipGetLastError());
-    return copy_dev(ctx, field, ctx->stage, (size_t)n * width * sizeof(double));
-}
-
 int resort(gz_context *ctx)
 {
     const long n = ctx->n;
@@ -1030,23 +1026,65 @@ int resort(gz_context *ctx)
     CHK(ensure_tmp(ctx, bytes));
     HIPCHK(hipcub::DeviceRadixSort::SortPairs(ctx->tmp, bytes, ctx->key2, ctx->key, ctx->ival, ctx->ival2, (int)n, 0, 3 * kFineLevel, ctx->stream));
     const int *order = ctx->ival2;
-    struct F { double *p; int w; } fields[] = {
-        {ctx->X, 1}, {ctx->Y, 1}, {ctx->Z, 1}, {ctx->VX, 1}, {ctx->VY, 1}, {ctx->VZ, 1}, {ctx->VKX, 1}, {ctx->VKY, 1},
-        {ctx->VKZ, 1}, {ctx->M, 1}, {ctx->H, 1}, {ctx->U, 1}, {ctx->UK, 1}, {ctx->NN, 1}, {ctx->DRK, 1}, {ctx->DIVV, 1},
-        {ctx->RHO, 1}, {ctx->PR, 1}, {ctx->T, 6}, {ctx->FW, 3}, {ctx->CN, 1}, {ctx->FCE, 1}, {ctx->GR, 3}, {ctx->GP, 3},
-        {ctx->GV, 9}, {ctx->ACC, 3}, {ctx->DTU, 1}, {ctx->MSV, 1}, {ctx->SOFT, 1},
-        {(double *)ctx->TIB, 1}, {(double *)ctx->DTS, 1}};
-    for (auto &f : fields) CHK(permute_doubles(ctx, order, f.p, f.w));
-    int *ints[3] = {ctx->TB, ctx->WAKE, ctx->perm};
-    for (int *a : ints) {
-        k_permute_int<<<nblk(n), kBlock, 0, ctx->stream>>>(n, order, a, ctx->ival);
+    // every per-particle column gathered into a spare buffer of its width, up to kPermCols
+    // columns per launch; the spare then takes the field's place and the field's old buffer
+    // becomes a spare (stream order: a later launch writes it only after this one read it).  The
+    // ordering positions X0, Y0, Z0 are gathered from X, Y, Z in the first launch.
+    struct F { double **src; double **dst; int w; } fields[] = {
+        {&ctx->X, &ctx->X0, 1}, {&ctx->Y, &ctx->Y0, 1}, {&ctx->Z, &ctx->Z0, 1},
+        {&ctx->X, nullptr, 1}, {&ctx->Y, nullptr, 1}, {&ctx->Z, nullptr, 1}, {&ctx->VX, nullptr, 1}, {&ctx->VY, nullptr, 1},
+        {&ctx->VZ, nullptr, 1}, {&ctx->VKX, nullptr, 1}, {&ctx->VKY, nullptr, 1}, {&ctx->VKZ, nullptr, 1}, {&ctx->M, nullptr, 1},
+        {&ctx->H, nullptr, 1}, {&ctx->U, nullptr, 1}, {&ctx->UK, nullptr, 1}, {&ctx->NN, nullptr, 1}, {&ctx->DRK, nullptr, 1},
+        {&ctx->DIVV, nullptr, 1}, {&ctx->RHO, nullptr, 1}, {&ctx->PR, nullptr, 1}, {&ctx->CN, nullptr, 1}, {&ctx->FCE, nullptr, 1},
+        {&ctx->DTU, nullptr, 1}, {&ctx->MSV, nullptr, 1}, {&ctx->SOFT, nullptr, 1}, {reinterpret_cast<double **>(&ctx->TIB), nullptr, 1},
+        {reinterpret_cast<double **>(&ctx->DTS), nullptr, 1}, {&ctx->ACC, nullptr, 3}, {&ctx->FW, nullptr, 3}, {&ctx->GR, nullptr, 3},
+        {&ctx->GP, nullptr, 3}, {&ctx->T, nullptr, 6}, {&ctx->GV, nullptr, 9}};
+    PermCols P{};
+    struct Swap { double **field; double *fresh; int w; };
+    std::vector<Swap> swaps;
+    auto flush = [&]() -> int {
+        if (P.ncol == 0) return 0;
+        k_permute_many<<<nblk(n), kBlock, 0, ctx->stream>>>(n, order, P);
         HIPCHK(hipGetLastError());
-        CHK(copy_dev(ctx, a, ctx->ival, n * sizeof(int)));
+        for (const Swap &w : swaps) {
+            ctx->spare[w.w].push_back(*w.field);
+            *w.field = w.fresh;
+        }
+        swaps.clear();
+        P = PermCols{};
+        return 0;
+    };
+    for (const F &f : fields) {
+        if (P.ncol + f.w > kPermCols) CHK(flush());
+        double *out = f.dst ? *f.dst : nullptr;
+        if (!out) {
+            auto &pool = ctx->spare[f.w];
+            if (pool.empty()) {
+                double *p = nullptr;
+                CHK(dalloc(ctx, &p, (size_t)ctx->cap * f.w));
+                pool.push_back(p);
+            }
+            out = pool.back();
+            pool.pop_back();
+            swaps.push_back({f.src, out, f.w});
+        }
+        for (int c = 0; c < f.w; c++) {
+            P.in[P.ncol] = (const unsigned long long *)(*f.src + (size_t)c * n);
+            P.out[P.ncol] = (unsigned long long *)(out + (size_t)c * n);
+            P.ncol++;
+        }
+    }
+    CHK(flush());
+    // the int columns: gathered into ival, which then takes the column's place
+    int **ints[3] = {&ctx->TB, &ctx->WAKE, &ctx->perm};
+    for (int **a : ints) {
+        k_permute_int<<<nblk(n), kBlock, 0, ctx->stream>>>(n, order, *a, ctx->ival);
+        HIPCHK(hipGetLastError());
+        std::swap(*a, ctx->ival);
     }
     k_permute_u8<<<nblk(n), kBlock, 0, ctx->stream>>>(n, order, ctx->ACT, (uint8_t *)ctx->stage);
     HIPCHK(hipGetLastError());
     CHK(copy_dev(ctx, ctx->ACT, ctx->stage, n));
-    CHK(copy_dev(ctx, ctx->X0, ctx->X, n * 8)); CHK(copy_dev(ctx, ctx->Y0, ctx->Y, n * 8)); CHK(copy_dev(ctx, ctx->Z0, ctx->Z, n * 8));
     ctx->dmax = 0;
     ctx->key_version++;
     ctx->sym_valid = false;
@@ -1172,22 +1210,23 @@ int run_step(gz_context *ctx)
         if (ctx->nact > 0) k_halfstep_kick<<<nblk(ctx->nact), kBlock, 0, ctx->stream>>>(T, S, ctx->nact, ctx->act, 0, BoxTable{});
         k_next_sync<<<1, 64, 0, ctx->stream>>>(T, ctx->u64c, ctx->u64c + 70, BoxTable{});
     }
-    {
-        Timed t(ctx, "drift");
-        k_drift<<<std::min(nblk(n), 2048u), kBlock, 0, ctx->stream>>>(T, S, ti_prev, ctx->u64c + 70, ctx->X0, ctx->Y0, ctx->Z0,
-                                                                     ctx->u64c + 64, ctx->u64c + 66, BoxTable{});
+    // the drift of every particle to the next sync point fused with the row flags and counts
+    // per block of particles of make_list_of_active_particles; then the list in ascending order
+    // (no scan over every particle)
+    const int nb_act = (int)((n + kActChunk - 1) / kActChunk);
+    if (!ctx->act_bits) {
+        const long m = (ctx->cap + kActChunk - 1) / kActChunk;
+        CHK(dalloc(ctx, &ctx->act_bits, (size_t)m * (kActChunk / 64))); CHK(dalloc(ctx, &ctx->act_bcount, m));
     }
     {
-        // make_list_of_active_particles: row flags and counts per block of particles, then the
-        // list in ascending order (no scan over every particle)
+        Timed t(ctx, "drift");
+        k_drift_active<<<nb_act, 256, 0, ctx->stream>>>(T, S, ti_prev, ctx->u64c + 70, ctx->X0, ctx->Y0, ctx->Z0, ctx->u64c + 64,
+                                                         ctx->u64c + 66, 0.99 * ctx->par.max_kernel_radius, ctx->ACT, ctx->act_bits,
+                                                         ctx->act_bcount, ctx->u64c + 67);
+        HIPCHK(hipGetLastError());
+    }
+    {
         Timed t(ctx, "timeline");
-        const int nb_act = (int)((n + kActChunk - 1) / kActChunk);
-        if (!ctx->act_bits) {
-            const long m = (ctx->cap + kActChunk - 1) / kActChunk;
-            CHK(dalloc(ctx, &ctx->act_bits, (size_t)m * (kActChunk / 64))); CHK(dalloc(ctx, &ctx->act_bcount, m));
-        }
-        k_active_count<<<nb_act, 256, 0, ctx->stream>>>(n, ctx->u64c + 70, ctx->TB, ctx->M, ctx->H, 0.99 * ctx->par.max_kernel_radius,
-                                                        ctx->ACT, ctx->SOFT, ctx->act_bits, ctx->act_bcount, ctx->u64c + 67);
         k_active_compact<<<nb_act, 256, 0, ctx->stream>>>(n, nb_act, ctx->act_bits, ctx->act_bcount, ctx->act, ctx->counters + 10);
         HIPCHK(hipGetLastError());
     }
@@ -1220,8 +1259,10 @@ int run_step(gz_context *ctx)
         CHK(resort(ctx));
         CHK(build_active(ctx));
     }
-    // compute_hydro_densities_and_forces, then do_second_halfstep_kick
+    // compute_hydro_densities_and_forces, then do_second_halfstep_kick (the columns' buffers
+    // change places in a re-ordering: the view is taken again)
     T = time_params(ctx);
+    S = particle_view(ctx);
     CHK(hydro_pass(ctx));
     if (ctx->nact > 0) {
         Timed t(ctx, "timeline");

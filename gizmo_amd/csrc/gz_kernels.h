@@ -160,9 +160,10 @@ __global__ void k_next_sync(TimeParams T, const unsigned long long *count, unsig
 __global__ void k_active_flags(long n, const unsigned long long *tl, const int *TB, const double *M, const double *H, double hclamp,
                                uint8_t *ACT, double *SOFT, int *flag, unsigned long long *hmax_act, BoxTable bt);
 __global__ void k_compact_count(long n, const int *flag, const int *scan, int *act, int *count_out);
-constexpr int kActChunk = 2048;  // particles per block of the active-list kernels
-__global__ void k_active_count(long n, const unsigned long long *tl, const int *TB, const double *M, const double *H, double hclamp,
-                               uint8_t *ACT, double *SOFT, unsigned long long *bits, int *bcount, unsigned long long *hmax_act);
+constexpr int kActChunk = 512;  // particles per block of the active-list kernels (and of the drift they are fused with)
+__global__ void k_drift_active(TimeParams T, ParticleView S, long long time0, const unsigned long long *tl, const double *X0,
+                               const double *Y0, const double *Z0, unsigned long long *disp, unsigned long long *hmax,
+                               double hclamp, uint8_t *ACT, unsigned long long *bits, int *bcount, unsigned long long *hmax_act);
 __global__ void k_active_compact(long n, int nblocks, const unsigned long long *bits, const int *bcount, int *act, int *count_out);
 __global__ void k_advance_boxes(BoxTable bt);
 __global__ void k_active_from_bins(long n, unsigned long long mask, const int *TB, const double *M, const double *H,
@@ -202,6 +203,13 @@ __global__ void k_morton_soa(long n, const double *X, const double *Y, const dou
 __global__ void k_permute_cols(long n, int width, const int *order, const unsigned long long *in, unsigned long long *out);
 __global__ void k_permute_int(long n, const int *order, const int *in, int *out);
 __global__ void k_permute_u8(long n, const int *order, const uint8_t *in, uint8_t *out);
+constexpr int kPermCols = 16;
+struct PermCols {
+    const unsigned long long *in[kPermCols];
+    unsigned long long *out[kPermCols];
+    int ncol;
+};
+__global__ void k_permute_many(long n, const int *order, PermCols P);
 __global__ void k_gather(long n, int width, const int *perm, const double *in, double *out, int soa);
 __global__ void k_gather_int(long n, const int *perm, const int *in, int *out);
 __global__ void k_gather_u8(long n, const int *perm, const uint8_t *in, uint8_t *out);

@@ -93,6 +93,22 @@ __global__ void k_permute_u8(long n, const int *__restrict__ order, const uint8_
     out[s] = in[order[s]];
 }
 
+// up to kPermCols 8-byte columns gathered by one order in one launch (every load of a lane
+// issued before its stores: the columns are distinct buffers)
+__global__ void __launch_bounds__(256) k_permute_many(long n, const int *__restrict__ order, PermCols P)
+{
+    const long s = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const long u = order[s];
+    unsigned long long v[kPermCols];
+#pragma unroll
+    for (int c = 0; c < kPermCols; c++)
+        if (c < P.ncol) v[c] = P.in[c][u];
+#pragma unroll
+    for (int c = 0; c < kPermCols; c++)
+        if (c < P.ncol) P.out[c][s] = v[c];
+}
+
 __global__ void k_gather_ll(long n, const int *__restrict__ perm, const long long *__restrict__ in, long long *__restrict__ out)
 {
     long s = blockIdx.x * (long)blockDim.x + threadIdx.x;

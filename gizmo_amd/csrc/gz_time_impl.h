@@ -312,7 +312,7 @@ __global__ void __launch_bounds__(256) k_halfstep_kick(TimeParams T, ParticleVie
 // it was when the particles were last ordered.
 __device__ __forceinline__ double drift_one(const TimeParams &T, const ParticleView &S, long i, long long time0, long long time1,
                                             const double *__restrict__ X0, const double *__restrict__ Y0,
-                                            const double *__restrict__ Z0)
+                                            const double *__restrict__ Z0, double *h_after = nullptr)
 {
     const double dt_drift = (time1 - time0) * T.tbi * 1.0;
     double x = S.X[i] + S.VKX[i] * dt_drift;
@@ -337,6 +337,9 @@ __device__ __forceinline__ double drift_one(const TimeParams &T, const ParticleV
         S.U[i] = u;
         S.H[i] = h;
         S.PR[i] = (T.gamma - 1) * u * rho;
+        if (h_after) *h_after = h;
+    } else if (h_after) {
+        *h_after = S.H[i];
     }
     if (x >= T.box) x -= T.box; else if (x < 0) x += T.box;
     if (y >= T.box) y -= T.box; else if (y < 0) y += T.box;
@@ -477,33 +480,45 @@ __global__ void __launch_bounds__(256) k_active_flags(long n, const unsigned lon
     wave_max_to(h, valid, box, hmax_act, bt, kSlotHrows);
 }
 
-// The active list of one box without a scan over every particle: blocks of kActChunk
-// particles (thread t takes particles t, t+256, ...), each wavefront's 64 row flags kept as a
-// ballot.  k_active_count: the flags of k_active_flags and each block's row count; then
-// k_active_compact: each block's offset (the counts of the blocks before it) and the rows in
-// ascending order.
-__global__ void __launch_bounds__(256) k_active_count(long n, const unsigned long long *__restrict__ tl, const int *__restrict__ TB,
-                                                      const double *__restrict__ M, const double *__restrict__ H, double hclamp,
-                                                      uint8_t *__restrict__ ACT, double *__restrict__ SOFT,
+// The drift of every particle to the next sync point (drift_one, as k_drift does for boxes)
+// fused with the first half of make_list_of_active_particles (src/core/run.cc:997-1009): block b
+// drifts the kActChunk particles of chunk b (thread t takes particles t, t+256, ...) and keeps
+// each wavefront's 64 row flags as a ballot and the block's row count; k_active_compact then
+// writes each block's rows at its offset (the counts of the blocks before it) in ascending
+// order.  No scan over every particle, and the flags read the drifted radius from registers
+// instead of a second sweep.
+__global__ void __launch_bounds__(256) k_drift_active(TimeParams T, ParticleView S, long long time0, const unsigned long long *__restrict__ tl,
+                                                      const double *__restrict__ X0, const double *__restrict__ Y0,
+                                                      const double *__restrict__ Z0, unsigned long long *__restrict__ disp,
+                                                      unsigned long long *__restrict__ hmax, double hclamp, uint8_t *__restrict__ ACT,
                                                       unsigned long long *__restrict__ bits, int *__restrict__ bcount,
                                                       unsigned long long *__restrict__ hmax_act)
 {
-    __shared__ int wcount[4];
+    const long long time1 = (long long)tl[0];
+    const unsigned long long mask = tl[1];
     const long base = (long)blockIdx.x * kActChunk;
     const int tid = threadIdx.x, w = tid >> 6;
-    const unsigned long long mask = tl[1];
-    double hm = 0;
+    double d1 = 0, h1 = 0, hm = 0;
     int count = 0;
     for (int k = 0; k < kActChunk / 256; k++) {
         const long i = base + k * 256 + tid;
         bool row = false;
-        if (i < n) {
-            const bool on = bin_on(mask, TB[i]);
+        if (i < S.n) {
+            // the flag's inputs loaded before the drift's stores (which may alias them for
+            // the compiler) and the drifted radius taken from the drift
+            const int tb = S.TB[i];
+            const double m = S.M[i];
+            double d = 0, hh;
+            if (time1 != time0) d = drift_one(T, S, i, time0, time1, X0, Y0, Z0, &hh);
+            else hh = S.H[i];
+            d1 = fmax(d1, d);
+            h1 = fmax(h1, hh);
+            const bool on = bin_on(mask, tb);
             ACT[i] = on ? 1 : 0;
-            row = on && (M[i] > 0);
+            row = on && (m > 0);
             if (row) {
-                double h = H[i];
-                SOFT[i] = h;
+                S.SOFT[i] = hh;
+                double h = hh;
                 if ((h < 0) || !isfinite(h) || (h > hclamp)) h = hclamp;
                 hm = fmax(hm, h);
             }
@@ -512,13 +527,22 @@ __global__ void __launch_bounds__(256) k_active_count(long n, const unsigned lon
         if ((tid & 63) == 0) bits[(long)blockIdx.x * (kActChunk / 64) + k * 4 + w] = b;
         count += __popcll(b);
     }
-    __shared__ double whm[4];
-    for (int o = 32; o > 0; o >>= 1) hm = fmax(hm, __shfl_xor(hm, o, 64));
-    if ((tid & 63) == 0) { wcount[w] = count; whm[w] = hm; }
+    __shared__ int wcount[4];
+    __shared__ double wd[4], wh[4], whm[4];
+    for (int o = 32; o > 0; o >>= 1) {
+        d1 = fmax(d1, __shfl_xor(d1, o, 64));
+        h1 = fmax(h1, __shfl_xor(h1, o, 64));
+        hm = fmax(hm, __shfl_xor(hm, o, 64));
+    }
+    if ((tid & 63) == 0) { wcount[w] = count; wd[w] = d1; wh[w] = h1; whm[w] = hm; }
     __syncthreads();
     if (tid == 0) {
         bcount[blockIdx.x] = wcount[0] + wcount[1] + wcount[2] + wcount[3];
+        d1 = fmax(fmax(wd[0], wd[1]), fmax(wd[2], wd[3]));
+        h1 = fmax(fmax(wh[0], wh[1]), fmax(wh[2], wh[3]));
         hm = fmax(fmax(whm[0], whm[1]), fmax(whm[2], whm[3]));
+        if (d1 > 0) atomic_max_bits(disp, d1);
+        if (h1 > 0) atomic_max_bits(hmax, h1);
         if (hm > 0) atomic_max_bits(hmax_act, hm);
     }
 }
@@ -531,7 +555,14 @@ __global__ void __launch_bounds__(256) k_active_compact(long n, int nblocks, con
     const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
     // the rows of the blocks before this one
     int before = 0;
-    for (int b = tid; b < (int)blockIdx.x; b += 256) before += bcount[b];
+    const int nprev = (int)blockIdx.x;
+    for (int b = tid; b < nprev; b += 1024) {  // four loads in flight per thread
+        const int v0 = bcount[b];
+        const int v1 = (b + 256 < nprev) ? bcount[b + 256] : 0;
+        const int v2 = (b + 512 < nprev) ? bcount[b + 512] : 0;
+        const int v3 = (b + 768 < nprev) ? bcount[b + 768] : 0;
+        before += (v0 + v1) + (v2 + v3);
+    }
     for (int o = 32; o > 0; o >>= 1) before += __shfl_xor(before, o, 64);
     if (lane == 0) spart[w] = before;
     const unsigned long long *mb = bits + (long)blockIdx.x * (kActChunk / 64);
