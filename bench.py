@@ -17,7 +17,7 @@ for the whole timed region.  Warmup: W steps of an identical untimed job.
 Both arithmetic forms run the same job: the plain FP64 form (the reported `value`, as the
 job is the FP64-at-scale configuration) and the FP32-mixed form beside it.  `roofline` covers
 the dominant kernel of the timed job, the drift of all 2M particles to each sync point fused
-with their active flags (k_drift_active: 253 algorithmic bytes per particle, 21 doubles and the
+with their active flags (k_drift_active: 229 algorithmic bytes per particle, 18 doubles and the
 time bin read, 10 doubles and the flag written), its average
 launch time measured with HIP events on the stream the kernels run on.  `roofline_force` is
 the force kernel over the same job (pair-gather model: n_active (S_i + S_out) + n_pairs
@@ -48,10 +48,10 @@ INTEG = dict(courant_fac=0.05, err_tol_int_accuracy=0.002, max_size_timestep=0.0
              time_begin=0.0, time_max=1.0, min_egy_spec=0.0)
 FAMILIES = ("timeline", "drift", "sort", "ngb_build", "density", "gradients", "force")
 # the drift of one particle (k_drift_active): reads X,Y,Z, VelKick3, DivVel, m, VelPred3, HydroAccel3,
-# rho, u, du/dt, h, the three ordering positions (21 doubles) and the time bin (int); writes X,Y,Z,
-# VelPred3, rho, u, h, P (10 doubles) and the active flag (1 byte); the radius cache of the active rows
-# (8 B each) and one flag bit per particle are left out
-DRIFT_BYTES = {"fp64": 253, "mixed": 253}
+# rho, u, du/dt, h (18 doubles) and the time bin (int); writes X,Y,Z, VelPred3, rho, u, h, P (10 doubles)
+# and the active flag (1 byte); the radius cache of the active rows (8 B each) and one flag bit per
+# particle are left out
+DRIFT_BYTES = {"fp64": 229, "mixed": 229}
 # HBM bytes per particle that k_drift_active moves, measured: FETCH_SIZE + WRITE_SIZE per dispatch at 2,097,152
 # particles (FETCH_SIZE scaled x2 by a 16 MiB device copy), profiles/sedov128_timestep_job_traffic.md
 DRIFT_TRAFFIC_PER_PARTICLE = 530.7e6 / 2097152

@@ -53,7 +53,8 @@ struct gz_context {
     long long *TIB = nullptr, *DTS = nullptr;
     // positions when the particles were last ordered, and the largest periodic distance any
     // particle has moved since (the search grid widens its cells by twice that)
-    double *X0 = nullptr, *Y0 = nullptr, *Z0 = nullptr;
+    // dmax: a bound on how far any particle moved since the last ordering (the sum of each
+    // step's longest drift along an axis)
     double dmax = 0, last_rmax = 0;
     double resort_fraction = 0.125;  // order again once dmax exceeds this fraction of the search radius
     int few_rows = 8192;             // at most this many active rows: one wavefront per row
@@ -324,7 +325,6 @@ int alloc_state(gz_context *ctx, long cap)
     CHK(dalloc(ctx, (char **)&ctx->rec_force, n * sizeof(Cell<double>)));
     CHK(dalloc(ctx, &ctx->VKX, n)); CHK(dalloc(ctx, &ctx->VKY, n)); CHK(dalloc(ctx, &ctx->VKZ, n));
     CHK(dalloc(ctx, &ctx->UK, n)); CHK(dalloc(ctx, &ctx->SOFT, n)); CHK(dalloc(ctx, &ctx->TIB, n)); CHK(dalloc(ctx, &ctx->DTS, n));
-    CHK(dalloc(ctx, &ctx->X0, n)); CHK(dalloc(ctx, &ctx->Y0, n)); CHK(dalloc(ctx, &ctx->Z0, n));
     // the 64-bit counters and maxima, then the 16 int counters, in one allocation: one copy
     // reads a stage's results back
     CHK(dalloc(ctx, &ctx->u64c, 80 + 8));
@@ -369,7 +369,7 @@ void free_state(gz_context *c)
     dfree(c->ca0); dfree(c->ca1); dfree(c->cells); dfree(c->cflag); dfree(c->cscan); dfree(c->need);
     dfree(c->rec_grad); dfree(c->rec_force); dfree(c->dacc);
     dfree(c->VKX); dfree(c->VKY); dfree(c->VKZ); dfree(c->UK); dfree(c->SOFT); dfree(c->TIB); dfree(c->DTS);
-    dfree(c->X0); dfree(c->Y0); dfree(c->Z0); dfree(c->u64c);
+    dfree(c->u64c);
     if (c->hu64c) hipHostFree(c->hu64c);
     c->hu64c = nullptr;
     if (c->tmp) hipFree(c->tmp);
@@ -1028,10 +1028,8 @@ int resort(gz_context *ctx)
     const int *order = ctx->ival2;
     // every per-particle column gathered into a spare buffer of its width, up to kPermCols
     // columns per launch; the spare then takes the field's place and the field's old buffer
-    // becomes a spare (stream order: a later launch writes it only after this one read it).  The
-    // ordering positions X0, Y0, Z0 are gathered from X, Y, Z in the first launch.
+    // becomes a spare (stream order: a later launch writes it only after this one read it)
     struct F { double **src; double **dst; int w; } fields[] = {
-        {&ctx->X, &ctx->X0, 1}, {&ctx->Y, &ctx->Y0, 1}, {&ctx->Z, &ctx->Z0, 1},
         {&ctx->X, nullptr, 1}, {&ctx->Y, nullptr, 1}, {&ctx->Z, nullptr, 1}, {&ctx->VX, nullptr, 1}, {&ctx->VY, nullptr, 1},
         {&ctx->VZ, nullptr, 1}, {&ctx->VKX, nullptr, 1}, {&ctx->VKY, nullptr, 1}, {&ctx->VKZ, nullptr, 1}, {&ctx->M, nullptr, 1},
         {&ctx->H, nullptr, 1}, {&ctx->U, nullptr, 1}, {&ctx->UK, nullptr, 1}, {&ctx->NN, nullptr, 1}, {&ctx->DRK, nullptr, 1},
@@ -1220,7 +1218,7 @@ int run_step(gz_context *ctx)
     }
     {
         Timed t(ctx, "drift");
-        k_drift_active<<<nb_act, 256, 0, ctx->stream>>>(T, S, ti_prev, ctx->u64c + 70, ctx->X0, ctx->Y0, ctx->Z0, ctx->u64c + 64,
+        k_drift_active<<<nb_act, 256, 0, ctx->stream>>>(T, S, ti_prev, ctx->u64c + 70, ctx->u64c + 64,
                                                          ctx->u64c + 66, 0.99 * ctx->par.max_kernel_radius, ctx->ACT, ctx->act_bits,
                                                          ctx->act_bcount, ctx->u64c + 67);
         HIPCHK(hipGetLastError());
@@ -1250,7 +1248,7 @@ int run_step(gz_context *ctx)
         memcpy(&d, &ctx->hu64c[64], sizeof d);
         memcpy(&hm, &ctx->hu64c[66], sizeof hm);
         memcpy(&ha, &ctx->hu64c[67], sizeof ha);
-        ctx->dmax = d;
+        ctx->dmax += d;  // the bound grows by the step's longest drift
         ctx->hmax_bound = hm;    // every radius after the drift
         ctx->hmax_rows = ha;     // the largest radius density() starts its rows from
     }
@@ -1392,7 +1390,6 @@ int gz_load(gz_context *ctx, long n, const gz_fields *f)
             HIPCHK(hipMemsetAsync(tdst[q], 0, n * sizeof(long long), ctx->stream));
         }
     }
-    CHK(copy_dev(ctx, ctx->X0, ctx->X, n * 8)); CHK(copy_dev(ctx, ctx->Y0, ctx->Y, n * 8)); CHK(copy_dev(ctx, ctx->Z0, ctx->Z, n * 8));
     ctx->dmax = 0;
     ctx->key_version++;
     ctx->hmax_bound = -1;

@@ -161,8 +161,8 @@ __global__ void k_active_flags(long n, const unsigned long long *tl, const int *
                                uint8_t *ACT, double *SOFT, int *flag, unsigned long long *hmax_act, BoxTable bt);
 __global__ void k_compact_count(long n, const int *flag, const int *scan, int *act, int *count_out);
 constexpr int kActChunk = 512;  // particles per block of the active-list kernels (and of the drift they are fused with)
-__global__ void k_drift_active(TimeParams T, ParticleView S, long long time0, const unsigned long long *tl, const double *X0,
-                               const double *Y0, const double *Z0, unsigned long long *disp, unsigned long long *hmax,
+__global__ void k_drift_active(TimeParams T, ParticleView S, long long time0, const unsigned long long *tl,
+                               unsigned long long *disp, unsigned long long *hmax,
                                double hclamp, uint8_t *ACT, unsigned long long *bits, int *bcount, unsigned long long *hmax_act);
 __global__ void k_active_compact(long n, int nblocks, const unsigned long long *bits, const int *bcount, int *act, int *count_out);
 __global__ void k_advance_boxes(BoxTable bt);

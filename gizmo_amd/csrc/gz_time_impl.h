@@ -315,9 +315,10 @@ __device__ __forceinline__ double drift_one(const TimeParams &T, const ParticleV
                                             const double *__restrict__ Z0, double *h_after = nullptr)
 {
     const double dt_drift = (time1 - time0) * T.tbi * 1.0;
-    double x = S.X[i] + S.VKX[i] * dt_drift;
-    double y = S.Y[i] + S.VKY[i] * dt_drift;
-    double z = S.Z[i] + S.VKZ[i] * dt_drift;
+    const double vkx = S.VKX[i], vky = S.VKY[i], vkz = S.VKZ[i];
+    double x = S.X[i] + vkx * dt_drift;
+    double y = S.Y[i] + vky * dt_drift;
+    double z = S.Z[i] + vkz * dt_drift;
     const double divv_fac = drift_log_change(S.DIVV[i] * dt_drift);
     const double h_drift_fac = exp(drift_log_change(S.DIVV[i] * dt_drift) / ((double)3));
     if (S.M[i] > 0) {
@@ -345,6 +346,11 @@ __device__ __forceinline__ double drift_one(const TimeParams &T, const ParticleV
     if (y >= T.box) y -= T.box; else if (y < 0) y += T.box;
     if (z >= T.box) z -= T.box; else if (z < 0) z += T.box;
     S.X[i] = x; S.Y[i] = y; S.Z[i] = z;
+    if (!X0) {
+        // no ordering positions: this drift's length along the longest axis, padded for the
+        // rounding of the position update (the caller sums these into a bound)
+        return fmax(fabs(vkx), fmax(fabs(vky), fabs(vkz))) * dt_drift * (1.0 + 1e-12) + 1e-13 * T.box;
+    }
     const double half = 0.5 * T.box;
     const double ex = fabs(wrap(x - X0[i], T.box, half)), ey = fabs(wrap(y - Y0[i], T.box, half)),
                  ez = fabs(wrap(z - Z0[i], T.box, half));
@@ -486,10 +492,11 @@ __global__ void __launch_bounds__(256) k_active_flags(long n, const unsigned lon
 // each wavefront's 64 row flags as a ballot and the block's row count; k_active_compact then
 // writes each block's rows at its offset (the counts of the blocks before it) in ascending
 // order.  No scan over every particle, and the flags read the drifted radius from registers
-// instead of a second sweep.
+// instead of a second sweep.  disp receives the longest drift of the step along an axis (the
+// caller sums these into its bound on the displacement since the last ordering: no ordering
+// positions are read).
 __global__ void __launch_bounds__(256) k_drift_active(TimeParams T, ParticleView S, long long time0, const unsigned long long *__restrict__ tl,
-                                                      const double *__restrict__ X0, const double *__restrict__ Y0,
-                                                      const double *__restrict__ Z0, unsigned long long *__restrict__ disp,
+                                                      unsigned long long *__restrict__ disp,
                                                       unsigned long long *__restrict__ hmax, double hclamp, uint8_t *__restrict__ ACT,
                                                       unsigned long long *__restrict__ bits, int *__restrict__ bcount,
                                                       unsigned long long *__restrict__ hmax_act)
@@ -509,7 +516,7 @@ __global__ void __launch_bounds__(256) k_drift_active(TimeParams T, ParticleView
             const int tb = S.TB[i];
             const double m = S.M[i];
             double d = 0, hh;
-            if (time1 != time0) d = drift_one(T, S, i, time0, time1, X0, Y0, Z0, &hh);
+            if (time1 != time0) d = drift_one(T, S, i, time0, time1, nullptr, nullptr, nullptr, &hh);
             else hh = S.H[i];
             d1 = fmax(d1, d);
             h1 = fmax(h1, hh);
