@@ -54,7 +54,7 @@ FAMILIES = ("timeline", "drift", "sort", "ngb_build", "density", "gradients", "f
 DRIFT_BYTES = {"fp64": 229, "mixed": 229}
 # HBM bytes per particle that k_drift_active moves, measured: FETCH_SIZE + WRITE_SIZE per dispatch at 2,097,152
 # particles (FETCH_SIZE scaled x2 by a 16 MiB device copy), profiles/sedov128_timestep_job_traffic.md
-DRIFT_TRAFFIC_PER_PARTICLE = 530.7e6 / 2097152
+DRIFT_TRAFFIC_PER_PARTICLE = 480.4e6 / 2097152
 
 
 def log(msg):

@@ -1015,6 +1015,21 @@ template <int K, typename R> struct ForceF { static int run(gz_context *c) { ret
 // re-ordering: particles that moved are put back in Morton order (keys of the current
 // positions), every per-particle field follows, and the displacement bound restarts at zero
 // ---------------------------------------------------------------------------------------
+// the spare buffers a re-ordering of the columns below takes (16 of width 1, 2 of width 3, one of
+// widths 6 and 9), allocated ahead so that the timestep loop's first re-ordering does not wait
+// for allocations (a re-ordering still allocates any that are missing)
+int reserve_spares(gz_context *ctx)
+{
+    const int need[10] = {0, 16, 0, 2, 0, 0, 1, 0, 0, 1};
+    for (int w = 1; w < 10; w++)
+        while ((int)ctx->spare[w].size() < need[w]) {
+            double *p = nullptr;
+            CHK(dalloc(ctx, &p, (size_t)ctx->cap * w));
+            ctx->spare[w].push_back(p);
+        }
+    return 0;
+}
+
 int resort(gz_context *ctx)
 {
     const long n = ctx->n;
@@ -1597,6 +1612,7 @@ int gz_run_begin(gz_context *ctx, const gz_integ_params *ip)
         ip->min_egy_spec < 0)
         return fail(ctx, GZ_E_ARG, "courant_fac, err_tol_int_accuracy, max_size_timestep must be > 0; min_size_timestep, min_egy_spec >= 0");
     CHK(set_device(ctx));
+    CHK(reserve_spares(ctx));
     const long n = ctx->n;
     ctx->ip = *ip;
     ctx->tbi = (ip->time_max - ip->time_begin) / (double)kTimebaseHost;  // src/core/init.cc:113
