@@ -88,17 +88,13 @@ def run_job(ic, precision, steps, warmup):
     every launch family for the per-kernel times and the roofline."""
     if warmup > 0:
         g = new_job(ic, precision)
-        for _ in range(warmup):
-            g.run_step()
+        g.run_steps(warmup)
         g.close()
     g = new_job(ic, precision)
     g.synchronize()
     g.timing_reset()
-    active, woken = [], []
     t0 = time.perf_counter()
-    for _ in range(steps):
-        active.append(g.run_step())
-        woken.append(g.timeline()["n_woken"])
+    active, woken = g.run_steps(steps)  # run()'s loop over the sync points, in the library
     g.synchronize()
     dt = time.perf_counter() - t0
     res = dict(seconds=dt, active=active, woken=woken, resorts=g.stat("resorts"), updates=g.stat("active_updates"),
@@ -108,7 +104,7 @@ def run_job(ic, precision, steps, warmup):
     g.synchronize()
     g.timing_enable(True)
     g.timing_reset()
-    active2 = [g.run_step() for _ in range(steps)]
+    active2, _ = g.run_steps(steps)
     g.synchronize()
     assert active2 == active, "the profiled job took other steps than the timed one"
     res["kernels"] = {k: g.timing(k) for k in FAMILIES}
@@ -257,8 +253,7 @@ def ensemble_bench(steps=100, n_boxes=256, seed=1004):
         g.synchronize()
         u0 = g.stat("active_updates")
         t0 = time.perf_counter()
-        for _ in range(steps):
-            g.run_step()
+        g.run_steps(steps)
         g.synchronize()
         t_l += time.perf_counter() - t0
         upd_l += g.stat("active_updates") - u0

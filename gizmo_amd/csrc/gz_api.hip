@@ -1655,6 +1655,23 @@ int gz_run_step(gz_context *ctx)
     return rc;
 }
 
+int gz_run_steps(gz_context *ctx, int nsteps, long long *n_active, long long *n_woken)
+{
+    if (!ctx || nsteps < 0) return GZ_E_ARG;
+    if (!ctx->run_started) return fail(ctx, GZ_E_STATE, "gz_run_steps before gz_run_begin");
+    CHK(set_device(ctx));
+    int rc = 0;
+    for (int k = 0; k < nsteps && !rc; k++) {
+        rc = run_step(ctx);
+        if (!rc) {
+            if (n_active) n_active[k] = ctx->nact;
+            if (n_woken) n_woken[k] = ctx->n_woken;
+        }
+    }
+    collect_timings(ctx);
+    return rc;
+}
+
 int gz_set_option(gz_context *ctx, const char *name, double value)
 {
     if (!ctx || !name) return GZ_E_ARG;

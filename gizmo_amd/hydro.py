@@ -92,6 +92,7 @@ def lib():
         L.gz_neighbors.restype = C.c_long
         L.gz_run_begin.argtypes = [C.c_void_p, C.POINTER(IntegParams)]
         L.gz_run_step.argtypes = [C.c_void_p]
+        L.gz_run_steps.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]
         L.gz_timeline_get.argtypes = [C.c_void_p, C.POINTER(Timeline)]
         L.gz_set_option.argtypes = [C.c_void_p, C.c_char_p, C.c_double]
         _lib = L
@@ -256,6 +257,15 @@ class Hydro:
         """One sync point of the run loop; returns the number of active particles updated."""
         self._check(lib().gz_run_step(self.ctx))
         return self.timeline()["n_active"]
+
+    def run_steps(self, nsteps):
+        """nsteps sync points in one call (no return to Python between them); returns the active
+        and woken counts of each step."""
+        act = np.zeros(max(int(nsteps), 1), dtype=np.int64)
+        woken = np.zeros_like(act)
+        self._check(lib().gz_run_steps(self.ctx, int(nsteps), act.ctypes.data_as(C.POINTER(C.c_longlong)),
+                                       woken.ctypes.data_as(C.POINTER(C.c_longlong))))
+        return act[:nsteps].tolist(), woken[:nsteps].tolist()
 
     def set_option(self, name, value):
         self._check(lib().gz_set_option(self.ctx, name.encode(), float(value)))

@@ -175,6 +175,12 @@ int gz_run_begin(gz_context *ctx, const gz_integ_params *ip);
    make_list_of_active_particles() (:997), compute_hydro_densities_and_forces()
    (src/core/accel.cc:68) and do_second_halfstep_kick() (src/core/kicks.cc:99). */
 int gz_run_step(gz_context *ctx);
+/* nsteps iterations of that loop in one call, as run()'s while loop takes them (src/core/run.cc:220-507,
+   one sync point per pass), with no return to the caller between them; n_active[k] and n_woken[k]
+   (either may be NULL) receive step k's active-particle count and woken count.  Stops at the
+   first step that fails and returns its code; the steps before it stand (gz_timeline_get tells
+   how many were taken). */
+int gz_run_steps(gz_context *ctx, int nsteps, long long *n_active, long long *n_woken);
 int gz_timeline_get(gz_context *ctx, gz_timeline *out);
 /* Tuning knobs with no counterpart in the reference: "resort_fraction" (default 0.125) -- the
    particles are put back in space-filling-curve order once any of them has moved farther than

@@ -99,3 +99,34 @@ def test_adaptive_bins_sedov(oracle_lib, resort_fraction):
     assert min(active) < st.n // 10, "bins never split"
     assert sum(w for _, w in hist) > 0, "no wake-up happened"
     check_fields(st, out, par)
+
+
+def test_run_steps_same_as_single_steps():
+    """gz_run_steps(k) takes the same k sync points as k calls of gz_run_step: identical active
+    and woken counts per step, timeline and state, bit for bit (the same launches in the same
+    order)."""
+    ic = ics.sedov_box(20, 1002, kernel=6, des_num_ngb=64.0)
+    integ = dict(courant_fac=0.05, err_tol_int_accuracy=0.002, max_size_timestep=0.01, min_size_timestep=0.0,
+                 time_begin=0.0, time_max=1.0, min_egy_spec=0.0)
+    par = dict(ic["params"])
+    outs, counts = [], []
+    for multi in (False, True):
+        print(f"run_steps: multi={multi}", flush=True)
+        g = Hydro(par, len(ic["mass"]), GZ_FP64)
+        g.load(dict(pos=ic["pos"], vel=ic["vel"], mass=ic["mass"], u=ic["u"], h=ic["h"]))
+        g.run_begin(integ)
+        if multi:
+            act, woken = g.run_steps(25)
+        else:
+            act, woken = [], []
+            for _ in range(25):
+                act.append(g.run_step())
+                woken.append(g.timeline()["n_woken"])
+        tl = g.timeline()
+        counts.append((act, woken, tl["ti_current"], tl["num_steps"], tl["bin_count"].tolist()))
+        outs.append(g.store(["pos", "timebin", "ti_begstep", "dt_step"] + list(FLOAT_FIELDS)))
+        g.close()
+    assert counts[0] == counts[1]
+    assert sum(counts[0][1]) > 0, "no wake-up happened"
+    for k in outs[0]:
+        assert np.array_equal(outs[0][k], outs[1][k]), k
