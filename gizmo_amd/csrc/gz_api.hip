@@ -56,6 +56,7 @@ struct gz_context {
     // dmax: a bound on how far any particle moved since the last ordering (the sum of each
     // step's longest drift along an axis)
     double dmax = 0, last_rmax = 0;
+    bool overflow_clear = false;  // counters[3] (the symmetric list's overflow) known to be zero on the stream
     double resort_fraction = 0.125;  // order again once dmax exceeds this fraction of the search radius
     int few_rows = 8192;             // at most this many active rows: one wavefront per row
     // the cell ranges and stencils depend only on the order (key_version) and the level: they
@@ -674,6 +675,7 @@ int build_list(gz_context *ctx, int nrows, const int *rows, const double *hrow, 
     }
     for (int attempt = 0; attempt < 8; attempt++) {
         HIPCHK(hipMemsetAsync(ctx->counters + 3, 0, sizeof(int), ctx->stream));
+        ctx->overflow_clear = false;
         {
             Timed t(ctx, "ngb_build");
             k_build_ell<SYM><<<xcd_grid(nblk(nrows)), kBlock, 0, ctx->stream>>>(nrows, rows, ctx->act, hrow, scale, ctx->X, ctx->Y, ctx->Z, ctx->H,
@@ -743,6 +745,7 @@ int density_stage(gz_context *ctx)
                                                          ctx->it_r, ctx->it_cn, ctx->it_ci, ctx->it_it, ctx->need, ctx->counters,
                                                          hmax > 0 ? nullptr : ctx->u64c + 67, ctx->u64c + 68, BoxTable{});
     HIPCHK(hipGetLastError());
+    ctx->overflow_clear = true;  // counters[0..7] cleared, the list overflow among them
     if (!(hmax > 0)) {
         HIPCHK(hipMemcpyAsync(ctx->hu64c + 67, ctx->u64c + 67, sizeof(unsigned long long), hipMemcpyDeviceToHost, ctx->stream));
         HIPCHK(hipStreamSynchronize(ctx->stream));
@@ -758,7 +761,8 @@ int density_stage(gz_context *ctx)
     int iters = 0;
     int remaining = na;  // rows of the coming round
     while (true) {
-        HIPCHK(hipMemsetAsync(ctx->counters, 0, 3 * sizeof(int), ctx->stream));
+        // the round's counters (k_density_init cleared them for the first round)
+        if (iters > 0) HIPCHK(hipMemsetAsync(ctx->counters, 0, 3 * sizeof(int), ctx->stream));
         {
             // over many levels the rounds take one workgroup per row whatever their number (the
             // cells of the rows' own levels hold too few rows to fill a wavefront): four
@@ -863,7 +867,8 @@ int ensure_sym(gz_context *ctx)
         CHK(dalloc(ctx, &ctx->ell_s, (size_t)ctx->cap_s * (size_t)ctx->ld_s));
     }
     for (int attempt = 0; attempt < 8; attempt++) {
-        HIPCHK(hipMemsetAsync(ctx->counters + 3, 0, sizeof(int), ctx->stream));
+        if (attempt > 0 || !ctx->overflow_clear) HIPCHK(hipMemsetAsync(ctx->counters + 3, 0, sizeof(int), ctx->stream));
+        ctx->overflow_clear = false;
         {
             Timed t(ctx, "ngb_build");
             if (wide && !few)
@@ -1744,6 +1749,7 @@ long gz_neighbors(gz_context *ctx, int symmetric, long *offsets, int *nbr, long 
         for (int attempt = 0; !rc && attempt < 12; attempt++) {
             rc = dalloc(ctx, &ell, (size_t)capl * ld);
             if (rc) break;
+            ctx->overflow_clear = false;
             if (hipMemsetAsync(ctx->counters + 3, 0, sizeof(int), ctx->stream) != hipSuccess) { rc = GZ_E_HIP; break; }
             if (ctx->nact > ctx->few_rows)
                 k_build_sym_cells<<<xcd_grid(ctx->grp.ngroups), 64, 0, ctx->stream>>>(grid_view(ctx), ctx->grp.view(BoxTable{}), ctx->act,
