@@ -329,6 +329,7 @@ int alloc_state(gz_context *ctx, long cap)
     // the 64-bit counters and maxima, then the 16 int counters, in one allocation: one copy
     // reads a stage's results back
     CHK(dalloc(ctx, &ctx->u64c, 80 + 8));
+    HIPCHK(hipMemset(ctx->u64c, 0, (80 + 8) * sizeof(unsigned long long)));  // the running pair count starts at zero
     ctx->counters = (int *)(ctx->u64c + 80);
     if (!ctx->hu64c) {
         hipError_t e = hipHostMalloc((void **)&ctx->hu64c, (80 + 8) * sizeof(unsigned long long), hipHostMallocDefault);
@@ -817,6 +818,8 @@ int density_stage(gz_context *ctx)
     return 0;
 }
 
+constexpr int kRowPairAtomics = 1024;  // at most this many rows count their pairs with one atomic each
+
 // the order in which the lanes of the force pass take the rows: longest symmetric list first
 // within each block of sched_span rows (the gradient pass, cheaper per pair, keeps the plain
 // order: its scattered writes would cost more than the even waves save)
@@ -963,13 +966,16 @@ int force_stage(gz_context *ctx)
             k_force_rows<KERN, R><<<xcd_grid(ctx->nact), kRowThreads, 0, ctx->stream>>>(
                 ctx->dp, gam<R>(ctx->dp), ctx->nact, ctx->act, F, rec, ctx->ell_s, ctx->ld_s, ctx->cnt_s,
                 ctx->all_active ? nullptr : ctx->ACT, ctx->TB, ctx->MSV, ctx->WAKE, ctx->ACC, ctx->DTU, ctx->MSV, ctx->counters + 2,
-                ctx->counters + 4, nullptr, BoxTable{});
+                ctx->counters + 4, ctx->nact <= kRowPairAtomics ? ctx->u64c + 65 : nullptr, BoxTable{});
         else
             k_force<KERN, R, false><<<xcd_grid(nblk(ctx->nact)), kBlock, 0, ctx->stream>>>(
                 ctx->dp, gam<R>(ctx->dp), ctx->nact, ctx->act, F, rec, ctx->ell_s, ctx->ld_s, ctx->cnt_s,
                 ctx->all_active ? nullptr : ctx->ACT, ctx->TB, ctx->MSV, ctx->WAKE, ctx->ACC, ctx->DTU, ctx->MSV, ctx->counters + 2,
                 ctx->counters + 4, nullptr, ctx->sched_valid ? ctx->sched : nullptr, BoxTable{});
-        k_sum_counts<<<std::min(nblk(ctx->nact), 256u), kBlock, 0, ctx->stream>>>(ctx->nact, ctx->cnt_s, ctx->u64c + 65);
+        // the pairs of the pass: each row's workgroup adds its own while a few hundred rows are
+        // active (atomics on one address execute one after another)
+        if (ctx->nact > ctx->few_rows || ctx->nact > kRowPairAtomics)
+            k_sum_counts<<<std::min(nblk(ctx->nact), 256u), kBlock, 0, ctx->stream>>>(ctx->nact, ctx->cnt_s, ctx->u64c + 65);
         HIPCHK(hipGetLastError());
     }
     CHK(read_counters(ctx));

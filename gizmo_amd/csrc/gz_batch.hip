@@ -235,6 +235,8 @@ int alloc_all(gz_batch *b)
     BCHK(balloc(b, &b->tab, nbx)); BCHK(balloc(b, &b->hist, 64 * nbx)); BCHK(balloc(b, &b->slot, kBoxSlots * nbx));
     BCHK(balloc(b, &b->sel, nbx)); BCHK(balloc(b, &b->due, nbx)); BCHK(balloc(b, &b->rank, nbx)); BCHK(balloc(b, &b->lim, 2 * nbx));
     BCHK(balloc(b, &b->counters, 16)); BCHK(balloc(b, &b->u64, 8));
+    BHIP(hipMemset(b->counters, 0, 16 * sizeof(int)));
+    BHIP(hipMemset(b->u64, 0, 8 * sizeof(unsigned long long)));  // the running pair count starts at zero
     BHIP(hipHostMalloc((void **)&b->hhist, 64 * nbx * sizeof(unsigned long long), hipHostMallocDefault));
     BHIP(hipHostMalloc((void **)&b->hslot, kBoxSlots * nbx * sizeof(unsigned long long), hipHostMallocDefault));
     BHIP(hipHostMalloc((void **)&b->hu64, 8 * sizeof(unsigned long long), hipHostMallocDefault));

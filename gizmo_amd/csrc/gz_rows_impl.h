@@ -821,10 +821,7 @@ __global__ void __launch_bounds__(kRowThreads) k_force_rows(DevParams P, Gam<R> 
     const double sound_i = (rhoi > 0) ? sqrt(P.gamma * F.PR[i] / rhoi) : sqrt(P.gamma * (P.gamma - 1.) * F.U[i]);
     const bool enabled = (mi > 0 && rhoi > 0);
     const int c = cnt[r];
-    if (lane == 0) {
-        if (c && pairs) atomicAdd(pairs, (unsigned long long)c);
-        if (wakeup) wakeup[i] = 0;  // hydro_force_initial_operations_preloop
-    }
+    if (lane == 0 && wakeup) wakeup[i] = 0;  // hydro_force_initial_operations_preloop
     if (enabled) {
         const Cell<R> I = rec[i];
         const double xi = F.X[i], yi = F.Y[i], zi = F.Z[i];
@@ -882,7 +879,11 @@ __global__ void __launch_bounds__(kRowThreads) k_force_rows(DevParams P, Gam<R> 
         __syncthreads();
         A0 = res[0]; A1 = res[1]; A2 = res[2]; dE = res[3]; msv = res[4];
     }
-    if (lane == 0) force_tail(P, F, i, enabled, sound_i, A0, A1, A2, dE, msv, ACC, DTU, MSV);
+    if (lane == 0) {
+        force_tail(P, F, i, enabled, sound_i, A0, A1, A2, dE, msv, ACC, DTU, MSV);
+        // the pairs of the row last: no load of the row waits behind the atomic
+        if (c && pairs) atomicAdd(pairs, (unsigned long long)c);
+    }
 }
 
 }  // namespace gz

@@ -238,3 +238,21 @@ def test_fp32_mixed_same_bits_both_shapes(cases, name):
         g.close()
     for k in DENSITY_FIELDS + GRAD_FIELDS + FORCE_FIELDS:
         assert np.array_equal(outs["lanes"][k], outs["waves"][k]), k
+
+
+# the force pass's pair count: each row's workgroup adds its own (at most 1024 rows), or a sum
+# over the list lengths after the pass (more rows, or one lane per row)
+@pytest.mark.parametrize("n_side,few_rows", [(10, 10**9), (10, 0), (11, 10**9)])
+def test_force_pair_count(n_side, few_rows):
+    ic = ics.lattice_box(n_side, 1001, kernel=3, des_num_ngb=32.0)
+    print(f"pair count: n={len(ic['mass'])} few_rows={few_rows}", flush=True)
+    g = Hydro(dict(ic["params"]), len(ic["mass"]), GZ_FP64)
+    g.set_option("few_rows", few_rows)
+    g.load(dict(pos=ic["pos"], vel=ic["vel"], mass=ic["mass"], u=ic["u"], h=ic["h"]))
+    p0 = g.stat("force_pairs_total")
+    g.hydro_pass()
+    pairs = g.stat("force_pairs_total") - p0
+    expect = g.stat("pairs_symmetric")
+    g.close()
+    assert p0 == 0  # a new context's running count
+    assert expect > 0 and pairs == expect
