@@ -1222,16 +1222,16 @@ int run_step(gz_context *ctx)
     {
         Timed t(ctx, "timeline");
         k_step_prep<<<1, 64, 0, ctx->stream>>>(ctx->counters, ctx->u64c);
+        // the rows' new steps with their first half-kicks (one launch)
         if (ctx->nact > 0)
             k_find_timesteps<<<nblk(ctx->nact), kBlock, 0, ctx->stream>>>(T, S, ctx->nact, ctx->act, ctx->counters + 6, ctx->u64c,
-                                                                          BoxTable{});
+                                                                          BoxTable{}, 1);
         if (ctx->need_wakeup) {
             int wakeup_bin_offset = 0;
             while ((1LL << wakeup_bin_offset) < (long long)kWakeup) wakeup_bin_offset++;
             k_wakeup_limits<<<1, 64, 0, ctx->stream>>>(T, ctx->u64c, ctx->counters + 8, BoxTable{});
             k_wakeups<<<nblk(n), kBlock, 0, ctx->stream>>>(T, S, ctx->counters + 8, wakeup_bin_offset, ctx->u64c, ctx->counters + 5, BoxTable{});
         }
-        if (ctx->nact > 0) k_halfstep_kick<<<nblk(ctx->nact), kBlock, 0, ctx->stream>>>(T, S, ctx->nact, ctx->act, 0, BoxTable{});
         k_next_sync<<<1, 64, 0, ctx->stream>>>(T, ctx->u64c, ctx->u64c + 70, BoxTable{});
     }
     // the drift of every particle to the next sync point fused with the row flags and counts

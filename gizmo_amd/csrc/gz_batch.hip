@@ -952,7 +952,7 @@ int gz_batch_step(gz_batch *b)
     BHIP(hipMemsetAsync(b->hist, 0, 64 * nbx * 8, b->stream));
     BHIP(hipMemsetAsync(b->slot, 0, kBoxSlots * nbx * 8, b->stream));
     BHIP(hipMemsetAsync(b->counters, 0, 16 * sizeof(int), b->stream));
-    if (b->nact > 0) k_find_timesteps<<<nb(b->nact), kB, 0, b->stream>>>(T, S, b->nact, b->act, b->counters + 6, nullptr, bt);
+    if (b->nact > 0) k_find_timesteps<<<nb(b->nact), kB, 0, b->stream>>>(T, S, b->nact, b->act, b->counters + 6, nullptr, bt, 0);
     k_bin_hist<<<std::min(nb(n), 1024u), kB, 0, b->stream>>>(n, b->TB, b->hist, bt);
     int wakeup_bin_offset = 0;
     while ((1LL << wakeup_bin_offset) < (long long)kWakeup) wakeup_bin_offset++;

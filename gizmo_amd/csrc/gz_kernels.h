@@ -147,7 +147,7 @@ struct ParticleView {         // device order; vectors SoA with column stride n
 };
 
 __global__ void k_find_timesteps(TimeParams T, ParticleView S, int nrows, const int *act, int *err, unsigned long long *count,
-                                 BoxTable bt);
+                                 BoxTable bt, int kick_first);
 __global__ void k_bin_hist(long n, const int *TB, unsigned long long *count, BoxTable bt);
 __global__ void k_wakeup_limits(TimeParams T, const unsigned long long *count, int *lim, BoxTable bt);
 __global__ void k_wakeups(TimeParams T, ParticleView S, const int *lim, int wakeup_bin_offset, unsigned long long *count,

@@ -111,7 +111,8 @@ __device__ void kick(const TimeParams &T, const ParticleView &S, long i, long lo
 // find_timesteps, src/core/timestep.cc:39-355: the new power-of-two step of each particle of
 // the active list; a larger bin only where it is synchronised now
 __global__ void __launch_bounds__(256) k_find_timesteps(TimeParams T, ParticleView S, int nrows, const int *__restrict__ act,
-                                                         int *__restrict__ err, unsigned long long *__restrict__ count, BoxTable bt)
+                                                         int *__restrict__ err, unsigned long long *__restrict__ count, BoxTable bt,
+                                                         int kick_first)
 {
     // count (one box, may be null): TimeBinCount kept up to date with the moves of the rows,
     // added up per block in LDS
@@ -145,6 +146,10 @@ __global__ void __launch_bounds__(256) k_find_timesteps(TimeParams T, ParticleVi
         S.TB[i] = bin;
         S.TIB[i] += S.DTS[i];
         S.DTS[i] = ti_step;
+        // do_first_halfstep_kick (src/core/kicks.cc:34-97) of the row, as k_halfstep_kick takes
+        // it: it reads only the row's own fields, which process_wake_ups (inactive particles
+        // only) does not touch
+        if (kick_first && S.M[i] > 0) kick(T, S, i, S.TIB[i], S.TIB[i] + S.DTS[i] / 2, 0);
         if (count && bin != binold) {
             atomicAdd(&moved[binold & 63], -1);
             atomicAdd(&moved[bin & 63], 1);
@@ -560,6 +565,8 @@ __global__ void __launch_bounds__(256) k_active_compact(long n, int nblocks, con
     __shared__ int spart[4];
     __shared__ int sprefix[kActChunk / 64 + 1];
     const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+    // a block without rows has nothing to place (the last block still writes the count)
+    if (bcount[blockIdx.x] == 0 && (int)blockIdx.x != nblocks - 1) return;
     // the rows of the blocks before this one
     int before = 0;
     const int nprev = (int)blockIdx.x;
