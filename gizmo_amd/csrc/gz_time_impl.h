@@ -313,8 +313,10 @@ __global__ void __launch_bounds__(256) k_halfstep_kick(TimeParams T, ParticleVie
 
 // drift_particle_impl, src/core/drift_particle_functions.h:86-250 (MFM gas; the kernel radius
 // moves with the flow, src/declarations/precompiler_logic.h:78) of particle i from time0 to
-// time1, then one periodic wrap per axis.  Returns the particle's periodic distance from where
-// it was when the particles were last ordered.
+// time1, then one periodic wrap per axis.  Returns the particle's periodic distance (largest
+// axis) from where it was when the particles were last ordered (X0, Y0, Z0), or without those
+// positions this drift's length along its longest axis; h_after (if given) receives the radius
+// after the drift.
 __device__ __forceinline__ double drift_one(const TimeParams &T, const ParticleView &S, long i, long long time0, long long time1,
                                             const double *__restrict__ X0, const double *__restrict__ Y0,
                                             const double *__restrict__ Z0, double *h_after = nullptr)

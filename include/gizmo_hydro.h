@@ -183,9 +183,10 @@ int gz_run_step(gz_context *ctx);
 int gz_run_steps(gz_context *ctx, int nsteps, long long *n_active, long long *n_woken);
 int gz_timeline_get(gz_context *ctx, gz_timeline *out);
 /* Tuning knobs with no counterpart in the reference: "resort_fraction" (default 0.125) -- the
-   particles are put back in space-filling-curve order once any of them has moved farther than
-   this fraction of the search radius since they were last ordered (the search grid widens its
-   cells by twice that distance meanwhile); "few_rows" (default 8192) -- passes over at most this
+   particles are put back in space-filling-curve order once a bound on how far any of them has
+   moved since they were last ordered (the sum of each step's longest drift along an axis)
+   exceeds this fraction of the search radius (the search grid widens its cells by twice that
+   bound meanwhile); "few_rows" (default 8192) -- passes over at most this
    many active particles run one workgroup per particle (its lanes share the particle's
    candidates or list entries) instead of one lane per particle; "wide_search" (default 0) --
    1: every particle searches at the grid level of its own kernel radius (cell tables of every
