@@ -924,17 +924,22 @@ int gradients_stage(gz_context *ctx)
     CHK(ensure_sym(ctx));
     Timed t(ctx, "gradients");
     GradRec<R> *rec = (GradRec<R> *)ctx->rec_grad;
-    if (ctx->pack_list)
-        k_pack_grad_list<R><<<xcd_grid(ctx->nact), 64, 0, ctx->stream>>>(ctx->nact, ctx->act, ctx->ell_s, ctx->ld_s, ctx->cnt_s, ctx->H,
-                                                                         ctx->M, ctx->RHO, ctx->PR, ctx->VX, ctx->VY, ctx->VZ, rec, ctx->X,
-                                                                         ctx->Y, ctx->Z, ctx->par.box_size, BoxTable{});
-    else
-        k_pack_grad<R><<<nblk(ctx->n), kBlock, 0, ctx->stream>>>(ctx->n, ctx->H, ctx->M, ctx->RHO, ctx->PR, ctx->VX, ctx->VY, ctx->VZ, rec,
-                                                                 ctx->X, ctx->Y, ctx->Z, ctx->par.box_size, BoxTable{});
+    // few rows among many particles: the row kernel makes the records it reads (no packing launch)
+    const bool make = ctx->nact <= ctx->few_rows && ctx->pack_list;
+    if (!make) {
+        if (ctx->pack_list)
+            k_pack_grad_list<R><<<xcd_grid(ctx->nact), 64, 0, ctx->stream>>>(ctx->nact, ctx->act, ctx->ell_s, ctx->ld_s, ctx->cnt_s, ctx->H,
+                                                                             ctx->M, ctx->RHO, ctx->PR, ctx->VX, ctx->VY, ctx->VZ, rec,
+                                                                             ctx->X, ctx->Y, ctx->Z, ctx->par.box_size, BoxTable{});
+        else
+            k_pack_grad<R><<<nblk(ctx->n), kBlock, 0, ctx->stream>>>(ctx->n, ctx->H, ctx->M, ctx->RHO, ctx->PR, ctx->VX, ctx->VY, ctx->VZ,
+                                                                     rec, ctx->X, ctx->Y, ctx->Z, ctx->par.box_size, BoxTable{});
+    }
     if (ctx->nact <= ctx->few_rows)
-        k_gradients_rows<KERN, R><<<xcd_grid(ctx->nact), kRowThreads, 0, ctx->stream>>>(ctx->dp, ctx->nact, ctx->act, ctx->X, ctx->Y, ctx->Z,
-                                                                               rec, ctx->T, ctx->CN, ctx->n, ctx->ell_s, ctx->ld_s,
-                                                                               ctx->cnt_s, ctx->GR, ctx->GP, ctx->GV, BoxTable{});
+        k_gradients_rows<KERN, R><<<xcd_grid(ctx->nact), kRowThreads, 0, ctx->stream>>>(
+            ctx->dp, ctx->nact, ctx->act, ctx->X, ctx->Y, ctx->Z, rec,
+            make ? GradSrc{ctx->H, ctx->M, ctx->RHO, ctx->PR, ctx->VX, ctx->VY, ctx->VZ, ctx->par.box_size} : GradSrc{}, ctx->T, ctx->CN,
+            ctx->n, ctx->ell_s, ctx->ld_s, ctx->cnt_s, ctx->GR, ctx->GP, ctx->GV, BoxTable{});
     else
         k_gradients<KERN, R, false><<<xcd_grid(nblk(ctx->nact)), kBlock, 0, ctx->stream>>>(ctx->dp, ctx->nact, ctx->act, ctx->X, ctx->Y, ctx->Z,
                                                                                     rec, ctx->T, ctx->CN, ctx->n, ctx->ell_s, ctx->ld_s,

@@ -276,10 +276,15 @@ __global__ void k_density_rows(DevParams P, GridView g, int nrows, const int *ac
                                const double *VZ, const double *M, const double *hcur, double *accout, BoxTable bt);
 __global__ void k_build_sym_rows(GridView g, int nrows, const int *act, const uint32_t *key, const double *X, const double *Y,
                                  const double *Z, const double *H, int *ell, long ld, int cap, int *cnt, int *overflow, BoxTable bt);
+// the fields a gradient record is made of (H null: the pass reads packed records)
+struct GradSrc {
+    const double *H, *M, *RHO, *PR, *VX, *VY, *VZ;
+    double box;
+};
 template <int KERN, typename R>
 __global__ void k_gradients_rows(DevParams P, int nrows, const int *act, const double *X, const double *Y, const double *Z,
-                                 const GradRec<R> *rec, const double *T, const double *CN, long n, const int *ell, long ld,
-                                 const int *cnt, double *GR, double *GP, double *GV, BoxTable bt);
+                                 const GradRec<R> *rec, GradSrc src, const double *T, const double *CN, long n, const int *ell,
+                                 long ld, const int *cnt, double *GR, double *GP, double *GV, BoxTable bt);
 template <int KERN, typename R>
 __global__ void k_force_rows(DevParams P, Gam<R> G, int nrows, const int *act, ForceView F, const Cell<R> *rec, const int *ell,
                              long ld, const int *cnt, const uint8_t *active, const int *timebin, const double *msv_in, int *wakeup,

@@ -41,8 +41,9 @@ template __global__ void k_build_ell<true>(int, const int *, const int *, const 
     GZ_INST_DROWS(KERN, R, 64)                                                                                                   \
     GZ_INST_DROWS(KERN, R, kRowThreads)                                                                                          \
     template __global__ void k_gradients_rows<KERN, R>(DevParams, int, const int *, const double *, const double *,             \
-                                                       const double *, const GradRec<R> *, const double *, const double *, long, \
-                                                       const int *, long, const int *, double *, double *, double *, BoxTable); \
+                                                       const double *, const GradRec<R> *, GradSrc, const double *,             \
+                                                       const double *, long, const int *, long, const int *, double *, double *,\
+                                                       double *, BoxTable);                                                     \
     template __global__ void k_force_rows<KERN, R>(DevParams, Gam<R>, int, const int *, ForceView, const Cell<R> *,            \
                                                    const int *, long, const int *, const uint8_t *, const int *,                \
                                                    const double *, int *, double *, double *, double *, int *, int *,           \

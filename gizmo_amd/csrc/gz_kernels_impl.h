@@ -744,6 +744,19 @@ __global__ void __launch_bounds__(256) k_density_after(DevParams P, int nrows, c
 // ---------------------------------------------------------------------------------------
 // records: the neighbour-side fields of a pass, packed per particle (device order)
 // ---------------------------------------------------------------------------------------
+// the gradient record of particle j, from the state (the record a pass packs, or builds where it
+// reads it)
+template <typename R>
+__device__ __forceinline__ GradRec<R> make_grad_rec(long j, const GradSrc &s, const double *__restrict__ X, const double *__restrict__ Y,
+                                                    const double *__restrict__ Z, const BoxTable &bt)
+{
+    GradRec<R> g;
+    g.h = (R)s.H[j]; g.m = (R)s.M[j]; g.rho = s.RHO[j]; g.p = s.PR[j];
+    g.v0 = s.VX[j]; g.v1 = s.VY[j]; g.v2 = s.VZ[j];
+    if constexpr (std::is_same<R, float>::value) g.fp = fine_pos(X[j], Y[j], Z[j], bt.b ? bt.b[bt.BOX[j]].box : s.box);
+    return g;
+}
+
 // the gradient record of particle j
 template <typename R>
 __device__ __forceinline__ void pack_grad_one(long j, const double *__restrict__ H, const double *__restrict__ M,
@@ -753,11 +766,7 @@ __device__ __forceinline__ void pack_grad_one(long j, const double *__restrict__
                                               const double *__restrict__ X, const double *__restrict__ Y,
                                               const double *__restrict__ Z, double box, const BoxTable &bt)
 {
-    GradRec<R> g;
-    g.h = (R)H[j]; g.m = (R)M[j]; g.rho = RHO[j]; g.p = PR[j];
-    g.v0 = VX[j]; g.v1 = VY[j]; g.v2 = VZ[j];
-    if constexpr (std::is_same<R, float>::value) g.fp = fine_pos(X[j], Y[j], Z[j], bt.b ? bt.b[bt.BOX[j]].box : box);
-    rec[j] = g;
+    rec[j] = make_grad_rec<R>(j, GradSrc{H, M, RHO, PR, VX, VY, VZ, box}, X, Y, Z, bt);
 }
 
 // every record (passes over many rows)

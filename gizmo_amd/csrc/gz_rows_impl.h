@@ -694,7 +694,7 @@ __global__ void __launch_bounds__(64) k_group_stencil(GridView g, int ngroups, c
 template <int KERN, typename R>
 __global__ void __launch_bounds__(kRowThreads) k_gradients_rows(DevParams P, int nrows, const int *__restrict__ act, const double *__restrict__ X,
                                                        const double *__restrict__ Y, const double *__restrict__ Z,
-                                                       const GradRec<R> *__restrict__ rec, const double *__restrict__ T,
+                                                       const GradRec<R> *__restrict__ rec, GradSrc src, const double *__restrict__ T,
                                                        const double *__restrict__ CN, long n, const int *__restrict__ ell, long ld,
                                                        const int *__restrict__ cnt, double *__restrict__ GR, double *__restrict__ GP,
                                                        double *__restrict__ GV, BoxTable bt)
@@ -714,7 +714,9 @@ __global__ void __launch_bounds__(kRowThreads) k_gradients_rows(DevParams P, int
     R mx[5], mn[5];
     R maxd = 0;
     R acc = 0;  // the running quantity this lane keeps
-    const GradRec<R> I = rec[i];
+    // records packed by an earlier launch, or made here from the state (few rows among many
+    // particles: no packing launch)
+    const GradRec<R> I = src.H ? make_grad_rec<R>(i, src, X, Y, Z, bt) : rec[i];
     const bool enabled = (I.m > 0 && I.rho > 0);
     const R hi = I.h;
     if (enabled) {
@@ -728,7 +730,7 @@ __global__ void __launch_bounds__(kRowThreads) k_gradients_rows(DevParams P, int
             int f = 0;
             if (q < c) {
                 const int j = ell[(size_t)q * ld + r];
-                const GradRec<R> J = rec[j];
+                const GradRec<R> J = src.H ? make_grad_rec<R>(j, src, X, Y, Z, bt) : rec[j];
                 if ((J.m > 0) && (J.rho > 0) && (J.h > 0)) {
                     R dp[3];
                     record_dp<R>(P, I, J, xi, yi, zi, X, Y, Z, j, dp[0], dp[1], dp[2]);
