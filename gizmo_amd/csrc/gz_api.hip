@@ -57,6 +57,7 @@ struct gz_context {
     // step's longest drift along an axis)
     double dmax = 0, last_rmax = 0;
     bool overflow_clear = false;  // counters[3] (the symmetric list's overflow) known to be zero on the stream
+    bool density_ready = false;   // the active list's launch set density()'s iteration state for its rows
     double resort_fraction = 0.125;  // order again once dmax exceeds this fraction of the search radius
     int few_rows = 8192;             // at most this many active rows: one wavefront per row
     // the cell ranges and stencils depend only on the order (key_version) and the level: they
@@ -699,6 +700,7 @@ int build_list(gz_context *ctx, int nrows, const int *rows, const double *hrow, 
 int build_active(gz_context *ctx)
 {
     const long n = ctx->n;
+    ctx->density_ready = false;  // a new list: density() starts its rows itself
     k_flag_active<<<nblk(n), kBlock, 0, ctx->stream>>>(n, ctx->all_active ? nullptr : ctx->ACT, ctx->M, ctx->iflag);
     HIPCHK(hipGetLastError());
     size_t bytes = 0;
@@ -741,11 +743,15 @@ int density_stage(gz_context *ctx)
     // one launch: the radius clamp, the iteration state, the counters, and the largest radius
     double hmax = ctx->hmax_rows;  // known when the step's timeline pass found it
     ctx->hmax_rows = -1;
-    if (!(hmax > 0)) HIPCHK(hipMemsetAsync(ctx->u64c + 67, 0, sizeof(unsigned long long), ctx->stream));
-    k_density_init<<<nblk(na), kBlock, 0, ctx->stream>>>(na, ctx->act, ctx->H, 0.99 * ctx->par.max_kernel_radius, ctx->it_h, ctx->it_l,
-                                                         ctx->it_r, ctx->it_cn, ctx->it_ci, ctx->it_it, ctx->need, ctx->counters,
-                                                         hmax > 0 ? nullptr : ctx->u64c + 67, ctx->u64c + 68, BoxTable{});
-    HIPCHK(hipGetLastError());
+    const bool ready = ctx->density_ready && hmax > 0;  // the step's active-list launch started the iteration
+    ctx->density_ready = false;
+    if (!ready) {
+        if (!(hmax > 0)) HIPCHK(hipMemsetAsync(ctx->u64c + 67, 0, sizeof(unsigned long long), ctx->stream));
+        k_density_init<<<nblk(na), kBlock, 0, ctx->stream>>>(na, ctx->act, ctx->H, 0.99 * ctx->par.max_kernel_radius, ctx->it_h,
+                                                             ctx->it_l, ctx->it_r, ctx->it_cn, ctx->it_ci, ctx->it_it, ctx->need,
+                                                             ctx->counters, hmax > 0 ? nullptr : ctx->u64c + 67, ctx->u64c + 68, BoxTable{});
+        HIPCHK(hipGetLastError());
+    }
     ctx->overflow_clear = true;  // counters[0..7] cleared, the list overflow among them
     if (!(hmax > 0)) {
         HIPCHK(hipMemcpyAsync(ctx->hu64c + 67, ctx->u64c + 67, sizeof(unsigned long long), hipMemcpyDeviceToHost, ctx->stream));
@@ -1049,6 +1055,7 @@ int reserve_spares(gz_context *ctx)
 int resort(gz_context *ctx)
 {
     const long n = ctx->n;
+    ctx->density_ready = false;
     Timed t(ctx, "sort");
     k_morton_soa<<<nblk(n), kBlock, 0, ctx->stream>>>(n, ctx->X, ctx->Y, ctx->Z, ctx->par.box_size, ctx->key2, ctx->ival);
     HIPCHK(hipGetLastError());
@@ -1256,7 +1263,12 @@ int run_step(gz_context *ctx)
     }
     {
         Timed t(ctx, "timeline");
-        k_active_compact<<<nb_act, 256, 0, ctx->stream>>>(n, nb_act, ctx->act_bits, ctx->act_bcount, ctx->act, ctx->counters + 10);
+        // with the list, the start of the coming density() iteration for its rows (the counters
+        // it needs zero were cleared by k_step_prep)
+        const DensityStart ds{ctx->H, 0.99 * ctx->par.max_kernel_radius, ctx->it_h, ctx->it_l, ctx->it_r, ctx->it_cn, ctx->it_ci,
+                              ctx->it_it, ctx->need};
+        k_active_compact<<<nb_act, 256, 0, ctx->stream>>>(n, nb_act, ctx->act_bits, ctx->act_bcount, ctx->act, ctx->counters + 10, ds);
+        ctx->density_ready = true;
         HIPCHK(hipGetLastError());
     }
     ctx->need_wakeup = 0;
@@ -1292,7 +1304,9 @@ int run_step(gz_context *ctx)
     // change places in a re-ordering: the view is taken again)
     T = time_params(ctx);
     S = particle_view(ctx);
-    CHK(hydro_pass(ctx));
+    const int rc_pass = hydro_pass(ctx);
+    ctx->density_ready = false;
+    CHK(rc_pass);
     if (ctx->nact > 0) {
         Timed t(ctx, "timeline");
         k_halfstep_kick<<<nblk(ctx->nact), kBlock, 0, ctx->stream>>>(T, S, ctx->nact, ctx->act, 1, BoxTable{});
@@ -1359,6 +1373,7 @@ int gz_load(gz_context *ctx, long n, const gz_fields *f)
     if (!f->pos || !f->vel || !f->mass || !f->u || !f->h) return fail(ctx, GZ_E_ARG, "pos, vel, mass, u and h are required");
     CHK(set_device(ctx));
     ctx->n = n;
+    ctx->density_ready = false;
     // order along the Morton curve: keys from the caller's positions, stable radix sort
     HIPCHK(hipMemcpyAsync(ctx->stage, f->pos, (size_t)n * 3 * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
     {

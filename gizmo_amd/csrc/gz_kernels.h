@@ -164,7 +164,17 @@ constexpr int kActChunk = 512;  // particles per block of the active-list kernel
 __global__ void k_drift_active(TimeParams T, ParticleView S, long long time0, const unsigned long long *tl,
                                unsigned long long *disp, unsigned long long *hmax,
                                double hclamp, uint8_t *ACT, unsigned long long *bits, int *bcount, unsigned long long *hmax_act);
-__global__ void k_active_compact(long n, int nblocks, const unsigned long long *bits, const int *bcount, int *act, int *count_out);
+// the start of density()'s radius iteration for each row of a new active list (as k_density_init
+// does it; H null: not done here)
+struct DensityStart {
+    double *H;
+    double hclamp;
+    double *hcur, *left, *right, *cn_cur;
+    int *cn_init, *iter;
+    uint8_t *need;
+};
+__global__ void k_active_compact(long n, int nblocks, const unsigned long long *bits, const int *bcount, int *act, int *count_out,
+                                 DensityStart ds);
 __global__ void k_advance_boxes(BoxTable bt);
 __global__ void k_active_from_bins(long n, unsigned long long mask, const int *TB, const double *M, const double *H,
                                    uint8_t *ACT, double *SOFT);

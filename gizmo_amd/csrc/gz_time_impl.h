@@ -562,7 +562,8 @@ __global__ void __launch_bounds__(256) k_drift_active(TimeParams T, ParticleView
 }
 
 __global__ void __launch_bounds__(256) k_active_compact(long n, int nblocks, const unsigned long long *__restrict__ bits,
-                                                        const int *__restrict__ bcount, int *__restrict__ act, int *__restrict__ count_out)
+                                                        const int *__restrict__ bcount, int *__restrict__ act, int *__restrict__ count_out,
+                                                        DensityStart ds)
 {
     __shared__ int spart[4];
     __shared__ int sprefix[kActChunk / 64 + 1];
@@ -595,7 +596,18 @@ __global__ void __launch_bounds__(256) k_active_compact(long n, int nblocks, con
         const unsigned long long m = mb[q];
         if ((m >> lane) & 1ull) {
             const int below = __popcll(m & ((lane == 0) ? 0ull : (~0ull >> (64 - lane))));
-            act[off + sprefix[q] + below] = (int)(base + k * 256 + tid);
+            const int r = off + sprefix[q] + below;
+            const long i = base + k * 256 + tid;
+            act[r] = (int)i;
+            if (ds.H) {
+                // density()'s pre-iteration radius clamp (src/hydro/density_loop.cc:818-838) and the
+                // row's iteration state, as k_density_init sets them
+                double h = ds.H[i];
+                if ((h < 0) || !isfinite(h) || (h > ds.hclamp)) { h = ds.hclamp; ds.H[i] = h; }
+                ds.hcur[r] = h;
+                ds.left[r] = 0; ds.right[r] = 0; ds.cn_cur[r] = 0; ds.cn_init[r] = 0; ds.iter[r] = 0;
+                ds.need[r] = 1;
+            }
         }
     }
     if (blockIdx.x == nblocks - 1 && tid == 0) *count_out = off + sprefix[kActChunk / 64];
