@@ -142,6 +142,8 @@ struct gz_context {
                                // [4] wake-up flagged, [5] woken count, [6] timestep stop code,
                                // [8..9] wake-up bin limits, [10] active count
     int *hcounters = nullptr;  // pinned host mirror
+    unsigned long long *rungs = nullptr;  // device: kRungCount face counts of the force passes (when counted)
+    bool rung_stats = false;              // the force passes count their Riemann rungs
     unsigned long long *u64c = nullptr;   // device: [0..63] bin histogram, [64] displacement bits, [65] force pairs
     unsigned long long *hu64c = nullptr;  // pinned host mirror
 
@@ -332,6 +334,8 @@ int alloc_state(gz_context *ctx, long cap)
     CHK(dalloc(ctx, &ctx->u64c, 80 + 8));
     HIPCHK(hipMemset(ctx->u64c, 0, (80 + 8) * sizeof(unsigned long long)));  // the running pair count starts at zero
     ctx->counters = (int *)(ctx->u64c + 80);
+    CHK(dalloc(ctx, &ctx->rungs, 16));
+    HIPCHK(hipMemset(ctx->rungs, 0, 16 * sizeof(unsigned long long)));
     if (!ctx->hu64c) {
         hipError_t e = hipHostMalloc((void **)&ctx->hu64c, (80 + 8) * sizeof(unsigned long long), hipHostMallocDefault);
         if (e != hipSuccess) return fail(ctx, GZ_E_MEMORY, "hipHostMalloc failed");
@@ -373,6 +377,7 @@ void free_state(gz_context *c)
     dfree(c->rec_grad); dfree(c->rec_force); dfree(c->dacc);
     dfree(c->VKX); dfree(c->VKY); dfree(c->VKZ); dfree(c->UK); dfree(c->SOFT); dfree(c->TIB); dfree(c->DTS);
     dfree(c->u64c);
+    dfree(c->rungs);
     if (c->hu64c) hipHostFree(c->hu64c);
     c->hu64c = nullptr;
     if (c->tmp) hipFree(c->tmp);
@@ -962,7 +967,7 @@ int force_stage(gz_context *ctx)
     if (ctx->nact == 0) return 0;
     CHK(ensure_sym(ctx));
     ForceView F{ctx->n, ctx->X, ctx->Y, ctx->Z, ctx->VX, ctx->VY, ctx->VZ, ctx->M, ctx->H, ctx->U, ctx->RHO, ctx->PR, ctx->CN,
-                ctx->FCE, ctx->DRK, ctx->GR, ctx->GP, ctx->GV, ctx->T};
+                ctx->FCE, ctx->DRK, ctx->GR, ctx->GP, ctx->GV, ctx->T, ctx->rungs};
     {
         Timed t(ctx, "force");
         Cell<R> *rec = (Cell<R> *)ctx->rec_force;
@@ -973,13 +978,16 @@ int force_stage(gz_context *ctx)
         else
             k_pack_force<R><<<nblk(ctx->n), kBlock, 0, ctx->stream>>>(ctx->par.gamma, F, rec, ctx->counters + 2, ctx->par.box_size,
                                                                       BoxTable{});
+        // the kernels that count the faces' Riemann rungs when asked (the same code otherwise)
+        auto rows_kernel = ctx->rung_stats ? k_force_rows<KERN, R, true> : k_force_rows<KERN, R, false>;
+        auto lanes_kernel = ctx->rung_stats ? k_force<KERN, R, false, true> : k_force<KERN, R, false, false>;
         if (ctx->nact <= ctx->few_rows)
-            k_force_rows<KERN, R><<<xcd_grid(ctx->nact), kRowThreads, 0, ctx->stream>>>(
+            rows_kernel<<<xcd_grid(ctx->nact), kRowThreads, 0, ctx->stream>>>(
                 ctx->dp, gam<R>(ctx->dp), ctx->nact, ctx->act, F, rec, ctx->ell_s, ctx->ld_s, ctx->cnt_s,
                 ctx->all_active ? nullptr : ctx->ACT, ctx->TB, ctx->MSV, ctx->WAKE, ctx->ACC, ctx->DTU, ctx->MSV, ctx->counters + 2,
                 ctx->counters + 4, ctx->nact <= kRowPairAtomics ? ctx->u64c + 65 : nullptr, BoxTable{});
         else
-            k_force<KERN, R, false><<<xcd_grid(nblk(ctx->nact)), kBlock, 0, ctx->stream>>>(
+            lanes_kernel<<<xcd_grid(nblk(ctx->nact)), kBlock, 0, ctx->stream>>>(
                 ctx->dp, gam<R>(ctx->dp), ctx->nact, ctx->act, F, rec, ctx->ell_s, ctx->ld_s, ctx->cnt_s,
                 ctx->all_active ? nullptr : ctx->ACT, ctx->TB, ctx->MSV, ctx->WAKE, ctx->ACC, ctx->DTU, ctx->MSV, ctx->counters + 2,
                 ctx->counters + 4, nullptr, ctx->sched_valid ? ctx->sched : nullptr, BoxTable{});
@@ -1316,6 +1324,23 @@ int run_step(gz_context *ctx)
     return 0;
 }
 
+// the Riemann functions on n given cases (k_riemann_probe), staged through device buffers the
+// caller provides (din: n*kProbeIn doubles, dout: n*kProbeOut doubles, drung: n)
+int probe_run(gz_context *ctx, int mode, long n, const double *in, double *out, unsigned *rung, double *din, double *dout,
+              unsigned *drung)
+{
+    HIPCHK(hipMemcpyAsync(din, in, (size_t)n * kProbeIn * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    if (ctx->par.precision == GZ_FP64)
+        k_riemann_probe<double><<<nblk(n), kBlock, 0, ctx->stream>>>(mode, gam<double>(ctx->dp), n, din, dout, drung);
+    else
+        k_riemann_probe<float><<<nblk(n), kBlock, 0, ctx->stream>>>(mode, gam<float>(ctx->dp), n, din, dout, drung);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(out, dout, (size_t)n * kProbeOut * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipMemcpyAsync(rung, drung, (size_t)n * sizeof(unsigned), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
 int set_device(gz_context *ctx)
 {
     HIPCHK(hipSetDevice(ctx->device));
@@ -1594,6 +1619,20 @@ int gz_stat(gz_context *ctx, const char *name, double *value)
         *value = (double)v;
         return 0;
     }
+    {
+        // the Riemann rung counts of the force passes since "riemann_stats" was last set
+        static const char *rung_names[kRungCount] = {"riemann_hllc", "riemann_vacuum", "riemann_roe", "riemann_pvrs", "riemann_kt",
+                                                     "riemann_exact", "riemann_exact_vacuum", "riemann_unphysical",
+                                                     "riemann_retry_first_order", "riemann_retry_zero_velocity", "riemann_fail",
+                                                     "riemann_faces", "entropic_considered", "entropic_applied"};
+        for (int b = 0; b < kRungCount; b++)
+            if (k == rung_names[b]) {
+                unsigned long long v = 0;
+                HIPCHK(hipMemcpy(&v, ctx->rungs + b, sizeof v, hipMemcpyDeviceToHost));
+                *value = (double)v;
+                return 0;
+            }
+    }
     static const char *known[] = {"density_iterations", "ngb_rebuilds", "n_active", "ell_widen", "active_updates", "resorts"};
     auto it = ctx->stats.find(k);
     if (it == ctx->stats.end()) {
@@ -1725,6 +1764,13 @@ int gz_set_option(gz_context *ctx, const char *name, double value)
         ctx->sym_valid = false;
         return 0;
     }
+    if (k == "riemann_stats") {
+        if (value != 0 && value != 1) return fail(ctx, GZ_E_ARG, "riemann_stats must be 0 or 1");
+        ctx->rung_stats = value != 0;
+        HIPCHK(hipMemsetAsync(ctx->rungs, 0, 16 * sizeof(unsigned long long), ctx->stream));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        return 0;
+    }
     if (k == "wide_search") {
         if (value != 0 && value != 1) return fail(ctx, GZ_E_ARG, "wide_search must be 0 or 1");
         ctx->wide = (int)value;
@@ -1748,6 +1794,26 @@ int gz_timeline_get(gz_context *ctx, gz_timeline *out)
     out->bin_active = ctx->mask;
     out->stop_code = ctx->stop_code;
     return 0;
+}
+
+int gz_riemann_probe(gz_context *ctx, int mode, long n, const double *in, double *out, unsigned *rung)
+{
+    if (!ctx) return GZ_E_ARG;
+    if (mode < 0 || mode > 4) return fail(ctx, GZ_E_ARG, "mode must be 0..4");
+    if (n < 0 || n > (1L << 24)) return fail(ctx, GZ_E_ARG, "n=%ld outside [0, 2^24]", n);
+    if (n > 0 && (!in || !out || !rung)) return fail(ctx, GZ_E_ARG, "in, out and rung are required");
+    if (n == 0) return 0;
+    CHK(set_device(ctx));
+    double *din = nullptr, *dout = nullptr;
+    unsigned *drung = nullptr;
+    int rc = dalloc(ctx, &din, (size_t)n * kProbeIn);
+    if (!rc) rc = dalloc(ctx, &dout, (size_t)n * kProbeOut);
+    if (!rc) rc = dalloc(ctx, &drung, (size_t)n);
+    if (!rc) rc = probe_run(ctx, mode, n, in, out, rung, din, dout, drung);
+    dfree(din);
+    dfree(dout);
+    dfree(drung);
+    return rc;
 }
 
 long gz_neighbors(gz_context *ctx, int symmetric, long *offsets, int *nbr, long cap)

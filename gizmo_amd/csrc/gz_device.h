@@ -422,15 +422,36 @@ __device__ __forceinline__ void reconstruct(R Qi, R gi0, R gi1, R gi2, R Qj, R g
     }
 }
 
-// get_wavespeeds_and_pressure_star, src/hydro/reimann.h:485-549
+// Which estimates and fallbacks of the Riemann ladder a face took (bit per rung; a face that
+// takes a rung several times, in its retries, counts once).  No counterpart in the reference:
+// the force pass counts them on request, so that tests can see each rung run on the device.
+enum RiemannRung : unsigned {
+    kRungHllc = 1u << 0,          // first HLLC estimate accepted (get_wavespeeds_and_pressure_star, :496-512)
+    kRungVacuum = 1u << 1,        // separating faster than sound: vacuum star state (:490-494)
+    kRungRoe = 1u << 2,           // Roe-averaged wave speeds (:514-530)
+    kRungPvrs = 1u << 3,          // primitive-variable estimate (:531-546)
+    kRungKt = 1u << 4,            // Kurganov-Tadmor flux (Riemann_solver, :340-342)
+    kRungExact = 1u << 5,         // exact solver (:352-355)
+    kRungExactVacuum = 1u << 6,   // exact solver without an iterative solution: vacuum sample (:640-666)
+    kRungUnphysical = 1u << 7,    // negative density or pressures (:284-290)
+    kRungRetryFirstOrder = 1u << 8,  // first-order states at 1.4x the limiter (hydro_core_meshless_functions.h:351-380)
+    kRungRetryZeroVel = 1u << 9,  // zero velocities at twice the limiter (:381-420)
+    kRungFail = 1u << 10,         // no positive pressure anywhere (:421-425)
+    kRungFace = 1u << 11,         // the face reached the Riemann solve
+    kEntropicConsidered = 1u << 12,  // the entropic switch was evaluated (hydro_core_meshless_functions.h:515-518)
+    kEntropicApplied = 1u << 13,  // and its energy flux replaced the Riemann one (:569-572)
+};
+constexpr int kRungCount = 14;
+
+// get_wavespeeds_and_pressure_star, src/hydro/reimann.h:485-549; returns the estimate taken
 template <typename R>
-__device__ inline void wavespeeds(const Gam<R> &G, const RState<R> &L, const RState<R> &Rs, const R *n,
-                                  R vL, R vR, R csL, R csR, R hL, R hR, ROut<R> &o, R ptl)
+__device__ inline unsigned wavespeeds(const Gam<R> &G, const RState<R> &L, const RState<R> &Rs, const R *n,
+                                      R vL, R vR, R csL, R csR, R hL, R hR, ROut<R> &o, R ptl)
 {
     const R tiny = R(Tiny<R>::v);
     if ((vR - vL) > dmax(csL, csR)) {
         o.P_M = tiny; o.S_M = 0;
-        return;
+        return kRungVacuum;
     }
     R PT_L = L.p, PT_R = Rs.p;
     R S_L = dmin(vL, vR) - dmax(csL, csR);
@@ -440,31 +461,30 @@ __device__ inline void wavespeeds(const Gam<R> &G, const RState<R> &L, const RSt
     o.S_M = ((PT_R - PT_L) + wL * vL - wR * vR) / (wL - wR);
     o.P_M = (PT_L * wR - PT_R * wL + wL * wR * (vR - vL)) / (wR - wL);
     if (o.P_M <= tiny) { o.P_M = tiny; o.S_M = 0; }
-    if ((o.P_M <= 0) || (isnan(o.P_M)) || (o.P_M > ptl)) {
-        R sL = sqrt(L.rho), sR = sqrt(Rs.rho);
-        R sinv = R(1) / (sL + sR);
-        R vr0 = (sL * L.v0 + sR * Rs.v0) * sinv;
-        R vr1 = (sL * L.v1 + sR * Rs.v1) * sinv;
-        R vr2 = (sL * L.v2 + sR * Rs.v2) * sinv;
-        R vline = vr0 * n[0] + vr1 * n[1] + vr2 * n[2];
-        R h_roe = (sL * hL + sR * hR) * sinv;
-        R cs_roe = sqrt(dmax(R(1.e-30), G.G9 * (h_roe - R(0.5) * (vr0 * vr0 + vr1 * vr1 + vr2 * vr2))));
-        S_R = dmax(vR + csR, vline + cs_roe);
-        S_L = dmin(vL - csL, vline - cs_roe);
-        wR = Rs.rho * (S_R - vR);
-        wL = -L.rho * (S_L - vL);
-        o.S_M = ((wR * vR + wL * vL) + (PT_L - PT_R)) / (wR + wL);
-        o.P_M = L.rho * (vL - S_L) * (vL - o.S_M) + PT_L;
-        if (o.P_M <= tiny) { o.P_M = tiny; o.S_M = 0; }
-        if ((o.P_M <= 0) || (isnan(o.P_M)) || (o.P_M > ptl)) {
-            o.P_M = R(0.5) * ((PT_L + PT_R) + (vL - vR) * R(0.25) * (L.rho + Rs.rho) * (csL + csR));
-            o.S_M = R(0.5) * (vR + vL) + R(2.0) * (PT_L - PT_R) / ((L.rho + Rs.rho) * (csL + csR));
-            R Sp = dmax(dmax(fabs(vL - csL), fabs(vR - csR)), dmax(fabs(vL + csL), fabs(vR + csR)));
-            if (o.S_M < -Sp) o.S_M = -Sp;
-            if (o.S_M > Sp) o.S_M = Sp;
-            if (o.P_M <= tiny) { o.P_M = tiny; o.S_M = 0; }
-        }
-    }
+    if (!((o.P_M <= 0) || (isnan(o.P_M)) || (o.P_M > ptl))) return kRungHllc;
+    R sL = sqrt(L.rho), sR = sqrt(Rs.rho);
+    R sinv = R(1) / (sL + sR);
+    R vr0 = (sL * L.v0 + sR * Rs.v0) * sinv;
+    R vr1 = (sL * L.v1 + sR * Rs.v1) * sinv;
+    R vr2 = (sL * L.v2 + sR * Rs.v2) * sinv;
+    R vline = vr0 * n[0] + vr1 * n[1] + vr2 * n[2];
+    R h_roe = (sL * hL + sR * hR) * sinv;
+    R cs_roe = sqrt(dmax(R(1.e-30), G.G9 * (h_roe - R(0.5) * (vr0 * vr0 + vr1 * vr1 + vr2 * vr2))));
+    S_R = dmax(vR + csR, vline + cs_roe);
+    S_L = dmin(vL - csL, vline - cs_roe);
+    wR = Rs.rho * (S_R - vR);
+    wL = -L.rho * (S_L - vL);
+    o.S_M = ((wR * vR + wL * vL) + (PT_L - PT_R)) / (wR + wL);
+    o.P_M = L.rho * (vL - S_L) * (vL - o.S_M) + PT_L;
+    if (o.P_M <= tiny) { o.P_M = tiny; o.S_M = 0; }
+    if (!((o.P_M <= 0) || (isnan(o.P_M)) || (o.P_M > ptl))) return kRungRoe;
+    o.P_M = R(0.5) * ((PT_L + PT_R) + (vL - vR) * R(0.25) * (L.rho + Rs.rho) * (csL + csR));
+    o.S_M = R(0.5) * (vR + vL) + R(2.0) * (PT_L - PT_R) / ((L.rho + Rs.rho) * (csL + csR));
+    R Sp = dmax(dmax(fabs(vL - csL), fabs(vR - csR)), dmax(fabs(vL + csL), fabs(vR + csR)));
+    if (o.S_M < -Sp) o.S_M = -Sp;
+    if (o.S_M > Sp) o.S_M = Sp;
+    if (o.P_M <= tiny) { o.P_M = tiny; o.S_M = 0; }
+    return kRungPvrs;
 }
 
 // Riemann_solver_KurganovTadmor_PWK, src/hydro/reimann.h:445-477 (MFM branch)
@@ -559,42 +579,59 @@ __device__ inline int iterative_riemann(const Gam<R> &G, const RState<R> &L, con
 
 // Riemann_solver_exact (:625-668), MFM: only P_M and S_M change
 template <typename R>
-__device__ inline void exact_riemann(const Gam<R> &G, const RState<R> &L, const RState<R> &Rs, R vL, R vR, R csL, R csR, ROut<R> &o)
+__device__ inline unsigned exact_riemann(const Gam<R> &G, const RState<R> &L, const RState<R> &Rs, R vL, R vR, R csL, R csR, ROut<R> &o)
 {
-    if ((L.p == 0 && Rs.p == 0) || (L.rho == 0 && Rs.rho == 0)) { o.P_M = o.S_M = 0; return; }
+    if ((L.p == 0 && Rs.p == 0) || (L.rho == 0 && Rs.rho == 0)) { o.P_M = o.S_M = 0; return kRungExact | kRungExactVacuum; }
     if ((L.rho > 0) && (Rs.rho > 0)) {
         if (!iterative_riemann(G, L, Rs, vL, vR, csL, csR, o)) {
             R SL = vL + G.G4 * csL, SR = vR - G.G4 * csR;
             if (R(0.0) <= SL) { o.P_M = 0; o.S_M = vL + G.G4 * csL; }
             else if (R(0.0) >= SR) { o.P_M = 0; o.S_M = vR - G.G4 * csR; }
             else { o.P_M = 0; o.S_M = R(0.0); }
+            return kRungExact | kRungExactVacuum;
         }
-    } else {
-        if (L.rho > 0) { o.P_M = 0; o.S_M = vL + G.G4 * csL; }
-        if (Rs.rho > 0) { o.P_M = 0; o.S_M = vR - G.G4 * csR; }
+        return kRungExact;
     }
+    if (L.rho > 0) { o.P_M = 0; o.S_M = vL + G.G4 * csL; }
+    if (Rs.rho > 0) { o.P_M = 0; o.S_M = vR - G.G4 * csR; }
+    return kRungExact | kRungExactVacuum;
 }
 
-// Riemann_solver (:282-370) with HLLC_Riemann_solver (:382-396, MFM fluxes)
+// the sound speeds, enthalpies and normal velocities Riemann_solver derives from the face states
+// (src/hydro/reimann.h:313-339, no EOS_GENERAL)
+template <typename R> struct RPrep { R csL, csR, hL, hR, vL, vR; };
 template <typename R>
-__device__ inline void riemann(const Gam<R> &G, const RState<R> &L, const RState<R> &Rs, const R *n, R ptl, ROut<R> &o)
+__device__ __forceinline__ RPrep<R> riemann_prep(const Gam<R> &G, const RState<R> &L, const RState<R> &Rs, const R *n)
 {
-    if ((L.p < 0 && Rs.p < 0) || (L.rho < 0) || (Rs.rho < 0)) { o.P_M = 0; return; }
-    R csL = sqrt(G.G0 * L.p / L.rho);
-    R csR = sqrt(G.G0 * Rs.p / Rs.rho);
+    RPrep<R> q;
+    q.csL = sqrt(G.G0 * L.p / L.rho);
+    q.csR = sqrt(G.G0 * Rs.p / Rs.rho);
     R uL = L.p / (G.G9 * L.rho);
     R uR = Rs.p / (G.G9 * Rs.rho);
-    R hL = L.p / L.rho + uL + R(0.5) * (L.v0 * L.v0 + L.v1 * L.v1 + L.v2 * L.v2);
-    R hR = Rs.p / Rs.rho + uR + R(0.5) * (Rs.v0 * Rs.v0 + Rs.v1 * Rs.v1 + Rs.v2 * Rs.v2);
-    R vL = L.v0 * n[0] + L.v1 * n[1] + L.v2 * n[2];
-    R vR = Rs.v0 * n[0] + Rs.v1 * n[1] + Rs.v2 * n[2];
-    wavespeeds(G, L, Rs, n, vL, vR, csL, csR, hL, hR, o, ptl);
+    q.hL = L.p / L.rho + uL + R(0.5) * (L.v0 * L.v0 + L.v1 * L.v1 + L.v2 * L.v2);
+    q.hR = Rs.p / Rs.rho + uR + R(0.5) * (Rs.v0 * Rs.v0 + Rs.v1 * Rs.v1 + Rs.v2 * Rs.v2);
+    q.vL = L.v0 * n[0] + L.v1 * n[1] + L.v2 * n[2];
+    q.vR = Rs.v0 * n[0] + Rs.v1 * n[1] + Rs.v2 * n[2];
+    return q;
+}
+
+// Riemann_solver (:282-370) with HLLC_Riemann_solver (:382-396, MFM fluxes); returns the rungs taken
+template <typename R>
+__device__ inline unsigned riemann(const Gam<R> &G, const RState<R> &L, const RState<R> &Rs, const R *n, R ptl, ROut<R> &o)
+{
+    if ((L.p < 0 && Rs.p < 0) || (L.rho < 0) || (Rs.rho < 0)) { o.P_M = 0; return kRungUnphysical; }
+    const RPrep<R> q = riemann_prep(G, L, Rs, n);
+    unsigned rung = wavespeeds(G, L, Rs, n, q.vL, q.vR, q.csL, q.csR, q.hL, q.hR, o, ptl);
     o.f0 = o.P_M * n[0];
     o.f1 = o.P_M * n[1];
     o.f2 = o.P_M * n[2];
     o.fp = o.P_M * o.S_M;
-    if ((o.P_M < 0) || (isnan(o.P_M))) kt_flux(L, Rs, n, vL, vR, csL, csR, hL, hR, o);
-    if ((o.P_M < 0) || (isnan(o.P_M)) || (o.P_M > ptl)) exact_riemann(G, L, Rs, vL, vR, csL, csR, o);
+    if ((o.P_M < 0) || (isnan(o.P_M))) {
+        kt_flux(L, Rs, n, q.vL, q.vR, q.csL, q.csR, q.hL, q.hR, o);
+        rung |= kRungKt;
+    }
+    if ((o.P_M < 0) || (isnan(o.P_M)) || (o.P_M > ptl)) rung |= exact_riemann(G, L, Rs, q.vL, q.vR, q.csL, q.csR, o);
+    return rung;
 }
 
 // ---------------------------------------------------------------------------------------
@@ -674,30 +711,32 @@ template <typename R> struct alignas(16) Cell : PosRec<R> {
 // the pressure limit, then zero velocities at twice it.  false: no positive pressure.
 template <typename T>
 __device__ inline bool riemann_ladder(const Gam<T> &G, RState<T> L, RState<T> Rs, const T *n, T ptl, T pI, T pJ, T rhoI, T rhoJ,
-                                      T vi0, T vi1, T vi2, T vj0, T vj1, T vj2, ROut<T> &o)
+                                      T vi0, T vi1, T vi2, T vj0, T vj1, T vj2, ROut<T> &o, unsigned &rung)
 {
     o.P_M = o.S_M = o.f0 = o.f1 = o.f2 = o.fp = 0;
-    riemann(G, L, Rs, n, ptl, o);
+    rung |= riemann(G, L, Rs, n, ptl, o);
     if ((o.P_M < 0) || (isnan(o.P_M)) || (o.P_M > T(1.4) * ptl)) {
         Rs.p = pI; L.p = pJ; Rs.rho = rhoI; L.rho = rhoJ;
         Rs.v0 = vi0; Rs.v1 = vi1; Rs.v2 = vi2;
         L.v0 = vj0; L.v1 = vj1; L.v2 = vj2;
-        riemann(G, L, Rs, n, T(1.4) * ptl, o);
+        rung |= kRungRetryFirstOrder | riemann(G, L, Rs, n, T(1.4) * ptl, o);
         if ((o.P_M < 0) || (isnan(o.P_M))) {
             Rs.p = pI; L.p = pJ; Rs.rho = rhoI; L.rho = rhoJ;
             Rs.v0 = Rs.v1 = Rs.v2 = 0; L.v0 = L.v1 = L.v2 = 0;
-            riemann(G, L, Rs, n, T(2.0) * ptl, o);
-            if ((o.P_M < 0) || (isnan(o.P_M))) return false;
+            rung |= kRungRetryZeroVel | riemann(G, L, Rs, n, T(2.0) * ptl, o);
+            if ((o.P_M < 0) || (isnan(o.P_M))) { rung |= kRungFail; return false; }
         }
     }
     return true;
 }
 
-// returns 0 skipped, 1 fluxes produced (maybe zero), -1 no Riemann solution (stop 1234)
+// returns 0 skipped, 1 fluxes produced (maybe zero), -1 no Riemann solution (stop 1234); rung
+// receives the RiemannRung bits of the face (callers that do not count them leave it unused)
 template <int KERN, typename R>
 __device__ inline int force_pair(const DevParams &P, const Gam<R> &G, const Cell<R> &I, const Cell<R> &J, R dp0, R dp1, R dp2, R r2,
-                                 R &F0, R &F1, R &F2, R &Fp, R &vsig_ts)
+                                 R &F0, R &F1, R &F2, R &Fp, R &vsig_ts, unsigned &rung)
 {
+    rung = 0;
     F0 = F1 = F2 = Fp = 0;
     vsig_ts = 0;
     if (J.m <= 0) return 0;
@@ -791,8 +830,9 @@ __device__ inline int force_pair(const DevParams &P, const Gam<R> &G, const Cell
     R ptl = R(1.1 * 1.0) * dmax(pit, pjt);
     if (recon == 0) ptl = dmax(ptl, dmax(dmax(I.p, J.p), R(2.) * dmax(I.rho, J.rho) * v2a));
     ROut<R> o;
+    rung = kRungFace;
     if (!riemann_ladder<R>(G, L, Rs, n, ptl, I.p, J.p, I.rho, J.rho, I.v0 - vf0, I.v1 - vf1, I.v2 - vf2, J.v0 - vf0, J.v1 - vf1,
-                           J.v2 - vf2, o))
+                           J.v2 - vf2, o, rung))
         return -1;
     if (!((o.P_M > 0) && (!isnan(o.P_M)))) return 1;
     o.fp += vf0 * o.f0 + vf1 * o.f1 + vf2 * o.f2;
@@ -802,6 +842,7 @@ __device__ inline int force_pair(const DevParams &P, const Gam<R> &G, const Cell
     {
         R smc = fabs(o.S_M) / dmin(I.cs, J.cs);
         if ((smc < R(0.5)) || (leak > R(1))) {
+            rung |= kEntropicConsidered;
             int use = 1;
             R fpm = o.P_M * An;
             R pdvf = o.P_M * vdr / R(1.0);
@@ -829,7 +870,10 @@ __device__ inline int force_pair(const DevParams &P, const Gam<R> &G, const Cell
                 }
             }
             if (cnum2 >= cnumcrit2) use = 1;
-            if (use) Fp += du_new - du_old;
+            if (use) {
+                Fp += du_new - du_old;
+                rung |= kEntropicApplied;
+            }
         }
     }
     return 1;

@@ -115,7 +115,20 @@ struct DensityOut {           // per particle, device order; vectors SoA with co
 struct ForceView {            // per particle, device order; vectors SoA with column stride n
     long n;
     const double *X, *Y, *Z, *VX, *VY, *VZ, *M, *H, *U, *RHO, *PR, *CN, *FCE, *DRK, *GR, *GP, *GV, *T;
+    unsigned long long *rungs;  // kRungCount face counts (the force kernels built to count them)
 };
+
+// Adds each RiemannRung bit of the calling lanes' faces to its count: one atomic per rung
+// that any of them took (the lanes of the wavefront that call it together)
+__device__ __forceinline__ void count_rungs(unsigned long long *__restrict__ cnt, unsigned bits)
+{
+    const unsigned long long live = __ballot(1);
+    const int leader = __ffsll((long long)live) - 1;
+    for (int b = 0; b < kRungCount; b++) {
+        const unsigned long long m = __ballot((bits >> b) & 1u);
+        if ((int)(threadIdx.x & 63) == leader && m) atomicAdd(&cnt[b], (unsigned long long)__popcll(m));
+    }
+}
 
 // the timestep loop: parameters of one step and the particle fields it reads and writes
 constexpr int kTimebins = 60;  // TIMEBINS, src/declarations/typedefs.h:8
@@ -295,17 +308,23 @@ template <int KERN, typename R>
 __global__ void k_gradients_rows(DevParams P, int nrows, const int *act, const double *X, const double *Y, const double *Z,
                                  const GradRec<R> *rec, GradSrc src, const double *T, const double *CN, long n, const int *ell,
                                  long ld, const int *cnt, double *GR, double *GP, double *GV, BoxTable bt);
-template <int KERN, typename R>
+// STATS: the force kernels also count the RiemannRung bits of their faces into F.rungs
+template <int KERN, typename R, bool STATS = false>
 __global__ void k_force_rows(DevParams P, Gam<R> G, int nrows, const int *act, ForceView F, const Cell<R> *rec, const int *ell,
                              long ld, const int *cnt, const uint8_t *active, const int *timebin, const double *msv_in, int *wakeup,
                              double *ACC, double *DTU, double *MSV, int *err, int *need_wakeup, unsigned long long *pairs,
                              BoxTable bt);
 
-template <int KERN, typename R, bool BOXES>
+template <int KERN, typename R, bool BOXES, bool STATS = false>
 __global__ void k_force(DevParams P, Gam<R> G, int nrows, const int *act, ForceView F, const Cell<R> *rec, const int *ell,
                         long ld, const int *cnt, const uint8_t *active, const int *timebin, const double *msv_in, int *wakeup,
                         double *ACC, double *DTU, double *MSV, int *err, int *need_wakeup, unsigned long long *pairs,
                         const int *sched, BoxTable bt);
+
+// the Riemann functions on given face states (diagnostic; layout of in/out at the definition)
+constexpr int kProbeIn = 24, kProbeOut = 8;
+template <typename R>
+__global__ void k_riemann_probe(int mode, Gam<R> G, long n, const double *in, double *out, unsigned *rung);
 
 // the order in which the force pass's lanes take the rows: within each block of span rows,
 // longest list first (a wavefront runs as long as its longest row), when the waves of the plain

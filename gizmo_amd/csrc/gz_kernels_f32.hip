@@ -45,6 +45,19 @@ namespace gz {
                                                    unsigned long long *, BoxTable);
 GZ_INST(3, float)
 GZ_INST(6, float)
+// the force kernels that count the Riemann rungs of their faces (one box)
+#define GZ_INST_STATS(KERN, R)                                                                                                   \
+    template __global__ void k_force<KERN, R, false, true>(DevParams, Gam<R>, int, const int *, ForceView, const Cell<R> *,     \
+                                                           const int *, long, const int *, const uint8_t *, const int *,        \
+                                                           const double *, int *, double *, double *, double *, int *, int *,   \
+                                                           unsigned long long *, const int *, BoxTable);                        \
+    template __global__ void k_force_rows<KERN, R, true>(DevParams, Gam<R>, int, const int *, ForceView, const Cell<R> *,      \
+                                                         const int *, long, const int *, const uint8_t *, const int *,          \
+                                                         const double *, int *, double *, double *, double *, int *, int *,     \
+                                                         unsigned long long *, BoxTable);
+GZ_INST_STATS(3, float)
+GZ_INST_STATS(6, float)
+template __global__ void k_riemann_probe<float>(int, Gam<float>, long, const double *, double *, unsigned *);
 template __global__ void k_pack_grad<float>(long, const double *, const double *, const double *, const double *, const double *,
                                              const double *, const double *, GradRec<float> *, const double *, const double *,
                                              const double *, double, BoxTable);

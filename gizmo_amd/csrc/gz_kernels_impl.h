@@ -1027,7 +1027,7 @@ __device__ inline void force_tail(const DevParams &P, const ForceView &F, int i,
     MSV[i] = ms;
 }
 
-template <int KERN, typename R, bool BOXES>
+template <int KERN, typename R, bool BOXES, bool STATS>
 __global__ void __launch_bounds__(256) k_force(DevParams P, Gam<R> G, int nrows, const int *__restrict__ act, ForceView F,
                                                const Cell<R> *__restrict__ rec, const int *__restrict__ ell, long ld,
                                                const int *__restrict__ cnt, const uint8_t *__restrict__ active,
@@ -1075,7 +1075,9 @@ __global__ void __launch_bounds__(256) k_force(DevParams P, Gam<R> G, int nrows,
             record_dp<R>(P, I, J, xi, yi, zi, F.X, F.Y, F.Z, j, dp0, dp1, dp2);
             const R r2 = dp0 * dp0 + dp1 * dp1 + dp2 * dp2;
             R f0, f1, f2, fp, vts;
-            int rc = force_pair<KERN, R>(P, G, I, J, dp0, dp1, dp2, r2, f0, f1, f2, fp, vts);
+            unsigned rung;
+            int rc = force_pair<KERN, R>(P, G, I, J, dp0, dp1, dp2, r2, f0, f1, f2, fp, vts, rung);
+            if constexpr (STATS) count_rungs(F.rungs, rung);
             if (rc < 0) {
                 atomicMax(err, 1234);
                 if (atomicCAS(err + 10, -1, i) == -1) err[11] = j;
@@ -1095,6 +1097,52 @@ __global__ void __launch_bounds__(256) k_force(DevParams P, Gam<R> G, int nrows,
         A0 = a0.value(); A1 = a1.value(); A2 = a2.value(); dE = ae.value(); msv = m;
     }
     force_tail(P, F, i, enabled, sound_i, A0, A1, A2, dE, msv, ACC, DTU, MSV);
+}
+
+// ---------------------------------------------------------------------------------------
+// The Riemann functions on given face states, one lane per case (a diagnostic with no
+// counterpart in the reference; the force pass reaches the same functions through the pair
+// body).  Case k reads kProbeIn values from in + k*kProbeIn: the left state (rho, p, v[3]), the
+// right state, the face normal n[3], the pressure limiter, then the first-order states the
+// ladder retries with (p_i, p_j, rho_i, rho_j, v_i[3], v_j[3]); it writes P_M, S_M, the momentum
+// flux[3], the energy flux and 1 (0: the ladder found no positive pressure) to
+// out + k*kProbeOut, and the RiemannRung bits taken to rung[k].  Modes: 0 the ladder of the pair
+// body with its two retries (hydro_core_meshless_functions.h:351-427), 1 Riemann_solver
+// (reimann.h:282-370), 2 get_wavespeeds_and_pressure_star alone (:485-549), 3
+// Riemann_solver_KurganovTadmor_PWK alone (:445-477), 4 Riemann_solver_exact alone (:625-668).
+// ---------------------------------------------------------------------------------------
+template <typename R>
+__global__ void __launch_bounds__(256) k_riemann_probe(int mode, Gam<R> G, long n, const double *__restrict__ in,
+                                                       double *__restrict__ out, unsigned *__restrict__ rung)
+{
+    const long k = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const double *c = in + k * kProbeIn;
+    RState<R> L, Rs;
+    L.rho = (R)c[0]; L.p = (R)c[1]; L.v0 = (R)c[2]; L.v1 = (R)c[3]; L.v2 = (R)c[4];
+    Rs.rho = (R)c[5]; Rs.p = (R)c[6]; Rs.v0 = (R)c[7]; Rs.v1 = (R)c[8]; Rs.v2 = (R)c[9];
+    const R nv[3] = {(R)c[10], (R)c[11], (R)c[12]};
+    const R ptl = (R)c[13];
+    ROut<R> o;
+    o.P_M = o.S_M = o.f0 = o.f1 = o.f2 = o.fp = 0;
+    unsigned rb = 0;
+    bool ok = true;
+    if (mode == 0) {
+        ok = riemann_ladder<R>(G, L, Rs, nv, ptl, (R)c[14], (R)c[15], (R)c[16], (R)c[17], (R)c[18], (R)c[19], (R)c[20], (R)c[21],
+                               (R)c[22], (R)c[23], o, rb);
+    } else if (mode == 1) {
+        rb = riemann(G, L, Rs, nv, ptl, o);
+    } else {
+        const RPrep<R> q = riemann_prep(G, L, Rs, nv);
+        if (mode == 2) rb = wavespeeds(G, L, Rs, nv, q.vL, q.vR, q.csL, q.csR, q.hL, q.hR, o, ptl);
+        else if (mode == 3) { kt_flux(L, Rs, nv, q.vL, q.vR, q.csL, q.csR, q.hL, q.hR, o); rb = kRungKt; }
+        else rb = exact_riemann(G, L, Rs, q.vL, q.vR, q.csL, q.csR, o);
+    }
+    double *w = out + k * kProbeOut;
+    w[0] = (double)o.P_M; w[1] = (double)o.S_M; w[2] = (double)o.f0; w[3] = (double)o.f1; w[4] = (double)o.f2; w[5] = (double)o.fp;
+    w[6] = ok ? 1.0 : 0.0;
+    w[7] = 0.0;
+    rung[k] = rb;
 }
 
 #ifdef GZ_DEFINE_COMMON_KERNELS

@@ -799,7 +799,7 @@ __global__ void __launch_bounds__(kRowThreads) k_gradients_rows(DevParams P, int
 
 // force of few rows: each lane evaluates one list entry (hydro_accumulate_neighbor), lane 0
 // accumulates in list order, then the writeback and final operations
-template <int KERN, typename R>
+template <int KERN, typename R, bool STATS>
 __global__ void __launch_bounds__(kRowThreads) k_force_rows(DevParams P, Gam<R> G, int nrows, const int *__restrict__ act, ForceView F,
                                                    const Cell<R> *__restrict__ rec, const int *__restrict__ ell, long ld,
                                                    const int *__restrict__ cnt, const uint8_t *__restrict__ active,
@@ -843,7 +843,9 @@ __global__ void __launch_bounds__(kRowThreads) k_force_rows(DevParams P, Gam<R> 
                 record_dp<R>(P, I, J, xi, yi, zi, F.X, F.Y, F.Z, j, dp0, dp1, dp2);
                 const R r2 = dp0 * dp0 + dp1 * dp1 + dp2 * dp2;
                 R f0, f1, f2, fp, vts;
-                rc = force_pair<KERN, R>(P, G, I, J, dp0, dp1, dp2, r2, f0, f1, f2, fp, vts);
+                unsigned rung;
+                rc = force_pair<KERN, R>(P, G, I, J, dp0, dp1, dp2, r2, f0, f1, f2, fp, vts, rung);
+                if constexpr (STATS) count_rungs(F.rungs, rung);
                 if (rc < 0) {
                     atomicMax(err, 1234);
                     if (atomicCAS(err + 10, -1, i) == -1) err[11] = j;

@@ -95,6 +95,7 @@ def lib():
         L.gz_run_steps.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]
         L.gz_timeline_get.argtypes = [C.c_void_p, C.POINTER(Timeline)]
         L.gz_set_option.argtypes = [C.c_void_p, C.c_char_p, C.c_double]
+        L.gz_riemann_probe.argtypes = [C.c_void_p, C.c_int, C.c_long, _dp, _dp, C.POINTER(C.c_uint)]
         _lib = L
     return _lib
 
@@ -274,6 +275,25 @@ class Hydro:
         t = Timeline()
         self._check(lib().gz_timeline_get(self.ctx, C.byref(t)))
         return timeline_dict(t)
+
+    RUNGS = ("riemann_hllc", "riemann_vacuum", "riemann_roe", "riemann_pvrs", "riemann_kt", "riemann_exact",
+             "riemann_exact_vacuum", "riemann_unphysical", "riemann_retry_first_order", "riemann_retry_zero_velocity",
+             "riemann_fail", "riemann_faces", "entropic_considered", "entropic_applied")
+
+    def rung_counts(self):
+        """The Riemann rung counts of the force passes since the "riemann_stats" option was set."""
+        return {k: int(self.stat(k)) for k in self.RUNGS}
+
+    def riemann_probe(self, mode, cases):
+        """The device's Riemann functions on face states (n x 24, see gz_riemann_probe): returns
+        (out n x 8, rung bits n)."""
+        c = np.ascontiguousarray(cases, dtype=np.float64).reshape(-1, 24)
+        n = len(c)
+        out = np.zeros((n, 8))
+        rung = np.zeros(n, dtype=np.uint32)
+        self._check(lib().gz_riemann_probe(self.ctx, int(mode), n, c.ctypes.data_as(_dp), out.ctypes.data_as(_dp),
+                                           rung.ctypes.data_as(C.POINTER(C.c_uint))))
+        return out, rung
 
     def neighbors(self, symmetric):
         off = np.zeros(self.n + 1, dtype=np.int64)

@@ -122,7 +122,16 @@ int gz_synchronize(gz_context *ctx);
    gz_stat: named counters of the last calls ("density_iterations", "pairs_density",
    "pairs_symmetric", "ngb_rebuilds", "n_active", "ell_widen"), and totals since the last
    gz_timing_reset ("active_updates": active particles through the hot path, "resorts",
-   "force_pairs_total": neighbour pairs of every force launch). */
+   "force_pairs_total": neighbour pairs of every force launch).  With the option
+   "riemann_stats" set to 1, the force passes of a single box also count, per face, the rungs of
+   the Riemann ladder it took (each at most once per face) since the option was last set:
+   "riemann_faces" (faces that reached the Riemann solve), "riemann_hllc", "riemann_vacuum",
+   "riemann_roe", "riemann_pvrs" (the estimate get_wavespeeds_and_pressure_star took,
+   src/hydro/reimann.h:485-549), "riemann_kt" (Kurganov-Tadmor, :445-477), "riemann_exact" and
+   "riemann_exact_vacuum" (Riemann_solver_exact, :625-668), "riemann_unphysical" (:284-290),
+   "riemann_retry_first_order", "riemann_retry_zero_velocity", "riemann_fail" (the pair body's
+   retries, src/hydro/hydro_core_meshless_functions.h:351-427), "entropic_considered" and
+   "entropic_applied" (the face entropic switch, :515-573). */
 int gz_permutation(gz_context *ctx, long *perm);
 int gz_stat(gz_context *ctx, const char *name, double *value);
 
@@ -199,6 +208,19 @@ int gz_timeline_get(gz_context *ctx, gz_timeline *out);
    changes a neighbour set or a sum, and none has a counterpart in the reference (tuning knobs
    of this library). */
 int gz_set_option(gz_context *ctx, const char *name, double value);
+
+/* The Riemann functions of the force pass on n given face states, on the context's device and
+   in its precision (a diagnostic for parity tests; the reference's counterparts are internal
+   functions of src/hydro/reimann.h and the pair body).  Case k reads 24 doubles at in + 24k:
+   left state (rho, p, v[3]), right state (rho, p, v[3]), face normal n[3], the pressure limiter,
+   then the first-order states of the ladder's retries (p_i, p_j, rho_i, rho_j, v_i[3], v_j[3]);
+   it writes 8 doubles at out + 8k: P_M, S_M, momentum flux[3], energy flux, 1 or 0 (0: the
+   ladder found no positive pressure), 0; and rung[k] = the rung bits taken, in the order of the
+   "riemann_*"/"entropic_*" stat names (bit 0 hllc ... bit 10 fail).  mode: 0 the ladder of the
+   pair body with its two retries (src/hydro/hydro_core_meshless_functions.h:351-427),
+   1 Riemann_solver (src/hydro/reimann.h:282-370), 2 get_wavespeeds_and_pressure_star (:485-549),
+   3 Riemann_solver_KurganovTadmor_PWK (:445-477), 4 Riemann_solver_exact (:625-668). */
+int gz_riemann_probe(gz_context *ctx, int mode, long n, const double *in, double *out, unsigned *rung);
 
 /* Neighbour lists of the resident state, in the caller's indexing, for parity tests.
    symmetric = 0: rows r2 < h_i^2 (self included); 1: r2 < max(h_i,h_j)^2 (self excluded).

@@ -967,6 +967,18 @@ static void or_gamma_init(or_gamma *g, double gamma)
 
 static inline double or_dot3(const double *a, const double *b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
 
+/* The rungs of the Riemann ladder a face took, one bit each (the device's RiemannRung bits,
+   in the order of orc_rung_counts): not a reference quantity, a record of which branch of
+   src/hydro/reimann.h and the pair body's retries ran. */
+enum {
+    OR_RUNG_HLLC = 1u << 0, OR_RUNG_VACUUM = 1u << 1, OR_RUNG_ROE = 1u << 2, OR_RUNG_PVRS = 1u << 3, OR_RUNG_KT = 1u << 4,
+    OR_RUNG_EXACT = 1u << 5, OR_RUNG_EXACT_VACUUM = 1u << 6, OR_RUNG_UNPHYSICAL = 1u << 7, OR_RUNG_RETRY1 = 1u << 8,
+    OR_RUNG_RETRY2 = 1u << 9, OR_RUNG_FAIL = 1u << 10, OR_RUNG_FACE = 1u << 11, OR_ENTROPIC_CONSIDERED = 1u << 12,
+    OR_ENTROPIC_APPLIED = 1u << 13
+};
+#define OR_RUNG_COUNT 14
+static long long or_rung_totals[OR_RUNG_COUNT];
+
 /* reconstruct_face_states, src/hydro/reimann.h:125-244 (non-MHD, tolerance 1, no CRK) */
 void orc_reconstruct(double Q_i, const double *Gi, double Q_j, const double *Gj,
                      const double *di, const double *dj, double *Q_L, double *Q_R, int mode)
@@ -1005,14 +1017,16 @@ void orc_reconstruct(double Q_i, const double *Gi, double Q_j, const double *Gj,
     }
 }
 
-/* get_wavespeeds_and_pressure_star, src/hydro/reimann.h:485-549 */
-static void or_wavespeeds(const or_gamma *G, const or_rstate *L, const or_rstate *R, const double *n,
-                          double vL, double vR, double csL, double csR, double hL, double hR,
-                          or_rout *out, double ptl)
+/* get_wavespeeds_and_pressure_star, src/hydro/reimann.h:485-549; returns the estimate taken */
+static unsigned or_wavespeeds(const or_gamma *G, const or_rstate *L, const or_rstate *R, const double *n,
+                              double vL, double vR, double csL, double csR, double hL, double hR,
+                              or_rout *out, double ptl)
 {
     double S_L, S_R;
+    unsigned rung;
     if ((vR - vL) > DMAX(csL, csR)) {
         out->P_M = OR_MIN_REAL; out->S_M = S_L = S_R = 0;
+        rung = OR_RUNG_VACUUM;
     } else {
         double PT_L = L->p, PT_R = R->p;
         S_L = DMIN(vL, vR) - DMAX(csL, csR);
@@ -1022,6 +1036,7 @@ static void or_wavespeeds(const or_gamma *G, const or_rstate *L, const or_rstate
         out->S_M = ((PT_R - PT_L) + rho_wt_L * vL - rho_wt_R * vR) / (rho_wt_L - rho_wt_R);
         out->P_M = (PT_L * rho_wt_R - PT_R * rho_wt_L + rho_wt_L * rho_wt_R * (vR - vL)) / (rho_wt_R - rho_wt_L);
         if (out->P_M <= OR_MIN_REAL) { out->P_M = OR_MIN_REAL; out->S_M = S_L = S_R = 0; }
+        rung = OR_RUNG_HLLC;
         if ((out->P_M <= 0) || (isnan(out->P_M)) || (out->P_M > ptl)) {
             double sqrt_rho_L = sqrt(L->rho);
             double sqrt_rho_R = sqrt(R->rho);
@@ -1038,6 +1053,7 @@ static void or_wavespeeds(const or_gamma *G, const or_rstate *L, const or_rstate
             out->S_M = ((rho_wt_R * vR + rho_wt_L * vL) + (PT_L - PT_R)) / (rho_wt_R + rho_wt_L);
             out->P_M = L->rho * (vL - S_L) * (vL - out->S_M) + PT_L;
             if (out->P_M <= OR_MIN_REAL) { out->P_M = OR_MIN_REAL; out->S_M = S_L = S_R = 0; }
+            rung = OR_RUNG_ROE;
             if ((out->P_M <= 0) || (isnan(out->P_M)) || (out->P_M > ptl)) {
                 out->P_M = 0.5 * ((PT_L + PT_R) + (vL - vR) * 0.25 * (L->rho + R->rho) * (csL + csR));
                 out->S_M = 0.5 * (vR + vL) + 2.0 * (PT_L - PT_R) / ((L->rho + R->rho) * (csL + csR));
@@ -1046,10 +1062,12 @@ static void or_wavespeeds(const or_gamma *G, const or_rstate *L, const or_rstate
                 if (out->S_M < S_L) out->S_M = S_L;
                 if (out->S_M > S_R) out->S_M = S_R;
                 if (out->P_M <= OR_MIN_REAL) { out->P_M = OR_MIN_REAL; out->S_M = S_L = S_R = 0; }
+                rung = OR_RUNG_PVRS;
             }
         }
     }
     (void)S_L; (void)S_R;
+    return rung;
 }
 
 /* Riemann_solver_KurganovTadmor_PWK, src/hydro/reimann.h:445-477 (MFM branch) */
@@ -1150,12 +1168,12 @@ static int or_iterative(const or_gamma *G, const or_rstate *L, const or_rstate *
 
 /* Riemann_solver_exact, src/hydro/reimann.h:625-668, MFM: only P_M and S_M change
    (sample_reimann_standard returns at once, the vacuum samplers return P_M = 0, :672-958). */
-static void or_exact(const or_gamma *G, const or_rstate *L, const or_rstate *R, double vL, double vR,
-                     double csL, double csR, or_rout *out)
+static unsigned or_exact(const or_gamma *G, const or_rstate *L, const or_rstate *R, double vL, double vR,
+                         double csL, double csR, or_rout *out)
 {
     if ((L->p == 0 && R->p == 0) || (L->rho == 0 && R->rho == 0)) {
         out->P_M = out->S_M = 0;
-        return;
+        return OR_RUNG_EXACT | OR_RUNG_EXACT_VACUUM;
     }
     if ((L->rho > 0) && (R->rho > 0)) {
         if (!or_iterative(G, L, R, vL, vR, csL, csR, out)) {
@@ -1165,34 +1183,72 @@ static void or_exact(const or_gamma *G, const or_rstate *L, const or_rstate *R, 
             if (0.0 <= S_L) { out->P_M = 0; out->S_M = vL + G->G4 * csL; }
             else if (0.0 >= S_R) { out->P_M = 0; out->S_M = vR - G->G4 * csR; }
             else { out->P_M = 0; out->S_M = 0.0; }
+            return OR_RUNG_EXACT | OR_RUNG_EXACT_VACUUM;
         }
-    } else {
-        if (L->rho > 0) { out->P_M = 0; out->S_M = vL + G->G4 * csL; }
-        if (R->rho > 0) { out->P_M = 0; out->S_M = vR - G->G4 * csR; }
+        return OR_RUNG_EXACT;
     }
+    if (L->rho > 0) { out->P_M = 0; out->S_M = vL + G->G4 * csL; }
+    if (R->rho > 0) { out->P_M = 0; out->S_M = vR - G->G4 * csR; }
+    return OR_RUNG_EXACT | OR_RUNG_EXACT_VACUUM;
 }
 
 /* Riemann_solver, src/hydro/reimann.h:282-370 (non-MHD, no EOS_GENERAL, no COOLING/GALSF),
-   with HLLC_Riemann_solver (:382-396, MFM fluxes).  Returns 0, or -1 on unphysical input
-   (P_M set to 0, fluxes left as they were). */
-static int or_riemann(const or_gamma *G, or_rstate L, or_rstate R, const double *n, double ptl, or_rout *out)
+   with HLLC_Riemann_solver (:382-396, MFM fluxes).  Returns the rungs taken; on unphysical
+   input (OR_RUNG_UNPHYSICAL) P_M is set to 0 and the fluxes are left as they were. */
+static void or_riemann_prep(const or_gamma *G, or_rstate *L, or_rstate *R, const double *n, double *hL, double *hR, double *vL,
+                            double *vR)
 {
-    if ((L.p < 0 && R.p < 0) || (L.rho < 0) || (R.rho < 0)) { out->P_M = 0; return -1; }
-    L.cs = sqrt(G->G0 * L.p / L.rho);
-    R.cs = sqrt(G->G0 * R.p / R.rho);
-    L.u = L.p / (G->G9 * L.rho);
-    R.u = R.p / (G->G9 * R.rho);
+    L->cs = sqrt(G->G0 * L->p / L->rho);
+    R->cs = sqrt(G->G0 * R->p / R->rho);
+    L->u = L->p / (G->G9 * L->rho);
+    R->u = R->p / (G->G9 * R->rho);
+    *hL = L->p / L->rho + L->u + 0.5 * (L->v[0] * L->v[0] + L->v[1] * L->v[1] + L->v[2] * L->v[2]);
+    *hR = R->p / R->rho + R->u + 0.5 * (R->v[0] * R->v[0] + R->v[1] * R->v[1] + R->v[2] * R->v[2]);
+    *vL = or_dot3(L->v, n);
+    *vR = or_dot3(R->v, n);
+}
+
+static unsigned or_riemann(const or_gamma *G, or_rstate L, or_rstate R, const double *n, double ptl, or_rout *out)
+{
+    if ((L.p < 0 && R.p < 0) || (L.rho < 0) || (R.rho < 0)) { out->P_M = 0; return OR_RUNG_UNPHYSICAL; }
+    double hL, hR, vL, vR;
+    or_riemann_prep(G, &L, &R, n, &hL, &hR, &vL, &vR);
     double csL = L.cs, csR = R.cs;
-    double hL = L.p / L.rho + L.u + 0.5 * (L.v[0] * L.v[0] + L.v[1] * L.v[1] + L.v[2] * L.v[2]);
-    double hR = R.p / R.rho + R.u + 0.5 * (R.v[0] * R.v[0] + R.v[1] * R.v[1] + R.v[2] * R.v[2]);
-    double vL = or_dot3(L.v, n);
-    double vR = or_dot3(R.v, n);
-    or_wavespeeds(G, &L, &R, n, vL, vR, csL, csR, hL, hR, out, ptl);
+    unsigned rung = or_wavespeeds(G, &L, &R, n, vL, vR, csL, csR, hL, hR, out, ptl);
     for (int k = 0; k < 3; k++) out->fv[k] = out->P_M * n[k];
     out->fp = out->P_M * out->S_M;
-    if ((out->P_M < 0) || (isnan(out->P_M))) or_kt(&L, &R, n, vL, vR, csL, csR, hL, hR, out);
-    if ((out->P_M < 0) || (isnan(out->P_M)) || (out->P_M > ptl)) or_exact(G, &L, &R, vL, vR, csL, csR, out);
-    return 0;
+    if ((out->P_M < 0) || (isnan(out->P_M))) {
+        or_kt(&L, &R, n, vL, vR, csL, csR, hL, hR, out);
+        rung |= OR_RUNG_KT;
+    }
+    if ((out->P_M < 0) || (isnan(out->P_M)) || (out->P_M > ptl)) rung |= or_exact(G, &L, &R, vL, vR, csL, csR, out);
+    return rung;
+}
+
+/* The Riemann solve of a face with the pair body's two retries
+   (src/hydro/hydro_core_meshless_functions.h:351-427): the reconstructed states, then the
+   first-order states (R = particle i, L = particle j, velocities in the face frame) at 1.4 times
+   the pressure limiter, then zero velocities at twice it.  Returns 1, or 0 when no positive
+   pressure was found (the reference's endrun(1234)); *rung receives the rungs taken. */
+static int or_ladder(const or_gamma *G, or_rstate Lst, or_rstate Rst, const double *n_unit, double ptl, double P_i, double P_j,
+                     double rho_i, double rho_j, const double *vi_f, const double *vj_f, or_rout *ro, unsigned *rung)
+{
+    memset(ro, 0, sizeof(*ro));
+    *rung |= or_riemann(G, Lst, Rst, n_unit, ptl, ro);
+    if ((ro->P_M < 0) || (isnan(ro->P_M)) || (ro->P_M > 1.4 * ptl)) {
+        Rst.p = P_i; Lst.p = P_j;
+        Rst.rho = rho_i; Lst.rho = rho_j;
+        for (int k = 0; k < 3; k++) { Rst.v[k] = vi_f[k]; Lst.v[k] = vj_f[k]; }
+        *rung |= OR_RUNG_RETRY1 | or_riemann(G, Lst, Rst, n_unit, 1.4 * ptl, ro);
+        if ((ro->P_M < 0) || (isnan(ro->P_M))) {
+            Rst.p = P_i; Lst.p = P_j;
+            Rst.rho = rho_i; Lst.rho = rho_j;
+            for (int k = 0; k < 3; k++) { Rst.v[k] = 0; Lst.v[k] = 0; }
+            *rung |= OR_RUNG_RETRY2 | or_riemann(G, Lst, Rst, n_unit, 2.0 * ptl, ro);
+            if ((ro->P_M < 0) || (isnan(ro->P_M))) { *rung |= OR_RUNG_FAIL; return 0; }
+        }
+    }
+    return 1;
 }
 
 int orc_riemann(double gamma, const double *left /*rho,p,vx,vy,vz*/, const double *right, const double *n,
@@ -1206,7 +1262,7 @@ int orc_riemann(double gamma, const double *left /*rho,p,vx,vy,vz*/, const doubl
     R.rho = right[0]; R.p = right[1]; R.v[0] = right[2]; R.v[1] = right[3]; R.v[2] = right[4];
     or_rout out;
     memset(&out, 0, sizeof(out));
-    int rc = or_riemann(&G, L, R, n, ptl, &out);
+    int rc = (or_riemann(&G, L, R, n, ptl, &out) & OR_RUNG_UNPHYSICAL) ? -1 : 0;
     *P_M = out.P_M; *S_M = out.S_M;
     fv[0] = out.fv[0]; fv[1] = out.fv[1]; fv[2] = out.fv[2];
     *fp = out.fp;
@@ -1231,6 +1287,55 @@ int orc_riemann_exact(double gamma, const double *left, const double *right, dou
     return 0;
 }
 
+/* The Riemann functions on given face states (the device's k_riemann_probe, same layout):
+   case k reads 24 values at in + 24k (left rho, p, v[3]; right rho, p, v[3]; n[3]; limiter;
+   p_i, p_j, rho_i, rho_j, v_i[3], v_j[3]) and writes P_M, S_M, fv[3], fp, ok, 0 at out + 8k and
+   the rung bits to rung[k].  mode 0 the ladder, 1 or_riemann, 2 or_wavespeeds, 3 or_kt,
+   4 or_exact. */
+int orc_riemann_probe(int mode, double gamma, long n, const double *in, double *out, unsigned *rung)
+{
+    or_gamma G;
+    or_gamma_init(&G, gamma);
+    if (mode < 0 || mode > 4) return -1;
+    for (long k = 0; k < n; k++) {
+        const double *c = in + 24 * k;
+        or_rstate L, R;
+        memset(&L, 0, sizeof(L)); memset(&R, 0, sizeof(R));
+        L.rho = c[0]; L.p = c[1]; L.v[0] = c[2]; L.v[1] = c[3]; L.v[2] = c[4];
+        R.rho = c[5]; R.p = c[6]; R.v[0] = c[7]; R.v[1] = c[8]; R.v[2] = c[9];
+        const double nv[3] = {c[10], c[11], c[12]};
+        const double ptl = c[13];
+        or_rout o;
+        memset(&o, 0, sizeof(o));
+        unsigned rb = 0;
+        int ok = 1;
+        if (mode == 0) {
+            ok = or_ladder(&G, L, R, nv, ptl, c[14], c[15], c[16], c[17], &c[18], &c[21], &o, &rb);
+        } else if (mode == 1) {
+            rb = or_riemann(&G, L, R, nv, ptl, &o);
+        } else {
+            double hL, hR, vL, vR;
+            or_riemann_prep(&G, &L, &R, nv, &hL, &hR, &vL, &vR);
+            if (mode == 2) rb = or_wavespeeds(&G, &L, &R, nv, vL, vR, L.cs, R.cs, hL, hR, &o, ptl);
+            else if (mode == 3) { or_kt(&L, &R, nv, vL, vR, L.cs, R.cs, hL, hR, &o); rb = OR_RUNG_KT; }
+            else rb = or_exact(&G, &L, &R, vL, vR, L.cs, R.cs, &o);
+        }
+        double *w = out + 8 * k;
+        w[0] = o.P_M; w[1] = o.S_M; w[2] = o.fv[0]; w[3] = o.fv[1]; w[4] = o.fv[2]; w[5] = o.fp; w[6] = ok; w[7] = 0;
+        rung[k] = rb;
+    }
+    return 0;
+}
+
+/* the rung counts of the force passes since the last reset (OR_RUNG_* order), then reset them if asked */
+void orc_rung_counts(long long *out, int reset)
+{
+    for (int b = 0; b < OR_RUNG_COUNT; b++) {
+        if (out) out[b] = or_rung_totals[b];
+        if (reset) or_rung_totals[b] = 0;
+    }
+}
+
 /* ------------------------------------------------------------------------- */
 /* force                                                                         */
 /* ------------------------------------------------------------------------- */
@@ -1247,8 +1352,9 @@ static inline double or_soundspeed(const orc_params *par, const orc_state *s, lo
    Returns 0 when the pair is skipped, 1 when fluxes were produced (possibly zero), -1234
    when every Riemann fallback failed (the reference's endrun(1234)). */
 static int or_force_pair(const orc_params *par, const or_gamma *G, const orc_state *s, long i, long j,
-                         double sound_i, double *Fv, double *Fp, double *vsig_ts)
+                         double sound_i, double *Fv, double *Fp, double *vsig_ts, unsigned *rung)
 {
+    *rung = 0;
     double box = par->box, half = 0.5 * par->box;
     Fv[0] = Fv[1] = Fv[2] = 0; *Fp = 0; *vsig_ts = 0;
     if (s->mass[j] <= 0) return 0;
@@ -1348,21 +1454,10 @@ static int or_force_pair(const orc_params *par, const or_gamma *G, const orc_sta
     if (recon_mode == 0)
         ptl = DMAX(ptl, DMAX(DMAX(Pressure_i, Pressure_j), 2. * DMAX(rho_i, rho_j) * v2_approach));
     or_rout ro;
-    memset(&ro, 0, sizeof(ro));
-    or_riemann(G, Lst, Rst, n_unit, ptl, &ro);
-    if ((ro.P_M < 0) || (isnan(ro.P_M)) || (ro.P_M > 1.4 * ptl)) {
-        Rst.p = Pressure_i; Lst.p = Pressure_j;
-        Rst.rho = rho_i; Lst.rho = rho_j;
-        for (int k = 0; k < 3; k++) { Rst.v[k] = vi[k] - v_frame[k]; Lst.v[k] = vj[k] - v_frame[k]; }
-        or_riemann(G, Lst, Rst, n_unit, 1.4 * ptl, &ro);
-        if ((ro.P_M < 0) || (isnan(ro.P_M))) {
-            Rst.p = Pressure_i; Lst.p = Pressure_j;
-            Rst.rho = rho_i; Lst.rho = rho_j;
-            for (int k = 0; k < 3; k++) { Rst.v[k] = 0; Lst.v[k] = 0; }
-            or_riemann(G, Lst, Rst, n_unit, 2.0 * ptl, &ro);
-            if ((ro.P_M < 0) || (isnan(ro.P_M))) return -1234;
-        }
-    }
+    double vi_f[3], vj_f[3];
+    for (int k = 0; k < 3; k++) { vi_f[k] = vi[k] - v_frame[k]; vj_f[k] = vj[k] - v_frame[k]; }
+    *rung = OR_RUNG_FACE;
+    if (!or_ladder(G, Lst, Rst, n_unit, ptl, Pressure_i, Pressure_j, rho_i, rho_j, vi_f, vj_f, &ro, rung)) return -1234;
     if (!((ro.P_M > 0) && (!isnan(ro.P_M)))) return 1; /* no valid solution: zero fluxes */
     ro.fp += or_dot3(v_frame, ro.fv);
     *Fp = Anorm * ro.fp;
@@ -1371,6 +1466,7 @@ static int or_force_pair(const orc_params *par, const or_gamma *G, const orc_sta
     {
         double SM_over_ceff = fabs(ro.S_M) / DMIN(sound_i, sound_j);
         if ((SM_over_ceff < 0.5) || (leak_vs_tol > 1)) {
+            *rung |= OR_ENTROPIC_CONSIDERED;
             int use = 1;
             double facenorm_pm = ro.P_M * Anorm;
             double PdV_fac = ro.P_M * vdotr2_phys / 1.0;
@@ -1398,7 +1494,10 @@ static int or_force_pair(const orc_params *par, const or_gamma *G, const orc_sta
                 }
             }
             if (cnum2 >= cnumcrit2) use = 1;
-            if (use) *Fp += du_new - du_old;
+            if (use) {
+                *Fp += du_new - du_old;
+                *rung |= OR_ENTROPIC_APPLIED;
+            }
         }
     }
     return 1;
@@ -1436,6 +1535,7 @@ int orc_force_ex(const orc_params *par, orc_state *s, const unsigned char *activ
 #pragma omp parallel
     {
         int *buf = (int *)malloc((size_t)n * sizeof(int));
+        long long rungs[OR_RUNG_COUNT] = {0};
 #pragma omp for schedule(dynamic, 64)
         for (long i = 0; i < n; i++) {
             if (active && !active[i]) continue;
@@ -1461,7 +1561,9 @@ int orc_force_ex(const orc_params *par, orc_state *s, const unsigned char *activ
                 long j = buf[q];
                 if (msv < sound_i) msv = sound_i;
                 double Fv[3], Fp, vts;
-                int rc = or_force_pair(par, &G, s, i, j, sound_i, Fv, &Fp, &vts);
+                unsigned rung;
+                int rc = or_force_pair(par, &G, s, i, j, sound_i, Fv, &Fp, &vts, &rung);
+                for (int b = 0; b < OR_RUNG_COUNT; b++) rungs[b] += (rung >> b) & 1u;
                 if (rc == -1234) {
 #pragma omp atomic write
                     status = 1234;
@@ -1498,6 +1600,10 @@ int orc_force_ex(const orc_params *par, orc_state *s, const unsigned char *activ
                 s->dtu[i] /= s->mass[i];
                 if (s->h[i] >= 0.99 * par->max_kernel_radius) s->dtu[i] = 0;
             }
+        }
+        for (int b = 0; b < OR_RUNG_COUNT; b++) {
+#pragma omp atomic
+            or_rung_totals[b] += rungs[b];
         }
         free(buf);
     }

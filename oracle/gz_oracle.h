@@ -92,6 +92,8 @@ void orc_reconstruct(double Q_i, const double *Gi, double Q_j, const double *Gj,
 int orc_riemann(double gamma, const double *left, const double *right, const double *n,
                 double ptl, double *P_M, double *S_M, double *fv, double *fp);
 int orc_riemann_exact(double gamma, const double *left, const double *right, double *P_M, double *S_M);
+int orc_riemann_probe(int mode, double gamma, long n, const double *in, double *out, unsigned *rung);
+void orc_rung_counts(long long *out, int reset);
 int orc_force(const orc_params *par, orc_state *s, const unsigned char *active);
 int orc_force_ex(const orc_params *par, orc_state *s, const unsigned char *active, int *need_wakeup);
 long long orc_get_timestep(const orc_params *par, const orc_integ *ip, const orc_timeline *tl, const orc_state *s, long i);

@@ -90,6 +90,9 @@ def lib():
         L.orc_reconstruct.argtypes = [C.c_double, _dp, C.c_double, _dp, _dp, _dp, _dp, _dp, C.c_int]
         L.orc_riemann.argtypes = [C.c_double, _dp, _dp, _dp, C.c_double, _dp, _dp, _dp, _dp]
         L.orc_riemann_exact.argtypes = [C.c_double, _dp, _dp, _dp, _dp]
+        L.orc_riemann_probe.argtypes = [C.c_int, C.c_double, C.c_long, _dp, _dp, C.POINTER(C.c_uint)]
+        L.orc_rung_counts.argtypes = [_llp, C.c_int]
+        L.orc_rung_counts.restype = None
         L.orc_force.argtypes = [C.POINTER(Params), C.POINTER(_CState), _up]
         L.orc_force_ex.argtypes = [C.POINTER(Params), C.POINTER(_CState), _up, _ip]
         L.orc_get_timestep.argtypes = [C.POINTER(Params), C.POINTER(Integ), C.POINTER(Timeline), C.POINTER(_CState), C.c_long]
@@ -273,6 +276,33 @@ def riemann_exact(gamma, left, right):
     PM, SM = C.c_double(), C.c_double()
     lib().orc_riemann_exact(gamma, _ptr(L), _ptr(R), C.byref(PM), C.byref(SM))
     return PM.value, SM.value
+
+
+# the rungs of the Riemann ladder, in bit order (the device's gz_stat names)
+RUNGS = ("riemann_hllc", "riemann_vacuum", "riemann_roe", "riemann_pvrs", "riemann_kt", "riemann_exact",
+         "riemann_exact_vacuum", "riemann_unphysical", "riemann_retry_first_order", "riemann_retry_zero_velocity",
+         "riemann_fail", "riemann_faces", "entropic_considered", "entropic_applied")
+PROBE_IN, PROBE_OUT = 24, 8
+
+
+def riemann_probe(mode, gamma, cases):
+    """The Riemann functions on face states (n x 24 cases, the layout of gz_riemann_probe):
+    returns (out n x 8, rung bits n)."""
+    c = np.ascontiguousarray(cases, dtype=np.float64).reshape(-1, PROBE_IN)
+    n = len(c)
+    out = np.zeros((n, PROBE_OUT))
+    rung = np.zeros(n, dtype=np.uint32)
+    rc = lib().orc_riemann_probe(int(mode), float(gamma), n, _ptr(c), _ptr(out), rung.ctypes.data_as(C.POINTER(C.c_uint)))
+    if rc:
+        raise ValueError("orc_riemann_probe mode")
+    return out, rung
+
+
+def rung_counts(reset=True):
+    """The force passes' rung counts since the last reset, by name."""
+    v = np.zeros(len(RUNGS), dtype=np.int64)
+    lib().orc_rung_counts(_ptr(v, _llp), int(reset))
+    return {k: int(x) for k, x in zip(RUNGS, v)}
 
 
 def hydro_pass(par, st, active=None):

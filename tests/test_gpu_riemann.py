@@ -1,0 +1,138 @@
+"""The device's Riemann ladder rung by rung against the oracle (whose rungs
+tests/test_oracle_riemann.py pins against closed forms).
+
+  * gz_riemann_probe runs each function of the ladder (the pair body's ladder with its two
+    retries, Riemann_solver, the wave-speed estimates, the Kurganov-Tadmor flux, the exact
+    solver) on crafted and random face states that reach every rung, the stop included: in FP64
+    every output equals the oracle's bit for bit (NaN where the oracle has NaN) and every case
+    takes the same rungs;
+  * the FP32-mixed form takes the same rungs as the FP64 oracle on the cases whose estimate
+    does not sit at a branch threshold, with the star pressure and speed within the stated bound;
+  * a whole force pass on evolved states (a Sedov blast after 30 adaptive steps, and colliding
+    Mach 5 streams whose separating faces open a vacuum) counts, face by face, the same rungs and
+    entropic-switch decisions as the oracle's pass on the same inputs, through both kernel shapes.
+"""
+import numpy as np
+import pytest
+
+import riemann_cases as RC
+from gizmo_amd import ics
+from gizmo_amd.hydro import GZ_FP32_MIXED, GZ_FP64, Hydro
+
+pytestmark = pytest.mark.gpu
+
+GAMMA = 5.0 / 3.0
+MODES = {0: "ladder", 1: "riemann_solver", 2: "wavespeeds", 3: "kurganov_tadmor", 4: "exact"}
+
+
+def _ctx(precision, gamma=GAMMA):
+    par = dict(box=1.0, kernel=6, gamma=gamma, des_num_ngb=64.0, max_num_ngb_dev=0.1, min_kernel_radius=0.0,
+               max_kernel_radius=0.25, time=0.0, time_begin=0.0)
+    return Hydro(par, 16, precision)
+
+
+@pytest.mark.parametrize("mode", list(MODES))
+def test_probe_fp64_bitwise(oracle_lib, mode):
+    O = oracle_lib
+    cases = RC.all_cases(4000)
+    print(f"probe fp64 mode={mode} ({MODES[mode]}) cases={len(cases)}", flush=True)
+    g = _ctx(GZ_FP64)
+    dev, rd = g.riemann_probe(mode, cases)
+    g.close()
+    ora, ro = O.riemann_probe(mode, GAMMA, cases)
+    assert np.array_equal(rd, ro)
+    same = (dev == ora) | (np.isnan(dev) & np.isnan(ora))
+    assert same.all(), np.argwhere(~same)[:5]
+    if mode == 0:
+        hit = {name for b, name in enumerate(O.RUNGS[:11]) if ((rd >> b) & 1).any()}
+        assert hit == set(O.RUNGS[:11]), set(O.RUNGS[:11]) - hit
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2, 4])
+def test_probe_fp32(oracle_lib, mode):
+    """FP32 arithmetic from the same face states: the same rung wherever the FP64 estimate is
+    not within FP32 rounding of the limiter, and P_M, S_M within 1e-4 of the state's pressure
+    and signal-speed scales (the FP32 form's pair arithmetic; its whole-pass error is measured
+    against the oracle in the parity tests)."""
+    O = oracle_lib
+    cases = RC.random_cases(4000, seed=17)
+    cases = cases[(cases[:, 1] / cases[:, 6] < 1e4) & (cases[:, 6] / cases[:, 1] < 1e4)
+                  & (cases[:, 0] / cases[:, 5] < 1e4) & (cases[:, 5] / cases[:, 0] < 1e4)]
+    g = _ctx(GZ_FP32_MIXED)
+    dev, rd = g.riemann_probe(mode, cases)
+    g.close()
+    ora, ro = O.riemann_probe(mode, GAMMA, cases)
+    same = rd == ro
+    frac = same.mean()
+    pscale = np.maximum(cases[:, 1], cases[:, 6])
+    cs = np.sqrt(GAMMA * np.maximum(cases[:, 1] / cases[:, 0], cases[:, 6] / cases[:, 5]))
+    vscale = cs + np.abs(cases[:, 2:5]).max(1) + np.abs(cases[:, 7:10]).max(1)
+    ep = np.abs(dev[same, 0] - ora[same, 0]) / pscale[same]
+    es = np.abs(dev[same, 1] - ora[same, 1]) / vscale[same]
+    print(f"probe fp32 mode={mode}: same rung {frac:.4f}, P_M err max {ep.max():.2e} p99 {np.quantile(ep, 0.99):.2e}, "
+          f"S_M err max {es.max():.2e} p99 {np.quantile(es, 0.99):.2e}", flush=True)
+    assert frac >= 0.98
+    assert np.quantile(ep, 0.99) <= 1e-5 and np.quantile(es, 0.99) <= 1e-5
+    assert ep.max() <= 1e-3 and es.max() <= 1e-3
+
+
+def _colliding_streams(n_side, mach, seed):
+    ic = ics.lattice_box(n_side, seed, kernel=6, des_num_ngb=64.0, mach_sigma=0.05)
+    cs = np.sqrt(GAMMA * 1.0 / 1.0)
+    x = ic["pos"][:, 0]
+    ic["vel"][:, 0] += np.where(x < 0.5, mach * cs, -mach * cs)
+    return ic
+
+
+def _evolved_sedov(O, n_side, steps):
+    ic = ics.sedov_box(n_side, 1002, kernel=6, des_num_ngb=64.0)
+    par = dict(ic["params"])
+    st = O.State.from_ic(ic)
+    r = O.Run(par, dict(courant_fac=0.05, err_tol_int_accuracy=0.002, max_size_timestep=0.01, min_size_timestep=0.0,
+                        time_begin=0.0, time_max=1.0, min_egy_spec=0.0), st)
+    r.begin()
+    for _ in range(steps):
+        r.step()
+    par["time"] = r.time
+    return par, st
+
+
+@pytest.mark.parametrize("state", ["sedov_evolved", "colliding_streams"])
+@pytest.mark.parametrize("few_rows", [0, 2_000_000_000])
+def test_pass_rung_counts_match_oracle(oracle_lib, state, few_rows):
+    O = oracle_lib
+    if state == "sedov_evolved":
+        par, s = _evolved_sedov(O, 16, 30)
+        s.timebin[:] = 0
+        s.wakeup[:] = 0
+    else:
+        ic = _colliding_streams(16, 5.0, 404)
+        par = dict(ic["params"])
+        s = O.State.from_ic(ic)
+        O.density(par, s)
+        O.gradients(par, s)
+    print(f"rung counts: {state} n={s.n} few_rows={few_rows}", flush=True)
+    so = s.copy()
+    O.rung_counts(reset=True)
+    O.force(par, so)
+    want = O.rung_counts(reset=True)
+    from oracle.oracle import FIELDS, INT_FIELDS
+    f = {k: getattr(s, k) for k in list(FIELDS) + list(INT_FIELDS)}
+    for k in ("accel", "dtu", "maxsig"):
+        f[k] = None
+    g = Hydro(par, s.n, GZ_FP64)
+    g.set_option("few_rows", few_rows)
+    g.load(f)
+    g.set_option("riemann_stats", 1)
+    g.hydro_force()
+    got = g.rung_counts()
+    out = g.store(["accel", "dtu"])
+    g.close()
+    print(state, {k: v for k, v in got.items() if v})
+    assert got == want
+    assert got["riemann_faces"] > 0 and got["riemann_hllc"] > 0
+    assert got["entropic_considered"] > 0 and got["entropic_applied"] > 0
+    if state == "colliding_streams":
+        assert got["riemann_vacuum"] > 0
+    # the counting kernels compute the same fluxes
+    np.testing.assert_allclose(out["accel"], so.accel, rtol=0, atol=1e-12 * np.abs(so.accel).max())

@@ -1,0 +1,233 @@
+"""Pins of the oracle's Riemann ladder rung by rung (src/hydro/reimann.h and the pair body's
+retries, src/hydro/hydro_core_meshless_functions.h:351-427), each against what the mathematics
+fixes rather than against the oracle's own formula:
+
+  * the first HLLC estimate is exact on a contact discontinuity (P* = p, S* = v.n);
+  * the Roe rung is Toro's HLLC star state (Toro 2009, eqs. 10.36-10.37) with Einfeldt's
+    Roe-averaged wave speeds, written here in Toro's notation;
+  * the primitive-variable rung is Toro's PVRS (eq. 9.20) with the signal-speed clamp, and it
+    agrees with the exact solver to second order in the wave strength;
+  * the Kurganov-Tadmor flux of equal states is the exact flux of the face: p n and p v.n;
+  * every function is symmetric under swapping the states and reversing the normal;
+  * the exact solver falls back to its vacuum sample exactly when the states separate faster
+    than the vacuum-generation speed 2(a_L + a_R)/(gamma - 1) (Toro eq. 4.40);
+  * the ladder's retries equal direct solves of the first-order and zero-velocity states at
+    1.4 and 2 times the limiter;
+  * every rung, including the failure that stops the run (1234), is reached by some case.
+"""
+import numpy as np
+import pytest
+
+import riemann_cases as RC
+
+GAMMA = 5.0 / 3.0
+MIN_REAL = 1e-56
+
+
+def _bits(rung, name, O):
+    return ((rung >> O.RUNGS.index(name)) & 1).astype(bool)
+
+
+def _prep(c, gamma=GAMMA):
+    """Sound speeds, total enthalpies and normal velocities of the face states (numpy)."""
+    L, R, n = c[:, 0:5], c[:, 5:10], c[:, 10:13]
+    csL = np.sqrt(gamma * L[:, 1] / L[:, 0])
+    csR = np.sqrt(gamma * R[:, 1] / R[:, 0])
+    HL = L[:, 1] / L[:, 0] * gamma / (gamma - 1) + 0.5 * (L[:, 2:5] ** 2).sum(1)
+    HR = R[:, 1] / R[:, 0] * gamma / (gamma - 1) + 0.5 * (R[:, 2:5] ** 2).sum(1)
+    vL = (L[:, 2:5] * n).sum(1)
+    vR = (R[:, 2:5] * n).sum(1)
+    return L, R, n, csL, csR, HL, HR, vL, vR
+
+
+def test_every_rung_is_reached(oracle_lib):
+    O = oracle_lib
+    c = RC.all_cases()
+    out, rung = O.riemann_probe(0, GAMMA, c)
+    for b, name in enumerate(O.RUNGS[:11]):
+        assert ((rung >> b) & 1).any(), f"no case reaches {name}"
+    # the failure is reported as such (no positive pressure: the reference's endrun(1234))
+    fail = _bits(rung, "riemann_fail", O)
+    assert (out[fail, 6] == 0).all() and (out[~fail, 6] == 1).all()
+
+
+def test_contact_is_exact_in_the_first_estimate(oracle_lib):
+    O = oracle_lib
+    rng = np.random.default_rng(3)
+    m = 200
+    c = np.zeros((m, RC.IN))
+    n = RC.unit(rng, m)
+    p = np.exp(rng.uniform(-5, 5, m))
+    vn = rng.normal(size=m)
+    for side in (0, 5):
+        c[:, side] = np.exp(rng.uniform(-4, 4, m))
+        c[:, side + 1] = p
+        tang = rng.normal(size=(m, 3))
+        tang -= (tang * n).sum(1, keepdims=True) * n
+        c[:, side + 2:side + 5] = vn[:, None] * n + tang
+    c[:, 10:13] = n
+    c[:, 13] = 10 * p
+    out, rung = O.riemann_probe(1, GAMMA, c)
+    assert _bits(rung, "riemann_hllc", O).all()
+    np.testing.assert_allclose(out[:, 0], p, rtol=1e-13)
+    np.testing.assert_allclose(out[:, 1], vn, rtol=1e-12, atol=1e-13 * np.abs(vn).max())
+
+
+def test_roe_rung_is_toro_hllc_with_einfeldt_speeds(oracle_lib):
+    O = oracle_lib
+    c = RC.random_cases(20000, seed=5)
+    out, rung = O.riemann_probe(2, GAMMA, c)
+    sel = _bits(rung, "riemann_roe", O) & (out[:, 0] > MIN_REAL)
+    assert sel.sum() >= 50
+    c, out = c[sel], out[sel]
+    L, R, n, csL, csR, HL, HR, vL, vR = _prep(c)
+    sL, sR = np.sqrt(L[:, 0]), np.sqrt(R[:, 0])
+    u_roe = (sL[:, None] * L[:, 2:5] + sR[:, None] * R[:, 2:5]) / (sL + sR)[:, None]
+    H_roe = (sL * HL + sR * HR) / (sL + sR)
+    a_roe = np.sqrt(np.maximum(1e-30, (GAMMA - 1) * (H_roe - 0.5 * (u_roe ** 2).sum(1))))
+    un = (u_roe * n).sum(1)
+    SL = np.minimum(vL - csL, un - a_roe)
+    SR = np.maximum(vR + csR, un + a_roe)
+    # Toro (10.37) and the HLLC star pressure p_L + rho_L (S_L - u_L)(S* - u_L)
+    Sstar = (R[:, 1] - L[:, 1] + L[:, 0] * vL * (SL - vL) - R[:, 0] * vR * (SR - vR)) / (L[:, 0] * (SL - vL) - R[:, 0] * (SR - vR))
+    Pstar = L[:, 1] + L[:, 0] * (SL - vL) * (Sstar - vL)
+    scale = np.maximum(L[:, 1], R[:, 1])
+    assert np.all(np.abs(out[:, 0] - Pstar) <= 1e-9 * scale + 1e-9 * np.abs(Pstar))
+    vscale = np.maximum(np.abs(SL), np.abs(SR))
+    assert np.all(np.abs(out[:, 1] - Sstar) <= 1e-9 * vscale)
+
+
+def test_pvrs_rung_is_toro_pvrs(oracle_lib):
+    O = oracle_lib
+    c = RC.random_cases(4000, seed=9)
+    c[:, 13] = 1e-300  # a limiter below every estimate: the primitive-variable estimate is taken
+    out, rung = O.riemann_probe(2, GAMMA, c)
+    pv = _bits(rung, "riemann_pvrs", O)
+    assert pv.sum() > 0.5 * len(c)
+    c, out = c[pv], out[pv]
+    L, R, n, csL, csR, HL, HR, vL, vR = _prep(c)
+    rho_bar = 0.5 * (L[:, 0] + R[:, 0])
+    a_bar = 0.5 * (csL + csR)
+    p_pvrs = 0.5 * (L[:, 1] + R[:, 1]) - 0.5 * (vR - vL) * rho_bar * a_bar
+    u_pvrs = 0.5 * (vL + vR) - 0.5 * (R[:, 1] - L[:, 1]) / (rho_bar * a_bar)
+    s_plus = np.maximum.reduce([np.abs(vL - csL), np.abs(vR - csR), np.abs(vL + csL), np.abs(vR + csR)])
+    u_pvrs = np.clip(u_pvrs, -s_plus, s_plus)
+    floor = p_pvrs <= MIN_REAL
+    p_pvrs[floor] = MIN_REAL
+    u_pvrs[floor] = 0
+    scale = np.maximum(L[:, 1], R[:, 1])
+    assert np.all(np.abs(out[:, 0] - p_pvrs) <= 1e-12 * scale)
+    assert np.all(np.abs(out[:, 1] - u_pvrs) <= 1e-12 * (s_plus + np.abs(u_pvrs)))
+
+
+def test_pvrs_agrees_with_exact_solver_for_weak_waves(oracle_lib):
+    O = oracle_lib
+    rng = np.random.default_rng(4)
+    for eps in (1e-2, 1e-3):
+        m = 100
+        c = np.zeros((m, RC.IN))
+        c[:, 0] = 1.0 + eps * rng.uniform(-1, 1, m)
+        c[:, 1] = 1.0 + eps * rng.uniform(-1, 1, m)
+        c[:, 5] = 1.0 + eps * rng.uniform(-1, 1, m)
+        c[:, 6] = 1.0 + eps * rng.uniform(-1, 1, m)
+        c[:, 10] = 1.0
+        c[:, 2] = eps * rng.uniform(-1, 1, m)
+        c[:, 7] = eps * rng.uniform(-1, 1, m)
+        c[:, 13] = 1e-300
+        pv, _ = O.riemann_probe(2, GAMMA, c)
+        ex, _ = O.riemann_probe(4, GAMMA, c)
+        assert np.abs(pv[:, 0] - ex[:, 0]).max() < 4 * eps ** 2
+        assert np.abs(pv[:, 1] - ex[:, 1]).max() < 4 * eps ** 2
+
+
+def test_kt_flux_of_equal_states(oracle_lib):
+    O = oracle_lib
+    rng = np.random.default_rng(8)
+    m = 100
+    c = np.zeros((m, RC.IN))
+    rho, p = np.exp(rng.uniform(-3, 3, m)), np.exp(rng.uniform(-3, 3, m))
+    v = rng.normal(size=(m, 3))
+    n = RC.unit(rng, m)
+    c[:, 0] = c[:, 5] = rho
+    c[:, 1] = c[:, 6] = p
+    c[:, 2:5] = c[:, 7:10] = v
+    c[:, 10:13] = n
+    out, _ = O.riemann_probe(3, GAMMA, c)
+    vn = (v * n).sum(1)
+    cs = np.sqrt(GAMMA * p / rho)
+    np.testing.assert_allclose(out[:, 0], p, rtol=1e-13)
+    np.testing.assert_allclose(out[:, 2:5], p[:, None] * n, rtol=1e-12, atol=1e-14 * p.max())
+    np.testing.assert_allclose(out[:, 5], p * vn, rtol=1e-12, atol=1e-14 * (p * np.abs(vn)).max())
+    np.testing.assert_allclose(out[:, 1], cs + np.abs(vn), rtol=1e-14)
+
+
+@pytest.mark.parametrize("mode", [1, 2, 3, 4, 0])
+def test_mirror_symmetry(oracle_lib, mode):
+    O = oracle_lib
+    c = RC.random_cases(3000, seed=21)
+    m = c.copy()
+    m[:, 0:5], m[:, 5:10] = c[:, 5:10], c[:, 0:5]
+    m[:, 10:13] = -c[:, 10:13]
+    m[:, 14], m[:, 15], m[:, 16], m[:, 17] = c[:, 15], c[:, 14], c[:, 17], c[:, 16]
+    m[:, 18:21], m[:, 21:24] = c[:, 21:24], c[:, 18:21]
+    a, ra = O.riemann_probe(mode, GAMMA, c)
+    b, rb = O.riemann_probe(mode, GAMMA, m)
+    ok = np.isfinite(a).all(1) & np.isfinite(b).all(1)
+    agree = ok & (ra == rb)
+    # the same rung on both sides, but for the faces whose estimate sits at the limiter
+    assert agree.sum() >= 0.99 * ok.sum()
+    a, b, c = a[agree], b[agree], c[agree]
+    scale = np.maximum(np.maximum(c[:, 1], c[:, 6]), np.abs(a[:, 0]))
+    vs = np.sqrt(GAMMA * np.maximum(c[:, 1] / c[:, 0], c[:, 6] / c[:, 5])) + np.abs(c[:, 2:5]).max(1) + np.abs(c[:, 7:10]).max(1)
+    assert np.all(np.abs(a[:, 0] - b[:, 0]) <= 1e-9 * scale)
+    sgn = 1.0 if mode == 3 else -1.0  # the KT speed is the largest signal speed, not the contact's
+    assert np.all(np.abs(a[:, 1] - sgn * b[:, 1]) <= 1e-9 * vs)
+    if mode in (0, 1, 3):
+        assert np.all(np.abs(a[:, 2:5] + b[:, 2:5]) <= 1e-9 * scale[:, None])
+        assert np.all(np.abs(a[:, 5] + b[:, 5]) <= 1e-9 * scale * vs)
+
+
+def test_exact_solver_vacuum_condition(oracle_lib):
+    O = oracle_lib
+    rng = np.random.default_rng(12)
+    m = 2000
+    c = np.zeros((m, RC.IN))
+    c[:, 0] = np.exp(rng.uniform(-2, 2, m)); c[:, 1] = np.exp(rng.uniform(-2, 2, m))
+    c[:, 5] = np.exp(rng.uniform(-2, 2, m)); c[:, 6] = np.exp(rng.uniform(-2, 2, m))
+    c[:, 10] = 1.0
+    csL = np.sqrt(GAMMA * c[:, 1] / c[:, 0])
+    csR = np.sqrt(GAMMA * c[:, 6] / c[:, 5])
+    crit = 2.0 / (GAMMA - 1.0) * (csL + csR)
+    du = crit * rng.uniform(0.5, 1.5, m)
+    du = np.where(np.abs(du - crit) < 1e-6 * crit, 1.2 * crit, du)
+    c[:, 2] = -0.5 * du
+    c[:, 7] = 0.5 * du
+    out, rung = O.riemann_probe(4, GAMMA, c)
+    vac = _bits(rung, "riemann_exact_vacuum", O)
+    assert np.array_equal(vac, du > crit)
+    assert (out[vac, 0] == 0).all() and (out[~vac, 0] > 0).all()
+
+
+def test_retries_are_direct_solves(oracle_lib):
+    O = oracle_lib
+    c = RC.all_cases(6000, seed=31)
+    out, rung = O.riemann_probe(0, GAMMA, c)
+    r1 = _bits(rung, "riemann_retry_first_order", O)
+    r2 = _bits(rung, "riemann_retry_zero_velocity", O)
+    one = r1 & ~r2
+    assert one.sum() > 100 and r2.sum() >= 2
+    # first-order states: right = particle i, left = particle j, limiter x1.4
+    d = c[one].copy()
+    d[:, 6], d[:, 1], d[:, 5], d[:, 0] = c[one, 14], c[one, 15], c[one, 16], c[one, 17]
+    d[:, 7:10], d[:, 2:5] = c[one, 18:21], c[one, 21:24]
+    d[:, 13] = 1.4 * c[one, 13]
+    direct, _ = O.riemann_probe(1, GAMMA, d)
+    assert np.array_equal(np.nan_to_num(out[one, :6], nan=7.0), np.nan_to_num(direct[:, :6], nan=7.0))
+    # zero velocities at twice the limiter
+    d = c[r2].copy()
+    d[:, 6], d[:, 1], d[:, 5], d[:, 0] = c[r2, 14], c[r2, 15], c[r2, 16], c[r2, 17]
+    d[:, 2:5] = 0
+    d[:, 7:10] = 0
+    d[:, 13] = 2.0 * c[r2, 13]
+    direct, _ = O.riemann_probe(1, GAMMA, d)
+    assert np.array_equal(np.nan_to_num(out[r2, :6], nan=7.0), np.nan_to_num(direct[:, :6], nan=7.0))
