@@ -10,7 +10,7 @@ without the built library every call raises.
 """
 import ctypes as C
 
-from .hydro import (GZ_FP64, Fields, GizmoHipError, IntegParams, Params, Timeline, fields_struct, lib, make_params,
+from .hydro import (GZ_FP64, Fields, GizmoHipError, IntegParams, Params, Timeline, fields_struct, integ_params, lib, make_params,
                     output_struct, timeline_dict)
 
 
@@ -48,7 +48,7 @@ def blib():
 
 def box_params(params, integ):
     """gz_box_params from a parameter dict (box, min/max_kernel_radius) and an integ dict."""
-    ip = IntegParams(**{k: float(integ[k]) for k, _ in IntegParams._fields_})
+    ip = integ_params(integ)
     return BoxParams(box_size=float(params["box"]), min_kernel_radius=float(params["min_kernel_radius"]),
                      max_kernel_radius=float(params["max_kernel_radius"]), integ=ip)
 

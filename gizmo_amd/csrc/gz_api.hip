@@ -30,6 +30,9 @@ namespace {
 
 constexpr int kBlock = 256;
 inline unsigned nblk(long n) { return (unsigned)((n + kBlock - 1) / kBlock); }
+// steps with more active particles than this drift every particle (the cells around that many
+// rows cover most of a box: one sweep costs less than walking them)
+constexpr long long kLazyMaxRows = 65536;
 
 }  // namespace
 
@@ -39,7 +42,7 @@ struct gz_context {
     int device = 0;
     hipStream_t stream = nullptr;
     long cap = 0, n = 0;
-    std::string err;
+    char err[512] = {0};  // the last error message (a fixed buffer: reporting an error allocates nothing)
 
     // particle state, device order (SoA; vectors with column stride n)
     double *X = nullptr, *Y = nullptr, *Z = nullptr, *VX = nullptr, *VY = nullptr, *VZ = nullptr;
@@ -51,12 +54,17 @@ struct gz_context {
     // timestep-loop fields: kicked velocity and energy, softening cache, step start and length
     double *VKX = nullptr, *VKY = nullptr, *VKZ = nullptr, *UK = nullptr, *SOFT = nullptr;
     long long *TIB = nullptr, *DTS = nullptr;
+    long long *TIC = nullptr;  // P.Ti_current: the time each particle's predicted fields stand at
     // positions when the particles were last ordered, and the largest periodic distance any
     // particle has moved since (the search grid widens its cells by twice that)
     // dmax: a bound on how far any particle moved since the last ordering (the sum of each
     // step's longest drift along an axis)
     double dmax = 0, last_rmax = 0;
     bool overflow_clear = false;  // counters[3] (the symmetric list's overflow) known to be zero on the stream
+    // the step in progress drifts lazily: the passes drift the cells around the active rows that
+    // they read (the particles there that stand behind All.Ti_Current) after building their grid
+    bool lazy = false;
+    bool last_full = true;  // the last step drifted every particle
     bool density_ready = false;   // the active list's launch set density()'s iteration state for its rows
     double resort_fraction = 0.125;  // order again once dmax exceeds this fraction of the search radius
     int few_rows = 8192;             // at most this many active rows: one wavefront per row
@@ -174,7 +182,7 @@ int fail(gz_context *c, int code, const char *fmt, ...)
     va_start(ap, fmt);
     vsnprintf(buf, sizeof buf, fmt, ap);
     va_end(ap);
-    if (c) c->err = buf;
+    if (c) snprintf(c->err, sizeof c->err, "%s", buf);
     return code;
 }
 
@@ -329,6 +337,7 @@ int alloc_state(gz_context *ctx, long cap)
     CHK(dalloc(ctx, (char **)&ctx->rec_force, n * sizeof(Cell<double>)));
     CHK(dalloc(ctx, &ctx->VKX, n)); CHK(dalloc(ctx, &ctx->VKY, n)); CHK(dalloc(ctx, &ctx->VKZ, n));
     CHK(dalloc(ctx, &ctx->UK, n)); CHK(dalloc(ctx, &ctx->SOFT, n)); CHK(dalloc(ctx, &ctx->TIB, n)); CHK(dalloc(ctx, &ctx->DTS, n));
+    CHK(dalloc(ctx, &ctx->TIC, n));
     // the 64-bit counters and maxima, then the 16 int counters, in one allocation: one copy
     // reads a stage's results back
     CHK(dalloc(ctx, &ctx->u64c, 80 + 8));
@@ -375,7 +384,7 @@ void free_state(gz_context *c)
     dfree(c->cstart); dfree(c->cend); dfree(c->nsten); dfree(c->sten); dfree(c->chmax);
     dfree(c->ca0); dfree(c->ca1); dfree(c->cells); dfree(c->cflag); dfree(c->cscan); dfree(c->need);
     dfree(c->rec_grad); dfree(c->rec_force); dfree(c->dacc);
-    dfree(c->VKX); dfree(c->VKY); dfree(c->VKZ); dfree(c->UK); dfree(c->SOFT); dfree(c->TIB); dfree(c->DTS);
+    dfree(c->VKX); dfree(c->VKY); dfree(c->VKZ); dfree(c->UK); dfree(c->SOFT); dfree(c->TIB); dfree(c->DTS); dfree(c->TIC);
     dfree(c->u64c);
     dfree(c->rungs);
     if (c->hu64c) hipHostFree(c->hu64c);
@@ -493,6 +502,22 @@ GridView grid_view(gz_context *ctx)
     g.cend = ctx->cend;
     g.cell_off = 0;
     return g;
+}
+
+TimeParams time_params(gz_context *ctx);
+ParticleView particle_view(gz_context *c);
+
+// In a step that drifts lazily: the particles of the cells a pass over the active rows reads on
+// the current grid, drifted to All.Ti_Current where they stand behind it (idempotent: once per
+// particle and step).  The largest radius drifted goes to u64c[75].
+int drift_stencil(gz_context *ctx)
+{
+    if (!ctx->lazy || ctx->nact == 0) return 0;
+    Timed t(ctx, "drift");
+    k_drift_stencil<<<xcd_grid(ctx->nact), 64, 0, ctx->stream>>>(time_params(ctx), particle_view(ctx), grid_view(ctx), ctx->nact,
+                                                                  ctx->act, ctx->key, ctx->ti_current, ctx->u64c + 75);
+    HIPCHK(hipGetLastError());
+    return 0;
 }
 
 // occupied cells of the active rows (one wavefront each in the cell-walk kernels)
@@ -766,6 +791,7 @@ int density_stage(gz_context *ctx)
     const bool wide = ctx->wide != 0;
     if (wide) CHK(build_wide(ctx, false));
     else CHK(build_grid(ctx, std::min(hmax, 0.5 * ctx->par.box_size)));
+    CHK(drift_stencil(ctx));
     const bool few = wide || na <= ctx->few_rows;
     if (!few) CHK(build_cell_rows(ctx));
     DensityIter it{ctx->it_h, ctx->it_hb, ctx->it_l, ctx->it_r, ctx->it_cn, ctx->it_ci, ctx->it_it};
@@ -812,6 +838,7 @@ int density_stage(gz_context *ctx)
             // a radius outgrew the cell side: a coarser grid for the rows still iterating
             CHK(max_on_device(ctx, ctx->it_h, na, &hmax));
             CHK(build_grid(ctx, std::min(hmax, 0.5 * ctx->par.box_size)));
+            CHK(drift_stencil(ctx));
             if (!few) CHK(build_cell_rows(ctx));
             ctx->stats["ngb_rebuilds"] += 1;
         }
@@ -863,11 +890,15 @@ int ensure_sym(gz_context *ctx)
         CHK(max_on_device(ctx, ctx->H, ctx->n, &hmax));
         ctx->hmax_bound = hmax;
     }
+    // in a lazy step a particle the walk reads may still be drifted to now, its radius growing by
+    // at most exp(0.3/3) (kernel_radius_drift_log_change caps one drift at 0.3 in log density)
+    if (ctx->lazy) hmax = std::min(hmax * std::exp(0.1) * (1.0 + 1e-12), ctx->par.max_kernel_radius);
     const bool wide = ctx->wide != 0;
     const bool few = ctx->nact <= ctx->few_rows;
     // the row walks drop stencil cells beyond reach: every radius is at most hmax
     if (wide) CHK(build_wide(ctx, true));
     else CHK(build_grid(ctx, std::min(hmax, 0.5 * ctx->par.box_size)));
+    CHK(drift_stencil(ctx));
     GridView gs = grid_view(ctx);
     gs.chmax = nullptr;
     gs.hall = hmax;
@@ -998,6 +1029,12 @@ int force_stage(gz_context *ctx)
         HIPCHK(hipGetLastError());
     }
     CHK(read_counters(ctx));
+    if (ctx->lazy && ctx->hmax_bound > 0) {
+        // the radii the passes' lazy drifts produced (u64c[75], read back with the counters)
+        double hl;
+        memcpy(&hl, &ctx->hu64c[75], sizeof hl);
+        ctx->hmax_bound = std::max(ctx->hmax_bound, hl);
+    }
     if (ctx->sym_check_pending) {
         ctx->sym_check_pending = false;
         const int over = ctx->hcounters[3];
@@ -1081,6 +1118,7 @@ int resort(gz_context *ctx)
         {&ctx->H, nullptr, 1}, {&ctx->U, nullptr, 1}, {&ctx->UK, nullptr, 1}, {&ctx->NN, nullptr, 1}, {&ctx->DRK, nullptr, 1},
         {&ctx->DIVV, nullptr, 1}, {&ctx->RHO, nullptr, 1}, {&ctx->PR, nullptr, 1}, {&ctx->CN, nullptr, 1}, {&ctx->FCE, nullptr, 1},
         {&ctx->DTU, nullptr, 1}, {&ctx->MSV, nullptr, 1}, {&ctx->SOFT, nullptr, 1}, {reinterpret_cast<double **>(&ctx->TIB), nullptr, 1},
+        {reinterpret_cast<double **>(&ctx->TIC), nullptr, 1},
         {reinterpret_cast<double **>(&ctx->DTS), nullptr, 1}, {&ctx->ACC, nullptr, 3}, {&ctx->FW, nullptr, 3}, {&ctx->GR, nullptr, 3},
         {&ctx->GP, nullptr, 3}, {&ctx->T, nullptr, 6}, {&ctx->GV, nullptr, 9}};
     PermCols P{};
@@ -1158,6 +1196,10 @@ TimeParams time_params(gz_context *ctx)
     T.max_h = ctx->par.max_kernel_radius;
     T.gamma = ctx->par.gamma;
     T.box = ctx->par.box_size;
+    T.full_frac = ctx->ip.domain_build_active_fraction;
+    T.n_total = ctx->n;
+    T.lazy_rows = kLazyMaxRows;
+    T.lazy_ok = ctx->wide ? 0 : 1;
     return T;
 }
 
@@ -1169,6 +1211,8 @@ ParticleView particle_view(gz_context *c)
     S.M = c->M; S.H = c->H; S.U = c->U; S.UK = c->UK; S.NN = c->NN; S.DIVV = c->DIVV; S.RHO = c->RHO; S.PR = c->PR;
     S.ACC = c->ACC; S.DTU = c->DTU; S.MSV = c->MSV; S.SOFT = c->SOFT;
     S.TB = c->TB; S.WAKE = c->WAKE; S.TIB = c->TIB; S.DTS = c->DTS;
+    S.TIC = c->TIC;
+    S.vkmax = c->u64c + 74;
     return S;
 }
 
@@ -1294,17 +1338,31 @@ int run_step(gz_context *ctx)
     ctx->all_active = false;
     ctx->sym_valid = false;
     ctx->stats["n_active"] = ctx->nact;
+    // a step that drifted only the active bins: the passes drift the cells they read
+    ctx->last_full = ctx->hu64c[72] != 0;
+    ctx->lazy = !ctx->last_full;
+    if (ctx->last_full) ctx->stats["full_drifts"] += 1;
     {
         double d, hm, ha;
         memcpy(&d, &ctx->hu64c[64], sizeof d);
         memcpy(&hm, &ctx->hu64c[66], sizeof hm);
         memcpy(&ha, &ctx->hu64c[67], sizeof ha);
-        ctx->dmax += d;  // the bound grows by the step's longest drift
-        ctx->hmax_bound = hm;    // every radius after the drift
+        ctx->dmax += d;  // the bound grows by the step's bound on any particle's motion
+        // every radius after the drift (a particle not drifted keeps its radius under the old bound)
+        ctx->hmax_bound = ctx->last_full ? hm : std::max(ctx->hmax_bound, hm);
         ctx->hmax_rows = ha;     // the largest radius density() starts its rows from
     }
-    // particles that moved far from where they were ordered are ordered again
+    // particles that moved far from where they were ordered are ordered again, after every
+    // particle is drifted to now (the reference's full drift before a decomposition,
+    // src/core/run.cc:380-390)
     if (ctx->last_rmax > 0 && ctx->dmax > ctx->resort_fraction * ctx->last_rmax) {
+        if (ctx->lazy) {
+            Timed t(ctx, "drift");
+            k_drift_behind<<<nblk(n), kBlock, 0, ctx->stream>>>(time_params(ctx), particle_view(ctx), ctx->ti_current, nullptr);
+            HIPCHK(hipGetLastError());
+            ctx->lazy = false;
+            ctx->hmax_bound = -1;  // taken again over every radius
+        }
         CHK(resort(ctx));
         CHK(build_active(ctx));
     }
@@ -1314,7 +1372,9 @@ int run_step(gz_context *ctx)
     S = particle_view(ctx);
     const int rc_pass = hydro_pass(ctx);
     ctx->density_ready = false;
+    if (rc_pass) ctx->lazy = false;
     CHK(rc_pass);
+    ctx->lazy = false;
     if (ctx->nact > 0) {
         Timed t(ctx, "timeline");
         k_halfstep_kick<<<nblk(ctx->nact), kBlock, 0, ctx->stream>>>(T, S, ctx->nact, ctx->act, 1, BoxTable{});
@@ -1377,7 +1437,7 @@ void gz_destroy(gz_context *ctx)
     delete ctx;
 }
 
-const char *gz_last_error(const gz_context *ctx) { return ctx ? ctx->err.c_str() : "no context"; }
+const char *gz_last_error(const gz_context *ctx) { return ctx ? ctx->err : "no context"; }
 
 int gz_set_params(gz_context *ctx, const gz_params *params)
 {
@@ -1450,9 +1510,9 @@ int gz_load(gz_context *ctx, long n, const gz_fields *f)
     else { CHK(copy_dev(ctx, ctx->VKX, ctx->VX, n * 8)); CHK(copy_dev(ctx, ctx->VKY, ctx->VY, n * 8)); CHK(copy_dev(ctx, ctx->VKZ, ctx->VZ, n * 8)); }
     if (f->u_kick) { CHK(upload_field(ctx, f->u_kick, 1, ctx->UK, false)); } else { CHK(copy_dev(ctx, ctx->UK, ctx->U, n * 8)); }
     if (f->soft) { CHK(upload_field(ctx, f->soft, 1, ctx->SOFT, false)); } else { CHK(copy_dev(ctx, ctx->SOFT, ctx->H, n * 8)); }
-    long long *tl[2] = {f->ti_begstep, f->dt_step};
-    long long *tdst[2] = {ctx->TIB, ctx->DTS};
-    for (int q = 0; q < 2; q++) {
+    long long *tl[3] = {f->ti_begstep, f->dt_step, f->ti_current};
+    long long *tdst[3] = {ctx->TIB, ctx->DTS, ctx->TIC};
+    for (int q = 0; q < 3; q++) {
         if (tl[q]) {
             HIPCHK(hipMemcpyAsync(ctx->stage, tl[q], n * sizeof(long long), hipMemcpyHostToDevice, ctx->stream));
             k_gather_ll<<<nblk(n), kBlock, 0, ctx->stream>>>(n, ctx->perm, (const long long *)ctx->stage, tdst[q]);
@@ -1523,9 +1583,9 @@ int gz_store(gz_context *ctx, gz_fields *f)
         CHK(download_field(ctx, o.dst, 1, o.src, false));
         HIPCHK(hipStreamSynchronize(ctx->stream));
     }
-    long long *tl[2] = {f->ti_begstep, f->dt_step};
-    const long long *tsrc[2] = {ctx->TIB, ctx->DTS};
-    for (int q = 0; q < 2; q++) {
+    long long *tl[3] = {f->ti_begstep, f->dt_step, f->ti_current};
+    const long long *tsrc[3] = {ctx->TIB, ctx->DTS, ctx->TIC};
+    for (int q = 0; q < 3; q++) {
         if (!tl[q]) continue;
         k_scatter_ll<<<nblk(n), kBlock, 0, ctx->stream>>>(n, ctx->perm, tsrc[q], (long long *)ctx->stage);
         HIPCHK(hipGetLastError());
@@ -1633,7 +1693,8 @@ int gz_stat(gz_context *ctx, const char *name, double *value)
                 return 0;
             }
     }
-    static const char *known[] = {"density_iterations", "ngb_rebuilds", "n_active", "ell_widen", "active_updates", "resorts"};
+    static const char *known[] = {"density_iterations", "ngb_rebuilds", "n_active", "ell_widen", "active_updates", "resorts",
+                                  "full_drifts"};
     auto it = ctx->stats.find(k);
     if (it == ctx->stats.end()) {
         for (const char *kn : known)
@@ -1658,6 +1719,7 @@ int gz_timing_reset(gz_context *ctx)
     ctx->timings.clear();
     ctx->stats["active_updates"] = 0;
     ctx->stats["resorts"] = 0;
+    ctx->stats["full_drifts"] = 0;
     HIPCHK(hipMemsetAsync(ctx->u64c + 65, 0, sizeof(unsigned long long), ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     return 0;
@@ -1679,8 +1741,10 @@ int gz_run_begin(gz_context *ctx, const gz_integ_params *ip)
     if (ctx->n <= 0) return fail(ctx, GZ_E_STATE, "nothing loaded");
     if (!(ip->time_max > ip->time_begin)) return fail(ctx, GZ_E_ARG, "time_max must exceed time_begin");
     if (!(ip->courant_fac > 0) || !(ip->err_tol_int_accuracy > 0) || !(ip->max_size_timestep > 0) || ip->min_size_timestep < 0 ||
-        ip->min_egy_spec < 0)
-        return fail(ctx, GZ_E_ARG, "courant_fac, err_tol_int_accuracy, max_size_timestep must be > 0; min_size_timestep, min_egy_spec >= 0");
+        ip->min_egy_spec < 0 || !(ip->domain_build_active_fraction >= 0))
+        return fail(ctx, GZ_E_ARG,
+                    "courant_fac, err_tol_int_accuracy, max_size_timestep must be > 0; min_size_timestep, min_egy_spec, "
+                    "domain_build_active_fraction >= 0");
     CHK(set_device(ctx));
     CHK(reserve_spares(ctx));
     const long n = ctx->n;
@@ -1701,7 +1765,14 @@ int gz_run_begin(gz_context *ctx, const gz_integ_params *ip)
     HIPCHK(hipMemsetAsync(ctx->TB, 0, n * sizeof(int), ctx->stream));
     HIPCHK(hipMemsetAsync(ctx->WAKE, 0, n * sizeof(int), ctx->stream));
     HIPCHK(hipMemsetAsync(ctx->TIB, 0, n * sizeof(long long), ctx->stream));
+    HIPCHK(hipMemsetAsync(ctx->TIC, 0, n * sizeof(long long), ctx->stream));
+    ctx->lazy = false;
+    ctx->last_full = true;
     HIPCHK(hipMemsetAsync(ctx->DTS, 0, n * sizeof(long long), ctx->stream));
+    // the bound on the kicked velocities, over every particle (the kicks raise it from here on)
+    HIPCHK(hipMemsetAsync(ctx->u64c + 74, 0, sizeof(unsigned long long), ctx->stream));
+    k_vk_max<<<nblk(n), kBlock, 0, ctx->stream>>>(particle_view(ctx));
+    HIPCHK(hipGetLastError());
     CHK(active_from_bins(ctx));
     ctx->dp.time = ctx->time;
     ctx->par.time = ctx->time;
@@ -1793,6 +1864,19 @@ int gz_timeline_get(gz_context *ctx, gz_timeline *out)
     for (int b = 0; b < kTimebins; b++) out->bin_count[b] = ctx->bin_count[b];
     out->bin_active = ctx->mask;
     out->stop_code = ctx->stop_code;
+    out->full_drift = ctx->last_full ? 1 : 0;
+    return 0;
+}
+
+int gz_drift_all(gz_context *ctx)
+{
+    if (!ctx) return GZ_E_ARG;
+    if (!ctx->run_started) return 0;  // outside a run every particle stands at the loaded time
+    CHK(set_device(ctx));
+    k_drift_behind<<<nblk(ctx->n), kBlock, 0, ctx->stream>>>(time_params(ctx), particle_view(ctx), ctx->ti_current, nullptr);
+    HIPCHK(hipGetLastError());
+    ctx->hmax_bound = -1;
+    HIPCHK(hipStreamSynchronize(ctx->stream));
     return 0;
 }
 

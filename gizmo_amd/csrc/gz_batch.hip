@@ -1109,6 +1109,7 @@ int gz_batch_timeline(gz_batch *b, int box_id, gz_timeline *out)
     for (int k = 0; k < kTimebins; k++) out->bin_count[k] = h.bins[k];
     out->bin_active = d.mask;
     out->stop_code = h.stop_code;
+    out->full_drift = 1;  // a batch drifts every particle of its stepping boxes
     return 0;
 }
 

@@ -140,6 +140,13 @@ struct TimeParams {
     long long ti_current;                                  // All.Ti_Current
     unsigned long long active_mask;                        // TimeBinActive, bit per bin
     double min_h, max_h, gamma, box;
+    // the drift of a step (single box): every particle when more than full_frac of them are in
+    // the active bins (All.DomainBuild_ActiveFraction, src/core/run.cc:360,380-390), when more
+    // than lazy_rows are, or when lazy_ok is 0; otherwise only the active bins, the others when
+    // a pass reads them
+    double full_frac;
+    long long n_total, lazy_rows;
+    int lazy_ok;
 };
 
 // the timeline of particle i's box over the single-box one
@@ -157,6 +164,11 @@ struct ParticleView {         // device order; vectors SoA with column stride n
     double *X, *Y, *Z, *VX, *VY, *VZ, *VKX, *VKY, *VKZ, *M, *H, *U, *UK, *NN, *DIVV, *RHO, *PR, *ACC, *DTU, *MSV, *SOFT;
     int *TB, *WAKE;
     long long *TIB, *DTS;
+    // P.Ti_current of each particle (the time its predicted fields stand at; nullptr: every
+    // particle at All.Ti_Current, the batches' eager drift), and the bits of a bound on every
+    // kicked velocity component |P.Vel| (raised by the kicks; nullptr: not kept)
+    long long *TIC = nullptr;
+    unsigned long long *vkmax = nullptr;
 };
 
 __global__ void k_find_timesteps(TimeParams T, ParticleView S, int nrows, const int *act, int *err, unsigned long long *count,
@@ -188,6 +200,14 @@ struct DensityStart {
 };
 __global__ void k_active_compact(long n, int nblocks, const unsigned long long *bits, const int *bcount, int *act, int *count_out,
                                  DensityStart ds);
+// the reference's lazy drift (src/core/predict.cc:175-210, src/mesh/gpu_neighbor_list.cc:2443-2510):
+// every particle of the 27 grid cells around each active row's home cell that stands behind
+// time1 is drifted to it before a pass reads the cells (one workgroup of 64 per row); every
+// particle behind time1 (before a re-ordering or a snapshot); the bound on the kicked velocities
+__global__ void k_drift_stencil(TimeParams T, ParticleView S, GridView g, int nrows, const int *act, const uint32_t *key,
+                                long long time1, unsigned long long *hmax);
+__global__ void k_drift_behind(TimeParams T, ParticleView S, long long time1, unsigned long long *hmax);
+__global__ void k_vk_max(ParticleView S);
 __global__ void k_advance_boxes(BoxTable bt);
 __global__ void k_active_from_bins(long n, unsigned long long mask, const int *TB, const double *M, const double *H,
                                    uint8_t *ACT, double *SOFT);

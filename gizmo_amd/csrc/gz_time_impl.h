@@ -5,7 +5,7 @@
 // non-comoving MFM run.  The host keeps the integer timeline and the bin occupation (60 counts,
 // read back once per step) and passes the active bins as a bit mask.
 #pragma once
-#include "gz_kernels.h"
+#include "gz_kernels_impl.h"
 
 namespace gz {
 
@@ -37,6 +37,15 @@ __device__ __forceinline__ double drift_log_change(double x)
     if (x > +0.3) return +0.3;
     if (x < -0.3) return -0.3;
     return x;
+}
+
+// the largest kicked velocity component of the calling lanes' particles (valid lanes only) into
+// the bound *vkmax: one conditional atomic per wavefront (all 64 lanes call it together)
+__device__ __forceinline__ void raise_vk_bound(const ParticleView &S, long i, bool valid)
+{
+    double m = valid ? fmax(fabs(S.VKX[i]), fmax(fabs(S.VKY[i]), fabs(S.VKZ[i]))) : 0.0;
+    for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
+    if ((threadIdx.x & 63) == 0 && m > 0) atomic_max_bits(S.vkmax, m);
 }
 
 // get_timestep, src/core/timestep.cc:357-1423 for a gas particle here: acceleration criterion
@@ -155,6 +164,7 @@ __global__ void __launch_bounds__(256) k_find_timesteps(TimeParams T, ParticleVi
             atomicAdd(&moved[bin & 63], 1);
         }
     }
+    if (S.vkmax) raise_vk_bound(S, r < nrows ? act[r] : 0, r < nrows && kick_first);
     if (!count) return;
     __syncthreads();
     if (threadIdx.x < 64 && moved[threadIdx.x]) atomicAdd(&count[threadIdx.x], (unsigned long long)(long long)moved[threadIdx.x]);
@@ -256,23 +266,26 @@ __global__ void __launch_bounds__(64) k_wakeup_limits(TimeParams T, const unsign
     }
 }
 
-// process_wake_ups, src/core/timestep.cc:1565-1774 (every particle stands at the sync point:
-// its Ti_current is All.Ti_Current, so a wake-up never truncates a step); the occupation
-// count follows each move
-__global__ void __launch_bounds__(256) k_wakeups(TimeParams T, ParticleView S, const int *__restrict__ lim, int wakeup_bin_offset,
-                                                  unsigned long long *__restrict__ count, int *__restrict__ nwoken, BoxTable bt)
+// process_wake_ups, src/core/timestep.cc:1565-1774; the occupation count follows each move.
+// A particle drifted to the sync point (P.Ti_current == All.Ti_Current: every particle when the
+// drift is eager, and every woken one when it is lazy, since the force pass that flagged it
+// read it) completes its old step from the midpoint and starts the new one with its first
+// half-kick when the new bin is active now; one still behind keeps a step truncated at the
+// sync point (:1727-1731) and takes no kick.
+__device__ __forceinline__ bool wakeup_one(const TimeParams &T0, const ParticleView &S, const int *__restrict__ lim,
+                                           int wakeup_bin_offset, unsigned long long *__restrict__ count, int *__restrict__ nwoken,
+                                           const BoxTable &bt, long i)
 {
-    const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
-    if (i >= S.n) return;
     const int w = S.WAKE[i];
-    if (!w) return;
-    if (S.M[i] <= 0) return;
+    if (!w) return false;
+    if (S.M[i] <= 0) return false;
     const int box = bt.b ? bt.BOX[i] : 0;
-    if (bt.b && !(bt.b[box].stepping && bt.b[box].need_wakeup)) return;
+    if (bt.b && !(bt.b[box].stepping && bt.b[box].need_wakeup)) return false;
+    TimeParams T = T0;
     resolve_time(bt, (int)i, T);
     const int lowest_occupied_active_bin = lim[2 * box], max_time_bin_active = lim[2 * box + 1];
     const int binold = S.TB[i];
-    if (bin_on(T.active_mask, binold)) return;
+    if (bin_on(T.active_mask, binold)) return false;
     int bin;
     if (w > 0) {
         bin = (w - 1) - wakeup_bin_offset;
@@ -283,18 +296,35 @@ __global__ void __launch_bounds__(256) k_wakeups(TimeParams T, ParticleView S, c
     }
     if (bin > max_time_bin_active) bin = max_time_bin_active;
     if (bin >= binold) bin = binold;
-    if (bin == binold) return;
+    if (bin == binold) return false;
+    const long long tic = S.TIC ? S.TIC[i] : T.ti_current;
     const long long old_midpoint = S.TIB[i] + S.DTS[i] / 2;
-    const bool starts_now = bin_on(T.active_mask, bin) && (S.DTS[i] == bin_len(binold));
+    const bool starts_now = (tic == T.ti_current) && bin_on(T.active_mask, bin) && (S.DTS[i] == bin_len(binold));
     S.TB[i] = bin;
     atomicAdd(nwoken, 1);
     if (bt.b) atomicAdd(&bt.slot[(size_t)box * kBoxSlots + kSlotWoken], 1ull);
     atomicAdd(&count[(size_t)64 * box + binold], ~0ull);  // -1
     atomicAdd(&count[(size_t)64 * box + bin], 1ull);
     if (starts_now) kick(T, S, i, old_midpoint, T.ti_current, 1);
-    S.TIB[i] = T.ti_current;
-    S.DTS[i] = bin_len(bin);
+    const long long dt_truncated = T.ti_current - tic;
+    if (dt_truncated > 0) {
+        S.TIB[i] = tic;
+        S.DTS[i] = dt_truncated;
+    } else {
+        S.TIB[i] = T.ti_current;
+        S.DTS[i] = bin_len(bin);
+    }
+    if (S.TIC && tic < T.ti_current) S.TIC[i] = T.ti_current;
     if (starts_now) kick(T, S, i, S.TIB[i], S.TIB[i] + S.DTS[i] / 2, 0);
+    return starts_now;
+}
+
+__global__ void __launch_bounds__(256) k_wakeups(TimeParams T, ParticleView S, const int *__restrict__ lim, int wakeup_bin_offset,
+                                                  unsigned long long *__restrict__ count, int *__restrict__ nwoken, BoxTable bt)
+{
+    const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    const bool kicked = i < S.n && wakeup_one(T, S, lim, wakeup_bin_offset, count, nwoken, bt, i);
+    if (S.vkmax) raise_vk_bound(S, kicked ? i : 0, kicked);
 }
 
 // do_first_halfstep_kick / do_second_halfstep_kick, src/core/kicks.cc:34-97,99-185
@@ -302,13 +332,15 @@ __global__ void __launch_bounds__(256) k_halfstep_kick(TimeParams T, ParticleVie
                                                         BoxTable bt)
 {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= nrows) return;
-    const int i = act[r];
-    if (!(S.M[i] > 0)) return;
-    resolve_time(bt, i, T);
-    const long long half = S.DTS[i] / 2;
-    if (second) kick(T, S, i, S.TIB[i] + half, S.TIB[i] + S.DTS[i], 1);
-    else kick(T, S, i, S.TIB[i], S.TIB[i] + half, 0);
+    const int i = r < nrows ? act[r] : 0;
+    const bool row = r < nrows && S.M[i] > 0;
+    if (row) {
+        resolve_time(bt, i, T);
+        const long long half = S.DTS[i] / 2;
+        if (second) kick(T, S, i, S.TIB[i] + half, S.TIB[i] + S.DTS[i], 1);
+        else kick(T, S, i, S.TIB[i], S.TIB[i] + half, 0);
+    }
+    if (S.vkmax) raise_vk_bound(S, i, row);
 }
 
 // drift_particle_impl, src/core/drift_particle_functions.h:86-250 (MFM gas; the kernel radius
@@ -424,7 +456,7 @@ __global__ void k_step_prep(int *__restrict__ counters, unsigned long long *__re
 {
     const int t = threadIdx.x;
     if (t < 16) counters[t] = 0;
-    if (t == 0) { u64c[64] = 0; u64c[66] = 0; u64c[67] = 0; u64c[68] = 0; }
+    if (t == 0) { u64c[64] = 0; u64c[66] = 0; u64c[67] = 0; u64c[68] = 0; u64c[72] = 0; u64c[73] = 0; u64c[75] = 0; }
 }
 
 // find_next_sync_point_and_drift's timeline part (src/core/run.cc:857-930) from the occupation,
@@ -446,6 +478,9 @@ __global__ void __launch_bounds__(64) k_next_sync(TimeParams T, const unsigned l
     }
     const long long ti_next_kick = wave_next_kick(count + (size_t)64 * b, tic);
     const unsigned long long mask = __ballot(k == 0 || (k < kTimebins && (ti_next_kick % bin_len(k)) == 0));
+    // NumForceUpdate: the particles of the bins active at the next kick (src/core/run.cc:942-955)
+    unsigned long long nf = (k < kTimebins && ((mask >> k) & 1ull)) ? count[(size_t)64 * b + k] : 0ull;
+    for (int o = 32; o > 0; o >>= 1) nf += __shfl_xor(nf, o, 64);
     if (k != 0) return;
     if (bt.b) {
         bt.b[b].need_wakeup = 0;  // consumed by the wake-up pass of this step
@@ -454,6 +489,13 @@ __global__ void __launch_bounds__(64) k_next_sync(TimeParams T, const unsigned l
     } else {
         tl[0] = (unsigned long long)ti_next_kick;
         tl[1] = mask;
+        // every particle is drifted at a step whose active fraction reaches the domain cadence
+        // (src/core/run.cc:360,380-390: the full drift before the decomposition), or where the
+        // lazy drift is not in use; otherwise only the active bins (find_next_sync_point_and_drift,
+        // src/core/run.cc:979-994) and the others when read
+        const bool full = !T.lazy_ok || (double)nf > T.full_frac * (double)T.n_total || (long long)nf > T.lazy_rows;
+        tl[2] = full ? 1ull : 0ull;
+        tl[3] = nf;
     }
 }
 
@@ -493,15 +535,18 @@ __global__ void __launch_bounds__(256) k_active_flags(long n, const unsigned lon
     wave_max_to(h, valid, box, hmax_act, bt, kSlotHrows);
 }
 
-// The drift of every particle to the next sync point (drift_one, as k_drift does for boxes)
-// fused with the first half of make_list_of_active_particles (src/core/run.cc:997-1009): block b
-// drifts the kActChunk particles of chunk b (thread t takes particles t, t+256, ...) and keeps
-// each wavefront's 64 row flags as a ballot and the block's row count; k_active_compact then
-// writes each block's rows at its offset (the counts of the blocks before it) in ascending
-// order.  No scan over every particle, and the flags read the drifted radius from registers
-// instead of a second sweep.  disp receives the longest drift of the step along an axis (the
-// caller sums these into its bound on the displacement since the last ordering: no ordering
-// positions are read).
+// The drift to the next sync point (drift_one, as k_drift does for boxes) fused with the first
+// half of make_list_of_active_particles (src/core/run.cc:997-1009): block b takes the kActChunk
+// particles of chunk b (thread t takes particles t, t+256, ...) and keeps each wavefront's 64
+// row flags as a ballot and the block's row count; k_active_compact then writes each block's rows
+// at its offset (the counts of the blocks before it) in ascending order.  No scan over every
+// particle, and the flags read the drifted radius from registers instead of a second sweep.
+// Which particles move: tl[2] set (the full drift of the step) every particle from its own
+// P.Ti_current; else the particles of the active bins only (src/core/run.cc:979-994), and a
+// particle of another bin reads only its bin.  disp receives the step's bound on how far any
+// particle can move, drifted now or later (the bound on the kicked velocities times the step,
+// padded for rounding): the caller sums these into its bound on the displacement since the last
+// ordering, so no ordering positions are read.  hmax receives the largest radius drifted.
 __global__ void __launch_bounds__(256) k_drift_active(TimeParams T, ParticleView S, long long time0, const unsigned long long *__restrict__ tl,
                                                       unsigned long long *__restrict__ disp,
                                                       unsigned long long *__restrict__ hmax, double hclamp, uint8_t *__restrict__ ACT,
@@ -510,6 +555,7 @@ __global__ void __launch_bounds__(256) k_drift_active(TimeParams T, ParticleView
 {
     const long long time1 = (long long)tl[0];
     const unsigned long long mask = tl[1];
+    const bool full = tl[2] != 0;
     const long base = (long)blockIdx.x * kActChunk;
     const int tid = threadIdx.x, w = tid >> 6;
     double d1 = 0, h1 = 0, hm = 0;
@@ -521,21 +567,28 @@ __global__ void __launch_bounds__(256) k_drift_active(TimeParams T, ParticleView
             // the flag's inputs loaded before the drift's stores (which may alias them for
             // the compiler) and the drifted radius taken from the drift
             const int tb = S.TB[i];
-            const double m = S.M[i];
-            double d = 0, hh;
-            if (time1 != time0) d = drift_one(T, S, i, time0, time1, nullptr, nullptr, nullptr, &hh);
-            else hh = S.H[i];
-            d1 = fmax(d1, d);
-            h1 = fmax(h1, hh);
             const bool on = bin_on(mask, tb);
-            ACT[i] = on ? 1 : 0;
-            row = on && (m > 0);
-            if (row) {
-                S.SOFT[i] = hh;
-                double h = hh;
-                if ((h < 0) || !isfinite(h) || (h > hclamp)) h = hclamp;
-                hm = fmax(hm, h);
+            if (full || on) {
+                const double m = S.M[i];
+                const long long t0 = S.TIC ? S.TIC[i] : time0;
+                double d = 0, hh;
+                if (time1 != t0) {
+                    d = drift_one(T, S, i, t0, time1, nullptr, nullptr, nullptr, &hh);
+                    if (S.TIC) S.TIC[i] = time1;
+                } else {
+                    hh = S.H[i];
+                }
+                d1 = fmax(d1, d);
+                h1 = fmax(h1, hh);
+                row = on && (m > 0);
+                if (row) {
+                    S.SOFT[i] = hh;
+                    double h = hh;
+                    if ((h < 0) || !isfinite(h) || (h > hclamp)) h = hclamp;
+                    hm = fmax(hm, h);
+                }
             }
+            ACT[i] = on ? 1 : 0;
         }
         const unsigned long long b = __ballot(row);
         if ((tid & 63) == 0) bits[(long)blockIdx.x * (kActChunk / 64) + k * 4 + w] = b;
@@ -555,6 +608,11 @@ __global__ void __launch_bounds__(256) k_drift_active(TimeParams T, ParticleView
         d1 = fmax(fmax(wd[0], wd[1]), fmax(wd[2], wd[3]));
         h1 = fmax(fmax(wh[0], wh[1]), fmax(wh[2], wh[3]));
         hm = fmax(fmax(whm[0], whm[1]), fmax(whm[2], whm[3]));
+        if (blockIdx.x == 0 && S.vkmax) {
+            // every particle moves at most the bound on the kicked velocities per unit time
+            const double vk = __longlong_as_double((long long)*S.vkmax);
+            d1 = fmax(d1, vk * ((time1 - time0) * T.tbi) * (1.0 + 1e-12) + 1e-13 * T.box);
+        }
         if (d1 > 0) atomic_max_bits(disp, d1);
         if (h1 > 0) atomic_max_bits(hmax, h1);
         if (hm > 0) atomic_max_bits(hmax_act, hm);
@@ -646,6 +704,69 @@ __global__ void __launch_bounds__(256) k_active_from_bins(long n, unsigned long 
     const bool on = bin_on(mask, TB[i]);
     ACT[i] = on ? 1 : 0;
     if (on && M[i] > 0) SOFT[i] = H[i];
+}
+
+// The particles a pass over the active rows may read, drifted to time1 before it reads them
+// (the lazy drift's hook in the neighbour search, src/mesh/gpu_neighbor_list.cc:2443-2510: the
+// members the walk found behind the search time are drifted to it).  The walks of this library
+// visit the 27 cells around each row's home cell of the grid g, so those cells are the set here
+// (a superset of the pairs accepted, as the reference's walk's is).  One workgroup of 64 lanes
+// per row; a particle behind time1 is claimed by one lane (compare-and-swap of its
+// P.Ti_current) and drifted from where it stands.  hmax receives the largest radius drifted.
+__global__ void __launch_bounds__(64) k_drift_stencil(TimeParams T, ParticleView S, GridView g, int nrows, const int *__restrict__ act,
+                                                      const uint32_t *__restrict__ key, long long time1,
+                                                      unsigned long long *__restrict__ hmax)
+{
+    const long r = xcd_block();
+    if (r >= nrows) return;
+    const int lane = threadIdx.x;
+    const int i = act[r];
+    const uint32_t home = key[i] >> g.shift;
+    const int nc = 1 << g.level;
+    const int cx = compact10(home >> 2), cy = compact10(home >> 1), cz = compact10(home);
+    double h1 = 0;
+    for (int q = 0; q < 27; q++) {
+        const int dx = q / 9 - 1, dy = (q / 3) % 3 - 1, dz = q % 3 - 1;
+        const int a = (cx + dx + nc) % nc, b = (cy + dy + nc) % nc, d = (cz + dz + nc) % nc;
+        const int c = (int)((spread10(a) << 2) | (spread10(b) << 1) | spread10(d));
+        const int st = g.cstart[c], en = g.cend[c];
+        for (int j = st + lane; j < en; j += 64) {
+            const long long t0 = __hip_atomic_load(&S.TIC[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (t0 == time1) continue;
+            if (atomicCAS((unsigned long long *)&S.TIC[j], (unsigned long long)t0, (unsigned long long)time1) !=
+                (unsigned long long)t0)
+                continue;
+            double hh;
+            drift_one(T, S, j, t0, time1, nullptr, nullptr, nullptr, &hh);
+            h1 = fmax(h1, hh);
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) h1 = fmax(h1, __shfl_xor(h1, o, 64));
+    if (lane == 0 && h1 > 0) atomic_max_bits(hmax, h1);
+}
+
+// every particle behind time1 drifted to it (gizmo_full_drift_to, src/core/predict.cc:148-171:
+// before a re-ordering, which reads every position, and for a snapshot of the whole state)
+__global__ void __launch_bounds__(256) k_drift_behind(TimeParams T, ParticleView S, long long time1, unsigned long long *__restrict__ hmax)
+{
+    const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    double h1 = 0;
+    if (i < S.n) {
+        const long long t0 = S.TIC[i];
+        if (t0 != time1) {
+            drift_one(T, S, i, t0, time1, nullptr, nullptr, nullptr, &h1);
+            S.TIC[i] = time1;
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) h1 = fmax(h1, __shfl_xor(h1, o, 64));
+    if ((threadIdx.x & 63) == 0 && h1 > 0 && hmax) atomic_max_bits(hmax, h1);
+}
+
+// the bound on the kicked velocities over every particle (the start of a run)
+__global__ void __launch_bounds__(256) k_vk_max(ParticleView S)
+{
+    const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    raise_vk_bound(S, i < S.n ? i : 0, i < S.n);
 }
 
 }  // namespace gz

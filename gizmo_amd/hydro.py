@@ -34,7 +34,7 @@ FIELD_WIDTH = dict(pos=3, vel=3, mass=1, h=1, u=1, numngb=1, drkern=1, divvel=1,
                    nvt=6, nvt_fw=3, cn=1, fce=1, grad_rho=3, grad_p=3, grad_v=9, accel=3, dtu=1, maxsig=1)
 INT_FIELDS = ("timebin", "wakeup")
 LOOP_FIELD_WIDTH = dict(vel_kick=3, u_kick=1, soft=1)
-I64_FIELDS = ("ti_begstep", "dt_step")
+I64_FIELDS = ("ti_begstep", "dt_step", "ti_current")
 TIMEBINS = 60
 
 
@@ -47,14 +47,23 @@ class IntegParams(C.Structure):
     """All.* parameters of the timestep loop (gz_integ_params)."""
     _fields_ = [("courant_fac", C.c_double), ("err_tol_int_accuracy", C.c_double),
                 ("max_size_timestep", C.c_double), ("min_size_timestep", C.c_double),
-                ("time_begin", C.c_double), ("time_max", C.c_double), ("min_egy_spec", C.c_double)]
+                ("time_begin", C.c_double), ("time_max", C.c_double), ("min_egy_spec", C.c_double),
+                ("domain_build_active_fraction", C.c_double)]
+
+
+def integ_params(integ):
+    """gz_integ_params from a dict; domain_build_active_fraction defaults to 0 (every step drifts
+    every particle)."""
+    return IntegParams(**{k: float(integ.get(k, 0.0) if k == "domain_build_active_fraction" else integ[k])
+                          for k, _ in IntegParams._fields_})
 
 
 class Timeline(C.Structure):
     """The integer timeline and the time bins (gz_timeline)."""
     _fields_ = [("ti_current", C.c_longlong), ("time", C.c_double), ("timebase_interval", C.c_double),
                 ("num_steps", C.c_longlong), ("n_active", C.c_longlong), ("n_woken", C.c_longlong),
-                ("bin_count", C.c_longlong * TIMEBINS), ("bin_active", C.c_ulonglong), ("stop_code", C.c_int)]
+                ("bin_count", C.c_longlong * TIMEBINS), ("bin_active", C.c_ulonglong), ("stop_code", C.c_int),
+                ("full_drift", C.c_int)]
 
 
 class GizmoHipError(RuntimeError):
@@ -96,6 +105,7 @@ def lib():
         L.gz_timeline_get.argtypes = [C.c_void_p, C.POINTER(Timeline)]
         L.gz_set_option.argtypes = [C.c_void_p, C.c_char_p, C.c_double]
         L.gz_riemann_probe.argtypes = [C.c_void_p, C.c_int, C.c_long, _dp, _dp, C.POINTER(C.c_uint)]
+        L.gz_drift_all.argtypes = [C.c_void_p]
         _lib = L
     return _lib
 
@@ -151,7 +161,7 @@ def timeline_dict(t):
     return dict(ti_current=t.ti_current, time=t.time, timebase_interval=t.timebase_interval,
                 num_steps=t.num_steps, n_active=t.n_active, n_woken=t.n_woken,
                 bin_count=np.array(t.bin_count[:], dtype=np.int64), bin_active=t.bin_active,
-                stop_code=t.stop_code)
+                stop_code=t.stop_code, full_drift=t.full_drift)
 
 
 def make_params(p, precision=GZ_FP64):
@@ -251,7 +261,7 @@ class Hydro:
     # ---- the timestep loop (gz_run_begin / gz_run_step) ----
     def run_begin(self, integ):
         """integ: dict of the IntegParams fields (All.CourantFac, ErrTolIntAccuracy, ...)."""
-        self._ip = IntegParams(**{k: float(integ[k]) for k, _ in IntegParams._fields_})
+        self._ip = integ_params(integ)
         self._check(lib().gz_run_begin(self.ctx, C.byref(self._ip)))
 
     def run_step(self):
@@ -267,6 +277,10 @@ class Hydro:
         self._check(lib().gz_run_steps(self.ctx, int(nsteps), act.ctypes.data_as(C.POINTER(C.c_longlong)),
                                        woken.ctypes.data_as(C.POINTER(C.c_longlong))))
         return act[:nsteps].tolist(), woken[:nsteps].tolist()
+
+    def drift_all(self):
+        """Every particle behind the current sync point drifted to it (gizmo_full_drift_to)."""
+        self._check(lib().gz_drift_all(self.ctx))
 
     def set_option(self, name, value):
         self._check(lib().gz_set_option(self.ctx, name.encode(), float(value)))

@@ -140,6 +140,71 @@ def lognormal_box(n_particles, seed, *, ng=256, sigma_ln=2.0, slope=-3.0, mach=2
     return dict(pos=np.ascontiguousarray(pos), vel=np.ascontiguousarray(vel), mass=mass, u=u, h=h, params=params)
 
 
+def shocktube_box(n_dense, seed, *, gamma=1.4, left=(1.0, 1.0), right=(0.125, 0.1), jitter=0.02, kernel=3,
+                  des_num_ngb=32.0, box=1.0):
+    """Sod's shock tube in a periodic box (the reference's shocktube test, 3D): the dense state
+    (rho, P) = left fills the slab box/4 <= x < 3 box/4, the light state (right) the rest, at rest,
+    gamma 1.4.  Equal-mass particles on two cubic lattices: n_dense per box side in the slab and
+    n_dense * (rho_right/rho_left)^(1/3) (rounded) outside, each coordinate jittered by
+    U(-jitter, +jitter) spacings.  The slab's faces at 3 box/4 and box/4 are Sod's problem and its
+    mirror image."""
+    rng = _rng(seed)
+    (rl, pl), (rr, pr) = left, right
+    n_light = int(round(n_dense * (rr / rl) ** (1.0 / 3.0)))
+    parts = []
+    for n_side, x0, x1 in ((n_dense, 0.25 * box, 0.75 * box), (n_light, 0.75 * box, 1.25 * box)):
+        dx = box / n_side
+        nx = int(round((x1 - x0) / dx))
+        gx = x0 + (np.arange(nx) + 0.5) * dx
+        g = (np.arange(n_side) + 0.5) * dx
+        X, Y, Z = np.meshgrid(gx, g, g, indexing="ij")
+        p = np.stack([X.ravel(), Y.ravel(), Z.ravel()], axis=1)
+        p += rng.uniform(-jitter, jitter, size=p.shape) * dx
+        parts.append(p % box)
+    pos = np.concatenate(parts)
+    n = len(pos)
+    dense = np.zeros(n, dtype=bool)
+    dense[:len(parts[0])] = True
+    m = rl * 0.5 * box ** 3 / len(parts[0])
+    mass = np.full(n, m)
+    u = np.where(dense, pl / ((gamma - 1.0) * rl), pr / ((gamma - 1.0) * rr))
+    nloc = np.where(dense, rl, rr) / m
+    h = (des_num_ngb / (VOLNORM * nloc)) ** (1.0 / 3.0)
+    params = dict(box=box, kernel=kernel, gamma=gamma, des_num_ngb=float(des_num_ngb),
+                  max_num_ngb_dev=max(des_num_ngb / 640.0, 0.05),
+                  min_kernel_radius=0.0, max_kernel_radius=0.25 * box, time=0.0, time_begin=0.0)
+    return dict(pos=np.ascontiguousarray(pos), vel=np.zeros((n, 3)), mass=mass, u=u, h=h, params=params,
+                dense=dense)
+
+
+def soundwave_box(n_side, *, amplitude=1e-6, gamma=5.0 / 3.0, kernel=3, des_num_ngb=32.0, box=1.0):
+    """A linear sound wave (the reference's soundwave test, 3D): a perfect cubic lattice (an
+    equilibrium of the pair forces) at density 1 and pressure 3/5 (c_s = 1 for gamma 5/3) with a
+    plane wave of wavelength box along x: particle displacement (amplitude/k) cos(k x0), so that
+    rho = 1 + amplitude sin(k x); velocity c_s amplitude sin(k x) along x (a wave moving to +x);
+    adiabatic energy u0 (rho)^(gamma-1).  One period takes box / c_s."""
+    n = n_side ** 3
+    dx = box / n_side
+    g = (np.arange(n_side, dtype=np.float64) + 0.5) * dx
+    X, Y, Z = np.meshgrid(g, g, g, indexing="ij")
+    pos = np.stack([X.ravel(), Y.ravel(), Z.ravel()], axis=1)
+    k = 2.0 * np.pi / box
+    x0 = pos[:, 0].copy()
+    pos[:, 0] = (x0 + amplitude / k * np.cos(k * x0)) % box
+    p0 = 1.0 / gamma
+    cs = np.sqrt(gamma * p0)
+    s = np.sin(k * x0)
+    vel = np.zeros((n, 3))
+    vel[:, 0] = cs * amplitude * s
+    u = p0 / (gamma - 1.0) * (1.0 + amplitude * s) ** (gamma - 1.0)
+    mass = np.full(n, box ** 3 / n)
+    h = np.full(n, h_guess(n, des_num_ngb, box))
+    params = dict(box=box, kernel=kernel, gamma=gamma, des_num_ngb=float(des_num_ngb),
+                  max_num_ngb_dev=max(des_num_ngb / 640.0, 0.05),
+                  min_kernel_radius=0.0, max_kernel_radius=0.25 * box, time=0.0, time_begin=0.0)
+    return dict(pos=np.ascontiguousarray(pos), vel=vel, mass=mass, u=u, h=h, params=params, period=box / cs)
+
+
 def workload(name, seed=None):
     """The named single-box workloads: 'c1' (32^3 cubic-spline lattice) and 'c2' (128^3 Sedov)."""
     if name == "c1":

@@ -86,6 +86,8 @@ typedef struct {
     double *soft;      /* P.ForceSoftening (kernel radius cached for the acceleration criterion) */
     long long *ti_begstep; /* P.Ti_begstep */
     long long *dt_step;    /* P.dt_step */
+    long long *ti_current; /* P.Ti_current (the time the particle's predicted fields stand at: the run
+                              loop drifts a particle only when it is active or read, see gz_drift_all) */
 } gz_fields;
 
 typedef struct gz_context gz_context;
@@ -147,8 +149,14 @@ int gz_timing(gz_context *ctx, const char *name, long *launches, double *total_m
 /* ---- the timestep loop around the hot path (src/core/run.cc:220-507) ----
    A gas-only, gravity-free, non-comoving run on the resident state: power-of-two time bins on
    the integer timeline (TIMEBINS = 60, src/declarations/typedefs.h:8-9), kick-drift-kick with
-   the reference's per-particle steps, wake-ups, and every particle drifted to each sync point.
-   The state stays on the device between steps; gz_store reads it back at any time. */
+   the reference's per-particle steps, wake-ups, and the reference's lazy drift: at a sync point
+   only the particles of the active bins are drifted (src/core/run.cc:979-994,
+   src/core/predict.cc:175-210), every other particle when a pass is about to read it
+   (src/mesh/gpu_neighbor_list.cc:2443-2510: here the particles of the grid cells around the
+   active rows), and every particle at a step whose active bins hold more than
+   domain_build_active_fraction of them (src/core/run.cc:360,380-390) and before the particles
+   are ordered again.  The state stays on the device between steps; gz_store reads it back at any
+   time (P.Ti_current tells where each particle's predicted fields stand). */
 #define GZ_TIMEBINS 60
 typedef struct {
     double courant_fac;          /* All.CourantFac */
@@ -158,6 +166,9 @@ typedef struct {
     double time_begin;           /* All.TimeBegin */
     double time_max;             /* All.TimeMax (sets All.Timebase_interval) */
     double min_egy_spec;         /* All.MinEgySpec (0 = no floor) */
+    double domain_build_active_fraction; /* All.DomainBuild_ActiveFraction: a step whose active bins hold more
+                                            than this fraction of the particles drifts every particle
+                                            (src/core/run.cc:360,380-390); 0 = every step */
 } gz_integ_params;
 
 typedef struct {
@@ -170,6 +181,8 @@ typedef struct {
     long long bin_count[GZ_TIMEBINS];  /* TimeBinCount */
     unsigned long long bin_active;     /* TimeBinActive as a bit mask */
     int stop_code;                     /* first controlled-stop code raised, else 0 */
+    int full_drift;                    /* 1: the last step drifted every particle; 0: only the active ones
+                                          and the particles its passes read */
 } gz_timeline;
 
 /* init() (src/core/init.cc:108-113,583,1051,1061) and run()'s first
@@ -191,6 +204,10 @@ int gz_run_step(gz_context *ctx);
    how many were taken). */
 int gz_run_steps(gz_context *ctx, int nsteps, long long *n_active, long long *n_woken);
 int gz_timeline_get(gz_context *ctx, gz_timeline *out);
+/* gizmo_full_drift_to(All.Ti_Current) (src/core/predict.cc:148-171): every particle that stands
+   behind the current sync point drifted to it, as the reference does before a snapshot
+   (src/core/run.cc:911).  No effect outside a run. */
+int gz_drift_all(gz_context *ctx);
 /* Tuning knobs with no counterpart in the reference: "resort_fraction" (default 0.125) -- the
    particles are put back in space-filling-curve order once a bound on how far any of them has
    moved since they were last ordered (the sum of each step's longest drift along an axis)

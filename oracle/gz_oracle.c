@@ -112,6 +112,14 @@ static inline double or_wrap_abs(double x, double box, double half)
     return (t > half) ? (box - t) : t;
 }
 
+/* the lazy drift of the timestep loop: the passes find their pairs on the predicted positions
+   and radii and drift a particle to the current time before they read it */
+static const orc_lazy *g_lazy = NULL;
+void orc_set_lazy(const orc_lazy *view) { g_lazy = view; }
+static inline const double *or_pos(const orc_state *s) { return g_lazy ? g_lazy->pos : s->pos; }
+static inline const double *or_hpred(const orc_state *s) { return g_lazy ? g_lazy->h : s->h; }
+static inline void or_read(long j) { if (g_lazy) g_lazy->ensure(j); }
+
 /* ------------------------------------------------------------------------- */
 /* uniform cell grid, for the neighbour candidate sets.  Not a reference       */
 /* function: the reference walks a BVH over 64-member Morton tiles               */
@@ -413,12 +421,16 @@ static void or_density_sums(const orc_params *par, const orc_state *s, long i, d
     const double h2 = h * h;
     /* keep the candidates inside h (the pair test below repeats it), ascending */
     long m = 0;
+    const double *P = or_pos(s);
     for (long q = 0; q < ncand; q++) {
         int j = cand[q];
-        double e0 = or_wrap(pi[0] - s->pos[3 * j + 0], box, half);
-        double e1 = or_wrap(pi[1] - s->pos[3 * j + 1], box, half);
-        double e2 = or_wrap(pi[2] - s->pos[3 * j + 2], box, half);
-        if (e0 * e0 + e1 * e1 + e2 * e2 < h2) cand[m++] = j;
+        double e0 = or_wrap(pi[0] - P[3 * j + 0], box, half);
+        double e1 = or_wrap(pi[1] - P[3 * j + 1], box, half);
+        double e2 = or_wrap(pi[2] - P[3 * j + 2], box, half);
+        if (e0 * e0 + e1 * e1 + e2 * e2 < h2) {
+            or_read(j);
+            cand[m++] = j;
+        }
     }
     or_sort_int(cand, m);
     ncand = m;
@@ -727,7 +739,7 @@ int orc_density(const orc_params *par, orc_state *s, const unsigned char *active
     }
     while (1) {
         or_grid g;
-        if (or_grid_build(&g, n, s->pos, box, rmax)) { status = -2; break; }
+        if (or_grid_build(&g, n, or_pos(s), box, rmax)) { status = -2; break; }
         double grid_r = g.cell;
         int need_grow = 0;
 #pragma omp parallel
@@ -848,10 +860,11 @@ int orc_gradients(const orc_params *par, orc_state *s, const unsigned char *acti
 {
     long n = s->n;
     double box = par->box, half = 0.5 * par->box;
-    double rmax = or_max_h(n, s->h, NULL);
+    const double *P = or_pos(s), *Hp = or_hpred(s);
+    double rmax = or_max_h(n, Hp, NULL);
     if (!(rmax > 0) || rmax > half) return -1;
     or_grid g;
-    if (or_grid_build(&g, n, s->pos, box, rmax)) return -2;
+    if (or_grid_build(&g, n, P, box, rmax)) return -2;
 #pragma omp parallel
     {
         int *buf = (int *)malloc((size_t)n * sizeof(int));
@@ -875,7 +888,10 @@ int orc_gradients(const orc_params *par, orc_state *s, const unsigned char *acti
             long m0 = or_grid_candidates(&g, pi, box, buf), m = 0;
             for (long q = 0; q < m0; q++) {
                 int j = buf[q];
-                if (j != i && or_accept(pi, &s->pos[3 * j], h_i, s->h[j], 1, box, half)) buf[m++] = j;
+                if (j != i && or_accept(pi, &P[3 * j], h_i, Hp[j], 1, box, half)) {
+                    or_read(j);
+                    buf[m++] = j;
+                }
             }
             or_sort_int(buf, m);
             for (long q = 0; q < m; q++) {
@@ -1524,10 +1540,11 @@ int orc_force_ex(const orc_params *par, orc_state *s, const unsigned char *activ
     double box = par->box, half = 0.5 * par->box;
     or_gamma G;
     or_gamma_init(&G, par->gamma);
-    double rmax = or_max_h(n, s->h, NULL);
+    const double *P = or_pos(s), *Hp = or_hpred(s);
+    double rmax = or_max_h(n, Hp, NULL);
     if (!(rmax > 0) || rmax > half) return -1;
     or_grid g;
-    if (or_grid_build(&g, n, s->pos, box, rmax)) return -2;
+    if (or_grid_build(&g, n, P, box, rmax)) return -2;
     int status = 0;
     /* the signal speed an inactive neighbour is tested against is its value before this call */
     double *msv_in = (double *)malloc((size_t)(n > 0 ? n : 1) * sizeof(double));
@@ -1554,7 +1571,10 @@ int orc_force_ex(const orc_params *par, orc_state *s, const unsigned char *activ
             long m0 = or_grid_candidates(&g, &s->pos[3 * i], box, buf), m = 0;
             for (long q = 0; q < m0; q++) {
                 int j = buf[q];
-                if (j != i && or_accept(&s->pos[3 * i], &s->pos[3 * j], s->h[i], s->h[j], 1, box, half)) buf[m++] = j;
+                if (j != i && or_accept(&s->pos[3 * i], &P[3 * j], s->h[i], Hp[j], 1, box, half)) {
+                    or_read(j);
+                    buf[m++] = j;
+                }
             }
             or_sort_int(buf, m);
             for (long q = 0; q < m; q++) {

@@ -52,6 +52,7 @@ typedef struct {
     double *soft;             /* P[].ForceSoftening: the kernel radius cached for the acceleration step */
     long long *ti_begstep;    /* P[].Ti_begstep */
     long long *dt_step;       /* P[].dt_step */
+    long long *ti_current;    /* P[].Ti_current: the time the particle's predicted fields stand at */
 } orc_state;
 
 #define ORC_TIMEBINS 60
@@ -63,6 +64,7 @@ typedef struct {              /* All.* parameters of the timestep loop */
     double min_size_timestep;    /* All.MinSizeTimestep */
     double time_begin, time_max; /* All.TimeBegin, All.TimeMax */
     double min_egy_spec;         /* All.MinEgySpec */
+    double domain_build_active_fraction; /* All.DomainBuild_ActiveFraction (full drift above it; 0: every step) */
 } orc_integ;
 
 typedef struct {              /* the integer timeline and the time bins */
@@ -76,7 +78,18 @@ typedef struct {              /* the integer timeline and the time bins */
     int stop_code;                           /* first controlled-stop code raised, else 0 */
     long long n_active;                      /* length of the active list of the last step */
     long long n_woken;                       /* particles moved to a shorter step by the last wake-up pass */
+    int full_drift;                          /* 1: the last step drifted every particle */
 } orc_timeline;
+
+/* The timestep loop's lazy drift (gz_oracle_time.c) as the passes see it: every particle's
+   position and kernel radius predicted at the current time, and the drift of one particle to it,
+   called before a pass reads that particle's fields (thread-safe, once per particle and step).
+   Set for the passes of a lazy step, NULL otherwise. */
+typedef struct {
+    const double *pos, *h;
+    void (*ensure)(long j);
+} orc_lazy;
+void orc_set_lazy(const orc_lazy *view);
 
 int orc_kernel(int kernel, double r, double h, double *wk, double *dwk);
 long orc_neighbors(const orc_params *par, long n, const double *pos, const double *h, int symmetric,
@@ -99,6 +112,7 @@ int orc_force_ex(const orc_params *par, orc_state *s, const unsigned char *activ
 long long orc_get_timestep(const orc_params *par, const orc_integ *ip, const orc_timeline *tl, const orc_state *s, long i);
 int orc_run_begin(const orc_params *par, const orc_integ *ip, orc_timeline *tl, orc_state *s);
 int orc_run_step(const orc_params *par, const orc_integ *ip, orc_timeline *tl, orc_state *s);
+void orc_drift_all(const orc_params *par, const orc_integ *ip, const orc_timeline *tl, orc_state *s);
 void orc_kick(const orc_params *par, const orc_integ *ip, const orc_timeline *tl, orc_state *s, long i,
               long long tstart, long long tend, int mode);
 void orc_drift(const orc_params *par, const orc_integ *ip, const orc_timeline *tl, orc_state *s, long i,

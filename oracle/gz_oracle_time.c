@@ -13,16 +13,29 @@
  * BOX_PERIODIC, no cooling, no comoving integration (cf_atime = cf_a2inv = cf_hubble_a = 1,
  * dt_displacement = MaxSizeTimestep, src/core/timestep.cc:1425-1433), no timestep dilation.
  *
- * Drift policy: every particle is drifted to each sync point (the reference's global drift,
- * move_particles / gizmo_full_drift_to, src/core/predict.cc:148-171, which its large-N
- * neighbour-loop path runs before every loop), so at find_timesteps() every particle's
- * Ti_current equals All.Ti_Current.  After each drift the position is wrapped into
- * [0, box) once per axis, which the reference does at its domain decompositions
- * (do_box_wrapping); the neighbour predicate takes nearest images either way.
+ * Drift policy, the reference's lazy drift: at a sync point only the particles of the active
+ * bins are drifted (find_next_sync_point_and_drift, src/core/run.cc:979-994; move_particles,
+ * src/core/predict.cc:175-210); every other particle stays at its own Ti_current until a pass is
+ * about to read it, and is then drifted to the current time in one piece
+ * (src/mesh/gpu_neighbor_list.cc:2443-2510).  The passes find their pairs on the positions and
+ * radii every particle would have at the current time (the pair test the reference applies
+ * after its drift), and a particle is drifted the first time a pass accepts it as a partner.
+ * (The reference drifts every member its walk's envelope found, a superset of the accepted
+ * partners that depends on its tile bounds; a particle drifted in one piece rather than two
+ * differs only in rounding unless the 0.3 log cap on density and radius or the energy
+ * positivity catch binds within the interval.)  A step whose active bins hold more than
+ * All.DomainBuild_ActiveFraction of the particles drifts every particle first, as the
+ * reference does before its domain decomposition (src/core/run.cc:360,380-390); 0 drifts every
+ * particle at every step.  After each drift the position is wrapped into [0, box) once per
+ * axis, which the reference does at its decompositions (do_box_wrapping); the pair test takes
+ * nearest images either way.
  */
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 #include "gz_oracle.h"
 
@@ -231,7 +244,7 @@ static void ot_process_wake_ups(const orc_params *par, const orc_integ *ip, orc_
             if (bin > max_time_bin_active) bin = max_time_bin_active;
             if (bin >= binold) bin = binold;
             if (bin == binold) continue;
-            const long long ti_current_i = tl->ti_current; /* every particle is drifted to the sync point */
+            const long long ti_current_i = s->ti_current[i];
             const long long old_midpoint = s->ti_begstep[i] + s->dt_step[i] / 2;
             const int new_step_starts_now = (ti_current_i == tl->ti_current) && tl->bin_active[bin] &&
                                             (s->dt_step[i] == ot_bin_len(binold));
@@ -250,6 +263,7 @@ static void ot_process_wake_ups(const orc_params *par, const orc_integ *ip, orc_
                     s->dt_step[i] = ot_bin_len(bin);
                 }
             }
+            if (ti_current_i < tl->ti_current) s->ti_current[i] = tl->ti_current;
             if (new_step_starts_now)
                 orc_kick(par, ip, tl, s, i, s->ti_begstep[i], s->ti_begstep[i] + s->dt_step[i] / 2, 0);
         }
@@ -314,25 +328,120 @@ static void ot_next_sync_point_and_drift(const orc_params *par, const orc_integ 
             if (ti_next_for_bin < ti_next_kick) ti_next_kick = ti_next_for_bin;
         }
     }
-    const long long ti_prev = tl->ti_current;
     tl->num_steps++;
     tl->ti_current = ti_next_kick;
     tl->time = ip->time_begin + tl->ti_current * tl->timebase_interval;
     tl->bin_active[0] = 1;
     for (int n = 1; n < ORC_TIMEBINS; n++) tl->bin_active[n] = ((ti_next_kick % ot_bin_len(n)) == 0);
+    /* NumForceUpdate (src/core/run.cc:942-955) against the domain cadence */
+    long long nf = 0;
+    for (int n = 0; n < ORC_TIMEBINS; n++) if (tl->bin_active[n]) nf += tl->bin_count[n];
+    tl->full_drift = (double)nf > ip->domain_build_active_fraction * (double)s->n;
+    const long long now = tl->ti_current;
 #pragma omp parallel for schedule(static)
-    for (long i = 0; i < s->n; i++) orc_drift(par, ip, tl, s, i, ti_prev, tl->ti_current);
+    for (long i = 0; i < s->n; i++) {
+        if (!tl->full_drift && !tl->bin_active[s->timebin[i]]) continue;
+        orc_drift(par, ip, tl, s, i, s->ti_current[i], now);
+        s->ti_current[i] = now;
+    }
 }
 
-static int ot_hydro(const orc_params *par0, const orc_timeline *tl, orc_state *s, const unsigned char *act, int *need_wakeup)
+/* ------------------------------------------------------------------------- */
+/* the lazy drift's hook for the passes of a step (orc_lazy)                    */
+/* ------------------------------------------------------------------------- */
+static struct {
+    const orc_params *par;
+    const orc_integ *ip;
+    const orc_timeline *tl;
+    orc_state *s;
+    unsigned char *state;  /* per particle: 0 behind, 1 being drifted, 2 at the current time */
+} ot_lazy;
+
+static void ot_ensure(long j)
+{
+    unsigned char *st = &ot_lazy.state[j];
+    if (__atomic_load_n(st, __ATOMIC_ACQUIRE) == 2) return;
+    unsigned char expect = 0;
+    if (__atomic_compare_exchange_n(st, &expect, 1, 0, __ATOMIC_ACQ_REL, __ATOMIC_ACQUIRE)) {
+        orc_drift(ot_lazy.par, ot_lazy.ip, ot_lazy.tl, ot_lazy.s, j, ot_lazy.s->ti_current[j], ot_lazy.tl->ti_current);
+        ot_lazy.s->ti_current[j] = ot_lazy.tl->ti_current;
+        __atomic_store_n(st, 2, __ATOMIC_RELEASE);
+        return;
+    }
+    while (__atomic_load_n(st, __ATOMIC_ACQUIRE) != 2) {}
+}
+
+/* the position and kernel radius orc_drift would give particle i at time1 (the same arithmetic) */
+static void ot_predict(const orc_params *par, const orc_timeline *tl, const orc_state *s, long i, long long time1, double *x, double *h)
+{
+    const long long time0 = s->ti_current[i];
+    for (int k = 0; k < 3; k++) x[k] = s->pos[3 * i + k];
+    *h = s->h[i];
+    if (time1 == time0) return;
+    const double dt_drift = (time1 - time0) * tl->timebase_interval * 1.0;
+    for (int k = 0; k < 3; k++) x[k] += s->vel_kick[3 * i + k] * dt_drift;
+    if (s->mass[i] > 0) {
+        const double h_drift_fac = exp(ot_drift_log_change(s->divvel[i] * dt_drift) / ((double)3));
+        *h *= h_drift_fac;
+        if (*h < par->min_kernel_radius) *h = par->min_kernel_radius;
+        if (*h > par->max_kernel_radius) *h = par->max_kernel_radius;
+    }
+    for (int k = 0; k < 3; k++) {
+        if (x[k] >= par->box) x[k] -= par->box;
+        else if (x[k] < 0) x[k] += par->box;
+    }
+}
+
+static int ot_hydro(const orc_params *par0, const orc_integ *ip, const orc_timeline *tl, orc_state *s, const unsigned char *act,
+                    int *need_wakeup)
 {
     orc_params par = *par0;
     par.time = tl->time;
+    const long n = s->n;
+    int lazy = 0;
+    for (long i = 0; i < n && !lazy; i++) lazy = s->ti_current[i] != tl->ti_current;
+    if (!lazy) {
+        int rc = orc_density(&par, s, act, NULL);
+        if (rc) return rc;
+        rc = orc_gradients(&par, s, act);
+        if (rc) return rc;
+        return orc_force_ex(&par, s, act, need_wakeup);
+    }
+    /* a lazy step: the passes see every particle where the drift would put it now, and drift a
+       particle before they read it */
+    double *pos = (double *)malloc((size_t)n * 3 * sizeof(double));
+    double *hp = (double *)malloc((size_t)n * sizeof(double));
+    unsigned char *state = (unsigned char *)malloc((size_t)n);
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < n; i++) {
+        ot_predict(&par, tl, s, i, tl->ti_current, &pos[3 * i], &hp[i]);
+        state[i] = s->ti_current[i] == tl->ti_current ? 2 : 0;
+    }
+    ot_lazy.par = &par; ot_lazy.ip = ip; ot_lazy.tl = tl; ot_lazy.s = s; ot_lazy.state = state;
+    orc_lazy view = {pos, hp, ot_ensure};
+    orc_set_lazy(&view);
     int rc = orc_density(&par, s, act, NULL);
-    if (rc) return rc;
-    rc = orc_gradients(&par, s, act);
-    if (rc) return rc;
-    return orc_force_ex(&par, s, act, need_wakeup);
+    if (!rc) {
+        /* the rows' new radii (the rows stand at the current time) */
+        for (long i = 0; i < n; i++) if (state[i] == 2) hp[i] = s->h[i];
+        rc = orc_gradients(&par, s, act);
+    }
+    if (!rc) rc = orc_force_ex(&par, s, act, need_wakeup);
+    orc_set_lazy(NULL);
+    free(pos); free(hp); free(state);
+    return rc;
+}
+
+/* gizmo_full_drift_to(All.Ti_Current), src/core/predict.cc:148-171: every particle behind the
+   current sync point drifted to it (before a snapshot) */
+void orc_drift_all(const orc_params *par, const orc_integ *ip, const orc_timeline *tl, orc_state *s)
+{
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < s->n; i++) {
+        if (s->ti_current[i] == tl->ti_current) continue;
+        orc_drift(par, ip, tl, s, i, s->ti_current[i], tl->ti_current);
+        s->ti_current[i] = tl->ti_current;
+    }
 }
 
 /* Start of a run: the timeline and bins of init() (src/core/init.cc:108-113,258-265,583),
@@ -342,6 +451,7 @@ static int ot_hydro(const orc_params *par0, const orc_timeline *tl, orc_state *s
 int orc_run_begin(const orc_params *par0, const orc_integ *ip, orc_timeline *tl, orc_state *s)
 {
     memset(tl, 0, sizeof *tl);
+    tl->full_drift = 1;
     tl->timebase_interval = (ip->time_max - ip->time_begin) / OT_TIMEBASE;
     tl->ti_current = 0;
     tl->time = ip->time_begin;
@@ -350,6 +460,7 @@ int orc_run_begin(const orc_params *par0, const orc_integ *ip, orc_timeline *tl,
         s->timebin[i] = 0;
         s->ti_begstep[i] = 0;
         s->dt_step[i] = 0;
+        s->ti_current[i] = 0;
         s->wakeup[i] = 0;
         for (int k = 0; k < 3; k++) s->vel[3 * i + k] = s->vel_kick[3 * i + k];
         s->u[i] = s->u_kick[i];
@@ -362,7 +473,7 @@ int orc_run_begin(const orc_params *par0, const orc_integ *ip, orc_timeline *tl,
     if (rc) return rc;
     for (long i = 0; i < s->n; i++) s->soft[i] = s->h[i];
     int nw = 0;
-    rc = ot_hydro(&par, tl, s, NULL, &nw);
+    rc = ot_hydro(&par, ip, tl, s, NULL, &nw);
     tl->need_wakeup |= nw;
     tl->n_active = 0;
     for (long i = 0; i < s->n; i++) tl->n_active += (s->mass[i] > 0);
@@ -393,7 +504,7 @@ int orc_run_step(const orc_params *par, const orc_integ *ip, orc_timeline *tl, o
     }
     tl->n_active = na;
     int nw = 0;
-    int rc = ot_hydro(par, tl, s, act, &nw);
+    int rc = ot_hydro(par, ip, tl, s, act, &nw);
     tl->need_wakeup |= nw;
     if (!rc) {
         for (long i = 0; i < n; i++) {

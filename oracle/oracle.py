@@ -38,7 +38,8 @@ class _CState(C.Structure):
                 ("vel", _dp), ("u", _dp), ("rho", _dp), ("press", _dp), ("nvt", _dp),
                 ("nvt_fw", _dp), ("cn", _dp), ("fce", _dp), ("grad_rho", _dp), ("grad_p", _dp),
                 ("grad_v", _dp), ("accel", _dp), ("dtu", _dp), ("maxsig", _dp),
-                ("vel_kick", _dp), ("u_kick", _dp), ("soft", _dp), ("ti_begstep", _llp), ("dt_step", _llp)]
+                ("vel_kick", _dp), ("u_kick", _dp), ("soft", _dp), ("ti_begstep", _llp), ("dt_step", _llp),
+                ("ti_current", _llp)]
 
 
 TIMEBINS = 60
@@ -48,7 +49,8 @@ class Integ(C.Structure):
     """All.* parameters of the timestep loop."""
     _fields_ = [("courant_fac", C.c_double), ("err_tol_int_accuracy", C.c_double),
                 ("max_size_timestep", C.c_double), ("min_size_timestep", C.c_double),
-                ("time_begin", C.c_double), ("time_max", C.c_double), ("min_egy_spec", C.c_double)]
+                ("time_begin", C.c_double), ("time_max", C.c_double), ("min_egy_spec", C.c_double),
+                ("domain_build_active_fraction", C.c_double)]
 
 
 class Timeline(C.Structure):
@@ -56,7 +58,7 @@ class Timeline(C.Structure):
     _fields_ = [("ti_current", C.c_longlong), ("time", C.c_double), ("timebase_interval", C.c_double),
                 ("num_steps", C.c_longlong), ("bin_count", C.c_longlong * TIMEBINS),
                 ("bin_active", C.c_ubyte * TIMEBINS), ("need_wakeup", C.c_int), ("stop_code", C.c_int),
-                ("n_active", C.c_longlong), ("n_woken", C.c_longlong)]
+                ("n_active", C.c_longlong), ("n_woken", C.c_longlong), ("full_drift", C.c_int)]
 
 
 # field name -> columns per particle
@@ -64,7 +66,7 @@ FIELDS = dict(pos=3, mass=1, h=1, numngb=1, drkern=1, divvel=1, vel=3, u=1, rho=
               nvt=6, nvt_fw=3, cn=1, fce=1, grad_rho=3, grad_p=3, grad_v=9, accel=3, dtu=1, maxsig=1,
               vel_kick=3, u_kick=1, soft=1)
 INT_FIELDS = ("timebin", "wakeup")
-I64_FIELDS = ("ti_begstep", "dt_step")
+I64_FIELDS = ("ti_begstep", "dt_step", "ti_current")
 
 
 def build(force=False):
@@ -97,8 +99,9 @@ def lib():
         L.orc_force_ex.argtypes = [C.POINTER(Params), C.POINTER(_CState), _up, _ip]
         L.orc_get_timestep.argtypes = [C.POINTER(Params), C.POINTER(Integ), C.POINTER(Timeline), C.POINTER(_CState), C.c_long]
         L.orc_get_timestep.restype = C.c_longlong
-        for f in (L.orc_run_begin, L.orc_run_step):
+        for f in (L.orc_run_begin, L.orc_run_step, L.orc_drift_all):
             f.argtypes = [C.POINTER(Params), C.POINTER(Integ), C.POINTER(Timeline), C.POINTER(_CState)]
+        L.orc_drift_all.restype = None
         L.orc_kick.argtypes = [C.POINTER(Params), C.POINTER(Integ), C.POINTER(Timeline), C.POINTER(_CState), C.c_long,
                                C.c_longlong, C.c_longlong, C.c_int]
         L.orc_drift.argtypes = [C.POINTER(Params), C.POINTER(Integ), C.POINTER(Timeline), C.POINTER(_CState), C.c_long,
@@ -127,6 +130,7 @@ class State:
         self.wakeup = np.zeros(self.n, dtype=np.int32)
         self.ti_begstep = np.zeros(self.n, dtype=np.int64)
         self.dt_step = np.zeros(self.n, dtype=np.int64)
+        self.ti_current = np.zeros(self.n, dtype=np.int64)
 
     ALL = list(FIELDS) + list(INT_FIELDS) + list(I64_FIELDS)
 
@@ -165,6 +169,7 @@ class State:
         cs.wakeup = _ptr(self.wakeup, _ip)
         cs.ti_begstep = _ptr(self.ti_begstep, _llp)
         cs.dt_step = _ptr(self.dt_step, _llp)
+        cs.ti_current = _ptr(self.ti_current, _llp)
         return cs
 
 
@@ -317,7 +322,8 @@ def num_threads():
 
 
 def integ(d):
-    return Integ(**{k: float(d[k]) for k, _ in Integ._fields_})
+    """orc_integ from a dict; domain_build_active_fraction defaults to 0 (every step drifts every particle)."""
+    return Integ(**{k: float(d.get(k, 0.0) if k == "domain_build_active_fraction" else d[k]) for k, _ in Integ._fields_})
 
 
 class Run:
@@ -341,6 +347,11 @@ class Run:
         if rc:
             raise RuntimeError(f"orc_run_step stop code {rc}")
         return int(self.tl.n_active)
+
+    def drift_all(self):
+        """gizmo_full_drift_to(All.Ti_Current): every particle behind the sync point drifted to it."""
+        cs = self.st._c()
+        lib().orc_drift_all(C.byref(self.par), C.byref(self.ip), C.byref(self.tl), C.byref(cs))
 
     def get_timestep(self, i):
         cs = self.st._c()

@@ -40,7 +40,7 @@ def periodic_diff(a, b, box):
     return np.abs(d - box * np.round(d / box)).max()
 
 
-def run_both(oracle_lib, ic, integ, steps, resort_fraction=None):
+def run_both(oracle_lib, ic, integ, steps, resort_fraction=None, few_rows=None):
     O = oracle_lib
     par = dict(ic["params"])
     st = O.State.from_ic(ic)
@@ -50,8 +50,11 @@ def run_both(oracle_lib, ic, integ, steps, resort_fraction=None):
     g.load(dict(pos=ic["pos"], vel=ic["vel"], mass=ic["mass"], u=ic["u"], h=ic["h"]))
     if resort_fraction is not None:
         g.set_option("resort_fraction", resort_fraction)
+    if few_rows is not None:
+        g.set_option("few_rows", few_rows)
     g.run_begin(integ)
-    print(f"run_both: n={st.n} steps={steps} resort_fraction={resort_fraction}", flush=True)
+    print(f"run_both: n={st.n} steps={steps} resort_fraction={resort_fraction} few_rows={few_rows} "
+          f"domain_build_active_fraction={integ.get('domain_build_active_fraction', 0.0)}", flush=True)
     hist = []
     for k in range(steps):
         na_o = r.step()
@@ -61,8 +64,18 @@ def run_both(oracle_lib, ic, integ, steps, resort_fraction=None):
         assert na_g == na_o, (k, na_g, na_o)
         assert np.array_equal(tl["bin_count"], np.array(r.tl.bin_count[:], dtype=np.int64)), k
         assert tl["n_woken"] == r.tl.n_woken, (k, tl["n_woken"], r.tl.n_woken)
-        hist.append((na_o, int(r.tl.n_woken)))
-    out = g.store(["pos", "timebin", "ti_begstep", "dt_step", "wakeup"] + list(FLOAT_FIELDS))
+        # the device drifts every particle the oracle drifts (its cells around the active rows
+        # hold every partner the oracle's pair test accepts), and at least at the same steps
+        tic = g.store(["ti_current"])["ti_current"]
+        assert (tic >= st.ti_current).all(), k
+        if r.tl.full_drift:
+            assert tl["full_drift"] == 1 and (tic == r.ti_current).all(), k
+        hist.append((na_o, int(r.tl.n_woken), int(r.tl.full_drift), int((tic != r.ti_current).sum())))
+    # every particle to the last sync point on both sides (gizmo_full_drift_to), then compare
+    r.drift_all()
+    g.drift_all()
+    out = g.store(["pos", "timebin", "ti_begstep", "dt_step", "wakeup", "ti_current"] + list(FLOAT_FIELDS))
+    assert (out["ti_current"] == r.ti_current).all() and (st.ti_current == r.ti_current).all()
     out["resorts"] = g.stat("resorts")
     g.close()
     return st, out, hist, par
@@ -80,42 +93,56 @@ def check_fields(st, out, par):
         assert e <= TOL, (k, e)
 
 
-def test_global_steps_lattice(oracle_lib):
-    ic = ics.lattice_box(16, 1001, kernel=3, des_num_ngb=32.0)
+def test_global_steps_lattice_c1(oracle_lib):
+    """The CPU-runnable configuration at its size: 32^3 jittered lattice, cubic spline, 32
+    neighbours, 10 global steps (one forced bin: the maximum step below every Courant step)."""
+    ic = ics.lattice_box(32, 1001, kernel=3, des_num_ngb=32.0)
     integ = dict(courant_fac=0.2, err_tol_int_accuracy=0.02, max_size_timestep=0.002, min_size_timestep=0.0,
                  time_begin=0.0, time_max=1.0, min_egy_spec=0.0)
     st, out, hist, par = run_both(oracle_lib, ic, integ, 10)
-    assert all(na == st.n for na, _ in hist)
+    assert all(h[0] == st.n for h in hist)
     check_fields(st, out, par)
 
 
-@pytest.mark.parametrize("resort_fraction", [None, 0.01])
-def test_adaptive_bins_sedov(oracle_lib, resort_fraction):
+@pytest.mark.parametrize("resort_fraction,few_rows,frac", [(None, None, 0.0), (0.01, None, 0.0), (None, None, 0.5),
+                                                           (0.01, None, 0.5), (None, 0, 0.5), (0.01, 0, 0.5)])
+def test_adaptive_bins_sedov(oracle_lib, resort_fraction, few_rows, frac):
+    """Adaptive bins with the eager drift (DomainBuild_ActiveFraction 0: every particle at every
+    sync point) and the reference's lazy drift (0.5: the blast's steps drift their active bins
+    and the particles their passes read), through the one-wavefront-per-row kernels (few active
+    rows, the default) and the one-lane-per-row kernels (few_rows 0), with and without the
+    particles being ordered again."""
     ic = ics.sedov_box(20, 1002, kernel=6, des_num_ngb=64.0)
     integ = dict(courant_fac=0.05, err_tol_int_accuracy=0.002, max_size_timestep=0.01, min_size_timestep=0.0,
-                 time_begin=0.0, time_max=1.0, min_egy_spec=0.0)
-    st, out, hist, par = run_both(oracle_lib, ic, integ, 40, resort_fraction)
+                 time_begin=0.0, time_max=1.0, min_egy_spec=0.0, domain_build_active_fraction=frac)
+    st, out, hist, par = run_both(oracle_lib, ic, integ, 40, resort_fraction, few_rows)
     if resort_fraction is not None:
         assert out["resorts"] > 0, "the particles were never ordered again"
     del out["resorts"]
-    active = [na for na, _ in hist]
+    active = [h[0] for h in hist]
     assert min(active) < st.n // 10, "bins never split"
-    assert sum(w for _, w in hist) > 0, "no wake-up happened"
+    assert sum(h[1] for h in hist) > 0, "no wake-up happened"
+    if frac > 0:
+        lazy = [h for h in hist if not h[2]]
+        assert len(lazy) >= 20 and max(h[3] for h in lazy) > st.n // 2, "the cold medium was drifted every step"
     check_fields(st, out, par)
 
 
-def test_run_steps_same_as_single_steps():
+@pytest.mark.parametrize("few_rows,frac", [(None, 0.0), (None, 0.5), (0, 0.5)])
+def test_run_steps_same_as_single_steps(few_rows, frac):
     """gz_run_steps(k) takes the same k sync points as k calls of gz_run_step: identical active
     and woken counts per step, timeline and state, bit for bit (the same launches in the same
     order)."""
     ic = ics.sedov_box(20, 1002, kernel=6, des_num_ngb=64.0)
     integ = dict(courant_fac=0.05, err_tol_int_accuracy=0.002, max_size_timestep=0.01, min_size_timestep=0.0,
-                 time_begin=0.0, time_max=1.0, min_egy_spec=0.0)
+                 time_begin=0.0, time_max=1.0, min_egy_spec=0.0, domain_build_active_fraction=frac)
     par = dict(ic["params"])
     outs, counts = [], []
     for multi in (False, True):
         print(f"run_steps: multi={multi}", flush=True)
         g = Hydro(par, len(ic["mass"]), GZ_FP64)
+        if few_rows is not None:
+            g.set_option("few_rows", few_rows)
         g.load(dict(pos=ic["pos"], vel=ic["vel"], mass=ic["mass"], u=ic["u"], h=ic["h"]))
         g.run_begin(integ)
         if multi:
@@ -127,7 +154,7 @@ def test_run_steps_same_as_single_steps():
                 woken.append(g.timeline()["n_woken"])
         tl = g.timeline()
         counts.append((act, woken, tl["ti_current"], tl["num_steps"], tl["bin_count"].tolist()))
-        outs.append(g.store(["pos", "timebin", "ti_begstep", "dt_step"] + list(FLOAT_FIELDS)))
+        outs.append(g.store(["pos", "timebin", "ti_begstep", "dt_step", "ti_current"] + list(FLOAT_FIELDS)))
         g.close()
     assert counts[0] == counts[1]
     assert sum(counts[0][1]) > 0, "no wake-up happened"

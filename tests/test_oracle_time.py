@@ -159,3 +159,113 @@ def test_run_loop_single_bin_conserves_momentum(oracle_lib):
     assert len(set(steps)) == 1 and steps[0] & (steps[0] - 1) == 0
     p1 = (st.mass[:, None] * st.vel_kick).sum(axis=0)
     assert np.abs(p1 - p0).max() <= 1e-13 * scale
+
+
+# ------------------------------------------------------------------------------------------
+# the lazy drift (src/core/run.cc:979-994, src/core/predict.cc:175-210,
+# src/mesh/gpu_neighbor_list.cc:2443-2510) and the wake-up bin rule
+# ------------------------------------------------------------------------------------------
+SEDOV_INTEG = dict(courant_fac=0.05, err_tol_int_accuracy=0.002, max_size_timestep=0.01, min_size_timestep=0.0,
+                   time_begin=0.0, time_max=1.0, min_egy_spec=0.0)
+
+
+def _sedov_run(O, frac, steps, n_side=14):
+    ic = ics.sedov_box(n_side, 1002, kernel=6, des_num_ngb=64.0)
+    par = dict(ic["params"])
+    st = O.State.from_ic(ic)
+    r = O.Run(par, dict(SEDOV_INTEG, domain_build_active_fraction=frac), st)
+    r.begin()
+    hist = []
+    for _ in range(steps):
+        na = r.step()
+        hist.append((r.ti_current, na, int(r.tl.n_woken), int(r.tl.full_drift), tuple(r.tl.bin_count[:])))
+    return r, st, par, hist
+
+
+def test_lazy_drift_reads_only_current_particles(oracle_lib):
+    """After every lazy step, no particle still behind the sync point is a partner of an active
+    particle under the pair test at the current time (its predicted position and radius): every
+    particle a pass read was drifted first, and the active ones always are."""
+    O = oracle_lib
+    ic = ics.sedov_box(14, 1002, kernel=6, des_num_ngb=64.0)
+    par = dict(ic["params"])
+    st = O.State.from_ic(ic)
+    r = O.Run(par, dict(SEDOV_INTEG, domain_build_active_fraction=1.0), st)
+    r.begin()
+    lazy_steps = 0
+    box = par["box"]
+    for _ in range(25):
+        r.step()
+        now = r.ti_current
+        act = np.array([r.tl.bin_active[b] for b in st.timebin], dtype=bool) & (st.mass > 0)
+        behind = st.ti_current != now
+        assert not (act & behind).any()
+        if r.tl.full_drift:
+            assert not behind.any()
+            continue
+        lazy_steps += 1
+        assert behind.any(), "a lazy step of a Sedov blast leaves the cold medium behind"
+        # predicted positions/radii of the particles behind (the oracle's own drift, applied to a copy)
+        cp = st.copy()
+        rc = O.Run(par, dict(SEDOV_INTEG, domain_build_active_fraction=1.0), cp)
+        rc.tl = r.tl
+        rc.drift_all()
+        ai = np.flatnonzero(act)
+        bj = np.flatnonzero(behind)
+        d = cp.pos[ai][:, None, :] - cp.pos[bj][None, :, :]
+        d -= box * np.round(d / box)
+        r2 = (d * d).sum(-1)
+        hm = np.maximum(cp.h[ai][:, None], cp.h[bj][None, :])
+        assert not (r2 < hm * hm).any(), "a partner of an active particle was left behind"
+    assert lazy_steps >= 10
+
+
+def test_lazy_drift_agrees_with_eager_to_rounding(oracle_lib):
+    """Lazy (a particle drifted in one piece when read) and eager (every particle at every sync
+    point) runs take the same integer timeline and agree to rounding once every particle is
+    drifted to the end: the cold medium the lazy run leaves behind has zero velocity divergence
+    and energy rate, so neither the 0.3 log cap nor the positivity catch can bind on it."""
+    O = oracle_lib
+    ra, sa, par, ha = _sedov_run(O, 0.0, 30)
+    rb, sb, _, hb = _sedov_run(O, 0.5, 30)
+    assert [h[:3] + h[4:] for h in ha] == [h[:3] + h[4:] for h in hb]
+    assert all(h[3] == 1 for h in ha) and sum(1 - h[3] for h in hb) >= 20
+    ra.drift_all()
+    rb.drift_all()
+    box = par["box"]
+    d = sa.pos - sb.pos
+    assert np.abs(d - box * np.round(d / box)).max() < 1e-13
+    for k in ("vel", "rho", "u", "h", "press", "accel", "dtu"):
+        a, b = getattr(sa, k), getattr(sb, k)
+        assert np.abs(a - b).max() <= 1e-12 * np.abs(a).max(), k
+
+
+def test_lazy_full_drift_at_the_domain_cadence(oracle_lib):
+    """A step whose active bins hold more than DomainBuild_ActiveFraction of the particles drifts
+    every particle (src/core/run.cc:360,380-390), any other step only the active bins."""
+    O = oracle_lib
+    ic = ics.sedov_box(14, 1002, kernel=6, des_num_ngb=64.0)
+    par = dict(ic["params"])
+    st = O.State.from_ic(ic)
+    r = O.Run(par, dict(SEDOV_INTEG, domain_build_active_fraction=0.5), st)
+    r.begin()
+    seen = set()
+    for _ in range(25):
+        bins_before = np.array(r.tl.bin_count[:])
+        r.step()
+        active = np.array([r.tl.bin_active[b] for b in range(60)], dtype=bool)
+        nf = int(np.array(r.tl.bin_count[:])[active].sum())
+        full = int(r.tl.full_drift)
+        assert full == int(nf > 0.5 * st.n), (nf, full)
+        seen.add(full)
+        if not full:
+            inactive = ~np.array([r.tl.bin_active[b] for b in st.timebin], dtype=bool)
+            assert (st.ti_current[inactive] < r.ti_current).any()
+    assert 0 in seen
+    # DomainBuild_ActiveFraction 0: every step drifts every particle
+    st = O.State.from_ic(ic)
+    r = O.Run(par, dict(SEDOV_INTEG, domain_build_active_fraction=0.0), st)
+    r.begin()
+    for _ in range(10):
+        r.step()
+        assert r.tl.full_drift == 1 and (st.ti_current == r.ti_current).all()
