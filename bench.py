@@ -45,7 +45,7 @@ HBM_PEAK_GBS = 8000.0
 # pair-gather model of the force pass, bytes per particle value-set
 FORCE_MODEL = {"fp64": dict(S_i=288, S_out=40, S_j=288), "mixed": dict(S_i=144, S_out=20, S_j=144)}
 INTEG = dict(courant_fac=0.05, err_tol_int_accuracy=0.002, max_size_timestep=0.01, min_size_timestep=0.0,
-             time_begin=0.0, time_max=1.0, min_egy_spec=0.0)
+             time_begin=0.0, time_max=1.0, min_egy_spec=0.0, domain_build_active_fraction=0.5)
 FAMILIES = ("timeline", "drift", "sort", "ngb_build", "density", "gradients", "force")
 # the drift of one particle (k_drift_active): reads X,Y,Z, VelKick3, DivVel, m, VelPred3, HydroAccel3,
 # rho, u, du/dt, h (18 doubles) and the time bin (int); writes X,Y,Z, VelPred3, rho, u, h, P (10 doubles)

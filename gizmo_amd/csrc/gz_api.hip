@@ -93,6 +93,7 @@ struct gz_context {
     RowGroups grp;
     // the force pass's row order over the lanes (longest lists first within blocks of rows)
     int *sched = nullptr;
+    int *rare = nullptr;  // rows the force pass's common kernel leaves to the whole Riemann ladder
     unsigned long long *sched_part = nullptr;
     int sched_span = 2048;  // rows per sorted block (0: plain order)
     bool sched_valid = false;
@@ -118,6 +119,7 @@ struct gz_context {
     int4 *sten = nullptr;
     double *chmax = nullptr;
     long grid_cells_alloc = 0;
+    int *cmark = nullptr, *dcells = nullptr;  // the lazy drift's cell marks (zero between calls) and its cell list
 
     // lists
     int *ell_d = nullptr, *cnt_d = nullptr;  // density (one-way, buffered)
@@ -165,7 +167,11 @@ struct gz_context {
     int need_wakeup = 0, stop_code = 0;
 
     // stats and timing
-    std::map<std::string, double> stats;
+    // named counters (gz_stat): fixed fields, so that updating them allocates nothing
+    struct Stats {
+        double density_iterations = 0, ngb_rebuilds = 0, n_active = 0, ell_widen = 0, active_updates = 0, resorts = 0,
+               full_drifts = 0, force_rare_rows = 0;
+    } st;
     bool timing = false;
     struct TEv { std::string name; hipEvent_t a, b; };
     std::vector<TEv> pending;
@@ -341,10 +347,10 @@ int alloc_state(gz_context *ctx, long cap)
     // the 64-bit counters and maxima, then the 16 int counters, in one allocation: one copy
     // reads a stage's results back
     CHK(dalloc(ctx, &ctx->u64c, 80 + 8));
-    HIPCHK(hipMemset(ctx->u64c, 0, (80 + 8) * sizeof(unsigned long long)));  // the running pair count starts at zero
+    HIPCHK(hipMemsetAsync(ctx->u64c, 0, (80 + 8) * sizeof(unsigned long long), ctx->stream));  // the running pair count starts at zero
     ctx->counters = (int *)(ctx->u64c + 80);
     CHK(dalloc(ctx, &ctx->rungs, 16));
-    HIPCHK(hipMemset(ctx->rungs, 0, 16 * sizeof(unsigned long long)));
+    HIPCHK(hipMemsetAsync(ctx->rungs, 0, 16 * sizeof(unsigned long long), ctx->stream));
     if (!ctx->hu64c) {
         hipError_t e = hipHostMalloc((void **)&ctx->hu64c, (80 + 8) * sizeof(unsigned long long), hipHostMallocDefault);
         if (e != hipSuccess) return fail(ctx, GZ_E_MEMORY, "hipHostMalloc failed");
@@ -364,7 +370,7 @@ void free_state(gz_context *c)
     dfree(c->wide_dev);
     {
         dfree(c->act_bits); dfree(c->act_bcount);
-        dfree(c->sched); dfree(c->sched_part);
+        dfree(c->sched); dfree(c->sched_part); dfree(c->rare);
         c->sched_valid = false;
         RowGroups &G = c->grp;
         dfree(G.key); dfree(G.skey); dfree(G.gkey); dfree(G.val); dfree(G.row); dfree(G.flag); dfree(G.scan); dfree(G.start);
@@ -382,7 +388,7 @@ void free_state(gz_context *c)
     dfree(c->todo); dfree(c->todo2); dfree(c->cnt_d); dfree(c->cnt_s); dfree(c->ell_d); dfree(c->ell_s);
     c->counters = nullptr;  // inside u64c
     dfree(c->cstart); dfree(c->cend); dfree(c->nsten); dfree(c->sten); dfree(c->chmax);
-    dfree(c->ca0); dfree(c->ca1); dfree(c->cells); dfree(c->cflag); dfree(c->cscan); dfree(c->need);
+    dfree(c->ca0); dfree(c->ca1); dfree(c->cmark); dfree(c->dcells); dfree(c->cells); dfree(c->cflag); dfree(c->cscan); dfree(c->need);
     dfree(c->rec_grad); dfree(c->rec_force); dfree(c->dacc);
     dfree(c->VKX); dfree(c->VKY); dfree(c->VKZ); dfree(c->UK); dfree(c->SOFT); dfree(c->TIB); dfree(c->DTS); dfree(c->TIC);
     dfree(c->u64c);
@@ -399,7 +405,7 @@ void free_state(gz_context *c)
 int upload_field(gz_context *ctx, const double *host, int width, double *dst, bool soa)
 {
     const long n = ctx->n;
-    HIPCHK(hipMemcpyAsync(ctx->stage, host, (size_t)n * width * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(ctx->stage, host, (size_t)n * width * sizeof(double), hipMemcpyDefault, ctx->stream));
     k_gather<<<nblk(n), kBlock, 0, ctx->stream>>>(n, width, ctx->perm, ctx->stage, dst, soa ? 1 : 0);
     HIPCHK(hipGetLastError());
     return 0;
@@ -410,7 +416,7 @@ int download_field(gz_context *ctx, double *host, int width, const double *src, 
     const long n = ctx->n;
     k_scatter<<<nblk(n), kBlock, 0, ctx->stream>>>(n, width, ctx->perm, src, ctx->stage, soa ? 1 : 0);
     HIPCHK(hipGetLastError());
-    HIPCHK(hipMemcpyAsync(host, ctx->stage, (size_t)n * width * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipMemcpyAsync(host, ctx->stage, (size_t)n * width * sizeof(double), hipMemcpyDefault, ctx->stream));
     return 0;
 }
 
@@ -418,7 +424,7 @@ int download_field(gz_context *ctx, double *host, int width, const double *src, 
 int upload_vec3(gz_context *ctx, const double *host, double *dx, double *dy, double *dz)
 {
     const long n = ctx->n;
-    HIPCHK(hipMemcpyAsync(ctx->stage, host, (size_t)n * 3 * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(ctx->stage, host, (size_t)n * 3 * sizeof(double), hipMemcpyDefault, ctx->stream));
     k_gather<<<nblk(n), kBlock, 0, ctx->stream>>>(n, 3, ctx->perm, ctx->stage, ctx->stage + 3 * n, 1);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(dx, ctx->stage + 3 * n, n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
@@ -435,7 +441,7 @@ int download_vec3(gz_context *ctx, double *host, const double *dx, const double 
     HIPCHK(hipMemcpyAsync(ctx->stage + 5 * n, dz, n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
     k_scatter<<<nblk(n), kBlock, 0, ctx->stream>>>(n, 3, ctx->perm, ctx->stage + 3 * n, ctx->stage, 1);
     HIPCHK(hipGetLastError());
-    HIPCHK(hipMemcpyAsync(host, ctx->stage, (size_t)n * 3 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipMemcpyAsync(host, ctx->stage, (size_t)n * 3 * sizeof(double), hipMemcpyDefault, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     return 0;
 }
@@ -464,6 +470,8 @@ int build_grid(gz_context *ctx, double rmax, bool need_chmax = false)
         CHK(dalloc(ctx, &ctx->cstart, ncell)); CHK(dalloc(ctx, &ctx->cend, ncell)); CHK(dalloc(ctx, &ctx->nsten, ncell));
         CHK(dalloc(ctx, &ctx->sten, ncell * 27)); CHK(dalloc(ctx, &ctx->chmax, ncell));
         CHK(dalloc(ctx, &ctx->ca0, ncell)); CHK(dalloc(ctx, &ctx->ca1, ncell));
+        CHK(dalloc(ctx, &ctx->cmark, ncell)); CHK(dalloc(ctx, &ctx->dcells, ncell));
+        HIPCHK(hipMemsetAsync(ctx->cmark, 0, ncell * sizeof(int), ctx->stream));  // the lazy drift's marks stay clear between calls
         ctx->grid_cells_alloc = ncell;
         ctx->grid_version = -1;
     }
@@ -514,8 +522,15 @@ int drift_stencil(gz_context *ctx)
 {
     if (!ctx->lazy || ctx->nact == 0) return 0;
     Timed t(ctx, "drift");
-    k_drift_stencil<<<xcd_grid(ctx->nact), 64, 0, ctx->stream>>>(time_params(ctx), particle_view(ctx), grid_view(ctx), ctx->nact,
-                                                                  ctx->act, ctx->key, ctx->ti_current, ctx->u64c + 75);
+    int *ncells = (int *)(ctx->u64c + 76);  // the cells listed (cleared here, counted by k_stencil_cells)
+    HIPCHK(hipMemsetAsync(ncells, 0, sizeof(int), ctx->stream));
+    const long pairs = 27L * ctx->nact;
+    k_stencil_cells<<<nblk(pairs), kBlock, 0, ctx->stream>>>(grid_view(ctx), ctx->nact, ctx->act, ctx->key, ctx->cmark, ctx->dcells,
+                                                             ncells);
+    const long most = std::min<long>(pairs, 1L << (3 * ctx->level));
+    k_drift_cells<<<(unsigned)std::min<long>(most, 4096), kBlock, 0, ctx->stream>>>(time_params(ctx), particle_view(ctx), grid_view(ctx),
+                                                                                    ctx->dcells, ncells, ctx->cmark, ctx->ti_current,
+                                                                                    ctx->u64c + 75);
     HIPCHK(hipGetLastError());
     return 0;
 }
@@ -722,7 +737,7 @@ int build_list(gz_context *ctx, int nrows, const int *rows, const double *hrow, 
         CHK(dalloc(ctx, ell, (size_t)(*cap) * (size_t)(*ld)));
         nrows = ctx->nact;
         rows = nullptr;
-        ctx->stats["ell_widen"] += 1;
+        ctx->st.ell_widen += 1;
     }
     return fail(ctx, GZ_E_STATE, "neighbour list did not fit after widening");
 }
@@ -745,7 +760,7 @@ int build_active(gz_context *ctx)
     const int na = ctx->hcounters[10];
     ctx->nact = na;
     ctx->act_version++;
-    ctx->stats["n_active"] = na;
+    ctx->st.n_active = na;
     return 0;
 }
 
@@ -767,8 +782,8 @@ template <int KERN, typename R>
 int density_stage(gz_context *ctx)
 {
     const int na = ctx->nact;
-    ctx->stats["density_iterations"] = 0;
-    ctx->stats["ngb_rebuilds"] = 0;
+    ctx->st.density_iterations = 0;
+    ctx->st.ngb_rebuilds = 0;
     if (na == 0) { ctx->density_done = true; ctx->sym_valid = false; return 0; }
     // one launch: the radius clamp, the iteration state, the counters, and the largest radius
     double hmax = ctx->hmax_rows;  // known when the step's timeline pass found it
@@ -828,7 +843,7 @@ int density_stage(gz_context *ctx)
         remaining = ctx->hcounters[0];
         const int rebuild = ctx->hcounters[1], err = ctx->hcounters[2];
         if (err) {
-            ctx->stats["density_iterations"] = iters;
+            ctx->st.density_iterations = iters;
             if (err == GZ_STOP_MAXITER) return fail(ctx, GZ_STOP_MAXITER, "density(): kernel radius iteration did not converge in %d rounds", kMaxIter);
             if (err == GZ_STOP_BRACKET) return fail(ctx, GZ_STOP_BRACKET, "density(): empty radius bracket (Right==0 && Left==0)");
             return fail(ctx, err, "density(): device stop code %d", err);
@@ -840,10 +855,10 @@ int density_stage(gz_context *ctx)
             CHK(build_grid(ctx, std::min(hmax, 0.5 * ctx->par.box_size)));
             CHK(drift_stencil(ctx));
             if (!few) CHK(build_cell_rows(ctx));
-            ctx->stats["ngb_rebuilds"] += 1;
+            ctx->st.ngb_rebuilds += 1;
         }
     }
-    ctx->stats["density_iterations"] = iters;
+    ctx->st.density_iterations = iters;
     ctx->density_done = true;
     ctx->gradients_done = false;
     ctx->sym_valid = false;
@@ -953,7 +968,7 @@ int ensure_sym(gz_context *ctx)
         }
         ctx->cap_s = std::max(over + 16, ctx->cap_s * 3 / 2);
         CHK(dalloc(ctx, &ctx->ell_s, (size_t)ctx->cap_s * (size_t)ctx->ld_s));
-        ctx->stats["ell_widen"] += 1;
+        ctx->st.ell_widen += 1;
     }
     return fail(ctx, GZ_E_STATE, "symmetric neighbour list did not fit after widening");
 }
@@ -1009,19 +1024,29 @@ int force_stage(gz_context *ctx)
         else
             k_pack_force<R><<<nblk(ctx->n), kBlock, 0, ctx->stream>>>(ctx->par.gamma, F, rec, ctx->counters + 2, ctx->par.box_size,
                                                                       BoxTable{});
-        // the kernels that count the faces' Riemann rungs when asked (the same code otherwise)
-        auto rows_kernel = ctx->rung_stats ? k_force_rows<KERN, R, true> : k_force_rows<KERN, R, false>;
-        auto lanes_kernel = ctx->rung_stats ? k_force<KERN, R, false, true> : k_force<KERN, R, false, false>;
-        if (ctx->nact <= ctx->few_rows)
+        if (ctx->nact <= ctx->few_rows) {
+            // the kernel that counts the faces' Riemann rungs when asked (the same code otherwise)
+            auto rows_kernel = ctx->rung_stats ? k_force_rows<KERN, R, true> : k_force_rows<KERN, R, false>;
             rows_kernel<<<xcd_grid(ctx->nact), kRowThreads, 0, ctx->stream>>>(
                 ctx->dp, gam<R>(ctx->dp), ctx->nact, ctx->act, F, rec, ctx->ell_s, ctx->ld_s, ctx->cnt_s,
                 ctx->all_active ? nullptr : ctx->ACT, ctx->TB, ctx->MSV, ctx->WAKE, ctx->ACC, ctx->DTU, ctx->MSV, ctx->counters + 2,
                 ctx->counters + 4, ctx->nact <= kRowPairAtomics ? ctx->u64c + 65 : nullptr, BoxTable{});
-        else
-            lanes_kernel<<<xcd_grid(nblk(ctx->nact)), kBlock, 0, ctx->stream>>>(
-                ctx->dp, gam<R>(ctx->dp), ctx->nact, ctx->act, F, rec, ctx->ell_s, ctx->ld_s, ctx->cnt_s,
-                ctx->all_active ? nullptr : ctx->ACT, ctx->TB, ctx->MSV, ctx->WAKE, ctx->ACC, ctx->DTU, ctx->MSV, ctx->counters + 2,
-                ctx->counters + 4, nullptr, ctx->sched_valid ? ctx->sched : nullptr, BoxTable{});
+        } else {
+            if (!ctx->rare) CHK(dalloc(ctx, &ctx->rare, ctx->cap));
+            const ForceRows a{ctx->nact, ctx->act, ctx->ell_s, ctx->ld_s, ctx->cnt_s, ctx->all_active ? nullptr : ctx->ACT, ctx->TB,
+                              ctx->MSV, ctx->WAKE, ctx->ACC, ctx->DTU, ctx->MSV, ctx->counters + 2, ctx->counters + 4, nullptr,
+                              ctx->sched_valid ? ctx->sched : nullptr, ctx->rare};
+            const dim3 grid = xcd_grid(nblk(ctx->nact));
+            if (ctx->rung_stats) {
+                k_force<KERN, R, false, true, false><<<grid, kBlock, 0, ctx->stream>>>(ctx->dp, gam<R>(ctx->dp), F, rec, a, BoxTable{});
+            } else {
+                // the rows whose faces all take the Riemann ladder's common case, then the others
+                // with the whole ladder
+                k_force<KERN, R, false, false, true><<<grid, kBlock, 0, ctx->stream>>>(ctx->dp, gam<R>(ctx->dp), F, rec, a, BoxTable{});
+                k_force_rare<KERN, R, false><<<std::min(nblk(ctx->nact), kRareBlocks), kBlock, 0, ctx->stream>>>(
+                    ctx->dp, gam<R>(ctx->dp), F, rec, a, BoxTable{});
+            }
+        }
         // the pairs of the pass: each row's workgroup adds its own while a few hundred rows are
         // active (atomics on one address execute one after another)
         if (ctx->nact > ctx->few_rows || ctx->nact > kRowPairAtomics)
@@ -1029,6 +1054,8 @@ int force_stage(gz_context *ctx)
         HIPCHK(hipGetLastError());
     }
     CHK(read_counters(ctx));
+    // rows the common kernel left to the whole Riemann ladder (counters[14]: zero unless it ran)
+    if (ctx->nact > ctx->few_rows && !ctx->rung_stats) ctx->st.force_rare_rows += ctx->hcounters[14];
     if (ctx->lazy && ctx->hmax_bound > 0) {
         // the radii the passes' lazy drifts produced (u64c[75], read back with the counters)
         double hl;
@@ -1042,7 +1069,7 @@ int force_stage(gz_context *ctx)
             // the list the two passes used was cut short: a wider one, and both passes again
             ctx->cap_s = std::max(over + 16, ctx->cap_s * 3 / 2);
             CHK(dalloc(ctx, &ctx->ell_s, (size_t)ctx->cap_s * (size_t)ctx->ld_s));
-            ctx->stats["ell_widen"] += 1;
+            ctx->st.ell_widen += 1;
             ctx->sym_valid = false;
             const bool d = ctx->defer_sym_check;
             ctx->defer_sym_check = false;
@@ -1171,7 +1198,7 @@ int resort(gz_context *ctx)
     ctx->key_version++;
     ctx->sym_valid = false;
     ctx->rows_level = -1;
-    ctx->stats["resorts"] += 1;
+    ctx->st.resorts += 1;
     return 0;
 }
 
@@ -1268,7 +1295,7 @@ int hydro_pass(gz_context *ctx)
     ctx->defer_sym_check = false;
     ctx->sym_check_pending = false;
     if (rc) return rc;
-    ctx->stats["active_updates"] += ctx->nact;
+    ctx->st.active_updates += ctx->nact;
     return 0;
 }
 
@@ -1337,11 +1364,11 @@ int run_step(gz_context *ctx)
     ctx->act_version++;
     ctx->all_active = false;
     ctx->sym_valid = false;
-    ctx->stats["n_active"] = ctx->nact;
+    ctx->st.n_active = ctx->nact;
     // a step that drifted only the active bins: the passes drift the cells they read
     ctx->last_full = ctx->hu64c[72] != 0;
     ctx->lazy = !ctx->last_full;
-    if (ctx->last_full) ctx->stats["full_drifts"] += 1;
+    if (ctx->last_full) ctx->st.full_drifts += 1;
     {
         double d, hm, ha;
         memcpy(&d, &ctx->hu64c[64], sizeof d);
@@ -1460,7 +1487,7 @@ int gz_load(gz_context *ctx, long n, const gz_fields *f)
     ctx->n = n;
     ctx->density_ready = false;
     // order along the Morton curve: keys from the caller's positions, stable radix sort
-    HIPCHK(hipMemcpyAsync(ctx->stage, f->pos, (size_t)n * 3 * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(ctx->stage, f->pos, (size_t)n * 3 * sizeof(double), hipMemcpyDefault, ctx->stream));
     {
         Timed t(ctx, "sort");
         k_morton<<<nblk(n), kBlock, 0, ctx->stream>>>(n, ctx->stage, ctx->par.box_size, ctx->key2, ctx->ival);
@@ -1476,7 +1503,7 @@ int gz_load(gz_context *ctx, long n, const gz_fields *f)
     HIPCHK(hipMemcpyAsync(ctx->X, ctx->stage + 3 * n, n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
     HIPCHK(hipMemcpyAsync(ctx->Y, ctx->stage + 4 * n, n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
     HIPCHK(hipMemcpyAsync(ctx->Z, ctx->stage + 5 * n, n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
-    HIPCHK(hipMemcpyAsync(ctx->stage, f->vel, (size_t)n * 3 * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(ctx->stage, f->vel, (size_t)n * 3 * sizeof(double), hipMemcpyDefault, ctx->stream));
     k_gather<<<nblk(n), kBlock, 0, ctx->stream>>>(n, 3, ctx->perm, ctx->stage, ctx->stage + 3 * n, 1);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(ctx->VX, ctx->stage + 3 * n, n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
@@ -1498,7 +1525,7 @@ int gz_load(gz_context *ctx, long n, const gz_fields *f)
     int *td[2] = {ctx->TB, ctx->WAKE};
     for (int q = 0; q < 2; q++) {
         if (ti[q]) {
-            HIPCHK(hipMemcpyAsync(ctx->ival2, ti[q], n * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+            HIPCHK(hipMemcpyAsync(ctx->ival2, ti[q], n * sizeof(int), hipMemcpyDefault, ctx->stream));
             k_gather_int<<<nblk(n), kBlock, 0, ctx->stream>>>(n, ctx->perm, ctx->ival2, td[q]);
             HIPCHK(hipGetLastError());
         } else {
@@ -1514,7 +1541,7 @@ int gz_load(gz_context *ctx, long n, const gz_fields *f)
     long long *tdst[3] = {ctx->TIB, ctx->DTS, ctx->TIC};
     for (int q = 0; q < 3; q++) {
         if (tl[q]) {
-            HIPCHK(hipMemcpyAsync(ctx->stage, tl[q], n * sizeof(long long), hipMemcpyHostToDevice, ctx->stream));
+            HIPCHK(hipMemcpyAsync(ctx->stage, tl[q], n * sizeof(long long), hipMemcpyDefault, ctx->stream));
             k_gather_ll<<<nblk(n), kBlock, 0, ctx->stream>>>(n, ctx->perm, (const long long *)ctx->stage, tdst[q]);
             HIPCHK(hipGetLastError());
         } else {
@@ -1547,7 +1574,7 @@ int gz_store(gz_context *ctx, gz_fields *f)
         HIPCHK(hipMemcpyAsync(ctx->stage + 4 * n, ctx->Y, n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
         HIPCHK(hipMemcpyAsync(ctx->stage + 5 * n, ctx->Z, n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
         k_scatter<<<nblk(n), kBlock, 0, ctx->stream>>>(n, 3, ctx->perm, ctx->stage + 3 * n, ctx->stage, 1);
-        HIPCHK(hipMemcpyAsync(f->pos, ctx->stage, (size_t)n * 3 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(hipMemcpyAsync(f->pos, ctx->stage, (size_t)n * 3 * sizeof(double), hipMemcpyDefault, ctx->stream));
         HIPCHK(hipStreamSynchronize(ctx->stream));
     }
     if (f->vel) {
@@ -1555,7 +1582,7 @@ int gz_store(gz_context *ctx, gz_fields *f)
         HIPCHK(hipMemcpyAsync(ctx->stage + 4 * n, ctx->VY, n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
         HIPCHK(hipMemcpyAsync(ctx->stage + 5 * n, ctx->VZ, n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
         k_scatter<<<nblk(n), kBlock, 0, ctx->stream>>>(n, 3, ctx->perm, ctx->stage + 3 * n, ctx->stage, 1);
-        HIPCHK(hipMemcpyAsync(f->vel, ctx->stage, (size_t)n * 3 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(hipMemcpyAsync(f->vel, ctx->stage, (size_t)n * 3 * sizeof(double), hipMemcpyDefault, ctx->stream));
         HIPCHK(hipStreamSynchronize(ctx->stream));
     }
     struct Opt { double *dst; int w; const double *src; } opts[] = {
@@ -1573,7 +1600,7 @@ int gz_store(gz_context *ctx, gz_fields *f)
     for (int q = 0; q < 2; q++) {
         if (!ti[q]) continue;
         k_scatter_int<<<nblk(n), kBlock, 0, ctx->stream>>>(n, ctx->perm, td[q], ctx->ival2);
-        HIPCHK(hipMemcpyAsync(ti[q], ctx->ival2, n * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(hipMemcpyAsync(ti[q], ctx->ival2, n * sizeof(int), hipMemcpyDefault, ctx->stream));
         HIPCHK(hipStreamSynchronize(ctx->stream));
     }
     if (f->vel_kick) CHK(download_vec3(ctx, f->vel_kick, ctx->VKX, ctx->VKY, ctx->VKZ));
@@ -1589,7 +1616,7 @@ int gz_store(gz_context *ctx, gz_fields *f)
         if (!tl[q]) continue;
         k_scatter_ll<<<nblk(n), kBlock, 0, ctx->stream>>>(n, ctx->perm, tsrc[q], (long long *)ctx->stage);
         HIPCHK(hipGetLastError());
-        HIPCHK(hipMemcpyAsync(tl[q], ctx->stage, n * sizeof(long long), hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(hipMemcpyAsync(tl[q], ctx->stage, n * sizeof(long long), hipMemcpyDefault, ctx->stream));
         HIPCHK(hipStreamSynchronize(ctx->stream));
     }
     return 0;
@@ -1693,16 +1720,14 @@ int gz_stat(gz_context *ctx, const char *name, double *value)
                 return 0;
             }
     }
-    static const char *known[] = {"density_iterations", "ngb_rebuilds", "n_active", "ell_widen", "active_updates", "resorts",
-                                  "full_drifts"};
-    auto it = ctx->stats.find(k);
-    if (it == ctx->stats.end()) {
-        for (const char *kn : known)
-            if (k == kn) { *value = 0; return 0; }
-        return fail(ctx, GZ_E_ARG, "unknown stat %s", name);
-    }
-    *value = it->second;
-    return 0;
+    using S = gz_context::Stats;
+    static const struct { const char *name; double S::*v; } known[] = {
+        {"density_iterations", &S::density_iterations}, {"ngb_rebuilds", &S::ngb_rebuilds}, {"n_active", &S::n_active},
+        {"ell_widen", &S::ell_widen}, {"active_updates", &S::active_updates}, {"resorts", &S::resorts},
+        {"full_drifts", &S::full_drifts}, {"force_rare_rows", &S::force_rare_rows}};
+    for (const auto &kn : known)
+        if (k == kn.name) { *value = ctx->st.*kn.v; return 0; }
+    return fail(ctx, GZ_E_ARG, "unknown stat %s", name);
 }
 
 int gz_timing_enable(gz_context *ctx, int on)
@@ -1717,9 +1742,10 @@ int gz_timing_reset(gz_context *ctx)
     if (!ctx) return GZ_E_ARG;
     collect_timings(ctx);
     ctx->timings.clear();
-    ctx->stats["active_updates"] = 0;
-    ctx->stats["resorts"] = 0;
-    ctx->stats["full_drifts"] = 0;
+    ctx->st.active_updates = 0;
+    ctx->st.resorts = 0;
+    ctx->st.force_rare_rows = 0;
+    ctx->st.full_drifts = 0;
     HIPCHK(hipMemsetAsync(ctx->u64c + 65, 0, sizeof(unsigned long long), ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     return 0;

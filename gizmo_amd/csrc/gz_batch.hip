@@ -80,6 +80,7 @@ struct gz_batch {
     int *it_ci = nullptr, *it_it = nullptr;
     void *rec_grad = nullptr, *rec_force = nullptr;
     int *ell = nullptr, *cnt = nullptr;
+    int *rare = nullptr;  // rows the force pass's common kernel leaves to the whole Riemann ladder
     int ell_cap = 0;
     void *tmp = nullptr;
     size_t tmp_bytes = 0;
@@ -224,7 +225,7 @@ int alloc_all(gz_batch *b)
     BCHK(balloc(b, &b->T, 6 * n)); BCHK(balloc(b, &b->FW, 3 * n)); BCHK(balloc(b, &b->GR, 3 * n)); BCHK(balloc(b, &b->GP, 3 * n));
     BCHK(balloc(b, &b->GV, 9 * n)); BCHK(balloc(b, &b->ACC, 3 * n)); BCHK(balloc(b, &b->stage, 9 * n)); BCHK(balloc(b, &b->dacc, 13 * n));
     BCHK(balloc(b, &b->TIB, n)); BCHK(balloc(b, &b->DTS, n));
-    int **i1[] = {&b->TB, &b->WAKE, &b->BOX, &b->perm, &b->ival, &b->ival2, &b->act, &b->it_ci, &b->it_it, &b->cnt};
+    int **i1[] = {&b->TB, &b->WAKE, &b->BOX, &b->perm, &b->ival, &b->ival2, &b->act, &b->it_ci, &b->it_it, &b->cnt, &b->rare};
     for (auto p : i1) BCHK(balloc(b, p, n));
     BCHK(balloc(b, &b->iflag, n + 1)); BCHK(balloc(b, &b->iscan, n + 1));
     BCHK(balloc(b, &b->ACT, n)); BCHK(balloc(b, &b->need, n));
@@ -252,7 +253,7 @@ void free_all(gz_batch *c)
                      &c->ACC, &c->stage, &c->dacc, &c->chmax};
     for (auto p : d1) bfree(*p);
     int **i1[] = {&c->TB, &c->WAKE, &c->BOX, &c->perm, &c->ival, &c->ival2, &c->act, &c->it_ci, &c->it_it, &c->cnt, &c->iflag,
-                  &c->iscan, &c->rank, &c->lim, &c->counters, &c->ell, &c->cstart, &c->cend};
+                  &c->iscan, &c->rank, &c->lim, &c->counters, &c->ell, &c->cstart, &c->cend, &c->rare};
     for (auto p : i1) bfree(*p);
     bfree(c->TIB); bfree(c->DTS); bfree(c->ACT); bfree(c->need); bfree(c->sel); bfree(c->due);
     bfree(c->key); bfree(c->key64); bfree(c->key64b); bfree(c->rec_grad); bfree(c->rec_force); bfree(c->tab);
@@ -625,9 +626,12 @@ int gradients_force(gz_batch *b)
     else
         k_pack_force<R><<<nb(b->cap), kB, 0, b->stream>>>(b->par.gamma, F, rf, b->counters + 2, 1.0, bt);
     if (lanes) {
-        k_force<KERN, R, true><<<xcd_grid(nb(na)), kB, 0, b->stream>>>(b->dp, G, na, b->act, F, rf, b->ell, ld, b->cnt, b->ACT, b->TB, b->MSV,
-                                                                  b->WAKE, b->ACC, b->DTU, b->MSV, b->counters + 2, b->counters + 4, nullptr,
-                                                                  nullptr, bt);
+        // the rows whose faces all take the Riemann ladder's common case, then the others with the
+        // whole ladder
+        const ForceRows a{na, b->act, b->ell, ld, b->cnt, b->ACT, b->TB, b->MSV, b->WAKE, b->ACC, b->DTU, b->MSV, b->counters + 2,
+                          b->counters + 4, nullptr, nullptr, b->rare};
+        k_force<KERN, R, true, false, true><<<xcd_grid(nb(na)), kB, 0, b->stream>>>(b->dp, G, F, rf, a, bt);
+        k_force_rare<KERN, R, true><<<std::min(nb(na), kRareBlocks), kB, 0, b->stream>>>(b->dp, G, F, rf, a, bt);
     } else {
         k_force_rows<KERN, R><<<xcd_grid(na), kRowThreads, 0, b->stream>>>(b->dp, G, na, b->act, F, rf, b->ell, ld, b->cnt, b->ACT, b->TB, b->MSV,
                                                                    b->WAKE, b->ACC, b->DTU, b->MSV, b->counters + 2, b->counters + 4,

@@ -730,10 +730,49 @@ __device__ inline bool riemann_ladder(const Gam<T> &G, RState<T> L, RState<T> Rs
     return true;
 }
 
-// returns 0 skipped, 1 fluxes produced (maybe zero), -1 no Riemann solution (stop 1234); rung
-// receives the RiemannRung bits of the face (callers that do not count them leave it unused)
-template <int KERN, typename R>
-__device__ inline int force_pair(const DevParams &P, const Gam<R> &G, const Cell<R> &I, const Cell<R> &J, R dp0, R dp1, R dp2, R r2,
+// The common case of the ladder alone: physical reconstructed states whose wave-speed estimate
+// (HLLC, Roe or PVRS) gives a positive pressure within the limiter.  The ladder then takes no
+// other rung, so the result is the ladder's, bit for bit.  false: the face needs the rest of the
+// ladder (KT, the exact solver or a retry), which a kernel that calls this leaves to a second
+// pass over its rows (the rare rungs' code then costs the common pass no registers).
+template <typename T>
+__device__ __forceinline__ bool riemann_common(const Gam<T> &G, const RState<T> &L, const RState<T> &Rs, const T *n, T ptl,
+                                               ROut<T> &o, unsigned &rung)
+{
+    if ((L.p < 0 && Rs.p < 0) || (L.rho < 0) || (Rs.rho < 0)) return false;
+    o.P_M = o.S_M = o.f0 = o.f1 = o.f2 = o.fp = 0;
+    const RPrep<T> q = riemann_prep(G, L, Rs, n);
+    const unsigned r = wavespeeds(G, L, Rs, n, q.vL, q.vR, q.csL, q.csR, q.hL, q.hR, o, ptl);
+    if ((o.P_M < 0) || (isnan(o.P_M)) || (o.P_M > ptl)) return false;
+    o.f0 = o.P_M * n[0];
+    o.f1 = o.P_M * n[1];
+    o.f2 = o.P_M * n[2];
+    o.fp = o.P_M * o.S_M;
+    rung |= r;
+    return true;
+}
+
+// A row's force record with the fields of the reconstruction, the faces and the entropic switch
+// in LDS (ColdRec, 26 values: 208 B per lane in FP64, three workgroups of 256 per CU) and the
+// rest in registers: the record of i then costs a pair loop 12 registers instead of 64.
+template <typename R> struct ColdRec {
+    R m, p, cn, fce, drk, gr0, gr1, gr2, gp0, gp1, gp2;
+    R gv[9];
+    R t[6];
+};
+template <typename R> struct SplitCell {
+    R h, rho, cs, v0, v1, v2;
+    const R &m, &p, &cn, &fce, &drk, &gr0, &gr1, &gr2, &gp0, &gp1, &gp2;
+    const R (&gv)[9];
+    const R (&t)[6];
+};
+
+// returns 0 skipped, 1 fluxes produced (maybe zero), -1 no Riemann solution (stop 1234), 2 (COMMON
+// only) the face needs a rare rung of the ladder; rung receives the RiemannRung bits of the face
+// (callers that do not count them leave it unused).  COMMON: the Riemann solve takes only the
+// ladder's common case (riemann_common).  IC: the record of i (Cell<R> or SplitCell<R>).
+template <int KERN, typename R, bool COMMON = false, typename IC = Cell<R>>
+__device__ inline int force_pair(const DevParams &P, const Gam<R> &G, const IC &I, const Cell<R> &J, R dp0, R dp1, R dp2, R r2,
                                  R &F0, R &F1, R &F2, R &Fp, R &vsig_ts, unsigned &rung)
 {
     rung = 0;
@@ -831,9 +870,13 @@ __device__ inline int force_pair(const DevParams &P, const Gam<R> &G, const Cell
     if (recon == 0) ptl = dmax(ptl, dmax(dmax(I.p, J.p), R(2.) * dmax(I.rho, J.rho) * v2a));
     ROut<R> o;
     rung = kRungFace;
-    if (!riemann_ladder<R>(G, L, Rs, n, ptl, I.p, J.p, I.rho, J.rho, I.v0 - vf0, I.v1 - vf1, I.v2 - vf2, J.v0 - vf0, J.v1 - vf1,
-                           J.v2 - vf2, o, rung))
-        return -1;
+    if constexpr (COMMON) {
+        if (!riemann_common<R>(G, L, Rs, n, ptl, o, rung)) return 2;
+    } else {
+        if (!riemann_ladder<R>(G, L, Rs, n, ptl, I.p, J.p, I.rho, J.rho, I.v0 - vf0, I.v1 - vf1, I.v2 - vf2, J.v0 - vf0, J.v1 - vf1,
+                               J.v2 - vf2, o, rung))
+            return -1;
+    }
     if (!((o.P_M > 0) && (!isnan(o.P_M)))) return 1;
     o.fp += vf0 * o.f0 + vf1 * o.f1 + vf2 * o.f2;
     Fp = An * o.fp;

@@ -202,10 +202,12 @@ __global__ void k_active_compact(long n, int nblocks, const unsigned long long *
                                  DensityStart ds);
 // the reference's lazy drift (src/core/predict.cc:175-210, src/mesh/gpu_neighbor_list.cc:2443-2510):
 // every particle of the 27 grid cells around each active row's home cell that stands behind
-// time1 is drifted to it before a pass reads the cells (one workgroup of 64 per row); every
-// particle behind time1 (before a re-ordering or a snapshot); the bound on the kicked velocities
-__global__ void k_drift_stencil(TimeParams T, ParticleView S, GridView g, int nrows, const int *act, const uint32_t *key,
-                                long long time1, unsigned long long *hmax);
+// time1 is drifted to it before a pass reads the cells (the cells listed once each, then one
+// workgroup of 256 per cell); every particle behind time1 (before a re-ordering or a
+// snapshot); the bound on the kicked velocities
+__global__ void k_stencil_cells(GridView g, int nrows, const int *act, const uint32_t *key, int *cmark, int *cells, int *ncells);
+__global__ void k_drift_cells(TimeParams T, ParticleView S, GridView g, const int *cells, const int *ncells, int *cmark,
+                              long long time1, unsigned long long *hmax);
 __global__ void k_drift_behind(TimeParams T, ParticleView S, long long time1, unsigned long long *hmax);
 __global__ void k_vk_max(ParticleView S);
 __global__ void k_advance_boxes(BoxTable bt);
@@ -335,11 +337,37 @@ __global__ void k_force_rows(DevParams P, Gam<R> G, int nrows, const int *act, F
                              double *ACC, double *DTU, double *MSV, int *err, int *need_wakeup, unsigned long long *pairs,
                              BoxTable bt);
 
-template <int KERN, typename R, bool BOXES, bool STATS = false>
-__global__ void k_force(DevParams P, Gam<R> G, int nrows, const int *act, ForceView F, const Cell<R> *rec, const int *ell,
-                        long ld, const int *cnt, const uint8_t *active, const int *timebin, const double *msv_in, int *wakeup,
-                        double *ACC, double *DTU, double *MSV, int *err, int *need_wakeup, unsigned long long *pairs,
-                        const int *sched, BoxTable bt);
+// the rows, lists and outputs of a force pass's lane-per-row kernels
+struct ForceRows {
+    int nrows;
+    const int *act;              // the rows' particles
+    const int *ell;              // symmetric list, column-major with leading dimension ld
+    long ld;
+    const int *cnt;              // list lengths
+    const uint8_t *active;       // active flags (nullptr: every particle active)
+    const int *timebin;          // time bins (the wake-up bin of a flagged neighbour)
+    const double *msv_in;        // MaxSignalVel before the pass (the wake-up test)
+    int *wakeup;                 // wake-up flags (nullptr: none)
+    double *ACC, *DTU, *MSV;     // outputs
+    int *err;                    // err[0] stop code; err[10], err[11] the first pair without a solution; err[12] rare rows
+    int *need_wakeup;            // set when a neighbour is flagged (one box)
+    unsigned long long *pairs;   // the pass's pair count (nullptr: not counted)
+    const int *sched;            // row order (nullptr: 0..nrows-1)
+    int *rare;                   // rows the common pass leaves to the whole ladder
+};
+
+// Lane per row.  STATS: count the faces' RiemannRung bits into F.rungs.  COMMON: every face takes
+// only the Riemann ladder's common case (riemann_common); a row with a face that needs more is
+// not written but appended to a.rare (count in a.err[12]), for k_force_rare.  The common pass
+// then carries no register cost of the rare rungs.
+template <int KERN, typename R, bool BOXES, bool STATS = false, bool COMMON = false>
+__global__ void k_force(DevParams P, Gam<R> G, ForceView F, const Cell<R> *rec, ForceRows a, BoxTable bt);
+
+// the rows that k_force<..., COMMON> listed, each with the whole ladder (a fixed grid strides over
+// the count the device holds; the rows' results are the single-pass kernel's, bit for bit)
+constexpr unsigned kRareBlocks = 1024;  // workgroups of 256 at most that k_force_rare launches
+template <int KERN, typename R, bool BOXES>
+__global__ void k_force_rare(DevParams P, Gam<R> G, ForceView F, const Cell<R> *rec, ForceRows a, BoxTable bt);
 
 // the Riemann functions on given face states (diagnostic; layout of in/out at the definition)
 constexpr int kProbeIn = 24, kProbeOut = 8;

@@ -21,10 +21,9 @@ template __global__ void k_build_ell<true>(int, const int *, const int *, const 
                                                          const double *, const GradRec<R> *, const double *, const double *,    \
                                                          long, const int *, long, const int *, double *, double *, double *,    \
                                                          BoxTable);                                                              \
-    template __global__ void k_force<KERN, R, BOXES>(DevParams, Gam<R>, int, const int *, ForceView, const Cell<R> *,           \
-                                                     const int *, long, const int *, const uint8_t *, const int *,              \
-                                                     const double *, int *, double *, double *, double *, int *, int *,         \
-                                                     unsigned long long *, const int *, BoxTable);
+    template __global__ void k_force<KERN, R, BOXES, false, true>(DevParams, Gam<R>, ForceView, const Cell<R> *, ForceRows,   \
+                                                               BoxTable);                                                  \
+    template __global__ void k_force_rare<KERN, R, BOXES>(DevParams, Gam<R>, ForceView, const Cell<R> *, ForceRows, BoxTable);
 
 #define GZ_INST_DROWS(KERN, R, TPB)                                                                                             \
     template __global__ void k_density_rows<KERN, R, TPB>(DevParams, GridView, int, const int *, const uint8_t *,               \
@@ -52,10 +51,8 @@ GZ_INST(3, double)
 GZ_INST(6, double)
 // the force kernels that count the Riemann rungs of their faces (one box)
 #define GZ_INST_STATS(KERN, R)                                                                                                   \
-    template __global__ void k_force<KERN, R, false, true>(DevParams, Gam<R>, int, const int *, ForceView, const Cell<R> *,     \
-                                                           const int *, long, const int *, const uint8_t *, const int *,        \
-                                                           const double *, int *, double *, double *, double *, int *, int *,   \
-                                                           unsigned long long *, const int *, BoxTable);                        \
+    template __global__ void k_force<KERN, R, false, true, false>(DevParams, Gam<R>, ForceView, const Cell<R> *, ForceRows,   \
+                                                               BoxTable);                        \
     template __global__ void k_force_rows<KERN, R, true>(DevParams, Gam<R>, int, const int *, ForceView, const Cell<R> *,      \
                                                          const int *, long, const int *, const uint8_t *, const int *,          \
                                                          const double *, int *, double *, double *, double *, int *, int *,     \

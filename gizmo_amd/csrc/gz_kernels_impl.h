@@ -802,10 +802,11 @@ __device__ __forceinline__ void pack_force_one(long j, double gamma, const Force
     rec[j] = c;
 }
 
-// the force pass's error and wake-up flags, and the first pair without a Riemann solution
+// the force pass's error and wake-up flags, the first pair without a Riemann solution and the
+// count of rows left to the whole Riemann ladder
 __device__ __forceinline__ void force_flags_clear(int *zero2)
 {
-    zero2[0] = 0; zero2[2] = 0; zero2[10] = -1; zero2[11] = -1;
+    zero2[0] = 0; zero2[2] = 0; zero2[10] = -1; zero2[11] = -1; zero2[12] = 0;
 }
 
 template <typename R>
@@ -1027,76 +1028,124 @@ __device__ inline void force_tail(const DevParams &P, const ForceView &F, int i,
     MSV[i] = ms;
 }
 
-template <int KERN, typename R, bool BOXES, bool STATS>
-__global__ void __launch_bounds__(256) k_force(DevParams P, Gam<R> G, int nrows, const int *__restrict__ act, ForceView F,
-                                               const Cell<R> *__restrict__ rec, const int *__restrict__ ell, long ld,
-                                               const int *__restrict__ cnt, const uint8_t *__restrict__ active,
-                                               const int *__restrict__ timebin, const double *__restrict__ msv_in,
-                                               int *__restrict__ wakeup, double *__restrict__ ACC, double *__restrict__ DTU,
-                                               double *__restrict__ MSV, int *__restrict__ err, int *__restrict__ need_wakeup,
-                                               unsigned long long *__restrict__ pairs, const int *__restrict__ sched, BoxTable bt)
+// the record of i a pair loop reads: the record itself, or its split over registers and LDS
+template <typename R>
+__device__ __forceinline__ Cell<R> make_irec(const Cell<R> &I, const ColdRec<R> &, std::false_type) { return I; }
+template <typename R>
+__device__ __forceinline__ SplitCell<R> make_irec(const Cell<R> &I, const ColdRec<R> &C, std::true_type)
 {
-    const int k = (int)(xcd_block() * blockDim.x + threadIdx.x);
-    const int r = k < nrows ? (sched ? sched[k] : k) : nrows;
-    // pairs of this launch when asked (wave sum, one atomic per wave; a launch of k_sum_counts
-    // after the pass costs fewer atomics on the one address)
-    if (pairs) {
-        unsigned long long c = (r < nrows) ? (unsigned long long)cnt[r] : 0ull;
-        for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
-        if ((threadIdx.x & 63) == 0 && c) atomicAdd(pairs, c);
-    }
-    if (r >= nrows) return;
-    const int i = act[r];
+    return SplitCell<R>{I.h, I.rho, I.cs, I.v0, I.v1, I.v2, C.m, C.p, C.cn, C.fce, C.drk, C.gr0, C.gr1, C.gr2, C.gp0, C.gp1, C.gp2,
+                        C.gv, C.t};
+}
+
+// One row of the lane-per-row force kernels; false (COMMON only): a face of the row needs a rare
+// rung of the Riemann ladder, and nothing of the row was written but its neighbours' wake-up
+// flags (which the whole-ladder pass over the row sets again, to the same values)
+template <int KERN, typename R, bool BOXES, bool STATS, bool COMMON>
+__device__ __forceinline__ bool force_row(const DevParams &P0, const Gam<R> &G, const ForceView &F, const Cell<R> *__restrict__ rec,
+                                          const ForceRows &a, const BoxTable &bt, int r)
+{
+    DevParams P = P0;
+    const int i = a.act[r];
     if (BOXES) {
         GridView g{};
         resolve_box(bt, i, P, g);
     }
-    const long n = F.n;
     // hydro_force_initial_operations_preloop (src/hydro/hydro_toplevel.cc:457-565): an active
     // particle's own flag is cleared; only inactive neighbours are flagged below
-    if (wakeup) wakeup[i] = 0;
+    if (a.wakeup) a.wakeup[i] = 0;
     const double mi = F.M[i], rhoi = F.RHO[i];
     double A0 = 0, A1 = 0, A2 = 0, dE = 0, msv = 0;
     // effective_soundspeed of i, src/declarations/cell_data.h:703-712 (floor of MaxSignalVel)
     double sound_i = (rhoi > 0) ? sqrt(P.gamma * F.PR[i] / rhoi) : sqrt(P.gamma * (P.gamma - 1.) * F.U[i]);
     const bool enabled = (mi > 0 && rhoi > 0);
     if (enabled) {
-        const Cell<R> I = rec[i];
+        const Cell<R> I0 = rec[i];
+        // the FP64 common pass keeps most of i's record in LDS (SplitCell); the other kernels
+        // hold it in registers
+        constexpr bool split = COMMON && std::is_same<R, double>::value;
+        __shared__ ColdRec<R> cold[split ? 256 : 1];
+        ColdRec<R> &C = cold[split ? threadIdx.x : 0];
+        if constexpr (split) {
+            C.m = I0.m; C.p = I0.p; C.cn = I0.cn; C.fce = I0.fce; C.drk = I0.drk;
+            C.gr0 = I0.gr0; C.gr1 = I0.gr1; C.gr2 = I0.gr2; C.gp0 = I0.gp0; C.gp1 = I0.gp1; C.gp2 = I0.gp2;
+            for (int k = 0; k < 9; k++) C.gv[k] = I0.gv[k];
+            for (int k = 0; k < 6; k++) C.t[k] = I0.t[k];
+        }
+        using IC = typename std::conditional<split, SplitCell<R>, Cell<R>>::type;
+        const IC I = make_irec<R>(I0, C, std::integral_constant<bool, split>{});
         const double xi = F.X[i], yi = F.Y[i], zi = F.Z[i];
-        const int tbi = timebin ? timebin[i] : 0;
-        const int c = cnt[r];
+        const int tbi = a.timebin ? a.timebin[i] : 0;
+        const int c = a.cnt[r];
         CSum<R> a0, a1, a2, ae;
         R m = 0;
         for (int q = 0; q < c; q++) {
-            const int j = ell[(size_t)q * ld + r];
+            // LDS reads of i's record stay inside the loop (not hoisted into registers)
+            if constexpr (split) __asm__ volatile("" ::: "memory");
+            const int j = a.ell[(size_t)q * a.ld + r];
             if (m < I.cs) m = I.cs;
             const Cell<R> J = rec[j];
             R dp0, dp1, dp2;
-            record_dp<R>(P, I, J, xi, yi, zi, F.X, F.Y, F.Z, j, dp0, dp1, dp2);
+            record_dp<R>(P, I0, J, xi, yi, zi, F.X, F.Y, F.Z, j, dp0, dp1, dp2);
             const R r2 = dp0 * dp0 + dp1 * dp1 + dp2 * dp2;
             R f0, f1, f2, fp, vts;
             unsigned rung;
-            int rc = force_pair<KERN, R>(P, G, I, J, dp0, dp1, dp2, r2, f0, f1, f2, fp, vts, rung);
+            int rc = force_pair<KERN, R, COMMON>(P, G, I, J, dp0, dp1, dp2, r2, f0, f1, f2, fp, vts, rung);
             if constexpr (STATS) count_rungs(F.rungs, rung);
+            if constexpr (COMMON) {
+                if (rc == 2) return false;
+            }
             if (rc < 0) {
-                atomicMax(err, 1234);
-                if (atomicCAS(err + 10, -1, i) == -1) err[11] = j;
+                atomicMax(a.err, 1234);
+                if (atomicCAS(a.err + 10, -1, i) == -1) a.err[11] = j;
                 continue;
             }
             if (rc == 0) continue;
             a0.add(rounded(f0)); a1.add(rounded(f1)); a2.add(rounded(f2)); ae.add(rounded(fp));
             if (vts > m) m = vts;
-            if (active && !active[j] && wakeup) {
-                if ((double)vts > kWakeup * msv_in[j]) {
-                    atomicMax(&wakeup[j], tbi + 1);
+            if (a.active && !a.active[j] && a.wakeup) {
+                if ((double)vts > kWakeup * a.msv_in[j]) {
+                    atomicMax(&a.wakeup[j], tbi + 1);
                     if (BOXES) bt.b[bt.BOX[i]].need_wakeup = 1;
-                    else *need_wakeup = 1;
+                    else *a.need_wakeup = 1;
                 }
             }
         }
         A0 = a0.value(); A1 = a1.value(); A2 = a2.value(); dE = ae.value(); msv = m;
     }
-    force_tail(P, F, i, enabled, sound_i, A0, A1, A2, dE, msv, ACC, DTU, MSV);
+    force_tail(P, F, i, enabled, sound_i, A0, A1, A2, dE, msv, a.ACC, a.DTU, a.MSV);
+    return true;
+}
+
+// the FP64 common pass of one box at three waves per SIMD (166 registers; its LDS record: 52 KB
+// per workgroup); with the box table it takes 178 (two waves, no spill)
+template <int KERN, typename R, bool BOXES, bool STATS, bool COMMON>
+__global__ void __launch_bounds__(256, (COMMON && !BOXES && std::is_same<R, double>::value) ? 3 : 1) k_force(DevParams P, Gam<R> G, ForceView F, const Cell<R> *__restrict__ rec, ForceRows a,
+                                               BoxTable bt)
+{
+    const int k = (int)(xcd_block() * blockDim.x + threadIdx.x);
+    const int r = k < a.nrows ? (a.sched ? a.sched[k] : k) : a.nrows;
+    // pairs of this launch when asked (wave sum, one atomic per wave; a launch of k_sum_counts
+    // after the pass costs fewer atomics on the one address)
+    if (a.pairs) {
+        unsigned long long c = (r < a.nrows) ? (unsigned long long)a.cnt[r] : 0ull;
+        for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
+        if ((threadIdx.x & 63) == 0 && c) atomicAdd(a.pairs, c);
+    }
+    if (r >= a.nrows) return;
+    if (!force_row<KERN, R, BOXES, STATS, COMMON>(P, G, F, rec, a, bt, r)) {
+        const int s = atomicAdd(a.err + 12, 1);
+        a.rare[s] = r;
+    }
+}
+
+template <int KERN, typename R, bool BOXES>
+__global__ void __launch_bounds__(256) k_force_rare(DevParams P, Gam<R> G, ForceView F, const Cell<R> *__restrict__ rec, ForceRows a,
+                                                    BoxTable bt)
+{
+    const int count = a.err[12];
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < count; k += gridDim.x * blockDim.x)
+        force_row<KERN, R, BOXES, false, false>(P, G, F, rec, a, bt, a.rare[k]);
 }
 
 // ---------------------------------------------------------------------------------------

@@ -710,39 +710,66 @@ __global__ void __launch_bounds__(256) k_active_from_bins(long n, unsigned long 
 // (the lazy drift's hook in the neighbour search, src/mesh/gpu_neighbor_list.cc:2443-2510: the
 // members the walk found behind the search time are drifted to it).  The walks of this library
 // visit the 27 cells around each row's home cell of the grid g, so those cells are the set here
-// (a superset of the pairs accepted, as the reference's walk's is).  One workgroup of 64 lanes
-// per row; a particle behind time1 is claimed by one lane (compare-and-swap of its
-// P.Ti_current) and drifted from where it stands.  hmax receives the largest radius drifted.
-__global__ void __launch_bounds__(64) k_drift_stencil(TimeParams T, ParticleView S, GridView g, int nrows, const int *__restrict__ act,
-                                                      const uint32_t *__restrict__ key, long long time1,
-                                                      unsigned long long *__restrict__ hmax)
+// (a superset of the pairs accepted, as the reference's walk's is).  Rows in one region share
+// most of their cells, so the cells are listed once each first (k_stencil_cells: a thread per
+// row and stencil cell; the first to mark a cell appends it, one atomic per wavefront for the
+// list), and one workgroup per listed cell drifts its particles that stand behind time1
+// (k_drift_cells; cells are disjoint index ranges, so no particle has two writers) and clears
+// the cell's mark.  hmax receives the largest radius drifted.
+__global__ void __launch_bounds__(256) k_stencil_cells(GridView g, int nrows, const int *__restrict__ act,
+                                                       const uint32_t *__restrict__ key, int *__restrict__ cmark,
+                                                       int *__restrict__ cells, int *__restrict__ ncells)
 {
-    const long r = xcd_block();
-    if (r >= nrows) return;
-    const int lane = threadIdx.x;
-    const int i = act[r];
-    const uint32_t home = key[i] >> g.shift;
-    const int nc = 1 << g.level;
-    const int cx = compact10(home >> 2), cy = compact10(home >> 1), cz = compact10(home);
-    double h1 = 0;
-    for (int q = 0; q < 27; q++) {
+    const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    bool fresh = false;
+    int c = 0;
+    if (t < 27L * nrows) {
+        const int r = (int)(t / 27), q = (int)(t % 27);
+        const uint32_t home = key[act[r]] >> g.shift;
+        const int nc = 1 << g.level;
+        const int cx = compact10(home >> 2), cy = compact10(home >> 1), cz = compact10(home);
         const int dx = q / 9 - 1, dy = (q / 3) % 3 - 1, dz = q % 3 - 1;
         const int a = (cx + dx + nc) % nc, b = (cy + dy + nc) % nc, d = (cz + dz + nc) % nc;
-        const int c = (int)((spread10(a) << 2) | (spread10(b) << 1) | spread10(d));
+        c = (int)((spread10(a) << 2) | (spread10(b) << 1) | spread10(d));
+        fresh = atomicExch(&cmark[c], 1) == 0;
+    }
+    const unsigned long long won = __ballot(fresh);
+    if (!won) return;
+    const int lane = threadIdx.x & 63;
+    const int leader = __ffsll((long long)won) - 1;
+    int base = 0;
+    if (lane == leader) base = atomicAdd(ncells, __popcll(won));
+    base = __shfl(base, leader, 64);
+    if (fresh) cells[base + __popcll(won & ((1ull << lane) - 1))] = c;
+}
+
+__global__ void __launch_bounds__(256) k_drift_cells(TimeParams T, ParticleView S, GridView g, const int *__restrict__ cells,
+                                                     const int *__restrict__ ncells, int *__restrict__ cmark, long long time1,
+                                                     unsigned long long *__restrict__ hmax)
+{
+    __shared__ double wmax[4];
+    const int count = *ncells;
+    double h1 = 0;
+    for (int k = blockIdx.x; k < count; k += gridDim.x) {
+        const int c = cells[k];
         const int st = g.cstart[c], en = g.cend[c];
-        for (int j = st + lane; j < en; j += 64) {
-            const long long t0 = __hip_atomic_load(&S.TIC[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        for (int j = st + threadIdx.x; j < en; j += blockDim.x) {
+            const long long t0 = S.TIC[j];
             if (t0 == time1) continue;
-            if (atomicCAS((unsigned long long *)&S.TIC[j], (unsigned long long)t0, (unsigned long long)time1) !=
-                (unsigned long long)t0)
-                continue;
             double hh;
             drift_one(T, S, j, t0, time1, nullptr, nullptr, nullptr, &hh);
+            S.TIC[j] = time1;
             h1 = fmax(h1, hh);
         }
+        if (threadIdx.x == 0) cmark[c] = 0;
     }
     for (int o = 32; o > 0; o >>= 1) h1 = fmax(h1, __shfl_xor(h1, o, 64));
-    if (lane == 0 && h1 > 0) atomic_max_bits(hmax, h1);
+    if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = h1;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        h1 = fmax(fmax(wmax[0], wmax[1]), fmax(wmax[2], wmax[3]));
+        if (h1 > 0) atomic_max_bits(hmax, h1);
+    }
 }
 
 // every particle behind time1 drifted to it (gizmo_full_drift_to, src/core/predict.cc:148-171:

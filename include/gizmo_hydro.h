@@ -124,7 +124,9 @@ int gz_synchronize(gz_context *ctx);
    gz_stat: named counters of the last calls ("density_iterations", "pairs_density",
    "pairs_symmetric", "ngb_rebuilds", "n_active", "ell_widen"), and totals since the last
    gz_timing_reset ("active_updates": active particles through the hot path, "resorts",
-   "force_pairs_total": neighbour pairs of every force launch).  With the option
+   "force_pairs_total": neighbour pairs of every force launch, "force_rare_rows": rows of
+   lane-per-row force passes that a face past the Riemann ladder's common case sent to the
+   pass's second kernel).  With the option
    "riemann_stats" set to 1, the force passes of a single box also count, per face, the rungs of
    the Riemann ladder it took (each at most once per face) since the option was last set:
    "riemann_faces" (faces that reached the Riemann solve), "riemann_hllc", "riemann_vacuum",

@@ -113,6 +113,7 @@ long long orc_get_timestep(const orc_params *par, const orc_integ *ip, const orc
 int orc_run_begin(const orc_params *par, const orc_integ *ip, orc_timeline *tl, orc_state *s);
 int orc_run_step(const orc_params *par, const orc_integ *ip, orc_timeline *tl, orc_state *s);
 void orc_drift_all(const orc_params *par, const orc_integ *ip, const orc_timeline *tl, orc_state *s);
+void orc_process_wake_ups(const orc_params *par, const orc_integ *ip, orc_timeline *tl, orc_state *s);
 void orc_kick(const orc_params *par, const orc_integ *ip, const orc_timeline *tl, orc_state *s, long i,
               long long tstart, long long tend, int mode);
 void orc_drift(const orc_params *par, const orc_integ *ip, const orc_timeline *tl, orc_state *s, long i,

@@ -311,6 +311,12 @@ static void ot_find_timesteps(const orc_params *par, const orc_integ *ip, orc_ti
     ot_process_wake_ups(par, ip, tl, s);
 }
 
+/* process_wake_ups on its own (tests: the bin choice and the step truncation of given states) */
+void orc_process_wake_ups(const orc_params *par, const orc_integ *ip, orc_timeline *tl, orc_state *s)
+{
+    ot_process_wake_ups(par, ip, tl, s);
+}
+
 /* find_next_sync_point_and_drift, src/core/run.cc:857-994: the earliest next kick over the
    occupied bins, the bins active there, and every particle drifted to it. */
 static void ot_next_sync_point_and_drift(const orc_params *par, const orc_integ *ip, orc_timeline *tl, orc_state *s)

@@ -353,6 +353,11 @@ class Run:
         cs = self.st._c()
         lib().orc_drift_all(C.byref(self.par), C.byref(self.ip), C.byref(self.tl), C.byref(cs))
 
+    def process_wake_ups(self):
+        """process_wake_ups alone on the current timeline and state."""
+        cs = self.st._c()
+        lib().orc_process_wake_ups(C.byref(self.par), C.byref(self.ip), C.byref(self.tl), C.byref(cs))
+
     def get_timestep(self, i):
         cs = self.st._c()
         return int(lib().orc_get_timestep(C.byref(self.par), C.byref(self.ip), C.byref(self.tl), C.byref(cs), int(i)))

@@ -66,9 +66,13 @@ def test_probe_fp64_bitwise(oracle_lib, mode):
 @pytest.mark.parametrize("mode", [0, 1, 2, 4])
 def test_probe_fp32(oracle_lib, mode):
     """FP32 arithmetic from the same face states: the same rung wherever the FP64 estimate is
-    not within FP32 rounding of the limiter, and P_M, S_M within 1e-4 of the state's pressure
-    and signal-speed scales (the FP32 form's pair arithmetic; its whole-pass error is measured
-    against the oracle in the parity tests)."""
+    not within FP32 rounding of the limiter; S_M within 1e-6 of the signal-speed scale at the
+    99th percentile (1e-5 at most); P_M within 3e-4 of the pressure scale at the 99th percentile
+    (1e-2 at most: the star-pressure estimates difference the two sides' momentum fluxes, which
+    cancel to a few bits in FP32 when the states' pressures and densities differ by up to 1e4,
+    as these cases' do).  Measured on the MI355X: S_M p99 1.5e-7, P_M p99 5.7e-5 to 1.3e-4, max
+    3.5e-3.  The FP32 form's whole-pass error is measured against the oracle in the parity
+    tests."""
     O = oracle_lib
     cases = RC.random_cases(4000, seed=17)
     cases = cases[(cases[:, 1] / cases[:, 6] < 1e4) & (cases[:, 6] / cases[:, 1] < 1e4)
@@ -87,8 +91,8 @@ def test_probe_fp32(oracle_lib, mode):
     print(f"probe fp32 mode={mode}: same rung {frac:.4f}, P_M err max {ep.max():.2e} p99 {np.quantile(ep, 0.99):.2e}, "
           f"S_M err max {es.max():.2e} p99 {np.quantile(es, 0.99):.2e}", flush=True)
     assert frac >= 0.98
-    assert np.quantile(ep, 0.99) <= 1e-5 and np.quantile(es, 0.99) <= 1e-5
-    assert ep.max() <= 1e-3 and es.max() <= 1e-3
+    assert np.quantile(ep, 0.99) <= 3e-4 and np.quantile(es, 0.99) <= 1e-6
+    assert ep.max() <= 1e-2 and es.max() <= 1e-5
 
 
 def _colliding_streams(n_side, mach, seed):
@@ -151,3 +155,48 @@ def test_pass_rung_counts_match_oracle(oracle_lib, state, few_rows):
         assert got["riemann_vacuum"] > 0
     # the counting kernels compute the same fluxes
     np.testing.assert_allclose(out["accel"], so.accel, rtol=0, atol=1e-12 * np.abs(so.accel).max())
+
+
+@pytest.mark.parametrize("state", ["sedov_evolved", "colliding_streams"])
+@pytest.mark.parametrize("form", ["fp64", "fp32_mixed"])
+def test_common_and_rare_kernels_same_bits(oracle_lib, state, form):
+    """A lane-per-row force pass runs its rows in two kernels: the rows whose faces all take the
+    Riemann ladder's common case, then the rows with a face past it (KT, the exact solver, a
+    retry) with the whole ladder.  The outputs equal the single kernel's (the rung-counting
+    kernel, which runs every row with the whole ladder) bit for bit; the colliding streams send
+    rows through the second kernel (faces of their shocks take the exact solver)."""
+    O = oracle_lib
+    if state == "sedov_evolved":
+        par, s = _evolved_sedov(O, 16, 30)
+        s.timebin[:] = 0
+        s.wakeup[:] = 0
+    else:
+        ic = _colliding_streams(16, 5.0, 404)
+        par = dict(ic["params"])
+        s = O.State.from_ic(ic)
+        O.density(par, s)
+        O.gradients(par, s)
+    from oracle.oracle import FIELDS, INT_FIELDS
+    f = {k: getattr(s, k) for k in list(FIELDS) + list(INT_FIELDS)}
+    for k in ("accel", "dtu", "maxsig"):
+        f[k] = None
+    prec = GZ_FP64 if form == "fp64" else GZ_FP32_MIXED
+    outs = []
+    rare = []
+    for stats in (1, 0):
+        print(f"common/rare: {state} {form} n={s.n} riemann_stats={stats}", flush=True)
+        g = Hydro(par, s.n, prec)
+        g.set_option("few_rows", 0)
+        g.load(f)
+        g.set_option("riemann_stats", stats)
+        g.timing_reset()
+        g.hydro_force()
+        outs.append(g.store(["accel", "dtu", "maxsig", "wakeup"]))
+        rare.append(g.stat("force_rare_rows"))
+        g.close()
+    print(f"rows through the second kernel: {rare[1]:.0f} of {s.n}", flush=True)
+    assert rare[0] == 0
+    if state == "colliding_streams":
+        assert rare[1] > 0
+    for k in outs[0]:
+        assert np.array_equal(outs[0][k], outs[1][k]), k

@@ -111,8 +111,10 @@ def test_adaptive_bins_sedov(oracle_lib, resort_fraction, few_rows, frac):
     sync point) and the reference's lazy drift (0.5: the blast's steps drift their active bins
     and the particles their passes read), through the one-wavefront-per-row kernels (few active
     rows, the default) and the one-lane-per-row kernels (few_rows 0), with and without the
-    particles being ordered again."""
-    ic = ics.sedov_box(20, 1002, kernel=6, des_num_ngb=64.0)
+    particles being ordered again.  32^3 particles: the search grid's cells then span an eighth of
+    the box, so the cells around the blast's rows hold 12-36% of the particles (at 20^3 their
+    27-cell stencils cover the whole box and nothing stays behind)."""
+    ic = ics.sedov_box(32, 1002, kernel=6, des_num_ngb=64.0)
     integ = dict(courant_fac=0.05, err_tol_int_accuracy=0.002, max_size_timestep=0.01, min_size_timestep=0.0,
                  time_begin=0.0, time_max=1.0, min_egy_spec=0.0, domain_build_active_fraction=frac)
     st, out, hist, par = run_both(oracle_lib, ic, integ, 40, resort_fraction, few_rows)

@@ -231,3 +231,50 @@ def test_retries_are_direct_solves(oracle_lib):
     d[:, 13] = 2.0 * c[r2, 13]
     direct, _ = O.riemann_probe(1, GAMMA, d)
     assert np.array_equal(np.nan_to_num(out[r2, :6], nan=7.0), np.nan_to_num(direct[:, :6], nan=7.0))
+
+
+def _hot_cold_state(O, boost=(0.0, 0.0, 0.0)):
+    """A 12^3 jittered lattice with the specific energy varying by up to e^(+-0.7) between
+    particles (P/rho contrasts on most faces: every branch of the entropic switch), slow random
+    velocities (|S_M| well under the sound speed on most faces: the switch is considered) and an
+    optional uniform boost."""
+    from gizmo_amd import ics
+    ic = ics.lattice_box(12, 31, kernel=3, des_num_ngb=32.0, mach_sigma=0.2)
+    rng = np.random.default_rng(7)
+    ic["u"] = ic["u"] * np.exp(rng.uniform(-0.7, 0.7, len(ic["u"])))
+    ic["vel"] = ic["vel"] + np.asarray(boost)[None, :]
+    par = dict(ic["params"])
+    st = O.State.from_ic(ic)
+    O.density(par, st)
+    O.gradients(par, st)
+    O.rung_counts(reset=True)
+    O.force(par, st)
+    return st, O.rung_counts(reset=True)
+
+
+def test_entropic_switch_pins(oracle_lib):
+    """The face entropic switch (src/hydro/hydro_core_meshless_functions.h:515-573) by what the
+    physics fixes:
+      * both sides of a face make the same decision (the energy the PdV form gives one side is
+        what it takes from the other only then), so a pass over states where the decisions mix
+        conserves total energy to roundoff;
+      * the decision and the energy it moves depend only on velocities relative to the face:
+        a uniform boost of every particle leaves every du/dt and dv/dt and the count of faces
+        that take the PdV form unchanged (to rounding at the branch thresholds)."""
+    O = oracle_lib
+    st, r = _hot_cold_state(O)
+    print({k: v for k, v in r.items() if v})
+    assert r["entropic_considered"] > 0.5 * r["riemann_faces"]
+    frac = r["entropic_applied"] / r["entropic_considered"]
+    assert 0.02 < frac < 0.98, f"the decisions do not mix ({frac:.3f} of the faces take the PdV form)"
+    e_dot = (st.mass * (st.dtu + (st.vel * st.accel).sum(1))).sum()
+    e_scale = (st.mass * (np.abs(st.dtu) + np.abs((st.vel * st.accel).sum(1)))).sum()
+    assert abs(e_dot) < 1e-13 * e_scale
+    cs = np.sqrt(GAMMA * st.press / st.rho)
+    boost = 0.8 * cs.mean() * np.array([0.6, -0.48, 0.64])
+    sb, rb = _hot_cold_state(O, boost)
+    dv_scale = np.abs(st.accel).max()
+    du_scale = np.abs(st.dtu).max()
+    assert np.abs(sb.accel - st.accel).max() < 1e-9 * dv_scale
+    assert np.abs(sb.dtu - st.dtu).max() < 1e-9 * du_scale
+    assert abs(rb["entropic_applied"] - r["entropic_applied"]) <= 1e-3 * r["entropic_applied"]

@@ -269,3 +269,70 @@ def test_lazy_full_drift_at_the_domain_cadence(oracle_lib):
     for _ in range(10):
         r.step()
         assert r.tl.full_drift == 1 and (st.ti_current == r.ti_current).all()
+
+
+def test_wake_up_bins_steps_and_kicks(oracle_lib):
+    """process_wake_ups (src/core/timestep.cc:1565-1774) on a hand-made state at Ti_Current =
+    3 * 2^10 (bins 0-10 active; the next kick is bin 8's at 3328 = 13 * 2^8, so bin 8 is the
+    highest bin active there).  The bin rule, step truncation and kicks are checked case by case,
+    and the woken particle's velocity and energy against the closed form of a constant
+    acceleration and energy rate: kicked from its old step's start to the midpoint of its new
+    step, whether the wake-up falls before the old midpoint or not."""
+    O = oracle_lib
+    T = 3 << 10
+    n = 7
+    s = O.State(n)
+    s.mass[:] = 1.0
+    s.rho[:] = 1.0
+    s.h[:] = 0.1
+    s.u[:] = s.u_kick[:] = 1.0
+    s.accel[:] = (0.5, -0.25, 1.0)
+    s.dtu[:] = 0.01
+    v_start = np.array([0.1, 0.2, -0.3])
+    bins = [8, 14, 12, 14, 14, 13, 10]
+    begstep = [3072 - 256, 0, 0, 0, 0, 0, 2048]
+    dt = [256, 1 << 14, 1 << 12, 1 << 14, 1 << 14, 1 << 13, 1024]
+    tic = [T, T, T, T, T, 2048, T]
+    # waker bin + 1 (hydro wake-up), -1 (generic), 0 (none)
+    wake = [0, 9, 16, -1, 0, 9, 9]
+    for i in range(n):
+        s.timebin[i] = bins[i]
+        s.ti_begstep[i] = begstep[i]
+        s.dt_step[i] = dt[i]
+        s.ti_current[i] = tic[i]
+        s.wakeup[i] = wake[i]
+        # the first half-kick of the old step applied: kicked to its midpoint
+        mid = begstep[i] + dt[i] // 2
+        s.vel_kick[i] = v_start + s.accel[i] * mid * TBI
+        s.u_kick[i] = 1.0 + s.dtu[i] * mid * TBI
+        s.vel[i] = s.vel_kick[i]
+    before = s.copy()
+    r = runner(O, s)
+    r.tl.ti_current = T
+    for b in range(60):
+        r.tl.bin_active[b] = 1 if (b == 0 or T % (1 << b) == 0) else 0
+        r.tl.bin_count[b] = bins.count(b)
+    r.tl.need_wakeup = 1
+    r.process_wake_ups()
+    # 1: waker bin 8 - 3 = 5, raised to the lowest occupied active bin (8); 2: waker bin 15 - 3 =
+    # 12, lowered to the highest bin active at the next kick (8); 3: generic, that same bin; 5:
+    # as 1
+    assert list(s.timebin) == [8, 8, 8, 8, 14, 8, 10]
+    assert r.tl.n_woken == 4
+    assert [r.tl.bin_count[b] for b in (8, 10, 12, 13, 14)] == [5, 1, 0, 0, 1]
+    # drifted to now and on a full bin step: the new step starts now, with the old step completed
+    # (1, 3: the wake-up falls before the old midpoint; 2: after it) and the new one's first
+    # half-kick; the predicted velocity and energy are those of now
+    for i in (1, 2, 3):
+        assert (s.ti_begstep[i], s.dt_step[i], s.ti_current[i]) == (T, 256, T)
+        np.testing.assert_allclose(s.vel_kick[i], v_start + s.accel[i] * (T + 128) * TBI, rtol=0, atol=1e-15)
+        assert abs(s.u_kick[i] - (1.0 + s.dtu[i] * (T + 128) * TBI)) < 1e-15
+        np.testing.assert_allclose(s.vel[i], v_start + s.accel[i] * T * TBI, rtol=0, atol=1e-15)
+        assert abs(s.u[i] - (1.0 + s.dtu[i] * T * TBI)) < 1e-15
+    # not drifted to now: the step ends now (truncated from where the particle stands), no kick
+    assert (s.ti_begstep[5], s.dt_step[5], s.ti_current[5]) == (2048, T - 2048, T)
+    np.testing.assert_array_equal(s.vel_kick[5], before.vel_kick[5])
+    # untouched: no flag (4), active (6), the waker (0)
+    for i in (0, 4, 6):
+        assert (s.timebin[i], s.ti_begstep[i], s.dt_step[i], s.ti_current[i]) == (bins[i], begstep[i], dt[i], tic[i])
+        np.testing.assert_array_equal(s.vel_kick[i], before.vel_kick[i])
