@@ -15,6 +15,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <new>
 #include <string>
 #include <vector>
 
@@ -173,6 +174,11 @@ struct gz_context {
                full_drifts = 0, force_rare_rows = 0;
     } st;
     bool timing = false;
+    // contexts of gz_hydro_pass_noalloc: every device buffer is carved from `pool` (the static
+    // scratch array of the device) and nothing is freed; host counters live in host_u64
+    unsigned char *pool = nullptr;
+    size_t pool_size = 0, pool_used = 0;
+    unsigned long long host_u64[80 + 8] = {};
     struct TEv { std::string name; hipEvent_t a, b; };
     std::vector<TEv> pending;
     std::map<std::string, std::pair<long, double>> timings;
@@ -205,11 +211,27 @@ int fail(gz_context *c, int code, const char *fmt, ...)
         if (rc_) return rc_;               \
     } while (0)
 
+// count elements from the context's pool (no allocation, nothing freed: a buffer that grows
+// takes new space), 256-byte aligned
+int pool_take(gz_context *ctx, void **p, size_t bytes)
+{
+    const size_t at = (ctx->pool_used + 255) & ~(size_t)255;
+    if (at > ctx->pool_size || bytes > ctx->pool_size - at) {
+        *p = nullptr;
+        return fail(ctx, GZ_E_MEMORY, "the static scratch of %zu bytes is exhausted (%zu in use, %zu more asked for)", ctx->pool_size,
+                    at, bytes);
+    }
+    *p = ctx->pool + at;
+    ctx->pool_used = at + bytes;
+    return 0;
+}
+
 template <typename T>
 int dalloc(gz_context *ctx, T **p, size_t count)
 {
-    if (*p) { hipFree(*p); *p = nullptr; }
     if (count == 0) count = 1;
+    if (ctx->pool) return pool_take(ctx, (void **)p, count * sizeof(T));
+    if (*p) { hipFree(*p); *p = nullptr; }
     hipError_t e = hipMalloc((void **)p, count * sizeof(T));
     if (e != hipSuccess) { *p = nullptr; return fail(ctx, GZ_E_MEMORY, "hipMalloc of %zu bytes failed: %s", count * sizeof(T), hipGetErrorString(e)); }
     return 0;
@@ -306,9 +328,14 @@ Gam<R> gam(const DevParams &d)
 int ensure_tmp(gz_context *ctx, size_t bytes)
 {
     if (bytes <= ctx->tmp_bytes) return 0;
+    size_t b = std::max(bytes, (size_t)1 << 20);
+    if (ctx->pool) {
+        CHK(pool_take(ctx, &ctx->tmp, b));
+        ctx->tmp_bytes = b;
+        return 0;
+    }
     if (ctx->tmp) hipFree(ctx->tmp);
     ctx->tmp = nullptr;
-    size_t b = std::max(bytes, (size_t)1 << 20);
     HIPCHK(hipMalloc(&ctx->tmp, b));
     ctx->tmp_bytes = b;
     return 0;
@@ -351,7 +378,9 @@ int alloc_state(gz_context *ctx, long cap)
     ctx->counters = (int *)(ctx->u64c + 80);
     CHK(dalloc(ctx, &ctx->rungs, 16));
     HIPCHK(hipMemsetAsync(ctx->rungs, 0, 16 * sizeof(unsigned long long), ctx->stream));
-    if (!ctx->hu64c) {
+    if (ctx->pool) {
+        ctx->hu64c = ctx->host_u64;
+    } else if (!ctx->hu64c) {
         hipError_t e = hipHostMalloc((void **)&ctx->hu64c, (80 + 8) * sizeof(unsigned long long), hipHostMallocDefault);
         if (e != hipSuccess) return fail(ctx, GZ_E_MEMORY, "hipHostMalloc failed");
     }
@@ -1434,7 +1463,18 @@ int set_device(gz_context *ctx)
     return 0;
 }
 
+// ---------------------------------------------------------------------------------------
+// gz_hydro_pass_noalloc: the working memory of a pass carved from one static device array
+// per device (device code sees `scratch`; host code uses only the address that
+// hipGetSymbolAddress returned for the current device, never the array's name)
+// ---------------------------------------------------------------------------------------
+constexpr int kNoAllocDevices = 64;
+unsigned char *g_scratch[kNoAllocDevices] = {};  // the array's address on each device, once set up
+gz_context g_noalloc_ctx[kNoAllocDevices];       // the host side of each device's pass
+
 }  // namespace
+
+static __device__ __align__(256) unsigned char scratch[GZ_NOALLOC_SCRATCH_BYTES];
 
 // ---------------------------------------------------------------------------------------
 // C ABI
@@ -1994,6 +2034,59 @@ long gz_neighbors(gz_context *ctx, int symmetric, long *offsets, int *nbr, long 
         }
     }
     return tot;
+}
+
+int gz_hydro_pass_noalloc(const gz_params *params, long n, const gz_fields *in, gz_fields *out, void *stream, char *err,
+                          size_t err_len)
+{
+    if (err && err_len) err[0] = 0;
+    auto report = [&](int rc, const char *msg) {
+        if (err && err_len) snprintf(err, err_len, "%s", msg);
+        return rc;
+    };
+    if (!params || !in || !out) return report(GZ_E_ARG, "gz_hydro_pass_noalloc: params, in and out are required");
+    if (n <= 0 || n > GZ_NOALLOC_MAX_N) {
+        char m[160];
+        snprintf(m, sizeof m, "gz_hydro_pass_noalloc: n=%ld outside (0, %ld] (the static scratch holds at most that many)", n,
+                 (long)GZ_NOALLOC_MAX_N);
+        return report(GZ_E_ARG, m);
+    }
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kNoAllocDevices)
+        return report(GZ_E_HIP, "gz_hydro_pass_noalloc: no current device (or its index is 64 or more)");
+    if (!g_scratch[dev]) {
+        // set-up, once per device, with that device current: the array's address on it
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            char m[200];
+            snprintf(m, sizeof m, "gz_hydro_pass_noalloc: hipGetSymbolAddress of the scratch array failed: %s", hipGetErrorString(e));
+            return report(GZ_E_HIP, m);
+        }
+        g_scratch[dev] = (unsigned char *)p;
+    }
+    // a fresh context over the array (its members allocate nothing until used, and a pass uses
+    // none of the ones that would: timing stays off)
+    gz_context *ctx = &g_noalloc_ctx[dev];
+    ctx->~gz_context();
+    new (ctx) gz_context();
+    ctx->device = dev;
+    ctx->stream = (hipStream_t)stream;
+    ctx->pool = g_scratch[dev];
+    ctx->pool_size = GZ_NOALLOC_SCRATCH_BYTES;
+    int rc = set_params(ctx, params);
+    if (!rc) rc = alloc_state(ctx, n);
+    if (!rc) rc = gz_load(ctx, n, in);
+    if (!rc) rc = gz_density(ctx);
+    if (!rc) rc = gz_hydro_gradient_calc(ctx);
+    if (!rc) rc = gz_hydro_force(ctx);
+    if (!rc) rc = gz_store(ctx, out);
+    if (!rc) {
+        const hipError_t e = hipStreamSynchronize(ctx->stream);
+        if (e != hipSuccess) rc = fail(ctx, GZ_E_HIP, "hipStreamSynchronize failed: %s", hipGetErrorString(e));
+    }
+    if (rc) report(rc, ctx->err);
+    return rc;
 }
 
 }  // extern "C"

@@ -106,6 +106,8 @@ def lib():
         L.gz_set_option.argtypes = [C.c_void_p, C.c_char_p, C.c_double]
         L.gz_riemann_probe.argtypes = [C.c_void_p, C.c_int, C.c_long, _dp, _dp, C.POINTER(C.c_uint)]
         L.gz_drift_all.argtypes = [C.c_void_p]
+        L.gz_hydro_pass_noalloc.argtypes = [C.POINTER(Params), C.c_long, C.POINTER(Fields), C.POINTER(Fields), C.c_void_p,
+                                            C.c_char_p, C.c_size_t]
         _lib = L
     return _lib
 
@@ -319,3 +321,33 @@ class Hydro:
         if tot2 != tot:
             raise GizmoHipError(-1, "neighbour count changed between calls")
         return off, nb[:tot]
+
+
+NOALLOC_MAX_N = 16384  # GZ_NOALLOC_MAX_N
+
+
+def _device_fields(ptrs):
+    """A gz_fields of device addresses (dict name -> int address, e.g. a tensor's data_ptr())."""
+    f = Fields()
+    kinds = dict({k: _dp for k in list(FIELD_WIDTH) + list(LOOP_FIELD_WIDTH)}, **{k: _ip for k in INT_FIELDS},
+                 **{k: _llp for k in I64_FIELDS})
+    for k, a in ptrs.items():
+        if k not in kinds:
+            raise KeyError(k)
+        if a:
+            setattr(f, k, C.cast(C.c_void_p(int(a)), kinds[k]))
+    return f
+
+
+def hydro_pass_noalloc(params, n, dev_in, dev_out, precision=GZ_FP64, stream=0):
+    """One density -> gradients -> force pass over device buffers through gz_hydro_pass_noalloc
+    (no device allocation: the library's static scratch).  dev_in / dev_out: dict field name ->
+    device address of n values of the field's width in caller order (gz_load's and gz_store's
+    layouts); stream: a hipStream_t as an int (0: the null stream)."""
+    p = make_params(params, precision)
+    fi = _device_fields(dev_in)
+    fo = _device_fields(dev_out)
+    err = C.create_string_buffer(512)
+    rc = lib().gz_hydro_pass_noalloc(C.byref(p), int(n), C.byref(fi), C.byref(fo), C.c_void_p(int(stream)), err, len(err))
+    if rc:
+        raise GizmoHipError(rc, err.value.decode())

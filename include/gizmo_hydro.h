@@ -22,6 +22,8 @@
 #ifndef GIZMO_HYDRO_H
 #define GIZMO_HYDRO_H
 
+#include <stddef.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -245,6 +247,24 @@ int gz_riemann_probe(gz_context *ctx, int mode, long n, const double *in, double
    symmetric = 0: rows r2 < h_i^2 (self included); 1: r2 < max(h_i,h_j)^2 (self excluded).
    Row order is ascending device order.  Call with nbr = NULL to get the total count. */
 long gz_neighbors(gz_context *ctx, int symmetric, long *offsets, int *nbr, long cap);
+
+/* One density() -> hydro_gradient_calc() -> hydro_force() pass over device buffers that
+   allocates nothing.  No counterpart in the reference: it is for callers that cannot allocate
+   device memory (code under HIP graph capture, tight serving loops, harnesses that forbid
+   hipMalloc).  in: device pointers in the caller's order with gz_load's layouts (pos, vel, mass,
+   u, h required; the optional inputs as gz_load takes them); out: device pointers with
+   gz_store's layouts (NULL fields are skipped).  The results are those of gz_create + gz_load +
+   the three stages + gz_store with the same parameters, bit for bit.  The pass runs on the
+   device current for the calling thread, on `stream` (a hipStream_t; NULL: the null stream),
+   and returns after the stream has finished it.  Every working buffer is carved from one static
+   device array of GZ_NOALLOC_SCRATCH_BYTES, which each device the library runs on holds until
+   the process ends; n is at most GZ_NOALLOC_MAX_N (a larger n returns GZ_E_ARG, a pass whose
+   lists outgrow the array GZ_E_MEMORY).  Calls that overlap on one device race: they share its
+   array.  err (may be NULL): receives the message of a failed call. */
+#define GZ_NOALLOC_MAX_N 16384L
+#define GZ_NOALLOC_SCRATCH_BYTES (48UL << 20)
+int gz_hydro_pass_noalloc(const gz_params *params, long n, const gz_fields *in, gz_fields *out, void *stream, char *err,
+                          size_t err_len);
 
 #ifdef __cplusplus
 }
