@@ -1022,7 +1022,7 @@ int gradients_stage(gz_context *ctx)
                                                                      rec, ctx->X, ctx->Y, ctx->Z, ctx->par.box_size, BoxTable{});
     }
     if (ctx->nact <= ctx->few_rows)
-        k_gradients_rows<KERN, R><<<xcd_grid(ctx->nact), kRowThreads, 0, ctx->stream>>>(
+        k_gradients_rows<KERN, R><<<xcd_grid(ctx->nact), kListThreads, 0, ctx->stream>>>(
             ctx->dp, ctx->nact, ctx->act, ctx->X, ctx->Y, ctx->Z, rec,
             make ? GradSrc{ctx->H, ctx->M, ctx->RHO, ctx->PR, ctx->VX, ctx->VY, ctx->VZ, ctx->par.box_size} : GradSrc{}, ctx->T, ctx->CN,
             ctx->n, ctx->ell_s, ctx->ld_s, ctx->cnt_s, ctx->GR, ctx->GP, ctx->GV, BoxTable{});
@@ -1056,7 +1056,7 @@ int force_stage(gz_context *ctx)
         if (ctx->nact <= ctx->few_rows) {
             // the kernel that counts the faces' Riemann rungs when asked (the same code otherwise)
             auto rows_kernel = ctx->rung_stats ? k_force_rows<KERN, R, true> : k_force_rows<KERN, R, false>;
-            rows_kernel<<<xcd_grid(ctx->nact), kRowThreads, 0, ctx->stream>>>(
+            rows_kernel<<<xcd_grid(ctx->nact), kListThreads, 0, ctx->stream>>>(
                 ctx->dp, gam<R>(ctx->dp), ctx->nact, ctx->act, F, rec, ctx->ell_s, ctx->ld_s, ctx->cnt_s,
                 ctx->all_active ? nullptr : ctx->ACT, ctx->TB, ctx->MSV, ctx->WAKE, ctx->ACC, ctx->DTU, ctx->MSV, ctx->counters + 2,
                 ctx->counters + 4, ctx->nact <= kRowPairAtomics ? ctx->u64c + 65 : nullptr, BoxTable{});

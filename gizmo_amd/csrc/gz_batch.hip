@@ -611,7 +611,7 @@ int gradients_force(gz_batch *b)
         k_gradients<KERN, R, true><<<xcd_grid(nb(na)), kB, 0, b->stream>>>(b->dp, na, b->act, b->X, b->Y, b->Z, rg, b->T, b->CN, b->cap, b->ell, ld,
                                                                       b->cnt, b->GR, b->GP, b->GV, bt);
     else
-        k_gradients_rows<KERN, R><<<xcd_grid(na), kRowThreads, 0, b->stream>>>(b->dp, na, b->act, b->X, b->Y, b->Z, rg, GradSrc{}, b->T, b->CN, b->cap,
+        k_gradients_rows<KERN, R><<<xcd_grid(na), kListThreads, 0, b->stream>>>(b->dp, na, b->act, b->X, b->Y, b->Z, rg, GradSrc{}, b->T, b->CN, b->cap,
                                                                        b->ell, ld, b->cnt, b->GR, b->GP, b->GV, bt);
     BHIP(hipGetLastError());
     ForceView F{b->cap, b->X, b->Y, b->Z, b->VX, b->VY, b->VZ, b->M, b->H, b->U, b->RHO, b->PR, b->CN, b->FCE, b->DRK, b->GR, b->GP,
@@ -633,7 +633,7 @@ int gradients_force(gz_batch *b)
         k_force<KERN, R, true, false, true><<<xcd_grid(nb(na)), kB, 0, b->stream>>>(b->dp, G, F, rf, a, bt);
         k_force_rare<KERN, R, true><<<std::min(nb(na), kRareBlocks), kB, 0, b->stream>>>(b->dp, G, F, rf, a, bt);
     } else {
-        k_force_rows<KERN, R><<<xcd_grid(na), kRowThreads, 0, b->stream>>>(b->dp, G, na, b->act, F, rf, b->ell, ld, b->cnt, b->ACT, b->TB, b->MSV,
+        k_force_rows<KERN, R><<<xcd_grid(na), kListThreads, 0, b->stream>>>(b->dp, G, na, b->act, F, rf, b->ell, ld, b->cnt, b->ACT, b->TB, b->MSV,
                                                                    b->WAKE, b->ACC, b->DTU, b->MSV, b->counters + 2, b->counters + 4,
                                                                    nullptr, bt);
     }

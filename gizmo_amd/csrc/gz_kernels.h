@@ -326,12 +326,17 @@ struct GradSrc {
     const double *H, *M, *RHO, *PR, *VX, *VY, *VZ;
     double box;
 };
-template <int KERN, typename R>
+// The gradient and force passes over few rows walk lists of ~1-2x the neighbour number:
+// kListThreads lanes per row take a list of up to that many entries in one round, and twice as
+// many rows fit on the chip at once as with kRowThreads (the row kernels' registers allow 2-3
+// wavefronts per SIMD)
+constexpr int kListThreads = 128;
+template <int KERN, typename R, int TPB = kListThreads>
 __global__ void k_gradients_rows(DevParams P, int nrows, const int *act, const double *X, const double *Y, const double *Z,
                                  const GradRec<R> *rec, GradSrc src, const double *T, const double *CN, long n, const int *ell,
                                  long ld, const int *cnt, double *GR, double *GP, double *GV, BoxTable bt);
 // STATS: the force kernels also count the RiemannRung bits of their faces into F.rungs
-template <int KERN, typename R, bool STATS = false>
+template <int KERN, typename R, bool STATS = false, int TPB = kListThreads>
 __global__ void k_force_rows(DevParams P, Gam<R> G, int nrows, const int *act, ForceView F, const Cell<R> *rec, const int *ell,
                              long ld, const int *cnt, const uint8_t *active, const int *timebin, const double *msv_in, int *wakeup,
                              double *ACC, double *DTU, double *MSV, int *err, int *need_wakeup, unsigned long long *pairs,

@@ -691,17 +691,17 @@ __global__ void __launch_bounds__(64) k_group_stencil(GridView g, int ngroups, c
 #endif
 
 // gradients of few rows: the terms of each list entry in LDS, accumulated in list order by lane 0
-template <int KERN, typename R>
-__global__ void __launch_bounds__(kRowThreads) k_gradients_rows(DevParams P, int nrows, const int *__restrict__ act, const double *__restrict__ X,
+template <int KERN, typename R, int TPB>
+__global__ void __launch_bounds__(TPB) k_gradients_rows(DevParams P, int nrows, const int *__restrict__ act, const double *__restrict__ X,
                                                        const double *__restrict__ Y, const double *__restrict__ Z,
                                                        const GradRec<R> *__restrict__ rec, GradSrc src, const double *__restrict__ T,
                                                        const double *__restrict__ CN, long n, const int *__restrict__ ell, long ld,
                                                        const int *__restrict__ cnt, double *__restrict__ GR, double *__restrict__ GP,
                                                        double *__restrict__ GV, BoxTable bt)
 {
-    __shared__ R sd[5][kRowThreads], sw[3][kRowThreads], srr[kRowThreads];
-    __shared__ int sf[kRowThreads];
-    __shared__ unsigned long long smask[kRowThreads / 64];
+    __shared__ R sd[5][TPB], sw[3][TPB], srr[TPB];
+    __shared__ int sf[TPB];
+    __shared__ unsigned long long smask[TPB / 64];
     const long r = xcd_block();
     if (r >= nrows) return;
     const int lane = threadIdx.x;  // thread of the workgroup
@@ -725,7 +725,7 @@ __global__ void __launch_bounds__(kRowThreads) k_gradients_rows(DevParams P, int
         kernel_hinv(hi, hinv, hinv3, hinv4);
         const R h2i = hi * hi;
         const int c = cnt[r];
-        for (int base = 0; base < c; base += kRowThreads) {
+        for (int base = 0; base < c; base += TPB) {
             const int q = base + lane;
             int f = 0;
             if (q < c) {
@@ -762,7 +762,7 @@ __global__ void __launch_bounds__(kRowThreads) k_gradients_rows(DevParams P, int
             if ((lane & 63) == 0) smask[lane >> 6] = mask;
             __syncthreads();
             if (lane < 26) {
-                for (int w = 0; w < kRowThreads / 64; w++) {
+                for (int w = 0; w < TPB / 64; w++) {
                 unsigned long long m = smask[w];
                 while (m) {
                     const int k = w * 64 + __ffsll((long long)m) - 1;
@@ -799,8 +799,8 @@ __global__ void __launch_bounds__(kRowThreads) k_gradients_rows(DevParams P, int
 
 // force of few rows: each lane evaluates one list entry (hydro_accumulate_neighbor), lane 0
 // accumulates in list order, then the writeback and final operations
-template <int KERN, typename R, bool STATS>
-__global__ void __launch_bounds__(kRowThreads) k_force_rows(DevParams P, Gam<R> G, int nrows, const int *__restrict__ act, ForceView F,
+template <int KERN, typename R, bool STATS, int TPB>
+__global__ void __launch_bounds__(TPB) k_force_rows(DevParams P, Gam<R> G, int nrows, const int *__restrict__ act, ForceView F,
                                                    const Cell<R> *__restrict__ rec, const int *__restrict__ ell, long ld,
                                                    const int *__restrict__ cnt, const uint8_t *__restrict__ active,
                                                    const int *__restrict__ timebin, const double *__restrict__ msv_in,
@@ -808,8 +808,8 @@ __global__ void __launch_bounds__(kRowThreads) k_force_rows(DevParams P, Gam<R> 
                                                    double *__restrict__ MSV, int *__restrict__ err, int *__restrict__ need_wakeup,
                                                    unsigned long long *__restrict__ pairs, BoxTable bt)
 {
-    __shared__ R sv[5][kRowThreads];
-    __shared__ unsigned long long smask[kRowThreads / 64];
+    __shared__ R sv[5][TPB];
+    __shared__ unsigned long long smask[TPB / 64];
     const long r = xcd_block();
     if (r >= nrows) return;
     const int lane = threadIdx.x;  // thread of the workgroup
@@ -833,7 +833,7 @@ __global__ void __launch_bounds__(kRowThreads) k_force_rows(DevParams P, Gam<R> 
         R acc = 0;       // lane 4
         CSum<R> sum;     // lanes 0-3
         if (lane == 4 && c > 0) acc = (acc < I.cs) ? I.cs : acc;
-        for (int base = 0; base < c; base += kRowThreads) {
+        for (int base = 0; base < c; base += TPB) {
             const int q = base + lane;
             int rc = 0;
             if (q < c) {
@@ -865,7 +865,7 @@ __global__ void __launch_bounds__(kRowThreads) k_force_rows(DevParams P, Gam<R> 
             if ((lane & 63) == 0) smask[lane >> 6] = mask;
             __syncthreads();
             if (lane < 5) {
-                for (int w = 0; w < kRowThreads / 64; w++) {
+                for (int w = 0; w < TPB / 64; w++) {
                     unsigned long long m = smask[w];
                     while (m) {
                         const int k = w * 64 + __ffsll((long long)m) - 1;
