@@ -16,15 +16,15 @@ for the whole timed region.  Warmup: W steps of an identical untimed job.
 
 Both arithmetic forms run the same job: the plain FP64 form (the reported `value`, as the
 job is the FP64-at-scale configuration) and the FP32-mixed form beside it.  `roofline` covers
-the dominant kernel of the timed job, the drift of all 2M particles to each sync point fused
-with their active flags (k_drift_active: 229 algorithmic bytes per particle, 18 doubles and the
-time bin read, 10 doubles and the flag written), its average
-launch time measured with HIP events on the stream the kernels run on.  `roofline_force` is
-the force kernel over the same job (pair-gather model: n_active (S_i + S_out) + n_pairs
-(S_j + 4), S_i = S_j = 288 B and S_out = 40 B per particle in FP64, 144/20 B in FP32, n_pairs
-counted on the device); with a few hundred active rows per step its launches are short and
-latency-bound.  `full_pass` adds one all-active pass of the hot path over the 2M particles
-(the kernels at full occupancy) and `roofline_force_full_pass` the force kernel there.
+the dominant kernel of the timed job: of the three pair kernels (density rounds, gradients,
+force; the units of work of the pass) the one with the most launch time, its launches timed
+with HIP events on the stream the kernels run on.  Its algorithmic bytes follow a pair-gather
+model: rows x (S_i + S_out) + pairs x (S_j + 4), the rows' own fields in and results out, and
+per pair the neighbour's fields and a 4 B list index, pairs counted on the device (PAIR_MODEL,
+FORCE_MODEL below).  With a few hundred active rows per step these launches are short and
+latency-bound.  `roofline_force` is the force kernel over the same job whatever dominates.
+`full_pass` adds one all-active pass of the hot path over the 2M particles (the kernels at
+full occupancy) and `roofline_force_full_pass` the force kernel there.
 The serving loop (`serving`, p50/p99 job latency) and the CPU baseline follow.
 The CPU baseline is the FP64 oracle (C + OpenMP) running the first steps of the same job on
 this host's cores.
@@ -46,15 +46,17 @@ HBM_PEAK_GBS = 8000.0
 FORCE_MODEL = {"fp64": dict(S_i=288, S_out=40, S_j=288), "mixed": dict(S_i=144, S_out=20, S_j=144)}
 INTEG = dict(courant_fac=0.05, err_tol_int_accuracy=0.002, max_size_timestep=0.01, min_size_timestep=0.0,
              time_begin=0.0, time_max=1.0, min_egy_spec=0.0, domain_build_active_fraction=0.5)
-FAMILIES = ("timeline", "drift", "sort", "ngb_build", "density", "gradients", "force")
-# the drift of one particle (k_drift_active): reads X,Y,Z, VelKick3, DivVel, m, VelPred3, HydroAccel3,
-# rho, u, du/dt, h (18 doubles) and the time bin (int); writes X,Y,Z, VelPred3, rho, u, h, P (10 doubles)
-# and the active flag (1 byte); the radius cache of the active rows (8 B each) and one flag bit per
-# particle are left out
-DRIFT_BYTES = {"fp64": 229, "mixed": 229}
-# HBM bytes per particle that k_drift_active moves, measured: FETCH_SIZE + WRITE_SIZE per dispatch at 2,097,152
-# particles (FETCH_SIZE scaled x2 by a 16 MiB device copy), profiles/sedov128_timestep_job_traffic.md
-DRIFT_TRAFFIC_PER_PARTICLE = 480.4e6 / 2097152
+# launch families the library times with HIP events on its stream (gz_timing)
+FAMILIES = ("timeline", "drift", "sort", "ngb_build", "density", "density_after", "pack_grad", "gradients", "pack_force",
+            "force")
+# algorithmic bytes of the three pair kernels (per row: the row's own fields in and its results
+# out; per pair: the neighbour's fields and its list index), FP64 / FP32-mixed:
+#   density (k_density_rows): row 56 B in (position, velocity, radius) + 104 B out (13 sums);
+#     pair 60 B (position, velocity, mass, a 4 B index), the same in both forms (FP64 state);
+#   gradients (k_gradients_rows): row 136 B in + 208 B out (15 gradients, 10 extrema, distance,
+#     written as FP64); pair 84 B (position 24 B + 7 fields 56 B + index);
+#   force: the FORCE_MODEL above.
+PAIR_MODEL = {"density": dict(S_i=56, S_out=104, S_j=56), "gradients": dict(S_i=136, S_out=208, S_j=80)}
 
 
 def log(msg):
@@ -109,6 +111,7 @@ def run_job(ic, precision, steps, warmup):
     assert active2 == active, "the profiled job took other steps than the timed one"
     res["kernels"] = {k: g.timing(k) for k in FAMILIES}
     res["pairs"] = g.stat("force_pairs_total")
+    res["pairs_density"] = g.stat("density_pairs_total")
     g.timing_enable(False)
     g.close()
     return res
@@ -150,16 +153,37 @@ def roofline(res, form):
             "avg_launch_ms": round(total_ms / max(launches, 1), 4)}
 
 
-def roofline_drift(res, n, form):
-    """The dominant kernel of the timed job: the drift of every particle to each sync point."""
-    launches, total_ms = res["kernels"]["drift"]
-    per_launch = n * DRIFT_BYTES[form]
-    avg_ms = total_ms / max(launches, 1)
-    achieved = per_launch / (avg_ms / 1e3) / 1e9
+def roofline_pair_kernel(res, form, family):
+    """A pair kernel of the timed job (the unit of work: density, gradients or force) over all
+    its launches: algorithmic bytes (rows x (S_i + S_out) + pairs x (S_j + 4), pairs counted on
+    the device) over the launches' HIP-event time.  The job's rows per launch are a few hundred,
+    so these launches are latency-bound."""
+    if family == "force":
+        return roofline(res, form)
+    m = PAIR_MODEL[family]
+    launches, total_ms = res["kernels"][family]
+    if family == "density":
+        # rows: every active row of each step times the step's rounds (an upper bound: rows that
+        # converged earlier skip later rounds), pairs: counted by the row kernels
+        rows = sum(res["active"]) * launches / max(len(res["active"]), 1)
+        pairs = res["pairs_density"]
+        name = f"k_density_rows<6,{'double' if form == 'fp64' else 'float'},256>"
+    else:
+        rows = sum(res["active"])
+        pairs = res["pairs"]
+        name = f"k_gradients_rows<6,{'double' if form == 'fp64' else 'float'},128>"
+    bytes_total = rows * (m["S_i"] + m["S_out"]) + pairs * (m["S_j"] + 4)
+    achieved = bytes_total / (total_ms / 1e3) / 1e9
     return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": int(DRIFT_TRAFFIC_PER_PARTICLE * n), "kernel": "k_drift_active",
-            "launches": launches, "bytes_per_launch": per_launch, "avg_launch_ms": round(avg_ms, 4),
-            "bytes_model": f"{DRIFT_BYTES[form]} B per particle x {n} particles"}
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": name, "launches": launches,
+            "bytes_per_launch": int(bytes_total / max(launches, 1)), "avg_launch_ms": round(total_ms / max(launches, 1), 4),
+            "bytes_model": f"rows x ({m['S_i']} + {m['S_out']}) B + pairs x ({m['S_j']} + 4) B"}
+
+
+def roofline_dominant(res, form):
+    """The pair kernel that takes the most time in the timed job."""
+    fam = max(("density", "gradients", "force"), key=lambda k: res["kernels"][k][1])
+    return roofline_pair_kernel(res, form, fam)
 
 
 def roofline_full_pass(fp, form):
@@ -452,13 +476,13 @@ def main():
         "woken_per_step": r["woken"],
         "resorts": r["resorts"],
         "kernel_ms": summary(r)["kernel_ms"],
-        "roofline": roofline_drift(r, n, head),
+        "roofline": roofline_dominant(r, head),
         "roofline_force": roofline(r, head),
         "p50_ms": None, "p99_ms": None,
     }
     for f in res:
         if f != head:
-            out[f] = dict(summary(res[f]), roofline=roofline_drift(res[f], n, f), roofline_force=roofline(res[f], f))
+            out[f] = dict(summary(res[f]), roofline=roofline_dominant(res[f], f), roofline_force=roofline(res[f], f))
     if "fp64" in res and "mixed" in res:
         out["mixed_over_fp64"] = round(summary(res["mixed"])["value"] / summary(res["fp64"])["value"], 3)
     if fp:

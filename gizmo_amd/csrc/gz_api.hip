@@ -7,7 +7,7 @@
 // hydro_gradient_calc() builds the symmetric list once and hydro_force() reuses it, as the
 // reference's hydro corridor does (src/hydro/hydro_corridor.cc:195).
 #include <hip/hip_runtime.h>
-#include <hipcub/hipcub.hpp>
+#include "gz_prim.h"
 
 #include <algorithm>
 #include <cmath>
@@ -34,6 +34,8 @@ inline unsigned nblk(long n) { return (unsigned)((n + kBlock - 1) / kBlock); }
 // steps with more active particles than this drift every particle (the cells around that many
 // rows cover most of a box: one sweep costs less than walking them)
 constexpr long long kLazyMaxRows = 65536;
+// counters[kPassGate]: k_pass_gate's verdict on a speculative first density round
+constexpr int kPassGate = 15;
 
 }  // namespace
 
@@ -67,6 +69,10 @@ struct gz_context {
     bool lazy = false;
     bool last_full = true;  // the last step drifted every particle
     bool density_ready = false;   // the active list's launch set density()'s iteration state for its rows
+    // run-loop steps with few rows launch the passes behind the first density round without
+    // waiting for it (speculate: allowed; spec: in flight, settled by the force pass's read)
+    bool speculate = false, spec = false;
+    bool spec_enabled = true;  // option "speculate"
     double resort_fraction = 0.125;  // order again once dmax exceeds this fraction of the search radius
     int few_rows = 8192;             // at most this many active rows: one wavefront per row
     // the cell ranges and stencils depend only on the order (key_version) and the level: they
@@ -127,6 +133,7 @@ struct gz_context {
     int cap_d = 0;
     long ld_d = 0;
     int *ell_s = nullptr, *cnt_s = nullptr;  // symmetric
+    size_t ell_s_alloc = 0;                  // entries allocated at ell_s (at least cap_s * ld_s)
     int cap_s = 0;
     long ld_s = 0;
     bool sym_valid = false;
@@ -575,9 +582,9 @@ int build_cell_rows(gz_context *ctx)
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemsetAsync(ctx->cflag + na, 0, sizeof(int), ctx->stream));
     size_t bytes = 0;
-    HIPCHK(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, ctx->cflag, ctx->cscan, na + 1, ctx->stream));
+    HIPCHK(gz_exclusive_sum(nullptr, bytes, ctx->cflag, ctx->cscan, na + 1, ctx->stream));
     CHK(ensure_tmp(ctx, bytes));
-    HIPCHK(hipcub::DeviceScan::ExclusiveSum(ctx->tmp, bytes, ctx->cflag, ctx->cscan, na + 1, ctx->stream));
+    HIPCHK(gz_exclusive_sum(ctx->tmp, bytes, ctx->cflag, ctx->cscan, na + 1, ctx->stream));
     k_cell_rows_list<<<nblk(na), kBlock, 0, ctx->stream>>>(na, ctx->act, ctx->key, shift, ctx->cflag, ctx->cscan, ctx->cells);
     HIPCHK(hipGetLastError());
     ctx->rows_level = ctx->level;
@@ -613,9 +620,9 @@ int max_on_device(gz_context *ctx, const double *v, long n, double *out)
 {
     size_t bytes = 0;
     double *dout = (double *)ctx->stage;  // stage is free between uploads
-    HIPCHK(hipcub::DeviceReduce::Max(nullptr, bytes, v, dout, (int)n, ctx->stream));
+    HIPCHK(gz_reduce_max(nullptr, bytes, v, dout, (int)n, ctx->stream));
     CHK(ensure_tmp(ctx, bytes));
-    HIPCHK(hipcub::DeviceReduce::Max(ctx->tmp, bytes, v, dout, (int)n, ctx->stream));
+    HIPCHK(gz_reduce_max(ctx->tmp, bytes, v, dout, (int)n, ctx->stream));
     HIPCHK(hipMemcpyAsync(out, dout, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     return 0;
@@ -625,9 +632,9 @@ int min_on_device(gz_context *ctx, const double *v, long n, double *out)
 {
     size_t bytes = 0;
     double *dout = (double *)ctx->stage;
-    HIPCHK(hipcub::DeviceReduce::Min(nullptr, bytes, v, dout, (int)n, ctx->stream));
+    HIPCHK(gz_reduce_min(nullptr, bytes, v, dout, (int)n, ctx->stream));
     CHK(ensure_tmp(ctx, bytes));
-    HIPCHK(hipcub::DeviceReduce::Min(ctx->tmp, bytes, v, dout, (int)n, ctx->stream));
+    HIPCHK(gz_reduce_min(ctx->tmp, bytes, v, dout, (int)n, ctx->stream));
     HIPCHK(hipMemcpyAsync(out, dout, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     return 0;
@@ -686,9 +693,9 @@ int build_wide(gz_context *ctx, bool lists)
         k_iota<<<nblk(n), kBlock, 0, ctx->stream>>>((int)n, ctx->ival);
         HIPCHK(hipGetLastError());
         size_t bytes = 0;
-        HIPCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, ctx->plev, (int *)ctx->key2, ctx->ival, ctx->lidx, (int)n, 0, 4, ctx->stream));
+        HIPCHK(gz_sort_pairs(nullptr, bytes, ctx->plev, (int *)ctx->key2, ctx->ival, ctx->lidx, (int)n, 0, 4, ctx->stream));
         CHK(ensure_tmp(ctx, bytes));
-        HIPCHK(hipcub::DeviceRadixSort::SortPairs(ctx->tmp, bytes, ctx->plev, (int *)ctx->key2, ctx->ival, ctx->lidx, (int)n, 0, 4, ctx->stream));
+        HIPCHK(gz_sort_pairs(ctx->tmp, bytes, ctx->plev, (int *)ctx->key2, ctx->ival, ctx->lidx, (int)n, 0, 4, ctx->stream));
         int counts[kFineLevel + 2];
         HIPCHK(hipMemcpyAsync(counts, ctx->lcount, sizeof counts, hipMemcpyDeviceToHost, ctx->stream));
         HIPCHK(hipStreamSynchronize(ctx->stream));
@@ -778,10 +785,10 @@ int build_active(gz_context *ctx)
     k_flag_active<<<nblk(n), kBlock, 0, ctx->stream>>>(n, ctx->all_active ? nullptr : ctx->ACT, ctx->M, ctx->iflag);
     HIPCHK(hipGetLastError());
     size_t bytes = 0;
-    HIPCHK(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, ctx->iflag, ctx->iscan, (int)(n + 1), ctx->stream));
+    HIPCHK(gz_exclusive_sum(nullptr, bytes, ctx->iflag, ctx->iscan, (int)(n + 1), ctx->stream));
     CHK(ensure_tmp(ctx, bytes));
     HIPCHK(hipMemsetAsync(ctx->iflag + n, 0, sizeof(int), ctx->stream));
-    HIPCHK(hipcub::DeviceScan::ExclusiveSum(ctx->tmp, bytes, ctx->iflag, ctx->iscan, (int)(n + 1), ctx->stream));
+    HIPCHK(gz_exclusive_sum(ctx->tmp, bytes, ctx->iflag, ctx->iscan, (int)(n + 1), ctx->stream));
     k_compact<<<nblk(n), kBlock, 0, ctx->stream>>>(n, ctx->iflag, ctx->iscan, ctx->act);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(ctx->hcounters + 10, ctx->iscan + n, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
@@ -806,6 +813,9 @@ __global__ void k_iota(int n, int *out)
 }
 
 namespace {
+
+template <int KERN, typename R>
+int density_rounds(gz_context *ctx, int iters, bool speculate);
 
 template <int KERN, typename R>
 int density_stage(gz_context *ctx)
@@ -838,39 +848,78 @@ int density_stage(gz_context *ctx)
     CHK(drift_stencil(ctx));
     const bool few = wide || na <= ctx->few_rows;
     if (!few) CHK(build_cell_rows(ctx));
+    // a step of the run loop with few rows may launch the passes behind the first round before
+    // its counters are read (they are gated on k_pass_gate); the radius bound the symmetric
+    // list's grid will take must then be known
+    const bool speculate = ctx->speculate && few && !wide && ready && ctx->hmax_bound > 0;
+    return density_rounds<KERN, R>(ctx, 0, speculate);
+}
+
+// The density rounds from round `iters` on (0: the first; iters > 0: resuming after the
+// counters of round iters were read back by a later read of the step).  speculate: the first
+// round is launched with k_pass_gate behind it and not waited for; the passes follow, and the
+// force pass's read of the counters settles the step (force_stage).
+template <int KERN, typename R>
+int density_rounds(gz_context *ctx, int iters, bool speculate)
+{
+    const int na = ctx->nact;
+    const bool wide = ctx->wide != 0;
+    const bool few = wide || na <= ctx->few_rows;
     DensityIter it{ctx->it_h, ctx->it_hb, ctx->it_l, ctx->it_r, ctx->it_cn, ctx->it_ci, ctx->it_it};
     DensityOut out{ctx->n, ctx->H, ctx->NN, ctx->DRK, ctx->DIVV, ctx->RHO, ctx->PR, ctx->T, ctx->FW, ctx->CN, ctx->FCE};
-    int iters = 0;
-    int remaining = na;  // rows of the coming round
+    double hmax = 0;
+    int remaining = iters > 0 ? ctx->hcounters[0] : na;  // rows of the coming round
+    bool read = iters > 0;  // the counters of round `iters` are on the host already
     while (true) {
-        // the round's counters (k_density_init cleared them for the first round)
-        if (iters > 0) HIPCHK(hipMemsetAsync(ctx->counters, 0, 3 * sizeof(int), ctx->stream));
-        {
+        if (!read) {
+            // the round's counters (k_density_init, or the step's start, cleared them for the first round)
+            if (iters > 0) HIPCHK(hipMemsetAsync(ctx->counters, 0, 3 * sizeof(int), ctx->stream));
             // over many levels the rounds take one workgroup per row whatever their number (the
             // cells of the rows' own levels hold too few rows to fill a wavefront): four
-            // wavefronts per row while few rows iterate, one while many do
-            Timed t(ctx, "density");
-            if (few && remaining <= ctx->few_rows)
-                k_density_rows<KERN, R, kRowThreads><<<xcd_grid(na), kRowThreads, 0, ctx->stream>>>(
-                    ctx->dp, grid_view(ctx), na, ctx->act, ctx->need, ctx->key, ctx->X, ctx->Y, ctx->Z, ctx->VX, ctx->VY, ctx->VZ, ctx->M,
-                    ctx->it_h, ctx->dacc, BoxTable{});
-            else if (few)
-                k_density_rows<KERN, R, 64><<<xcd_grid(na), 64, 0, ctx->stream>>>(
-                    ctx->dp, grid_view(ctx), na, ctx->act, ctx->need, ctx->key, ctx->X, ctx->Y, ctx->Z, ctx->VX, ctx->VY, ctx->VZ, ctx->M,
-                    ctx->it_h, ctx->dacc, BoxTable{});
-            else
-                k_density_cells<KERN, R><<<xcd_grid(ctx->ncells), 64, 0, ctx->stream>>>(
-                    ctx->dp, grid_view(ctx), cell_rows(ctx), ctx->act, ctx->need, ctx->X, ctx->Y, ctx->Z, ctx->VX, ctx->VY, ctx->VZ,
-                    ctx->M, ctx->it_h, ctx->dacc);
-            k_density_after<<<nblk(na), kBlock, 0, ctx->stream>>>(ctx->dp, na, ctx->act, ctx->need, ctx->dacc, ctx->M, ctx->U,
-                                                                  wide ? 1e300 : ctx->cell - 2.0 * ctx->dmax, it, out, ctx->counters + 0, ctx->counters + 1,
-                                                                  ctx->counters + 2, ctx->u64c + 68, BoxTable{});
+            // wavefronts per row while few rows iterate, one while many do; with timing on, the
+            // rows kernels count each row's pairs (the pass's work)
+            int *npairs = (ctx->timing && few) ? ctx->cnt_d : nullptr;
+            {
+                Timed t(ctx, "density");
+                if (few && remaining <= ctx->few_rows)
+                    k_density_rows<KERN, R, kRowThreads><<<xcd_grid(na), kRowThreads, 0, ctx->stream>>>(
+                        ctx->dp, grid_view(ctx), na, ctx->act, ctx->need, ctx->key, ctx->X, ctx->Y, ctx->Z, ctx->VX, ctx->VY, ctx->VZ,
+                        ctx->M, ctx->it_h, ctx->dacc, BoxTable{}, npairs);
+                else if (few)
+                    k_density_rows<KERN, R, 64><<<xcd_grid(na), 64, 0, ctx->stream>>>(
+                        ctx->dp, grid_view(ctx), na, ctx->act, ctx->need, ctx->key, ctx->X, ctx->Y, ctx->Z, ctx->VX, ctx->VY, ctx->VZ,
+                        ctx->M, ctx->it_h, ctx->dacc, BoxTable{}, npairs);
+                else
+                    k_density_cells<KERN, R><<<xcd_grid(ctx->ncells), 64, 0, ctx->stream>>>(
+                        ctx->dp, grid_view(ctx), cell_rows(ctx), ctx->act, ctx->need, ctx->X, ctx->Y, ctx->Z, ctx->VX, ctx->VY, ctx->VZ,
+                        ctx->M, ctx->it_h, ctx->dacc);
+            }
+            if (npairs) k_sum_counts<<<std::min(nblk(na), 256u), kBlock, 0, ctx->stream>>>(na, npairs, ctx->u64c + 77);
+            {
+                Timed t(ctx, "density_after");
+                k_density_after<<<nblk(na), kBlock, 0, ctx->stream>>>(ctx->dp, na, ctx->act, ctx->need, ctx->dacc, ctx->M, ctx->U,
+                                                                      wide ? 1e300 : ctx->cell - 2.0 * ctx->dmax, it, out, ctx->counters + 0,
+                                                                      ctx->counters + 1, ctx->counters + 2, ctx->u64c + 68, BoxTable{});
+            }
             HIPCHK(hipGetLastError());
+            if (speculate && iters == 0) {
+                k_pass_gate<<<1, 64, 0, ctx->stream>>>(ctx->counters, ctx->u64c + 68, ctx->hmax_bound, ctx->counters + kPassGate);
+                HIPCHK(hipGetLastError());
+                ctx->spec = true;
+                ctx->st.density_iterations = 1;
+                ctx->density_done = true;
+                ctx->gradients_done = false;
+                ctx->sym_valid = false;
+                return 0;
+            }
+            CHK(read_counters(ctx));
+            iters++;
+            remaining = ctx->hcounters[0];
         }
-        CHK(read_counters(ctx));
-        iters++;
-        remaining = ctx->hcounters[0];
-        const int rebuild = ctx->hcounters[1], err = ctx->hcounters[2];
+        // resuming: the round's stop code was settled by k_pass_gate (counters[2] then belongs
+        // to the force pass)
+        const int rebuild = ctx->hcounters[1], err = read ? 0 : ctx->hcounters[2];
+        read = false;
         if (err) {
             ctx->st.density_iterations = iters;
             if (err == GZ_STOP_MAXITER) return fail(ctx, GZ_STOP_MAXITER, "density(): kernel radius iteration did not converge in %d rounds", kMaxIter);
@@ -926,6 +975,18 @@ int build_sched(gz_context *ctx)
     return 0;
 }
 
+// room for the symmetric list's cap_s x ld_s entries: a larger earlier allocation is kept (the
+// row count, and with it ld_s, changes every step of the run loop; freeing and allocating again
+// would wait for the device each time)
+int sym_alloc(gz_context *ctx)
+{
+    const size_t need = (size_t)ctx->cap_s * (size_t)ctx->ld_s;
+    if (ctx->ell_s && ctx->ell_s_alloc >= need) return 0;
+    CHK(dalloc(ctx, &ctx->ell_s, need));
+    ctx->ell_s_alloc = need;
+    return 0;
+}
+
 int ensure_sym(gz_context *ctx)
 {
     if (ctx->sym_valid) return 0;
@@ -953,7 +1014,7 @@ int ensure_sym(gz_context *ctx)
     if (ctx->ell_s == nullptr || ctx->ld_s != need_ld || ctx->cap_s < cap_hint) {
         ctx->cap_s = std::max(ctx->cap_s, cap_hint);
         ctx->ld_s = need_ld;
-        CHK(dalloc(ctx, &ctx->ell_s, (size_t)ctx->cap_s * (size_t)ctx->ld_s));
+        CHK(sym_alloc(ctx));
     }
     for (int attempt = 0; attempt < 8; attempt++) {
         if (attempt > 0 || !ctx->overflow_clear) HIPCHK(hipMemsetAsync(ctx->counters + 3, 0, sizeof(int), ctx->stream));
@@ -996,7 +1057,7 @@ int ensure_sym(gz_context *ctx)
             return 0;
         }
         ctx->cap_s = std::max(over + 16, ctx->cap_s * 3 / 2);
-        CHK(dalloc(ctx, &ctx->ell_s, (size_t)ctx->cap_s * (size_t)ctx->ld_s));
+        CHK(sym_alloc(ctx));
         ctx->st.ell_widen += 1;
     }
     return fail(ctx, GZ_E_STATE, "symmetric neighbour list did not fit after widening");
@@ -1008,11 +1069,11 @@ int gradients_stage(gz_context *ctx)
     if (!ctx->density_done) return fail(ctx, GZ_E_STATE, "hydro_gradient_calc() before density()");
     if (ctx->nact == 0) { ctx->gradients_done = true; return 0; }
     CHK(ensure_sym(ctx));
-    Timed t(ctx, "gradients");
     GradRec<R> *rec = (GradRec<R> *)ctx->rec_grad;
     // few rows among many particles: the row kernel makes the records it reads (no packing launch)
     const bool make = ctx->nact <= ctx->few_rows && ctx->pack_list;
     if (!make) {
+        Timed t(ctx, "pack_grad");
         if (ctx->pack_list)
             k_pack_grad_list<R><<<xcd_grid(ctx->nact), 64, 0, ctx->stream>>>(ctx->nact, ctx->act, ctx->ell_s, ctx->ld_s, ctx->cnt_s, ctx->H,
                                                                              ctx->M, ctx->RHO, ctx->PR, ctx->VX, ctx->VY, ctx->VZ, rec,
@@ -1021,13 +1082,14 @@ int gradients_stage(gz_context *ctx)
             k_pack_grad<R><<<nblk(ctx->n), kBlock, 0, ctx->stream>>>(ctx->n, ctx->H, ctx->M, ctx->RHO, ctx->PR, ctx->VX, ctx->VY, ctx->VZ,
                                                                      rec, ctx->X, ctx->Y, ctx->Z, ctx->par.box_size, BoxTable{});
     }
+    Timed t(ctx, "gradients");
     if (ctx->nact <= ctx->few_rows)
         k_gradients_rows<KERN, R><<<xcd_grid(ctx->nact), kListThreads, 0, ctx->stream>>>(
-            ctx->dp, ctx->nact, ctx->act, ctx->X, ctx->Y, ctx->Z, rec,
+            ctx->dp, ctx->nact, ctx->act, ctx->X, ctx->Y, ctx->Z, ctx->H, rec,
             make ? GradSrc{ctx->H, ctx->M, ctx->RHO, ctx->PR, ctx->VX, ctx->VY, ctx->VZ, ctx->par.box_size} : GradSrc{}, ctx->T, ctx->CN,
             ctx->n, ctx->ell_s, ctx->ld_s, ctx->cnt_s, ctx->GR, ctx->GP, ctx->GV, BoxTable{});
     else
-        k_gradients<KERN, R, false><<<xcd_grid(nblk(ctx->nact)), kBlock, 0, ctx->stream>>>(ctx->dp, ctx->nact, ctx->act, ctx->X, ctx->Y, ctx->Z,
+        k_gradients<KERN, R, false><<<xcd_grid(nblk(ctx->nact)), kBlock, 0, ctx->stream>>>(ctx->dp, ctx->nact, ctx->act, ctx->X, ctx->Y, ctx->Z, ctx->H,
                                                                                     rec, ctx->T, ctx->CN, ctx->n, ctx->ell_s, ctx->ld_s,
                                                                                     ctx->cnt_s, ctx->GR, ctx->GP, ctx->GV, BoxTable{});
     HIPCHK(hipGetLastError());
@@ -1044,22 +1106,27 @@ int force_stage(gz_context *ctx)
     ForceView F{ctx->n, ctx->X, ctx->Y, ctx->Z, ctx->VX, ctx->VY, ctx->VZ, ctx->M, ctx->H, ctx->U, ctx->RHO, ctx->PR, ctx->CN,
                 ctx->FCE, ctx->DRK, ctx->GR, ctx->GP, ctx->GV, ctx->T, ctx->rungs};
     {
-        Timed t(ctx, "force");
         Cell<R> *rec = (Cell<R> *)ctx->rec_force;
-        if (ctx->pack_list)
-            k_pack_force_list<R><<<xcd_grid(ctx->nact), 64, 0, ctx->stream>>>(ctx->nact, ctx->act, ctx->ell_s, ctx->ld_s, ctx->cnt_s,
-                                                                              ctx->par.gamma, F, rec, ctx->counters + 2, ctx->par.box_size,
-                                                                              BoxTable{});
-        else
-            k_pack_force<R><<<nblk(ctx->n), kBlock, 0, ctx->stream>>>(ctx->par.gamma, F, rec, ctx->counters + 2, ctx->par.box_size,
-                                                                      BoxTable{});
+        {
+            Timed t(ctx, "pack_force");
+            if (ctx->pack_list)
+                k_pack_force_list<R><<<xcd_grid(ctx->nact), 64, 0, ctx->stream>>>(ctx->nact, ctx->act, ctx->ell_s, ctx->ld_s, ctx->cnt_s,
+                                                                                  ctx->par.gamma, F, rec, ctx->counters + 2,
+                                                                                  ctx->par.box_size, BoxTable{});
+            else
+                k_pack_force<R><<<nblk(ctx->n), kBlock, 0, ctx->stream>>>(ctx->par.gamma, F, rec, ctx->counters + 2, ctx->par.box_size,
+                                                                          BoxTable{});
+        }
+        Timed t(ctx, "force");
+        // a speculative first density round: the pass's kernels with lasting effects wait on its gate
+        const int *gate = ctx->spec ? ctx->counters + kPassGate : nullptr;
         if (ctx->nact <= ctx->few_rows) {
             // the kernel that counts the faces' Riemann rungs when asked (the same code otherwise)
             auto rows_kernel = ctx->rung_stats ? k_force_rows<KERN, R, true> : k_force_rows<KERN, R, false>;
             rows_kernel<<<xcd_grid(ctx->nact), kListThreads, 0, ctx->stream>>>(
                 ctx->dp, gam<R>(ctx->dp), ctx->nact, ctx->act, F, rec, ctx->ell_s, ctx->ld_s, ctx->cnt_s,
                 ctx->all_active ? nullptr : ctx->ACT, ctx->TB, ctx->MSV, ctx->WAKE, ctx->ACC, ctx->DTU, ctx->MSV, ctx->counters + 2,
-                ctx->counters + 4, ctx->nact <= kRowPairAtomics ? ctx->u64c + 65 : nullptr, BoxTable{});
+                ctx->counters + 4, ctx->nact <= kRowPairAtomics ? ctx->u64c + 65 : nullptr, BoxTable{}, gate);
         } else {
             if (!ctx->rare) CHK(dalloc(ctx, &ctx->rare, ctx->cap));
             const ForceRows a{ctx->nact, ctx->act, ctx->ell_s, ctx->ld_s, ctx->cnt_s, ctx->all_active ? nullptr : ctx->ACT, ctx->TB,
@@ -1079,10 +1146,32 @@ int force_stage(gz_context *ctx)
         // the pairs of the pass: each row's workgroup adds its own while a few hundred rows are
         // active (atomics on one address execute one after another)
         if (ctx->nact > ctx->few_rows || ctx->nact > kRowPairAtomics)
-            k_sum_counts<<<std::min(nblk(ctx->nact), 256u), kBlock, 0, ctx->stream>>>(ctx->nact, ctx->cnt_s, ctx->u64c + 65);
+            k_sum_counts<<<std::min(nblk(ctx->nact), 256u), kBlock, 0, ctx->stream>>>(ctx->nact, ctx->cnt_s, ctx->u64c + 65, gate);
         HIPCHK(hipGetLastError());
     }
     CHK(read_counters(ctx));
+    if (ctx->spec) {
+        ctx->spec = false;
+        const int g = ctx->hcounters[kPassGate];
+        if (g > 0) {
+            ctx->st.density_iterations = 1;
+            if (g == GZ_STOP_MAXITER) return fail(ctx, GZ_STOP_MAXITER, "density(): kernel radius iteration did not converge in %d rounds", kMaxIter);
+            if (g == GZ_STOP_BRACKET) return fail(ctx, GZ_STOP_BRACKET, "density(): empty radius bracket (Right==0 && Left==0)");
+            return fail(ctx, g, "density(): device stop code %d", g);
+        }
+        if (g < 0) {
+            // rows still iterate after the first round, or a radius outgrew the bound the list's
+            // grid took: the rounds go on from the counters just read, then the list, the
+            // gradients and this pass again (the gated kernels did nothing)
+            {
+                const int rc_d = density_rounds<KERN, R>(ctx, 1, false);
+                if (rc_d) return rc_d;
+            }
+            int rc = gradients_stage<KERN, R>(ctx);
+            if (!rc) rc = force_stage<KERN, R>(ctx);
+            return rc;
+        }
+    }
     // rows the common kernel left to the whole Riemann ladder (counters[14]: zero unless it ran)
     if (ctx->nact > ctx->few_rows && !ctx->rung_stats) ctx->st.force_rare_rows += ctx->hcounters[14];
     if (ctx->lazy && ctx->hmax_bound > 0) {
@@ -1097,7 +1186,7 @@ int force_stage(gz_context *ctx)
         if (over) {
             // the list the two passes used was cut short: a wider one, and both passes again
             ctx->cap_s = std::max(over + 16, ctx->cap_s * 3 / 2);
-            CHK(dalloc(ctx, &ctx->ell_s, (size_t)ctx->cap_s * (size_t)ctx->ld_s));
+            CHK(sym_alloc(ctx));
             ctx->st.ell_widen += 1;
             ctx->sym_valid = false;
             const bool d = ctx->defer_sym_check;
@@ -1161,9 +1250,9 @@ int resort(gz_context *ctx)
     k_morton_soa<<<nblk(n), kBlock, 0, ctx->stream>>>(n, ctx->X, ctx->Y, ctx->Z, ctx->par.box_size, ctx->key2, ctx->ival);
     HIPCHK(hipGetLastError());
     size_t bytes = 0;
-    HIPCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, ctx->key2, ctx->key, ctx->ival, ctx->ival2, (int)n, 0, 3 * kFineLevel, ctx->stream));
+    HIPCHK(gz_sort_pairs(nullptr, bytes, ctx->key2, ctx->key, ctx->ival, ctx->ival2, (int)n, 0, 3 * kFineLevel, ctx->stream));
     CHK(ensure_tmp(ctx, bytes));
-    HIPCHK(hipcub::DeviceRadixSort::SortPairs(ctx->tmp, bytes, ctx->key2, ctx->key, ctx->ival, ctx->ival2, (int)n, 0, 3 * kFineLevel, ctx->stream));
+    HIPCHK(gz_sort_pairs(ctx->tmp, bytes, ctx->key2, ctx->key, ctx->ival, ctx->ival2, (int)n, 0, 3 * kFineLevel, ctx->stream));
     const int *order = ctx->ival2;
     // every per-particle column gathered into a spare buffer of its width, up to kPermCols
     // columns per launch; the spare then takes the field's place and the field's old buffer
@@ -1426,7 +1515,10 @@ int run_step(gz_context *ctx)
     // change places in a re-ordering: the view is taken again)
     T = time_params(ctx);
     S = particle_view(ctx);
+    ctx->speculate = ctx->spec_enabled;
     const int rc_pass = hydro_pass(ctx);
+    ctx->speculate = false;
+    ctx->spec = false;
     ctx->density_ready = false;
     if (rc_pass) ctx->lazy = false;
     CHK(rc_pass);
@@ -1436,7 +1528,9 @@ int run_step(gz_context *ctx)
         k_halfstep_kick<<<nblk(ctx->nact), kBlock, 0, ctx->stream>>>(T, S, ctx->nact, ctx->act, 1, BoxTable{});
         HIPCHK(hipGetLastError());
     }
-    if (ctx->stop_code) return fail(ctx, ctx->stop_code, "timestep loop: controlled stop %d (step beyond the end of the timeline)", ctx->stop_code);
+    if (ctx->stop_code)
+        return fail(ctx, ctx->stop_code, "timestep loop: controlled stop %d (%s)", ctx->stop_code,
+                    ctx->stop_code == 818 ? "a step of size zero on the integer timeline" : "step beyond the end of the timeline");
     return 0;
 }
 
@@ -1533,9 +1627,9 @@ int gz_load(gz_context *ctx, long n, const gz_fields *f)
         k_morton<<<nblk(n), kBlock, 0, ctx->stream>>>(n, ctx->stage, ctx->par.box_size, ctx->key2, ctx->ival);
         HIPCHK(hipGetLastError());
         size_t bytes = 0;
-        HIPCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, ctx->key2, ctx->key, ctx->ival, ctx->perm, (int)n, 0, 3 * kFineLevel, ctx->stream));
+        HIPCHK(gz_sort_pairs(nullptr, bytes, ctx->key2, ctx->key, ctx->ival, ctx->perm, (int)n, 0, 3 * kFineLevel, ctx->stream));
         CHK(ensure_tmp(ctx, bytes));
-        HIPCHK(hipcub::DeviceRadixSort::SortPairs(ctx->tmp, bytes, ctx->key2, ctx->key, ctx->ival, ctx->perm, (int)n, 0, 3 * kFineLevel, ctx->stream));
+        HIPCHK(gz_sort_pairs(ctx->tmp, bytes, ctx->key2, ctx->key, ctx->ival, ctx->perm, (int)n, 0, 3 * kFineLevel, ctx->stream));
     }
     // positions and velocities split into SoA columns
     k_gather<<<nblk(n), kBlock, 0, ctx->stream>>>(n, 3, ctx->perm, ctx->stage, ctx->stage + 3 * n, 1);
@@ -1740,6 +1834,13 @@ int gz_stat(gz_context *ctx, const char *name, double *value)
         *value = s;
         return 0;
     }
+    if (k == "density_pairs_total") {
+        // pairs of the few-row density rounds since gz_timing_reset, counted while timing is on
+        unsigned long long v = 0;
+        HIPCHK(hipMemcpy(&v, ctx->u64c + 77, sizeof v, hipMemcpyDeviceToHost));
+        *value = (double)v;
+        return 0;
+    }
     if (k == "force_pairs_total") {
         unsigned long long v = 0;
         HIPCHK(hipMemcpy(&v, ctx->u64c + 65, sizeof v, hipMemcpyDeviceToHost));
@@ -1787,6 +1888,7 @@ int gz_timing_reset(gz_context *ctx)
     ctx->st.force_rare_rows = 0;
     ctx->st.full_drifts = 0;
     HIPCHK(hipMemsetAsync(ctx->u64c + 65, 0, sizeof(unsigned long long), ctx->stream));
+    HIPCHK(hipMemsetAsync(ctx->u64c + 77, 0, sizeof(unsigned long long), ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     return 0;
 }
@@ -1906,6 +2008,11 @@ int gz_set_option(gz_context *ctx, const char *name, double value)
         ctx->rung_stats = value != 0;
         HIPCHK(hipMemsetAsync(ctx->rungs, 0, 16 * sizeof(unsigned long long), ctx->stream));
         HIPCHK(hipStreamSynchronize(ctx->stream));
+        return 0;
+    }
+    if (k == "speculate") {
+        if (value != 0 && value != 1) return fail(ctx, GZ_E_ARG, "speculate must be 0 or 1");
+        ctx->spec_enabled = value != 0;
         return 0;
     }
     if (k == "wide_search") {

@@ -19,7 +19,7 @@
 // stepping flag, and writes only those (the grid columns of every entry, or one whole entry of
 // a box that is not stepping).
 #include <hip/hip_runtime.h>
-#include <hipcub/hipcub.hpp>
+#include "gz_prim.h"
 
 #include <algorithm>
 #include <cmath>
@@ -329,9 +329,9 @@ int compact_rows(gz_batch *b)
 {
     const long n = b->cap;
     size_t bytes = 0;
-    BHIP(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, b->iflag, b->iscan, (int)(n + 1), b->stream));
+    BHIP(gz_exclusive_sum(nullptr, bytes, b->iflag, b->iscan, (int)(n + 1), b->stream));
     BCHK(ensure_tmp(b, bytes));
-    BHIP(hipcub::DeviceScan::ExclusiveSum(b->tmp, bytes, b->iflag, b->iscan, (int)(n + 1), b->stream));
+    BHIP(gz_exclusive_sum(b->tmp, bytes, b->iflag, b->iscan, (int)(n + 1), b->stream));
     k_compact_count<<<nb(n + 1), kB, 0, b->stream>>>(n, b->iflag, b->iscan, b->act, b->counters + 10);
     BHIP(hipGetLastError());
     return 0;
@@ -395,9 +395,9 @@ int reorder(gz_batch *b, const std::vector<uint8_t> &due_h)
     int bits = 30;
     while ((1LL << (bits - 30)) <= (long long)b->max_boxes + 1) bits++;
     size_t bytes = 0;
-    BHIP(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, b->key64, b->key64b, b->ival, b->ival2, (int)n, 0, bits, b->stream));
+    BHIP(gz_sort_pairs(nullptr, bytes, b->key64, b->key64b, b->ival, b->ival2, (int)n, 0, bits, b->stream));
     BCHK(ensure_tmp(b, bytes));
-    BHIP(hipcub::DeviceRadixSort::SortPairs(b->tmp, bytes, b->key64, b->key64b, b->ival, b->ival2, (int)n, 0, bits, b->stream));
+    BHIP(gz_sort_pairs(b->tmp, bytes, b->key64, b->key64b, b->ival, b->ival2, (int)n, 0, bits, b->stream));
     BCHK(permute_all(b, b->ival2));
     long off = 0;
     for (int id : b->order) {
@@ -608,10 +608,10 @@ int gradients_force(gz_batch *b)
                                                           bt);
     const bool lanes = na > b->few_rows;  // many rows: one lane per row
     if (lanes)
-        k_gradients<KERN, R, true><<<xcd_grid(nb(na)), kB, 0, b->stream>>>(b->dp, na, b->act, b->X, b->Y, b->Z, rg, b->T, b->CN, b->cap, b->ell, ld,
+        k_gradients<KERN, R, true><<<xcd_grid(nb(na)), kB, 0, b->stream>>>(b->dp, na, b->act, b->X, b->Y, b->Z, b->H, rg, b->T, b->CN, b->cap, b->ell, ld,
                                                                       b->cnt, b->GR, b->GP, b->GV, bt);
     else
-        k_gradients_rows<KERN, R><<<xcd_grid(na), kListThreads, 0, b->stream>>>(b->dp, na, b->act, b->X, b->Y, b->Z, rg, GradSrc{}, b->T, b->CN, b->cap,
+        k_gradients_rows<KERN, R><<<xcd_grid(na), kListThreads, 0, b->stream>>>(b->dp, na, b->act, b->X, b->Y, b->Z, b->H, rg, GradSrc{}, b->T, b->CN, b->cap,
                                                                        b->ell, ld, b->cnt, b->GR, b->GP, b->GV, bt);
     BHIP(hipGetLastError());
     ForceView F{b->cap, b->X, b->Y, b->Z, b->VX, b->VY, b->VZ, b->M, b->H, b->U, b->RHO, b->PR, b->CN, b->FCE, b->DRK, b->GR, b->GP,
@@ -787,10 +787,10 @@ int gz_batch_add(gz_batch *b, long n, const gz_fields *f, const gz_box_params *b
     uint32_t *ksrt = b->key + off;
     int *psrt = b->perm + off;
     size_t bytes = 0;
-    BHIP(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, (const uint32_t *)b->ival2, ksrt, b->ival, psrt, (int)n, 0, 3 * kFineLevel,
+    BHIP(gz_sort_pairs(nullptr, bytes, (const uint32_t *)b->ival2, ksrt, b->ival, psrt, (int)n, 0, 3 * kFineLevel,
                                             b->stream));
     BCHK(ensure_tmp(b, bytes));
-    BHIP(hipcub::DeviceRadixSort::SortPairs(b->tmp, bytes, (const uint32_t *)b->ival2, ksrt, b->ival, psrt, (int)n, 0, 3 * kFineLevel,
+    BHIP(gz_sort_pairs(b->tmp, bytes, (const uint32_t *)b->ival2, ksrt, b->ival, psrt, (int)n, 0, 3 * kFineLevel,
                                             b->stream));
     // a host field (width doubles per particle, given order) into the box's range of its columns
     auto up = [&](const double *src, int w, double *dst) -> int {

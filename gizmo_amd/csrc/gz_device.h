@@ -49,9 +49,10 @@ __device__ __forceinline__ void kernel_hinv(R h, R &hinv, R &hinv3, R &hinv4)
     hinv4 = hinv3 * hinv;
 }
 
-// MODE 0: wk and dwk; MODE -1: wk only (as kernel_main's mode argument)
+// MODE 0: wk and dwk; MODE -1: wk only (as kernel_main's mode argument).  kernel_main_q takes
+// 1 - u as well (q; the edge factor of both kernels), kernel_main forms it from u.
 template <int KERN, int MODE, typename R>
-__device__ __forceinline__ void kernel_main(R u, R hinv3, R hinv4, R &wk, R &dwk)
+__device__ __forceinline__ void kernel_main_q(R u, R q, R hinv3, R hinv4, R &wk, R &dwk)
 {
     if (u >= R(1)) { wk = 0; dwk = 0; return; }
     if constexpr (KERN == 3) {
@@ -59,13 +60,13 @@ __device__ __forceinline__ void kernel_main(R u, R hinv3, R hinv4, R &wk, R &dwk
             if (MODE >= 0) dwk = u * (R(18.0) * u - R(12.0));
             if (MODE <= 0) wk = (R(1.0) + R(6.0) * (u - R(1.0)) * u * u);
         } else {
-            R t1 = (R(1.0) - u);
+            R t1 = q;
             R t2 = t1 * t1;
             if (MODE >= 0) dwk = R(-6.0) * t2;
             if (MODE <= 0) wk = R(2.0) * t2 * t1;
         }
     } else {
-        R t1 = (R(1) - u);
+        R t1 = q;
         R t3 = t1 * t1 * t1;
         if (MODE >= 0) dwk = R(-20.0) * u * t3;
         if (MODE <= 0) wk = t3 * t1 * (R(1.0) + R(4.0) * u);
@@ -73,6 +74,43 @@ __device__ __forceinline__ void kernel_main(R u, R hinv3, R hinv4, R &wk, R &dwk
     const R norm = R(KernelNorm<KERN>::v);
     if (MODE >= 0) dwk *= norm * hinv4;
     if (MODE <= 0) wk *= norm * hinv3;
+}
+template <int KERN, int MODE, typename R>
+__device__ __forceinline__ void kernel_main(R u, R hinv3, R hinv4, R &wk, R &dwk)
+{
+    kernel_main_q<KERN, MODE, R>(u, R(1) - u, hinv3, hinv4, wk, dwk);
+}
+
+// The kernel of a pair at distance r from a particle of radius h (wk, dwk left as they are
+// beyond h).  FP64 form: the reference's kernel_hinv and kernel_main at u = r / h.  FP32 form:
+// the edge factor 1 - u from the FP64 distance rd and radius h, the rest in FP32.  A partner
+// near the edge of h sits where the kernel is steep (W ~ (1 - u)^4), so the FP32 rounding of r
+// and h would enter its weight many times over: a hot partner at u = 0.993 of a cold particle
+// carries most of its pressure gradient, and an FP32 u moves that gradient by 5e-5.
+template <int KERN, int MODE, typename R>
+__device__ __forceinline__ void pair_kernel(R r, double rd, double h, R &wk, R &dwk)
+{
+    R hinv, hinv3, hinv4;
+    kernel_hinv((R)h, hinv, hinv3, hinv4);
+    if constexpr (std::is_same<R, double>::value) {
+        (void)rd;
+        if (r < h) kernel_main<KERN, MODE>(r * hinv, hinv3, hinv4, wk, dwk);
+    } else {
+        (void)r;
+        if (rd < h) {
+            const R q = (R)((h - rd) * (double)hinv);
+            kernel_main_q<KERN, MODE>(R(1) - q, q, hinv3, hinv4, wk, dwk);
+        }
+    }
+}
+
+// |dp|^2 of a pair in FP64 from its components (the FP32 form's differences are exact to a few
+// 1e-10 of the distance: fine cells and FP32 offsets); the FP64 form's is its own r2
+template <typename R>
+__device__ __forceinline__ double pair_r2d(R dp0, R dp1, R dp2, R r2)
+{
+    if constexpr (std::is_same<R, double>::value) return r2;
+    return (double)dp0 * (double)dp0 + (double)dp1 * (double)dp1 + (double)dp2 * (double)dp2;
 }
 
 // periodic nearest image, TMP_WRAP_*_S (src/declarations/macros.h:67-95)
@@ -697,8 +735,10 @@ __device__ __forceinline__ void fine_dp(const FinePos &a, const FinePos &b, floa
     d2 = (float)dc[2] * cs + (a.o2 - b.o2);
 }
 
+// h is FP64 in both forms (the FP32 form takes the kernel's edge factor from it, pair_kernel)
 template <typename R> struct alignas(16) Cell : PosRec<R> {
-    R h, m, rho, p, cs, cn, fce, drk;
+    double h;
+    R m, rho, p, cs, cn, fce, drk;
     R v0, v1, v2;
     R gr0, gr1, gr2, gp0, gp1, gp2;
     R gv[9];
@@ -780,18 +820,19 @@ __device__ inline int force_pair(const DevParams &P, const Gam<R> &G, const IC &
     vsig_ts = 0;
     if (J.m <= 0) return 0;
     if (J.rho <= 0) return 0;
-    if ((r2 >= I.h * I.h) && (r2 >= J.h * J.h)) return 0;
+    const double r2d = pair_r2d<R>(dp0, dp1, dp2, r2);
+    if ((r2d >= I.h * I.h) && (r2d >= J.h * J.h)) return 0;
     if (r2 <= 0) return 0;
     const R r = sqrt(r2);
     const R rinv = R(1.0) / r;
     const R dv0 = I.v0 - J.v0, dv1 = I.v1 - J.v1, dv2 = I.v2 - J.v2;
     R vsig = I.cs + J.cs;
-    R hinv, hinv3, hinv4, u;
     R wk_i = 0, dwk_i = 0, wk_j = 0, dwk_j = 0;
-    kernel_hinv(I.h, hinv, hinv3, hinv4);
-    if (r < I.h) { u = r * hinv; kernel_main<KERN, 0>(u, hinv3, hinv4, wk_i, dwk_i); }
-    kernel_hinv(J.h, hinv, hinv3, hinv4);
-    if (r < J.h) { u = r * hinv; kernel_main<KERN, 0>(u, hinv3, hinv4, wk_j, dwk_j); }
+    {
+        const double rd = std::is_same<R, double>::value ? 0.0 : sqrt(r2d);
+        pair_kernel<KERN, 0, R>(r, rd, I.h, wk_i, dwk_i);
+        pair_kernel<KERN, 0, R>(r, rd, J.h, wk_j, dwk_j);
+    }
     const R cnumcrit2 = R(kCnDanger) * R(kCnDanger) - I.cn * I.cn;
     const R vdotr2 = dp0 * dv0 + dp1 * dv1 + dp2 * dv2;
     if (vdotr2 < 0) vsig -= R(1.0) * vdotr2 * rinv;
@@ -825,15 +866,21 @@ __device__ inline int force_pair(const DevParams &P, const Gam<R> &G, const IC &
         }
         An = sqrt(An);
     }
-    if (An == 0) return 1;
-    if constexpr (sizeof(R) != sizeof(double)) {
-        // A face whose area vector is below 1e-18 (a partner at the very edge of its kernel,
-        // W ~ 0) carries no flux at FP32 precision: its squares underflow FP32 and its normal
-        // would not be a unit vector.  In FP64 its flux is of the order of the area itself.
-        if (!(dmax(fabs(A0), dmax(fabs(A1), fabs(A2))) > R(1e-18))) return 1;
-    }
     R n[3];
-    {
+    if constexpr (sizeof(R) != sizeof(double)) {
+        // FP32: the area and the normal from the vector scaled by its largest component, so
+        // that no square underflows or overflows whatever the unit system (the FP64 form keeps
+        // the reference's sqrt of the sum of squares); a face with no area carries no flux
+        const R amax = dmax(fabs(A0), dmax(fabs(A1), fabs(A2)));
+        if (!(amax > 0)) return 1;
+        const R s = R(1.0) / amax;
+        const R a0 = A0 * s, a1 = A1 * s, a2 = A2 * s;
+        const R len = sqrt(a0 * a0 + a1 * a1 + a2 * a2);
+        An = amax * len;
+        const R inv = R(1.0) / len;
+        n[0] = a0 * inv; n[1] = a1 * inv; n[2] = a2 * inv;
+    } else {
+        if (An == 0) return 1;
         R inv = R(1.0) / An;
         n[0] = A0 * inv; n[1] = A1 * inv; n[2] = A2 * inv;
     }

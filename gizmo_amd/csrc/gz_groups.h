@@ -9,7 +9,7 @@
 // group's 27-cell stencil.
 #pragma once
 #include <hip/hip_runtime.h>
-#include <hipcub/hipcub.hpp>
+#include "gz_prim.h"
 
 #include "gz_kernels.h"
 
@@ -45,10 +45,10 @@ struct RowGroups {
 inline hipError_t group_tmp_bytes(int nrows, size_t &bytes)
 {
     size_t a = 0, b = 0;
-    hipError_t e = hipcub::DeviceRadixSort::SortPairs(nullptr, a, (const uint32_t *)nullptr, (uint32_t *)nullptr, (const int *)nullptr,
+    hipError_t e = gz_sort_pairs(nullptr, a, (const uint32_t *)nullptr, (uint32_t *)nullptr, (const int *)nullptr,
                                                       (int *)nullptr, nrows, 0, 32);
     if (e != hipSuccess) return e;
-    e = hipcub::DeviceScan::ExclusiveSum(nullptr, b, (const int *)nullptr, (int *)nullptr, nrows + 1);
+    e = gz_exclusive_sum(nullptr, b, (const int *)nullptr, (int *)nullptr, nrows + 1);
     bytes = a > b ? a : b;
     return e;
 }
@@ -70,13 +70,13 @@ inline hipError_t group_rows(RowGroups &G, hipStream_t s, void *tmp, size_t tmp_
     const uint32_t *sk = G.key;
     if (sort) {
         size_t bytes = tmp_bytes;
-        e = hipcub::DeviceRadixSort::SortPairs(tmp, bytes, G.key, G.skey, G.val, G.row, nrows, 0, 32, s);
+        e = gz_sort_pairs(tmp, bytes, G.key, G.skey, G.val, G.row, nrows, 0, 32, s);
         if (e != hipSuccess) return e;
         sk = G.skey;
     }
     k_group_heads<<<nbk, B, 0, s>>>(nrows, sk, G.flag);
     size_t bytes = tmp_bytes;
-    e = hipcub::DeviceScan::ExclusiveSum(tmp, bytes, G.flag, G.scan, nrows + 1, s);
+    e = gz_exclusive_sum(tmp, bytes, G.flag, G.scan, nrows + 1, s);
     if (e != hipSuccess) return e;
     k_group_list<<<nbk, B, 0, s>>>(nrows, sk, G.flag, G.scan, G.start, G.gkey, G.count);
     return hipGetLastError();

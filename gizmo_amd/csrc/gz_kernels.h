@@ -213,7 +213,8 @@ __global__ void k_vk_max(ParticleView S);
 __global__ void k_advance_boxes(BoxTable bt);
 __global__ void k_active_from_bins(long n, unsigned long long mask, const int *TB, const double *M, const double *H,
                                    uint8_t *ACT, double *SOFT);
-__global__ void k_sum_counts(int nrows, const int *cnt, unsigned long long *out);
+__global__ void k_sum_counts(int nrows, const int *cnt, unsigned long long *out, const int *gate = nullptr);
+__global__ void k_pass_gate(const int *counters, const unsigned long long *hfinal, double hlim, int *gate);
 // groups of rows sharing one cell of one grid (the search over many levels, batches of boxes)
 __global__ void k_row_group_key(int nrows, const int *act, const uint8_t *need, const uint32_t *key, const double *hslot,
                                 const double *H, double box, double margin, const WideGrid *W, BoxTable bt, uint32_t *gkey, int *val);
@@ -306,7 +307,7 @@ __global__ void k_pack_force_list(int nrows, const int *act, const int *ell, lon
 // BOXES: each row's box geometry from bt (batches); else the single box of the arguments
 template <int KERN, typename R, bool BOXES>
 __global__ void k_gradients(DevParams P, int nrows, const int *act, const double *X, const double *Y, const double *Z,
-                            const GradRec<R> *rec, const double *T, const double *CN, long n, const int *ell, long ld,
+                            const double *H, const GradRec<R> *rec, const double *T, const double *CN, long n, const int *ell, long ld,
                             const int *cnt, double *GR, double *GP, double *GV, BoxTable bt);
 
 // one workgroup per active row, for passes over few rows: kRowThreads lanes (4 wavefronts)
@@ -318,7 +319,8 @@ constexpr int kRowThreads = 256;
 template <int KERN, typename R, int TPB>
 __global__ void k_density_rows(DevParams P, GridView g, int nrows, const int *act, const uint8_t *need, const uint32_t *key,
                                const double *X, const double *Y, const double *Z, const double *VX, const double *VY,
-                               const double *VZ, const double *M, const double *hcur, double *accout, BoxTable bt);
+                               const double *VZ, const double *M, const double *hcur, double *accout, BoxTable bt,
+                               int *npairs = nullptr);
 __global__ void k_build_sym_rows(GridView g, int nrows, const int *act, const uint32_t *key, const double *X, const double *Y,
                                  const double *Z, const double *H, int *ell, long ld, int cap, int *cnt, int *overflow, BoxTable bt);
 // the fields a gradient record is made of (H null: the pass reads packed records)
@@ -333,14 +335,14 @@ struct GradSrc {
 constexpr int kListThreads = 128;
 template <int KERN, typename R, int TPB = kListThreads>
 __global__ void k_gradients_rows(DevParams P, int nrows, const int *act, const double *X, const double *Y, const double *Z,
-                                 const GradRec<R> *rec, GradSrc src, const double *T, const double *CN, long n, const int *ell,
+                                 const double *H, const GradRec<R> *rec, GradSrc src, const double *T, const double *CN, long n, const int *ell,
                                  long ld, const int *cnt, double *GR, double *GP, double *GV, BoxTable bt);
 // STATS: the force kernels also count the RiemannRung bits of their faces into F.rungs
 template <int KERN, typename R, bool STATS = false, int TPB = kListThreads>
 __global__ void k_force_rows(DevParams P, Gam<R> G, int nrows, const int *act, ForceView F, const Cell<R> *rec, const int *ell,
                              long ld, const int *cnt, const uint8_t *active, const int *timebin, const double *msv_in, int *wakeup,
                              double *ACC, double *DTU, double *MSV, int *err, int *need_wakeup, unsigned long long *pairs,
-                             BoxTable bt);
+                             BoxTable bt, const int *gate = nullptr);
 
 // the rows, lists and outputs of a force pass's lane-per-row kernels
 struct ForceRows {

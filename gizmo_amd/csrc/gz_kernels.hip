@@ -18,7 +18,7 @@ template __global__ void k_build_ell<true>(int, const int *, const int *, const 
 
 #define GZ_INST_LANES(KERN, R, BOXES)                                                                                           \
     template __global__ void k_gradients<KERN, R, BOXES>(DevParams, int, const int *, const double *, const double *,           \
-                                                         const double *, const GradRec<R> *, const double *, const double *,    \
+                                                         const double *, const double *, const GradRec<R> *, const double *, const double *,    \
                                                          long, const int *, long, const int *, double *, double *, double *,    \
                                                          BoxTable);                                                              \
     template __global__ void k_force<KERN, R, BOXES, false, true>(DevParams, Gam<R>, ForceView, const Cell<R> *, ForceRows,   \
@@ -29,7 +29,7 @@ template __global__ void k_build_ell<true>(int, const int *, const int *, const 
     template __global__ void k_density_rows<KERN, R, TPB>(DevParams, GridView, int, const int *, const uint8_t *,               \
                                                           const uint32_t *, const double *, const double *, const double *,     \
                                                           const double *, const double *, const double *, const double *,       \
-                                                          const double *, double *, BoxTable);
+                                                          const double *, double *, BoxTable, int *);
 
 #define GZ_INST(KERN, R)                                                                                                         \
     template __global__ void k_density_cells<KERN, R>(DevParams, GridView, CellRows, const int *, const uint8_t *,              \
@@ -40,13 +40,13 @@ template __global__ void k_build_ell<true>(int, const int *, const int *, const 
     GZ_INST_DROWS(KERN, R, 64)                                                                                                   \
     GZ_INST_DROWS(KERN, R, kRowThreads)                                                                                          \
     template __global__ void k_gradients_rows<KERN, R>(DevParams, int, const int *, const double *, const double *,             \
-                                                       const double *, const GradRec<R> *, GradSrc, const double *,             \
+                                                       const double *, const double *, const GradRec<R> *, GradSrc, const double *,             \
                                                        const double *, long, const int *, long, const int *, double *, double *,\
                                                        double *, BoxTable);                                                     \
     template __global__ void k_force_rows<KERN, R>(DevParams, Gam<R>, int, const int *, ForceView, const Cell<R> *,            \
                                                    const int *, long, const int *, const uint8_t *, const int *,                \
                                                    const double *, int *, double *, double *, double *, int *, int *,           \
-                                                   unsigned long long *, BoxTable);
+                                                   unsigned long long *, BoxTable, const int *);
 GZ_INST(3, double)
 GZ_INST(6, double)
 // the force kernels that count the Riemann rungs of their faces (one box)
@@ -56,7 +56,7 @@ GZ_INST(6, double)
     template __global__ void k_force_rows<KERN, R, true>(DevParams, Gam<R>, int, const int *, ForceView, const Cell<R> *,      \
                                                          const int *, long, const int *, const uint8_t *, const int *,          \
                                                          const double *, int *, double *, double *, double *, int *, int *,     \
-                                                         unsigned long long *, BoxTable);
+                                                         unsigned long long *, BoxTable, const int *);
 GZ_INST_STATS(3, double)
 GZ_INST_STATS(6, double)
 template __global__ void k_riemann_probe<double>(int, Gam<double>, long, const double *, double *, unsigned *);

@@ -657,8 +657,10 @@ __device__ __forceinline__ void wave_max_to(double v, bool valid, int box, unsig
 
 // the sum of the list lengths (the pairs a force pass evaluates): one atomic per block
 #ifdef GZ_DEFINE_COMMON_KERNELS
-__global__ void __launch_bounds__(256) k_sum_counts(int nrows, const int *__restrict__ cnt, unsigned long long *__restrict__ out)
+__global__ void __launch_bounds__(256) k_sum_counts(int nrows, const int *__restrict__ cnt, unsigned long long *__restrict__ out,
+                                                    const int *__restrict__ gate)
 {
+    if (gate && *gate) return;
     __shared__ unsigned long long part[4];
     unsigned long long s = 0;
     for (long r = blockIdx.x * (long)blockDim.x + threadIdx.x; r < nrows; r += (long)gridDim.x * blockDim.x) s += (unsigned long long)cnt[r];
@@ -669,6 +671,23 @@ __global__ void __launch_bounds__(256) k_sum_counts(int nrows, const int *__rest
         s = part[0] + part[1] + part[2] + part[3];
         if (s) atomicAdd(out, s);
     }
+}
+
+// Whether the passes launched behind a step's first density round, before its counters are
+// read, stand: gate = the round's stop code (counters[2]); else -1 while rows still iterate
+// (counters[0]); else -2 when a final radius (hfinal, bits of the largest) exceeds hlim, the
+// bound every radius was taken under when the symmetric list's grid was sized; else 0.  The
+// passes' kernels with effects that a second run would not undo (wake-up flags, pair counts)
+// read it and do nothing unless it is 0.
+__global__ void k_pass_gate(const int *__restrict__ counters, const unsigned long long *__restrict__ hfinal, double hlim,
+                            int *__restrict__ gate)
+{
+    if (threadIdx.x != 0) return;
+    int g = 0;
+    if (counters[2]) g = counters[2];
+    else if (counters[0]) g = -1;
+    else if (__longlong_as_double((long long)*hfinal) > hlim) g = -2;
+    *gate = g;
 }
 #endif
 
@@ -789,7 +808,7 @@ __device__ __forceinline__ void pack_force_one(long j, double gamma, const Force
 {
     const long n = F.n;
     Cell<R> c;
-    c.h = (R)F.H[j]; c.m = (R)F.M[j]; c.rho = (R)F.RHO[j]; c.p = (R)F.PR[j];
+    c.h = F.H[j]; c.m = (R)F.M[j]; c.rho = (R)F.RHO[j]; c.p = (R)F.PR[j];
     c.cn = (R)F.CN[j]; c.fce = (R)F.FCE[j]; c.drk = (R)F.DRK[j];
     c.v0 = (R)F.VX[j]; c.v1 = (R)F.VY[j]; c.v2 = (R)F.VZ[j];
     c.gr0 = (R)F.GR[j]; c.gr1 = (R)F.GR[n + j]; c.gr2 = (R)F.GR[2 * n + j];
@@ -894,7 +913,7 @@ __device__ inline void gradients_tail(const GradRec<R> &I, bool enabled, const R
 template <int KERN, typename R, bool BOXES>
 __global__ void __launch_bounds__(256) k_gradients(DevParams P, int nrows, const int *__restrict__ act, const double *__restrict__ X,
                                                    const double *__restrict__ Y, const double *__restrict__ Z,
-                                                   const GradRec<R> *__restrict__ rec, const double *__restrict__ T,
+                                                   const double *__restrict__ H, const GradRec<R> *__restrict__ rec, const double *__restrict__ T,
                                                    const double *__restrict__ CN, long n, const int *__restrict__ ell, long ld,
                                                    const int *__restrict__ cnt, double *__restrict__ GR, double *__restrict__ GP,
                                                    double *__restrict__ GV, BoxTable bt)
@@ -916,9 +935,9 @@ __global__ void __launch_bounds__(256) k_gradients(DevParams P, int nrows, const
     const R hi = I.h;
     if (enabled) {
         const double xi = X[i], yi = Y[i], zi = Z[i];
-        R hinv, hinv3, hinv4;
-        kernel_hinv(hi, hinv, hinv3, hinv4);
-        const R h2i = hi * hi;
+        // the row's radius in FP64 (the kernel's edge factor, pair_kernel)
+        const double hid = std::is_same<R, double>::value ? (double)hi : H[i];
+        const double h2i = hid * hid;
         const int c = cnt[r];
         for (int q = 0; q < c; q++) {
             const int j = ell[(size_t)q * ld + r];
@@ -928,13 +947,13 @@ __global__ void __launch_bounds__(256) k_gradients(DevParams P, int nrows, const
             record_dp<R>(P, I, J, xi, yi, zi, X, Y, Z, j, dp[0], dp[1], dp[2]);
             const R r2 = dp[0] * dp[0] + dp[1] * dp[1] + dp[2] * dp[2];
             if (r2 <= 0) continue;
-            if ((r2 >= h2i) && (r2 >= J.h * J.h)) continue;
+            const double r2d = pair_r2d<R>(dp[0], dp[1], dp[2], r2);
+            if ((r2d >= h2i) && (r2d >= (double)J.h * (double)J.h)) continue;
             const R rr = sqrt(r2);
+            const double rd = std::is_same<R, double>::value ? 0.0 : sqrt(r2d);
+            const bool inside = std::is_same<R, double>::value ? (rr < hi) : (rd < hid);
             R wk = 0, dwk = 0;
-            if (rr < hi) {
-                const R u = rr * hinv;
-                kernel_main<KERN, -1>(u, hinv3, hinv4, wk, dwk);
-            }
+            pair_kernel<KERN, -1, R>(rr, rd, hid, wk, dwk);
             if (rr > maxd) maxd = rr;
             R d[5];
             d[0] = (R)(J.rho - I.rho);
@@ -946,7 +965,7 @@ __global__ void __launch_bounds__(256) k_gradients(DevParams P, int nrows, const
                 if (d[b] < mn[b]) mn[b] = d[b];
                 else if (d[b] > mx[b]) mx[b] = d[b];
             }
-            if (rr < hi) {
+            if (inside) {
                 for (int kk = 0; kk < 3; kk++) {
                     const R w = -wk * dp[kk];
                     for (int b = 0; b < 5; b++) G[kk][b] += rounded(w * d[b]);

@@ -179,14 +179,18 @@ __global__ void __launch_bounds__(TPB) k_density_rows(DevParams P, GridView g, i
                                                      const double *__restrict__ Z, const double *__restrict__ VX,
                                                      const double *__restrict__ VY, const double *__restrict__ VZ,
                                                      const double *__restrict__ M, const double *__restrict__ hcur,
-                                                     double *__restrict__ accout, BoxTable bt)
+                                                     double *__restrict__ accout, BoxTable bt, int *__restrict__ npairs)
 {
     __shared__ RowStencil rs;
     __shared__ R sc[13][TPB];
     __shared__ int sf[TPB];
     __shared__ unsigned long long smask[TPB / 64];
     const long r = xcd_block();
-    if (r >= nrows || !need[r]) return;
+    if (r >= nrows) return;
+    if (!need[r]) {
+        if (npairs && threadIdx.x == 0) npairs[r] = 0;
+        return;
+    }
     const int tid = threadIdx.x;
     const int i = act[r];
     resolve_box(bt, i, P, g);
@@ -216,6 +220,7 @@ __global__ void __launch_bounds__(TPB) k_density_rows(DevParams P, GridView g, i
     const R rxi = (R)wrap(xi - ox, g.box, g.half), ryi = (R)wrap(yi - oy, g.box, g.half), rzi = (R)wrap(zi - oz, g.box, g.half);
     const int total = rs.pref[rs.nst];
     R acc = 0;  // thread q < 13 keeps sum q
+    int accepted = 0;  // thread 13: the row's pairs (the pass's work, counted when npairs is given)
     for (int base = 0; base < total; base += TPB) {
         const int v = base + tid;
         int f = 0;
@@ -281,10 +286,13 @@ __global__ void __launch_bounds__(TPB) k_density_rows(DevParams P, GridView g, i
                     }
                 }
             }
+        } else if (tid == 13 && npairs) {
+            for (int w = 0; w < TPB / 64; w++) accepted += __popcll(smask[w]);
         }
         __syncthreads();
     }
     if (tid < 13) accout[(size_t)tid * nrows + r] = (double)acc;
+    if (tid == 13 && npairs) npairs[r] = accepted;
 }
 
 // the symmetric list of few rows (the same entries, in the same order, as k_build_sym_cells)
@@ -694,7 +702,7 @@ __global__ void __launch_bounds__(64) k_group_stencil(GridView g, int ngroups, c
 template <int KERN, typename R, int TPB>
 __global__ void __launch_bounds__(TPB) k_gradients_rows(DevParams P, int nrows, const int *__restrict__ act, const double *__restrict__ X,
                                                        const double *__restrict__ Y, const double *__restrict__ Z,
-                                                       const GradRec<R> *__restrict__ rec, GradSrc src, const double *__restrict__ T,
+                                                       const double *__restrict__ H, const GradRec<R> *__restrict__ rec, GradSrc src, const double *__restrict__ T,
                                                        const double *__restrict__ CN, long n, const int *__restrict__ ell, long ld,
                                                        const int *__restrict__ cnt, double *__restrict__ GR, double *__restrict__ GP,
                                                        double *__restrict__ GV, BoxTable bt)
@@ -721,9 +729,9 @@ __global__ void __launch_bounds__(TPB) k_gradients_rows(DevParams P, int nrows, 
     const R hi = I.h;
     if (enabled) {
         const double xi = X[i], yi = Y[i], zi = Z[i];
-        R hinv, hinv3, hinv4;
-        kernel_hinv(hi, hinv, hinv3, hinv4);
-        const R h2i = hi * hi;
+        // the row's radius in FP64 (the kernel's edge factor, pair_kernel)
+        const double hid = std::is_same<R, double>::value ? (double)hi : H[i];
+        const double h2i = hid * hid;
         const int c = cnt[r];
         for (int base = 0; base < c; base += TPB) {
             const int q = base + lane;
@@ -735,13 +743,14 @@ __global__ void __launch_bounds__(TPB) k_gradients_rows(DevParams P, int nrows, 
                     R dp[3];
                     record_dp<R>(P, I, J, xi, yi, zi, X, Y, Z, j, dp[0], dp[1], dp[2]);
                     const R r2 = dp[0] * dp[0] + dp[1] * dp[1] + dp[2] * dp[2];
-                    if ((r2 > 0) && !((r2 >= h2i) && (r2 >= J.h * J.h))) {
+                    const double r2d = pair_r2d<R>(dp[0], dp[1], dp[2], r2);
+                    if ((r2 > 0) && !((r2d >= h2i) && (r2d >= (double)J.h * (double)J.h))) {
                         const R rr = sqrt(r2);
+                        const double rd = std::is_same<R, double>::value ? 0.0 : sqrt(r2d);
                         R wk = 0, dwk = 0;
                         f = 1;
-                        if (rr < hi) {
-                            const R u = rr * hinv;
-                            kernel_main<KERN, -1>(u, hinv3, hinv4, wk, dwk);
+                        if (std::is_same<R, double>::value ? (rr < hi) : (rd < hid)) {
+                            pair_kernel<KERN, -1, R>(rr, rd, hid, wk, dwk);
                             f = 2;
                             for (int kk = 0; kk < 3; kk++) sw[kk][lane] = -wk * dp[kk];
                         }
@@ -806,12 +815,13 @@ __global__ void __launch_bounds__(TPB) k_force_rows(DevParams P, Gam<R> G, int n
                                                    const int *__restrict__ timebin, const double *__restrict__ msv_in,
                                                    int *__restrict__ wakeup, double *__restrict__ ACC, double *__restrict__ DTU,
                                                    double *__restrict__ MSV, int *__restrict__ err, int *__restrict__ need_wakeup,
-                                                   unsigned long long *__restrict__ pairs, BoxTable bt)
+                                                   unsigned long long *__restrict__ pairs, BoxTable bt, const int *__restrict__ gate)
 {
     __shared__ R sv[5][TPB];
     __shared__ unsigned long long smask[TPB / 64];
     const long r = xcd_block();
     if (r >= nrows) return;
+    if (gate && *gate) return;  // the density round behind this launch did not stand (k_pass_gate)
     const int lane = threadIdx.x;  // thread of the workgroup
     const int i = act[r];
     {

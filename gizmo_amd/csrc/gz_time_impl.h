@@ -50,8 +50,12 @@ __device__ __forceinline__ void raise_vk_bound(const ParticleView &S, long i, bo
 
 // get_timestep, src/core/timestep.cc:357-1423 for a gas particle here: acceleration criterion
 // with the kernel radius as softening (:417-440, src/gravity/forcetree.cc:85-91), Courant on
-// MaxSignalVel (:711-728), velocity divergence (:1092-1098), caps and integer step (:1360-1417)
-__device__ long long get_timestep(const TimeParams &T, const ParticleView &S, long i)
+// MaxSignalVel (:711-728), velocity divergence (:1092-1098), caps and integer step (:1360-1417).
+// zero_step: the step failed the integer-timeline check that ends the run with stop 818
+// (:1398-1411); the step handed back is then 2, as there.  After the floor of 2 and the cap at
+// half the timeline the check cannot fail for a finite or NaN dt (NaN takes the minimum step),
+// so the flag only guards the timeline's invariants.
+__device__ long long get_timestep(const TimeParams &T, const ParticleView &S, long i, bool &zero_step)
 {
     const long n = S.n;
     double ax = 0.0, ay = 0.0, az = 0.0;
@@ -84,7 +88,8 @@ __device__ long long get_timestep(const TimeParams &T, const ParticleView &S, lo
     if (dt > 0.5 * kTimebase * T.tbi) dt = 0.5 * kTimebase * T.tbi;
     long long ti_step = (long long)(dt / T.tbi);
     if (ti_step <= 1) ti_step = 2;
-    if (!(ti_step > 0 && ti_step < kTimebase)) ti_step = 2;
+    zero_step = !(ti_step > 0 && ti_step < kTimebase);
+    if (zero_step) ti_step = 2;
     return ti_step;
 }
 
@@ -132,7 +137,12 @@ __global__ void __launch_bounds__(256) k_find_timesteps(TimeParams T, ParticleVi
     if (r < nrows) {
         const int i = act[r];
         resolve_time(bt, i, T);
-        long long ti_step = get_timestep(T, S, i);
+        bool zero_step;
+        long long ti_step = get_timestep(T, S, i, zero_step);
+        if (zero_step) {
+            if (bt.b) atomicMax(&bt.slot[(size_t)bt.BOX[i] * kBoxSlots + kSlotStop], 818ull);
+            else atomicMax(err, 818);
+        }
         ti_step = (long long)(((double)ti_step) / 1.0);
         long long ti_min = kTimebase;
         while (ti_min > ti_step) ti_min >>= 1;

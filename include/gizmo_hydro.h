@@ -225,9 +225,14 @@ int gz_drift_all(gz_context *ctx);
    built one wavefront per cell of the rows' own levels, the density rounds one workgroup per
    row); "row_schedule" (default 2048; 0 or a power of two from 64 to 2048) -- the
    one-lane-per-particle force pass takes the rows of each block of this many longest list
-   first (a wavefront lasts as long as its longest list; 0: plain order).  None of these
-   changes a neighbour set or a sum, and none has a counterpart in the reference (tuning knobs
-   of this library). */
+   first (a wavefront lasts as long as its longest list; 0: plain order); "speculate" (default
+   1) -- a run-loop step over at most few_rows rows launches the symmetric list, the gradients
+   and the force pass behind its first density round without waiting for that round's counters
+   (the force kernels, whose wake-up flags and pair counts a second run would not undo, wait on
+   a device flag: the round converged every row and no radius outgrew the list's grid); one
+   read of the counters after the force pass then settles the step, and when the flag says no,
+   the rounds go on and the three passes run again.  None of these changes a neighbour set or a
+   sum, and none has a counterpart in the reference (tuning knobs of this library). */
 int gz_set_option(gz_context *ctx, const char *name, double value);
 
 /* The Riemann functions of the force pass on n given face states, on the context's device and

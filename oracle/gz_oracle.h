@@ -109,7 +109,7 @@ int orc_riemann_probe(int mode, double gamma, long n, const double *in, double *
 void orc_rung_counts(long long *out, int reset);
 int orc_force(const orc_params *par, orc_state *s, const unsigned char *active);
 int orc_force_ex(const orc_params *par, orc_state *s, const unsigned char *active, int *need_wakeup);
-long long orc_get_timestep(const orc_params *par, const orc_integ *ip, const orc_timeline *tl, const orc_state *s, long i);
+long long orc_get_timestep(const orc_params *par, const orc_integ *ip, orc_timeline *tl, const orc_state *s, long i);
 int orc_run_begin(const orc_params *par, const orc_integ *ip, orc_timeline *tl, orc_state *s);
 int orc_run_step(const orc_params *par, const orc_integ *ip, orc_timeline *tl, orc_state *s);
 void orc_drift_all(const orc_params *par, const orc_integ *ip, const orc_timeline *tl, orc_state *s);

@@ -84,7 +84,7 @@ static void ot_stop(orc_timeline *tl, int code)
    the acceleration criterion (:417-440), the Courant criterion on MaxSignalVel (:711-728),
    the velocity-divergence criterion (:1092-1098), then the caps and the integer conversion
    (:1360-1417). */
-long long orc_get_timestep(const orc_params *par, const orc_integ *ip, const orc_timeline *tl, const orc_state *s, long i)
+long long orc_get_timestep(const orc_params *par, const orc_integ *ip, orc_timeline *tl, const orc_state *s, long i)
 {
     double ax = 0.0, ay = 0.0, az = 0.0; /* cf_a2inv * GravAccel: no gravity */
     ax += s->accel[3 * i + 0];
@@ -116,7 +116,10 @@ long long orc_get_timestep(const orc_params *par, const orc_integ *ip, const orc
     if (dt > 0.5 * OT_TIMEBASE * tl->timebase_interval) dt = 0.5 * OT_TIMEBASE * tl->timebase_interval;
     long long ti_step = (long long)(dt / tl->timebase_interval);
     if (ti_step <= 1) ti_step = 2;
-    if (!(ti_step > 0 && ti_step < OT_TIMEBASE)) ti_step = 2; /* the stop 818 path hands back 2 */
+    if (!(ti_step > 0 && ti_step < OT_TIMEBASE)) {
+        ot_stop(tl, 818); /* endrun(818) of :1398-1411, which hands back 2 */
+        ti_step = 2;
+    }
     return ti_step;
 }
 

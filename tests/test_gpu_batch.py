@@ -169,10 +169,48 @@ def test_box_in_mix_against_oracle(oracle_lib):
     d = out["pos"] - st.pos
     box = par["box"]
     assert np.abs(d - box * np.round(d / box)).max() <= 1e-9 * box
+    import parity
+    sc = parity.scales(st, par["gamma"])
     for k in ("u", "rho", "h", "accel", "vel"):
-        a, e = np.asarray(out[k]).reshape(st.n, -1), np.asarray(getattr(st, k)).reshape(st.n, -1)
-        err = float((np.abs(a - e) / np.maximum(np.abs(e).max(axis=0), 1e-300)).max())
+        err = parity.err(k, out[k], getattr(st, k), sc)
         assert err <= 1e-9, (k, err)
+
+
+PASS_FIELDS = ("h", "numngb", "drkern", "divvel", "rho", "press", "nvt", "nvt_fw", "cn", "fce", "grad_rho", "grad_p",
+               "grad_v", "accel", "dtu", "maxsig")
+
+
+def test_box_in_mix_fp32_against_oracle(oracle_lib):
+    """The FP32-mixed batch (the ensemble's form): a box of a mix against the FP64 oracle after
+    the batch's first pass (every particle active), at the oracle's converged radii with a band
+    so wide that the first evaluation stands (identical inputs and neighbour sets): every
+    quantity of every particle within 1e-5 per particle (tests/parity.py), du/dt within 2e-5
+    (the blast's cold particles next to its hot centre; see test_gpu_parity.py)."""
+    import parity
+    O = oracle_lib
+    ic, integ = box_ic("sedov12_b2")
+    other, integ_o = box_ic("lattice14")
+    par = dict(ic["params"])
+    s1 = O.State.from_ic(ic)
+    O.density(par, s1)
+    pfix = dict(par, max_num_ngb_dev=1e30)
+    ref = s1.copy()
+    O.density(pfix, ref)
+    O.gradients(pfix, ref)
+    O.force(pfix, ref)
+    n = s1.n + len(other["mass"])
+    from gizmo_amd.hydro import GZ_FP32_MIXED
+    b = Batch(pfix, n, 2, GZ_FP32_MIXED)
+    b.add(particles(other), other["params"], integ_o)
+    bid = b.add(dict(pos=s1.pos, vel=s1.vel, mass=s1.mass, u=s1.u, h=s1.h), par, integ)
+    print(f"fp32 box vs oracle: n={s1.n} in a mix of {n}", flush=True)
+    b.begin()
+    out = b.store(bid, PASS_FIELDS)
+    b.close()
+    pe = parity.errs(PASS_FIELDS, out, ref, parity.scales(ref, par["gamma"]))
+    print("per-particle:", " ".join(f"{k}={v:.1e}" for k, v in pe.items()), flush=True)
+    for k, e in pe.items():
+        assert e <= (2e-5 if k == "dtu" else 1e-5), (k, e)
 
 
 def test_batch_rejects_what_does_not_fit():

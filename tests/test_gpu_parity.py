@@ -226,21 +226,68 @@ def test_active_subset_and_wakeup(cases, oracle_lib, path):
     g.close()
 
 
+# FP32-mixed at the oracle's radii, largest per-particle error allowed per quantity (tests/parity.py
+# measures: each particle against its own value, floored at its physical scale): 1e-5 for every
+# quantity, except du/dt in the blast, 2e-5: the cold particles next to the hot centre (pressure
+# ratio 1e6 across a face) take their energy rate from FP32 Riemann solves of those faces, whose
+# star pressure carries a few 1e-5 of rounding (measured on the MI355X: lattice32 all <= 2.1e-6;
+# sedov24 all <= 5.6e-6, du/dt 1.3e-5)
+FP32_BAR = {"lattice32": {}, "sedov24": {"dtu": 2e-5}}
+
+
+def fp32_reference(O, c):
+    """The oracle at the radii its own iteration converged to, with the band so wide that the
+    first evaluation stands (the device gets the same radii and the same band: identical inputs
+    and neighbour sets)."""
+    par = dict(c["par"], max_num_ngb_dev=1e30)
+    s = c["s1"].copy()
+    O.density(par, s)
+    O.gradients(par, s)
+    O.force(par, s)
+    return par, s
+
+
 @pytest.mark.parametrize("name", ["lattice32", "sedov24"])
-def test_fp32_mixed_error(cases, name, path):
-    """FP32-mixed form against the FP64 oracle on identical inputs (errors relative to each
-    field's largest magnitude; printed per quantity)."""
+def test_fp32_mixed_error(cases, oracle_lib, name, path):
+    """FP32-mixed form against the FP64 oracle on identical inputs (the oracle's converged radii):
+    every quantity of every particle within its bar (FP32_BAR)."""
+    c = cases[name]
+    par, ref = fp32_reference(oracle_lib, c)
+    s1 = c["s1"]
+    g = gpu_from(s1, par, precision=GZ_FP32_MIXED, path=path)
+    g.hydro_pass()
+    out = g.store()
+    g.close()
+    fields = DENSITY_FIELDS + GRAD_FIELDS + FORCE_FIELDS
+    pe = parity.errs(fields, out, ref, pp(par, ref))
+    print(name, path, "per-particle:", " ".join(f"{k}={v:.2e}" for k, v in pe.items()))
+    for k in fields:
+        assert pe[k] <= FP32_BAR[name].get(k, 1e-5), (k, pe[k])
+
+
+@pytest.mark.parametrize("name", ["lattice32", "sedov24"])
+def test_fp32_mixed_error_own_radii(cases, name, path):
+    """FP32-mixed form iterating its own radii (a full pass from the initial conditions) against
+    the FP64 oracle's: the radius iteration stops within the same tolerance band, at a point of
+    it that FP32 sums may move, so particles whose radius lands elsewhere see other neighbour
+    sets.  The radius and density stay within 1e-5 of the field's largest value, and 99% of the
+    particles within 1e-5 per particle in every quantity."""
     c = cases[name]
     g = gpu_from(c["s0"], c["par"], precision=GZ_FP32_MIXED, path=path)
     g.hydro_pass()
     out = g.store()
+    g.close()
     errs = {k: relerr(out[k], getattr(c["s3"], k)) for k in DENSITY_FIELDS + GRAD_FIELDS + FORCE_FIELDS}
     print(name, path, "field-max:", " ".join(f"{k}={v:.2e}" for k, v in errs.items()))
-    pe = parity.errs(DENSITY_FIELDS + GRAD_FIELDS + FORCE_FIELDS, out, c["s3"], pp(c["par"], c["s3"]))
-    print(name, path, "per-particle:", " ".join(f"{k}={v:.2e}" for k, v in pe.items()))
+    sc = pp(c["par"], c["s3"])
+    for k in DENSITY_FIELDS + GRAD_FIELDS + FORCE_FIELDS:
+        a = np.asarray(out[k], dtype=np.float64).reshape(c["s3"].n, -1)
+        b = np.asarray(getattr(c["s3"], k), dtype=np.float64).reshape(c["s3"].n, -1)
+        d = np.sqrt(((a - b) ** 2).sum(1))
+        den = np.maximum(np.sqrt((b ** 2).sum(1)), sc.get(k, 0.0))
+        e = d / np.where(den > 0, den, 1e-300)
+        assert np.quantile(e, 0.99) <= 1e-5, (k, np.quantile(e, 0.99))
     assert errs["h"] <= 1e-5 and errs["rho"] <= 1e-5
-    assert max(errs.values()) <= 1e-3
-    g.close()
 
 
 @pytest.mark.parametrize("name", ["lattice32", "sedov24"])

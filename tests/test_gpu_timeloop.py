@@ -10,9 +10,10 @@ Bars, FP64 plain form:
     after the whole run (the device sums in its Morton order and the oracle in the caller's, so
     rounding differences enter at 1e-16 and grow through the steps);
   * positions compared as periodic differences.
-Workloads: global steps on a 16^3 cubic-spline lattice (every particle in one bin, the forced
-single-bin recipe of the reference's own tests), and adaptive bins on a 20^3 Sedov blast whose
-hot centre runs up to 2^8 times finer steps than the cold medium and wakes its neighbours.
+Workloads: global steps on a 32^3 cubic-spline lattice (every particle in one bin, the forced
+single-bin recipe of the reference's own tests), adaptive bins on a 32^3 Sedov blast whose
+hot centre runs up to 2^8 times finer steps than the cold medium and wakes its neighbours, and
+the first 10 sync points of the benchmark's 128^3 (2M-particle) Sedov job.
 """
 import numpy as np
 import pytest
@@ -127,6 +128,21 @@ def test_adaptive_bins_sedov(oracle_lib, resort_fraction, few_rows, frac):
     if frac > 0:
         lazy = [h for h in hist if not h[2]]
         assert len(lazy) >= 20 and max(h[3] for h in lazy) > st.n // 2, "the cold medium was drifted every step"
+    check_fields(st, out, par)
+
+
+def test_sedov128_job_against_oracle(oracle_lib):
+    """The 2M-particle job of the benchmark at its size (128^3 Sedov blast, Wendland C2, 64
+    neighbours, CourantFac 0.05, ErrTolIntAccuracy 0.002, MaxSizeTimestep 0.01,
+    DomainBuild_ActiveFraction 0.5): its first 10 sync points against the oracle's run(), the
+    timeline identical at each and every field within 1e-9 per particle at the end."""
+    ic = ics.sedov_box(128, 1002, kernel=6, des_num_ngb=64.0)
+    integ = dict(courant_fac=0.05, err_tol_int_accuracy=0.002, max_size_timestep=0.01, min_size_timestep=0.0,
+                 time_begin=0.0, time_max=1.0, min_egy_spec=0.0, domain_build_active_fraction=0.5)
+    st, out, hist, par = run_both(oracle_lib, ic, integ, 10)
+    del out["resorts"]
+    print("sedov128 steps (active, woken, full drift, behind):", hist, flush=True)
+    assert max(h[0] for h in hist) < st.n // 10, "the job's steps are partial steps"
     check_fields(st, out, par)
 
 

@@ -103,10 +103,67 @@ def test_full_pass_against_oracle(wide_case):
         assert err <= 1e-12, (k, err)
 
 
-def test_fp32_mixed_wide_runs(wide_case):
-    ic, par = wide_case["ic"], wide_case["par"]
-    g = device(ic, par, 1, GZ_FP32_MIXED)
+@pytest.mark.parametrize("few_rows", [0, 10**9])
+def test_fp32_mixed_wide_against_oracle(wide_case, oracle_lib, few_rows):
+    """The FP32-mixed form on the lognormal field (gamma = 1.001, radii spanning 130x) against
+    the FP64 oracle at the oracle's converged radii, with the band so wide that the first
+    evaluation stands for every particle that has its neighbours: 99% of the particles within
+    1e-5 per particle in every quantity (measured on the MI355X: p99 <= 1.7e-6).  The largest
+    errors sit at the few particles whose radius the iteration still moves (a handful of
+    neighbours, where the radius search never settles): there the FP32 sums move the radius
+    itself (up to 2e-4), and with it the neighbour sets."""
+    import parity
+    O = oracle_lib
+    par = dict(wide_case["par"], max_num_ngb_dev=1e30)
+    s1 = wide_case["s1"]
+    ref = s1.copy()
+    O.density(par, ref)
+    O.gradients(par, ref)
+    O.force(par, ref)
+    g = Hydro(par, s1.n, GZ_FP32_MIXED)
+    g.set_option("wide_search", 1)
+    g.set_option("few_rows", few_rows)
+    g.load(dict(pos=s1.pos, vel=s1.vel, mass=s1.mass, u=s1.u, h=s1.h))
+    print(f"wide fp32: n={s1.n} few_rows={few_rows}", flush=True)
     g.hydro_pass()
-    out = g.store(("rho", "accel"))
+    out = g.store(FIELDS)
     g.close()
-    assert np.isfinite(out["rho"]).all() and np.isfinite(out["accel"]).all()
+    sc = parity.scales(ref, par["gamma"])
+    for k in FIELDS:
+        a = np.asarray(out[k], dtype=np.float64).reshape(s1.n, -1)
+        b = np.asarray(getattr(ref, k), dtype=np.float64).reshape(s1.n, -1)
+        d = np.sqrt(((a - b) ** 2).sum(1))
+        den = np.maximum(np.sqrt((b ** 2).sum(1)), sc.get(k, 0.0))
+        e = d / np.where(den > 0, den, 1e-300)
+        print(f"  {k}: p99 {np.quantile(e, 0.99):.2e} max {e.max():.2e}", flush=True)
+        assert np.isfinite(a).all(), k
+        assert np.quantile(e, 0.99) <= 1e-5, (k, np.quantile(e, 0.99))
+
+
+def test_fp32_mixed_loop_near_isothermal(wide_case):
+    """The FP32-mixed form's Riemann ladder at gamma = 1.001 (the lognormal field, Mach 2) over 30
+    sync points of the timestep loop with the search over many levels: no face fails the ladder
+    (a failed face would stop the loop with code 1234) and every field stays finite, as in the
+    FP64 run of the same loop (the rungs each form took are printed)."""
+    ic, par = wide_case["ic"], wide_case["par"]
+    integ = dict(courant_fac=0.1, err_tol_int_accuracy=0.01, max_size_timestep=0.01, min_size_timestep=0.0,
+                 time_begin=0.0, time_max=1.0, min_egy_spec=0.0)
+    counts = {}
+    for name, precision in (("fp64", GZ_FP64), ("fp32_mixed", GZ_FP32_MIXED)):
+        g = device(ic, par, 1, precision)
+        g.set_option("riemann_stats", 1)
+        g.run_begin(integ)
+        print(f"near-isothermal loop {name}: n={len(ic['mass'])} gamma={par['gamma']}", flush=True)
+        acts = [g.run_step() for _ in range(30)]
+        tl = g.timeline()
+        out = g.store(["vel", "u", "rho", "h", "accel", "dtu"])
+        rungs = g.rung_counts()
+        g.close()
+        assert tl["stop_code"] == 0
+        for k, v in out.items():
+            assert np.isfinite(v).all(), (name, k)
+        counts[name] = (sum(acts), tl["ti_current"], {k: v for k, v in rungs.items() if v})
+        print(f"near-isothermal loop {name}: {sum(acts)} active updates, ti {tl['ti_current']}, rungs {counts[name][2]}",
+              flush=True)
+    r32 = counts["fp32_mixed"][2]
+    assert r32.get("riemann_faces", 0) > 0 and r32.get("riemann_fail", 0) == 0, r32
