@@ -170,6 +170,44 @@ __device__ __forceinline__ void row_stencil_wide(const GridView &g, const WideGr
     stencil_finish(rs, code, st, en);
 }
 
+// The place of this thread's entry among the flagged entries of its workgroup, in thread order
+// (total: the flagged entries of the workgroup).  One barrier: smask[TPB / 64] must not be read
+// by another round of the caller before the caller's next barrier.
+template <int TPB>
+__device__ __forceinline__ int flagged_rank(bool flag, unsigned long long *smask, int &total)
+{
+    const unsigned long long mask = __ballot(flag);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) smask[w] = mask;
+    __syncthreads();
+    int before = 0;
+    total = 0;
+    for (int k = 0; k < TPB / 64; k++) {
+        const int c = __popcll(smask[k]);
+        if (k < w) before += c;
+        total += c;
+    }
+    return before + __popcll(mask & ((1ull << lane) - 1ull));
+}
+
+// acc + s[0] + s[1] + ... + s[n-1] (or minus each, neg), added one after another in that order;
+// the loads run ahead of the chain of additions (the terms of a round sit contiguously in LDS)
+template <typename R>
+__device__ __forceinline__ R ordered_sum(R acc, const R *s, int n, bool neg)
+{
+    int k = 0;
+    for (; k + 4 <= n; k += 4) {
+        R a = s[k], b = s[k + 1], c = s[k + 2], d = s[k + 3];
+        if (neg) { a = -a; b = -b; c = -c; d = -d; }
+        acc = acc + a;
+        acc = acc + b;
+        acc = acc + c;
+        acc = acc + d;
+    }
+    for (; k < n; k++) acc = acc + (neg ? -s[k] : s[k]);
+    return acc;
+}
+
 // density sums of the rows still iterating (DensitySpec::pair_kernel,
 // src/hydro/density_loop.h:1141-1385), the terms in k_density_cells' order
 template <int KERN, typename R, int TPB>
@@ -182,8 +220,7 @@ __global__ void __launch_bounds__(TPB) k_density_rows(DevParams P, GridView g, i
                                                      double *__restrict__ accout, BoxTable bt, int *__restrict__ npairs)
 {
     __shared__ RowStencil rs;
-    __shared__ R sc[13][TPB];
-    __shared__ int sf[TPB];
+    __shared__ R sc[13][TPB + 1];  // the accepted candidates' terms, in candidate order (padded: no bank conflicts)
     __shared__ unsigned long long smask[TPB / 64];
     const long r = xcd_block();
     if (r >= nrows) return;
@@ -223,7 +260,8 @@ __global__ void __launch_bounds__(TPB) k_density_rows(DevParams P, GridView g, i
     int accepted = 0;  // thread 13: the row's pairs (the pass's work, counted when npairs is given)
     for (int base = 0; base < total; base += TPB) {
         const int v = base + tid;
-        int f = 0;
+        bool ok = false;
+        R t[13];
         if (v < total) {
             const int j = row_candidate(rs, v);
             const double mj = M[j];
@@ -246,49 +284,37 @@ __global__ void __launch_bounds__(TPB) k_density_rows(DevParams P, GridView g, i
                     const R u = rr * hinv;
                     R wk = 0, dwk = 0;
                     kernel_main<KERN, 0>(u, hinv3, hinv4, wk, dwk);
-                    sc[0][tid] = wk;
-                    sc[1][tid] = (R)mj * wk;
-                    sc[2][tid] = -(R(3.0) * hinv * wk + u * dwk);
-                    f = 1;
+                    t[0] = wk;
+                    t[1] = (R)mj * wk;
+                    t[2] = -(R(3.0) * hinv * wk + u * dwk);
+                    ok = true;
+                    // the particle itself (rr = 0) adds to the first three sums only: its other
+                    // terms are +0, which leave a sum that starts at +0 unchanged bit for bit
+                    for (int q = 3; q < 13; q++) t[q] = 0;
                     if (rr > 0) {
-                        sc[4][tid] = wk * (dp0 * dp0);
-                        sc[5][tid] = wk * (dp1 * dp1);
-                        sc[6][tid] = wk * (dp2 * dp2);
-                        sc[7][tid] = wk * (dp0 * dp1);
-                        sc[8][tid] = wk * (dp1 * dp2);
-                        sc[9][tid] = wk * (dp0 * dp2);
-                        sc[10][tid] = wk * dp0;
-                        sc[11][tid] = wk * dp1;
-                        sc[12][tid] = wk * dp2;
+                        t[4] = wk * (dp0 * dp0);
+                        t[5] = wk * (dp1 * dp1);
+                        t[6] = wk * (dp2 * dp2);
+                        t[7] = wk * (dp0 * dp1);
+                        t[8] = wk * (dp1 * dp2);
+                        t[9] = wk * (dp0 * dp2);
+                        t[10] = wk * dp0;
+                        t[11] = wk * dp1;
+                        t[12] = wk * dp2;
                         const R dv0 = vxi - (R)VX[j], dv1 = vyi - (R)VY[j], dv2 = vzi - (R)VZ[j];
-                        sc[3][tid] = dwk * (dp0 * dv0 + dp1 * dv1 + dp2 * dv2) / rr;
-                        f = 2;
+                        t[3] = dwk * (dp0 * dv0 + dp1 * dv1 + dp2 * dv2) / rr;
                     }
                 }
             }
         }
-        // threads 0-12 add the accepted candidates in order, wavefront by wavefront
-        const unsigned long long mask = __ballot(f != 0);
-        sf[tid] = f;
-        if ((tid & 63) == 0) smask[tid >> 6] = mask;
+        // the accepted candidates' terms packed in candidate order; threads 0-12 then add them
+        int nok;
+        const int pos = flagged_rank<TPB>(ok, smask, nok);
+        if (ok)
+            for (int q = 0; q < 13; q++) sc[q][pos] = t[q];
         __syncthreads();
-        if (tid < 13) {
-            for (int w = 0; w < TPB / 64; w++) {
-                unsigned long long m = smask[w];
-                while (m) {
-                    const int k = w * 64 + __ffsll((long long)m) - 1;
-                    m &= m - 1;
-                    const int fk = sf[k];
-                    if (tid < 3) acc += sc[tid][k];
-                    else if (fk == 2) {
-                        if (tid == 3) acc -= sc[3][k];
-                        else acc += sc[tid][k];
-                    }
-                }
-            }
-        } else if (tid == 13 && npairs) {
-            for (int w = 0; w < TPB / 64; w++) accepted += __popcll(smask[w]);
-        }
+        if (tid < 13) acc = ordered_sum(acc, sc[tid], nok, tid == 3);
+        else if (tid == 13) accepted += nok;
         __syncthreads();
     }
     if (tid < 13) accout[(size_t)tid * nrows + r] = (double)acc;
@@ -707,8 +733,8 @@ __global__ void __launch_bounds__(TPB) k_gradients_rows(DevParams P, int nrows, 
                                                        const int *__restrict__ cnt, double *__restrict__ GR, double *__restrict__ GP,
                                                        double *__restrict__ GV, BoxTable bt)
 {
-    __shared__ R sd[5][TPB], sw[3][TPB], srr[TPB];
-    __shared__ int sf[TPB];
+    // the accepted entries' values packed in list order (padded rows: no bank conflicts)
+    __shared__ R sd[5][TPB + 1], sw[3][TPB + 1], srr[TPB + 1];
     __shared__ unsigned long long smask[TPB / 64];
     const long r = xcd_block();
     if (r >= nrows) return;
@@ -735,7 +761,8 @@ __global__ void __launch_bounds__(TPB) k_gradients_rows(DevParams P, int nrows, 
         const int c = cnt[r];
         for (int base = 0; base < c; base += TPB) {
             const int q = base + lane;
-            int f = 0;
+            bool ok = false;
+            R w3[3] = {0, 0, 0}, d5[5], rr = 0;
             if (q < c) {
                 const int j = ell[(size_t)q * ld + r];
                 const GradRec<R> J = src.H ? make_grad_rec<R>(j, src, X, Y, Z, bt) : rec[j];
@@ -745,51 +772,58 @@ __global__ void __launch_bounds__(TPB) k_gradients_rows(DevParams P, int nrows, 
                     const R r2 = dp[0] * dp[0] + dp[1] * dp[1] + dp[2] * dp[2];
                     const double r2d = pair_r2d<R>(dp[0], dp[1], dp[2], r2);
                     if ((r2 > 0) && !((r2d >= h2i) && (r2d >= (double)J.h * (double)J.h))) {
-                        const R rr = sqrt(r2);
+                        rr = sqrt(r2);
                         const double rd = std::is_same<R, double>::value ? 0.0 : sqrt(r2d);
                         R wk = 0, dwk = 0;
-                        f = 1;
+                        ok = true;
+                        // an entry beyond the row's own radius (inside the neighbour's) takes part
+                        // in the extrema only: its weights stay +0, and its products (+-0) leave
+                        // the gradient sums, which start at +0 and never reach -0, unchanged
                         if (std::is_same<R, double>::value ? (rr < hi) : (rd < hid)) {
                             pair_kernel<KERN, -1, R>(rr, rd, hid, wk, dwk);
-                            f = 2;
-                            for (int kk = 0; kk < 3; kk++) sw[kk][lane] = -wk * dp[kk];
+                            for (int kk = 0; kk < 3; kk++) w3[kk] = -wk * dp[kk];
                         }
-                        srr[lane] = rr;
-                        sd[0][lane] = (R)(J.rho - I.rho);
-                        sd[1][lane] = (R)(J.p - I.p);
-                        sd[2][lane] = (R)(J.v0 - I.v0);
-                        sd[3][lane] = (R)(J.v1 - I.v1);
-                        sd[4][lane] = (R)(J.v2 - I.v2);
+                        d5[0] = (R)(J.rho - I.rho);
+                        d5[1] = (R)(J.p - I.p);
+                        d5[2] = (R)(J.v0 - I.v0);
+                        d5[3] = (R)(J.v1 - I.v1);
+                        d5[4] = (R)(J.v2 - I.v2);
                     }
                 }
             }
-            // threads 0-14 keep G[kk][b] (kk = lane/5, b = lane%5), 15-19 the minima, 20-24 the
-            // maxima, 25 the largest distance (with mn <= 0 <= mx the reference's else-if is two
-            // ifs); entries in order, wavefront by wavefront
-            const unsigned long long mask = __ballot(f != 0);
-            sf[lane] = f;
-            if ((lane & 63) == 0) smask[lane >> 6] = mask;
+            // the accepted entries packed in list order; threads 0-14 then keep G[kk][b]
+            // (kk = lane/5, b = lane%5), 15-19 the minima, 20-24 the maxima, 25 the largest
+            // distance (with mn <= 0 <= mx the reference's else-if is two ifs)
+            int nok;
+            const int pos = flagged_rank<TPB>(ok, smask, nok);
+            if (ok) {
+                for (int kk = 0; kk < 3; kk++) sw[kk][pos] = w3[kk];
+                for (int b = 0; b < 5; b++) sd[b][pos] = d5[b];
+                srr[pos] = rr;
+            }
             __syncthreads();
-            if (lane < 26) {
-                for (int w = 0; w < TPB / 64; w++) {
-                unsigned long long m = smask[w];
-                while (m) {
-                    const int k = w * 64 + __ffsll((long long)m) - 1;
-                    m &= m - 1;
-                    if (lane < 15) {
-                        if (sf[k] == 2) acc += rounded(sw[lane / 5][k] * sd[lane % 5][k]);
-                    } else if (lane < 20) {
-                        const R d = sd[lane - 15][k];
-                        if (d < acc) acc = d;
-                    } else if (lane < 25) {
-                        const R d = sd[lane - 20][k];
-                        if (d > acc) acc = d;
-                    } else {
-                        const R rr = srr[k];
-                        if (rr > acc) acc = rr;
-                    }
+            if (lane < 15) {
+                const R *a = sw[lane / 5], *b = sd[lane % 5];
+                int k = 0;
+                for (; k + 4 <= nok; k += 4) {
+                    const R p0 = rounded(a[k] * b[k]), p1 = rounded(a[k + 1] * b[k + 1]);
+                    const R p2 = rounded(a[k + 2] * b[k + 2]), p3 = rounded(a[k + 3] * b[k + 3]);
+                    acc = acc + p0;
+                    acc = acc + p1;
+                    acc = acc + p2;
+                    acc = acc + p3;
                 }
+                for (; k < nok; k++) acc = acc + rounded(a[k] * b[k]);
+            } else if (lane < 26) {
+                // minima (15-19, as maxima of the negated values: exact), maxima, distance
+                const R *a = lane < 25 ? sd[(lane - 15) % 5] : srr;
+                const bool mn = lane < 20;
+                R m = mn ? -acc : acc;
+                for (int k = 0; k < nok; k++) {
+                    const R d = mn ? -a[k] : a[k];
+                    if (d > m) m = d;
                 }
+                acc = mn ? -m : m;
             }
             __syncthreads();
         }
@@ -817,7 +851,7 @@ __global__ void __launch_bounds__(TPB) k_force_rows(DevParams P, Gam<R> G, int n
                                                    double *__restrict__ MSV, int *__restrict__ err, int *__restrict__ need_wakeup,
                                                    unsigned long long *__restrict__ pairs, BoxTable bt, const int *__restrict__ gate)
 {
-    __shared__ R sv[5][TPB];
+    __shared__ R sv[5][TPB + 1];  // the entries' fluxes packed in list order (padded: no bank conflicts)
     __shared__ unsigned long long smask[TPB / 64];
     const long r = xcd_block();
     if (r >= nrows) return;
@@ -846,13 +880,13 @@ __global__ void __launch_bounds__(TPB) k_force_rows(DevParams P, Gam<R> G, int n
         for (int base = 0; base < c; base += TPB) {
             const int q = base + lane;
             int rc = 0;
+            R f0, f1, f2, fp, vts;
             if (q < c) {
                 const int j = ell[(size_t)q * ld + r];
                 const Cell<R> J = rec[j];
                 R dp0, dp1, dp2;
                 record_dp<R>(P, I, J, xi, yi, zi, F.X, F.Y, F.Z, j, dp0, dp1, dp2);
                 const R r2 = dp0 * dp0 + dp1 * dp1 + dp2 * dp2;
-                R f0, f1, f2, fp, vts;
                 unsigned rung;
                 rc = force_pair<KERN, R>(P, G, I, J, dp0, dp1, dp2, r2, f0, f1, f2, fp, vts, rung);
                 if constexpr (STATS) count_rungs(F.rungs, rung);
@@ -861,7 +895,6 @@ __global__ void __launch_bounds__(TPB) k_force_rows(DevParams P, Gam<R> G, int n
                     if (atomicCAS(err + 10, -1, i) == -1) err[11] = j;
                 }
                 if (rc > 0) {
-                    sv[0][lane] = f0; sv[1][lane] = f1; sv[2][lane] = f2; sv[3][lane] = fp; sv[4][lane] = vts;
                     if (active && !active[j] && wakeup) {
                         if ((double)vts > kWakeup * msv_in[j]) {
                             atomicMax(&wakeup[j], tbi + 1);
@@ -871,19 +904,27 @@ __global__ void __launch_bounds__(TPB) k_force_rows(DevParams P, Gam<R> G, int n
                     }
                 }
             }
-            const unsigned long long mask = __ballot(rc > 0);
-            if ((lane & 63) == 0) smask[lane >> 6] = mask;
+            // the entries with a flux packed in list order; lanes 0-4 then take them in turn
+            int nok;
+            const int pos = flagged_rank<TPB>(rc > 0, smask, nok);
+            if (rc > 0) {
+                sv[0][pos] = f0; sv[1][pos] = f1; sv[2][pos] = f2; sv[3][pos] = fp; sv[4][pos] = vts;
+            }
             __syncthreads();
-            if (lane < 5) {
-                for (int w = 0; w < TPB / 64; w++) {
-                    unsigned long long m = smask[w];
-                    while (m) {
-                        const int k = w * 64 + __ffsll((long long)m) - 1;
-                        m &= m - 1;
-                        if (lane < 4) sum.add(sv[lane][k]);
-                        else if (sv[4][k] > acc) acc = sv[4][k];
-                    }
+            if (lane < 4) {
+                const R *a = sv[lane];
+                int k = 0;
+                for (; k + 4 <= nok; k += 4) {
+                    const R x0 = a[k], x1 = a[k + 1], x2 = a[k + 2], x3 = a[k + 3];
+                    sum.add(x0);
+                    sum.add(x1);
+                    sum.add(x2);
+                    sum.add(x3);
                 }
+                for (; k < nok; k++) sum.add(a[k]);
+            } else if (lane == 4) {
+                for (int k = 0; k < nok; k++)
+                    if (sv[4][k] > acc) acc = sv[4][k];
             }
             __syncthreads();
         }
