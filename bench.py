@@ -1,6 +1,7 @@
 """Benchmark of the hydro hot path on one MI355X.
 
     python bench.py [--gpus 1] [--steps 50] [--warmup 2] [--n-side 128] [--no-cpu] [--no-full-pass]
+                    [--no-serve] [--no-tune] [--no-legs] [--workload sedov|ensemble|lognormal]
 
 Workload (the 2M-particle Sedov job): a 128^3 jittered lattice in a periodic unit box, cold
 medium rho = 1, P = 1e-6, gamma 5/3, E = 1 deposited kernel-weighted into the 64 central
@@ -25,7 +26,14 @@ FORCE_MODEL below).  With a few hundred active rows per step these launches are 
 latency-bound.  `roofline_force` is the force kernel over the same job whatever dominates.
 `full_pass` adds one all-active pass of the hot path over the 2M particles (the kernels at
 full occupancy) and `roofline_force_full_pass` the force kernel there.
-The serving loop (`serving`, p50/p99 job latency) and the CPU baseline follow.
+Before each form's timed job its launch options (few_rows, row_schedule, speculate: shapes
+only, never results) are autotuned on the job's first 10 sync points (`autotune`,
+gizmo_amd/tune.py; --no-tune keeps the library's defaults).
+The serving loop (`serving`, p50/p99 job latency), compact legs of the other two workloads and
+the CPU baseline follow: `ensemble` (48 boxes of the 256-box mix, 30 steps: batched FP32-mixed,
+batched FP64 and looped FP64, so that batching and arithmetic form are reported apart) and
+`lognormal` (1M particles of the lognormal field, one pass per form and the FP32-mixed errors
+against FP64); --no-legs skips them, --workload ensemble|lognormal runs them at full size.
 The CPU baseline is the FP64 oracle (C + OpenMP) running the first steps of the same job on
 this host's cores.
 """
@@ -77,22 +85,38 @@ def physical_cores():
         return os.cpu_count()
 
 
-def new_job(ic, precision):
+def new_hydro(ic, precision, options=None):
     from gizmo_amd.hydro import Hydro
     g = Hydro(dict(ic["params"]), len(ic["mass"]), precision)
+    for k, v in (options or {}).items():
+        g.set_option(k, v)
     g.load(dict(pos=ic["pos"], vel=ic["vel"], mass=ic["mass"], u=ic["u"], h=ic["h"]))
+    return g
+
+
+def new_job(ic, precision, options=None):
+    g = new_hydro(ic, precision, options)
     g.run_begin(INTEG)
     return g
 
 
-def run_job(ic, precision, steps, warmup):
+def tune_job(ic, precision, steps=10):
+    """The job's launch options (few_rows, row_schedule, speculate) autotuned on its first
+    `steps` sync points (gizmo_amd/tune.py), each candidate a fresh job."""
+    from gizmo_amd.tune import SINGLE_BOX_SPACE, job_sample, tune
+    best, table = tune(job_sample(lambda: new_hydro(ic, precision), INTEG, steps), SINGLE_BOX_SPACE, reps=2, log=log)
+    return {"options": best, "sample": f"sync points 1..{steps} of the same job, best of 2",
+            "table_ms": [[o, round(1e3 * t, 3)] for o, t in table]}
+
+
+def run_job(ic, precision, steps, warmup, options=None):
     """The timed job (no events on the stream), then the same job again with HIP events around
     every launch family for the per-kernel times and the roofline."""
     if warmup > 0:
-        g = new_job(ic, precision)
+        g = new_job(ic, precision, options)
         g.run_steps(warmup)
         g.close()
-    g = new_job(ic, precision)
+    g = new_job(ic, precision, options)
     g.synchronize()
     g.timing_reset()
     t0 = time.perf_counter()
@@ -102,7 +126,7 @@ def run_job(ic, precision, steps, warmup):
     res = dict(seconds=dt, active=active, woken=woken, resorts=g.stat("resorts"), updates=g.stat("active_updates"),
                tl=g.timeline())
     g.close()
-    g = new_job(ic, precision)
+    g = new_job(ic, precision, options)
     g.synchronize()
     g.timing_enable(True)
     g.timing_reset()
@@ -241,34 +265,61 @@ def ensemble_boxes(n_boxes=256, seed=1004):
     return draw_jobs(n_boxes, seed, n_min=4096, n_max=262144, steps=(100, 100))
 
 
-def ensemble_bench(steps=100, n_boxes=256, seed=1004):
-    """The ensemble: every box started together in one batch (FP32-mixed) and advanced `steps`
-    sync points, against looping the plain FP64 form box by box (one context, begin and
-    `steps` steps per box).  Both time the steps only; the start of each box is untimed."""
+def ensemble_bench(steps=100, n_boxes=256, seed=1004, tune_sample=(12, 5)):
+    """The ensemble: every box started together in one batch and advanced `steps` sync points,
+    in the FP32-mixed and the plain FP64 form, against looping the plain FP64 form box by box
+    (one context, begin and `steps` steps per box): batched FP64 over looped FP64 is the gain
+    of batching alone, batched mixed over batched FP64 that of the arithmetic form.  The
+    batch's launch shape (few_rows) is autotuned first on a sample (the first boxes, a few
+    steps, gizmo_amd/tune.py).  Every leg times the steps only; the start of each box is
+    untimed."""
     from gizmo_amd.batch import Batch
     from gizmo_amd.hydro import GZ_FP32_MIXED, GZ_FP64, Hydro
     from gizmo_amd.serve import JOB_INTEG, job_params
+    from gizmo_amd.tune import BATCH_SPACE, batch_sample, tune
     boxes = ensemble_boxes(n_boxes, seed)
     fields = [j.ic() for j in boxes]
     total = sum(j.n for j in boxes)
     log(f"ensemble: {n_boxes} boxes, {total} particles")
-    b = Batch(job_params(), total, n_boxes, GZ_FP32_MIXED)
-    for f in fields:
-        b.add(f, job_params(), JOB_INTEG)
-    t0 = time.perf_counter()
-    b.begin()
-    b.synchronize()
-    t_begin = time.perf_counter() - t0
-    u0 = b.stat("active_updates")
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        b.step()
-    b.synchronize()
-    t_b = time.perf_counter() - t0
-    upd_b = b.stat("active_updates") - u0
-    dev_bytes = b.stat("device_bytes")
-    b.close()
-    log(f"ensemble batched mixed: {t_b:.2f} s, {upd_b / t_b / 1e6:.1f} M updates/s (start {t_begin:.2f} s)")
+
+    def make_batch(prec, flist):
+        def make():
+            b = Batch(job_params(), sum(len(f["mass"]) for f in flist), len(flist), prec)
+            for f in flist:
+                b.add(f, job_params(), JOB_INTEG)
+            return b
+        return make
+
+    tuned = {}
+    if tune_sample:
+        nb, ns = tune_sample
+        best, table = tune(batch_sample(make_batch(GZ_FP32_MIXED, fields[:nb]), ns), BATCH_SPACE, reps=1, log=log)
+        tuned = {"options": best, "sample": f"first {nb} boxes, {ns} steps, FP32-mixed",
+                 "table_ms": [[o, round(1e3 * t, 3)] for o, t in table]}
+
+    def batched(prec, name):
+        b = make_batch(prec, fields)()
+        for k, v in tuned.get("options", {}).items():
+            b.set_option(k, v)
+        t0 = time.perf_counter()
+        b.begin()
+        b.synchronize()
+        t_begin = time.perf_counter() - t0
+        u0 = b.stat("active_updates")
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            b.step()
+        b.synchronize()
+        t_b = time.perf_counter() - t0
+        upd = b.stat("active_updates") - u0
+        dev_bytes = b.stat("device_bytes")
+        b.close()
+        log(f"ensemble batched {name}: {t_b:.2f} s, {upd / t_b / 1e6:.1f} M updates/s (start {t_begin:.2f} s)")
+        return {"seconds": round(t_b, 3), "updates": int(upd), "updates_per_s": round(upd / t_b, 1),
+                "start_seconds": round(t_begin, 3), "device_bytes": int(dev_bytes)}
+
+    bm = batched(GZ_FP32_MIXED, "mixed")
+    bf = batched(GZ_FP64, "fp64")
     t_l, upd_l = 0.0, 0.0
     for k, f in enumerate(fields):
         g = Hydro(job_params(), len(f["mass"]), GZ_FP64)
@@ -283,11 +334,12 @@ def ensemble_bench(steps=100, n_boxes=256, seed=1004):
         upd_l += g.stat("active_updates") - u0
         g.close()
     log(f"ensemble looped fp64: {t_l:.2f} s, {upd_l / t_l / 1e6:.1f} M updates/s")
-    return {"boxes": n_boxes, "particles": total, "steps": steps,
-            "batched_mixed": {"seconds": round(t_b, 3), "updates": int(upd_b), "updates_per_s": round(upd_b / t_b, 1),
-                              "start_seconds": round(t_begin, 3), "device_bytes": int(dev_bytes)},
-            "looped_fp64": {"seconds": round(t_l, 3), "updates": int(upd_l), "updates_per_s": round(upd_l / t_l, 1)},
-            "batched_over_looped": round((upd_b / t_b) / (upd_l / t_l), 3)}
+    lf = {"seconds": round(t_l, 3), "updates": int(upd_l), "updates_per_s": round(upd_l / t_l, 1)}
+    return {"boxes": n_boxes, "particles": total, "steps": steps, "autotune": tuned,
+            "batched_mixed": bm, "batched_fp64": bf, "looped_fp64": lf,
+            "batched_fp64_over_looped_fp64": round(bf["updates_per_s"] / lf["updates_per_s"], 3),
+            "batched_mixed_over_batched_fp64": round(bm["updates_per_s"] / bf["updates_per_s"], 3),
+            "batched_mixed_over_looped_fp64": round(bm["updates_per_s"] / lf["updates_per_s"], 3)}
 
 
 LOGNORMAL_QUANTITIES = {"h": "h", "rho": "rho", "grad_rho": "grad_rho", "grad_p": "grad_p", "grad_v": "grad_v",
@@ -337,7 +389,8 @@ def lognormal_bench(n_particles=4 * 1024 * 1024, ng=256, reps=2):
             dts.append(time.perf_counter() - t1)
         dt = min(dts)
         rate[form] = dict(ms=round(1e3 * dt, 2), updates_per_s=round(n_particles / dt, 1),
-                          density_iterations=g.stat("density_iterations"))
+                          density_iterations=g.stat("density_iterations"),
+                          force_rare_rows_all_passes=g.stat("force_rare_rows"))
         out[form] = g.store(names)
         g.close()
         log(f"lognormal {form}: {1e3 * dt:.1f} ms per pass, {n_particles / dt / 1e6:.1f} M updates/s")
@@ -392,6 +445,8 @@ def main():
     ap.add_argument("--only", choices=["both", "fp64", "mixed"], default="both")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving-loop latency runs")
     ap.add_argument("--serve-jobs", type=int, default=48)
+    ap.add_argument("--no-tune", action="store_true", help="the library's default launch options, no autotuning")
+    ap.add_argument("--no-legs", action="store_true", help="skip the compact ensemble and lognormal legs")
     ap.add_argument("--workload", choices=["sedov", "ensemble", "lognormal"], default="sedov",
                     help="sedov: the 2M Sedov job (+ serving); ensemble: 256 boxes batched vs looped; "
                          "lognormal: 4M lognormal field, FP32-mixed against FP64")
@@ -436,10 +491,13 @@ def main():
 
     forms = {"fp64": GZ_FP64, "mixed": GZ_FP32_MIXED}
     todo = ["fp64", "mixed"] if a.only == "both" else [a.only]
-    res, fp = {}, {}
+    res, fp, tuned = {}, {}, {}
     for f in todo:
+        if not a.no_tune:
+            tuned[f] = tune_job(ic, forms[f])
+            log(f"{f}: tuned options {tuned[f]['options']}")
         log(f"{f}: {a.steps} steps ...")
-        res[f] = run_job(ic, forms[f], a.steps, a.warmup)
+        res[f] = run_job(ic, forms[f], a.steps, a.warmup, tuned.get(f, {}).get("options"))
         r = res[f]
         log(f"{f}: {1e3 * r['seconds']:.1f} ms, {sum(r['active'])} updates, resorts {r['resorts']:.0f}, "
             f"kernels {{{', '.join(f'{k}: {v[1]:.1f}' for k, v in r['kernels'].items())}}} ms")
@@ -483,6 +541,8 @@ def main():
     for f in res:
         if f != head:
             out[f] = dict(summary(res[f]), roofline=roofline_dominant(res[f], f), roofline_force=roofline(res[f], f))
+    if tuned:
+        out["autotune"] = tuned
     if "fp64" in res and "mixed" in res:
         out["mixed_over_fp64"] = round(summary(res["mixed"])["value"] / summary(res["fp64"])["value"], 3)
     if fp:
@@ -502,6 +562,10 @@ def main():
         out["p50_ms"] = round(1e3 * sv["p50_s"], 1)
         out["p99_ms"] = round(1e3 * sv["p99_s"], 1)
         out["latency_form"] = "mixed" if "mixed" in serving else head
+    if not a.no_legs:
+        # compact legs of the other two workloads, timed by the same run
+        out["ensemble"] = ensemble_bench(steps=30, n_boxes=48, tune_sample=None if a.no_tune else (8, 5))
+        out["lognormal"] = lognormal_bench(n_particles=1 << 20, ng=128)
     if not a.no_cpu:
         log("cpu baseline ...")
         out["cpu_baseline"] = cpu_baseline(ic, a.steps)

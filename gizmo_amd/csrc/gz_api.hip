@@ -34,7 +34,7 @@ inline unsigned nblk(long n) { return (unsigned)((n + kBlock - 1) / kBlock); }
 // steps with more active particles than this drift every particle (the cells around that many
 // rows cover most of a box: one sweep costs less than walking them)
 constexpr long long kLazyMaxRows = 65536;
-// counters[kPassGate]: k_pass_gate's verdict on a speculative first density round
+// counters[kPassGate]: the gate word of a speculative first density round (written by the round's closing launch)
 constexpr int kPassGate = 15;
 
 }  // namespace
@@ -73,6 +73,13 @@ struct gz_context {
     // waiting for it (speculate: allowed; spec: in flight, settled by the force pass's read)
     bool speculate = false, spec = false;
     bool spec_enabled = true;  // option "speculate"
+    bool fd = false;           // option "fd_correction": the first-derivative volume correction of the density
+    int force_split = -1;      // option "force_split": -1 by the last pass's rare share, 0 one kernel, 1 two kernels
+    double rare_share = 0;     // share of the rows the last two-kernel force pass left to k_force_rare
+    bool split_ran = false;
+    bool force_flags_clear = false;  // the last gradient launch cleared the force pass's flags
+    bool in_run_steps = false;       // inside gz_run_steps' loop
+    bool prepped = false;            // the last step's closing kick cleared this step's counters
     double resort_fraction = 0.125;  // order again once dmax exceeds this fraction of the search radius
     int few_rows = 8192;             // at most this many active rows: one wavefront per row
     // the cell ranges and stencils depend only on the order (key_version) and the level: they
@@ -134,6 +141,10 @@ struct gz_context {
     long ld_d = 0;
     int *ell_s = nullptr, *cnt_s = nullptr;  // symmetric
     size_t ell_s_alloc = 0;                  // entries allocated at ell_s (at least cap_s * ld_s)
+    // the lazy drift of the rows' stencils: (time, active list, grid level, order) of the last
+    // one, and whether the listing counter is known to be clear
+    long long stencil_drifted[4] = {-1, -1, -1, -1};
+    bool ncells_clear = false;
     int cap_s = 0;
     long ld_s = 0;
     bool sym_valid = false;
@@ -320,6 +331,7 @@ int set_params(gz_context *ctx, const gz_params *p)
     d.max_kernel_radius = p->max_kernel_radius;
     d.time = p->time;
     d.time_begin = p->time_begin;
+    d.fd4alpha = ctx->fd ? 4.0 * fd_alpha(p->kernel) : 0.0;
     return 0;
 }
 
@@ -370,7 +382,7 @@ int alloc_state(gz_context *ctx, long cap)
     CHK(dalloc(ctx, &ctx->cnt_d, n)); CHK(dalloc(ctx, &ctx->cnt_s, n));
     CHK(dalloc(ctx, &ctx->cells, n)); CHK(dalloc(ctx, &ctx->cflag, n + 1)); CHK(dalloc(ctx, &ctx->cscan, n + 1));
     CHK(dalloc(ctx, &ctx->need, n));
-    CHK(dalloc(ctx, &ctx->dacc, 13 * n));
+    CHK(dalloc(ctx, &ctx->dacc, kDensSumsFD * n));
     static_assert(sizeof(GradRec<float>) <= sizeof(GradRec<double>) && sizeof(Cell<float>) <= sizeof(Cell<double>),
                   "the record buffers hold either form");
     CHK(dalloc(ctx, (char **)&ctx->rec_grad, n * sizeof(GradRec<double>)));
@@ -557,9 +569,17 @@ ParticleView particle_view(gz_context *c);
 int drift_stencil(gz_context *ctx)
 {
     if (!ctx->lazy || ctx->nact == 0) return 0;
+    // the same rows on the same grid at the same time: their cells were drifted already (the
+    // symmetric list's walk after the density rounds on an unchanged grid)
+    const long long mark[4] = {ctx->ti_current, (long long)ctx->act_version, (long long)ctx->level, (long long)ctx->key_version};
+    if (std::equal(mark, mark + 4, ctx->stencil_drifted)) return 0;
+    std::copy(mark, mark + 4, ctx->stencil_drifted);
     Timed t(ctx, "drift");
-    int *ncells = (int *)(ctx->u64c + 76);  // the cells listed (cleared here, counted by k_stencil_cells)
-    HIPCHK(hipMemsetAsync(ncells, 0, sizeof(int), ctx->stream));
+    // the cells listed (counted by k_stencil_cells): cleared by the step's k_step_prep for the
+    // step's first listing, here for any other
+    int *ncells = (int *)(ctx->u64c + 76);
+    if (!ctx->ncells_clear) HIPCHK(hipMemsetAsync(ncells, 0, sizeof(int), ctx->stream));
+    ctx->ncells_clear = false;
     const long pairs = 27L * ctx->nact;
     k_stencil_cells<<<nblk(pairs), kBlock, 0, ctx->stream>>>(grid_view(ctx), ctx->nact, ctx->act, ctx->key, ctx->cmark, ctx->dcells,
                                                              ncells);
@@ -849,7 +869,7 @@ int density_stage(gz_context *ctx)
     const bool few = wide || na <= ctx->few_rows;
     if (!few) CHK(build_cell_rows(ctx));
     // a step of the run loop with few rows may launch the passes behind the first round before
-    // its counters are read (they are gated on k_pass_gate); the radius bound the symmetric
+    // its counters are read (they are gated on the round's gate word); the radius bound the symmetric
     // list's grid will take must then be known
     const bool speculate = ctx->speculate && few && !wide && ready && ctx->hmax_bound > 0;
     return density_rounds<KERN, R>(ctx, 0, speculate);
@@ -857,7 +877,7 @@ int density_stage(gz_context *ctx)
 
 // The density rounds from round `iters` on (0: the first; iters > 0: resuming after the
 // counters of round iters were read back by a later read of the step).  speculate: the first
-// round is launched with k_pass_gate behind it and not waited for; the passes follow, and the
+// round's closing launch writes a gate word and is not waited for; the passes follow, and the
 // force pass's read of the counters settles the step (force_stage).
 template <int KERN, typename R>
 int density_rounds(gz_context *ctx, int iters, bool speculate)
@@ -895,16 +915,18 @@ int density_rounds(gz_context *ctx, int iters, bool speculate)
                         ctx->M, ctx->it_h, ctx->dacc);
             }
             if (npairs) k_sum_counts<<<std::min(nblk(na), 256u), kBlock, 0, ctx->stream>>>(na, npairs, ctx->u64c + 77);
+            // the speculative first round: its closing launch also writes the pass's gate word
+            const bool gate_now = speculate && iters == 0;
             {
                 Timed t(ctx, "density_after");
                 k_density_after<<<nblk(na), kBlock, 0, ctx->stream>>>(ctx->dp, na, ctx->act, ctx->need, ctx->dacc, ctx->M, ctx->U,
                                                                       wide ? 1e300 : ctx->cell - 2.0 * ctx->dmax, it, out, ctx->counters + 0,
-                                                                      ctx->counters + 1, ctx->counters + 2, ctx->u64c + 68, BoxTable{});
+                                                                      ctx->counters + 1, ctx->counters + 2, ctx->u64c + 68, BoxTable{},
+                                                                      ctx->hmax_bound, gate_now ? ctx->counters + kPassGate : nullptr,
+                                                                      (unsigned int *)(ctx->u64c + 79));
             }
             HIPCHK(hipGetLastError());
-            if (speculate && iters == 0) {
-                k_pass_gate<<<1, 64, 0, ctx->stream>>>(ctx->counters, ctx->u64c + 68, ctx->hmax_bound, ctx->counters + kPassGate);
-                HIPCHK(hipGetLastError());
+            if (gate_now) {
                 ctx->spec = true;
                 ctx->st.density_iterations = 1;
                 ctx->density_done = true;
@@ -916,7 +938,7 @@ int density_rounds(gz_context *ctx, int iters, bool speculate)
             iters++;
             remaining = ctx->hcounters[0];
         }
-        // resuming: the round's stop code was settled by k_pass_gate (counters[2] then belongs
+        // resuming: the round's stop code was settled by its gate word (counters[2] then belongs
         // to the force pass)
         const int rebuild = ctx->hcounters[1], err = read ? 0 : ctx->hcounters[2];
         read = false;
@@ -1083,11 +1105,13 @@ int gradients_stage(gz_context *ctx)
                                                                      rec, ctx->X, ctx->Y, ctx->Z, ctx->par.box_size, BoxTable{});
     }
     Timed t(ctx, "gradients");
+    // with its records made in the row kernels too, the force pass's flags are cleared here
+    ctx->force_flags_clear = make;
     if (ctx->nact <= ctx->few_rows)
         k_gradients_rows<KERN, R><<<xcd_grid(ctx->nact), kListThreads, 0, ctx->stream>>>(
             ctx->dp, ctx->nact, ctx->act, ctx->X, ctx->Y, ctx->Z, ctx->H, rec,
             make ? GradSrc{ctx->H, ctx->M, ctx->RHO, ctx->PR, ctx->VX, ctx->VY, ctx->VZ, ctx->par.box_size} : GradSrc{}, ctx->T, ctx->CN,
-            ctx->n, ctx->ell_s, ctx->ld_s, ctx->cnt_s, ctx->GR, ctx->GP, ctx->GV, BoxTable{});
+            ctx->n, ctx->ell_s, ctx->ld_s, ctx->cnt_s, ctx->GR, ctx->GP, ctx->GV, BoxTable{}, make ? ctx->counters + 2 : nullptr);
     else
         k_gradients<KERN, R, false><<<xcd_grid(nblk(ctx->nact)), kBlock, 0, ctx->stream>>>(ctx->dp, ctx->nact, ctx->act, ctx->X, ctx->Y, ctx->Z, ctx->H,
                                                                                     rec, ctx->T, ctx->CN, ctx->n, ctx->ell_s, ctx->ld_s,
@@ -1107,7 +1131,12 @@ int force_stage(gz_context *ctx)
                 ctx->FCE, ctx->DRK, ctx->GR, ctx->GP, ctx->GV, ctx->T, ctx->rungs};
     {
         Cell<R> *rec = (Cell<R> *)ctx->rec_force;
-        {
+        // few rows among many particles: the row kernel makes the records of its row and list
+        // entries (no packing launch) when the gradient launch cleared the pass's flags
+        const bool make = ctx->nact <= kMakeForceRows && ctx->nact <= ctx->few_rows && ctx->pack_list && ctx->force_flags_clear &&
+                          !ctx->rung_stats;
+        ctx->force_flags_clear = false;
+        if (!make) {
             Timed t(ctx, "pack_force");
             if (ctx->pack_list)
                 k_pack_force_list<R><<<xcd_grid(ctx->nact), 64, 0, ctx->stream>>>(ctx->nact, ctx->act, ctx->ell_s, ctx->ld_s, ctx->cnt_s,
@@ -1122,7 +1151,9 @@ int force_stage(gz_context *ctx)
         const int *gate = ctx->spec ? ctx->counters + kPassGate : nullptr;
         if (ctx->nact <= ctx->few_rows) {
             // the kernel that counts the faces' Riemann rungs when asked (the same code otherwise)
-            auto rows_kernel = ctx->rung_stats ? k_force_rows<KERN, R, true> : k_force_rows<KERN, R, false>;
+            auto rows_kernel = ctx->rung_stats ? k_force_rows<KERN, R, true>
+                               : make          ? k_force_rows<KERN, R, false, kListThreads, true>
+                                               : k_force_rows<KERN, R, false>;
             rows_kernel<<<xcd_grid(ctx->nact), kListThreads, 0, ctx->stream>>>(
                 ctx->dp, gam<R>(ctx->dp), ctx->nact, ctx->act, F, rec, ctx->ell_s, ctx->ld_s, ctx->cnt_s,
                 ctx->all_active ? nullptr : ctx->ACT, ctx->TB, ctx->MSV, ctx->WAKE, ctx->ACC, ctx->DTU, ctx->MSV, ctx->counters + 2,
@@ -1133,8 +1164,17 @@ int force_stage(gz_context *ctx)
                               ctx->MSV, ctx->WAKE, ctx->ACC, ctx->DTU, ctx->MSV, ctx->counters + 2, ctx->counters + 4, nullptr,
                               ctx->sched_valid ? ctx->sched : nullptr, ctx->rare};
             const dim3 grid = xcd_grid(nblk(ctx->nact));
+            // one kernel with the whole ladder when most rows took a later rung in the last pass
+            // (the two kernels then evaluate most rows twice): by the option "force_split", or
+            // on its default by the last pass's share of such rows in the FP32 form (where the
+            // whole ladder costs few registers more; the FP64 whole-ladder kernel spills)
+            const bool single = ctx->force_split == 0 ||
+                                (ctx->force_split < 0 && std::is_same<R, float>::value && ctx->rare_share > kRareShareSingle);
+            ctx->split_ran = !ctx->rung_stats && !single;
             if (ctx->rung_stats) {
                 k_force<KERN, R, false, true, false><<<grid, kBlock, 0, ctx->stream>>>(ctx->dp, gam<R>(ctx->dp), F, rec, a, BoxTable{});
+            } else if (single) {
+                k_force<KERN, R, false, false, false><<<grid, kBlock, 0, ctx->stream>>>(ctx->dp, gam<R>(ctx->dp), F, rec, a, BoxTable{});
             } else {
                 // the rows whose faces all take the Riemann ladder's common case, then the others
                 // with the whole ladder
@@ -1150,6 +1190,11 @@ int force_stage(gz_context *ctx)
         HIPCHK(hipGetLastError());
     }
     CHK(read_counters(ctx));
+    if (ctx->split_ran) {
+        // rows the common kernel left to k_force_rare (err[12] = counters[14])
+        ctx->rare_share = (double)ctx->hcounters[14] / (double)std::max(ctx->nact, 1);
+        ctx->split_ran = false;
+    }
     if (ctx->spec) {
         ctx->spec = false;
         const int g = ctx->hcounters[kPassGate];
@@ -1430,7 +1475,11 @@ int run_step(gz_context *ctx)
     // make_list_of_active_particles (:997-1009) with the softening cache
     {
         Timed t(ctx, "timeline");
-        k_step_prep<<<1, 64, 0, ctx->stream>>>(ctx->counters, ctx->u64c);
+        // the step's counters: cleared by the last step's closing kick when it ran in the same
+        // run of steps, else here
+        if (!ctx->prepped) k_step_prep<<<1, 64, 0, ctx->stream>>>(ctx->counters, ctx->u64c);
+        ctx->prepped = false;
+        ctx->ncells_clear = true;
         // the rows' new steps with their first half-kicks (one launch)
         if (ctx->nact > 0)
             k_find_timesteps<<<nblk(ctx->nact), kBlock, 0, ctx->stream>>>(T, S, ctx->nact, ctx->act, ctx->counters + 6, ctx->u64c,
@@ -1441,7 +1490,6 @@ int run_step(gz_context *ctx)
             k_wakeup_limits<<<1, 64, 0, ctx->stream>>>(T, ctx->u64c, ctx->counters + 8, BoxTable{});
             k_wakeups<<<nblk(n), kBlock, 0, ctx->stream>>>(T, S, ctx->counters + 8, wakeup_bin_offset, ctx->u64c, ctx->counters + 5, BoxTable{});
         }
-        k_next_sync<<<1, 64, 0, ctx->stream>>>(T, ctx->u64c, ctx->u64c + 70, BoxTable{});
     }
     // the drift of every particle to the next sync point fused with the row flags and counts
     // per block of particles of make_list_of_active_particles; then the list in ascending order
@@ -1453,9 +1501,10 @@ int run_step(gz_context *ctx)
     }
     {
         Timed t(ctx, "drift");
+        // every block takes the next sync point from the occupation itself (no k_next_sync launch)
         k_drift_active<<<nb_act, 256, 0, ctx->stream>>>(T, S, ti_prev, ctx->u64c + 70, ctx->u64c + 64,
                                                          ctx->u64c + 66, 0.99 * ctx->par.max_kernel_radius, ctx->ACT, ctx->act_bits,
-                                                         ctx->act_bcount, ctx->u64c + 67);
+                                                         ctx->act_bcount, ctx->u64c + 67, ctx->u64c);
         HIPCHK(hipGetLastError());
     }
     {
@@ -1525,8 +1574,13 @@ int run_step(gz_context *ctx)
     ctx->lazy = false;
     if (ctx->nact > 0) {
         Timed t(ctx, "timeline");
-        k_halfstep_kick<<<nblk(ctx->nact), kBlock, 0, ctx->stream>>>(T, S, ctx->nact, ctx->act, 1, BoxTable{});
+        // within a run of several steps, the closing kick also clears the next step's counters
+        // (read back above for this step already)
+        const bool prep = ctx->in_run_steps;
+        k_halfstep_kick<<<nblk(ctx->nact), kBlock, 0, ctx->stream>>>(T, S, ctx->nact, ctx->act, 1, BoxTable{},
+                                                                     prep ? ctx->counters : nullptr, prep ? ctx->u64c : nullptr);
         HIPCHK(hipGetLastError());
+        ctx->prepped = prep;
     }
     if (ctx->stop_code)
         return fail(ctx, ctx->stop_code, "timestep loop: controlled stop %d (%s)", ctx->stop_code,
@@ -1971,12 +2025,15 @@ int gz_run_steps(gz_context *ctx, int nsteps, long long *n_active, long long *n_
     CHK(set_device(ctx));
     int rc = 0;
     for (int k = 0; k < nsteps && !rc; k++) {
+        ctx->in_run_steps = k + 1 < nsteps;  // another step of this call follows
         rc = run_step(ctx);
         if (!rc) {
             if (n_active) n_active[k] = ctx->nact;
             if (n_woken) n_woken[k] = ctx->n_woken;
         }
     }
+    ctx->in_run_steps = false;
+    ctx->prepped = false;
     collect_timings(ctx);
     return rc;
 }
@@ -2013,6 +2070,18 @@ int gz_set_option(gz_context *ctx, const char *name, double value)
     if (k == "speculate") {
         if (value != 0 && value != 1) return fail(ctx, GZ_E_ARG, "speculate must be 0 or 1");
         ctx->spec_enabled = value != 0;
+        return 0;
+    }
+    if (k == "force_split") {
+        if (value != -1 && value != 0 && value != 1) return fail(ctx, GZ_E_ARG, "force_split must be -1, 0 or 1");
+        ctx->force_split = (int)value;
+        return 0;
+    }
+    if (k == "fd_correction") {
+        if (value != 0 && value != 1) return fail(ctx, GZ_E_ARG, "fd_correction must be 0 or 1");
+        ctx->fd = value != 0;
+        ctx->dp.fd4alpha = ctx->fd ? 4.0 * fd_alpha(ctx->par.kernel) : 0.0;
+        ctx->density_done = false;
         return 0;
     }
     if (k == "wide_search") {

@@ -223,7 +223,7 @@ int alloc_all(gz_batch *b)
                      &b->Z0, &b->it_h, &b->it_hb, &b->it_l, &b->it_r, &b->it_cn};
     for (auto p : d1) BCHK(balloc(b, p, n));
     BCHK(balloc(b, &b->T, 6 * n)); BCHK(balloc(b, &b->FW, 3 * n)); BCHK(balloc(b, &b->GR, 3 * n)); BCHK(balloc(b, &b->GP, 3 * n));
-    BCHK(balloc(b, &b->GV, 9 * n)); BCHK(balloc(b, &b->ACC, 3 * n)); BCHK(balloc(b, &b->stage, 9 * n)); BCHK(balloc(b, &b->dacc, 13 * n));
+    BCHK(balloc(b, &b->GV, 9 * n)); BCHK(balloc(b, &b->ACC, 3 * n)); BCHK(balloc(b, &b->stage, 9 * n)); BCHK(balloc(b, &b->dacc, kDensSumsFD * n));
     BCHK(balloc(b, &b->TIB, n)); BCHK(balloc(b, &b->DTS, n));
     int **i1[] = {&b->TB, &b->WAKE, &b->BOX, &b->perm, &b->ival, &b->ival2, &b->act, &b->it_ci, &b->it_it, &b->cnt, &b->rare};
     for (auto p : i1) BCHK(balloc(b, p, n));
@@ -1151,6 +1151,12 @@ int gz_batch_set_option(gz_batch *b, const char *name, double value)
     if (!strcmp(name, "few_rows")) {
         if (!(value >= 0) || value > 2e9) return fail(b, GZ_E_ARG, "few_rows must be in [0, 2e9]");
         b->few_rows = (int)value;
+        return 0;
+    }
+    if (!strcmp(name, "fd_correction")) {
+        // the first-derivative volume correction of the density, for the boxes' coming passes
+        if (value != 0 && value != 1) return fail(b, GZ_E_ARG, "fd_correction must be 0 or 1");
+        b->dp.fd4alpha = value != 0 ? 4.0 * fd_alpha(b->par.kernel) : 0.0;
         return 0;
     }
     return fail(b, GZ_E_ARG, "unknown option %s", name);

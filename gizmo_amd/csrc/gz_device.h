@@ -32,7 +32,29 @@ struct DevParams {
     double G0, G1, G2, G3, G4, G5, G6, G7, G8, G9;  // src/hydro/reimann.h:1-10 (GAMMA_G*)
     double des_num_ngb, max_num_ngb_dev, min_kernel_radius, max_kernel_radius;
     double time, time_begin;
+    // HYDRO_PARTITION_UNITY_IMPROVE_FD (src/hydro/density_loop.cc:622-633): 4 x the kernel's
+    // KERNEL_AWPMHD_FD_ALPHA (src/mesh/kernel.h:185-206) when on, 0 when off
+    double fd4alpha = 0;
 };
+
+// density accumulators per row: Ngb, Rho, DrkernNgb, DivVel, NV_T (6), face weights (3), and
+// with the first-derivative volume correction GradH_numer (3) and GradH_denom
+constexpr int kDensSums = 13;
+constexpr int kDensSumsFD = 17;
+
+// KERNEL_AWPMHD_FD_ALPHA of a kernel (src/mesh/kernel.h:185-206)
+__host__ __device__ inline double fd_alpha(int kernel)
+{
+    switch (kernel) {
+    case 3: return 3.0 / 16.0;
+    case 4: return 23.0 / 150.0;
+    case 5: return 7.0 / 54.0;
+    case 6: return 1.0 / 6.0;
+    case 7: return 5.0 / 39.0;
+    case 9: return 5.0 / 48.0;
+    default: return 3.0 / 16.0;
+    }
+}
 
 // ---------------------------------------------------------------------------------------
 // kernel: kernel_hinv / kernel_main, src/mesh/kernel.h:245-260, :268-440

@@ -177,7 +177,10 @@ __global__ void k_bin_hist(long n, const int *TB, unsigned long long *count, Box
 __global__ void k_wakeup_limits(TimeParams T, const unsigned long long *count, int *lim, BoxTable bt);
 __global__ void k_wakeups(TimeParams T, ParticleView S, const int *lim, int wakeup_bin_offset, unsigned long long *count,
                           int *nwoken, BoxTable bt);
-__global__ void k_halfstep_kick(TimeParams T, ParticleView S, int nrows, const int *act, int second, BoxTable bt);
+// prep (may be null): the counters and maxima of the coming sync point, cleared by block 0 as
+// k_step_prep would (a run of several sync points then needs no launch for it)
+__global__ void k_halfstep_kick(TimeParams T, ParticleView S, int nrows, const int *act, int second, BoxTable bt,
+                                int *prep_counters = nullptr, unsigned long long *prep_u64c = nullptr);
 __global__ void k_drift(TimeParams T, ParticleView S, long long time0, const unsigned long long *tl, const double *X0,
                         const double *Y0, const double *Z0, unsigned long long *disp, unsigned long long *hmax, BoxTable bt);
 __global__ void k_step_prep(int *counters, unsigned long long *u64c);
@@ -186,9 +189,12 @@ __global__ void k_active_flags(long n, const unsigned long long *tl, const int *
                                uint8_t *ACT, double *SOFT, int *flag, unsigned long long *hmax_act, BoxTable bt);
 __global__ void k_compact_count(long n, const int *flag, const int *scan, int *act, int *count_out);
 constexpr int kActChunk = 512;  // particles per block of the active-list kernels (and of the drift they are fused with)
-__global__ void k_drift_active(TimeParams T, ParticleView S, long long time0, const unsigned long long *tl,
+// count (may be null): the bin occupation, from which every block takes the next sync point
+// itself (k_next_sync's work; block 0 writes it to tl) instead of reading tl
+__global__ void k_drift_active(TimeParams T, ParticleView S, long long time0, unsigned long long *tl,
                                unsigned long long *disp, unsigned long long *hmax,
-                               double hclamp, uint8_t *ACT, unsigned long long *bits, int *bcount, unsigned long long *hmax_act);
+                               double hclamp, uint8_t *ACT, unsigned long long *bits, int *bcount, unsigned long long *hmax_act,
+                               const unsigned long long *count = nullptr);
 // the start of density()'s radius iteration for each row of a new active list (as k_density_init
 // does it; H null: not done here)
 struct DensityStart {
@@ -214,7 +220,6 @@ __global__ void k_advance_boxes(BoxTable bt);
 __global__ void k_active_from_bins(long n, unsigned long long mask, const int *TB, const double *M, const double *H,
                                    uint8_t *ACT, double *SOFT);
 __global__ void k_sum_counts(int nrows, const int *cnt, unsigned long long *out, const int *gate = nullptr);
-__global__ void k_pass_gate(const int *counters, const unsigned long long *hfinal, double hlim, int *gate);
 // groups of rows sharing one cell of one grid (the search over many levels, batches of boxes)
 __global__ void k_row_group_key(int nrows, const int *act, const uint8_t *need, const uint32_t *key, const double *hslot,
                                 const double *H, double box, double margin, const WideGrid *W, BoxTable bt, uint32_t *gkey, int *val);
@@ -286,9 +291,12 @@ __global__ void k_density_cells(DevParams P, GridView g, CellRows cr, const int 
 __global__ void k_density_init(int na, const int *act, double *H, double hclamp, double *hcur, double *left, double *right,
                                double *cn_cur, int *cn_init, int *iter, uint8_t *need, int *counters, unsigned long long *hmax,
                                unsigned long long *hfinal, BoxTable bt);
+// gate (may be null): the last block to finish writes the speculative pass's gate word from the
+// round's counters (done: a zero counter the blocks take turns on, left zero)
 __global__ void k_density_after(DevParams P, int nrows, const int *act, uint8_t *need, const double *accin, const double *M,
                                 const double *U, double cell, DensityIter it, DensityOut out, int *nnext, int *rebuild, int *err,
-                                unsigned long long *hfinal, BoxTable bt);
+                                unsigned long long *hfinal, BoxTable bt, double hlim = 0, int *gate = nullptr,
+                                unsigned int *done = nullptr);
 
 template <typename R>
 __global__ void k_pack_grad(long n, const double *H, const double *M, const double *RHO, const double *PR, const double *VX,
@@ -336,9 +344,12 @@ constexpr int kListThreads = 128;
 template <int KERN, typename R, int TPB = kListThreads>
 __global__ void k_gradients_rows(DevParams P, int nrows, const int *act, const double *X, const double *Y, const double *Z,
                                  const double *H, const GradRec<R> *rec, GradSrc src, const double *T, const double *CN, long n, const int *ell,
-                                 long ld, const int *cnt, double *GR, double *GP, double *GV, BoxTable bt);
-// STATS: the force kernels also count the RiemannRung bits of their faces into F.rungs
-template <int KERN, typename R, bool STATS = false, int TPB = kListThreads>
+                                 long ld, const int *cnt, double *GR, double *GP, double *GV, BoxTable bt,
+                                 int *force_flags = nullptr);
+// STATS: the force kernels also count the RiemannRung bits of their faces into F.rungs.  MAKE:
+// the kernel makes the records of its row and list entries from the state instead of reading
+// rec (few rows among many particles: no packing launch; more registers, fewer waves)
+template <int KERN, typename R, bool STATS = false, int TPB = kListThreads, bool MAKE = false>
 __global__ void k_force_rows(DevParams P, Gam<R> G, int nrows, const int *act, ForceView F, const Cell<R> *rec, const int *ell,
                              long ld, const int *cnt, const uint8_t *active, const int *timebin, const double *msv_in, int *wakeup,
                              double *ACC, double *DTU, double *MSV, int *err, int *need_wakeup, unsigned long long *pairs,
@@ -373,6 +384,13 @@ __global__ void k_force(DevParams P, Gam<R> G, ForceView F, const Cell<R> *rec, 
 // the rows that k_force<..., COMMON> listed, each with the whole ladder (a fixed grid strides over
 // the count the device holds; the rows' results are the single-pass kernel's, bit for bit)
 constexpr unsigned kRareBlocks = 1024;  // workgroups of 256 at most that k_force_rare launches
+// FP32 force pass over many rows: above this share of rows left to k_force_rare by the last
+// pass, the next pass runs one kernel with the whole ladder (the two kernels would evaluate
+// most rows twice)
+constexpr double kRareShareSingle = 0.25;
+// few-row force passes over at most this many rows make their records in the row kernel (its
+// MAKE form holds one wave per SIMD: this many rows of two wavefronts fill the chip once)
+constexpr int kMakeForceRows = 512;
 template <int KERN, typename R, bool BOXES>
 __global__ void k_force_rare(DevParams P, Gam<R> G, ForceView F, const Cell<R> *rec, ForceRows a, BoxTable bt);
 

@@ -42,17 +42,23 @@ template __global__ void k_build_ell<true>(int, const int *, const int *, const 
     template __global__ void k_gradients_rows<KERN, R>(DevParams, int, const int *, const double *, const double *,             \
                                                        const double *, const double *, const GradRec<R> *, GradSrc, const double *,             \
                                                        const double *, long, const int *, long, const int *, double *, double *,\
-                                                       double *, BoxTable);                                                     \
+                                                       double *, BoxTable, int *);                                              \
     template __global__ void k_force_rows<KERN, R>(DevParams, Gam<R>, int, const int *, ForceView, const Cell<R> *,            \
                                                    const int *, long, const int *, const uint8_t *, const int *,                \
                                                    const double *, int *, double *, double *, double *, int *, int *,           \
-                                                   unsigned long long *, BoxTable, const int *);
+                                                   unsigned long long *, BoxTable, const int *);                                \
+    template __global__ void k_force_rows<KERN, R, false, kListThreads, true>(DevParams, Gam<R>, int, const int *, ForceView,  \
+                                                   const Cell<R> *, const int *, long, const int *, const uint8_t *,            \
+                                                   const int *, const double *, int *, double *, double *, double *, int *,     \
+                                                   int *, unsigned long long *, BoxTable, const int *);
 GZ_INST(3, double)
 GZ_INST(6, double)
 // the force kernels that count the Riemann rungs of their faces (one box)
 #define GZ_INST_STATS(KERN, R)                                                                                                   \
     template __global__ void k_force<KERN, R, false, true, false>(DevParams, Gam<R>, ForceView, const Cell<R> *, ForceRows,   \
-                                                               BoxTable);                        \
+                                                               BoxTable);                                                  \
+    template __global__ void k_force<KERN, R, false, false, false>(DevParams, Gam<R>, ForceView, const Cell<R> *, ForceRows,  \
+                                                                BoxTable);                        \
     template __global__ void k_force_rows<KERN, R, true>(DevParams, Gam<R>, int, const int *, ForceView, const Cell<R> *,      \
                                                          const int *, long, const int *, const uint8_t *, const int *,          \
                                                          const double *, int *, double *, double *, double *, int *, int *,     \

@@ -366,6 +366,13 @@ __device__ inline void density_finalize(const DevParams &P, const DensityOut &o,
         if ((h > 0) && (nn > 0)) rho = mass * nn / (kVolNorm * pow(h, 3.0));
         else if (h <= 0) rho = 0;
         else rho = mass / (kVolNorm * pow(h, 3.0));
+        if (P.fd4alpha > 0 && acc[16] != 0) {
+            // the first-derivative partition-of-unity volume correction (src/hydro/density_loop.cc:622-633):
+            // grad H = -GradH_numer / GradH_denom, the density divided by 1 + 4 alpha |grad H|^2
+            const double inv_denom = -1.0 / acc[16];
+            const double g0 = acc[13] * inv_denom, g1 = acc[14] * inv_denom, g2 = acc[15] * inv_denom;
+            rho /= 1.0 + P.fd4alpha * (g0 * g0 + g1 * g1 + g2 * g2);
+        }
         o.p[i] = (P.gamma - 1) * u * rho;
     }
     o.rho[i] = rho;
@@ -517,8 +524,9 @@ __global__ void __launch_bounds__(64) k_density_cells(DevParams P, GridView g, C
         const bool rel = std::is_same<R, float>::value && (1 << g.level) >= 8;
         const R rxi = (R)wrap(xi - lb.ox, g.box, g.half), ryi = (R)wrap(yi - lb.oy, g.box, g.half),
                 rzi = (R)wrap(zi - lb.oz, g.box, g.half);
-        R a[13];
-        for (int q = 0; q < 13; q++) a[q] = 0;
+        const bool fd = P.fd4alpha > 0;
+        R a[kDensSumsFD];
+        for (int q = 0; q < kDensSumsFD; q++) a[q] = 0;
         const R hR = (R)(on ? h : 1.0);
         const R h2 = hR * hR;
         R hinv, hinv3, hinv4;
@@ -583,12 +591,22 @@ __global__ void __launch_bounds__(64) k_density_cells(DevParams P, GridView g, C
                         a[12] += rounded(wk * dp2);
                         const R dv0 = vxi - (R)svx[k], dv1 = vyi - (R)svy[k], dv2 = vzi - (R)svz[k];
                         a[3] -= rounded(dwk * (dp0 * dv0 + dp1 * dv1 + dp2 * dv2) / rr);
+                        if (fd) {  // GradH_numer, GradH_denom (src/hydro/density_loop.h:1281-1286)
+                            a[13] += rounded((dwk / rr) * dp0);
+                            a[14] += rounded((dwk / rr) * dp1);
+                            a[15] += rounded((dwk / rr) * dp2);
+                            a[16] += rounded(u * dwk);
+                        }
                     }
                 }
             }
         }
-        if (on)
-            for (int q = 0; q < 13; q++) accout[(size_t)q * cr.nrows + r] = (double)a[q];
+        if (on) {
+            // static bounds: the sums stay in registers
+            for (int q = 0; q < kDensSums; q++) accout[(size_t)q * cr.nrows + r] = (double)a[q];
+            if (fd)
+                for (int q = kDensSums; q < kDensSumsFD; q++) accout[(size_t)q * cr.nrows + r] = (double)a[q];
+        }
     }
 }
 
@@ -673,22 +691,6 @@ __global__ void __launch_bounds__(256) k_sum_counts(int nrows, const int *__rest
     }
 }
 
-// Whether the passes launched behind a step's first density round, before its counters are
-// read, stand: gate = the round's stop code (counters[2]); else -1 while rows still iterate
-// (counters[0]); else -2 when a final radius (hfinal, bits of the largest) exceeds hlim, the
-// bound every radius was taken under when the symmetric list's grid was sized; else 0.  The
-// passes' kernels with effects that a second run would not undo (wake-up flags, pair counts)
-// read it and do nothing unless it is 0.
-__global__ void k_pass_gate(const int *__restrict__ counters, const unsigned long long *__restrict__ hfinal, double hlim,
-                            int *__restrict__ gate)
-{
-    if (threadIdx.x != 0) return;
-    int g = 0;
-    if (counters[2]) g = counters[2];
-    else if (counters[0]) g = -1;
-    else if (__longlong_as_double((long long)*hfinal) > hlim) g = -2;
-    *gate = g;
-}
 #endif
 
 // after_iter_body (src/hydro/density_loop.h:516-961) on the sums of the round and, on
@@ -698,7 +700,8 @@ __global__ void __launch_bounds__(256) k_density_after(DevParams P, int nrows, c
                                                        const double *__restrict__ accin, const double *__restrict__ M,
                                                        const double *__restrict__ U, double cell, DensityIter it, DensityOut out,
                                                        int *__restrict__ nnext, int *__restrict__ rebuild, int *__restrict__ err,
-                                                       unsigned long long *__restrict__ hfinal, BoxTable bt)
+                                                       unsigned long long *__restrict__ hfinal, BoxTable bt, double hlim,
+                                                       int *__restrict__ gate, unsigned int *__restrict__ done)
 {
     // every lane reaches the wave-wide counts and maxima at the end
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
@@ -714,8 +717,12 @@ __global__ void __launch_bounds__(256) k_density_after(DevParams P, int nrows, c
             box = bt.BOX[i];
             cell = bt.b[box].rowlim;
         }
-        double acc[13];
-        for (int q = 0; q < 13; q++) acc[q] = accin[(size_t)q * nrows + r];
+        double acc[kDensSumsFD];
+        for (int q = 0; q < kDensSums; q++) acc[q] = accin[(size_t)q * nrows + r];
+        if (P.fd4alpha > 0)
+            for (int q = kDensSums; q < kDensSumsFD; q++) acc[q] = accin[(size_t)q * nrows + r];
+        else
+            for (int q = kDensSums; q < kDensSumsFD; q++) acc[q] = 0;
         const double h = it.hcur[r];
         IterState sc;
         sc.left = it.left[r]; sc.right = it.right[r]; sc.cn_cur = it.cn_cur[r]; sc.cn_init = it.cn_init[r];
@@ -754,6 +761,22 @@ __global__ void __launch_bounds__(256) k_density_after(DevParams P, int nrows, c
         for (int k = 0; k < (int)((blockDim.x + 63) >> 6); k++) { c += wcont[k]; g |= wgrow[k]; }
         if (c) atomicAdd(nnext, c);
         if (g) atomicOr(rebuild, 1);
+        if (gate) {
+            // the last block to finish, when every block's counts and maxima are in, writes
+            // whether the passes launched behind this round before its counters are read stand:
+            // the round's stop code; else -1 while rows still iterate; else -2 when a final
+            // radius exceeds hlim, the bound every radius was taken under when the symmetric
+            // list's grid was sized; else 0.  The passes' kernels with effects that a second
+            // run would not undo (wake-up flags, pair counts) read it and do nothing unless 0.
+            __threadfence();
+            if (atomicAdd(done, 1u) == gridDim.x - 1) {
+                __threadfence();
+                const int e = atomicOr(err, 0), rows = atomicOr(nnext, 0);
+                const double hf = __longlong_as_double((long long)atomicOr(hfinal, 0ull));
+                *gate = e ? e : (rows ? -1 : (hf > hlim ? -2 : 0));
+                *done = 0u;
+            }
+        }
     }
 }
 #endif
@@ -803,8 +826,7 @@ __global__ void k_pack_grad(long n, const double *__restrict__ H, const double *
 
 // the force record of particle j (effective_soundspeed, src/declarations/cell_data.h:703-712)
 template <typename R>
-__device__ __forceinline__ void pack_force_one(long j, double gamma, const ForceView &F, Cell<R> *__restrict__ rec, double box,
-                                               const BoxTable &bt)
+__device__ __forceinline__ Cell<R> force_rec_of(long j, double gamma, const ForceView &F, double box, const BoxTable &bt)
 {
     const long n = F.n;
     Cell<R> c;
@@ -818,7 +840,14 @@ __device__ __forceinline__ void pack_force_one(long j, double gamma, const Force
     const double rho = F.RHO[j];
     c.cs = (R)((rho > 0) ? sqrt(gamma * F.PR[j] / rho) : sqrt(gamma * (gamma - 1.) * F.U[j]));
     if constexpr (std::is_same<R, float>::value) c.fp = fine_pos(F.X[j], F.Y[j], F.Z[j], bt.b ? bt.b[bt.BOX[j]].box : box);
-    rec[j] = c;
+    return c;
+}
+
+template <typename R>
+__device__ __forceinline__ void pack_force_one(long j, double gamma, const ForceView &F, Cell<R> *__restrict__ rec, double box,
+                                               const BoxTable &bt)
+{
+    rec[j] = force_rec_of<R>(j, gamma, F, box, bt);
 }
 
 // the force pass's error and wake-up flags, the first pair without a Riemann solution and the
@@ -1151,11 +1180,16 @@ __global__ void __launch_bounds__(256, (COMMON && !BOXES && std::is_same<R, doub
         for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
         if ((threadIdx.x & 63) == 0 && c) atomicAdd(a.pairs, c);
     }
-    if (r >= a.nrows) return;
-    if (!force_row<KERN, R, BOXES, STATS, COMMON>(P, G, F, rec, a, bt, r)) {
-        const int s = atomicAdd(a.err + 12, 1);
-        a.rare[s] = r;
-    }
+    const bool done = (r >= a.nrows) || force_row<KERN, R, BOXES, STATS, COMMON>(P, G, F, rec, a, bt, r);
+    // rows left to the whole ladder: appended a wavefront at a time (one atomic per wavefront, the
+    // wavefront's rows kept together and in lane order)
+    const unsigned long long left = __ballot(!done);
+    if (!left) return;
+    const int lane = threadIdx.x & 63, lead = __ffsll((long long)left) - 1;
+    int base = 0;
+    if (lane == lead) base = atomicAdd(a.err + 12, __popcll(left));
+    base = __shfl(base, lead, 64);
+    if (!done) a.rare[base + __popcll(left & ((1ull << lane) - 1ull))] = r;
 }
 
 template <int KERN, typename R, bool BOXES>

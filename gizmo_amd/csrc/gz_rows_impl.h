@@ -220,7 +220,7 @@ __global__ void __launch_bounds__(TPB) k_density_rows(DevParams P, GridView g, i
                                                      double *__restrict__ accout, BoxTable bt, int *__restrict__ npairs)
 {
     __shared__ RowStencil rs;
-    __shared__ R sc[13][TPB + 1];  // the accepted candidates' terms, in candidate order (padded: no bank conflicts)
+    __shared__ R sc[kDensSumsFD][TPB + 1];  // the accepted candidates' terms, in candidate order (padded: no bank conflicts)
     __shared__ unsigned long long smask[TPB / 64];
     const long r = xcd_block();
     if (r >= nrows) return;
@@ -256,12 +256,14 @@ __global__ void __launch_bounds__(TPB) k_density_rows(DevParams P, GridView g, i
     }
     const R rxi = (R)wrap(xi - ox, g.box, g.half), ryi = (R)wrap(yi - oy, g.box, g.half), rzi = (R)wrap(zi - oz, g.box, g.half);
     const int total = rs.pref[rs.nst];
-    R acc = 0;  // thread q < 13 keeps sum q
-    int accepted = 0;  // thread 13: the row's pairs (the pass's work, counted when npairs is given)
+    const bool fd = P.fd4alpha > 0;
+    const int nsum = fd ? kDensSumsFD : kDensSums;
+    R acc = 0;  // thread q < nsum keeps sum q
+    int accepted = 0;  // thread 17: the row's pairs (the pass's work, counted when npairs is given)
     for (int base = 0; base < total; base += TPB) {
         const int v = base + tid;
         bool ok = false;
-        R t[13];
+        R t[kDensSumsFD];
         if (v < total) {
             const int j = row_candidate(rs, v);
             const double mj = M[j];
@@ -290,7 +292,7 @@ __global__ void __launch_bounds__(TPB) k_density_rows(DevParams P, GridView g, i
                     ok = true;
                     // the particle itself (rr = 0) adds to the first three sums only: its other
                     // terms are +0, which leave a sum that starts at +0 unchanged bit for bit
-                    for (int q = 3; q < 13; q++) t[q] = 0;
+                    for (int q = 3; q < kDensSumsFD; q++) t[q] = 0;
                     if (rr > 0) {
                         t[4] = wk * (dp0 * dp0);
                         t[5] = wk * (dp1 * dp1);
@@ -303,6 +305,12 @@ __global__ void __launch_bounds__(TPB) k_density_rows(DevParams P, GridView g, i
                         t[12] = wk * dp2;
                         const R dv0 = vxi - (R)VX[j], dv1 = vyi - (R)VY[j], dv2 = vzi - (R)VZ[j];
                         t[3] = dwk * (dp0 * dv0 + dp1 * dv1 + dp2 * dv2) / rr;
+                        if (fd) {  // GradH_numer, GradH_denom (src/hydro/density_loop.h:1281-1286)
+                            t[13] = (dwk / rr) * dp0;
+                            t[14] = (dwk / rr) * dp1;
+                            t[15] = (dwk / rr) * dp2;
+                            t[16] = u * dwk;
+                        }
                     }
                 }
             }
@@ -310,15 +318,19 @@ __global__ void __launch_bounds__(TPB) k_density_rows(DevParams P, GridView g, i
         // the accepted candidates' terms packed in candidate order; threads 0-12 then add them
         int nok;
         const int pos = flagged_rank<TPB>(ok, smask, nok);
-        if (ok)
-            for (int q = 0; q < 13; q++) sc[q][pos] = t[q];
+        if (ok) {
+            // static bounds: the terms stay in registers
+            for (int q = 0; q < kDensSums; q++) sc[q][pos] = t[q];
+            if (fd)
+                for (int q = kDensSums; q < kDensSumsFD; q++) sc[q][pos] = t[q];
+        }
         __syncthreads();
-        if (tid < 13) acc = ordered_sum(acc, sc[tid], nok, tid == 3);
-        else if (tid == 13) accepted += nok;
+        if (tid < nsum) acc = ordered_sum(acc, sc[tid], nok, tid == 3);
+        else if (tid == kDensSumsFD) accepted += nok;
         __syncthreads();
     }
-    if (tid < 13) accout[(size_t)tid * nrows + r] = (double)acc;
-    if (tid == 13 && npairs) npairs[r] = accepted;
+    if (tid < nsum) accout[(size_t)tid * nrows + r] = (double)acc;
+    if (tid == kDensSumsFD && npairs) npairs[r] = accepted;
 }
 
 // the symmetric list of few rows (the same entries, in the same order, as k_build_sym_cells)
@@ -731,12 +743,14 @@ __global__ void __launch_bounds__(TPB) k_gradients_rows(DevParams P, int nrows, 
                                                        const double *__restrict__ H, const GradRec<R> *__restrict__ rec, GradSrc src, const double *__restrict__ T,
                                                        const double *__restrict__ CN, long n, const int *__restrict__ ell, long ld,
                                                        const int *__restrict__ cnt, double *__restrict__ GR, double *__restrict__ GP,
-                                                       double *__restrict__ GV, BoxTable bt)
+                                                       double *__restrict__ GV, BoxTable bt, int *__restrict__ force_flags)
 {
     // the accepted entries' values packed in list order (padded rows: no bank conflicts)
     __shared__ R sd[5][TPB + 1], sw[3][TPB + 1], srr[TPB + 1];
     __shared__ unsigned long long smask[TPB / 64];
     const long r = xcd_block();
+    // the coming force pass's error and wake-up flags (its records are made in its own kernel)
+    if (r == 0 && threadIdx.x == 0 && force_flags) force_flags_clear(force_flags);
     if (r >= nrows) return;
     const int lane = threadIdx.x;  // thread of the workgroup
     const int i = act[r];
@@ -842,7 +856,7 @@ __global__ void __launch_bounds__(TPB) k_gradients_rows(DevParams P, int nrows, 
 
 // force of few rows: each lane evaluates one list entry (hydro_accumulate_neighbor), lane 0
 // accumulates in list order, then the writeback and final operations
-template <int KERN, typename R, bool STATS, int TPB>
+template <int KERN, typename R, bool STATS, int TPB, bool MAKE>
 __global__ void __launch_bounds__(TPB) k_force_rows(DevParams P, Gam<R> G, int nrows, const int *__restrict__ act, ForceView F,
                                                    const Cell<R> *__restrict__ rec, const int *__restrict__ ell, long ld,
                                                    const int *__restrict__ cnt, const uint8_t *__restrict__ active,
@@ -855,7 +869,7 @@ __global__ void __launch_bounds__(TPB) k_force_rows(DevParams P, Gam<R> G, int n
     __shared__ unsigned long long smask[TPB / 64];
     const long r = xcd_block();
     if (r >= nrows) return;
-    if (gate && *gate) return;  // the density round behind this launch did not stand (k_pass_gate)
+    if (gate && *gate) return;  // the density round behind this launch did not stand (its gate word)
     const int lane = threadIdx.x;  // thread of the workgroup
     const int i = act[r];
     {
@@ -863,13 +877,14 @@ __global__ void __launch_bounds__(TPB) k_force_rows(DevParams P, Gam<R> G, int n
         resolve_box(bt, i, P, g);
     }
     const double mi = F.M[i], rhoi = F.RHO[i];
+    constexpr bool make = MAKE;
     double A0 = 0, A1 = 0, A2 = 0, dE = 0, msv = 0;
     const double sound_i = (rhoi > 0) ? sqrt(P.gamma * F.PR[i] / rhoi) : sqrt(P.gamma * (P.gamma - 1.) * F.U[i]);
     const bool enabled = (mi > 0 && rhoi > 0);
     const int c = cnt[r];
     if (lane == 0 && wakeup) wakeup[i] = 0;  // hydro_force_initial_operations_preloop
     if (enabled) {
-        const Cell<R> I = rec[i];
+        const Cell<R> I = make ? force_rec_of<R>(i, P.gamma, F, P.box, bt) : rec[i];
         const double xi = F.X[i], yi = F.Y[i], zi = F.Z[i];
         const int tbi = timebin ? timebin[i] : 0;
         // lanes 0-3 keep the momentum and energy sums, lane 4 the signal speed: max(c_s, vsig)
@@ -883,7 +898,7 @@ __global__ void __launch_bounds__(TPB) k_force_rows(DevParams P, Gam<R> G, int n
             R f0, f1, f2, fp, vts;
             if (q < c) {
                 const int j = ell[(size_t)q * ld + r];
-                const Cell<R> J = rec[j];
+                const Cell<R> J = make ? force_rec_of<R>(j, P.gamma, F, P.box, bt) : rec[j];
                 R dp0, dp1, dp2;
                 record_dp<R>(P, I, J, xi, yi, zi, F.X, F.Y, F.Z, j, dp0, dp1, dp2);
                 const R r2 = dp0 * dp0 + dp1 * dp1 + dp2 * dp2;

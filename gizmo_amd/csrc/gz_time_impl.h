@@ -39,6 +39,15 @@ __device__ __forceinline__ double drift_log_change(double x)
     return x;
 }
 
+// the start of a sync point (threads 0-63 of one workgroup): the step's counters and maxima
+// cleared (the bin occupation and the running pair count kept)
+__device__ __forceinline__ void step_prep(int *__restrict__ counters, unsigned long long *__restrict__ u64c)
+{
+    const int t = threadIdx.x;
+    if (t < 16) counters[t] = 0;
+    if (t == 0) { u64c[64] = 0; u64c[66] = 0; u64c[67] = 0; u64c[68] = 0; u64c[72] = 0; u64c[73] = 0; u64c[75] = 0; u64c[76] = 0; }
+}
+
 // the largest kicked velocity component of the calling lanes' particles (valid lanes only) into
 // the bound *vkmax: one conditional atomic per wavefront (all 64 lanes call it together)
 __device__ __forceinline__ void raise_vk_bound(const ParticleView &S, long i, bool valid)
@@ -255,6 +264,31 @@ __device__ __forceinline__ long long wave_next_kick(const unsigned long long *__
     return t;
 }
 
+// find_next_sync_point_and_drift's timeline part for one box, by one wavefront (lane k = bin k):
+// the next kick, the bins active there, the particles in them (NumForceUpdate), and whether
+// every particle is drifted at it (src/core/run.cc:857-955)
+struct NextSync {
+    long long ti;
+    unsigned long long mask, nf;
+    bool full;
+};
+__device__ __forceinline__ NextSync next_sync_wave(const TimeParams &T, const unsigned long long *__restrict__ count, long long tic)
+{
+    const int k = threadIdx.x & 63;
+    NextSync s;
+    s.ti = wave_next_kick(count, tic);
+    s.mask = __ballot(k == 0 || (k < kTimebins && (s.ti % bin_len(k)) == 0));
+    unsigned long long nf = (k < kTimebins && ((s.mask >> k) & 1ull)) ? count[k] : 0ull;
+    for (int o = 32; o > 0; o >>= 1) nf += __shfl_xor(nf, o, 64);
+    s.nf = nf;
+    // every particle is drifted at a step whose active fraction reaches the domain cadence
+    // (src/core/run.cc:360,380-390: the full drift before the decomposition), or where the lazy
+    // drift is not in use; otherwise only the active bins (find_next_sync_point_and_drift,
+    // src/core/run.cc:979-994) and the others when read
+    s.full = !T.lazy_ok || (double)nf > T.full_frac * (double)T.n_total || (long long)nf > T.lazy_rows;
+    return s;
+}
+
 // the bin limits of process_wake_ups (src/core/timestep.cc:1577-1625) from the occupation
 // after find_timesteps, one wavefront per box (block b = box b, lane k = bin k): lim[2b] the
 // lowest occupied active bin, lim[2b+1] the highest bin active at the next kick
@@ -339,9 +373,11 @@ __global__ void __launch_bounds__(256) k_wakeups(TimeParams T, ParticleView S, c
 
 // do_first_halfstep_kick / do_second_halfstep_kick, src/core/kicks.cc:34-97,99-185
 __global__ void __launch_bounds__(256) k_halfstep_kick(TimeParams T, ParticleView S, int nrows, const int *__restrict__ act, int second,
-                                                        BoxTable bt)
+                                                        BoxTable bt, int *__restrict__ prep_counters,
+                                                        unsigned long long *__restrict__ prep_u64c)
 {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (prep_counters && blockIdx.x == 0 && threadIdx.x < 64) step_prep(prep_counters, prep_u64c);
     const int i = r < nrows ? act[r] : 0;
     const bool row = r < nrows && S.M[i] > 0;
     if (row) {
@@ -461,12 +497,9 @@ __global__ void __launch_bounds__(256) k_drift(TimeParams T, ParticleView S, lon
     }
 }
 
-// the start of a step: counters and maxima cleared (the bin occupation and the running pair count kept)
 __global__ void k_step_prep(int *__restrict__ counters, unsigned long long *__restrict__ u64c)
 {
-    const int t = threadIdx.x;
-    if (t < 16) counters[t] = 0;
-    if (t == 0) { u64c[64] = 0; u64c[66] = 0; u64c[67] = 0; u64c[68] = 0; u64c[72] = 0; u64c[73] = 0; u64c[75] = 0; }
+    step_prep(counters, u64c);
 }
 
 // find_next_sync_point_and_drift's timeline part (src/core/run.cc:857-930) from the occupation,
@@ -486,26 +519,17 @@ __global__ void __launch_bounds__(64) k_next_sync(TimeParams T, const unsigned l
             return;
         }
     }
-    const long long ti_next_kick = wave_next_kick(count + (size_t)64 * b, tic);
-    const unsigned long long mask = __ballot(k == 0 || (k < kTimebins && (ti_next_kick % bin_len(k)) == 0));
-    // NumForceUpdate: the particles of the bins active at the next kick (src/core/run.cc:942-955)
-    unsigned long long nf = (k < kTimebins && ((mask >> k) & 1ull)) ? count[(size_t)64 * b + k] : 0ull;
-    for (int o = 32; o > 0; o >>= 1) nf += __shfl_xor(nf, o, 64);
+    const NextSync ns = next_sync_wave(T, count + (size_t)64 * b, tic);
     if (k != 0) return;
     if (bt.b) {
         bt.b[b].need_wakeup = 0;  // consumed by the wake-up pass of this step
-        bt.b[b].ti_next = ti_next_kick;
-        bt.b[b].mask_next = mask;
+        bt.b[b].ti_next = ns.ti;
+        bt.b[b].mask_next = ns.mask;
     } else {
-        tl[0] = (unsigned long long)ti_next_kick;
-        tl[1] = mask;
-        // every particle is drifted at a step whose active fraction reaches the domain cadence
-        // (src/core/run.cc:360,380-390: the full drift before the decomposition), or where the
-        // lazy drift is not in use; otherwise only the active bins (find_next_sync_point_and_drift,
-        // src/core/run.cc:979-994) and the others when read
-        const bool full = !T.lazy_ok || (double)nf > T.full_frac * (double)T.n_total || (long long)nf > T.lazy_rows;
-        tl[2] = full ? 1ull : 0ull;
-        tl[3] = nf;
+        tl[0] = (unsigned long long)ns.ti;
+        tl[1] = ns.mask;
+        tl[2] = ns.full ? 1ull : 0ull;
+        tl[3] = ns.nf;
     }
 }
 
@@ -557,15 +581,32 @@ __global__ void __launch_bounds__(256) k_active_flags(long n, const unsigned lon
 // particle can move, drifted now or later (the bound on the kicked velocities times the step,
 // padded for rounding): the caller sums these into its bound on the displacement since the last
 // ordering, so no ordering positions are read.  hmax receives the largest radius drifted.
-__global__ void __launch_bounds__(256) k_drift_active(TimeParams T, ParticleView S, long long time0, const unsigned long long *__restrict__ tl,
+__global__ void __launch_bounds__(256) k_drift_active(TimeParams T, ParticleView S, long long time0, unsigned long long *__restrict__ tl,
                                                       unsigned long long *__restrict__ disp,
                                                       unsigned long long *__restrict__ hmax, double hclamp, uint8_t *__restrict__ ACT,
                                                       unsigned long long *__restrict__ bits, int *__restrict__ bcount,
-                                                      unsigned long long *__restrict__ hmax_act)
+                                                      unsigned long long *__restrict__ hmax_act,
+                                                      const unsigned long long *__restrict__ occupation)
 {
-    const long long time1 = (long long)tl[0];
-    const unsigned long long mask = tl[1];
-    const bool full = tl[2] != 0;
+    __shared__ unsigned long long s_tl[3];
+    const unsigned long long *count_bins = occupation;
+    if (count_bins) {
+        // the next sync point from the occupation, in every block (k_next_sync's work; the
+        // first block also writes it for the host and the later launches)
+        if (threadIdx.x < 64) {
+            const NextSync ns = next_sync_wave(T, count_bins, T.ti_current);
+            if (threadIdx.x == 0) {
+                s_tl[0] = (unsigned long long)ns.ti;
+                s_tl[1] = ns.mask;
+                s_tl[2] = ns.full ? 1ull : 0ull;
+                if (blockIdx.x == 0) { tl[0] = s_tl[0]; tl[1] = s_tl[1]; tl[2] = s_tl[2]; tl[3] = ns.nf; }
+            }
+        }
+        __syncthreads();
+    }
+    const long long time1 = (long long)(count_bins ? s_tl[0] : tl[0]);
+    const unsigned long long mask = count_bins ? s_tl[1] : tl[1];
+    const bool full = (count_bins ? s_tl[2] : tl[2]) != 0;
     const long base = (long)blockIdx.x * kActChunk;
     const int tid = threadIdx.x, w = tid >> 6;
     double d1 = 0, h1 = 0, hm = 0;

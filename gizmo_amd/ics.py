@@ -50,6 +50,31 @@ def lattice_box(n_side, seed, *, gamma=5.0 / 3.0, pressure=1.0, density=1.0, mac
                 params=params)
 
 
+def gradient_box(n_side, *, k=2.0, gamma=5.0 / 3.0, pressure=1.0, kernel=6, des_num_ngb=64.0, box=1.0):
+    """Particles whose number density grows as exp(k x) along x: a cubic lattice of n_side^3
+    points at cell centres in (xi, y, z), mapped along x by x = box ln(1 + xi (e^k - 1)) / k
+    (so that d xi / dx is proportional to exp(k x / box)); equal masses (total mass box^3),
+    uniform pressure, at rest.  The periodic box joins the densest and the most dilute slabs at
+    x = 0; away from that face, a kernel radius that keeps the neighbour count fixed varies as
+    n^(-1/3): dh/dx = -k h / (3 box)."""
+    n = n_side ** 3
+    d = box / n_side
+    g = (np.arange(n_side, dtype=np.float64) + 0.5) * d
+    X, Y, Z = np.meshgrid(g, g, g, indexing="ij")
+    xi = X.ravel() / box
+    x = box * np.log1p(xi * np.expm1(k)) / k
+    pos = np.stack([x, Y.ravel(), Z.ravel()], axis=1) % box
+    mass = np.full(n, box ** 3 / n)
+    nloc = n * k * np.exp(k * x / box) / (np.expm1(k) * box ** 3)  # particles per volume at x
+    h = (des_num_ngb / (VOLNORM * nloc)) ** (1.0 / 3.0)
+    rho = mass * nloc
+    u = pressure / ((gamma - 1.0) * rho)
+    params = dict(box=box, kernel=kernel, gamma=gamma, des_num_ngb=float(des_num_ngb),
+                  max_num_ngb_dev=max(des_num_ngb / 640.0, 0.05),
+                  min_kernel_radius=0.0, max_kernel_radius=0.25 * box, time=0.0, time_begin=0.0)
+    return dict(pos=np.ascontiguousarray(pos), vel=np.zeros((n, 3)), mass=mass, u=u, h=h, params=params, k=k)
+
+
 def _wendland_c2_shape(q):
     q = np.clip(q, 0.0, 1.0)
     return (1.0 - q) ** 4 * (1.0 + 4.0 * q)

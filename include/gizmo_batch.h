@@ -68,7 +68,8 @@ int gz_batch_timing_reset(gz_batch *b);
 /* Options: "few_rows" (default 8192) -- passes over at most this many active rows (all boxes
    together) take one wavefront per row; more rows are walked in groups that share a cell of
    their box's grid (the same lists and sums either way).  No counterpart in the reference (a
-   tuning knob of this library, as gz_set_option's "few_rows"). */
+   tuning knob of this library, as gz_set_option's "few_rows").  "fd_correction" (default 0):
+   the first-derivative volume correction of the density for every box, as gz_set_option's. */
 int gz_batch_set_option(gz_batch *b, const char *name, double value);
 
 #ifdef __cplusplus

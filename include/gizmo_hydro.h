@@ -231,8 +231,19 @@ int gz_drift_all(gz_context *ctx);
    (the force kernels, whose wake-up flags and pair counts a second run would not undo, wait on
    a device flag: the round converged every row and no radius outgrew the list's grid); one
    read of the counters after the force pass then settles the step, and when the flag says no,
-   the rounds go on and the three passes run again.  None of these changes a neighbour set or a
-   sum, and none has a counterpart in the reference (tuning knobs of this library). */
+   the rounds go on and the three passes run again; "force_split" (default -1) -- the force pass
+   over more than few_rows rows runs in two kernels, the first evaluating every face through the
+   Riemann ladder's common case and leaving rows with a face that needs a later rung to the
+   second (1), in one kernel with the whole ladder (0), or (-1) in two unless it is the FP32 form
+   and more than a quarter of the rows took the second kernel in the last two-kernel pass.
+   None of these changes a neighbour set or a
+   sum, and none has a counterpart in the reference (tuning knobs of this library).
+   One option selects physics, the reference's compile-time HYDRO_PARTITION_UNITY_IMPROVE_FD
+   (src/hydro/density_loop.cc:622-633, src/hydro/density_loop.h:1281-1286): "fd_correction"
+   (default 0) -- 1: the density pass also sums GradH_numer and GradH_denom, and the final
+   density is divided by 1 + 4 alpha |grad H|^2 (grad H = -GradH_numer / GradH_denom, alpha =
+   the kernel's KERNEL_AWPMHD_FD_ALPHA, src/mesh/kernel.h:185-206) before the pressure is
+   taken from it. */
 int gz_set_option(gz_context *ctx, const char *name, double value);
 
 /* The Riemann functions of the force pass on n given face states, on the context's device and

@@ -44,6 +44,22 @@
 #define OR_VOLNORM 4.188790204786
 #define OR_WAKEUP 4.1
 #define OR_NDIMS 3
+#define OR_NACC 17 /* density accumulators: the 13 of the pass and GradH_numer[3], GradH_denom */
+
+/* KERNEL_AWPMHD_FD_ALPHA, the first-derivative partition-of-unity coefficient of each kernel
+   (src/mesh/kernel.h:185-206) */
+double orc_fd_alpha(int kernel)
+{
+    switch (kernel) {
+    case 3: return 3.0 / 16.0;
+    case 4: return 23.0 / 150.0;
+    case 5: return 7.0 / 54.0;
+    case 6: return 1.0 / 6.0;
+    case 7: return 5.0 / 39.0;
+    case 9: return 5.0 / 48.0;
+    default: return 3.0 / 16.0;
+    }
+}
 
 #define DMAX(a, b) (((a) > (b)) ? (a) : (b))
 #define DMIN(a, b) (((a) < (b)) ? (a) : (b))
@@ -408,14 +424,14 @@ static double or_face_closure(const double *nvt_raw6, const double *fw, const do
 
 /* ------------------------------------------------------------------------- */
 /* density pair sums: DensitySpec::pair_kernel, src/hydro/density_loop.h:1141-1385 */
-/* (the gas-i, no-extra-physics branches).  acc[13] = Ngb, Rho, DrkernNgb,        */
+/* (the gas-i, no-extra-physics branches).  acc[OR_NACC] = Ngb, Rho, DrkernNgb,    */
 /* DivVel, NV_T[6], face_w[3].                                                    */
 /* ------------------------------------------------------------------------- */
 static void or_density_sums(const orc_params *par, const orc_state *s, long i, double h,
                             int *cand, long ncand, double *acc)
 {
     double box = par->box, half = 0.5 * par->box;
-    for (int k = 0; k < 13; k++) acc[k] = 0;
+    for (int k = 0; k < OR_NACC; k++) acc[k] = 0;
     const double *pi = &s->pos[3 * i];
     const double *vi = &s->vel[3 * i];
     const double h2 = h * h;
@@ -469,6 +485,13 @@ static void or_density_sums(const orc_params *par, const orc_state *s, long i, d
             double dv1 = vi[1] - s->vel[3 * j + 1];
             double dv2 = vi[2] - s->vel[3 * j + 2];
             acc[3] -= dwk * (dp0 * dv0 + dp1 * dv1 + dp2 * dv2) / r;
+            if (par->fd_correction) {
+                /* GradH_numer, GradH_denom (src/hydro/density_loop.h:1281-1286) */
+                acc[13] += (dwk / r) * dp0;
+                acc[14] += (dwk / r) * dp1;
+                acc[15] += (dwk / r) * dp2;
+                acc[16] += u * dwk;
+            }
         }
     }
 }
@@ -699,6 +722,15 @@ static void or_density_finalize(const orc_params *par, orc_state *s, long i, dou
             if (s->h[i] <= 0) s->rho[i] = 0;
             else s->rho[i] = s->mass[i] / (OR_VOLNORM * pow(s->h[i], (double)OR_NDIMS));
         }
+        if (par->fd_correction && acc[16] != 0) {
+            /* the first-derivative partition-of-unity volume correction (src/hydro/density_loop.cc:622-633):
+               the gradient of the kernel size implied by the particle distribution,
+               grad H = -GradH_numer / GradH_denom, divides the density by 1 + 4 alpha |grad H|^2 */
+            const double inv_denom = -1.0 / acc[16];
+            const double g0 = acc[13] * inv_denom, g1 = acc[14] * inv_denom, g2 = acc[15] * inv_denom;
+            const double fd = 1.0 + 4.0 * orc_fd_alpha(par->kernel) * (g0 * g0 + g1 * g1 + g2 * g2);
+            s->rho[i] /= fd;
+        }
         s->press[i] = (par->gamma - 1) * s->u[i] * s->rho[i];
     }
     if (s->numngb[i] > 0) s->numngb[i] = pow(s->numngb[i], 1.0 / OR_NDIMS);
@@ -732,7 +764,7 @@ int orc_density(const orc_params *par, orc_state *s, const unsigned char *active
     double *hcur = (double *)malloc((size_t)(n > 0 ? n : 1) * sizeof(double));
     or_scratch *sc = (or_scratch *)calloc((size_t)(n > 0 ? n : 1), sizeof(or_scratch));
     int *iters = (int *)calloc((size_t)(n > 0 ? n : 1), sizeof(int));
-    double *accs = (double *)malloc((size_t)(n > 0 ? n : 1) * 13 * sizeof(double));
+    double *accs = (double *)malloc((size_t)(n > 0 ? n : 1) * OR_NACC * sizeof(double));
     for (long i = 0; i < n; i++) {
         todo[i] = (!active || active[i]) && (s->mass[i] > 0);
         hcur[i] = s->h[i];
@@ -756,9 +788,9 @@ int orc_density(const orc_params *par, orc_state *s, const unsigned char *active
                         break;
                     }
                     long m = or_grid_candidates(&g, &s->pos[3 * i], box, buf);
-                    or_density_sums(par, s, i, h, buf, m, &accs[13 * i]);
+                    or_density_sums(par, s, i, h, buf, m, &accs[OR_NACC * i]);
                     double nh = h;
-                    int rc = or_after_iter(par, s->cn[i], iters[i], h, &accs[13 * i], &sc[i], &nh);
+                    int rc = or_after_iter(par, s->cn[i], iters[i], h, &accs[OR_NACC * i], &sc[i], &nh);
                     iters[i]++;
                     if (rc <= 0) {
                         if (rc < 0) {
@@ -789,7 +821,7 @@ int orc_density(const orc_params *par, orc_state *s, const unsigned char *active
 #pragma omp parallel for schedule(static)
     for (long i = 0; i < n; i++) {
         if (!(!active || active[i]) || !(s->mass[i] > 0)) continue;
-        or_density_finalize(par, s, i, hcur[i], &accs[13 * i]);
+        or_density_finalize(par, s, i, hcur[i], &accs[OR_NACC * i]);
     }
     if (iters_out) for (long i = 0; i < n; i++) iters_out[i] = iters[i];
     free(todo); free(hcur); free(sc); free(iters); free(accs);
@@ -797,7 +829,7 @@ int orc_density(const orc_params *par, orc_state *s, const unsigned char *active
 }
 
 /* density at a fixed, given radius: one evaluation of the pair sums plus the finalize.
-   raw (optional, 13*n) receives the accumulators.  This is the fixed-h form in which
+   raw (optional, OR_NACC*n) receives the accumulators.  This is the fixed-h form in which
    the device path is compared to 1e-12 and bit-for-bit. */
 int orc_density_fixed_h(const orc_params *par, orc_state *s, const unsigned char *active, double *raw)
 {
@@ -810,13 +842,13 @@ int orc_density_fixed_h(const orc_params *par, orc_state *s, const unsigned char
 #pragma omp parallel
     {
         int *buf = (int *)malloc((size_t)n * sizeof(int));
-        double acc[13];
+        double acc[OR_NACC];
 #pragma omp for schedule(dynamic, 64)
         for (long i = 0; i < n; i++) {
             if ((active && !active[i]) || !(s->mass[i] > 0)) continue;
             long m = or_grid_candidates(&g, &s->pos[3 * i], par->box, buf);
             or_density_sums(par, s, i, s->h[i], buf, m, acc);
-            if (raw) for (int k = 0; k < 13; k++) raw[13 * i + k] = acc[k];
+            if (raw) for (int k = 0; k < OR_NACC; k++) raw[OR_NACC * i + k] = acc[k];
             or_density_finalize(par, s, i, s->h[i], acc);
         }
         free(buf);

@@ -18,6 +18,7 @@ typedef struct {
     double min_kernel_radius; /* All.MinKernelRadius */
     double max_kernel_radius; /* All.MaxKernelRadius */
     double time, time_begin;  /* All.Time, All.TimeBegin */
+    int fd_correction;        /* HYDRO_PARTITION_UNITY_IMPROVE_FD (src/hydro/density_loop.cc:622-633): 0 off, 1 on */
 } orc_params;
 
 typedef struct {
@@ -92,6 +93,7 @@ typedef struct {
 void orc_set_lazy(const orc_lazy *view);
 
 int orc_kernel(int kernel, double r, double h, double *wk, double *dwk);
+double orc_fd_alpha(int kernel);
 long orc_neighbors(const orc_params *par, long n, const double *pos, const double *h, int symmetric,
                    const unsigned char *rowmask, long *offsets, int *nbr, long cap);
 double orc_matrix_invert(const double *T, double *Tinv);

@@ -29,7 +29,7 @@ class Params(C.Structure):
     _fields_ = [("box", C.c_double), ("kernel", C.c_int), ("gamma", C.c_double),
                 ("des_num_ngb", C.c_double), ("max_num_ngb_dev", C.c_double),
                 ("min_kernel_radius", C.c_double), ("max_kernel_radius", C.c_double),
-                ("time", C.c_double), ("time_begin", C.c_double)]
+                ("time", C.c_double), ("time_begin", C.c_double), ("fd_correction", C.c_int)]
 
 
 class _CState(C.Structure):
@@ -80,6 +80,8 @@ def lib():
         build()
         L = C.CDLL(_LIB)
         L.orc_kernel.argtypes = [C.c_int, C.c_double, C.c_double, _dp, _dp]
+        L.orc_fd_alpha.argtypes = [C.c_int]
+        L.orc_fd_alpha.restype = C.c_double
         L.orc_neighbors.argtypes = [C.POINTER(Params), C.c_long, _dp, _dp, C.c_int, _up, _lp, _ip, C.c_long]
         L.orc_neighbors.restype = C.c_long
         L.orc_matrix_invert.argtypes = [_dp, _dp]
@@ -115,8 +117,12 @@ def _ptr(a, t=_dp):
     return a.ctypes.data_as(t) if a is not None else None
 
 
+PARAM_DEFAULTS = {"fd_correction": 0}
+NACC = 17  # density accumulators per particle (OR_NACC)
+
+
 def params(d):
-    return Params(**{k: d[k] for k, _ in Params._fields_})
+    return Params(**{k: d.get(k, PARAM_DEFAULTS.get(k)) if k in PARAM_DEFAULTS else d[k] for k, _ in Params._fields_})
 
 
 class State:
@@ -189,6 +195,10 @@ def kernel(kid, r, h):
     return wk.value, dwk.value
 
 
+def fd_alpha(kid):
+    return float(lib().orc_fd_alpha(int(kid)))
+
+
 def neighbors(par, pos, h, symmetric, rowmask=None):
     pos = np.ascontiguousarray(pos, dtype=np.float64)
     h = np.ascontiguousarray(h, dtype=np.float64)
@@ -227,7 +237,7 @@ def density_fixed_h(par, st, active=None):
     p = params(par)
     cs = st._c()
     m, mp = _mask(active, st.n)
-    raw = np.zeros((st.n, 13))
+    raw = np.zeros((st.n, NACC))
     rc = lib().orc_density_fixed_h(C.byref(p), C.byref(cs), mp, _ptr(raw))
     if rc:
         raise RuntimeError(f"orc_density_fixed_h failed {rc}")

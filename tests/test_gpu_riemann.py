@@ -183,10 +183,13 @@ def test_common_and_rare_kernels_same_bits(oracle_lib, state, form):
     prec = GZ_FP64 if form == "fp64" else GZ_FP32_MIXED
     outs = []
     rare = []
-    for stats in (1, 0):
-        print(f"common/rare: {state} {form} n={s.n} riemann_stats={stats}", flush=True)
+    # the rung-counting kernel, the two kernels, and the one whole-ladder kernel (option
+    # force_split 0, which the FP32 form takes when most rows need the second kernel)
+    for stats, split in ((1, -1), (0, 1), (0, 0)):
+        print(f"common/rare: {state} {form} n={s.n} riemann_stats={stats} force_split={split}", flush=True)
         g = Hydro(par, s.n, prec)
         g.set_option("few_rows", 0)
+        g.set_option("force_split", split)
         g.load(f)
         g.set_option("riemann_stats", stats)
         g.timing_reset()
@@ -195,8 +198,9 @@ def test_common_and_rare_kernels_same_bits(oracle_lib, state, form):
         rare.append(g.stat("force_rare_rows"))
         g.close()
     print(f"rows through the second kernel: {rare[1]:.0f} of {s.n}", flush=True)
-    assert rare[0] == 0
+    assert rare[0] == 0 and rare[2] == 0
     if state == "colliding_streams":
         assert rare[1] > 0
-    for k in outs[0]:
-        assert np.array_equal(outs[0][k], outs[1][k]), k
+    for o in outs[1:]:
+        for k in outs[0]:
+            assert np.array_equal(outs[0][k], o[k]), k

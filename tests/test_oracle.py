@@ -322,3 +322,36 @@ def test_force_conservation(oracle_lib, kern, nngb):
     e_scale = (st.mass * (np.abs(st.dtu) + np.abs((st.vel * st.accel).sum(1)))).sum()
     assert abs(e_dot) < 1e-13 * e_scale
     assert np.all(st.maxsig > 0)
+
+
+def test_fd_correction_pins(oracle_lib):
+    """The first-derivative volume correction (src/hydro/density_loop.cc:622-633):
+      * each kernel's alpha equals -(2 pi / 3) int_0^1 xi^5 dzeta/dxi dxi over the kernel's own
+        shape (the definition src/mesh/kernel.h:185-191 states), integrated numerically from
+        the oracle's kernel;
+      * on particles whose number density grows as exp(k x), the kernel-size gradient that the
+        correction implies, |grad H| = sqrt((rho / rho_fd - 1) / (4 alpha)), equals the closed
+        form k h / 3 of a radius that keeps the neighbour count fixed (Wendland C2, 64
+        neighbours; median over the particles away from the box face that joins the densest
+        and the most dilute slabs, within 10%);
+      * with the option off the density is untouched, and the correction only lowers it."""
+    from scipy.integrate import quad
+    O = oracle_lib
+    for kern in (3, 6):
+        v, _ = quad(lambda x: x ** 5 * O.kernel(kern, x, 1.0)[1], 0.0, 1.0, points=[0.5], epsabs=1e-14, epsrel=1e-13, limit=200)
+        assert abs(-(2 * np.pi / 3) * v - O.fd_alpha(kern)) < 1e-12 * O.fd_alpha(kern), kern
+    ic = ics.gradient_box(24, k=2.0, kernel=6, des_num_ngb=64.0)
+    par = dict(ic["params"])
+    st = O.State.from_ic(ic)
+    O.density(par, st)
+    off, on = st.copy(), st.copy()
+    O.density_fixed_h(par, off)
+    O.density_fixed_h(dict(par, fd_correction=1), on)
+    assert np.array_equal(off.rho, st.rho)
+    assert (on.rho <= off.rho).all() and (on.rho < off.rho).any()
+    grad_h = np.sqrt((off.rho / on.rho - 1.0) / (4.0 * O.fd_alpha(6)))
+    x = st.pos[:, 0]
+    inner = (x > 0.3) & (x < 0.7)
+    ratio = grad_h[inner] / (ic["k"] * st.h[inner] / 3.0)
+    assert abs(np.median(ratio) - 1.0) < 0.1, np.median(ratio)
+    assert np.array_equal(on.h, off.h) and np.array_equal(on.nvt, off.nvt)
