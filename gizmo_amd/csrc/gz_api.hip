@@ -774,7 +774,7 @@ int build_list(gz_context *ctx, int nrows, const int *rows, const double *hrow, 
     if (*ell == nullptr || *ld != need_ld || *cap < cap_hint) {
         *cap = std::max(*cap, cap_hint);
         *ld = need_ld;
-        CHK(dalloc(ctx, ell, (size_t)(*cap) * (size_t)(*ld)));
+        CHK(dalloc(ctx, ell, (size_t)ell_width(*cap) * (size_t)(*ld)));
     }
     for (int attempt = 0; attempt < 8; attempt++) {
         HIPCHK(hipMemsetAsync(ctx->counters + 3, 0, sizeof(int), ctx->stream));
@@ -790,7 +790,7 @@ int build_list(gz_context *ctx, int nrows, const int *rows, const double *hrow, 
         if (over == 0) return 0;
         // widen and rebuild every row of this list (a narrower list cannot keep its rows)
         *cap = std::max(over + 16, *cap * 3 / 2);
-        CHK(dalloc(ctx, ell, (size_t)(*cap) * (size_t)(*ld)));
+        CHK(dalloc(ctx, ell, (size_t)ell_width(*cap) * (size_t)(*ld)));
         nrows = ctx->nact;
         rows = nullptr;
         ctx->st.ell_widen += 1;
@@ -1002,7 +1002,7 @@ int build_sched(gz_context *ctx)
 // would wait for the device each time)
 int sym_alloc(gz_context *ctx)
 {
-    const size_t need = (size_t)ctx->cap_s * (size_t)ctx->ld_s;
+    const size_t need = (size_t)ell_width(ctx->cap_s) * (size_t)ctx->ld_s;
     if (ctx->ell_s && ctx->ell_s_alloc >= need) return 0;
     CHK(dalloc(ctx, &ctx->ell_s, need));
     ctx->ell_s_alloc = need;
@@ -2188,7 +2188,7 @@ long gz_neighbors(gz_context *ctx, int symmetric, long *offsets, int *nbr, long 
     }
     if (rc) { dfree(ell); dfree(cnt); return -rc; }
     ctx->sym_valid = false;  // the grid changed
-    std::vector<int> c(ctx->nact), e((size_t)capl * ld);
+    std::vector<int> c(ctx->nact), e((size_t)ell_width(capl) * ld);
     std::vector<int> p(n);
     hipMemcpy(c.data(), cnt, ctx->nact * sizeof(int), hipMemcpyDeviceToHost);
     hipMemcpy(e.data(), ell, e.size() * sizeof(int), hipMemcpyDeviceToHost);
@@ -2206,7 +2206,7 @@ long gz_neighbors(gz_context *ctx, int symmetric, long *offsets, int *nbr, long 
         for (long u = 0; u < n; u++) {
             int r = slot_of[u];
             if (r < 0) continue;
-            for (int q = 0; q < c[r]; q++) nbr[offsets[u] + q] = p[e[(size_t)q * ld + r]];
+            for (int q = 0; q < c[r]; q++) nbr[offsets[u] + q] = p[e[ell_at(ld, r, q)]];
         }
     }
     return tot;

@@ -564,7 +564,7 @@ int sym_list(gz_batch *b, const std::vector<uint8_t> &in_pass)
     const int hint = std::max(64, (int)std::ceil(b->par.des_num_ngb * 2.0) + 32);
     if (!b->ell || b->ell_cap < hint) {
         b->ell_cap = std::max(b->ell_cap, hint);
-        BCHK(balloc(b, &b->ell, (size_t)b->ell_cap * ld_max));
+        BCHK(balloc(b, &b->ell, (size_t)ell_width(b->ell_cap) * ld_max));
     }
     for (int attempt = 0;; attempt++) {
         BHIP(hipMemsetAsync(b->counters + 3, 0, sizeof(int), b->stream));
@@ -582,7 +582,7 @@ int sym_list(gz_batch *b, const std::vector<uint8_t> &in_pass)
         if (over == 0) break;
         if (attempt == 7) return fail(b, GZ_E_STATE, "symmetric neighbour list did not fit after widening");
         b->ell_cap = std::max(over + 16, b->ell_cap * 3 / 2);
-        BCHK(balloc(b, &b->ell, (size_t)b->ell_cap * ld_max));
+        BCHK(balloc(b, &b->ell, (size_t)ell_width(b->ell_cap) * ld_max));
         b->stats["ell_widen"] += 1;
     }
     return 0;

@@ -37,6 +37,17 @@ struct DevParams {
     double fd4alpha = 0;
 };
 
+// Neighbour lists (ELL): entry q of row r lives in chunks of kEllChunk consecutive entries of one
+// row (one 64-byte line); chunk k of every row (ld rows apart), then chunk k+1.  A row's appends
+// fill its lines one after another, so a line is written once whatever the other rows do.
+constexpr int kEllChunk = 16;
+__host__ __device__ __forceinline__ size_t ell_at(long ld, long r, int q)
+{
+    return ((size_t)(q / kEllChunk) * (size_t)ld + (size_t)r) * kEllChunk + (size_t)(q % kEllChunk);
+}
+// entries to allocate per row for a list of width cap
+__host__ __device__ __forceinline__ int ell_width(int cap) { return (cap + kEllChunk - 1) / kEllChunk * kEllChunk; }
+
 // density accumulators per row: Ngb, Rho, DrkernNgb, DivVel, NV_T (6), face weights (3), and
 // with the first-derivative volume correction GradH_numer (3) and GradH_denom
 constexpr int kDensSums = 13;
@@ -596,6 +607,24 @@ __device__ inline R guess_p(const Gam<R> &G, const RState<R> &L, const RState<R>
 template <typename R>
 __device__ inline int iterative_riemann(const Gam<R> &G, const RState<R> &L, const RState<R> &Rs, R vL, R vR, R csL, R csR, ROut<R> &o)
 {
+    if constexpr (std::is_same<R, float>::value) {
+        // FP32 form: the Newton iteration in FP64.  Near gamma = 1 the FP32 pressure function
+        // (p^((gamma-1)/(2 gamma)) - 1) cancels to noise and the iteration would run to its
+        // 1000-step limit without reaching the 1e-6 tolerance; in FP64 it converges in a few
+        // steps (the rung is rare, its cost is not the pass's)
+        const double g = (double)G.G0;
+        const Gam<double> Gd{g, (g - 1.0) / (2.0 * g), (g + 1.0) / (2.0 * g), 2.0 * g / (g - 1.0), 2.0 / (g - 1.0), 2.0 / (g + 1.0),
+                             (g - 1.0) / (g + 1.0), 0.5 * (g - 1.0), 1.0 / g, g - 1.0};
+        const RState<double> Ld{(double)L.rho, (double)L.p, (double)L.v0, (double)L.v1, (double)L.v2};
+        const RState<double> Rd{(double)Rs.rho, (double)Rs.p, (double)Rs.v0, (double)Rs.v1, (double)Rs.v2};
+        ROut<double> od{0, 0, 0, 0, 0, 0};
+        const int ok = iterative_riemann<double>(Gd, Ld, Rd, (double)vL, (double)vR, (double)csL, (double)csR, od);
+        if (ok) {
+            o.P_M = (R)od.P_M;
+            o.S_M = (R)od.S_M;
+        }
+        return ok;
+    }
     R dvel = vR - vL;
     R check = G.G4 * (csR + csL) - dvel;
     if (check < 0) return 0;

@@ -7,9 +7,10 @@
 // the 27 neighbour-cell ranges of a cell, sorted by start, visit candidates in ascending
 // device index: the canonical summation order, shared with the CPU oracle.
 //
-// Neighbour lists are ELLPACK, column-major: entry q of row r sits at ell[q*ld + r], so the
-// 64 lanes of a wave (64 consecutive rows) read one coalesced line per q.  One lane per
-// active particle; wave64.
+// Neighbour lists are ELLPACK in row chunks: entry q of row r sits at ell[ell_at(ld, r, q)],
+// sixteen consecutive entries of a row in one 64-byte line (a row's appends fill whole lines;
+// a lane reading its row takes one line per sixteen entries, a workgroup reading one row four
+// lines per 64 entries).  One lane per active particle; wave64.
 #pragma once
 #include <type_traits>
 #include "gz_kernels.h"
@@ -331,7 +332,7 @@ __global__ void __launch_bounds__(256) k_build_ell(int nrows, const int *__restr
         for (int j = e.x; j < e.y; j++) {
             if (SYMMETRIC && j == i) continue;
             if (accept_pair<SYMMETRIC>(xi, yi, zi, X[j], Y[j], Z[j], hi, H[j], g.box, g.half)) {
-                if (c < cap) ell[(size_t)c * ld + r] = j;
+                if (c < cap) ell[ell_at(ld, r, c)] = j;
                 c++;
             }
         }
@@ -882,7 +883,7 @@ __global__ void __launch_bounds__(64) k_pack_grad_list(int nrows, const int *__r
     if (r >= nrows) return;
     const int c = cnt[r];
     for (int q = threadIdx.x; q <= c; q += 64) {
-        const int j = q == 0 ? act[r] : ell[(size_t)(q - 1) * ld + r];
+        const int j = q == 0 ? act[r] : ell[ell_at(ld, r, q - 1)];
         pack_grad_one<R>(j, H, M, RHO, PR, VX, VY, VZ, rec, X, Y, Z, box, bt);
     }
 }
@@ -897,7 +898,7 @@ __global__ void __launch_bounds__(64) k_pack_force_list(int nrows, const int *__
     if (r >= nrows) return;
     const int c = cnt[r];
     for (int q = threadIdx.x; q <= c; q += 64) {
-        const int j = q == 0 ? act[r] : ell[(size_t)(q - 1) * ld + r];
+        const int j = q == 0 ? act[r] : ell[ell_at(ld, r, q - 1)];
         pack_force_one<R>(j, gamma, F, rec, box, bt);
     }
 }
@@ -969,7 +970,7 @@ __global__ void __launch_bounds__(256) k_gradients(DevParams P, int nrows, const
         const double h2i = hid * hid;
         const int c = cnt[r];
         for (int q = 0; q < c; q++) {
-            const int j = ell[(size_t)q * ld + r];
+            const int j = ell[ell_at(ld, r, q)];
             const GradRec<R> J = rec[j];
             if (!(J.m > 0) || !(J.rho > 0) || !(J.h > 0)) continue;
             R dp[3];
@@ -1130,7 +1131,7 @@ __device__ __forceinline__ bool force_row(const DevParams &P0, const Gam<R> &G, 
         for (int q = 0; q < c; q++) {
             // LDS reads of i's record stay inside the loop (not hoisted into registers)
             if constexpr (split) __asm__ volatile("" ::: "memory");
-            const int j = a.ell[(size_t)q * a.ld + r];
+            const int j = a.ell[ell_at(a.ld, r, q)];
             if (m < I.cs) m = I.cs;
             const Cell<R> J = rec[j];
             R dp0, dp1, dp2;

@@ -374,7 +374,7 @@ __global__ void __launch_bounds__(kRowThreads) k_build_sym_rows(GridView g, int 
             if (q < w) off += cq;
             all += cq;
         }
-        if (acc && n + off + pos < cap) ell[(size_t)(n + off + pos) * ld + r] = j;
+        if (acc && n + off + pos < cap) ell[ell_at(ld, r, n + off + pos)] = j;
         n += all;
     }
     if (tid == 0) {
@@ -474,7 +474,7 @@ __global__ void __launch_bounds__(64) k_build_sym_wide(GridView g, int nrows, co
         }
         int cntk;
         const int pos = kept_rank(acc, cntk);
-        if (acc && n + pos < cap) ell[(size_t)(n + pos) * ld + r] = j;
+        if (acc && n + pos < cap) ell[ell_at(ld, r, n + pos)] = j;
         n += cntk;
     }
     const int nA = n;
@@ -499,7 +499,7 @@ __global__ void __launch_bounds__(64) k_build_sym_wide(GridView g, int nrows, co
             }
             int cntk;
             const int pos = kept_rank(acc, cntk);
-            if (acc && n + pos < cap) ell[(size_t)(n + pos) * ld + r] = j;
+            if (acc && n + pos < cap) ell[ell_at(ld, r, n + pos)] = j;
             n += cntk;
         }
     }
@@ -507,20 +507,20 @@ __global__ void __launch_bounds__(64) k_build_sym_wide(GridView g, int nrows, co
     if (n > nA && n <= cap) {
         // ascending order: each entry's rank among the row's distinct entries
         if (n <= 1024) {
-            for (int q = lane; q < n; q += 64) srt[q] = ell[(size_t)q * ld + r];
+            for (int q = lane; q < n; q += 64) srt[q] = ell[ell_at(ld, r, q)];
             __syncthreads();
             for (int q = lane; q < n; q += 64) {
                 const int e = srt[q];
                 int rank = 0;
                 for (int t = 0; t < n; t++) rank += srt[t] < e;
-                ell[(size_t)rank * ld + r] = e;
+                ell[ell_at(ld, r, rank)] = e;
             }
         } else if (lane == 0) {
             for (int q = 1; q < n; q++) {
-                const int e = ell[(size_t)q * ld + r];
+                const int e = ell[ell_at(ld, r, q)];
                 int t = q - 1;
-                while (t >= 0 && ell[(size_t)t * ld + r] > e) { ell[(size_t)(t + 1) * ld + r] = ell[(size_t)t * ld + r]; t--; }
-                ell[(size_t)(t + 1) * ld + r] = e;
+                while (t >= 0 && ell[ell_at(ld, r, t)] > e) { ell[ell_at(ld, r, t + 1)] = ell[ell_at(ld, r, t)]; t--; }
+                ell[ell_at(ld, r, t + 1)] = e;
             }
         }
     }
@@ -594,7 +594,7 @@ __global__ void __launch_bounds__(64) k_build_sym_cells(GridView g, CellRows cr,
                     const int jj = sj[k];
                     if (jj == i) continue;
                     if (accept_pair<true>(xi, yi, zi, sx[k], sy[k], sz[k], hi, sh[k], g.box, g.half)) {
-                        if (n < cap) ell[(size_t)n * ld + r] = jj;
+                        if (n < cap) ell[ell_at(ld, r, n)] = jj;
                         n++;
                     }
                 }
@@ -639,8 +639,8 @@ __global__ void __launch_bounds__(64) k_build_sym_cells(GridView g, CellRows cr,
                         if (jj == i || !accept_pair<true>(xi, yi, zi, sx[k], sy[k], sz[k], hi, sh[k], g.box, g.half)) continue;
                         if (n < cap) {
                             int q = n;
-                            while (q > 0 && ell[(size_t)(q - 1) * ld + r] > jj) { ell[(size_t)q * ld + r] = ell[(size_t)(q - 1) * ld + r]; q--; }
-                            ell[(size_t)q * ld + r] = jj;
+                            while (q > 0 && ell[ell_at(ld, r, q - 1)] > jj) { ell[ell_at(ld, r, q)] = ell[ell_at(ld, r, q - 1)]; q--; }
+                            ell[ell_at(ld, r, q)] = jj;
                         }
                         n++;
                     }
@@ -778,7 +778,7 @@ __global__ void __launch_bounds__(TPB) k_gradients_rows(DevParams P, int nrows, 
             bool ok = false;
             R w3[3] = {0, 0, 0}, d5[5], rr = 0;
             if (q < c) {
-                const int j = ell[(size_t)q * ld + r];
+                const int j = ell[ell_at(ld, r, q)];
                 const GradRec<R> J = src.H ? make_grad_rec<R>(j, src, X, Y, Z, bt) : rec[j];
                 if ((J.m > 0) && (J.rho > 0) && (J.h > 0)) {
                     R dp[3];
@@ -897,7 +897,7 @@ __global__ void __launch_bounds__(TPB) k_force_rows(DevParams P, Gam<R> G, int n
             int rc = 0;
             R f0, f1, f2, fp, vts;
             if (q < c) {
-                const int j = ell[(size_t)q * ld + r];
+                const int j = ell[ell_at(ld, r, q)];
                 const Cell<R> J = make ? force_rec_of<R>(j, P.gamma, F, P.box, bt) : rec[j];
                 R dp0, dp1, dp2;
                 record_dp<R>(P, I, J, xi, yi, zi, F.X, F.Y, F.Z, j, dp0, dp1, dp2);
