@@ -2165,7 +2165,7 @@ long gz_neighbors(gz_context *ctx, int symmetric, long *offsets, int *nbr, long 
         ld = ((long)ctx->nact + 63) / 64 * 64;
         capl = 64;
         for (int attempt = 0; !rc && attempt < 12; attempt++) {
-            rc = dalloc(ctx, &ell, (size_t)capl * ld);
+            rc = dalloc(ctx, &ell, (size_t)ell_width(capl) * ld);
             if (rc) break;
             ctx->overflow_clear = false;
             if (hipMemsetAsync(ctx->counters + 3, 0, sizeof(int), ctx->stream) != hipSuccess) { rc = GZ_E_HIP; break; }

@@ -38,9 +38,13 @@ struct DevParams {
 };
 
 // Neighbour lists (ELL): entry q of row r lives in chunks of kEllChunk consecutive entries of one
-// row (one 64-byte line); chunk k of every row (ld rows apart), then chunk k+1.  A row's appends
-// fill its lines one after another, so a line is written once whatever the other rows do.
-constexpr int kEllChunk = 16;
+// row; chunk k of every row (ld rows apart), then chunk k+1.  kEllChunk = 1 is column-major: the
+// lanes of a wave, one row each, read entry q from whole lines shared by 16 rows.  Chunks of 16
+// (one line per row chunk) write each line once as a row appends, but lane-per-row readers then
+// touch 64 lines per load and fetch them again as the gathered records evict them (measured on
+// the 2M all-active pass: list writes 3.8 -> 1.1 GB, gradient and force fetches up to 10x, the
+// pass 0.7 ms slower; the list build was no faster).
+constexpr int kEllChunk = 1;
 __host__ __device__ __forceinline__ size_t ell_at(long ld, long r, int q)
 {
     return ((size_t)(q / kEllChunk) * (size_t)ld + (size_t)r) * kEllChunk + (size_t)(q % kEllChunk);

@@ -7,10 +7,9 @@
 // the 27 neighbour-cell ranges of a cell, sorted by start, visit candidates in ascending
 // device index: the canonical summation order, shared with the CPU oracle.
 //
-// Neighbour lists are ELLPACK in row chunks: entry q of row r sits at ell[ell_at(ld, r, q)],
-// sixteen consecutive entries of a row in one 64-byte line (a row's appends fill whole lines;
-// a lane reading its row takes one line per sixteen entries, a workgroup reading one row four
-// lines per 64 entries).  One lane per active particle; wave64.
+// Neighbour lists are ELLPACK: entry q of row r sits at ell[ell_at(ld, r, q)], column-major
+// (the 64 rows of a wave read entry q from four 64-byte lines).  One lane per active particle;
+// wave64.
 #pragma once
 #include <type_traits>
 #include "gz_kernels.h"

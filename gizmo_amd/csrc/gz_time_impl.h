@@ -143,9 +143,22 @@ __global__ void __launch_bounds__(256) k_find_timesteps(TimeParams T, ParticleVi
     if (count && threadIdx.x < 64) moved[threadIdx.x] = 0;
     if (count) __syncthreads();
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    double vkmax_row = 0;  // the row's largest kicked velocity component after its kick
     if (r < nrows) {
         const int i = act[r];
         resolve_time(bt, i, T);
+        // the row's inputs loaded ahead of its stores (for the compiler the stores may alias
+        // them, so loads after them would wait for each other in turn)
+        const long n = S.n;
+        const int binold = S.TB[i];
+        const long long tib0 = S.TIB[i], dts0 = S.DTS[i];
+        const double m = S.M[i];
+        double dtu = 0, uk = 0, a[3] = {0, 0, 0}, vk[3] = {0, 0, 0};
+        if (kick_first) {
+            vk[0] = S.VKX[i]; vk[1] = S.VKY[i]; vk[2] = S.VKZ[i];
+            dtu = S.DTU[i]; uk = S.UK[i];
+            a[0] = S.ACC[i]; a[1] = S.ACC[n + i]; a[2] = S.ACC[2 * n + i];
+        }
         bool zero_step;
         long long ti_step = get_timestep(T, S, i, zero_step);
         if (zero_step) {
@@ -157,7 +170,6 @@ __global__ void __launch_bounds__(256) k_find_timesteps(TimeParams T, ParticleVi
         while (ti_min > ti_step) ti_min >>= 1;
         ti_step = ti_min;
         int bin = timestep_bin(ti_step);
-        const int binold = S.TB[i];
         if (bin > binold) {
             while (!bin_on(T.active_mask, bin) && bin > binold) bin--;
             ti_step = bin_len(bin);
@@ -172,18 +184,42 @@ __global__ void __launch_bounds__(256) k_find_timesteps(TimeParams T, ParticleVi
             ti_step = ti_min;
         }
         S.TB[i] = bin;
-        S.TIB[i] += S.DTS[i];
+        const long long tib = tib0 + dts0;
+        S.TIB[i] = tib;
         S.DTS[i] = ti_step;
         // do_first_halfstep_kick (src/core/kicks.cc:34-97) of the row, as k_halfstep_kick takes
-        // it: it reads only the row's own fields, which process_wake_ups (inactive particles
-        // only) does not touch
-        if (kick_first && S.M[i] > 0) kick(T, S, i, S.TIB[i], S.TIB[i] + S.DTS[i] / 2, 0);
+        // it (kick(), mode 0, on the values loaded above): it reads only the row's own fields,
+        // which process_wake_ups (inactive particles only) does not touch
+        if (kick_first && m > 0 && bin_on(T.active_mask, bin)) {
+            const double dt_hydrokick = ((tib + ti_step / 2) - tib) * T.tbi;
+            const double du_tot = dtu * dt_hydrokick + 0.0;
+            const double dEnt = uk + du_tot;
+            double u = positivity(uk, dEnt);
+            if (T.min_egy) {
+                if (u < T.min_egy) { u = T.min_egy; S.DTU[i] = 0; }
+            }
+            S.UK[i] = u;
+            const double fac = m * 1.0 * dt_hydrokick;
+            const double inv = 1.0 / m;
+            for (int k = 0; k < 3; k++) {
+                double dp = 0.0;
+                dp += a[k] * fac;
+                vk[k] += dp * inv;
+            }
+            S.VKX[i] = vk[0]; S.VKY[i] = vk[1]; S.VKZ[i] = vk[2];
+        }
+        if (kick_first) vkmax_row = fmax(fabs(vk[0]), fmax(fabs(vk[1]), fabs(vk[2])));
         if (count && bin != binold) {
             atomicAdd(&moved[binold & 63], -1);
             atomicAdd(&moved[bin & 63], 1);
         }
     }
-    if (S.vkmax) raise_vk_bound(S, r < nrows ? act[r] : 0, r < nrows && kick_first);
+    if (S.vkmax) {
+        // the bound on the kicked velocities (raise_vk_bound on the values in hand)
+        double mv = vkmax_row;
+        for (int o = 32; o > 0; o >>= 1) mv = fmax(mv, __shfl_xor(mv, o, 64));
+        if ((threadIdx.x & 63) == 0 && mv > 0) atomic_max_bits(S.vkmax, mv);
+    }
     if (!count) return;
     __syncthreads();
     if (threadIdx.x < 64 && moved[threadIdx.x]) atomicAdd(&count[threadIdx.x], (unsigned long long)(long long)moved[threadIdx.x]);
