@@ -108,6 +108,10 @@ struct gz_context {
     // the force pass's row order over the lanes (longest lists first within blocks of rows)
     int *sched = nullptr;
     int *rare = nullptr;  // rows the force pass's common kernel leaves to the whole Riemann ladder
+    // the rows still iterating in the density rounds of the row kernels: rl_a holds the coming
+    // round's rows (hcounters[0] of them) when rl_valid, rl_b receives the next round's
+    int *rl_a = nullptr, *rl_b = nullptr;
+    bool rl_valid = false;
     unsigned long long *sched_part = nullptr;
     int sched_span = 2048;  // rows per sorted block (0: plain order)
     bool sched_valid = false;
@@ -418,8 +422,9 @@ void free_state(gz_context *c)
     dfree(c->wide_dev);
     {
         dfree(c->act_bits); dfree(c->act_bcount);
-        dfree(c->sched); dfree(c->sched_part); dfree(c->rare);
+        dfree(c->sched); dfree(c->sched_part); dfree(c->rare); dfree(c->rl_a); dfree(c->rl_b);
         c->sched_valid = false;
+        c->rl_valid = false;
         RowGroups &G = c->grp;
         dfree(G.key); dfree(G.skey); dfree(G.gkey); dfree(G.val); dfree(G.row); dfree(G.flag); dfree(G.scan); dfree(G.start);
         dfree(G.count); dfree(G.lev); dfree(G.code); dfree(G.nst); dfree(G.sten);
@@ -890,10 +895,20 @@ int density_rounds(gz_context *ctx, int iters, bool speculate)
     double hmax = 0;
     int remaining = iters > 0 ? ctx->hcounters[0] : na;  // rows of the coming round
     bool read = iters > 0;  // the counters of round `iters` are on the host already
+    // the row kernels' rounds after the first take only the rows still iterating (a list the
+    // closing kernel writes), not a workgroup per row that mostly exits at once
+    if (few && !ctx->rl_a) {
+        CHK(dalloc(ctx, &ctx->rl_a, ctx->cap));
+        CHK(dalloc(ctx, &ctx->rl_b, ctx->cap));
+    }
+    if (iters == 0) ctx->rl_valid = false;
     while (true) {
         if (!read) {
             // the round's counters (k_density_init, or the step's start, cleared them for the first round)
             if (iters > 0) HIPCHK(hipMemsetAsync(ctx->counters, 0, 3 * sizeof(int), ctx->stream));
+            const int *rows = (few && ctx->rl_valid) ? ctx->rl_a : nullptr;
+            const int nround = rows ? remaining : na;
+            int *next_rows = few ? ctx->rl_b : nullptr;
             // over many levels the rounds take one workgroup per row whatever their number (the
             // cells of the rows' own levels hold too few rows to fill a wavefront): four
             // wavefronts per row while few rows iterate, one while many do; with timing on, the
@@ -902,30 +917,33 @@ int density_rounds(gz_context *ctx, int iters, bool speculate)
             {
                 Timed t(ctx, "density");
                 if (few && remaining <= ctx->few_rows)
-                    k_density_rows<KERN, R, kRowThreads><<<xcd_grid(na), kRowThreads, 0, ctx->stream>>>(
+                    k_density_rows<KERN, R, kRowThreads><<<xcd_grid(nround), kRowThreads, 0, ctx->stream>>>(
                         ctx->dp, grid_view(ctx), na, ctx->act, ctx->need, ctx->key, ctx->X, ctx->Y, ctx->Z, ctx->VX, ctx->VY, ctx->VZ,
-                        ctx->M, ctx->it_h, ctx->dacc, BoxTable{}, npairs);
+                        ctx->M, ctx->it_h, ctx->dacc, BoxTable{}, npairs, rows, nround);
                 else if (few)
-                    k_density_rows<KERN, R, 64><<<xcd_grid(na), 64, 0, ctx->stream>>>(
+                    k_density_rows<KERN, R, 64><<<xcd_grid(nround), 64, 0, ctx->stream>>>(
                         ctx->dp, grid_view(ctx), na, ctx->act, ctx->need, ctx->key, ctx->X, ctx->Y, ctx->Z, ctx->VX, ctx->VY, ctx->VZ,
-                        ctx->M, ctx->it_h, ctx->dacc, BoxTable{}, npairs);
+                        ctx->M, ctx->it_h, ctx->dacc, BoxTable{}, npairs, rows, nround);
                 else
                     k_density_cells<KERN, R><<<xcd_grid(ctx->ncells), 64, 0, ctx->stream>>>(
                         ctx->dp, grid_view(ctx), cell_rows(ctx), ctx->act, ctx->need, ctx->X, ctx->Y, ctx->Z, ctx->VX, ctx->VY, ctx->VZ,
                         ctx->M, ctx->it_h, ctx->dacc);
             }
-            if (npairs) k_sum_counts<<<std::min(nblk(na), 256u), kBlock, 0, ctx->stream>>>(na, npairs, ctx->u64c + 77);
+            if (npairs) k_sum_counts<<<std::min(nblk(nround), 256u), kBlock, 0, ctx->stream>>>(nround, npairs, ctx->u64c + 77);
             // the speculative first round: its closing launch also writes the pass's gate word
             const bool gate_now = speculate && iters == 0;
             {
                 Timed t(ctx, "density_after");
-                k_density_after<<<nblk(na), kBlock, 0, ctx->stream>>>(ctx->dp, na, ctx->act, ctx->need, ctx->dacc, ctx->M, ctx->U,
-                                                                      wide ? 1e300 : ctx->cell - 2.0 * ctx->dmax, it, out, ctx->counters + 0,
-                                                                      ctx->counters + 1, ctx->counters + 2, ctx->u64c + 68, BoxTable{},
-                                                                      ctx->hmax_bound, gate_now ? ctx->counters + kPassGate : nullptr,
-                                                                      (unsigned int *)(ctx->u64c + 79));
+                k_density_after<<<nblk(nround), kBlock, 0, ctx->stream>>>(
+                    ctx->dp, na, ctx->act, ctx->need, ctx->dacc, ctx->M, ctx->U, wide ? 1e300 : ctx->cell - 2.0 * ctx->dmax, it, out,
+                    ctx->counters + 0, ctx->counters + 1, ctx->counters + 2, ctx->u64c + 68, BoxTable{}, ctx->hmax_bound,
+                    gate_now ? ctx->counters + kPassGate : nullptr, (unsigned int *)(ctx->u64c + 79), rows, nround, next_rows);
             }
             HIPCHK(hipGetLastError());
+            if (few) {
+                std::swap(ctx->rl_a, ctx->rl_b);
+                ctx->rl_valid = true;
+            }
             if (gate_now) {
                 ctx->spec = true;
                 ctx->st.density_iterations = 1;

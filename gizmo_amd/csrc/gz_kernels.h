@@ -296,7 +296,7 @@ __global__ void k_density_init(int na, const int *act, double *H, double hclamp,
 __global__ void k_density_after(DevParams P, int nrows, const int *act, uint8_t *need, const double *accin, const double *M,
                                 const double *U, double cell, DensityIter it, DensityOut out, int *nnext, int *rebuild, int *err,
                                 unsigned long long *hfinal, BoxTable bt, double hlim = 0, int *gate = nullptr,
-                                unsigned int *done = nullptr);
+                                unsigned int *done = nullptr, const int *rows = nullptr, int nlist = 0, int *next_rows = nullptr);
 
 template <typename R>
 __global__ void k_pack_grad(long n, const double *H, const double *M, const double *RHO, const double *PR, const double *VX,
@@ -328,7 +328,7 @@ template <int KERN, typename R, int TPB>
 __global__ void k_density_rows(DevParams P, GridView g, int nrows, const int *act, const uint8_t *need, const uint32_t *key,
                                const double *X, const double *Y, const double *Z, const double *VX, const double *VY,
                                const double *VZ, const double *M, const double *hcur, double *accout, BoxTable bt,
-                               int *npairs = nullptr);
+                               int *npairs = nullptr, const int *rows = nullptr, int nlist = 0);
 __global__ void k_build_sym_rows(GridView g, int nrows, const int *act, const uint32_t *key, const double *X, const double *Y,
                                  const double *Z, const double *H, int *ell, long ld, int cap, int *cnt, int *overflow, BoxTable bt);
 // the fields a gradient record is made of (H null: the pass reads packed records)

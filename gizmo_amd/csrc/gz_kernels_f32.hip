@@ -24,7 +24,7 @@ namespace gz {
     template __global__ void k_density_rows<KERN, R, TPB>(DevParams, GridView, int, const int *, const uint8_t *,               \
                                                           const uint32_t *, const double *, const double *, const double *,     \
                                                           const double *, const double *, const double *, const double *,       \
-                                                          const double *, double *, BoxTable, int *);
+                                                          const double *, double *, BoxTable, int *, const int *, int);
 
 #define GZ_INST(KERN, R)                                                                                                         \
     template __global__ void k_density_cells<KERN, R>(DevParams, GridView, CellRows, const int *, const uint8_t *,              \

@@ -701,10 +701,14 @@ __global__ void __launch_bounds__(256) k_density_after(DevParams P, int nrows, c
                                                        const double *__restrict__ U, double cell, DensityIter it, DensityOut out,
                                                        int *__restrict__ nnext, int *__restrict__ rebuild, int *__restrict__ err,
                                                        unsigned long long *__restrict__ hfinal, BoxTable bt, double hlim,
-                                                       int *__restrict__ gate, unsigned int *__restrict__ done)
+                                                       int *__restrict__ gate, unsigned int *__restrict__ done,
+                                                       const int *__restrict__ rows, int nlist, int *__restrict__ next_rows)
 {
-    // every lane reaches the wave-wide counts and maxima at the end
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    // every lane reaches the wave-wide counts and maxima at the end.  Lane t takes row t, or the
+    // t-th row of the list of rows of this round (rows, nlist); next_rows (may be null) receives
+    // the rows that iterate again, in any order (each row's sums do not depend on the others)
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int r = rows ? (t < nlist ? rows[t] : nrows) : t;
     const bool row = r < nrows && need[r];
     double h_final = 0, h_grow = 0;
     bool cont = false;
@@ -756,10 +760,12 @@ __global__ void __launch_bounds__(256) k_density_after(DevParams P, int nrows, c
     } else {
         block_max_to(h_final, hfinal);  // synchronises the block
     }
+    __shared__ int wbase[16];
     if (threadIdx.x == 0) {
         int c = 0, g = 0;
-        for (int k = 0; k < (int)((blockDim.x + 63) >> 6); k++) { c += wcont[k]; g |= wgrow[k]; }
-        if (c) atomicAdd(nnext, c);
+        for (int k = 0; k < (int)((blockDim.x + 63) >> 6); k++) { wbase[k] = c; c += wcont[k]; g |= wgrow[k]; }
+        const int base = c ? atomicAdd(nnext, c) : 0;
+        for (int k = 0; k < (int)((blockDim.x + 63) >> 6); k++) wbase[k] += base;
         if (g) atomicOr(rebuild, 1);
         if (gate) {
             // the last block to finish, when every block's counts and maxima are in, writes
@@ -771,11 +777,19 @@ __global__ void __launch_bounds__(256) k_density_after(DevParams P, int nrows, c
             __threadfence();
             if (atomicAdd(done, 1u) == gridDim.x - 1) {
                 __threadfence();
-                const int e = atomicOr(err, 0), rows = atomicOr(nnext, 0);
+                const int e = atomicOr(err, 0), nrem = atomicOr(nnext, 0);
                 const double hf = __longlong_as_double((long long)atomicOr(hfinal, 0ull));
-                *gate = e ? e : (rows ? -1 : (hf > hlim ? -2 : 0));
+                *gate = e ? e : (nrem ? -1 : (hf > hlim ? -2 : 0));
                 *done = 0u;
             }
+        }
+    }
+    if (next_rows) {
+        // the rows that iterate again, each wave's at the place its block reserved
+        __syncthreads();
+        if (cont) {
+            const unsigned long long below = (1ull << (threadIdx.x & 63)) - 1ull;
+            next_rows[wbase[threadIdx.x >> 6] + (int)__popcll(cm & below)] = r;
         }
     }
 }

@@ -217,15 +217,19 @@ __global__ void __launch_bounds__(TPB) k_density_rows(DevParams P, GridView g, i
                                                      const double *__restrict__ Z, const double *__restrict__ VX,
                                                      const double *__restrict__ VY, const double *__restrict__ VZ,
                                                      const double *__restrict__ M, const double *__restrict__ hcur,
-                                                     double *__restrict__ accout, BoxTable bt, int *__restrict__ npairs)
+                                                     double *__restrict__ accout, BoxTable bt, int *__restrict__ npairs,
+                                                     const int *__restrict__ rows, int nlist)
 {
     __shared__ RowStencil rs;
     __shared__ R sc[kDensSumsFD][TPB + 1];  // the accepted candidates' terms, in candidate order (padded: no bank conflicts)
     __shared__ unsigned long long smask[TPB / 64];
-    const long r = xcd_block();
-    if (r >= nrows) return;
+    // workgroup b takes row b, or the b-th row of the list of rows still iterating; the pair
+    // count goes to npairs[b]
+    const long b = xcd_block();
+    if (b >= (rows ? nlist : nrows)) return;
+    const long r = rows ? rows[b] : b;
     if (!need[r]) {
-        if (npairs && threadIdx.x == 0) npairs[r] = 0;
+        if (npairs && threadIdx.x == 0) npairs[b] = 0;
         return;
     }
     const int tid = threadIdx.x;
@@ -330,7 +334,7 @@ __global__ void __launch_bounds__(TPB) k_density_rows(DevParams P, GridView g, i
         __syncthreads();
     }
     if (tid < nsum) accout[(size_t)tid * nrows + r] = (double)acc;
-    if (tid == kDensSumsFD && npairs) npairs[r] = accepted;
+    if (tid == kDensSumsFD && npairs) npairs[b] = accepted;
 }
 
 // the symmetric list of few rows (the same entries, in the same order, as k_build_sym_cells)
