@@ -188,7 +188,7 @@ __global__ void k_next_sync(TimeParams T, const unsigned long long *count, unsig
 __global__ void k_active_flags(long n, const unsigned long long *tl, const int *TB, const double *M, const double *H, double hclamp,
                                uint8_t *ACT, double *SOFT, int *flag, unsigned long long *hmax_act, BoxTable bt);
 __global__ void k_compact_count(long n, const int *flag, const int *scan, int *act, int *count_out);
-constexpr int kActChunk = 512;  // particles per block of the active-list kernels (and of the drift they are fused with)
+constexpr int kActChunk = 2048;  // particles per block of the active-list kernels (and of the drift they are fused with)
 // count (may be null): the bin occupation, from which every block takes the next sync point
 // itself (k_next_sync's work; block 0 writes it to tl) instead of reading tl
 __global__ void k_drift_active(TimeParams T, ParticleView S, long long time0, unsigned long long *tl,

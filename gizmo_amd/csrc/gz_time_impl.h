@@ -626,6 +626,19 @@ __global__ void __launch_bounds__(256) k_drift_active(TimeParams T, ParticleView
 {
     __shared__ unsigned long long s_tl[3];
     const unsigned long long *count_bins = occupation;
+    const long base = (long)blockIdx.x * kActChunk;
+    const int tid = threadIdx.x, w = tid >> 6;
+    // every time bin of the thread's particles loaded first, before any store (the stores may
+    // alias them for the compiler, and a load issued after a store waits for it) and while the
+    // next sync point is worked out: the sweep takes one memory round trip, not one per
+    // particle; tbs[0] is the current particle's, shifted on
+    constexpr int kPer = kActChunk / 256;
+    int tbs[kPer];
+#pragma unroll
+    for (int k = 0; k < kPer; k++) {
+        const long i = base + k * 256 + tid;
+        tbs[k] = i < S.n ? S.TB[i] : 0;
+    }
     if (count_bins) {
         // the next sync point from the occupation, in every block (k_next_sync's work; the
         // first block also writes it for the host and the later launches)
@@ -643,17 +656,16 @@ __global__ void __launch_bounds__(256) k_drift_active(TimeParams T, ParticleView
     const long long time1 = (long long)(count_bins ? s_tl[0] : tl[0]);
     const unsigned long long mask = count_bins ? s_tl[1] : tl[1];
     const bool full = (count_bins ? s_tl[2] : tl[2]) != 0;
-    const long base = (long)blockIdx.x * kActChunk;
-    const int tid = threadIdx.x, w = tid >> 6;
     double d1 = 0, h1 = 0, hm = 0;
     int count = 0;
-    for (int k = 0; k < kActChunk / 256; k++) {
+    for (int k = 0; k < kPer; k++) {
         const long i = base + k * 256 + tid;
+        const int tb = tbs[0];
+#pragma unroll
+        for (int q = 0; q + 1 < kPer; q++) tbs[q] = tbs[q + 1];
         bool row = false;
         if (i < S.n) {
-            // the flag's inputs loaded before the drift's stores (which may alias them for
-            // the compiler) and the drifted radius taken from the drift
-            const int tb = S.TB[i];
+            // the drifted radius taken from the drift
             const bool on = bin_on(mask, tb);
             if (full || on) {
                 const double m = S.M[i];
