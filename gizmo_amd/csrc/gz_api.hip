@@ -75,6 +75,7 @@ struct gz_context {
     bool spec_enabled = true;  // option "speculate"
     bool fd = false;           // option "fd_correction": the first-derivative volume correction of the density
     int force_split = -1;      // option "force_split": -1 by the last pass's rare share, 0 one kernel, 1 two kernels
+    int force_waves = 0;       // option "force_waves": the FP64 common force kernel's waves per SIMD (0: by wide_search)
     double rare_share = 0;     // share of the rows the last two-kernel force pass left to k_force_rare
     bool split_ran = false;
     bool force_flags_clear = false;  // the last gradient launch cleared the force pass's flags
@@ -1196,7 +1197,18 @@ int force_stage(gz_context *ctx)
             } else {
                 // the rows whose faces all take the Riemann ladder's common case, then the others
                 // with the whole ladder
-                k_force<KERN, R, false, false, true><<<grid, kBlock, 0, ctx->stream>>>(ctx->dp, gam<R>(ctx->dp), F, rec, a, BoxTable{});
+                // FP64: two or three waves per SIMD by the option "force_waves" (default: three
+                // with the search over many levels, else two)
+                const int waves = ctx->force_waves ? ctx->force_waves : (ctx->wide ? 3 : 2);
+                if constexpr (std::is_same<R, double>::value) {
+                    if (waves == 3)
+                        k_force_w3<KERN><<<grid, kBlock, 0, ctx->stream>>>(ctx->dp, gam<R>(ctx->dp), F, rec, a, BoxTable{});
+                    else
+                        k_force<KERN, R, false, false, true><<<grid, kBlock, 0, ctx->stream>>>(ctx->dp, gam<R>(ctx->dp), F, rec, a,
+                                                                                                BoxTable{});
+                } else {
+                    k_force<KERN, R, false, false, true><<<grid, kBlock, 0, ctx->stream>>>(ctx->dp, gam<R>(ctx->dp), F, rec, a, BoxTable{});
+                }
                 k_force_rare<KERN, R, false><<<std::min(nblk(ctx->nact), kRareBlocks), kBlock, 0, ctx->stream>>>(
                     ctx->dp, gam<R>(ctx->dp), F, rec, a, BoxTable{});
             }
@@ -2088,6 +2100,11 @@ int gz_set_option(gz_context *ctx, const char *name, double value)
     if (k == "speculate") {
         if (value != 0 && value != 1) return fail(ctx, GZ_E_ARG, "speculate must be 0 or 1");
         ctx->spec_enabled = value != 0;
+        return 0;
+    }
+    if (k == "force_waves") {
+        if (value != 0 && value != 2 && value != 3) return fail(ctx, GZ_E_ARG, "force_waves must be 0, 2 or 3");
+        ctx->force_waves = (int)value;
         return 0;
     }
     if (k == "force_split") {

@@ -380,6 +380,9 @@ struct ForceRows {
 // then carries no register cost of the rare rungs.
 template <int KERN, typename R, bool BOXES, bool STATS = false, bool COMMON = false>
 __global__ void k_force(DevParams P, Gam<R> G, ForceView F, const Cell<R> *rec, ForceRows a, BoxTable bt);
+// k_force<KERN, double, false, false, true> at three waves per SIMD instead of two (option "force_waves")
+template <int KERN>
+__global__ void k_force_w3(DevParams P, Gam<double> G, ForceView F, const Cell<double> *rec, ForceRows a, BoxTable bt);
 
 // the rows that k_force<..., COMMON> listed, each with the whole ladder (a fixed grid strides over
 // the count the device holds; the rows' results are the single-pass kernel's, bit for bit)

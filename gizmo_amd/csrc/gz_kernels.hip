@@ -53,6 +53,8 @@ template __global__ void k_build_ell<true>(int, const int *, const int *, const 
                                                    int *, unsigned long long *, BoxTable, const int *);
 GZ_INST(3, double)
 GZ_INST(6, double)
+template __global__ void k_force_w3<3>(DevParams, Gam<double>, ForceView, const Cell<double> *, ForceRows, BoxTable);
+template __global__ void k_force_w3<6>(DevParams, Gam<double>, ForceView, const Cell<double> *, ForceRows, BoxTable);
 // the force kernels that count the Riemann rungs of their faces (one box)
 #define GZ_INST_STATS(KERN, R)                                                                                                   \
     template __global__ void k_force<KERN, R, false, true, false>(DevParams, Gam<R>, ForceView, const Cell<R> *, ForceRows,   \

@@ -1179,11 +1179,10 @@ __device__ __forceinline__ bool force_row(const DevParams &P0, const Gam<R> &G, 
     return true;
 }
 
-// the FP64 common pass of one box at three waves per SIMD (166 registers; its LDS record: 52 KB
-// per workgroup); with the box table it takes 178 (two waves, no spill)
+// one lane per row of k_force and k_force_w3 (the kernels differ in their launch bounds only)
 template <int KERN, typename R, bool BOXES, bool STATS, bool COMMON>
-__global__ void __launch_bounds__(256, (COMMON && !BOXES && std::is_same<R, double>::value) ? 3 : 1) k_force(DevParams P, Gam<R> G, ForceView F, const Cell<R> *__restrict__ rec, ForceRows a,
-                                               BoxTable bt)
+__device__ __forceinline__ void force_lanes(const DevParams &P, const Gam<R> &G, const ForceView &F, const Cell<R> *__restrict__ rec,
+                                            const ForceRows &a, const BoxTable &bt)
 {
     const int k = (int)(xcd_block() * blockDim.x + threadIdx.x);
     const int r = k < a.nrows ? (a.sched ? a.sched[k] : k) : a.nrows;
@@ -1204,6 +1203,26 @@ __global__ void __launch_bounds__(256, (COMMON && !BOXES && std::is_same<R, doub
     if (lane == lead) base = atomicAdd(a.err + 12, __popcll(left));
     base = __shfl(base, lead, 64);
     if (!done) a.rare[base + __popcll(left & ((1ull << lane) - 1ull))] = r;
+}
+
+// the FP64 common pass of one box at two waves per SIMD (172 registers, no scratch; its LDS
+// record: 52 KB per workgroup); with the box table it takes 178 (two waves, no spill)
+template <int KERN, typename R, bool BOXES, bool STATS, bool COMMON>
+__global__ void __launch_bounds__(256, (COMMON && !BOXES && std::is_same<R, double>::value) ? 2 : 1) k_force(DevParams P, Gam<R> G, ForceView F, const Cell<R> *__restrict__ rec, ForceRows a,
+                                               BoxTable bt)
+{
+    force_lanes<KERN, R, BOXES, STATS, COMMON>(P, G, F, rec, a, bt);
+}
+
+// the FP64 common pass of one box at three waves per SIMD (166 registers).  Which is faster
+// depends on the field: on a 2M-row lattice pass the rows resident at three waves outgrow the
+// L2 with their neighbour records (7.3 ms against 6.7 at two); on the 4M lognormal field of
+// radii over many levels three waves hide more of the loads' latency (20.2 ms against 23.8)
+template <int KERN>
+__global__ void __launch_bounds__(256, 3) k_force_w3(DevParams P, Gam<double> G, ForceView F, const Cell<double> *__restrict__ rec,
+                                                     ForceRows a, BoxTable bt)
+{
+    force_lanes<KERN, double, false, false, true>(P, G, F, rec, a, bt);
 }
 
 template <int KERN, typename R, bool BOXES>

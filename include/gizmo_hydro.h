@@ -235,7 +235,9 @@ int gz_drift_all(gz_context *ctx);
    over more than few_rows rows runs in two kernels, the first evaluating every face through the
    Riemann ladder's common case and leaving rows with a face that needs a later rung to the
    second (1), in one kernel with the whole ladder (0), or (-1) in two unless it is the FP32 form
-   and more than a quarter of the rows took the second kernel in the last two-kernel pass.
+   and more than a quarter of the rows took the second kernel in the last two-kernel pass;
+   "force_waves" (default 0; 2 or 3) -- the FP64 form's first force kernel of those two runs at
+   this many waves per SIMD (0: three with wide_search, else two).
    None of these changes a neighbour set or a
    sum, and none has a counterpart in the reference (tuning knobs of this library).
    One option selects physics, the reference's compile-time HYDRO_PARTITION_UNITY_IMPROVE_FD

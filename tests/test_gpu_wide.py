@@ -72,7 +72,8 @@ def test_full_pass_same_bits_as_one_grid(wide_case):
     # one grid; many levels with the default split; every list in row groups; every list one
     # wavefront per row; the force pass's lanes in plain row order
     cases = {"one grid": (0, {}), "wide": (1, {}), "wide groups": (1, dict(few_rows=0)), "wide rows": (1, dict(few_rows=10**9)),
-             "wide rows in order": (1, dict(row_schedule=0)), "one grid rows in order": (0, dict(row_schedule=0))}
+             "wide rows in order": (1, dict(row_schedule=0)), "one grid rows in order": (0, dict(row_schedule=0)),
+             "wide, force at two waves": (1, dict(force_waves=2)), "one grid, force at three waves": (0, dict(force_waves=3))}
     for name, (wide, opts) in cases.items():
         print(f"full pass: {name}", flush=True)
         g = device(ic, par, wide, **opts)
