@@ -1225,8 +1225,11 @@ __global__ void __launch_bounds__(256, 3) k_force_w3(DevParams P, Gam<double> G,
     force_lanes<KERN, double, false, false, true>(P, G, F, rec, a, bt);
 }
 
+// two waves per SIMD: the whole ladder then spills a few registers (FP64 ~300 wanted) but hides
+// more latency than one wave of the unspilled kernel (4M lognormal pass: FP64 93.4 -> 88.0 ms,
+// FP32 68.9 -> 64.5 ms)
 template <int KERN, typename R, bool BOXES>
-__global__ void __launch_bounds__(256) k_force_rare(DevParams P, Gam<R> G, ForceView F, const Cell<R> *__restrict__ rec, ForceRows a,
+__global__ void __launch_bounds__(256, 2) k_force_rare(DevParams P, Gam<R> G, ForceView F, const Cell<R> *__restrict__ rec, ForceRows a,
                                                     BoxTable bt)
 {
     const int count = a.err[12];
