@@ -885,6 +885,8 @@ int density_stage(gz_context *ctx)
 // counters of round iters were read back by a later read of the step).  speculate: the first
 // round's closing launch writes a gate word and is not waited for; the passes follow, and the
 // force pass's read of the counters settles the step (force_stage).
+constexpr int kListedRounds = 16384;  // density rounds over more rows than this take the list of rows still iterating
+
 template <int KERN, typename R>
 int density_rounds(gz_context *ctx, int iters, bool speculate)
 {
@@ -897,8 +899,11 @@ int density_rounds(gz_context *ctx, int iters, bool speculate)
     int remaining = iters > 0 ? ctx->hcounters[0] : na;  // rows of the coming round
     bool read = iters > 0;  // the counters of round `iters` are on the host already
     // the row kernels' rounds after the first take only the rows still iterating (a list the
-    // closing kernel writes), not a workgroup per row that mostly exits at once
-    if (few && !ctx->rl_a) {
+    // closing kernel writes), not a workgroup per row that mostly exits at once; over a few
+    // thousand rows those launches cost little, and the list's offsets would add a wait on an
+    // atomic's result to every closing kernel
+    const bool listed = few && na > kListedRounds;
+    if (listed && !ctx->rl_a) {
         CHK(dalloc(ctx, &ctx->rl_a, ctx->cap));
         CHK(dalloc(ctx, &ctx->rl_b, ctx->cap));
     }
@@ -907,9 +912,9 @@ int density_rounds(gz_context *ctx, int iters, bool speculate)
         if (!read) {
             // the round's counters (k_density_init, or the step's start, cleared them for the first round)
             if (iters > 0) HIPCHK(hipMemsetAsync(ctx->counters, 0, 3 * sizeof(int), ctx->stream));
-            const int *rows = (few && ctx->rl_valid) ? ctx->rl_a : nullptr;
+            const int *rows = (listed && ctx->rl_valid) ? ctx->rl_a : nullptr;
             const int nround = rows ? remaining : na;
-            int *next_rows = few ? ctx->rl_b : nullptr;
+            int *next_rows = listed ? ctx->rl_b : nullptr;
             // over many levels the rounds take one workgroup per row whatever their number (the
             // cells of the rows' own levels hold too few rows to fill a wavefront): four
             // wavefronts per row while few rows iterate, one while many do; with timing on, the
@@ -941,7 +946,7 @@ int density_rounds(gz_context *ctx, int iters, bool speculate)
                     gate_now ? ctx->counters + kPassGate : nullptr, (unsigned int *)(ctx->u64c + 79), rows, nround, next_rows);
             }
             HIPCHK(hipGetLastError());
-            if (few) {
+            if (listed) {
                 std::swap(ctx->rl_a, ctx->rl_b);
                 ctx->rl_valid = true;
             }

@@ -764,8 +764,12 @@ __global__ void __launch_bounds__(256) k_density_after(DevParams P, int nrows, c
     if (threadIdx.x == 0) {
         int c = 0, g = 0;
         for (int k = 0; k < (int)((blockDim.x + 63) >> 6); k++) { wbase[k] = c; c += wcont[k]; g |= wgrow[k]; }
-        const int base = c ? atomicAdd(nnext, c) : 0;
-        for (int k = 0; k < (int)((blockDim.x + 63) >> 6); k++) wbase[k] += base;
+        if (next_rows) {
+            const int base = c ? atomicAdd(nnext, c) : 0;
+            for (int k = 0; k < (int)((blockDim.x + 63) >> 6); k++) wbase[k] += base;
+        } else if (c) {
+            atomicAdd(nnext, c);  // no result: no wait for it
+        }
         if (g) atomicOr(rebuild, 1);
         if (gate) {
             // the last block to finish, when every block's counts and maxima are in, writes
